@@ -1,0 +1,234 @@
+"""Test infrastructure only: ctypes binding of the C oracle (librc_oracle.so).
+
+Imported by tests/, ``__graft_entry__.smoke`` and bench.py's CPU-baseline leg,
+never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+MAXSP = 8
+
+
+class RcoMech(C.Structure):
+    _fields_ = [("nsp", C.c_int), ("mw", C.c_double * MAXSP), ("rconstant", C.c_double),
+                ("t_low", C.c_double), ("t_crit", C.c_double), ("t_high", C.c_double),
+                ("low", (C.c_double * 7) * MAXSP), ("high", (C.c_double * 7) * MAXSP),
+                ("low_b1", C.c_double * MAXSP), ("high_b1", C.c_double * MAXSP),
+                ("model", C.c_int), ("coef_f", C.c_double * MAXSP), ("coef_r", C.c_double * MAXSP),
+                ("a_f", C.c_double), ("e_act", C.c_double)]
+
+
+class RcoParams(C.Structure):
+    _fields_ = [("cfl", C.c_double), ("limit_id", C.c_int), ("combustion", C.c_int),
+                ("p_out", C.c_double),
+                ("ox_y", C.c_double * MAXSP), ("ox_t", C.c_double), ("ox_g", C.c_double),
+                ("fu_y", C.c_double * MAXSP), ("fu_t", C.c_double), ("fu_g", C.c_double)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "librc_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(path)
+        P = C.POINTER
+        L.rco_create.argtypes = [P(RcoMech), P(RcoParams), C.c_int, P(C.c_double), C.c_int, P(C.c_int),
+                                 C.c_int, P(C.c_int), P(C.c_int), P(C.c_void_p)]
+        L.rco_destroy.argtypes = [C.c_void_p]
+        L.rco_last_error.argtypes = [C.c_void_p]
+        L.rco_last_error.restype = C.c_char_p
+        L.rco_sizes.argtypes = [C.c_void_p, P(C.c_int)]
+        L.rco_darray.argtypes = [C.c_void_p, C.c_char_p]
+        L.rco_darray.restype = P(C.c_double)
+        L.rco_iarray.argtypes = [C.c_void_p, C.c_char_p]
+        L.rco_iarray.restype = P(C.c_int)
+        for n in ("rco_timestep", "rco_gradient", "rco_limiter", "rco_flux_sourceterm",
+                  "rco_lusgs_steady", "rco_stage"):
+            getattr(L, n).argtypes = [C.c_void_p]
+        L.rco_set_pa.argtypes = [C.c_void_p, P(C.c_double)]
+        L.rco_residual.argtypes = [C.c_void_p, P(C.c_double)]
+        L.rco_gas_param.argtypes = [P(RcoMech), P(C.c_double), C.c_double, P(C.c_double)]
+        L.rco_enthalpy.argtypes = [P(RcoMech), C.c_double, P(C.c_double)]
+        L.rco_newton.argtypes = [P(RcoMech), C.c_double, P(C.c_double), C.c_double, P(C.c_double)]
+        L.rco_production_rate.argtypes = [P(RcoMech), C.c_double, C.c_double, P(C.c_double), P(C.c_double)]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def make_mech(mech) -> RcoMech:
+    m = RcoMech()
+    m.nsp = mech.nsp
+    for k in range(mech.nsp):
+        m.mw[k] = mech.mole_weight[k]
+        m.low_b1[k] = mech.low_b1[k]
+        m.high_b1[k] = mech.high_b1[k]
+        m.coef_f[k] = mech.coef_f[k]
+        m.coef_r[k] = mech.coef_r[k]
+        for j in range(7):
+            m.low[k][j] = mech.low_coef[k, j]
+            m.high[k][j] = mech.high_coef[k, j]
+    m.rconstant = mech.rconstant
+    m.t_low, m.t_crit, m.t_high = mech.t_low, mech.t_crit, mech.t_high
+    m.model = mech.model
+    m.a_f, m.e_act = mech.a_f, mech.e_act
+    return m
+
+
+def make_params(case) -> RcoParams:
+    p = RcoParams()
+    p.cfl = case.info.cfl
+    p.limit_id = case.info.limit_id
+    p.combustion = 1 if case.info.combustion else 0
+    p.p_out = case.p_out
+    for k in range(case.mech.nsp):
+        p.ox_y[k] = case.oxidizer.y[k]
+        p.fu_y[k] = case.fuel.y[k]
+    p.ox_t = case.oxidizer.temp
+    p.ox_g = case.inlet_mass_flux
+    p.fu_t = case.fuel.temp
+    p.fu_g = 0.0
+    return p
+
+
+class OracleCase:
+    """One case held by the C oracle."""
+
+    def __init__(self, case):
+        L = lib()
+        self.case = case
+        self._mech = make_mech(case.mech)
+        self._par = make_params(case)
+        m = case.mesh
+        coor = np.ascontiguousarray(m.coor, dtype=np.float64)
+        cells = np.ascontiguousarray(m.cells, dtype=np.int32)
+        be = np.ascontiguousarray(m.bedges, dtype=np.int32)
+        bt = np.ascontiguousarray(m.btype, dtype=np.int32)
+        h = C.c_void_p()
+        rc = L.rco_create(C.byref(self._mech), C.byref(self._par), len(coor), _dp(coor), len(cells), _ip(cells),
+                          len(be), _ip(be), _ip(bt), C.byref(h))
+        self.h = h
+        if rc:
+            raise RuntimeError("oracle set-up failed: " + L.rco_last_error(h).decode())
+        s = (C.c_int * 4)()
+        L.rco_sizes(h, s)
+        self.ne, self.nf, self.ng, self.nvar = s[0], s[1], s[2], s[3]
+        self.set_pa(case.pa0)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.rco_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc:
+            raise RuntimeError(_LIB.rco_last_error(self.h).decode())
+
+    _DSHAPE = {"PA": "nv", "WA": "nv", "hWA": "nv", "gradC": "g", "GPA": "gv", "alpaF": "f", "step": "e",
+               "dd": "e", "VECX": "f2", "vol": "e", "cellxy": "e2", "reflen": "e2", "wdis": "e3",
+               "AMLS": "a", "vol_Ghostcell": "gg", "CellXY_Ghostcell": "gg2", "production_rate": "ps"}
+    _ISHAPE = {"Neighbor": "f2", "FNode": "f2", "NeighF": "e3", "NeighC": "e4", "Neigh": "e",
+               "Num_ghostcell": "e", "Ghostcell": "e3", "Hostcell": "gg", "BCtype_Ghostcell": "gg"}
+
+    def _shape(self, code):
+        ne, nf, ng, nv = self.ne, self.nf, self.ng, self.nvar
+        return {"nv": (ne, nv), "g": (ne, nv, 2), "gv": (ng, nv), "f": (nf,), "e": (ne,), "f2": (nf, 2),
+                "e2": (ne, 2), "e3": (ne, 3), "e4": (ne, 4), "a": (ne, 3, 2), "gg": (ng,), "gg2": (ng, 2),
+                "ps": (ne, self.case.mech.nsp)}[code]
+
+    def get(self, name: str) -> np.ndarray:
+        """A copy of a reference array, cell-major (row i is cell i+1)."""
+        if name in self._DSHAPE:
+            shp = self._shape(self._DSHAPE[name])
+            p = _LIB.rco_darray(self.h, name.encode())
+            return np.ctypeslib.as_array(p, shape=(int(np.prod(shp)),)).reshape(shp).copy()
+        if name in self._ISHAPE:
+            shp = self._shape(self._ISHAPE[name])
+            p = _LIB.rco_iarray(self.h, name.encode())
+            return np.ctypeslib.as_array(p, shape=(int(np.prod(shp)),)).reshape(shp).copy()
+        if name == "total_volume":
+            return float(_LIB.rco_darray(self.h, b"total_volume")[0])
+        raise KeyError(name)
+
+    def set_pa(self, pa):
+        pa = np.ascontiguousarray(pa, dtype=np.float64)
+        self._check(_LIB.rco_set_pa(self.h, _dp(pa)))
+
+    def timestep(self):
+        self._check(_LIB.rco_timestep(self.h))
+
+    def gradient(self):
+        self._check(_LIB.rco_gradient(self.h))
+
+    def limiter(self):
+        self._check(_LIB.rco_limiter(self.h))
+
+    def reconstruction(self):
+        self.gradient()
+        if self.case.info.limit_id > 0:
+            self.limiter()
+
+    def flux_sourceterm(self):
+        self._check(_LIB.rco_flux_sourceterm(self.h))
+
+    def lusgs_steady(self):
+        self._check(_LIB.rco_lusgs_steady(self.h))
+
+    def stage(self):
+        self._check(_LIB.rco_stage(self.h))
+
+    def residual(self) -> np.ndarray:
+        out = np.zeros(self.nvar + 1)
+        _LIB.rco_residual(self.h, _dp(out))
+        return out
+
+
+def gas_param(mech, y, t):
+    m = make_mech(mech)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(3)
+    rc = lib().rco_gas_param(C.byref(m), _dp(y), float(t), _dp(out))
+    if rc:
+        raise ValueError("temperature out of range")
+    return out
+
+
+def enthalpy(mech, t):
+    m = make_mech(mech)
+    out = np.zeros(mech.nsp)
+    rc = lib().rco_enthalpy(C.byref(m), float(t), _dp(out))
+    if rc:
+        raise ValueError("temperature out of range")
+    return out
+
+
+def newton(mech, et, y, t0):
+    m = make_mech(mech)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(4)
+    rc = lib().rco_newton(C.byref(m), float(et), _dp(y), float(t0), _dp(out))
+    if rc:
+        raise ValueError("temperature out of range")
+    return out
+
+
+def production_rate(mech, t, rho, y):
+    m = make_mech(mech)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(mech.nsp)
+    lib().rco_production_rate(C.byref(m), float(t), float(rho), _dp(y), _dp(out))
+    return out

@@ -1,0 +1,975 @@
+/* Test infrastructure only: serial C restatement of the reference's steady
+ * residual + LU-SGS path (the oracle).  Not linked into the product; used by
+ * tests/ and bench.py's CPU-baseline leg as the checker.
+ *
+ * Each function cites the reference routine it restates.  Operation order
+ * follows the Fortran expressions (left-to-right sums, the same
+ * parenthesisation), so results agree with the reference to rounding of
+ * the math library.  Built with -ffp-contract=off (no FMA contraction), like
+ * the reference on baseline x86-64.
+ */
+#include "rc_oracle.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NFLOW 4
+#define SMALL 1.0e-16              /* module.f90:19 */
+
+struct rco_case {
+    rco_mech m;
+    rco_params p;
+    int ne, nnt, nf, nvar, nsp, ng;
+    double *coor;                  /* Coor(2,NNT) */
+    int *n4;                       /* N(4,NE), 1-based nodes, N(4)=N(3) for triangles */
+    int *fnode;                    /* FNode(2,NF) */
+    int *neighbor;                 /* Neighbor(2,NF) */
+    double *vecx;                  /* VECX(2,NF) */
+    double *vol, *cellxy, *reflen; /* vol(NE), CellXY(2,NE), reflen(2,NE) */
+    int *neighf;                   /* NeighF(3,NE) */
+    int *neighc;                   /* NeighC(0:3,NE) */
+    int *neigh;                    /* Neigh(NE) */
+    int *hostcell, *hostface, *bctype_g;  /* per ghost */
+    int *ghostcell, *num_ghost;    /* Ghostcell(3,NE), Num_ghostcell(NE) */
+    double *coor_gn, *vol_g, *cellxy_g;   /* Coor_Ghostnode(2,4,NG), vol_Ghostcell, CellXY_Ghostcell(2,NG) */
+    double *wdis, *amls, *xyh;     /* wdis(3,NE), AMLS(2,3,NE), XYH(2,NE) */
+    double total_volume;
+    double *pa, *wa, *hwa, *gradc, *gpa, *alpaf, *step, *prod, *dd;
+    char err[256];
+};
+
+#define PA(k, i)    c->pa[((i) - 1) * c->nvar + (k) - 1]
+#define WA(k, i)    c->wa[((i) - 1) * c->nvar + (k) - 1]
+#define HWA(k, i)   c->hwa[((i) - 1) * c->nvar + (k) - 1]
+#define GPA(k, j)   c->gpa[((j) - 1) * c->nvar + (k) - 1]
+#define GRADC(r, k, i) c->gradc[(((i) - 1) * c->nvar + (k) - 1) * 2 + (r) - 1]
+#define COOR(d, n)  c->coor[((n) - 1) * 2 + (d) - 1]
+#define CELLXY(d, i) c->cellxy[((i) - 1) * 2 + (d) - 1]
+#define REFLEN(d, i) c->reflen[((i) - 1) * 2 + (d) - 1]
+#define VECX(d, f)  c->vecx[((f) - 1) * 2 + (d) - 1]
+#define NEIGHBOR(s, f) c->neighbor[((f) - 1) * 2 + (s) - 1]
+#define FNODE(s, f) c->fnode[((f) - 1) * 2 + (s) - 1]
+#define NEIGHF(k, i) c->neighf[((i) - 1) * 3 + (k) - 1]
+#define NEIGHC(k, i) c->neighc[((i) - 1) * 4 + (k)]
+#define NODE(k, i)  c->n4[((i) - 1) * 4 + (k) - 1]
+#define WDIS(k, i)  c->wdis[((i) - 1) * 3 + (k) - 1]
+#define AMLS(r, k, i) c->amls[(((i) - 1) * 3 + (k) - 1) * 2 + (r) - 1]
+#define GHOSTCELL(k, i) c->ghostcell[((i) - 1) * 3 + (k) - 1]
+#define CGN(d, v, j) c->coor_gn[(((j) - 1) * 4 + (v) - 1) * 2 + (d) - 1]
+#define CXYG(d, j)  c->cellxy_g[((j) - 1) * 2 + (d) - 1]
+
+static int fail(rco_case *c, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof c->err, fmt, ap);
+    va_end(ap);
+    return 1;
+}
+
+/* real**integer as compiler-rt's __powidf2 evaluates it */
+static double powi(double a, int b)
+{
+    const int recip = b < 0;
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* thermochemistry: src/combustion_modeling/thermo_property.f90            */
+/* ------------------------------------------------------------------------ */
+
+/* ComputeGasParameter, thermo_property.f90:78-112 */
+int rco_gas_param(const rco_mech *m, const double *y, double t, double *out)
+{
+    double basis[7], cps[RCO_MAXSP];
+    const double (*coef)[7];
+    int j, i;
+    for (j = 1; j <= 7; j++) basis[j - 1] = powi(t, j - 3);
+    if (m->t_low <= t && t <= m->t_crit) coef = m->low;
+    else if (m->t_crit < t && t <= m->t_high) coef = m->high;
+    else return 1;
+    for (i = 0; i < m->nsp; i++) {
+        double s = 0.0;
+        for (j = 0; j < 7; j++) s = s + coef[i][j] * basis[j];
+        cps[i] = s * (m->rconstant / m->mw[i]);
+    }
+    double cp = 0.0, r = 0.0;
+    for (i = 0; i < m->nsp; i++) cp = cp + cps[i] * y[i];
+    for (i = 0; i < m->nsp; i++) r = r + (m->rconstant / m->mw[i]) * y[i];
+    out[0] = cp;
+    out[1] = cp / (cp - r);
+    out[2] = r;
+    return 0;
+}
+
+/* compute_enthalpy, thermo_property.f90:114-144 */
+int rco_enthalpy(const rco_mech *m, double t, double *h)
+{
+    double basis[7];
+    const double (*coef)[7];
+    const double *b1;
+    int j, i;
+    basis[0] = -powi(t, -2);
+    basis[1] = log(t) / t;
+    for (j = 3; j <= 7; j++) basis[j - 1] = powi(t, j - 3) / (double)(j - 2);
+    if (m->t_low <= t && t <= m->t_crit) { coef = m->low; b1 = m->low_b1; }
+    else if (m->t_crit < t && t <= m->t_high) { coef = m->high; b1 = m->high_b1; }
+    else return 1;
+    for (i = 0; i < m->nsp; i++) {
+        double s = 0.0;
+        for (j = 0; j < 7; j++) s = s + coef[i][j] * basis[j];
+        h[i] = (m->rconstant / m->mw[i]) * (t * s + b1[i]);
+    }
+    return 0;
+}
+
+/* Newton_temperature, thermo_property.f90:146-180 */
+int rco_newton(const rco_mech *m, double et, const double *y, double t0, double *out)
+{
+    double t = t0, h[RCO_MAXSP], g[3] = {0, 0, 0};
+    int it = 0, i;
+    while (it < 200) {
+        it++;
+        if (rco_enthalpy(m, t, h)) return 1;
+        if (rco_gas_param(m, y, t, g)) return 1;
+        double hh = 0.0;
+        for (i = 0; i < m->nsp; i++) hh = hh + y[i] * h[i];
+        double ft = hh - g[2] * t - et;
+        double dft = g[0] - g[2];
+        double dtt = -ft / dft;
+        t = t + dtt;
+        t = fmax(t, m->t_low);
+        t = fmin(t, m->t_high);
+        if (fabs(dtt) < 1.0e-5) break;
+    }
+    out[0] = t; out[1] = g[0]; out[2] = g[1]; out[3] = g[2];
+    return 0;
+}
+
+/* compute_production_rate, chemical_kinetics.f90:291-347 */
+void rco_production_rate(const rco_mech *m, double t, double rho, const double *y2, double *w)
+{
+    double y[RCO_MAXSP], kf;
+    int k;
+    for (k = 0; k < m->nsp; k++) y[k] = fmin(1.0, fmax(0.0, y2[k]));
+    if (m->model == 0) {
+        kf = m->a_f * exp(-m->e_act / (m->rconstant * t));
+        kf = kf * pow(y[0] * rho / (m->mw[0] * 1000.0), 0.2);
+        kf = kf * pow(y[1] * rho / (m->mw[1] * 1000.0), 1.3);
+    } else {
+        kf = 0.0;
+        if (t > 710.0) kf = 1e4;
+        kf = kf * pow(y[0] * rho / (m->mw[0] * 1000.0), 1.0);
+        kf = kf * pow(y[1] * rho / (m->mw[1] * 1000.0), 0.0);
+    }
+    for (k = 0; k < m->nsp; k++)
+        w[k] = 1000.0 * m->mw[k] * (m->coef_r[k] - m->coef_f[k]) * kf;
+}
+
+/* ------------------------------------------------------------------------ */
+/* set-up                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* GaussInteg_Tri, gaussInteg4.f90:4-42; fun selects 1, x or y, or the
+ * scaled coordinate (x-cx)/rx of compute_reconstruction_matrix:153-167 */
+typedef struct { int kind; double cx, rx; } gfun;
+static double gval(const gfun *f, const double *v)
+{
+    switch (f->kind) {
+    case 0: return 1.0;
+    case 1: return v[0];
+    case 2: return v[1];
+    case 3: return (v[0] - f->cx) / f->rx;
+    default: return (v[1] - f->cx) / f->rx;
+    }
+}
+
+static double gauss_tri(const gfun *f, const double *xv1, const double *xv2, const double *xv3)
+{
+    double x1 = xv1[0], y1 = xv1[1], x2 = xv2[0], y2 = xv2[1], x3 = xv3[0], y3 = xv3[1];
+    double area = 0.5 * fabs((x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1));
+    double v[13][2];
+    int d;
+    for (d = 0; d < 2; d++) {
+        double a, b;
+        v[0][d] = (xv1[d] + xv2[d] + xv3[d]) / 3.0;
+        a = 0.479308067841920; b = 0.260345966079040;
+        v[1][d] = a * xv1[d] + b * (xv2[d] + xv3[d]);
+        v[2][d] = a * xv2[d] + b * (xv1[d] + xv3[d]);
+        v[3][d] = a * xv3[d] + b * (xv1[d] + xv2[d]);
+        a = 0.869739794195568; b = 0.065130102902216;
+        v[4][d] = a * xv1[d] + b * (xv2[d] + xv3[d]);
+        v[5][d] = a * xv2[d] + b * (xv1[d] + xv3[d]);
+        v[6][d] = a * xv3[d] + b * (xv1[d] + xv2[d]);
+        a = 0.048690315425316; b = 0.312865496004874;
+        v[7][d] = a * xv1[d] + b * xv2[d] + (1 - a - b) * xv3[d];
+        v[8][d] = a * xv1[d] + b * xv3[d] + (1 - a - b) * xv2[d];
+        v[9][d] = a * xv2[d] + b * xv1[d] + (1 - a - b) * xv3[d];
+        v[10][d] = a * xv2[d] + b * xv3[d] + (1 - a - b) * xv1[d];
+        v[11][d] = a * xv3[d] + b * xv1[d] + (1 - a - b) * xv2[d];
+        v[12][d] = a * xv3[d] + b * xv2[d] + (1 - a - b) * xv1[d];
+    }
+    double r = -0.149570044467682 * gval(f, v[0])
+        + 0.175615257433208 * (gval(f, v[1]) + gval(f, v[2]) + gval(f, v[3]))
+        + 0.053347235608838 * (gval(f, v[4]) + gval(f, v[5]) + gval(f, v[6]))
+        + 0.077113760890257 * (gval(f, v[7]) + gval(f, v[8]) + gval(f, v[9]) + gval(f, v[10])
+                               + gval(f, v[11]) + gval(f, v[12]));
+    return r * area;
+}
+
+/* GaussInteg_Quad, gaussInteg4.f90:44-50 */
+static double gauss_quad(const gfun *f, const double *a, const double *b, const double *cc, const double *d)
+{
+    return gauss_tri(f, a, b, cc) + gauss_tri(f, a, cc, d);
+}
+
+/* pseudo_inverse_lapack (la_math.f90:1-62) forms pinv(A) of the 3x2 LSQ
+ * matrix with dgesvd.  The restatement uses a thin QR factorisation by
+ * Gram-Schmidt with one re-orthogonalisation pass, A = QR, pinv = R^-1 Q^T,
+ * which is backward stable like the SVD route (the two agree to
+ * O(cond(A) eps)). */
+static void pinv32(const double a[3][2], double b[2][3])
+{
+    double q[3][2], r[2][2];
+    double v[3];
+    int i;
+    /* modified Gram-Schmidt with one re-orthogonalisation pass */
+    double n0 = sqrt(a[0][0] * a[0][0] + a[1][0] * a[1][0] + a[2][0] * a[2][0]);
+    for (i = 0; i < 3; i++) q[i][0] = a[i][0] / n0;
+    r[0][0] = n0;
+    for (i = 0; i < 3; i++) v[i] = a[i][1];
+    double r01 = q[0][0] * v[0] + q[1][0] * v[1] + q[2][0] * v[2];
+    for (i = 0; i < 3; i++) v[i] = v[i] - r01 * q[i][0];
+    double s = q[0][0] * v[0] + q[1][0] * v[1] + q[2][0] * v[2];
+    for (i = 0; i < 3; i++) v[i] = v[i] - s * q[i][0];
+    r01 = r01 + s;
+    double n1 = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (i = 0; i < 3; i++) q[i][1] = v[i] / n1;
+    r[0][1] = r01; r[1][1] = n1; r[1][0] = 0.0;
+    /* B = R^-1 Q^T */
+    for (i = 0; i < 3; i++) {
+        double y1 = q[i][1] / r[1][1];
+        double y0 = (q[i][0] - r[0][1] * y1) / r[0][0];
+        b[0][i] = y0;
+        b[1][i] = y1;
+    }
+}
+
+static void *xcalloc(size_t n, size_t s) { void *p = calloc(n ? n : 1, s); if (!p) abort(); return p; }
+
+int rco_create(const rco_mech *mech, const rco_params *par, int nnt, const double *coor,
+               int ne, const int *cells, int nb, const int *bedges, const int *btype, rco_case **out)
+{
+    rco_case *c = (rco_case *)xcalloc(1, sizeof *c);
+    *out = c;
+    c->m = *mech;
+    c->p = *par;
+    c->ne = ne; c->nnt = nnt; c->nsp = mech->nsp; c->nvar = NFLOW + mech->nsp - 1;
+    c->coor = (double *)xcalloc(2 * nnt, sizeof(double));
+    memcpy(c->coor, coor, sizeof(double) * 2 * nnt);
+    c->n4 = (int *)xcalloc(4 * ne, sizeof(int));
+    int i, k;
+    for (i = 1; i <= ne; i++) {
+        for (k = 1; k <= 3; k++) NODE(k, i) = cells[(i - 1) * 3 + k - 1] + 1;
+        NODE(4, i) = NODE(3, i);   /* mesh.f90:33-35 */
+    }
+
+    /* change / face, mesh.f90:120-199: faces in order of first appearance */
+    int maxf = 3 * ne;
+    c->fnode = (int *)xcalloc(2 * maxf, sizeof(int));
+    c->neighbor = (int *)xcalloc(2 * maxf, sizeof(int));
+    c->vecx = (double *)xcalloc(2 * maxf, sizeof(double));
+    int *numnf = (int *)xcalloc(nnt + 1, sizeof(int));
+    int *nodeface = (int *)xcalloc((size_t)(nnt + 1) * 50, sizeof(int));
+    int nf = 0, j, kf;
+    for (j = 1; j <= ne; j++) {
+        for (kf = 1; kf <= 3; kf++) {
+            int n1 = NODE(kf, j), n2 = NODE(kf % 3 + 1, j);
+            int idx = 0, inum;
+            for (inum = 1; inum <= numnf[n2]; inum++) {
+                int f = nodeface[n2 * 50 + inum - 1];
+                if ((FNODE(1, f) == n2 && FNODE(2, f) == n1) || (FNODE(1, f) == n1 && FNODE(2, f) == n2)) { idx = f; break; }
+            }
+            if (idx == 0) {
+                nf++;
+                NEIGHBOR(1, nf) = j;
+                NEIGHBOR(2, nf) = -100;
+                VECX(1, nf) = COOR(2, n2) - COOR(2, n1);
+                VECX(2, nf) = -(COOR(1, n2) - COOR(1, n1));
+                FNODE(1, nf) = n1; FNODE(2, nf) = n2;
+                numnf[n1]++; numnf[n2]++;
+                if (numnf[n1] > 50 || numnf[n2] > 50) { free(numnf); free(nodeface); return fail(c, "error in NumNFace"); }
+                nodeface[n1 * 50 + numnf[n1] - 1] = nf;
+                nodeface[n2 * 50 + numnf[n2] - 1] = nf;
+            } else {
+                NEIGHBOR(2, idx) = j;
+            }
+        }
+    }
+    free(numnf); free(nodeface);
+    c->nf = nf;
+
+    /* BC tagging, mesh.f90:76-93 */
+    for (i = 1; i <= nf; i++) {
+        if (NEIGHBOR(2, i) < 0) {
+            int n1 = FNODE(1, i), n2 = FNODE(2, i), found = 0, b;
+            for (b = 0; b < nb; b++) {
+                int a1 = bedges[2 * b] + 1, a2 = bedges[2 * b + 1] + 1;
+                if ((n1 == a1 && n2 == a2) || (n2 == a1 && n1 == a2)) { NEIGHBOR(2, i) = -btype[b]; found = 1; break; }
+            }
+            if (!found) return fail(c, "not find boundary of this type (face %d)", i);
+        }
+    }
+
+    /* VolumeAndCenter, mesh.f90:206-286 */
+    c->vol = (double *)xcalloc(ne, sizeof(double));
+    c->cellxy = (double *)xcalloc(2 * ne, sizeof(double));
+    c->reflen = (double *)xcalloc(2 * ne, sizeof(double));
+    gfun f1 = {0, 0, 0}, fx = {1, 0, 0}, fy = {2, 0, 0};
+    for (i = 1; i <= ne; i++) {
+        const double *x1 = &COOR(1, NODE(1, i)), *x2 = &COOR(1, NODE(2, i)), *x3 = &COOR(1, NODE(3, i)), *x4 = &COOR(1, NODE(4, i));
+        c->vol[i - 1] = gauss_quad(&f1, x1, x2, x3, x4);
+        CELLXY(1, i) = gauss_quad(&fx, x1, x2, x3, x4) / c->vol[i - 1];
+        CELLXY(2, i) = gauss_quad(&fy, x1, x2, x3, x4) / c->vol[i - 1];
+        if (c->vol[i - 1] < 0.0) return fail(c, "negative volume %d", i);
+        REFLEN(1, i) = 0.0; REFLEN(2, i) = 0.0;
+        int a, b, d;
+        for (a = 1; a <= 2; a++)
+            for (b = a + 1; b <= 3; b++)
+                for (d = 1; d <= 2; d++)
+                    REFLEN(d, i) = fmax(0.5 * fabs(COOR(d, NODE(a, i)) - COOR(d, NODE(b, i))), REFLEN(d, i));
+    }
+    double sv = 0.0;
+    for (i = 1; i <= ne; i++) sv = sv + c->vol[i - 1];
+    c->total_volume = sv;
+
+    /* compute_reconstruction_matrix, reconstruction_initialization.f90:1-150 */
+    c->neighf = (int *)xcalloc(3 * ne, sizeof(int));
+    c->neighc = (int *)xcalloc(4 * ne, sizeof(int));
+    c->neigh = (int *)xcalloc(ne, sizeof(int));
+    int *ntemp = (int *)xcalloc(ne + 1, sizeof(int));
+    for (i = 1; i <= nf; i++) {
+        int NL = NEIGHBOR(1, i), NR = NEIGHBOR(2, i);
+        ntemp[NL]++; NEIGHF(ntemp[NL], NL) = i;
+        if (NR > 0) { ntemp[NR]++; NEIGHF(ntemp[NR], NR) = i; }
+    }
+    free(ntemp);
+    for (i = 1; i <= ne; i++) {
+        for (k = 1; k <= 3; k++) {
+            int iface = NEIGHF(k, i);
+            for (kf = 1; kf <= 2; kf++) {
+                int ic = NEIGHBOR(kf, iface);
+                if (ic != i && ic > 0) { c->neigh[i - 1]++; NEIGHC(c->neigh[i - 1], i) = ic; }
+            }
+        }
+        NEIGHC(0, i) = i;
+    }
+
+    /* construct_ghostcells, ghostcell.f90:203-379 */
+    int ng = 0;
+    for (i = 1; i <= ne; i++) ng += 3 - c->neigh[i - 1];
+    c->ng = ng;
+    c->hostcell = (int *)xcalloc(ng, sizeof(int));
+    c->hostface = (int *)xcalloc(ng, sizeof(int));
+    c->bctype_g = (int *)xcalloc(ng, sizeof(int));
+    c->ghostcell = (int *)xcalloc(3 * ne, sizeof(int));
+    c->num_ghost = (int *)xcalloc(ne, sizeof(int));
+    c->coor_gn = (double *)xcalloc(8 * ng, sizeof(double));
+    c->vol_g = (double *)xcalloc(ng, sizeof(double));
+    c->cellxy_g = (double *)xcalloc(2 * ng, sizeof(double));
+    j = 0;
+    for (i = 1; i <= ne; i++) {
+        if (c->neigh[i - 1] < 3) {
+            for (kf = 1; kf <= 3; kf++) {
+                int iface = NEIGHF(kf, i);
+                if (NEIGHBOR(2, iface) < 0) {
+                    j++;
+                    c->hostcell[j - 1] = i; c->hostface[j - 1] = iface;
+                    c->bctype_g[j - 1] = -NEIGHBOR(2, iface);
+                    c->num_ghost[i - 1]++;
+                    GHOSTCELL(c->num_ghost[i - 1], i) = j;
+                    int n1 = FNODE(1, iface), n2 = FNODE(2, iface), k1, k2, k3, k4;
+                    for (k1 = 1; k1 <= 3; k1++) if (n1 == NODE(k1, i)) break;
+                    k2 = k1 % 3 + 1;
+                    if (NODE(k2, i) != n2) return fail(c, "error in index setting of nodes when looking for ghostcells");
+                    k3 = k2 % 3 + 1; k4 = k3 % 3 + 1;
+                    int n3 = NODE(k3, i), n4 = NODE(k4, i);
+                    double sav1 = VECX(1, iface), sav2 = VECX(2, iface);
+                    double sav = fmax(sqrt(sav1 * sav1 + sav2 * sav2), SMALL);
+                    double s1n = sav1 / sav, s2n = sav2 / sav;
+                    CGN(1, 1, j) = COOR(1, n2); CGN(2, 1, j) = COOR(2, n2);
+                    CGN(1, 2, j) = COOR(1, n1); CGN(2, 2, j) = COOR(2, n1);
+                    double d1 = (COOR(1, n1) - COOR(1, n4)) * s1n + (COOR(2, n1) - COOR(2, n4)) * s2n;
+                    if (d1 < 0) return fail(c, "error in calculating the distance (n1,n4)");
+                    CGN(1, 3, j) = COOR(1, n4) + 2 * d1 * s1n;
+                    CGN(2, 3, j) = COOR(2, n4) + 2 * d1 * s2n;
+                    double d2 = (COOR(1, n2) - COOR(1, n3)) * s1n + (COOR(2, n2) - COOR(2, n3)) * s2n;
+                    if (d2 < 0) return fail(c, "error in calculating the distance (n2,n3)");
+                    CGN(1, 4, j) = COOR(1, n3) + 2 * d2 * s1n;
+                    CGN(2, 4, j) = COOR(2, n3) + 2 * d2 * s2n;
+                    const double *g1 = &CGN(1, 1, j), *g2 = &CGN(1, 2, j), *g3 = &CGN(1, 3, j), *g4 = &CGN(1, 4, j);
+                    c->vol_g[j - 1] = gauss_quad(&f1, g1, g2, g3, g4);
+                    CXYG(1, j) = gauss_quad(&fx, g1, g2, g3, g4) / c->vol_g[j - 1];
+                    CXYG(2, j) = gauss_quad(&fy, g1, g2, g3, g4) / c->vol_g[j - 1];
+                    if (c->vol_g[j - 1] < 0.0) return fail(c, "negative ghost volume %d", j);
+                }
+            }
+        }
+        if (c->num_ghost[i - 1] + c->neigh[i - 1] != 3) return fail(c, "error in number of ghost cells for cell %d", i);
+    }
+
+    c->wdis = (double *)xcalloc(3 * ne, sizeof(double));
+    c->amls = (double *)xcalloc(6 * ne, sizeof(double));
+    c->xyh = (double *)xcalloc(2 * ne, sizeof(double));
+    for (i = 1; i <= ne; i++) {
+        const double *x1 = &COOR(1, NODE(1, i)), *x2 = &COOR(1, NODE(2, i)), *x3 = &COOR(1, NODE(3, i)), *x4 = &COOR(1, NODE(4, i));
+        int io;
+        for (io = 1; io <= 2; io++) {
+            gfun fs = {2 + io, CELLXY(io, i), REFLEN(io, i)};
+            c->xyh[(i - 1) * 2 + io - 1] = gauss_quad(&fs, x1, x2, x3, x4) / c->vol[i - 1];
+        }
+        if (fabs(c->xyh[(i - 1) * 2]) + fabs(c->xyh[(i - 1) * 2 + 1]) > 1.e-8) return fail(c, "error in XYH %d", i);
+    }
+    for (i = 1; i <= ne; i++) {
+        double a1[3][2], b1[2][3];
+        int nn = c->neigh[i - 1];
+        for (k = 1; k <= nn; k++) {
+            j = NEIGHC(k, i);
+            const double *x1 = &COOR(1, NODE(1, j)), *x2 = &COOR(1, NODE(2, j)), *x3 = &COOR(1, NODE(3, j)), *x4 = &COOR(1, NODE(4, j));
+            double dx = (CELLXY(1, j) - 0.0) - CELLXY(1, i), dy = (CELLXY(2, j) - 0.0) - CELLXY(2, i);
+            double dist = sqrt(dx * dx + dy * dy);
+            WDIS(k, i) = powi(dist, -1);
+            int io;
+            for (io = 1; io <= 2; io++) {
+                gfun fs = {2 + io, CELLXY(io, i), REFLEN(io, i)};
+                a1[k - 1][io - 1] = WDIS(k, i) * (gauss_quad(&fs, x1, x2, x3, x4) / c->vol[j - 1] - c->xyh[(i - 1) * 2 + io - 1]);
+            }
+        }
+        for (k = 1; k <= c->num_ghost[i - 1]; k++) {
+            j = GHOSTCELL(k, i);
+            const double *g1 = &CGN(1, 1, j), *g2 = &CGN(1, 2, j), *g3 = &CGN(1, 3, j), *g4 = &CGN(1, 4, j);
+            double dx = CXYG(1, j) - CELLXY(1, i), dy = CXYG(2, j) - CELLXY(2, i);
+            double dist = sqrt(dx * dx + dy * dy);
+            WDIS(k + nn, i) = powi(dist, -1);
+            int io;
+            for (io = 1; io <= 2; io++) {
+                gfun fs = {2 + io, CELLXY(io, i), REFLEN(io, i)};
+                a1[k + nn - 1][io - 1] = WDIS(k + nn, i) * (gauss_quad(&fs, g1, g2, g3, g4) / c->vol_g[j - 1] - c->xyh[(i - 1) * 2 + io - 1]);
+            }
+        }
+        pinv32((const double(*)[2])a1, b1);
+        for (k = 1; k <= 3; k++) { AMLS(1, k, i) = b1[0][k - 1]; AMLS(2, k, i) = b1[1][k - 1]; }
+    }
+
+    int nv = c->nvar;
+    c->pa = (double *)xcalloc((size_t)nv * ne, sizeof(double));
+    c->wa = (double *)xcalloc((size_t)nv * ne, sizeof(double));
+    c->hwa = (double *)xcalloc((size_t)nv * ne, sizeof(double));
+    c->gradc = (double *)xcalloc((size_t)2 * nv * ne, sizeof(double));
+    c->gpa = (double *)xcalloc((size_t)nv * ng, sizeof(double));
+    c->alpaf = (double *)xcalloc(nf, sizeof(double));
+    c->step = (double *)xcalloc(ne, sizeof(double));
+    c->prod = (double *)xcalloc((size_t)c->nsp * ne, sizeof(double));
+    c->dd = (double *)xcalloc(ne, sizeof(double));
+    return 0;
+}
+
+void rco_destroy(rco_case *c)
+{
+    if (!c) return;
+    void *ps[] = {c->coor, c->n4, c->fnode, c->neighbor, c->vecx, c->vol, c->cellxy, c->reflen,
+                  c->neighf, c->neighc, c->neigh, c->hostcell, c->hostface, c->bctype_g,
+                  c->ghostcell, c->num_ghost, c->coor_gn, c->vol_g, c->cellxy_g, c->wdis, c->amls,
+                  c->xyh, c->pa, c->wa, c->hwa, c->gradc, c->gpa, c->alpaf, c->step, c->prod, c->dd};
+    size_t i;
+    for (i = 0; i < sizeof ps / sizeof ps[0]; i++) free(ps[i]);
+    free(c);
+}
+
+const char *rco_last_error(const rco_case *c) { return c->err; }
+
+void rco_sizes(const rco_case *c, int *o) { o[0] = c->ne; o[1] = c->nf; o[2] = c->ng; o[3] = c->nvar; }
+
+double *rco_darray(rco_case *c, const char *n)
+{
+    if (!strcmp(n, "PA")) return c->pa;
+    if (!strcmp(n, "WA")) return c->wa;
+    if (!strcmp(n, "hWA")) return c->hwa;
+    if (!strcmp(n, "gradC")) return c->gradc;
+    if (!strcmp(n, "GPA")) return c->gpa;
+    if (!strcmp(n, "alpaF")) return c->alpaf;
+    if (!strcmp(n, "step")) return c->step;
+    if (!strcmp(n, "dd")) return c->dd;
+    if (!strcmp(n, "production_rate")) return c->prod;
+    if (!strcmp(n, "VECX")) return c->vecx;
+    if (!strcmp(n, "vol")) return c->vol;
+    if (!strcmp(n, "cellxy")) return c->cellxy;
+    if (!strcmp(n, "reflen")) return c->reflen;
+    if (!strcmp(n, "wdis")) return c->wdis;
+    if (!strcmp(n, "AMLS")) return c->amls;
+    if (!strcmp(n, "vol_Ghostcell")) return c->vol_g;
+    if (!strcmp(n, "CellXY_Ghostcell")) return c->cellxy_g;
+    if (!strcmp(n, "total_volume")) return &c->total_volume;
+    return NULL;
+}
+
+int *rco_iarray(rco_case *c, const char *n)
+{
+    if (!strcmp(n, "Neighbor")) return c->neighbor;
+    if (!strcmp(n, "FNode")) return c->fnode;
+    if (!strcmp(n, "NeighF")) return c->neighf;
+    if (!strcmp(n, "NeighC")) return c->neighc;
+    if (!strcmp(n, "Neigh")) return c->neigh;
+    if (!strcmp(n, "Num_ghostcell")) return c->num_ghost;
+    if (!strcmp(n, "Ghostcell")) return c->ghostcell;
+    if (!strcmp(n, "Hostcell")) return c->hostcell;
+    if (!strcmp(n, "BCtype_Ghostcell")) return c->bctype_g;
+    return NULL;
+}
+
+/* ------------------------------------------------------------------------ */
+/* state                                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* primitve_to_conserved_variables, update_flow_variables.f90:254-288 */
+static int prim_to_cons(rco_case *c)
+{
+    int i, k, nsp = c->nsp;
+    for (i = 1; i <= c->ne; i++) {
+        double pp = PA(1, i), uu = PA(2, i), vv = PA(3, i), tt = PA(4, i);
+        double yy[RCO_MAXSP], g[3], h[RCO_MAXSP], s = 0.0;
+        for (k = 0; k < nsp - 1; k++) { yy[k] = PA(NFLOW + 1 + k, i); s = s + yy[k]; }
+        yy[nsp - 1] = 1.0 - s;
+        if (rco_gas_param(&c->m, yy, tt, g)) return fail(c, "temperature %g out of range (cell %d)", tt, i);
+        double rr = pp / (g[2] * tt);
+        if (rco_enthalpy(&c->m, tt, h)) return fail(c, "temperature %g out of range (cell %d)", tt, i);
+        double hh = 0.0;
+        for (k = 0; k < nsp; k++) hh = hh + yy[k] * h[k];
+        WA(1, i) = rr;
+        WA(2, i) = rr * uu;
+        WA(3, i) = rr * vv;
+        WA(4, i) = rr * (hh + 0.5 * (uu * uu + vv * vv)) - pp;
+        for (k = 0; k < nsp - 1; k++) WA(NFLOW + 1 + k, i) = rr * yy[k];
+    }
+    return 0;
+}
+
+int rco_set_pa(rco_case *c, const double *pa)
+{
+    memcpy(c->pa, pa, sizeof(double) * c->nvar * c->ne);
+    return prim_to_cons(c);
+}
+
+/* TimeStep (inviscid, steady), TimeStep.f90:4-103 */
+int rco_timestep(rco_case *c)
+{
+    int i, k, nsp = c->nsp, nvar = c->nvar;
+    double *alpac = (double *)xcalloc(c->ne + 1, sizeof(double));
+    for (i = 1; i <= c->nf; i++) {
+        int NL = NEIGHBOR(1, i), NR = NEIGHBOR(2, i);
+        double dx = VECX(1, i), dy = VECX(2, i), pp, uu, vv, tt, yy[RCO_MAXSP], s = 0.0, g[3];
+        if (NR > 0) {
+            pp = 0.5 * (PA(1, NL) + PA(1, NR));
+            uu = 0.5 * (PA(2, NL) + PA(2, NR));
+            vv = 0.5 * (PA(3, NL) + PA(3, NR));
+            tt = 0.5 * (PA(4, NL) + PA(4, NR));
+            for (k = 0; k < nvar - NFLOW; k++) { yy[k] = 0.5 * (PA(NFLOW + 1 + k, NL) + PA(NFLOW + 1 + k, NR)); s = s + yy[k]; }
+        } else {
+            pp = PA(1, NL); uu = PA(2, NL); vv = PA(3, NL); tt = PA(4, NL);
+            for (k = 0; k < nvar - NFLOW; k++) { yy[k] = PA(NFLOW + 1 + k, NL); s = s + yy[k]; }
+        }
+        yy[nsp - 1] = 1.0 - s;
+        if (rco_gas_param(&c->m, yy, tt, g)) { free(alpac); return fail(c, "timestep: temperature %g out of range", tt); }
+        double rr = pp / (g[2] * tt);
+        if (rr < 0. || pp < 0. || tt < 0.) { free(alpac); return fail(c, "error in computing timestep, cells %d %d", NL, NR); }
+        double un = uu * dx + vv * dy;
+        double alpa = fabs(un) + sqrt(g[1] * pp / rr) * sqrt(dx * dx + dy * dy);
+        c->alpaf[i - 1] = alpa;
+        alpac[NL] = alpac[NL] + alpa;
+        if (NR > 0) alpac[NR] = alpac[NR] + alpa;
+    }
+    for (i = 1; i <= c->ne; i++) c->step[i - 1] = c->p.cfl * c->vol[i - 1] / alpac[i];
+    free(alpac);
+    return 0;
+}
+
+static void inlet_state(const rco_case *c, int which, double pl, double *pr, double *ur, double *vr,
+                        double *tr, double *rr, double *yr, double *g, int *bad)
+{
+    /* oxidizer_inlet_bc oxidizer.f90:130-145 / fuel_inlet_bc fuel.f90:200-215 */
+    const double *yin = which == 3 ? c->p.ox_y : c->p.fu_y;
+    int k;
+    for (k = 0; k < c->nsp; k++) yr[k] = yin[k];
+    *tr = which == 3 ? c->p.ox_t : c->p.fu_t;
+    if (rco_gas_param(&c->m, yr, *tr, g)) *bad = 1;
+    *pr = pl;
+    *rr = *pr / (g[2] * *tr);
+    *ur = (which == 3 ? c->p.ox_g : c->p.fu_g) / *rr;
+    *vr = 0.0;
+}
+
+/* update_ghostcell_variables, update_flow_variables.f90:294-473 */
+static int ghost_update(rco_case *c)
+{
+    int j, k, nsp = c->nsp;
+    for (j = 1; j <= c->ng; j++) {
+        int i = c->hostcell[j - 1];
+        double pl = PA(1, i), ul = PA(2, i), vl = PA(3, i), tl = PA(4, i), yl[RCO_MAXSP], yr[RCO_MAXSP], s = 0.0, g[3];
+        for (k = 0; k < nsp - 1; k++) { yl[k] = PA(NFLOW + 1 + k, i); s = s + yl[k]; }
+        yl[nsp - 1] = 1.0 - s;
+        if (rco_gas_param(&c->m, yl, tl, g)) return fail(c, "ghost: temperature %g out of range", tl);
+        double rl = pl / (g[2] * tl);
+        const double *xv1 = &CGN(1, 1, j), *xv2 = &CGN(1, 2, j);
+        double sav1 = xv2[1] - xv1[1];
+        double sav2 = -(xv2[0] - xv1[0]);
+        double sav = fmax(sqrt(sav1 * sav1 + sav2 * sav2), SMALL);
+        double s1n = sav1 / sav, s2n = sav2 / sav;
+        double ac = sqrt(g[1] * pl / rl);
+        double pr, ur, vr, tr, rr, vn;
+        int bad = 0;
+        switch (c->bctype_g[j - 1]) {
+        case 1:
+            pr = pl; tr = tl; for (k = 0; k < nsp; k++) yr[k] = yl[k];
+            ur = -ul; vr = -vl;
+            break;
+        case 2: case 6:
+            pr = pl; tr = tl; for (k = 0; k < nsp; k++) yr[k] = yl[k];
+            vn = ul * s1n + vl * s2n;
+            ur = ul - 2 * vn * s1n;
+            vr = vl - 2 * vn * s2n;
+            break;
+        case 3: case 4:
+            inlet_state(c, c->bctype_g[j - 1], pl, &pr, &ur, &vr, &tr, &rr, yr, g, &bad);
+            if (bad) return fail(c, "ghost inlet: temperature out of range");
+            break;
+        case 5:
+            vn = ul * s1n + vl * s2n;
+            if (fabs(vn) > ac) { pr = pl; ur = ul; vr = vl; tr = tl; }
+            else { pr = c->p.p_out; ur = ul; vr = vl; tr = tl; }
+            for (k = 0; k < nsp; k++) yr[k] = yl[k];
+            break;
+        default:
+            return fail(c, "error in BCtype of ghost cells: %d", c->bctype_g[j - 1]);
+        }
+        GPA(1, j) = pr; GPA(2, j) = ur; GPA(3, j) = vr; GPA(4, j) = tr;
+        for (k = 0; k < nsp - 1; k++) GPA(NFLOW + 1 + k, j) = yr[k];
+    }
+    return 0;
+}
+
+/* gradient, gradient.f90:4-45 (the limiter is separate, limiter.f90) */
+int rco_gradient(rco_case *c)
+{
+    int i, k, nei, nvar = c->nvar;
+    if (ghost_update(c)) return 1;
+    double C[3][RCO_MAXSP + NFLOW];
+    for (i = 1; i <= c->ne; i++) {
+        int nn = c->neigh[i - 1];
+        for (nei = 1; nei <= nn; nei++) {
+            int j = NEIGHC(nei, i);
+            for (k = 1; k <= nvar; k++) C[nei - 1][k - 1] = WDIS(nei, i) * (PA(k, j) - PA(k, i));
+        }
+        for (nei = 1; nei <= c->num_ghost[i - 1]; nei++) {
+            int j = GHOSTCELL(nei, i);
+            for (k = 1; k <= nvar; k++) C[nei + nn - 1][k - 1] = WDIS(nei + nn, i) * (GPA(k, j) - PA(k, i));
+        }
+        for (k = 1; k <= nvar; k++) {
+            int r;
+            for (r = 1; r <= 2; r++) {
+                double s = 0.0;
+                int m;
+                for (m = 1; m <= 3; m++) s = s + AMLS(r, m, i) * C[m - 1][k - 1];
+                GRADC(r, k, i) = s;
+            }
+        }
+    }
+    return 0;
+}
+
+/* Limiter_Barth, limiter.f90:5-107 */
+int rco_limiter(rco_case *c)
+{
+    int i, k, nei, kf, nvar = c->nvar;
+    double pmin[RCO_MAXSP + NFLOW], pmax[RCO_MAXSP + NFLOW], phi[RCO_MAXSP + NFLOW];
+    for (i = 1; i <= c->ne; i++) {
+        for (k = 1; k <= nvar; k++) { pmin[k - 1] = PA(k, i); pmax[k - 1] = PA(k, i); }
+        for (nei = 1; nei <= c->neigh[i - 1]; nei++) {
+            int j = NEIGHC(nei, i);
+            for (k = 1; k <= nvar; k++) { pmin[k - 1] = fmin(pmin[k - 1], PA(k, j)); pmax[k - 1] = fmax(pmax[k - 1], PA(k, j)); }
+        }
+        for (nei = 1; nei <= c->num_ghost[i - 1]; nei++) {
+            int j = GHOSTCELL(nei, i);
+            for (k = 1; k <= nvar; k++) { pmin[k - 1] = fmin(pmin[k - 1], GPA(k, j)); pmax[k - 1] = fmax(pmax[k - 1], GPA(k, j)); }
+        }
+        for (k = 1; k <= nvar; k++) phi[k - 1] = 1.0;
+        for (k = 1; k <= nvar; k++) {
+            for (kf = 1; kf <= 3; kf++) {
+                int iface = NEIGHF(kf, i);
+                int n1 = FNODE(1, iface), n2 = FNODE(2, iface);
+                double xy0 = 0.5 * (COOR(1, n1) + COOR(1, n2));
+                double xy1 = 0.5 * (COOR(2, n1) + COOR(2, n2));
+                double v0 = (xy0 - CELLXY(1, i)) / REFLEN(1, i);
+                double v1 = (xy1 - CELLXY(2, i)) / REFLEN(2, i);
+                double fv = PA(k, i) + (GRADC(1, k, i) * v0 + GRADC(2, k, i) * v1);
+                double pf;
+                if (fv > PA(k, i)) pf = fmin(1.0, (pmax[k - 1] - PA(k, i)) / (fv - PA(k, i)));
+                else if (fv < PA(k, i)) pf = fmin(1.0, (pmin[k - 1] - PA(k, i)) / (fv - PA(k, i)));
+                else pf = 1.0;
+                phi[k - 1] = fmin(phi[k - 1], pf);
+            }
+        }
+        for (k = 1; k <= nvar; k++) { GRADC(1, k, i) = phi[k - 1] * GRADC(1, k, i); GRADC(2, k, i) = phi[k - 1] * GRADC(2, k, i); }
+    }
+    return 0;
+}
+
+/* flux_lax, flux_lax.f90:4-299, and sourceterm, sourceterm.f90:2-70,
+ * composed as flux_sourceterm, fvm2d.f90:196-224 */
+int rco_flux_sourceterm(rco_case *c)
+{
+    int i, k, nsp = c->nsp, nvar = c->nvar, ns1 = nsp - 1;
+    memset(c->hwa, 0, sizeof(double) * nvar * c->ne);
+    for (i = 1; i <= c->nf; i++) {
+        int NL = NEIGHBOR(1, i), NR = NEIGHBOR(2, i);
+        double dx = VECX(1, i), dy = VECX(2, i);
+        int n1 = FNODE(1, i), n2 = FNODE(2, i);
+        double sav1 = dx, sav2 = dy;
+        double sav = fmax(sqrt(dx * dx + dy * dy), SMALL);
+        double s1n = sav1 / sav, s2n = sav2 / sav;
+        double xy0 = COOR(1, n1) + 0.5 * (COOR(1, n2) - COOR(1, n1));
+        double xy1 = COOR(2, n1) + 0.5 * (COOR(2, n2) - COOR(2, n1));
+        double v0 = (xy0 - CELLXY(1, NL)) / REFLEN(1, NL);
+        double v1 = (xy1 - CELLXY(2, NL)) / REFLEN(2, NL);
+        double pl = PA(1, NL) + (GRADC(1, 1, NL) * v0 + GRADC(2, 1, NL) * v1);
+        double ul = PA(2, NL) + (GRADC(1, 2, NL) * v0 + GRADC(2, 2, NL) * v1);
+        double vl = PA(3, NL) + (GRADC(1, 3, NL) * v0 + GRADC(2, 3, NL) * v1);
+        double tl = PA(4, NL) + (GRADC(1, 4, NL) * v0 + GRADC(2, 4, NL) * v1);
+        double yl[RCO_MAXSP], yr[RCO_MAXSP], gl[3], gr[3], s = 0.0;
+        for (k = 1; k <= ns1; k++) { yl[k - 1] = PA(k + NFLOW, NL) + (GRADC(1, k + NFLOW, NL) * v0 + GRADC(2, k + NFLOW, NL) * v1); }
+        for (k = 0; k < ns1; k++) s = s + yl[k];
+        yl[nsp - 1] = 1.0 - s;
+        if (rco_gas_param(&c->m, yl, tl, gl)) return fail(c, "flux: temperature %g out of range (face %d)", tl, i);
+        double rl = pl / (gl[2] * tl);
+        double pr, ur, vr, tr, rr, vn;
+        int bad = 0;
+        if (NR > 0) {
+            double w0 = (xy0 - (CELLXY(1, NR) - 0.0)) / REFLEN(1, NR);
+            double w1 = (xy1 - (CELLXY(2, NR) - 0.0)) / REFLEN(2, NR);
+            pr = PA(1, NR) + (GRADC(1, 1, NR) * w0 + GRADC(2, 1, NR) * w1);
+            ur = PA(2, NR) + (GRADC(1, 2, NR) * w0 + GRADC(2, 2, NR) * w1);
+            vr = PA(3, NR) + (GRADC(1, 3, NR) * w0 + GRADC(2, 3, NR) * w1);
+            tr = PA(4, NR) + (GRADC(1, 4, NR) * w0 + GRADC(2, 4, NR) * w1);
+            s = 0.0;
+            for (k = 1; k <= ns1; k++) yr[k - 1] = PA(k + NFLOW, NR) + (GRADC(1, k + NFLOW, NR) * w0 + GRADC(2, k + NFLOW, NR) * w1);
+            for (k = 0; k < ns1; k++) s = s + yr[k];
+            yr[nsp - 1] = 1.0 - s;
+            if (rco_gas_param(&c->m, yr, tr, gr)) return fail(c, "flux: temperature %g out of range (face %d)", tr, i);
+            rr = pr / (gr[2] * tr);
+        } else if (NR == -1) {
+            rr = rl; pr = pl; tr = tl; ur = -ul; vr = -vl;
+            for (k = 0; k < nsp; k++) yr[k] = yl[k];
+            gr[0] = gl[0]; gr[1] = gl[1]; gr[2] = gl[2];
+        } else if (NR == -2 || NR == -6) {
+            rr = rl; pr = pl; tr = tl;
+            vn = ul * s1n + vl * s2n;
+            ur = ul - 2 * vn * s1n;
+            vr = vl - 2 * vn * s2n;
+            for (k = 0; k < nsp; k++) yr[k] = yl[k];
+            gr[0] = gl[0]; gr[1] = gl[1]; gr[2] = gl[2];
+        } else if (NR == -3 || NR == -4) {
+            inlet_state(c, -NR, pl, &pr, &ur, &vr, &tr, &rr, yr, gr, &bad);
+            if (bad) return fail(c, "flux inlet: temperature out of range");
+            pl = pr; ul = ur; vl = vr; tl = tr; rl = rr;
+            for (k = 0; k < nsp; k++) yl[k] = yr[k];
+            gl[0] = gr[0]; gl[1] = gr[1]; gl[2] = gr[2];
+        } else if (NR == -5) {
+            double ac = sqrt(gl[1] * pl / rl);
+            vn = ul * s1n + vl * s2n;
+            if (fabs(vn) > ac) {
+                rr = rl; ur = ul; vr = vl; pr = pl; tr = tl;
+            } else {
+                pr = c->p.p_out;
+                ur = ul + s1n * (pl - pr) / (rl * ac);
+                vr = vl + s2n * (pl - pr) / (rl * ac);
+                rr = rl + (pr - pl) / (ac * ac);
+                tr = pr / (gl[2] * rr);
+                pl = pr; ul = ur; vl = vr; tl = tr; rl = rr;
+            }
+            for (k = 0; k < nsp; k++) yr[k] = yl[k];
+            gr[0] = gl[0]; gr[1] = gl[1]; gr[2] = gl[2];
+        } else {
+            return fail(c, "error,no face of this type! %d %d", i, NR);
+        }
+        if (rl < 0. || pl < 0. || (NR > 0 && (rr < 0. || pr < 0.)))
+            return fail(c, "error,if: face %d rl=%g rr=%g pl=%g pr=%g", i, rl, rr, pl, pr);
+        double h[RCO_MAXSP], hl = 0.0, hr = 0.0;
+        if (rco_enthalpy(&c->m, tl, h)) return fail(c, "flux enthalpy: temperature %g out of range", tl);
+        for (k = 0; k < nsp; k++) hl = hl + yl[k] * h[k];
+        if (rco_enthalpy(&c->m, tr, h)) return fail(c, "flux enthalpy: temperature %g out of range", tr);
+        for (k = 0; k < nsp; k++) hr = hr + yr[k] * h[k];
+        double vaml = ul * ul + vl * vl, vamr = ur * ur + vr * vr;
+        hl = hl + 0.5 * vaml;
+        hr = hr + 0.5 * vamr;
+        double el = hl - pl / rl, er = hr - pr / rr;
+        double acl = sqrt(gl[1] * pl / rl), acr = sqrt(gr[1] * pr / rr);
+        double uln = sav1 * ul + sav2 * vl, urn = sav1 * ur + sav2 * vr;
+        double coex = fmax(fabs(uln) + acl * sav, fabs(urn) + acr * sav);
+        double ruln = rl * uln, rurn = rr * urn;
+        double flu[RCO_MAXSP + NFLOW];
+        flu[0] = 0.5 * (ruln + rurn - coex * (rr - rl));
+        flu[1] = 0.5 * (ruln * ul + rurn * ur + sav1 * (pl + pr) - coex * (rr * ur - rl * ul));
+        flu[2] = 0.5 * (ruln * vl + rurn * vr + sav2 * (pl + pr) - coex * (rr * vr - rl * vl));
+        flu[3] = 0.5 * (ruln * hl + rurn * hr - coex * (rr * er - rl * el));
+        for (k = 0; k < ns1; k++) flu[NFLOW + k] = 0.5 * (ruln * yl[k] + rurn * yr[k] - coex * (rr * yr[k] - rl * yl[k]));
+        for (k = 1; k <= nvar; k++) {
+            HWA(k, NL) = HWA(k, NL) - flu[k - 1] * 1.0;
+            if (NR > 0) HWA(k, NR) = HWA(k, NR) + flu[k - 1] * 1.0;
+        }
+    }
+    if (!c->p.combustion) return 0;
+    for (i = 1; i <= c->ne; i++) {
+        double p = PA(1, i), t = PA(4, i), y[RCO_MAXSP], s = 0.0, g[3], w[RCO_MAXSP];
+        for (k = 0; k < ns1; k++) { y[k] = PA(NFLOW + 1 + k, i); s = s + y[k]; }
+        y[nsp - 1] = 1.0 - s;
+        if (rco_gas_param(&c->m, y, t, g)) return fail(c, "source: temperature %g out of range", t);
+        double rho = p / (t * g[2]);
+        rco_production_rate(&c->m, t, rho, y, w);
+        for (k = 0; k < ns1; k++) c->prod[(i - 1) * nsp + k] = w[k];
+        for (k = 1; k <= NFLOW; k++) HWA(k, i) = HWA(k, i) + c->vol[i - 1] * 0.0;
+        for (k = 0; k < ns1; k++) HWA(NFLOW + 1 + k, i) = HWA(NFLOW + 1 + k, i) + c->vol[i - 1] * w[k];
+    }
+    return 0;
+}
+
+/* the neighbour term of both LU-SGS sweeps, Time_LUSGS_steady.f90:72-111 */
+static int lusgs_neighbor(rco_case *c, int NR, double dx, double dy, double af, double *dflux)
+{
+    int k, nsp = c->nsp, ns1 = nsp - 1;
+    double pp = PA(1, NR), uu = PA(2, NR), vv = PA(3, NR), tt = PA(4, NR), yy[RCO_MAXSP], s = 0.0, g[3], h[RCO_MAXSP];
+    for (k = 0; k < ns1; k++) { yy[k] = PA(NFLOW + 1 + k, NR); s = s + yy[k]; }
+    yy[nsp - 1] = 1.0 - s;
+    if (rco_gas_param(&c->m, yy, tt, g)) return fail(c, "lusgs: temperature %g out of range", tt);
+    if (rco_enthalpy(&c->m, tt, h)) return fail(c, "lusgs: temperature %g out of range", tt);
+    double hh = 0.0;
+    for (k = 0; k < nsp; k++) hh = hh + yy[k] * h[k];
+    double gama = g[1], rcp = g[2];
+    double rr = pp / (rcp * tt);
+    double rou = rr * uu, rov = rr * vv;
+    double roe = rr * (hh + 0.5 * (uu * uu + vv * vv)) - pp;
+    double ry[RCO_MAXSP], dry[RCO_MAXSP];
+    for (k = 0; k < nsp; k++) ry[k] = rr * yy[k];
+    double drr = HWA(1, NR), dru = HWA(2, NR), drv = HWA(3, NR), dre = HWA(4, NR);
+    s = 0.0;
+    for (k = 0; k < ns1; k++) { dry[k] = HWA(NFLOW + 1 + k, NR); s = s + dry[k]; }
+    dry[nsp - 1] = drr - s;
+    double duu = (dru - uu * drr) / rr;
+    double dvv = (drv - vv * drr) / rr;
+    double s1 = 0.0, s2 = 0.0;
+    for (k = 0; k < nsp; k++) s1 = s1 + h[k] * dry[k];
+    for (k = 0; k < nsp; k++) s2 = s2 + (c->m.rconstant / c->m.mw[k]) * dry[k];
+    double dpp = (gama - 1) * (dre - s1 + tt * s2 * gama / (gama - 1)
+                               - 0.5 * (dru * uu + duu * rou + drv * vv + dvv * rov));
+    double un = uu * dx + vv * dy;
+    double dun = duu * dx + dvv * dy;
+    dflux[0] = dflux[0] + 0.5 * (-af * drr + dru * dx + drv * dy);
+    dflux[1] = dflux[1] + 0.5 * (-af * dru + dru * un + rou * dun + dpp * dx);
+    dflux[2] = dflux[2] + 0.5 * (-af * drv + drv * un + rov * dun + dpp * dy);
+    dflux[3] = dflux[3] + 0.5 * (-af * dre + dun * (roe + pp) + un * (dre + dpp));
+    for (k = 0; k < ns1; k++) dflux[NFLOW + k] = dflux[NFLOW + k] + 0.5 * (-af * dry[k] + dry[k] * un + ry[k] * dun);
+    return 0;
+}
+
+/* update_primitve_variables, update_flow_variables.f90:208-252 */
+static int update_primitive(rco_case *c)
+{
+    int i, k, nsp = c->nsp, ns1 = nsp - 1;
+    for (i = 1; i <= c->ne; i++) {
+        double rho = WA(1, i), u = WA(2, i) / rho, v = WA(3, i) / rho, y[RCO_MAXSP], s = 0.0, o[4];
+        for (k = 0; k < ns1; k++) y[k] = WA(NFLOW + 1 + k, i) / rho;
+        for (k = 0; k < ns1; k++) s = s + y[k];
+        y[nsp - 1] = 1.0 - s;
+        if (rco_newton(&c->m, WA(4, i) / rho - 0.5 * (u * u + v * v), y, PA(4, i), o))
+            return fail(c, "update: temperature out of range (cell %d)", i);
+        double t = o[0];
+        double p = rho * o[3] * t;
+        PA(1, i) = p; PA(2, i) = u; PA(3, i) = v; PA(4, i) = t;
+        for (k = 0; k < ns1; k++) PA(NFLOW + 1 + k, i) = y[k];
+    }
+    return 0;
+}
+
+/* LU_SGS_steady, Time_LUSGS_steady.f90:32-204 */
+int rco_lusgs_steady(rco_case *c)
+{
+    int i, k, nei, nvar = c->nvar;
+    double dflux[RCO_MAXSP + NFLOW];
+    for (i = 1; i <= c->ne; i++) {
+        double d = 0.0;
+        for (nei = 1; nei <= 3; nei++) d = d + 0.5 * c->alpaf[NEIGHF(nei, i) - 1];
+        c->dd[i - 1] = d + c->vol[i - 1] / c->step[i - 1];
+    }
+    for (i = 1; i <= c->ne; i++) {
+        for (k = 0; k < nvar; k++) dflux[k] = 0.0;
+        for (nei = 1; nei <= 3; nei++) {
+            int iface = NEIGHF(nei, i);
+            double dx = VECX(1, iface), dy = VECX(2, iface);
+            int NR = NEIGHBOR(2, iface);
+            if (i == NR) { NR = NEIGHBOR(1, iface); dx = -dx; dy = -dy; }
+            if (NR > 0 && NR < i)
+                if (lusgs_neighbor(c, NR, dx, dy, c->alpaf[iface - 1], dflux)) return 1;
+        }
+        for (k = 1; k <= nvar; k++) HWA(k, i) = (HWA(k, i) - dflux[k - 1]) / c->dd[i - 1];
+    }
+    for (i = c->ne; i >= 1; i--) {
+        for (k = 0; k < nvar; k++) dflux[k] = 0.0;
+        for (nei = 1; nei <= 3; nei++) {
+            int iface = NEIGHF(nei, i);
+            double dx = VECX(1, iface), dy = VECX(2, iface);
+            int NR = NEIGHBOR(2, iface);
+            if (i == NR) { NR = NEIGHBOR(1, iface); dx = -dx; dy = -dy; }
+            if (NR > 0 && NR > i)
+                if (lusgs_neighbor(c, NR, dx, dy, c->alpaf[iface - 1], dflux)) return 1;
+        }
+        for (k = 1; k <= nvar; k++) HWA(k, i) = HWA(k, i) - dflux[k - 1] / c->dd[i - 1];
+    }
+    for (i = 1; i <= c->ne; i++)
+        for (k = 1; k <= nvar; k++) WA(k, i) = WA(k, i) + HWA(k, i);
+    return update_primitive(c);
+}
+
+/* lu_sgs_stage, Time_LUSGS_steady.f90:4-30 (reconstruction = gradient +
+ * Limiter_Barth when limit_ID>0, fvm2d.f90:183-191) */
+int rco_stage(rco_case *c)
+{
+    if (rco_timestep(c)) return 1;
+    if (rco_gradient(c)) return 1;
+    if (c->p.limit_id > 0 && rco_limiter(c)) return 1;
+    if (rco_flux_sourceterm(c)) return 1;
+    if (rco_lusgs_steady(c)) return 1;
+    if (isnan(WA(1, 1))) return fail(c, "there is NAN in WA(1,1)");
+    return 0;
+}
+
+/* Residual + step_residual normalisation, residual.f90:4-21,61-79 */
+void rco_residual(rco_case *c, double *res)
+{
+    int i, k, nvar = c->nvar;
+    double romax = 0.0;
+    for (k = 0; k < nvar; k++) res[k] = 0.0;
+    for (i = 1; i <= c->ne; i++) {
+        for (k = 1; k <= nvar; k++) res[k - 1] = res[k - 1] + c->vol[i - 1] * fabs(HWA(k, i));
+        romax = fmax(romax, WA(1, i));
+    }
+    for (k = 0; k < nvar; k++) res[k] = res[k] / c->total_volume;
+    res[nvar] = romax;
+}

@@ -1,0 +1,70 @@
+/* Test infrastructure only (not the product): a plain-C, serial restatement
+ * of the reference solver's steady residual + LU-SGS path, used by tests/
+ * and by bench.py's CPU-baseline leg as the checker.  Every routine follows
+ * the reference loop order and operation order; see rc_oracle.c for the
+ * file:line each one restates.
+ *
+ * Arrays mirror the reference's column-major Fortran arrays with 1-based
+ * cell/face/node numbers kept as values (Neighbor, NeighF, NeighC hold
+ * 1-based indices, boundary codes are negative) so the restatement reads
+ * like the original.  Flat storage is 0-based: PA(k,i) -> pa[(i-1)*nvar+k-1].
+ */
+#ifndef RC_ORACLE_H
+#define RC_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RCO_MAXSP 8
+
+typedef struct rco_mech {
+    int nsp;
+    double mw[RCO_MAXSP];
+    double rconstant;
+    double t_low, t_crit, t_high;
+    double low[RCO_MAXSP][7], high[RCO_MAXSP][7];
+    double low_b1[RCO_MAXSP], high_b1[RCO_MAXSP];
+    int model;
+    double coef_f[RCO_MAXSP], coef_r[RCO_MAXSP];
+    double a_f, e_act;
+} rco_mech;
+
+typedef struct rco_params {
+    double cfl;
+    int limit_id;
+    int combustion;
+    double p_out;
+    double ox_y[RCO_MAXSP], ox_t, ox_g;   /* oxidizer inlet: Y, T, mass flux */
+    double fu_y[RCO_MAXSP], fu_t, fu_g;   /* fuel inlet */
+} rco_params;
+
+typedef struct rco_case rco_case;
+
+int  rco_create(const rco_mech *mech, const rco_params *par, int nnt, const double *coor,
+                int ne, const int *cells, int nb, const int *bedges, const int *btype,
+                rco_case **out);
+void rco_destroy(rco_case *c);
+const char *rco_last_error(const rco_case *c);
+void rco_sizes(const rco_case *c, int *out4);            /* ne, nf, nghost, nvar */
+double *rco_darray(rco_case *c, const char *name);
+int *rco_iarray(rco_case *c, const char *name);
+
+int  rco_set_pa(rco_case *c, const double *pa);           /* PA(nvar,ne) -> PA, WA */
+int  rco_timestep(rco_case *c);
+int  rco_gradient(rco_case *c);
+int  rco_limiter(rco_case *c);
+int  rco_flux_sourceterm(rco_case *c);
+int  rco_lusgs_steady(rco_case *c);
+int  rco_stage(rco_case *c);                              /* lu_sgs_stage */
+void rco_residual(rco_case *c, double *res);              /* nvar L1 norms / total volume, then romax */
+
+/* leaf thermochemistry, for unit tests */
+int  rco_gas_param(const rco_mech *m, const double *y, double t, double *cp_gam_r);
+int  rco_enthalpy(const rco_mech *m, double t, double *h);
+int  rco_newton(const rco_mech *m, double et, const double *y, double t0, double *t_cp_gam_r);
+void rco_production_rate(const rco_mech *m, double t, double rho, const double *y, double *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,328 @@
+"""Input decks of the reference solver: mechanism, inlet and solver settings.
+
+The reference reads positional, list-directed text files from its working
+directory (one comment line before each value group):
+
+* ``combustion/species.in``            -- ``setup_species_info``     (src/combustion_modeling/species.f90:14-51)
+* ``combustion/thermo.in``             -- ``setup_thermo_property``  (src/combustion_modeling/thermo_property.f90:18-59)
+* ``combustion/chemical_kinetics.in``  -- ``setup_chemical_kinetics`` (src/combustion_modeling/chemical_kinetics.f90:198-239)
+* ``combustion/oxidizer.in``           -- ``setup_oxidizer_info``    (src/combustion_modeling/oxidizer.f90:90-118)
+* ``combustion/fuel.in``               -- ``setup_fuel_info``        (src/combustion_modeling/fuel.f90:159-187)
+* ``flow/flow_solver_info.in``         -- ``startup``                (src/startup.f90:55-128)
+
+This module reads and writes those layouts so that the same deck drives the
+reference and this library.  The values held by :func:`default_mechanism`
+are the reference's shipped H2/O2/H2O/Ar mechanism (species masses, the
+7-term cp/R polynomials and the one-step reaction), i.e. input data.
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+import re
+from typing import List, Optional
+
+import numpy as np
+
+NFLOW = 4  # src/module.f90:8
+
+
+@dataclasses.dataclass
+class Mechanism:
+    """Species, thermodynamic and kinetic tables (module ``species``,
+    ``thermo_property`` and ``chemical_kinetics`` of the reference)."""
+    names: List[str]
+    mole_weight: np.ndarray          # (Nsp,) kg/mol
+    rconstant: float                 # J/mol/K
+    tref: float
+    t_low: float
+    t_crit: float
+    t_high: float
+    low_coef: np.ndarray             # (Nsp, 7)  cp/R polynomial, basis T^(j-3)
+    low_b1: np.ndarray               # (Nsp,)
+    high_coef: np.ndarray            # (Nsp, 7)
+    high_b1: np.ndarray              # (Nsp,)
+    hof: np.ndarray                  # (Nsp,) J/mol (unused on the path)
+    model: int                       # code_chemical_model: 0 Arrhenius, 1 threshold
+    coef_f: np.ndarray               # (Nsp,) forward stoichiometric coefficients of reaction 1
+    coef_r: np.ndarray               # (Nsp,) reverse stoichiometric coefficients of reaction 1
+    a_f: float
+    beta: float
+    e_act: float
+    dentropy: float = 0.0
+    denthalpy: float = 0.0
+    p_a: float = 1.0e5
+
+    @property
+    def nsp(self) -> int:
+        return len(self.names)
+
+    @property
+    def nvar(self) -> int:
+        return NFLOW + self.nsp - 1
+
+    @property
+    def r_species(self) -> np.ndarray:
+        # R_species(i)= Rconstant/mole_weight(i)  (species.f90:42-44)
+        return self.rconstant / self.mole_weight
+
+
+@dataclasses.dataclass
+class Inlet:
+    """Mass-flux inlet (``oxidizer.in`` / ``fuel.in``).  ``mass_flow`` is the
+    deck value; the reference divides it by the inlet height measure set in
+    ``compute_fuel_oxidizer_diameter`` (src/initialization.f90:22-23,194-195)."""
+    y: np.ndarray                    # (Nsp,)
+    temp: float
+    pres: float
+    mass_flow: float
+
+
+@dataclasses.dataclass
+class SolverInfo:
+    """The fields of ``flow/flow_solver_info.in`` (src/startup.f90:60-120)."""
+    combustion: bool = True
+    if_right_domain: bool = False
+    n_previous_par: int = 1
+    npar: int = 1
+    unit: float = 1.0
+    steady: bool = True
+    viscous: bool = False
+    read_backup: bool = True
+    explicit: bool = False
+    implicit_time_method: int = 5
+    cfl: float = 5.0
+    totaltime: float = 1.0e30
+    nstep_max: int = 2000
+    save_start_time: float = 0.0
+    save_end_time: float = 0.0
+    fix_dt: bool = False
+    inloop: int = 1
+    innerloop: int = 1
+    reltol: float = 1.0e-3
+    n_screen: int = 1
+    n_file_output: int = 100000000
+    n_file_backup: int = 100000000
+    n_write_monitor: int = 100000000
+    rc_detector: float = 3.0
+    n_wbap: int = 10
+    limit_id: int = 1
+    p_out_ratio: float = 0.02
+    a_p: float = 0.0
+    f_p: float = 0.0
+
+
+def default_mechanism(model: int = 1, b1_in_kelvin: bool = False) -> Mechanism:
+    """The reference's shipped mechanism (src/combustion/species.in,
+    thermo.in, chemical_kinetics.in).
+
+    ``b1_in_kelvin`` divides the tabulated H2O formation constant (given in
+    J/mol in the shipped file, but multiplied by R_k as if it were K,
+    thermo_property.f90:131) by R; both codes read whatever the file holds.
+    """
+    names = ["H2", "O2", "H2O", "Ar"]
+    mw = np.array([2.0e-3, 32.0e-3, 18.0e-3, 40.0e-3])
+    rconst = 8.31446261815324
+    coef = np.zeros((4, 7))
+    coef[:, 2] = 3.6084
+    b1 = np.array([0.0, 0.0, -241826.0, 0.0])
+    if b1_in_kelvin:
+        b1 = b1 / rconst
+    return Mechanism(
+        names=names, mole_weight=mw, rconstant=rconst, tref=298.15,
+        t_low=200.0, t_crit=1000.0, t_high=60000.0,
+        low_coef=coef.copy(), low_b1=b1.copy(), high_coef=coef.copy(), high_b1=b1.copy(),
+        hof=np.array([0.0, 0.0, -241826.0, 0.0]),
+        model=model,
+        coef_f=np.array([2.0, 1.0, 0.0, 0.0]), coef_r=np.array([0.0, 0.0, 2.0, 0.0]),
+        a_f=2.12e11, beta=0.0, e_act=2.025e5, dentropy=0.0, denthalpy=0.0, p_a=1.0e5)
+
+
+# ----------------------------------------------------------------------------
+# list-directed text helpers
+# ----------------------------------------------------------------------------
+
+class _Reader:
+    """Reads value records of a list-directed file, skipping one comment line
+    where the reference does ``read(iofile,*)``."""
+
+    def __init__(self, path: str):
+        with open(path) as f:
+            self.lines = f.read().splitlines()
+        self.pos = 0
+
+    def skip(self):
+        self.pos += 1
+
+    def values(self) -> List[str]:
+        line = self.lines[self.pos]
+        self.pos += 1
+        toks = [t for t in re.split(r"[,\s]+", line.strip()) if t]
+        return toks
+
+
+def _f(tok: str) -> float:
+    return float(tok.lower().replace("d", "e"))
+
+
+def _b(tok: str) -> bool:
+    t = tok.strip().lower().strip(".")
+    return t in ("true", "t")
+
+
+def read_mechanism(case_dir: str, model: Optional[int] = None) -> Mechanism:
+    comb = os.path.join(case_dir, "combustion")
+    r = _Reader(os.path.join(comb, "species.in"))
+    r.skip(); nsp = int(r.values()[0]); r.skip()
+    names, mw = [], []
+    for _ in range(nsp):
+        v = r.values(); names.append(v[0]); mw.append(_f(v[1]))
+    r.skip(); rconst = _f(r.values()[0])
+
+    r = _Reader(os.path.join(comb, "thermo.in"))
+    r.skip(); tref = _f(r.values()[0])
+    r.skip(); order = int(r.values()[0])
+    if order != 7:
+        raise ValueError("only 7-term cp polynomials are supported")
+    r.skip(); tl, tc, th = [_f(t) for t in r.values()[:3]]
+    low, lb1, high, hb1 = [], [], [], []
+    for _ in range(nsp):
+        low.append([_f(t) for t in r.values()[:7]]); lb1.append(_f(r.values()[0]))
+        high.append([_f(t) for t in r.values()[:7]]); hb1.append(_f(r.values()[0]))
+    r.skip(); hof = [_f(r.values()[0]) for _ in range(nsp)]
+
+    r = _Reader(os.path.join(comb, "chemical_kinetics.in"))
+    r.skip(); v = r.values(); nreact, code = int(v[0]), int(v[1])
+    if nreact != 1:
+        raise ValueError("the reference's production rate handles one reaction")
+    r.skip()
+    cf, cr = [], []
+    for _ in range(nsp):
+        v = r.values(); cf.append(_f(v[0])); cr.append(_f(v[1]))
+    r.skip(); a, beta, e = [_f(t) for t in r.values()[:3]]
+    r.skip(); ds, dh = [_f(t) for t in r.values()[:2]]
+    r.skip(); p_a = _f(r.values()[0])
+    return Mechanism(names=names, mole_weight=np.array(mw), rconstant=rconst, tref=tref,
+                     t_low=tl, t_crit=tc, t_high=th,
+                     low_coef=np.array(low), low_b1=np.array(lb1),
+                     high_coef=np.array(high), high_b1=np.array(hb1), hof=np.array(hof),
+                     model=code if model is None else model,
+                     coef_f=np.array(cf), coef_r=np.array(cr), a_f=a, beta=beta, e_act=e,
+                     dentropy=ds, denthalpy=dh, p_a=p_a)
+
+
+def read_inlet(path: str, nsp: int) -> Inlet:
+    r = _Reader(path)
+    r.skip(); y = [_f(r.values()[0]) for _ in range(nsp)]
+    r.skip(); v = r.values(); t, p = _f(v[0]), _f(v[1])
+    r.skip(); m = _f(r.values()[0])
+    return Inlet(y=np.array(y), temp=t, pres=p, mass_flow=m)
+
+
+def write_mechanism(case_dir: str, mech: Mechanism) -> None:
+    comb = os.path.join(case_dir, "combustion")
+    os.makedirs(comb, exist_ok=True)
+    with open(os.path.join(comb, "species.in"), "w") as f:
+        f.write("# number of species\n%d\n# species name, mole weight (kg/mol)\n" % mech.nsp)
+        for n, m in zip(mech.names, mech.mole_weight):
+            f.write("%s , %.17g\n" % (n, m))
+        f.write("# gas constant R (J/mol/K)\n%.17g\n" % mech.rconstant)
+    with open(os.path.join(comb, "thermo.in"), "w") as f:
+        f.write("# reference temperature\n%.17g\n# order of Cp polynomial\n7\n" % mech.tref)
+        f.write("# Cp polynomial coefficients\n%.17g,%.17g,%.17g\n" % (mech.t_low, mech.t_crit, mech.t_high))
+        for k in range(mech.nsp):
+            f.write(", ".join("%.17g" % c for c in mech.low_coef[k]) + "\n")
+            f.write("%.17g\n" % mech.low_b1[k])
+            f.write(", ".join("%.17g" % c for c in mech.high_coef[k]) + "\n")
+            f.write("%.17g\n" % mech.high_b1[k])
+        f.write("# heat of formation (J/mol)\n")
+        for h in mech.hof:
+            f.write("%.17g\n" % h)
+    with open(os.path.join(comb, "chemical_kinetics.in"), "w") as f:
+        f.write("# number of reactions, chemical model\n1, %d\n" % mech.model)
+        f.write("# forward reaction coefficient, reverse reaction coefficient\n")
+        for a, b in zip(mech.coef_f, mech.coef_r):
+            f.write("%.17g, %.17g\n" % (a, b))
+        f.write("# pre-exponential factor, temperature exponent, activation energy (J)\n")
+        f.write("%.17g, %.17g, %.17g\n" % (mech.a_f, mech.beta, mech.e_act))
+        f.write("# entropy change, enthalpy change\n%.17g, %.17g\n" % (mech.dentropy, mech.denthalpy))
+        f.write("# p_a\n%.17g\n" % mech.p_a)
+
+
+def write_inlet(path: str, inlet: Inlet, what: str) -> None:
+    with open(path, "w") as f:
+        f.write("# %s mass fraction\n" % what)
+        for y in inlet.y:
+            f.write("%.17g\n" % y)
+        f.write("# temperature (K), pressure (Pa)\n%.17g, %.17g\n" % (inlet.temp, inlet.pres))
+        f.write("# mass flow rate (kg/s)\n%.17g\n" % inlet.mass_flow)
+
+
+def write_solver_info(case_dir: str, s: SolverInfo) -> None:
+    d = os.path.join(case_dir, "flow")
+    os.makedirs(d, exist_ok=True)
+    L = lambda b: ".true." if b else ".false."
+    rows = [
+        ("# If_turn_on_combustion", L(s.combustion)),
+        ("# IfRightDomain", L(s.if_right_domain)),
+        ("# NPreviousPar", str(s.n_previous_par)),
+        ("# Npar, number of processors", str(s.npar)),
+        ("# unit", "%.17g" % s.unit),
+        ("# IfSteady, IfViscous, IfReadBackup, IfExplicit",
+         ", ".join(L(b) for b in (s.steady, s.viscous, s.read_backup, s.explicit))),
+        ("# implicit_time_method", str(s.implicit_time_method)),
+        ("# cfl,totaltime,nstep_max,save_start_time, save_end_time",
+         "%.17g, %.17g, %d, %.17g, %.17g" % (s.cfl, s.totaltime, s.nstep_max, s.save_start_time, s.save_end_time)),
+        ("# IfFixDt", L(s.fix_dt)),
+        ("# InLoop, InnerLoop, relTol", "%d, %d, %.17g" % (s.inloop, s.innerloop, s.reltol)),
+        ("# NScreenOutput,NFileOutput,NFileBackup,NWriteMonitor",
+         "%d, %d, %d, %d" % (s.n_screen, s.n_file_output, s.n_file_backup, s.n_write_monitor)),
+        ("# rc_detector", "%.17g" % s.rc_detector),
+        ("# n_wbap,limit_ID", "%d, %d" % (s.n_wbap, s.limit_id)),
+        ("# number of monitor points", "0"),
+        ("# x_monitor_point, y_monitor_point", None),
+        ("# exit pressure ratio, A_p, f_p", "%.17g, %.17g, %.17g" % (s.p_out_ratio, s.a_p, s.f_p)),
+    ]
+    with open(os.path.join(d, "flow_solver_info.in"), "w") as f:
+        for c, v in rows:
+            f.write(c + "\n")
+            if v is not None:
+                f.write(v + "\n")
+
+
+def write_grid(path: str, coor: np.ndarray, cells: np.ndarray, bedges: np.ndarray,
+               btype: np.ndarray, nface: int) -> None:
+    """``grid1.in`` for one rank (src/mesh.f90:10-65).  ``cells`` are 0-based
+    counter-clockwise triangles; ``bedges`` 0-based node pairs with their BC
+    type in ``btype`` (one boundary group per type)."""
+    ne, nn = len(cells), len(coor)
+    types = sorted(set(int(t) for t in btype))
+    groups = [(t, bedges[btype == t]) for t in types]
+    imax = max(len(g) for _, g in groups) if groups else 0
+    with open(path, "w") as f:
+        f.write("%d %d %d %d %d %d\n" % (ne, ne, nn, nn, nface, nface))
+        np.savetxt(f, np.column_stack([np.full(ne, 3), cells + 1]), fmt="%d")
+        np.savetxt(f, np.column_stack([coor, np.zeros((nn, 2))]), fmt=["%.17g", "%.17g", "%d", "%d"])
+        f.write("0\n")
+        f.write("%d %d\n" % (len(groups), imax))
+        for t, g in groups:
+            f.write("%d %d\n" % (t, len(g)))
+            np.savetxt(f, g + 1, fmt="%d")
+        f.write("0\n")
+
+
+def write_backup(path: str, cellxy: np.ndarray, pa: np.ndarray, ttime: float = 0.0) -> None:
+    """``fvp1.sav`` (src/output.f90:256-273 writer, :316-327 reader): ``ttime``,
+    ``NE``, then ``cx cy PA(1:Nvar)`` per cell."""
+    with open(path, "w") as f:
+        f.write("%.17g\n%d\n" % (ttime, len(pa)))
+        np.savetxt(f, np.column_stack([cellxy, pa]), fmt="%.17g")
+
+
+def read_backup(path: str, nvar: int):
+    with open(path) as f:
+        ttime = _f(f.readline().split()[0])
+        ne = int(f.readline().split()[0])
+        data = np.loadtxt(f, ndmin=2)
+    data = data.reshape(ne, 2 + nvar)
+    return ttime, data[:, :2], data[:, 2:]
