@@ -1,0 +1,167 @@
+/* rocketcombustor.h -- C-ABI of the MI355X (gfx950) FP64 steady residual +
+ * LU-SGS path of RocketCombustor.
+ *
+ * The reference has no plugin or FFI layer: its hot operators are
+ * argument-less Fortran subroutines sharing `module mainvar` state.  Each
+ * entry point below replaces one of them for a host that keeps the
+ * reference's call sequence (fvm2d.f90 main loop -> lu_sgs_stage); the
+ * Fortran binding is include/rocketcombustor_iface.f90 (ISO_C_BINDING).
+ *
+ * Conventions
+ *  - FP64 throughout, int32 indices.
+ *  - Arrays crossing the boundary use the reference's column-major layout:
+ *    PA(Nvar,NE), WA(Nvar,NE), hWA(Nvar,NE), GradC(2,Nvar,NE), alpaF(NF),
+ *    step(NE); in C terms pa[i*nvar + k] is PA(k+1,i+1).
+ *  - Node numbers in cell and boundary lists are 1-based, as in grid<N>.in
+ *    and N(4,NE) (src/mesh.f90:20-58).
+ *  - The library deep-copies at create/set, keeps all state resident in
+ *    device memory between calls and copies back only on rc_get_* calls.
+ *  - Calls are synchronous (stream-ordered, the stream is synchronised before
+ *    return) unless documented otherwise.  A handle is not re-entrant;
+ *    separate handles may be driven from separate host threads.
+ *  - Every call returns 0 on success or an RC_ERR_* code; rc_last_error()
+ *    gives the message.  Where the reference executes `stop`, the library
+ *    returns the code instead (a Fortran host turns it into `stop`).
+ */
+#ifndef ROCKETCOMBUSTOR_H
+#define ROCKETCOMBUSTOR_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RC_MAXSP 8
+
+enum {
+    RC_OK = 0,
+    RC_ERR_ARGUMENT = 1,      /* invalid argument / unsupported input */
+    RC_ERR_TEMPERATURE = 2,   /* T outside the polynomial range: thermo_property.f90:97-100,137-141 */
+    RC_ERR_NEGATIVE = 3,      /* rho or p < 0: flux_lax.f90:239-242, TimeStep.f90:66-70 */
+    RC_ERR_NAN = 4,           /* NaN in WA(1,1): Time_LUSGS_steady.f90:24-27 */
+    RC_ERR_MESH = 5,          /* mesh set-up checks of mesh.f90 / ghostcell.f90 / reconstruction_initialization.f90 */
+    RC_ERR_DEVICE = 6,        /* HIP runtime error */
+    RC_ERR_MEMORY = 7         /* device memory exhausted / budget exceeded */
+};
+
+/* Species, thermo and kinetics tables: module species (species.f90:4-12),
+ * thermo_property (thermo_property.f90:4-16), chemical_kinetics
+ * (chemical_kinetics.f90:186-195), filled by setup_species_info,
+ * setup_thermo_property and setup_chemical_kinetics. */
+typedef struct rc_mechanism {
+    int nsp;                            /* species count (Nvar = 4 + nsp - 1); the kernels are built for nsp = 4 */
+    double mole_weight[RC_MAXSP];       /* kg/mol */
+    double rconstant;                   /* J/mol/K */
+    double t_low, t_crit, t_high;       /* polynomial ranges */
+    double low_coef[RC_MAXSP][7];       /* cp/R coefficients of T^(j-3), j=1..7 */
+    double high_coef[RC_MAXSP][7];
+    double low_b1[RC_MAXSP], high_b1[RC_MAXSP];
+    int model;                          /* code_chemical_model: 0 Arrhenius, 1 threshold */
+    double coef_f[RC_MAXSP], coef_r[RC_MAXSP];
+    double a_f, beta, e_act;
+} rc_mechanism;
+
+/* Run parameters that the reference keeps in module globals: CFL
+ * (startup.f90:91), limit_ID (:108), If_turn_on_combustion (:61), p_out
+ * (initialization.f90:32-33), the inlet states of oxidizer.in / fuel.in with
+ * the mass flux after initialization.f90:22-23. */
+typedef struct rc_params {
+    double cfl;
+    int limit_id;                       /* >0: Barth limiter after the gradient */
+    int combustion;                     /* 0: no source term (sourceterm.f90:308) */
+    double p_out;
+    double ox_y[RC_MAXSP], ox_t, ox_mass_flux;
+    double fu_y[RC_MAXSP], fu_t, fu_mass_flux;
+} rc_params;
+
+typedef struct rc_case rc_case;
+
+/* Replaces ReadMesh (mesh.f90:2-115: faces by `change`, BC tags,
+ * VolumeAndCenter) and compute_reconstruction_matrix
+ * (reconstruction_initialization.f90:1-169: NeighF/NeighC, ghost cells of
+ * ghostcell.f90:203-379, inverse-distance weights, pinv(A) of la_math.f90:1-62)
+ * for a one-rank triangle mesh.
+ *   coor[2*nnt]      node coordinates Coor(2,NNT)
+ *   cellnodes[3*ne]  1-based counter-clockwise triangle nodes (N(1:3,NE))
+ *   bnodes[2*nb]     1-based node pairs of boundary faces, btype[nb] their
+ *                    reference BC type (1 wall, 2 symmetry, 3 oxidizer inlet,
+ *                    4 fuel inlet, 5 outlet, 6 slip wall)
+ *   device           HIP device ordinal
+ * The cell order given is the LU-SGS sweep order, exactly as in the
+ * reference (Time_LUSGS_steady.f90:56,127); see rc_mesh_order_multicolor. */
+int rc_case_create(const rc_mechanism *mech, const rc_params *par,
+                   int nnt, const double *coor, int ne, const int *cellnodes,
+                   int nb, const int *bnodes, const int *btype,
+                   int device, rc_case **out);
+void rc_case_destroy(rc_case *h);
+const char *rc_last_error(const rc_case *h);           /* thread-local message when h is NULL */
+
+/* sizes[0..5] = NE, NF, number of ghost cells, Nvar, number of LU-SGS
+ * wavefront levels, resident device bytes. */
+int rc_case_info(const rc_case *h, long long *sizes6);
+
+/* Replaces the PA load of Read_Backup (output.f90:314-327) followed by
+ * primitve_to_conserved_variables (update_flow_variables.f90:254-288). */
+int rc_set_primitive(rc_case *h, const double *pa);
+
+/* Replaces TimeStep, steady inviscid branch (TimeStep.f90:4-103): alpaF,
+ * local step(i) = CFL*vol/sum(alpaF). */
+int rc_timestep(rc_case *h);
+/* Replaces gradient (gradient.f90:4-45) including
+ * update_ghostcell_variables (update_flow_variables.f90:294-473). */
+int rc_gradient(rc_case *h);
+/* Replaces Limiter_Barth (limiter.f90:5-107), applied to the current GradC. */
+int rc_limiter_barth(rc_case *h);
+/* Replaces reconstruction (fvm2d.f90:183-191): gradient, then
+ * Limiter_Barth when limit_ID>0, fused in one kernel. */
+int rc_reconstruction(rc_case *h);
+/* Replaces flux_sourceterm (fvm2d.f90:196-224): hWA = 0, flux_lax
+ * (flux_lax.f90:4-299), sourceterm (sourceterm.f90:2-70). */
+int rc_flux_sourceterm(rc_case *h);
+/* Replaces LU_SGS_steady (Time_LUSGS_steady.f90:32-204) including
+ * update_primitve_variables (update_flow_variables.f90:208-252). */
+int rc_lusgs_steady(rc_case *h);
+/* Replaces lu_sgs_stage (Time_LUSGS_steady.f90:4-30): the four calls above
+ * and the NaN check on WA(1,1). */
+int rc_lusgs_stage(rc_case *h);
+
+/* Replaces the steady branch of the main loop (fvm2d.f90:48-150 with
+ * IfSteady, .not.IfExplicit): nsteps calls of lu_sgs_stage, and every
+ * `every` steps (and at step 1) the step_residual norms
+ * (residual.f90:61-79): res_out[(n)*(nvar+1) + k] holds Res(k)/total_volume
+ * and, at k = nvar, max(rho), for the n-th reported step.  Runs
+ * asynchronously between reports; `res_out` may be NULL.  Returns the
+ * number of completed steps in *done. */
+int rc_run_steady(rc_case *h, int nsteps, int every, double *res_out, int *done);
+
+/* Replaces Residual + the normalisation of step_residual (residual.f90:4-21,
+ * 61-79): res[k] = sum_i vol_i |hWA(k,i)| / total_volume, res[nvar] = max rho. */
+int rc_residual(rc_case *h, double *res);
+
+/* Copies of the resident arrays, reference layout (see Conventions). */
+int rc_get_primitive(rc_case *h, double *pa);           /* PA(Nvar,NE) */
+int rc_get_conserved(rc_case *h, double *wa);           /* WA(Nvar,NE) */
+int rc_get_increment(rc_case *h, double *hwa);          /* hWA(Nvar,NE): residual after flux_sourceterm, dW after LU-SGS */
+int rc_get_gradient(rc_case *h, double *gradc);         /* GradC(2,Nvar,NE) */
+int rc_get_timestep(rc_case *h, double *step, double *alpaf); /* step(NE), alpaF(NF); either may be NULL */
+int rc_get_ghost(rc_case *h, double *gpa);              /* GPA(Nvar,Nghost) */
+
+/* Per-kernel device time summed over the steps run while timing was on,
+ * in ms: t[0] timestep faces, t[1] timestep cells, t[2] ghost update,
+ * t[3] gradient+limiter, t[4] flux faces, t[5] residual assembly + source,
+ * t[6] LU-SGS forward sweep (all wavefronts), t[7] backward sweep, t[8]
+ * update; t[9] the number of timed steps.  Collected with HIP events on the
+ * case's stream; enabling it synchronises after every step. */
+int rc_set_timing(rc_case *h, int enable);
+int rc_get_timing(rc_case *h, double *t10);
+
+/* No counterpart in the reference: a host-side utility that returns a
+ * cell permutation (perm[k] = old 0-based index of new cell k) ordering the
+ * cells colour by colour (greedy colouring of the face-adjacency graph in
+ * reverse Cuthill-McKee order), so that the LU-SGS sweeps of the reordered
+ * mesh run as ncolors wavefronts.  Writing the same order into grid1.in
+ * makes the reference's serial sweeps perform the identical computation. */
+int rc_mesh_order_multicolor(int ne, const int *cellnodes, int *perm, int *ncolors);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

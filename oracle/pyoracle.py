@@ -101,7 +101,7 @@ def make_params(case) -> RcoParams:
     p.ox_t = case.oxidizer.temp
     p.ox_g = case.inlet_mass_flux
     p.fu_t = case.fuel.temp
-    p.fu_g = 0.0
+    p.fu_g = case.fuel_mass_flux
     return p
 
 

@@ -1,0 +1,609 @@
+// C-ABI of the library (include/rocketcombustor.h): case set-up, resident
+// device state, the per-operator entry points and the steady step loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rc_device.h"
+
+namespace rc {
+void launch_ts_face(const DevMesh &, const DevState &, const DevThermo *, hipStream_t);
+void launch_ts_cell(const DevMesh &, const DevState &, const DevParams *, hipStream_t);
+void launch_ghost(const DevMesh &, const DevState &, const DevThermo *, const DevParams *, hipStream_t);
+void launch_gradlim(const DevMesh &, const DevState &, bool, bool, hipStream_t);
+void launch_flux(const DevMesh &, const DevState &, const DevThermo *, const DevParams *, hipStream_t);
+void launch_residual(const DevMesh &, const DevState &, const DevThermo *, const DevParams *, hipStream_t);
+void launch_lusgs_level(const DevMesh &, const DevState &, const DevThermo *, bool, int, int, hipStream_t);
+void launch_update(const DevMesh &, const DevState &, const DevThermo *, hipStream_t);
+void launch_prim_to_cons(const DevMesh &, const DevState &, const DevThermo *, hipStream_t);
+void launch_resnorm(const DevMesh &, const DevState &, double *, int, double, double *, hipStream_t);
+}  // namespace rc
+
+using namespace rc;
+
+static thread_local std::string g_error;
+static constexpr int kTimeSlots = 9;
+
+struct rc_case {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    HostMesh hm;
+    rc_mechanism mech{};
+    rc_params par{};
+    int ne = 0, nf = 0, ng = 0, nvar = kNvar, nlev = 0;
+    DevMesh M{};
+    DevState S{};
+    DevThermo *d_th = nullptr;
+    DevParams *d_par = nullptr;
+    double *d_partial = nullptr, *d_resout = nullptr;
+    int nblk_res = 0;
+    void *arena = nullptr;
+    size_t arena_bytes = 0;
+    int *h_err = nullptr;        // pinned
+    double *h_res = nullptr;     // pinned
+    int steps_done = 0;
+    bool timing = false;
+    // timing slots: 0 timestep faces, 1 timestep cells, 2 ghost update,
+    // 3 gradient+limiter, 4 flux faces, 5 residual+source, 6 LU forward,
+    // 7 LU backward, 8 update
+    // events are recorded without synchronising; each step uses kTimeSlots+1
+    // consecutive pool entries, drained into tms[] at the next sync point
+    std::vector<hipEvent_t> evpool;
+    size_t ev_used = 0;
+    double tms[kTimeSlots] = {};
+    long timed_steps = 0;
+    std::string err;
+};
+
+namespace {
+
+int set_err(rc_case *h, int code, const std::string &msg) {
+    if (h) h->err = msg;
+    g_error = msg;
+    return code;
+}
+
+#define HIPCHK(h, x)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            return set_err(h, RC_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+    } while (0)
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+int pad64(int n) { return (int)align_up((size_t)std::max(n, 1), 64); }
+
+const char *kernel_name(int k) {
+    switch (k) {
+    case K_TS_FACE: return "timestep (face)";
+    case K_TS_CELL: return "timestep (cell)";
+    case K_GHOST: return "ghost-cell update";
+    case K_GRADLIM: return "gradient/limiter";
+    case K_FLUX: return "flux_lax";
+    case K_RES: return "sourceterm";
+    case K_FWD: return "LU-SGS forward sweep";
+    case K_BWD: return "LU-SGS backward sweep";
+    case K_UPDATE: return "update_primitve_variables";
+    case K_P2C: return "primitve_to_conserved_variables";
+    default: return "?";
+    }
+}
+
+// reads the device error flag (stream must be synchronised)
+int check_device_error(rc_case *h) {
+    HIPCHK(h, hipMemcpyAsync(h->h_err, h->S.err, 4 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->h_err[0] == 0) return RC_OK;
+    const int code = h->h_err[0];
+    char buf[256];
+    const char *what = code == RC_ERR_TEMPERATURE ? "temperature outside the thermo polynomial range"
+                     : code == RC_ERR_NEGATIVE ? "negative density or pressure"
+                     : code == RC_ERR_NAN ? "NaN in WA(1,1)" : "error";
+    std::snprintf(buf, sizeof buf, "%s in %s (index %d, step %d)", what, kernel_name(h->h_err[2]), h->h_err[1] + 1,
+                  h->h_err[3]);
+    HIPCHK(h, hipMemsetAsync(h->S.err, 0, 4 * sizeof(int), h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return set_err(h, code, buf);
+}
+
+int drain_timing(rc_case *h);
+
+int sync_check(rc_case *h) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int rc = drain_timing(h);
+    if (rc) return rc;
+    return check_device_error(h);
+}
+
+DevThermo make_thermo(const rc_mechanism &m) {
+    DevThermo t{};
+    t.nsp = m.nsp;
+    t.model = m.model;
+    for (int k = 0; k < m.nsp; ++k) {
+        t.rsp[k] = m.rconstant / m.mole_weight[k];
+        t.mw[k] = m.mole_weight[k];
+        t.low_b1[k] = m.low_b1[k];
+        t.high_b1[k] = m.high_b1[k];
+        t.dnu[k] = m.coef_r[k] - m.coef_f[k];
+        for (int j = 0; j < 7; ++j) {
+            t.low[k][j] = m.low_coef[k][j];
+            t.high[k][j] = m.high_coef[k][j];
+        }
+    }
+    t.t_low = m.t_low;
+    t.t_crit = m.t_crit;
+    t.t_high = m.t_high;
+    t.a_f = m.a_f;
+    t.e_act = m.e_act;
+    t.rconstant = m.rconstant;
+    return t;
+}
+
+DevParams make_params(const rc_params &p, int nsp) {
+    DevParams d{};
+    d.cfl = p.cfl;
+    d.p_out = p.p_out;
+    for (int k = 0; k < nsp; ++k) {
+        d.ox_y[k] = p.ox_y[k];
+        d.fu_y[k] = p.fu_y[k];
+    }
+    d.ox_t = p.ox_t;
+    d.ox_g = p.ox_mass_flux;
+    d.fu_t = p.fu_t;
+    d.fu_g = p.fu_mass_flux;
+    d.limit_id = p.limit_id;
+    d.combustion = p.combustion;
+    return d;
+}
+
+// carves typed sub-arrays out of one device allocation
+struct Carver {
+    size_t off = 0;
+    char *base = nullptr;
+    template <class T>
+    T *take(size_t n) {
+        off = align_up(off, 256);
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += n * sizeof(T);
+        return p;
+    }
+};
+
+template <class T>
+std::vector<T> soa(const std::vector<T> &aos, int n, int w, int stride) {
+    // aos[i*w + c] -> soa[c*stride + i]
+    std::vector<T> out((size_t)w * stride, T(0));
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < w; ++c) out[(size_t)c * stride + i] = aos[(size_t)i * w + c];
+    return out;
+}
+
+int layout_and_upload(rc_case *h) {
+    const HostMesh &m = h->hm;
+    const int sc = pad64(m.ne), sf = pad64(m.nf), sg = pad64(m.ng);
+    h->nblk_res = std::min(1024, std::max(1, (m.ne + 255) / 256));
+    auto plan = [&](Carver &c, DevMesh &M, DevState &S) {
+        M.ne = m.ne; M.nf = m.nf; M.ng = m.ng; M.sc = sc; M.sf = sf; M.sg = sg;
+        S.pa = c.take<double>((size_t)kNvar * sc);
+        S.wa = c.take<double>((size_t)kNvar * sc);
+        S.hwa = c.take<double>((size_t)kNvar * sc);
+        S.gradc = c.take<double>((size_t)2 * kNvar * sc);
+        S.step = c.take<double>(sc);
+        S.dd = c.take<double>(sc);
+        S.alpaf = c.take<double>(sf);
+        S.fflux = c.take<double>((size_t)kNvar * sf);
+        S.gpa = c.take<double>((size_t)kNvar * sg);
+        S.err = c.take<int>(4);
+        M.vol = c.take<double>(sc);
+        M.wdis = c.take<double>((size_t)3 * sc);
+        M.amls = c.take<double>((size_t)6 * sc);
+        M.lvv = c.take<double>((size_t)6 * sc);
+        M.stencil = c.take<int>((size_t)3 * sc);
+        M.cface = c.take<int>((size_t)3 * sc);
+        M.sched = c.take<int>(sc);
+        M.vecx = c.take<double>((size_t)2 * sf);
+        M.fvl = c.take<double>((size_t)2 * sf);
+        M.fvr = c.take<double>((size_t)2 * sf);
+        M.fnl = c.take<int>(sf);
+        M.fnr = c.take<int>(sf);
+        M.gnrm = c.take<double>((size_t)2 * sg);
+        M.ghost_host = c.take<int>(sg);
+        M.ghost_type = c.take<int>(sg);
+        M.ghost_par = nullptr;
+        M.face_par = nullptr;
+        M.cell_par = nullptr;
+        h->d_th = c.take<DevThermo>(1);
+        h->d_par = c.take<DevParams>(1);
+        h->d_partial = c.take<double>((size_t)h->nblk_res * (kNvar + 1));
+        h->d_resout = c.take<double>(kNvar + 1);
+    };
+    Carver dry;
+    DevMesh M{};
+    DevState S{};
+    plan(dry, M, S);
+    h->arena_bytes = align_up(dry.off, 256);
+    HIPCHK(h, hipMalloc(&h->arena, h->arena_bytes));
+    HIPCHK(h, hipMemsetAsync(h->arena, 0, h->arena_bytes, h->stream));
+    Carver real;
+    real.base = static_cast<char *>(h->arena);
+    plan(real, M, S);
+    h->M = M;
+    h->S = S;
+    auto up = [&](const void *dst, const void *src, size_t bytes) -> int {
+        HIPCHK(h, hipMemcpyAsync(const_cast<void *>(dst), src, bytes, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        return RC_OK;
+    };
+    int rc;
+    std::vector<double> vol(m.vol);
+    vol.resize(sc, 0.0);
+    if ((rc = up(M.vol, vol.data(), sizeof(double) * sc))) return rc;
+    auto wd = soa(m.wdis, m.ne, 3, sc);
+    if ((rc = up(M.wdis, wd.data(), sizeof(double) * wd.size()))) return rc;
+    auto am = soa(m.amls, m.ne, 6, sc);
+    if ((rc = up(M.amls, am.data(), sizeof(double) * am.size()))) return rc;
+    auto lv = soa(m.lvv, m.ne, 6, sc);
+    if ((rc = up(M.lvv, lv.data(), sizeof(double) * lv.size()))) return rc;
+    auto stn = soa(m.stencil, m.ne, 3, sc);
+    if ((rc = up(M.stencil, stn.data(), sizeof(int) * stn.size()))) return rc;
+    auto cf = soa(m.cface, m.ne, 3, sc);
+    if ((rc = up(M.cface, cf.data(), sizeof(int) * cf.size()))) return rc;
+    if ((rc = up(M.sched, m.sched.data(), sizeof(int) * m.ne))) return rc;
+    auto vx = soa(m.vecx, m.nf, 2, sf);
+    if ((rc = up(M.vecx, vx.data(), sizeof(double) * vx.size()))) return rc;
+    auto fl = soa(m.fvl, m.nf, 2, sf);
+    if ((rc = up(M.fvl, fl.data(), sizeof(double) * fl.size()))) return rc;
+    auto fr = soa(m.fvr, m.nf, 2, sf);
+    if ((rc = up(M.fvr, fr.data(), sizeof(double) * fr.size()))) return rc;
+    if ((rc = up(M.fnl, m.fnl.data(), sizeof(int) * m.nf))) return rc;
+    if ((rc = up(M.fnr, m.fnr.data(), sizeof(int) * m.nf))) return rc;
+    if (m.ng) {
+        auto gn = soa(m.gnrm, m.ng, 2, sg);
+        if ((rc = up(M.gnrm, gn.data(), sizeof(double) * gn.size()))) return rc;
+        if ((rc = up(M.ghost_host, m.ghost_host.data(), sizeof(int) * m.ng))) return rc;
+        if ((rc = up(M.ghost_type, m.ghost_type.data(), sizeof(int) * m.ng))) return rc;
+    }
+    DevThermo th = make_thermo(h->mech);
+    DevParams dp = make_params(h->par, h->mech.nsp);
+    if ((rc = up(h->d_th, &th, sizeof th))) return rc;
+    if ((rc = up(h->d_par, &dp, sizeof dp))) return rc;
+    return RC_OK;
+}
+
+// the stages of one step; with timing on, events bracket every kernel
+int ensure_events(rc_case *h, size_t n) {
+    while (h->evpool.size() < n) {
+        hipEvent_t e;
+        HIPCHK(h, hipEventCreate(&e));
+        h->evpool.push_back(e);
+    }
+    return RC_OK;
+}
+void mark(rc_case *h, int k) {
+    if (!h->timing) return;
+    if (ensure_events(h, h->ev_used + kTimeSlots + 1)) return;
+    (void)hipEventRecord(h->evpool[h->ev_used + k], h->stream);
+}
+// accumulates the recorded steps' kernel times (after the stream has drained)
+int drain_timing(rc_case *h) {
+    if (h->ev_used == 0) return RC_OK;
+    HIPCHK(h, hipEventSynchronize(h->evpool[h->ev_used - 1]));
+    for (size_t b = 0; b + kTimeSlots < h->ev_used; b += kTimeSlots + 1) {
+        for (int k = 0; k < kTimeSlots; ++k) {
+            float ms = 0.f;
+            HIPCHK(h, hipEventElapsedTime(&ms, h->evpool[b + k], h->evpool[b + k + 1]));
+            h->tms[k] += ms;
+        }
+        h->timed_steps++;
+    }
+    h->ev_used = 0;
+    return RC_OK;
+}
+void enqueue_timestep(rc_case *h) {
+    mark(h, 0);
+    launch_ts_face(h->M, h->S, h->d_th, h->stream);
+    mark(h, 1);
+    launch_ts_cell(h->M, h->S, h->d_par, h->stream);
+}
+void enqueue_reconstruction(rc_case *h, bool grad, bool limit) {
+    mark(h, 2);
+    if (grad) launch_ghost(h->M, h->S, h->d_th, h->d_par, h->stream);
+    mark(h, 3);
+    launch_gradlim(h->M, h->S, grad, limit, h->stream);
+}
+void enqueue_flux(rc_case *h) {
+    mark(h, 4);
+    launch_flux(h->M, h->S, h->d_th, h->d_par, h->stream);
+    mark(h, 5);
+    launch_residual(h->M, h->S, h->d_th, h->d_par, h->stream);
+}
+void enqueue_fwd(rc_case *h) {
+    mark(h, 6);
+    for (int l = 0; l < h->nlev; ++l)
+        launch_lusgs_level(h->M, h->S, h->d_th, true, h->hm.lvl_ptr[l], h->hm.lvl_ptr[l + 1], h->stream);
+}
+void enqueue_bwd(rc_case *h) {
+    mark(h, 7);
+    for (int l = h->nlev - 1; l >= 0; --l)
+        launch_lusgs_level(h->M, h->S, h->d_th, false, h->hm.lvl_ptr[l], h->hm.lvl_ptr[l + 1], h->stream);
+}
+void enqueue_update(rc_case *h) {
+    mark(h, 8);
+    launch_update(h->M, h->S, h->d_th, h->stream);
+}
+
+int enqueue_stage(rc_case *h) {
+    h->S.step_no = h->steps_done + 1;
+    if (h->timing && h->ev_used + kTimeSlots + 1 > 4096) {
+        int rc = drain_timing(h);
+        if (rc) return rc;
+    }
+    enqueue_timestep(h);
+    enqueue_reconstruction(h, true, h->par.limit_id > 0);
+    enqueue_flux(h);
+    enqueue_fwd(h);
+    enqueue_bwd(h);
+    enqueue_update(h);
+    if (h->timing) {
+        int rc = ensure_events(h, h->ev_used + kTimeSlots + 1);
+        if (rc) return rc;
+        HIPCHK(h, hipEventRecord(h->evpool[h->ev_used + kTimeSlots], h->stream));
+        h->ev_used += kTimeSlots + 1;
+    }
+    h->steps_done++;
+    return RC_OK;
+}
+
+int residual_async(rc_case *h, double *host_out) {
+    launch_resnorm(h->M, h->S, h->d_partial, h->nblk_res, 1.0 / h->hm.total_volume, h->d_resout, h->stream);
+    HIPCHK(h, hipMemcpyAsync(host_out, h->d_resout, sizeof(double) * (kNvar + 1), hipMemcpyDeviceToHost, h->stream));
+    return RC_OK;
+}
+
+template <class F>
+int get_cells(rc_case *h, double *out, const double *dev, int width, int stride, F) {
+    if (!out) return RC_OK;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::vector<double> tmp((size_t)width * stride);
+    HIPCHK(h, hipMemcpy(tmp.data(), dev, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+    const int n = stride == h->M.sc ? h->ne : (stride == h->M.sf ? h->nf : h->ng);
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < width; ++c) out[(size_t)i * width + c] = tmp[(size_t)c * stride + i];
+    return RC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rc_last_error(const rc_case *h) { return h ? h->err.c_str() : g_error.c_str(); }
+
+int rc_case_create(const rc_mechanism *mech, const rc_params *par, int nnt, const double *coor, int ne,
+                   const int *cellnodes, int nb, const int *bnodes, const int *btype, int device, rc_case **out) {
+    if (!out) return set_err(nullptr, RC_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (!mech || !par || !coor || !cellnodes || (nb > 0 && (!bnodes || !btype)))
+        return set_err(nullptr, RC_ERR_ARGUMENT, "NULL argument");
+    if (mech->nsp != kNsp)
+        return set_err(nullptr, RC_ERR_ARGUMENT, "the kernels are built for 4 species (Nvar = 7)");
+    if (mech->model != 0 && mech->model != 1)
+        return set_err(nullptr, RC_ERR_ARGUMENT, "No such code_chemical_model");
+    rc_case *h = new rc_case();
+    h->mech = *mech;
+    h->par = *par;
+    h->device = device;
+    std::vector<int> c0(3 * (size_t)ne), b0(2 * (size_t)std::max(nb, 0));
+    for (size_t k = 0; k < c0.size(); ++k) c0[k] = cellnodes[k] - 1;
+    for (size_t k = 0; k < b0.size(); ++k) b0[k] = bnodes[k] - 1;
+    std::string msg;
+    int rc = build_mesh(nnt, coor, ne, c0.data(), nb, b0.data(), btype, h->hm, msg);
+    if (rc) {
+        delete h;
+        return set_err(nullptr, rc, msg);
+    }
+    h->ne = h->hm.ne;
+    h->nf = h->hm.nf;
+    h->ng = h->hm.ng;
+    h->nlev = (int)h->hm.lvl_ptr.size() - 1;
+    auto bail = [&](int code) {
+        g_error = h->err;
+        rc_case_destroy(h);
+        return code;
+    };
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return bail(set_err(h, RC_ERR_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e)));
+    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(set_err(h, RC_ERR_DEVICE, std::string("stream: ") + hipGetErrorString(e)));
+    if (hipHostMalloc(&h->h_err, 4 * sizeof(int)) != hipSuccess ||
+        hipHostMalloc(&h->h_res, (kNvar + 1) * sizeof(double)) != hipSuccess)
+        return bail(set_err(h, RC_ERR_MEMORY, "pinned host allocation failed"));
+    rc = layout_and_upload(h);
+    if (rc) return bail(rc);
+    *out = h;
+    return RC_OK;
+}
+
+void rc_case_destroy(rc_case *h) {
+    if (!h) return;
+    if (h->stream) {
+        (void)hipSetDevice(h->device);
+        (void)hipStreamSynchronize(h->stream);
+    }
+    if (h->arena) (void)hipFree(h->arena);
+    for (auto &ev : h->evpool) (void)hipEventDestroy(ev);
+    if (h->h_err) (void)hipHostFree(h->h_err);
+    if (h->h_res) (void)hipHostFree(h->h_res);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int rc_case_info(const rc_case *h, long long *s) {
+    if (!h || !s) return set_err(nullptr, RC_ERR_ARGUMENT, "NULL argument");
+    s[0] = h->ne; s[1] = h->nf; s[2] = h->ng; s[3] = h->nvar; s[4] = h->nlev; s[5] = (long long)h->arena_bytes;
+    return RC_OK;
+}
+
+int rc_set_primitive(rc_case *h, const double *pa) {
+    if (!h || !pa) return set_err(h, RC_ERR_ARGUMENT, "NULL argument");
+    HIPCHK(h, hipSetDevice(h->device));
+    const int sc = h->M.sc;
+    std::vector<double> tmp((size_t)kNvar * sc, 0.0);
+    for (int i = 0; i < h->ne; ++i)
+        for (int k = 0; k < kNvar; ++k) tmp[(size_t)k * sc + i] = pa[(size_t)i * kNvar + k];
+    HIPCHK(h, hipMemcpyAsync(h->S.pa, tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice, h->stream));
+    launch_prim_to_cons(h->M, h->S, h->d_th, h->stream);
+    return sync_check(h);
+}
+
+int rc_timestep(rc_case *h) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    HIPCHK(h, hipSetDevice(h->device));
+    enqueue_timestep(h);
+    return sync_check(h);
+}
+
+int rc_gradient(rc_case *h) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    HIPCHK(h, hipSetDevice(h->device));
+    enqueue_reconstruction(h, true, false);
+    return sync_check(h);
+}
+
+int rc_limiter_barth(rc_case *h) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    HIPCHK(h, hipSetDevice(h->device));
+    enqueue_reconstruction(h, false, true);
+    return sync_check(h);
+}
+
+int rc_reconstruction(rc_case *h) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    HIPCHK(h, hipSetDevice(h->device));
+    enqueue_reconstruction(h, true, h->par.limit_id > 0);
+    return sync_check(h);
+}
+
+int rc_flux_sourceterm(rc_case *h) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    HIPCHK(h, hipSetDevice(h->device));
+    enqueue_flux(h);
+    return sync_check(h);
+}
+
+int rc_lusgs_steady(rc_case *h) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    HIPCHK(h, hipSetDevice(h->device));
+    enqueue_fwd(h);
+    enqueue_bwd(h);
+    enqueue_update(h);
+    return sync_check(h);
+}
+
+int rc_lusgs_stage(rc_case *h) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc = enqueue_stage(h);
+    if (rc) return rc;
+    return sync_check(h);
+}
+
+int rc_run_steady(rc_case *h, int nsteps, int every, double *res_out, int *done) {
+    if (!h || nsteps < 0) return set_err(h, RC_ERR_ARGUMENT, "bad argument");
+    HIPCHK(h, hipSetDevice(h->device));
+    if (every <= 0) every = nsteps + 1;
+    int nrep = 0;
+    for (int s = 1; s <= nsteps; ++s) {
+        int rc = enqueue_stage(h);
+        if (rc) return rc;
+        if (s == 1 || s % every == 0) {
+            if (res_out) {
+                rc = residual_async(h, h->h_res);
+                if (rc) return rc;
+            }
+            rc = sync_check(h);
+            if (rc) {
+                if (done) *done = s;
+                return rc;
+            }
+            if (res_out) std::memcpy(res_out + (size_t)nrep * (kNvar + 1), h->h_res, sizeof(double) * (kNvar + 1));
+            ++nrep;
+        }
+    }
+    int rc = sync_check(h);
+    if (done) *done = nsteps;
+    return rc;
+}
+
+int rc_residual(rc_case *h, double *res) {
+    if (!h || !res) return set_err(h, RC_ERR_ARGUMENT, "NULL argument");
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc = residual_async(h, h->h_res);
+    if (rc) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::memcpy(res, h->h_res, sizeof(double) * (kNvar + 1));
+    return RC_OK;
+}
+
+int rc_get_primitive(rc_case *h, double *pa) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    return get_cells(h, pa, h->S.pa, kNvar, h->M.sc, 0);
+}
+int rc_get_conserved(rc_case *h, double *wa) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    return get_cells(h, wa, h->S.wa, kNvar, h->M.sc, 0);
+}
+int rc_get_increment(rc_case *h, double *hwa) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    return get_cells(h, hwa, h->S.hwa, kNvar, h->M.sc, 0);
+}
+int rc_get_gradient(rc_case *h, double *g) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    return get_cells(h, g, h->S.gradc, 2 * kNvar, h->M.sc, 0);
+}
+int rc_get_timestep(rc_case *h, double *step, double *alpaf) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    int rc = get_cells(h, step, h->S.step, 1, h->M.sc, 0);
+    if (rc) return rc;
+    return get_cells(h, alpaf, h->S.alpaf, 1, h->M.sf, 0);
+}
+int rc_get_ghost(rc_case *h, double *gpa) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    if (h->ng == 0) return RC_OK;
+    return get_cells(h, gpa, h->S.gpa, kNvar, h->M.sg, 0);
+}
+
+int rc_set_timing(rc_case *h, int enable) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->timing = enable != 0;
+    h->ev_used = 0;
+    for (double &t : h->tms) t = 0.0;
+    h->timed_steps = 0;
+    return RC_OK;
+}
+
+int rc_get_timing(rc_case *h, double *t10) {
+    if (!h || !t10) return set_err(h, RC_ERR_ARGUMENT, "NULL argument");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int rc = drain_timing(h);
+    if (rc) return rc;
+    for (int k = 0; k < kTimeSlots; ++k) t10[k] = h->tms[k];
+    t10[kTimeSlots] = (double)h->timed_steps;
+    return RC_OK;
+}
+
+int rc_mesh_order_multicolor(int ne, const int *cellnodes, int *perm, int *ncolors) {
+    if (ne <= 0 || !cellnodes || !perm) return set_err(nullptr, RC_ERR_ARGUMENT, "bad argument");
+    std::vector<int> c0(3 * (size_t)ne);
+    for (size_t k = 0; k < c0.size(); ++k) c0[k] = cellnodes[k] - 1;
+    int rc = order_multicolor(ne, c0.data(), perm, ncolors);
+    if (rc) return set_err(nullptr, rc, "mesh is not a manifold triangle mesh");
+    return RC_OK;
+}
+
+}  // extern "C"

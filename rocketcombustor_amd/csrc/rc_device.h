@@ -1,0 +1,171 @@
+// Device-side views and FP64 leaf functions shared by the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rc_internal.h"
+
+namespace rc {
+
+// Structure-of-arrays views of one case (or of a packed batch of cases).
+// Cell arrays have stride sc, face arrays sf, ghost arrays sg:
+//   pa[k*sc+i], gradc[(k*2+r)*sc+i], stencil[s*sc+i], amls[(s*2+r)*sc+i],
+//   lvv[(kf*2+d)*sc+i], cface[kf*sc+i], vecx[d*sf+f], fflux[k*sf+f].
+struct DevMesh {
+    int ne, nf, ng;
+    int sc, sf, sg;
+    const int *__restrict__ fnl;
+    const int *__restrict__ fnr;
+    const int *__restrict__ stencil;
+    const int *__restrict__ cface;
+    const int *__restrict__ ghost_host;
+    const int *__restrict__ ghost_type;
+    const int *__restrict__ ghost_par;   // parameter-set index per ghost (batches)
+    const int *__restrict__ face_par;    // parameter-set index per face (batches)
+    const int *__restrict__ cell_par;    // parameter-set index per cell (batches)
+    const int *__restrict__ sched;
+    const double *__restrict__ vecx;
+    const double *__restrict__ fvl;
+    const double *__restrict__ fvr;
+    const double *__restrict__ vol;
+    const double *__restrict__ wdis;
+    const double *__restrict__ amls;
+    const double *__restrict__ lvv;
+    const double *__restrict__ gnrm;
+};
+
+struct DevState {
+    double *pa, *wa, *hwa, *gradc, *gpa, *alpaf, *step, *dd, *fflux;
+    int *err;        // [0] code, [1] index, [2] kernel, [3] step
+    int step_no;
+};
+
+enum KernelId { K_TS_FACE = 1, K_TS_CELL, K_GHOST, K_GRADLIM, K_FLUX, K_RES, K_FWD, K_BWD, K_UPDATE, K_P2C };
+
+__device__ inline void flag_error(int *err, int code, int index, int kernel, int step) {
+    if (atomicCAS(err, 0, code) == 0) {
+        err[1] = index;
+        err[2] = kernel;
+        err[3] = step;
+    }
+}
+
+// real**integer as compiler-rt's __powidf2 evaluates it (the reference's
+// `tem**(j-3)`); with a constant exponent the loop unrolls to the same
+// multiply chain.
+__device__ __forceinline__ double powi_d(double a, int b) {
+    const bool recip = b < 0;
+    double r = 1.0;
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+// ComputeGasParameter, thermo_property.f90:78-112.  Returns false when T is
+// outside [t_low, t_high] (the reference stops there); the high-range
+// coefficients are then used so that the arithmetic stays finite.
+template <int NSP>
+__device__ __forceinline__ bool gas_param(const DevThermo &th, const double *y, double t,
+                                          double &cp, double &gam, double &r) {
+    double basis[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) basis[j] = powi_d(t, j - 2);
+    bool ok = true;
+    const bool low = (th.t_low <= t && t <= th.t_crit);
+    if (!low && !(th.t_crit < t && t <= th.t_high)) ok = false;
+    const double(*coef)[7] = low ? th.low : th.high;
+    double c = 0.0, rr = 0.0;
+    double cps[NSP];
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) s = s + coef[i][j] * basis[j];
+        cps[i] = s * th.rsp[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) c = c + cps[i] * y[i];
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) rr = rr + th.rsp[i] * y[i];
+    cp = c;
+    gam = c / (c - rr);
+    r = rr;
+    return ok;
+}
+
+// compute_enthalpy, thermo_property.f90:114-144
+template <int NSP>
+__device__ __forceinline__ bool enthalpy(const DevThermo &th, double t, double *h) {
+    double basis[7];
+    basis[0] = -powi_d(t, -2);
+    basis[1] = log(t) / t;
+#pragma unroll
+    for (int j = 2; j < 7; ++j) basis[j] = powi_d(t, j - 2) / (double)(j - 1);
+    bool ok = true;
+    const bool low = (th.t_low <= t && t <= th.t_crit);
+    if (!low && !(th.t_crit < t && t <= th.t_high)) ok = false;
+    const double(*coef)[7] = low ? th.low : th.high;
+    const double *b1 = low ? th.low_b1 : th.high_b1;
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) s = s + coef[i][j] * basis[j];
+        h[i] = th.rsp[i] * (t * s + b1[i]);
+    }
+    return ok;
+}
+
+// Newton_temperature, thermo_property.f90:146-180
+template <int NSP>
+__device__ __forceinline__ bool newton_temperature(const DevThermo &th, double et, const double *y, double t0,
+                                                   double &t, double &cp, double &gam, double &r) {
+    t = t0;
+    bool ok = true;
+    double h[NSP];
+    for (int it = 0; it < 200; ++it) {
+        ok = enthalpy<NSP>(th, t, h) && ok;
+        ok = gas_param<NSP>(th, y, t, cp, gam, r) && ok;
+        if (!ok) break;
+        double hh = 0.0;
+#pragma unroll
+        for (int i = 0; i < NSP; ++i) hh = hh + y[i] * h[i];
+        const double ft = hh - r * t - et;
+        const double dft = cp - r;
+        const double dtt = -ft / dft;
+        t = t + dtt;
+        t = fmax(t, th.t_low);
+        t = fmin(t, th.t_high);
+        if (fabs(dtt) < 1.0e-5) break;
+    }
+    return ok;
+}
+
+// compute_production_rate, chemical_kinetics.f90:291-347 (one reaction;
+// model 0: A exp(-E/RT) [H2]^0.2 [O2]^1.3, model 1: 1e4 [H2] above 710 K)
+template <int NSP>
+__device__ __forceinline__ void production_rate(const DevThermo &th, double t, double rho, const double *y2, double *w) {
+    double y[NSP];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) y[k] = fmin(1.0, fmax(0.0, y2[k]));
+    double kf;
+    if (th.model == 0) {
+        kf = th.a_f * exp(-th.e_act / (th.rconstant * t));
+        kf = kf * pow(y[0] * rho / (th.mw[0] * 1000.0), 0.2);
+        kf = kf * pow(y[1] * rho / (th.mw[1] * 1000.0), 1.3);
+    } else {
+        kf = 0.0;
+        if (t > 710.0) kf = 1e4;
+        kf = kf * (y[0] * rho / (th.mw[0] * 1000.0));   // x**1.0 == x
+        kf = kf * 1.0;                                   // x**0.0 == 1
+    }
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) w[k] = 1000.0 * th.mw[k] * th.dnu[k] * kf;
+}
+
+}  // namespace rc

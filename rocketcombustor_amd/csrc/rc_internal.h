@@ -1,0 +1,79 @@
+// Internal declarations shared by the host set-up (rc_mesh.cpp), the HIP
+// kernels (rc_kernels.hip) and the C-ABI (rc_api.hip).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rocketcombustor.h"
+
+namespace rc {
+
+constexpr int kNFlow = 4;          // src/module.f90:8
+constexpr int kNsp = 4;            // species count the kernels are built for
+constexpr int kNvar = kNFlow + kNsp - 1;
+constexpr double kSmall = 1.0e-16; // src/module.f90:19
+
+// Host-side mesh set-up: topology and geometry in the reference's numbering
+// (0-based here), plus the derived per-face / per-cell operands the kernels read.
+struct HostMesh {
+    int ne = 0, nf = 0, ng = 0, nnt = 0;
+    std::vector<double> coor;        // 2*nnt  (x,y)
+    std::vector<int> cn;             // 3*ne   0-based nodes, CCW
+    std::vector<int> fnode;          // 2*nf   face nodes (n1,n2) as created
+    std::vector<int> fnl, fnr;       // nf     owner; neighbour (>=0) or -BCtype
+    std::vector<double> vecx;        // 2*nf   (y2-y1, -(x2-x1))
+    std::vector<double> vol, cellxy, reflen;   // ne, 2*ne, 2*ne
+    std::vector<int> neighf;         // 3*ne   faces of each cell, ascending
+    std::vector<int> neigh;          // ne     number of interior neighbours
+    std::vector<int> neighc;         // 3*ne   interior neighbours (first neigh[i])
+    std::vector<int> num_ghost;      // ne
+    std::vector<int> ghostcell;      // 3*ne   ghost ids of a cell
+    std::vector<int> ghost_host, ghost_face, ghost_type;  // ng
+    std::vector<double> ghost_nodes; // 8*ng   (x,y) of the 4 mirrored nodes
+    std::vector<double> ghost_vol, ghost_xy;  // ng, 2*ng
+    std::vector<double> xyh, wdis, amls;      // 2*ne, 3*ne, 6*ne ([i][slot][r])
+    double total_volume = 0.0;
+    // derived operands
+    std::vector<int> stencil;        // 3*ne  slot -> cell (>=0) or -(ghost+1)
+    std::vector<double> lvv;         // 6*ne  limiter face-midpoint offsets [i][kf][d]
+    std::vector<double> fvl, fvr;    // 2*nf  flux-point offsets of owner / neighbour
+    std::vector<double> gnrm;        // 2*ng  unit normal used by the ghost update
+    std::vector<int> cface;          // 3*ne  face f if cell is the owner, ~f otherwise
+    std::vector<int> level;          // ne    LU-SGS wavefront level
+    std::vector<int> sched;          // ne    cells sorted by level
+    std::vector<int> lvl_ptr;        // nlev+1
+};
+
+// Builds HostMesh from 0-based inputs; returns RC_OK or RC_ERR_MESH with msg.
+int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, const int *bedges0,
+               const int *btype, HostMesh &m, std::string &msg);
+
+// Multicolour ordering (no reference counterpart).
+int order_multicolor(int ne, const int *cells0, int *perm, int *ncolors);
+
+// Thermo/kinetics tables on the device.
+struct DevThermo {
+    int nsp;
+    int model;
+    double rsp[RC_MAXSP];          // R_species
+    double mw[RC_MAXSP];
+    double t_low, t_crit, t_high;
+    double low[RC_MAXSP][7], high[RC_MAXSP][7];
+    double low_b1[RC_MAXSP], high_b1[RC_MAXSP];
+    double dnu[RC_MAXSP];          // coef_r - coef_f of reaction 1
+    double a_f, e_act, rconstant;
+};
+
+// Run parameters on the device.
+struct DevParams {
+    double cfl;
+    double p_out;
+    double ox_y[RC_MAXSP], ox_t, ox_g;
+    double fu_y[RC_MAXSP], fu_t, fu_g;
+    int limit_id;
+    int combustion;
+};
+
+}  // namespace rc

@@ -1,0 +1,561 @@
+// FP64 HIP kernels for gfx950 of the steady residual + LU-SGS step.
+//
+// Every kernel is a gather: a thread owns one face or one cell and writes
+// only its own outputs, so no atomics are needed and results are
+// deterministic.  Per-cell sums over faces run over the cell's faces in
+// ascending face number (NeighF order), which is the order in which the
+// reference's face loops accumulate into the same cell; the LU-SGS sweeps
+// run level by level over wavefronts of the given cell order.
+#include "rc_device.h"
+
+namespace rc {
+
+#define PA_(k, i) S.pa[(k) * M.sc + (i)]
+#define WA_(k, i) S.wa[(k) * M.sc + (i)]
+#define HWA_(k, i) S.hwa[(k) * M.sc + (i)]
+#define GR_(k, r, i) S.gradc[((k) * 2 + (r)) * M.sc + (i)]
+#define GPA_(k, j) S.gpa[(k) * M.sg + (j)]
+
+constexpr int NV = kNvar;
+
+// TimeStep face loop (TimeStep.f90:16-92, inviscid): spectral radius per face
+template <int NSP>
+__global__ void __launch_bounds__(256) k_timestep_face(DevMesh M, DevState S, const DevThermo *__restrict__ thp) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= M.nf) return;
+    const DevThermo &th = *thp;
+    const int nl = M.fnl[f], nr = M.fnr[f];
+    const double dx = M.vecx[f], dy = M.vecx[M.sf + f];
+    double pp, uu, vv, tt, y[NSP], s = 0.0;
+    if (nr >= 0) {
+        pp = 0.5 * (PA_(0, nl) + PA_(0, nr));
+        uu = 0.5 * (PA_(1, nl) + PA_(1, nr));
+        vv = 0.5 * (PA_(2, nl) + PA_(2, nr));
+        tt = 0.5 * (PA_(3, nl) + PA_(3, nr));
+#pragma unroll
+        for (int k = 0; k < NSP - 1; ++k) { y[k] = 0.5 * (PA_(kNFlow + k, nl) + PA_(kNFlow + k, nr)); s = s + y[k]; }
+    } else {
+        pp = PA_(0, nl); uu = PA_(1, nl); vv = PA_(2, nl); tt = PA_(3, nl);
+#pragma unroll
+        for (int k = 0; k < NSP - 1; ++k) { y[k] = PA_(kNFlow + k, nl); s = s + y[k]; }
+    }
+    y[NSP - 1] = 1.0 - s;
+    double cp, gam, r;
+    if (!gas_param<NSP>(th, y, tt, cp, gam, r)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_TS_FACE, S.step_no);
+    const double rr = pp / (r * tt);
+    if (rr < 0. || pp < 0. || tt < 0.) flag_error(S.err, RC_ERR_NEGATIVE, f, K_TS_FACE, S.step_no);
+    const double un = uu * dx + vv * dy;
+    S.alpaf[f] = fabs(un) + sqrt(gam * pp / rr) * sqrt(dx * dx + dy * dy);
+}
+
+// TimeStep cell loop (TimeStep.f90:94-96) and the LU-SGS diagonal
+// dd(i) = sum 0.5*alpaF + vol/step (Time_LUSGS_steady.f90:46-53)
+__global__ void __launch_bounds__(256) k_timestep_cell(DevMesh M, DevState S, const DevParams *__restrict__ pars) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M.ne) return;
+    const double cfl = pars[M.cell_par ? M.cell_par[i] : 0].cfl;
+    double a = 0.0, d = 0.0, al[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int cf = M.cface[k * M.sc + i];
+        al[k] = S.alpaf[cf >= 0 ? cf : ~cf];
+        a = a + al[k];
+    }
+    const double v = M.vol[i];
+    const double st = cfl * v / a;
+    S.step[i] = st;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d = d + 0.5 * al[k];
+    S.dd[i] = d + v / st;
+}
+
+// update_ghostcell_variables (update_flow_variables.f90:294-473)
+template <int NSP>
+__global__ void __launch_bounds__(256) k_ghost(DevMesh M, DevState S, const DevThermo *__restrict__ thp,
+                                               const DevParams *__restrict__ pars) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= M.ng) return;
+    const DevThermo &th = *thp;
+    const DevParams &P = pars[M.ghost_par ? M.ghost_par[g] : 0];
+    const int i = M.ghost_host[g];
+    const double pl = PA_(0, i), ul = PA_(1, i), vl = PA_(2, i), tl = PA_(3, i);
+    double yl[NSP], yr[NSP], s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) { yl[k] = PA_(kNFlow + k, i); s = s + yl[k]; }
+    yl[NSP - 1] = 1.0 - s;
+    double cp, gam, r;
+    if (!gas_param<NSP>(th, yl, tl, cp, gam, r)) flag_error(S.err, RC_ERR_TEMPERATURE, g, K_GHOST, S.step_no);
+    const double rl = pl / (r * tl);
+    const double s1n = M.gnrm[g], s2n = M.gnrm[M.sg + g];
+    const double ac = sqrt(gam * pl / rl);
+    double pr, ur, vr, tr;
+    const int type = M.ghost_type[g];
+    if (type == 1) {
+        pr = pl; tr = tl; ur = -ul; vr = -vl;
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) yr[k] = yl[k];
+    } else if (type == 2 || type == 6) {
+        pr = pl; tr = tl;
+        const double vn = ul * s1n + vl * s2n;
+        ur = ul - 2 * vn * s1n;
+        vr = vl - 2 * vn * s2n;
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) yr[k] = yl[k];
+    } else if (type == 3 || type == 4) {
+        // oxidizer_inlet_bc / fuel_inlet_bc (oxidizer.f90:130-145, fuel.f90:200-215)
+        const double *yin = type == 3 ? P.ox_y : P.fu_y;
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) yr[k] = yin[k];
+        tr = type == 3 ? P.ox_t : P.fu_t;
+        double c2, g2, r2;
+        if (!gas_param<NSP>(th, yr, tr, c2, g2, r2)) flag_error(S.err, RC_ERR_TEMPERATURE, g, K_GHOST, S.step_no);
+        pr = pl;
+        const double rr = pr / (r2 * tr);
+        ur = (type == 3 ? P.ox_g : P.fu_g) / rr;
+        vr = 0.0;
+    } else {  // 5: outlet, supersonic copy or fixed p_out
+        const double vn = ul * s1n + vl * s2n;
+        pr = (fabs(vn) > ac) ? pl : P.p_out;
+        ur = ul; vr = vl; tr = tl;
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) yr[k] = yl[k];
+    }
+    GPA_(0, g) = pr; GPA_(1, g) = ur; GPA_(2, g) = vr; GPA_(3, g) = tr;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) GPA_(kNFlow + k, g) = yr[k];
+}
+
+// gradient (gradient.f90:4-45) and/or Limiter_Barth (limiter.f90:5-107),
+// one thread per cell.  GRAD: compute GradC = AMLS * (wdis * dPA) over the
+// 3 stencil slots (interior neighbours, then ghosts); LIMIT: scale each
+// variable's gradient by the Barth factor over the 3 face midpoints.
+template <bool GRAD, bool LIMIT>
+__global__ void __launch_bounds__(256) k_gradlim(DevMesh M, DevState S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M.ne) return;
+    double pi[NV], pmin[NV], pmax[NV], g0[NV], g1[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) { pi[k] = PA_(k, i); pmin[k] = pi[k]; pmax[k] = pi[k]; g0[k] = 0.0; g1[k] = 0.0; }
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int j = M.stencil[s * M.sc + i];
+        const double w = M.wdis[s * M.sc + i];
+        const double a0 = M.amls[(s * 2 + 0) * M.sc + i];
+        const double a1 = M.amls[(s * 2 + 1) * M.sc + i];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const double pj = (j >= 0) ? PA_(k, j) : GPA_(k, -j - 1);
+            if (GRAD) {
+                const double cc = w * (pj - pi[k]);
+                g0[k] = g0[k] + a0 * cc;
+                g1[k] = g1[k] + a1 * cc;
+            }
+            if (LIMIT) { pmin[k] = fmin(pmin[k], pj); pmax[k] = fmax(pmax[k], pj); }
+        }
+    }
+    if (!GRAD) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) { g0[k] = GR_(k, 0, i); g1[k] = GR_(k, 1, i); }
+    }
+    if (LIMIT) {
+        double v0[3], v1[3];
+#pragma unroll
+        for (int kf = 0; kf < 3; ++kf) { v0[kf] = M.lvv[(kf * 2) * M.sc + i]; v1[kf] = M.lvv[(kf * 2 + 1) * M.sc + i]; }
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double phi = 1.0;
+#pragma unroll
+            for (int kf = 0; kf < 3; ++kf) {
+                const double fv = pi[k] + (g0[k] * v0[kf] + g1[k] * v1[kf]);
+                double pf;
+                if (fv > pi[k]) pf = fmin(1.0, (pmax[k] - pi[k]) / (fv - pi[k]));
+                else if (fv < pi[k]) pf = fmin(1.0, (pmin[k] - pi[k]) / (fv - pi[k]));
+                else pf = 1.0;
+                phi = fmin(phi, pf);
+            }
+            g0[k] = phi * g0[k];
+            g1[k] = phi * g1[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) { GR_(k, 0, i) = g0[k]; GR_(k, 1, i) = g1[k]; }
+}
+
+// flux_lax (flux_lax.f90:17-296), one thread per face: reconstructed L/R
+// states at the face point, boundary branch, Rusanov flux -> fflux(7,f)
+template <int NSP>
+__global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, const DevThermo *__restrict__ thp,
+                                              const DevParams *__restrict__ pars) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= M.nf) return;
+    const DevThermo &th = *thp;
+    const int nl = M.fnl[f], nr = M.fnr[f];
+    const double dx = M.vecx[f], dy = M.vecx[M.sf + f];
+    const double sav1 = dx, sav2 = dy;
+    const double sav = fmax(sqrt(dx * dx + dy * dy), kSmall);
+    const double s1n = sav1 / sav, s2n = sav2 / sav;
+    const double v0 = M.fvl[f], v1 = M.fvl[M.sf + f];
+    double ql[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) ql[k] = PA_(k, nl) + (GR_(k, 0, nl) * v0 + GR_(k, 1, nl) * v1);
+    double pl = ql[0], ul = ql[1], vl = ql[2], tl = ql[3];
+    double yl[NSP], yr[NSP], s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) { yl[k] = ql[kNFlow + k]; s = s + yl[k]; }
+    yl[NSP - 1] = 1.0 - s;
+    double cpl, gaml, rcl;
+    if (!gas_param<NSP>(th, yl, tl, cpl, gaml, rcl)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX, S.step_no);
+    double rl = pl / (rcl * tl);
+    double pr, ur, vr, tr, rr, cpr, gamr, rcr;
+    if (nr >= 0) {
+        const double w0 = M.fvr[f], w1 = M.fvr[M.sf + f];
+        double qr[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) qr[k] = PA_(k, nr) + (GR_(k, 0, nr) * w0 + GR_(k, 1, nr) * w1);
+        pr = qr[0]; ur = qr[1]; vr = qr[2]; tr = qr[3];
+        s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NSP - 1; ++k) { yr[k] = qr[kNFlow + k]; s = s + yr[k]; }
+        yr[NSP - 1] = 1.0 - s;
+        if (!gas_param<NSP>(th, yr, tr, cpr, gamr, rcr)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX, S.step_no);
+        rr = pr / (rcr * tr);
+    } else {
+        const DevParams &P = pars[M.face_par ? M.face_par[f] : 0];
+        const int type = -nr;
+        cpr = cpl; gamr = gaml; rcr = rcl;
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) yr[k] = yl[k];
+        if (type == 1) {
+            rr = rl; pr = pl; tr = tl; ur = -ul; vr = -vl;
+        } else if (type == 2 || type == 6) {
+            rr = rl; pr = pl; tr = tl;
+            const double vn = ul * s1n + vl * s2n;
+            ur = ul - 2 * vn * s1n;
+            vr = vl - 2 * vn * s2n;
+        } else if (type == 3 || type == 4) {
+            const double *yin = type == 3 ? P.ox_y : P.fu_y;
+#pragma unroll
+            for (int k = 0; k < NSP; ++k) yr[k] = yin[k];
+            tr = type == 3 ? P.ox_t : P.fu_t;
+            if (!gas_param<NSP>(th, yr, tr, cpr, gamr, rcr)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX, S.step_no);
+            pr = pl;
+            rr = pr / (rcr * tr);
+            ur = (type == 3 ? P.ox_g : P.fu_g) / rr;
+            vr = 0.0;
+            pl = pr; ul = ur; vl = vr; tl = tr; rl = rr;
+#pragma unroll
+            for (int k = 0; k < NSP; ++k) yl[k] = yr[k];
+            cpl = cpr; gaml = gamr; rcl = rcr;
+        } else {  // 5: nozzle outlet
+            const double ac = sqrt(gaml * pl / rl);
+            const double vn = ul * s1n + vl * s2n;
+            if (fabs(vn) > ac) {
+                rr = rl; ur = ul; vr = vl; pr = pl; tr = tl;
+            } else {
+                pr = P.p_out;
+                ur = ul + s1n * (pl - pr) / (rl * ac);
+                vr = vl + s2n * (pl - pr) / (rl * ac);
+                rr = rl + (pr - pl) / (ac * ac);
+                tr = pr / (rcl * rr);
+                pl = pr; ul = ur; vl = vr; tl = tr; rl = rr;
+            }
+        }
+    }
+    if (rl < 0. || pl < 0. || (nr >= 0 && (rr < 0. || pr < 0.))) flag_error(S.err, RC_ERR_NEGATIVE, f, K_FLUX, S.step_no);
+    double h[NSP], hl = 0.0, hr = 0.0;
+    if (!enthalpy<NSP>(th, tl, h)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX, S.step_no);
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) hl = hl + yl[k] * h[k];
+    if (!enthalpy<NSP>(th, tr, h)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX, S.step_no);
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) hr = hr + yr[k] * h[k];
+    const double vaml = ul * ul + vl * vl, vamr = ur * ur + vr * vr;
+    hl = hl + 0.5 * vaml;
+    hr = hr + 0.5 * vamr;
+    const double el = hl - pl / rl, er = hr - pr / rr;
+    const double acl = sqrt(gaml * pl / rl), acr = sqrt(gamr * pr / rr);
+    const double uln = sav1 * ul + sav2 * vl, urn = sav1 * ur + sav2 * vr;
+    const double coex = fmax(fabs(uln) + acl * sav, fabs(urn) + acr * sav);
+    const double ruln = rl * uln, rurn = rr * urn;
+    double *F = S.fflux;
+    const int sf = M.sf;
+    F[0 * sf + f] = 0.5 * (ruln + rurn - coex * (rr - rl));
+    F[1 * sf + f] = 0.5 * (ruln * ul + rurn * ur + sav1 * (pl + pr) - coex * (rr * ur - rl * ul));
+    F[2 * sf + f] = 0.5 * (ruln * vl + rurn * vr + sav2 * (pl + pr) - coex * (rr * vr - rl * vl));
+    F[3 * sf + f] = 0.5 * (ruln * hl + rurn * hr - coex * (rr * er - rl * el));
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k)
+        F[(kNFlow + k) * sf + f] = 0.5 * (ruln * yl[k] + rurn * yr[k] - coex * (rr * yr[k] - rl * yl[k]));
+}
+
+// residual assembly of flux_sourceterm (fvm2d.f90:201-219): hWA = sum of
+// -F (owner) / +F (neighbour) over the cell's faces in face order, then the
+// chemical source vol*omega of sourceterm (sourceterm.f90:310-366)
+template <int NSP>
+__global__ void __launch_bounds__(256) k_residual(DevMesh M, DevState S, const DevThermo *__restrict__ thp,
+                                                  const DevParams *__restrict__ pars) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M.ne) return;
+    double h[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) h[k] = 0.0;
+#pragma unroll
+    for (int kf = 0; kf < 3; ++kf) {
+        const int cf = M.cface[kf * M.sc + i];
+        const int f = cf >= 0 ? cf : ~cf;
+        if (cf >= 0) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) h[k] = h[k] - S.fflux[k * M.sf + f];
+        } else {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) h[k] = h[k] + S.fflux[k * M.sf + f];
+        }
+    }
+    const DevParams &P = pars[M.cell_par ? M.cell_par[i] : 0];
+    if (P.combustion) {
+        const DevThermo &th = *thp;
+        const double p = PA_(0, i), t = PA_(3, i);
+        double y[NSP], s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NSP - 1; ++k) { y[k] = PA_(kNFlow + k, i); s = s + y[k]; }
+        y[NSP - 1] = 1.0 - s;
+        double cp, gam, r;
+        if (!gas_param<NSP>(th, y, t, cp, gam, r)) flag_error(S.err, RC_ERR_TEMPERATURE, i, K_RES, S.step_no);
+        const double rho = p / (t * r);
+        double w[NSP];
+        production_rate<NSP>(th, t, rho, y, w);
+        const double v = M.vol[i];
+#pragma unroll
+        for (int k = 0; k < kNFlow; ++k) h[k] = h[k] + v * 0.0;
+#pragma unroll
+        for (int k = 0; k < NSP - 1; ++k) h[kNFlow + k] = h[kNFlow + k] + v * w[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) HWA_(k, i) = h[k];
+}
+
+// the neighbour term of both LU-SGS sweeps (Time_LUSGS_steady.f90:72-111):
+// the linearised Rusanov flux change of neighbour NR's increment
+template <int NSP>
+__device__ __forceinline__ bool lusgs_neighbor(const DevMesh &M, const DevState &S, const DevThermo &th, int nr,
+                                               double dx, double dy, double af, double *dflux) {
+    const double pp = PA_(0, nr), uu = PA_(1, nr), vv = PA_(2, nr), tt = PA_(3, nr);
+    double yy[NSP], s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) { yy[k] = PA_(kNFlow + k, nr); s = s + yy[k]; }
+    yy[NSP - 1] = 1.0 - s;
+    double cp, gama, rcp, h[NSP];
+    bool ok = gas_param<NSP>(th, yy, tt, cp, gama, rcp);
+    ok = enthalpy<NSP>(th, tt, h) && ok;
+    double hh = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) hh = hh + yy[k] * h[k];
+    const double rr = pp / (rcp * tt);
+    const double rou = rr * uu, rov = rr * vv;
+    const double roe = rr * (hh + 0.5 * (uu * uu + vv * vv)) - pp;
+    const double drr = HWA_(0, nr), dru = HWA_(1, nr), drv = HWA_(2, nr), dre = HWA_(3, nr);
+    double dry[NSP];
+    s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) { dry[k] = HWA_(kNFlow + k, nr); s = s + dry[k]; }
+    dry[NSP - 1] = drr - s;
+    const double duu = (dru - uu * drr) / rr;
+    const double dvv = (drv - vv * drr) / rr;
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) s1 = s1 + h[k] * dry[k];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) s2 = s2 + th.rsp[k] * dry[k];
+    const double dpp = (gama - 1) * (dre - s1 + tt * s2 * gama / (gama - 1)
+                                     - 0.5 * (dru * uu + duu * rou + drv * vv + dvv * rov));
+    const double un = uu * dx + vv * dy;
+    const double dun = duu * dx + dvv * dy;
+    dflux[0] = dflux[0] + 0.5 * (-af * drr + dru * dx + drv * dy);
+    dflux[1] = dflux[1] + 0.5 * (-af * dru + dru * un + rou * dun + dpp * dx);
+    dflux[2] = dflux[2] + 0.5 * (-af * drv + drv * un + rov * dun + dpp * dy);
+    dflux[3] = dflux[3] + 0.5 * (-af * dre + dun * (roe + pp) + un * (dre + dpp));
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k)
+        dflux[kNFlow + k] = dflux[kNFlow + k] + 0.5 * (-af * dry[k] + dry[k] * un + (rr * yy[k]) * dun);
+    return ok;
+}
+
+// one wavefront of the forward (FWD) or backward sweep of LU_SGS_steady
+// (Time_LUSGS_steady.f90:56-120 / 127-190) over sched[lo, hi)
+template <int NSP, bool FWD>
+__global__ void __launch_bounds__(256) k_lusgs(DevMesh M, DevState S, const DevThermo *__restrict__ thp, int lo, int hi) {
+    const int t = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= hi) return;
+    const int i = M.sched[t];
+    const DevThermo &th = *thp;
+    double dflux[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) dflux[k] = 0.0;
+    bool ok = true;
+#pragma unroll
+    for (int kf = 0; kf < 3; ++kf) {
+        const int cf = M.cface[kf * M.sc + i];
+        const int f = cf >= 0 ? cf : ~cf;
+        double dx = M.vecx[f], dy = M.vecx[M.sf + f];
+        int nr = M.fnr[f];
+        if (nr == i) { nr = M.fnl[f]; dx = -dx; dy = -dy; }
+        if (nr >= 0 && (FWD ? nr < i : nr > i))
+            ok = lusgs_neighbor<NSP>(M, S, th, nr, dx, dy, S.alpaf[f], dflux) && ok;
+    }
+    if (!ok) flag_error(S.err, RC_ERR_TEMPERATURE, i, FWD ? K_FWD : K_BWD, S.step_no);
+    const double d = S.dd[i];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        if (FWD) HWA_(k, i) = (HWA_(k, i) - dflux[k]) / d;
+        else HWA_(k, i) = HWA_(k, i) - dflux[k] / d;
+    }
+}
+
+// WA += dW (Time_LUSGS_steady.f90:193-197) and update_primitve_variables
+// (update_flow_variables.f90:208-252), plus the NaN check of lu_sgs_stage
+template <int NSP>
+__global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, const DevThermo *__restrict__ thp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M.ne) return;
+    const DevThermo &th = *thp;
+    double w[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) { w[k] = WA_(k, i) + HWA_(k, i); WA_(k, i) = w[k]; }
+    const double rho = w[0], u = w[1] / rho, v = w[2] / rho;
+    double y[NSP], s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) y[k] = w[kNFlow + k] / rho;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) s = s + y[k];
+    y[NSP - 1] = 1.0 - s;
+    double t, cp, gam, r;
+    if (!newton_temperature<NSP>(th, w[3] / rho - 0.5 * (u * u + v * v), y, PA_(3, i), t, cp, gam, r))
+        flag_error(S.err, RC_ERR_TEMPERATURE, i, K_UPDATE, S.step_no);
+    PA_(0, i) = rho * r * t;
+    PA_(1, i) = u;
+    PA_(2, i) = v;
+    PA_(3, i) = t;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) PA_(kNFlow + k, i) = y[k];
+    if (i == 0 && isnan(w[0])) flag_error(S.err, RC_ERR_NAN, 0, K_UPDATE, S.step_no);
+}
+
+// primitve_to_conserved_variables (update_flow_variables.f90:254-288)
+template <int NSP>
+__global__ void __launch_bounds__(256) k_prim_to_cons(DevMesh M, DevState S, const DevThermo *__restrict__ thp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M.ne) return;
+    const DevThermo &th = *thp;
+    const double pp = PA_(0, i), uu = PA_(1, i), vv = PA_(2, i), tt = PA_(3, i);
+    double yy[NSP], s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) { yy[k] = PA_(kNFlow + k, i); s = s + yy[k]; }
+    yy[NSP - 1] = 1.0 - s;
+    double cp, gam, r, h[NSP];
+    bool ok = gas_param<NSP>(th, yy, tt, cp, gam, r);
+    const double rr = pp / (r * tt);
+    ok = enthalpy<NSP>(th, tt, h) && ok;
+    if (!ok) flag_error(S.err, RC_ERR_TEMPERATURE, i, K_P2C, S.step_no);
+    double hh = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) hh = hh + yy[k] * h[k];
+    WA_(0, i) = rr;
+    WA_(1, i) = rr * uu;
+    WA_(2, i) = rr * vv;
+    WA_(3, i) = rr * (hh + 0.5 * (uu * uu + vv * vv)) - pp;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) WA_(kNFlow + k, i) = rr * yy[k];
+}
+
+// Residual (residual.f90:4-21): per-block partial sums of vol*|hWA| and
+// max(WA(1)); a second pass folds the partials in block order, so the
+// result is deterministic.
+__global__ void __launch_bounds__(256) k_resnorm_partial(DevMesh M, DevState S, double *partial) {
+    __shared__ double sh[256][NV + 1];
+    const int tid = threadIdx.x;
+    double acc[NV + 1];
+#pragma unroll
+    for (int k = 0; k <= NV; ++k) acc[k] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + tid; i < M.ne; i += gridDim.x * blockDim.x) {
+        const double v = M.vol[i];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += v * fabs(HWA_(k, i));
+        acc[NV] = fmax(acc[NV], WA_(0, i));
+    }
+#pragma unroll
+    for (int k = 0; k <= NV; ++k) sh[tid][k] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) sh[tid][k] += sh[tid + w][k];
+            sh[tid][NV] = fmax(sh[tid][NV], sh[tid + w][NV]);
+        }
+        __syncthreads();
+    }
+    if (tid <= NV) partial[blockIdx.x * (NV + 1) + tid] = sh[0][tid];
+}
+
+__global__ void k_resnorm_final(const double *partial, int nblk, double inv_volume, double *out) {
+    const int k = threadIdx.x;
+    if (k > NV) return;
+    double a = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+        const double x = partial[b * (NV + 1) + k];
+        a = (k == NV) ? fmax(a, x) : a + x;
+    }
+    out[k] = (k == NV) ? a : a * inv_volume;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline dim3 grid_for(int n, int bs = 256) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+void launch_ts_face(const DevMesh &M, const DevState &S, const DevThermo *th, hipStream_t st) {
+    if (M.nf) k_timestep_face<kNsp><<<grid_for(M.nf), 256, 0, st>>>(M, S, th);
+}
+
+void launch_ts_cell(const DevMesh &M, const DevState &S, const DevParams *par, hipStream_t st) {
+    k_timestep_cell<<<grid_for(M.ne), 256, 0, st>>>(M, S, par);
+}
+
+void launch_ghost(const DevMesh &M, const DevState &S, const DevThermo *th, const DevParams *par, hipStream_t st) {
+    if (M.ng) k_ghost<kNsp><<<grid_for(M.ng), 256, 0, st>>>(M, S, th, par);
+}
+
+void launch_gradlim(const DevMesh &M, const DevState &S, bool grad, bool limit, hipStream_t st) {
+    if (grad && limit) k_gradlim<true, true><<<grid_for(M.ne), 256, 0, st>>>(M, S);
+    else if (grad) k_gradlim<true, false><<<grid_for(M.ne), 256, 0, st>>>(M, S);
+    else if (limit) k_gradlim<false, true><<<grid_for(M.ne), 256, 0, st>>>(M, S);
+}
+
+void launch_flux(const DevMesh &M, const DevState &S, const DevThermo *th, const DevParams *par, hipStream_t st) {
+    k_flux<kNsp><<<grid_for(M.nf), 256, 0, st>>>(M, S, th, par);
+}
+
+void launch_residual(const DevMesh &M, const DevState &S, const DevThermo *th, const DevParams *par, hipStream_t st) {
+    k_residual<kNsp><<<grid_for(M.ne), 256, 0, st>>>(M, S, th, par);
+}
+
+void launch_lusgs_level(const DevMesh &M, const DevState &S, const DevThermo *th, bool fwd, int lo, int hi, hipStream_t st) {
+    if (hi <= lo) return;
+    if (fwd) k_lusgs<kNsp, true><<<grid_for(hi - lo), 256, 0, st>>>(M, S, th, lo, hi);
+    else k_lusgs<kNsp, false><<<grid_for(hi - lo), 256, 0, st>>>(M, S, th, lo, hi);
+}
+
+void launch_update(const DevMesh &M, const DevState &S, const DevThermo *th, hipStream_t st) {
+    k_update<kNsp><<<grid_for(M.ne), 256, 0, st>>>(M, S, th);
+}
+
+void launch_prim_to_cons(const DevMesh &M, const DevState &S, const DevThermo *th, hipStream_t st) {
+    k_prim_to_cons<kNsp><<<grid_for(M.ne), 256, 0, st>>>(M, S, th);
+}
+
+void launch_resnorm(const DevMesh &M, const DevState &S, double *partial, int nblk, double inv_volume, double *out,
+                    hipStream_t st) {
+    k_resnorm_partial<<<nblk, 256, 0, st>>>(M, S, partial);
+    k_resnorm_final<<<1, 64, 0, st>>>(partial, nblk, inv_volume, out);
+}
+
+}  // namespace rc

@@ -1,0 +1,469 @@
+// Host-side mesh set-up of the library: faces, boundary tags, cell volumes
+// and centroids, reconstruction stencils, mirrored ghost cells and the
+// least-squares matrices, computed with the reference's numbering
+// conventions so that every per-cell sum runs in the reference's order.
+//
+//   faces in order of first appearance over cells     src/mesh.f90:120-199
+//   boundary tags Neighbor(2,f) = -type                src/mesh.f90:76-93
+//   13-point triangle rule for vol / centroid          src/gaussInteg4.f90:4-50, mesh.f90:206-255
+//   NeighF ascending, NeighC, Neigh                    src/reconstruction_initialization.f90:11-50
+//   mirrored ghost cells                               src/ghostcell.f90:203-379
+//   wdis = 1/|dc|, A = wdis*(<x_j>-<x_i>)/reflen       src/reconstruction_initialization.f90:93-150
+//   pinv(A) (dgesvd in the reference, la_math.f90:1-62; here a thin QR)
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+#include <queue>
+#include <unordered_map>
+
+#include "rc_internal.h"
+
+namespace rc {
+namespace {
+
+// real**integer as compiler-rt's __powidf2 evaluates it
+double powi(double a, int b) {
+    const bool recip = b < 0;
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+// integrand selector of the quadrature calls: 1, x, y, or (x-cx)/rx, (y-cy)/ry
+struct Integrand {
+    int kind;
+    double c, r;
+    double operator()(const double *v) const {
+        switch (kind) {
+        case 0: return 1.0;
+        case 1: return v[0];
+        case 2: return v[1];
+        case 3: return (v[0] - c) / r;
+        default: return (v[1] - c) / r;
+        }
+    }
+};
+
+// 13-point, 7th-order triangle rule (weights and points of gaussInteg4.f90:14-25)
+double tri_rule(const Integrand &f, const double *p1, const double *p2, const double *p3) {
+    const double x1 = p1[0], y1 = p1[1], x2 = p2[0], y2 = p2[1], x3 = p3[0], y3 = p3[1];
+    const double area = 0.5 * std::fabs((x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1));
+    double q[13][2];
+    for (int d = 0; d < 2; ++d) {
+        q[0][d] = (p1[d] + p2[d] + p3[d]) / 3.0;
+        double a = 0.479308067841920, b = 0.260345966079040;
+        q[1][d] = a * p1[d] + b * (p2[d] + p3[d]);
+        q[2][d] = a * p2[d] + b * (p1[d] + p3[d]);
+        q[3][d] = a * p3[d] + b * (p1[d] + p2[d]);
+        a = 0.869739794195568; b = 0.065130102902216;
+        q[4][d] = a * p1[d] + b * (p2[d] + p3[d]);
+        q[5][d] = a * p2[d] + b * (p1[d] + p3[d]);
+        q[6][d] = a * p3[d] + b * (p1[d] + p2[d]);
+        a = 0.048690315425316; b = 0.312865496004874;
+        const double w = 1 - a - b;
+        q[7][d] = a * p1[d] + b * p2[d] + w * p3[d];
+        q[8][d] = a * p1[d] + b * p3[d] + w * p2[d];
+        q[9][d] = a * p2[d] + b * p1[d] + w * p3[d];
+        q[10][d] = a * p2[d] + b * p3[d] + w * p1[d];
+        q[11][d] = a * p3[d] + b * p1[d] + w * p2[d];
+        q[12][d] = a * p3[d] + b * p2[d] + w * p1[d];
+    }
+    const double s = -0.149570044467682 * f(q[0])
+        + 0.175615257433208 * (f(q[1]) + f(q[2]) + f(q[3]))
+        + 0.053347235608838 * (f(q[4]) + f(q[5]) + f(q[6]))
+        + 0.077113760890257 * (f(q[7]) + f(q[8]) + f(q[9]) + f(q[10]) + f(q[11]) + f(q[12]));
+    return s * area;
+}
+
+// quadrilateral (v1,v2,v3,v4) as two triangles; a triangle repeats v3
+double quad_rule(const Integrand &f, const double *a, const double *b, const double *c, const double *d) {
+    return tri_rule(f, a, b, c) + tri_rule(f, a, c, d);
+}
+
+// pinv of a 3x2 matrix through a thin QR (Gram-Schmidt, one
+// re-orthogonalisation pass): pinv = R^-1 Q^T.
+void pinv32(const double a[3][2], double b[2][3]) {
+    double q0[3], v[3];
+    const double n0 = std::sqrt(a[0][0] * a[0][0] + a[1][0] * a[1][0] + a[2][0] * a[2][0]);
+    for (int i = 0; i < 3; ++i) q0[i] = a[i][0] / n0;
+    for (int i = 0; i < 3; ++i) v[i] = a[i][1];
+    double r01 = q0[0] * v[0] + q0[1] * v[1] + q0[2] * v[2];
+    for (int i = 0; i < 3; ++i) v[i] = v[i] - r01 * q0[i];
+    const double s = q0[0] * v[0] + q0[1] * v[1] + q0[2] * v[2];
+    for (int i = 0; i < 3; ++i) v[i] = v[i] - s * q0[i];
+    r01 = r01 + s;
+    const double n1 = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int i = 0; i < 3; ++i) {
+        const double y1 = (v[i] / n1) / n1;
+        const double y0 = (q0[i] - r01 * y1) / n0;
+        b[0][i] = y0;
+        b[1][i] = y1;
+    }
+}
+
+inline uint64_t edge_key(int a, int b) {
+    const uint32_t lo = (uint32_t)std::min(a, b), hi = (uint32_t)std::max(a, b);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+int fail(std::string &msg, const char *fmt, int a = 0, int b = 0) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, fmt, a, b);
+    msg = buf;
+    return RC_ERR_MESH;
+}
+
+}  // namespace
+
+int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, const int *bedges0,
+               const int *btype, HostMesh &m, std::string &msg) {
+    if (nnt <= 0 || ne <= 0 || nb < 0) return fail(msg, "empty mesh");
+    m.nnt = nnt;
+    m.ne = ne;
+    m.coor.assign(coor, coor + 2 * (size_t)nnt);
+    m.cn.assign(cells0, cells0 + 3 * (size_t)ne);
+    for (size_t k = 0; k < m.cn.size(); ++k)
+        if (m.cn[k] < 0 || m.cn[k] >= nnt) return fail(msg, "cell node %d out of range (cell %d)", m.cn[k], (int)(k / 3));
+    const double *X = m.coor.data();
+
+    // ---- faces in order of first appearance (mesh.f90:120-199)
+    std::unordered_map<uint64_t, int> fidx;
+    fidx.reserve((size_t)ne * 2);
+    m.fnode.clear(); m.fnl.clear(); m.fnr.clear(); m.vecx.clear();
+    for (int j = 0; j < ne; ++j) {
+        for (int kf = 0; kf < 3; ++kf) {
+            const int n1 = m.cn[3 * j + kf], n2 = m.cn[3 * j + (kf + 1) % 3];
+            if (n1 == n2) return fail(msg, "degenerate cell %d", j);
+            auto it = fidx.find(edge_key(n1, n2));
+            if (it == fidx.end()) {
+                const int f = (int)m.fnl.size();
+                fidx.emplace(edge_key(n1, n2), f);
+                m.fnl.push_back(j);
+                m.fnr.push_back(-100);
+                m.fnode.push_back(n1);
+                m.fnode.push_back(n2);
+                m.vecx.push_back(X[2 * n2 + 1] - X[2 * n1 + 1]);
+                m.vecx.push_back(-(X[2 * n2] - X[2 * n1]));
+            } else {
+                if (m.fnr[it->second] != -100) return fail(msg, "face shared by more than two cells (cell %d)", j);
+                m.fnr[it->second] = j;
+            }
+        }
+    }
+    m.nf = (int)m.fnl.size();
+
+    // ---- boundary tags (mesh.f90:76-93)
+    std::unordered_map<uint64_t, int> btag;
+    btag.reserve((size_t)nb * 2);
+    for (int b = 0; b < nb; ++b) {
+        const int t = btype[b];
+        if (t < 1 || t > 6) return fail(msg, "unsupported boundary type %d", t);
+        btag.emplace(edge_key(bedges0[2 * b], bedges0[2 * b + 1]), t);
+    }
+    for (int f = 0; f < m.nf; ++f) {
+        if (m.fnr[f] < 0) {
+            auto it = btag.find(edge_key(m.fnode[2 * f], m.fnode[2 * f + 1]));
+            if (it == btag.end()) return fail(msg, "not find boundary of this type (face %d)", f + 1);
+            m.fnr[f] = -it->second;
+        }
+    }
+
+    // ---- volumes, centroids, reflen (mesh.f90:206-255)
+    m.vol.assign(ne, 0.0);
+    m.cellxy.assign(2 * (size_t)ne, 0.0);
+    m.reflen.assign(2 * (size_t)ne, 0.0);
+    const Integrand one{0, 0, 0}, fx{1, 0, 0}, fy{2, 0, 0};
+    for (int i = 0; i < ne; ++i) {
+        const double *p1 = X + 2 * m.cn[3 * i], *p2 = X + 2 * m.cn[3 * i + 1], *p3 = X + 2 * m.cn[3 * i + 2];
+        const double v = quad_rule(one, p1, p2, p3, p3);
+        m.vol[i] = v;
+        m.cellxy[2 * i] = quad_rule(fx, p1, p2, p3, p3) / v;
+        m.cellxy[2 * i + 1] = quad_rule(fy, p1, p2, p3, p3) / v;
+        if (v < 0.0) return fail(msg, "negative volume %d", i + 1);
+        // a clockwise triangle gives owner-inward VECX; the reference assumes CCW
+        const double cr = (p2[0] - p1[0]) * (p3[1] - p1[1]) - (p3[0] - p1[0]) * (p2[1] - p1[1]);
+        if (!(cr > 0.0)) return fail(msg, "cell %d is not counter-clockwise", i + 1);
+        for (int a = 0; a < 2; ++a)
+            for (int b = a + 1; b < 3; ++b)
+                for (int d = 0; d < 2; ++d)
+                    m.reflen[2 * i + d] = std::max(0.5 * std::fabs(X[2 * m.cn[3 * i + a] + d] - X[2 * m.cn[3 * i + b] + d]),
+                                                   m.reflen[2 * i + d]);
+    }
+    double sv = 0.0;
+    for (int i = 0; i < ne; ++i) sv = sv + m.vol[i];
+    m.total_volume = sv;
+
+    // ---- NeighF / NeighC (reconstruction_initialization.f90:11-50)
+    std::vector<int> cnt(ne, 0);
+    m.neighf.assign(3 * (size_t)ne, -1);
+    for (int f = 0; f < m.nf; ++f) {
+        const int nl = m.fnl[f], nr = m.fnr[f];
+        if (cnt[nl] >= 3) return fail(msg, "cell %d has more than three faces", nl + 1);
+        m.neighf[3 * nl + cnt[nl]++] = f;
+        if (nr >= 0) {
+            if (cnt[nr] >= 3) return fail(msg, "cell %d has more than three faces", nr + 1);
+            m.neighf[3 * nr + cnt[nr]++] = f;
+        }
+    }
+    m.neigh.assign(ne, 0);
+    m.neighc.assign(3 * (size_t)ne, -1);
+    m.cface.assign(3 * (size_t)ne, 0);
+    for (int i = 0; i < ne; ++i) {
+        if (cnt[i] != 3) return fail(msg, "cell %d has %d faces", i + 1, cnt[i]);
+        for (int k = 0; k < 3; ++k) {
+            const int f = m.neighf[3 * i + k];
+            const int sides[2] = {m.fnl[f], m.fnr[f]};
+            for (int s = 0; s < 2; ++s) {
+                const int ic = sides[s];
+                if (ic != i && ic >= 0) m.neighc[3 * i + m.neigh[i]++] = ic;
+            }
+            m.cface[3 * i + k] = (m.fnl[f] == i) ? f : ~f;
+        }
+    }
+
+    // ---- ghost cells (ghostcell.f90:203-379)
+    int ng = 0;
+    for (int i = 0; i < ne; ++i) ng += 3 - m.neigh[i];
+    m.ng = ng;
+    m.ghost_host.assign(ng, 0);
+    m.ghost_face.assign(ng, 0);
+    m.ghost_type.assign(ng, 0);
+    m.ghost_nodes.assign(8 * (size_t)ng, 0.0);
+    m.ghost_vol.assign(ng, 0.0);
+    m.ghost_xy.assign(2 * (size_t)ng, 0.0);
+    m.gnrm.assign(2 * (size_t)ng, 0.0);
+    m.num_ghost.assign(ne, 0);
+    m.ghostcell.assign(3 * (size_t)ne, -1);
+    int j = 0;
+    for (int i = 0; i < ne; ++i) {
+        if (m.neigh[i] < 3) {
+            for (int kf = 0; kf < 3; ++kf) {
+                const int f = m.neighf[3 * i + kf];
+                if (m.fnr[f] >= 0) continue;
+                m.ghost_host[j] = i;
+                m.ghost_face[j] = f;
+                m.ghost_type[j] = -m.fnr[f];
+                m.ghostcell[3 * i + m.num_ghost[i]++] = j;
+                const int n1 = m.fnode[2 * f], n2 = m.fnode[2 * f + 1];
+                int k1 = 0;
+                while (k1 < 3 && m.cn[3 * i + k1] != n1) ++k1;
+                const int k2 = (k1 + 1) % 3;
+                if (k1 == 3 || m.cn[3 * i + k2] != n2) return fail(msg, "ghost cell node order (cell %d)", i + 1);
+                const int k3 = (k2 + 1) % 3, k4 = (k3 + 1) % 3;
+                const int n3 = m.cn[3 * i + k3], n4 = m.cn[3 * i + k4];
+                const double sav1 = m.vecx[2 * f], sav2 = m.vecx[2 * f + 1];
+                const double sav = std::max(std::sqrt(sav1 * sav1 + sav2 * sav2), kSmall);
+                const double s1n = sav1 / sav, s2n = sav2 / sav;
+                double *g = &m.ghost_nodes[8 * (size_t)j];
+                g[0] = X[2 * n2]; g[1] = X[2 * n2 + 1];
+                g[2] = X[2 * n1]; g[3] = X[2 * n1 + 1];
+                const double d1 = (X[2 * n1] - X[2 * n4]) * s1n + (X[2 * n1 + 1] - X[2 * n4 + 1]) * s2n;
+                if (d1 < 0) return fail(msg, "ghost distance (n1,n4) negative (cell %d)", i + 1);
+                g[4] = X[2 * n4] + 2 * d1 * s1n;
+                g[5] = X[2 * n4 + 1] + 2 * d1 * s2n;
+                const double d2 = (X[2 * n2] - X[2 * n3]) * s1n + (X[2 * n2 + 1] - X[2 * n3 + 1]) * s2n;
+                if (d2 < 0) return fail(msg, "ghost distance (n2,n3) negative (cell %d)", i + 1);
+                g[6] = X[2 * n3] + 2 * d2 * s1n;
+                g[7] = X[2 * n3 + 1] + 2 * d2 * s2n;
+                const double gv = quad_rule(one, g, g + 2, g + 4, g + 6);
+                m.ghost_vol[j] = gv;
+                m.ghost_xy[2 * j] = quad_rule(fx, g, g + 2, g + 4, g + 6) / gv;
+                m.ghost_xy[2 * j + 1] = quad_rule(fy, g, g + 2, g + 4, g + 6) / gv;
+                if (gv < 0.0) return fail(msg, "negative ghost volume %d", j + 1);
+                // unit normal as update_ghostcell_variables forms it from the
+                // first two ghost nodes (update_flow_variables.f90:318-332)
+                const double gs1 = g[3] - g[1];
+                const double gs2 = -(g[2] - g[0]);
+                const double gsav = std::max(std::sqrt(gs1 * gs1 + gs2 * gs2), kSmall);
+                m.gnrm[2 * j] = gs1 / gsav;
+                m.gnrm[2 * j + 1] = gs2 / gsav;
+                ++j;
+            }
+        }
+        if (m.num_ghost[i] + m.neigh[i] != 3) return fail(msg, "error in number of ghost cells for cell %d", i + 1);
+    }
+
+    // ---- LSQ matrices (reconstruction_initialization.f90:71-150)
+    m.xyh.assign(2 * (size_t)ne, 0.0);
+    m.wdis.assign(3 * (size_t)ne, 1.0);
+    m.amls.assign(6 * (size_t)ne, 0.0);
+    m.stencil.assign(3 * (size_t)ne, 0);
+    for (int i = 0; i < ne; ++i) {
+        const double *p1 = X + 2 * m.cn[3 * i], *p2 = X + 2 * m.cn[3 * i + 1], *p3 = X + 2 * m.cn[3 * i + 2];
+        for (int io = 0; io < 2; ++io) {
+            const Integrand fs{3 + io, m.cellxy[2 * i + io], m.reflen[2 * i + io]};
+            m.xyh[2 * i + io] = quad_rule(fs, p1, p2, p3, p3) / m.vol[i];
+        }
+        if (std::fabs(m.xyh[2 * i]) + std::fabs(m.xyh[2 * i + 1]) > 1.e-8) return fail(msg, "error in XYH, cell %d", i + 1);
+    }
+    for (int i = 0; i < ne; ++i) {
+        double a1[3][2], b1[2][3];
+        const int nn = m.neigh[i];
+        for (int k = 0; k < nn; ++k) {
+            const int jc = m.neighc[3 * i + k];
+            m.stencil[3 * i + k] = jc;
+            const double *q1 = X + 2 * m.cn[3 * jc], *q2 = X + 2 * m.cn[3 * jc + 1], *q3 = X + 2 * m.cn[3 * jc + 2];
+            const double dx = (m.cellxy[2 * jc] - 0.0) - m.cellxy[2 * i];
+            const double dy = (m.cellxy[2 * jc + 1] - 0.0) - m.cellxy[2 * i + 1];
+            const double w = powi(std::sqrt(dx * dx + dy * dy), -1);
+            m.wdis[3 * i + k] = w;
+            for (int io = 0; io < 2; ++io) {
+                const Integrand fs{3 + io, m.cellxy[2 * i + io], m.reflen[2 * i + io]};
+                a1[k][io] = w * (quad_rule(fs, q1, q2, q3, q3) / m.vol[jc] - m.xyh[2 * i + io]);
+            }
+        }
+        for (int k = 0; k < m.num_ghost[i]; ++k) {
+            const int g = m.ghostcell[3 * i + k];
+            m.stencil[3 * i + nn + k] = -(g + 1);
+            const double *gn = &m.ghost_nodes[8 * (size_t)g];
+            const double dx = m.ghost_xy[2 * g] - m.cellxy[2 * i];
+            const double dy = m.ghost_xy[2 * g + 1] - m.cellxy[2 * i + 1];
+            const double w = powi(std::sqrt(dx * dx + dy * dy), -1);
+            m.wdis[3 * i + nn + k] = w;
+            for (int io = 0; io < 2; ++io) {
+                const Integrand fs{3 + io, m.cellxy[2 * i + io], m.reflen[2 * i + io]};
+                a1[nn + k][io] = w * (quad_rule(fs, gn, gn + 2, gn + 4, gn + 6) / m.ghost_vol[g] - m.xyh[2 * i + io]);
+            }
+        }
+        pinv32(a1, b1);
+        for (int k = 0; k < 3; ++k) {
+            m.amls[6 * (size_t)i + 2 * k] = b1[0][k];
+            m.amls[6 * (size_t)i + 2 * k + 1] = b1[1][k];
+        }
+    }
+
+    // ---- derived kernel operands
+    m.lvv.assign(6 * (size_t)ne, 0.0);
+    for (int i = 0; i < ne; ++i) {
+        for (int kf = 0; kf < 3; ++kf) {
+            const int f = m.neighf[3 * i + kf];
+            const int n1 = m.fnode[2 * f], n2 = m.fnode[2 * f + 1];
+            for (int d = 0; d < 2; ++d) {
+                const double xy = 0.5 * (X[2 * n1 + d] + X[2 * n2 + d]);       // limiter.f90:228
+                m.lvv[6 * (size_t)i + 2 * kf + d] = (xy - m.cellxy[2 * i + d]) / m.reflen[2 * i + d];
+            }
+        }
+    }
+    m.fvl.assign(2 * (size_t)m.nf, 0.0);
+    m.fvr.assign(2 * (size_t)m.nf, 0.0);
+    for (int f = 0; f < m.nf; ++f) {
+        const int n1 = m.fnode[2 * f], n2 = m.fnode[2 * f + 1];
+        const int nl = m.fnl[f], nr = m.fnr[f];
+        for (int d = 0; d < 2; ++d) {
+            const double xy = X[2 * n1 + d] + 0.5 * (X[2 * n2 + d] - X[2 * n1 + d]);  // flux_lax.f90:34
+            m.fvl[2 * f + d] = (xy - m.cellxy[2 * nl + d]) / m.reflen[2 * nl + d];
+            if (nr >= 0) m.fvr[2 * f + d] = (xy - (m.cellxy[2 * nr + d] - 0.0)) / m.reflen[2 * nr + d];
+        }
+    }
+
+    // ---- LU-SGS wavefront levels of the given order: level(i) = 1 + max
+    // level(j) over neighbours j < i.  Cells of one level are independent in
+    // the forward sweep (neighbours below them are in lower levels) and in
+    // the backward sweep (neighbours above are in higher levels), so a
+    // level-by-level sweep is the reference's serial sweep.
+    m.level.assign(ne, 0);
+    int nlev = 0;
+    for (int i = 0; i < ne; ++i) {
+        int l = 0;
+        for (int k = 0; k < m.neigh[i]; ++k) {
+            const int jc = m.neighc[3 * i + k];
+            if (jc < i) l = std::max(l, m.level[jc] + 1);
+        }
+        m.level[i] = l;
+        nlev = std::max(nlev, l + 1);
+    }
+    m.lvl_ptr.assign(nlev + 1, 0);
+    for (int i = 0; i < ne; ++i) m.lvl_ptr[m.level[i] + 1]++;
+    for (int l = 0; l < nlev; ++l) m.lvl_ptr[l + 1] += m.lvl_ptr[l];
+    m.sched.assign(ne, 0);
+    std::vector<int> pos(m.lvl_ptr.begin(), m.lvl_ptr.end() - 1);
+    for (int i = 0; i < ne; ++i) m.sched[pos[m.level[i]]++] = i;
+    return RC_OK;
+}
+
+int order_multicolor(int ne, const int *cells0, int *perm, int *ncolors) {
+    if (ne <= 0) return RC_ERR_ARGUMENT;
+    // face adjacency
+    std::unordered_map<uint64_t, int> first;
+    first.reserve((size_t)ne * 2);
+    std::vector<int> adj(3 * (size_t)ne, -1), deg(ne, 0);
+    for (int j = 0; j < ne; ++j) {
+        for (int kf = 0; kf < 3; ++kf) {
+            const int n1 = cells0[3 * j + kf], n2 = cells0[3 * j + (kf + 1) % 3];
+            const uint64_t key = edge_key(n1, n2);
+            auto it = first.find(key);
+            if (it == first.end()) {
+                first.emplace(key, j);
+            } else {
+                const int o = it->second;
+                if (deg[o] >= 3 || deg[j] >= 3) return RC_ERR_MESH;
+                adj[3 * o + deg[o]++] = j;
+                adj[3 * j + deg[j]++] = o;
+            }
+        }
+    }
+    // reverse Cuthill-McKee, component by component, each started from a
+    // pseudo-peripheral cell (end of a BFS from the lowest-degree cell)
+    std::vector<int> order;
+    order.reserve(ne);
+    std::vector<char> seen(ne, 0);
+    std::vector<int> dist(ne, -1);
+    auto bfs_far = [&](int s) {
+        std::vector<int> q{s};
+        dist[s] = 0;
+        int far = s;
+        for (size_t h = 0; h < q.size(); ++h) {
+            const int u = q[h];
+            if (dist[u] > dist[far] || (dist[u] == dist[far] && deg[u] < deg[far])) far = u;
+            for (int k = 0; k < deg[u]; ++k)
+                if (dist[adj[3 * u + k]] < 0) { dist[adj[3 * u + k]] = dist[u] + 1; q.push_back(adj[3 * u + k]); }
+        }
+        for (int u : q) dist[u] = -1;
+        return far;
+    };
+    for (int s0 = 0; s0 < ne; ++s0) {
+        if (seen[s0]) continue;
+        const int start = bfs_far(bfs_far(s0));
+        std::vector<int> q{start};
+        seen[start] = 1;
+        for (size_t h = 0; h < q.size(); ++h) {
+            const int u = q[h];
+            int nb[3], nn = 0;
+            for (int k = 0; k < deg[u]; ++k)
+                if (!seen[adj[3 * u + k]]) nb[nn++] = adj[3 * u + k];
+            std::sort(nb, nb + nn, [&](int a, int b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+            for (int k = 0; k < nn; ++k) { seen[nb[k]] = 1; q.push_back(nb[k]); }
+        }
+        order.insert(order.end(), q.rbegin(), q.rend());
+    }
+    // greedy colouring in RCM order
+    std::vector<int> color(ne, -1), rank(ne);
+    for (int k = 0; k < ne; ++k) rank[order[k]] = k;
+    int nc = 0;
+    for (int k = 0; k < ne; ++k) {
+        const int u = order[k];
+        unsigned used = 0;
+        for (int t = 0; t < deg[u]; ++t)
+            if (color[adj[3 * u + t]] >= 0) used |= 1u << color[adj[3 * u + t]];
+        int c = 0;
+        while (used & (1u << c)) ++c;
+        color[u] = c;
+        nc = std::max(nc, c + 1);
+    }
+    std::vector<int> idx(ne);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) {
+        return color[a] != color[b] ? color[a] < color[b] : rank[a] < rank[b];
+    });
+    for (int k = 0; k < ne; ++k) perm[k] = idx[k];
+    if (ncolors) *ncolors = nc;
+    return RC_OK;
+}
+
+}  // namespace rc

@@ -258,6 +258,26 @@ def write_inlet(path: str, inlet: Inlet, what: str) -> None:
         f.write("# mass flow rate (kg/s)\n%.17g\n" % inlet.mass_flow)
 
 
+def write_transport_placeholders(case_dir: str, nsp: int) -> None:
+    """``transport.in`` and ``diffusion.in`` in the layouts that
+    ``setup_transport_property`` (transport_property.f90:31-42) and
+    ``setup_diffusion_velocity`` (diffusion_velocity.f90:30-35) read.  The
+    reference reads both at start-up even when IfViscous is false; on the
+    inviscid path their values are never used, so neutral numbers are written."""
+    comb = os.path.join(case_dir, "combustion")
+    os.makedirs(comb, exist_ok=True)
+    with open(os.path.join(comb, "transport.in"), "w") as f:
+        f.write("# transport_lowest_tem,transport_critical_tem,transport_highest_tem\n200.0, 1000.0, 60000.0\n")
+        f.write("# mu/K polynomial coefficients A,B,C,D (unused without viscous fluxes)\n")
+        for _ in range(4 * nsp):
+            f.write("0.5, 0.0, 0.0, 1.0\n")
+    with open(os.path.join(comb, "diffusion.in"), "w") as f:
+        f.write("# Boltzmann's constant k_B\n1.38064852E-23\n")
+        f.write("# collision diameter (A), LJ energy (K) (unused without viscous fluxes)\n")
+        for _ in range(nsp):
+            f.write("3.0, 100.0\n")
+
+
 def write_solver_info(case_dir: str, s: SolverInfo) -> None:
     d = os.path.join(case_dir, "flow")
     os.makedirs(d, exist_ok=True)

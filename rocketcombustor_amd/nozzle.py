@@ -117,8 +117,10 @@ def _incircle(ax, ay, bx, by, cx, cy, dx, dy):
 
 
 def nozzle_mesh(nx: int, ny: int, geom: Optional[NozzleGeometry] = None, seed: int = 1,
-                jitter: float = 0.2) -> Mesh:
-    """Triangulated nozzle with ``2*nx*ny`` cells in natural (x-major) order."""
+                jitter: float = 0.2, wall_type: int = 6, fuel_faces: int = 0) -> Mesh:
+    """Triangulated nozzle with ``2*nx*ny`` cells in natural (x-major) order.
+    ``wall_type`` 6 (slip) or 1 (viscous wall); the top ``fuel_faces`` inlet
+    faces become a fuel inlet (type 4)."""
     geom = geom or NozzleGeometry()
     L = geom.length
     dxi, deta = L / nx, 1.0 / ny
@@ -159,7 +161,10 @@ def nozzle_mesh(nx: int, ny: int, geom: Optional[NozzleGeometry] = None, seed: i
     axis = np.column_stack([i * (ny + 1), (i + 1) * (ny + 1)])
     wall = np.column_stack([i * (ny + 1) + ny, (i + 1) * (ny + 1) + ny])
     bedges = np.vstack([inlet, outlet, axis, wall]).astype(np.int32)
-    btype = np.concatenate([np.full(ny, 3), np.full(ny, 5), np.full(nx, 2), np.full(nx, 6)]).astype(np.int32)
+    in_type = np.full(ny, 3)
+    if fuel_faces:
+        in_type[ny - fuel_faces:] = 4
+    btype = np.concatenate([in_type, np.full(ny, 5), np.full(nx, 2), np.full(nx, wall_type)]).astype(np.int32)
     nface = nx * (ny + 1) + (nx + 1) * ny + nx * ny
     return Mesh(coor, cells, bedges, btype, nface)
 
@@ -221,6 +226,15 @@ class Case:
         return self.oxidizer.mass_flow / (2.0 * (y.max() - y.min()))
 
     @property
+    def fuel_mass_flux(self) -> float:
+        """G of type-4 faces: 0.5*mass_flow/(ymax-ymin) (src/initialization.f90:23,195)."""
+        ins = self.mesh.bedges[self.mesh.btype == 4]
+        if len(ins) == 0:
+            return 0.0
+        y = self.mesh.coor[ins.ravel(), 1]
+        return 0.5 * self.fuel.mass_flow / (y.max() - y.min())
+
+    @property
     def p_out(self) -> float:
         # p_static= p_out_ratio*oxidizer_pre_in  (src/initialization.f90:32)
         return self.info.p_out_ratio * self.oxidizer.pres
@@ -232,10 +246,15 @@ class Case:
 def nozzle_case(nx: int, ny: int, p0: float = 5.0e6, t0: float = 3000.0, phi: float = 1.0,
                 geom: Optional[NozzleGeometry] = None, mech: Optional[deck.Mechanism] = None,
                 cfl: float = 5.0, mesh_seed: int = 1, y_seed: int = 2, y_perturb: float = 0.01,
-                limit_id: int = 1, combustion: bool = True, p_out_ratio: float = 0.02) -> Case:
+                limit_id: int = 1, combustion: bool = True, p_out_ratio: float = 0.02,
+                wall_type: int = 6, fuel_faces: int = 0, subsonic_exit: bool = False) -> Case:
+    """A nozzle case.  ``subsonic_exit`` starts from the subsonic branch of
+    the area-Mach relation in the divergent part too (the outlet then takes
+    the fixed-pressure branch); the default mechanism is the reference's with
+    kinetics model 1 and the formation constant in K."""
     geom = geom or NozzleGeometry()
-    mech = mech or deck.default_mechanism()
-    mesh = nozzle_mesh(nx, ny, geom, seed=mesh_seed)
+    mech = mech or deck.default_mechanism(model=1, b1_in_kelvin=True)
+    mesh = nozzle_mesh(nx, ny, geom, seed=mesh_seed, wall_type=wall_type, fuel_faces=fuel_faces)
     nsp = mech.nsp
     # H2/O2 at equivalence ratio phi: mass ratio H2:O2 = phi*4:32
     y_in = np.zeros(nsp)
@@ -249,14 +268,15 @@ def nozzle_case(nx: int, ny: int, p0: float = 5.0e6, t0: float = 3000.0, phi: fl
     rho_in = p_in / (r_mix * t_in)
     g_in = rho_in * m_c * math.sqrt(gam * r_mix * t_in)
     oxid = deck.Inlet(y=y_in, temp=t_in, pres=p0, mass_flow=g_in * 2.0 * geom.h_chamber)
-    fuel = deck.Inlet(y=np.eye(nsp)[0], temp=300.0, pres=p0, mass_flow=0.0)
+    # fuel stream (type-4 faces only): pure H2 at 300 K with a tenth of the oxidizer mass flux
+    fuel = deck.Inlet(y=np.eye(nsp)[0], temp=300.0, pres=p0, mass_flow=0.1 * g_in * 2.0 * geom.h_chamber)
 
     # initial field
     cxy = mesh.centroids()
     x, y = cxy[:, 0], cxy[:, 1]
     h = geom.height(x)
     throat_x = geom.l_chamber + geom.l_converge
-    mach = area_mach(h / geom.h_throat, gam, x >= throat_x)
+    mach = area_mach(h / geom.h_throat, gam, (x >= throat_x) & (not subsonic_exit))
     t = t0 / (1.0 + 0.5 * (gam - 1.0) * mach * mach)
     p = p0 * (t / t0) ** (gam / (gam - 1.0))
     c = np.sqrt(gam * r_mix * t)
@@ -292,3 +312,5 @@ def write_reference_case(case_dir: str, case: Case, nstep_max: Optional[int] = N
             src = os.path.join(extra_decks, name)
             with open(src) as f, open(os.path.join(case_dir, "combustion", name), "w") as g:
                 g.write(f.read())
+    else:
+        deck.write_transport_placeholders(case_dir, case.mech.nsp)

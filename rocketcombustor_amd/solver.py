@@ -1,0 +1,164 @@
+"""Host-side mirror of the reference's operator interface for the steady
+path, driving the HIP library through its C-ABI.
+
+The reference's operators are argument-less subroutines over module state;
+here that state lives in a device-resident :class:`SteadyCase`, and the
+methods keep the reference's names and call sequence:
+
+    timestep()          TimeStep            src/TimeStep.f90:4
+    gradient()          gradient            src/gradient.f90:4
+    limiter_barth()     Limiter_Barth       src/limiter.f90:5
+    reconstruction()    reconstruction      src/fvm2d.f90:183
+    flux_sourceterm()   flux_sourceterm     src/fvm2d.f90:196
+    lu_sgs_steady()     LU_SGS_steady       src/Time_LUSGS_steady.f90:32
+    lu_sgs_stage()      lu_sgs_stage        src/Time_LUSGS_steady.f90:4
+    run_steady(n)       steady main loop    src/fvm2d.f90:48-150
+    residual()          Residual/step_residual  src/residual.f90:4-21,61-79
+
+Errors where the reference stops raise :class:`native.RcError`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import native
+from .native import NVAR, check, dptr, iptr, lib
+
+
+class SteadyCase:
+    """One nozzle/chamber case resident on a GPU."""
+
+    def __init__(self, case, device: int = 0):
+        L = lib()
+        self.case = case
+        m = case.mesh
+        self._mech = native.make_mechanism(case.mech)
+        self._par = native.make_params(case.info.cfl, case.info.limit_id, case.info.combustion, case.p_out,
+                                       case.oxidizer, case.inlet_mass_flux, case.fuel, case.fuel_mass_flux)
+        coor = np.ascontiguousarray(m.coor, dtype=np.float64)
+        cells1 = np.ascontiguousarray(m.cells, dtype=np.int32) + 1
+        bn1 = np.ascontiguousarray(m.bedges, dtype=np.int32) + 1
+        bt = np.ascontiguousarray(m.btype, dtype=np.int32)
+        h = C.c_void_p()
+        rc = L.rc_case_create(C.byref(self._mech), C.byref(self._par), len(coor), dptr(coor), len(cells1),
+                              iptr(cells1), len(bn1), iptr(bn1), iptr(bt), int(device), C.byref(h))
+        check(rc, None)
+        self.h = h
+        info = (C.c_longlong * 6)()
+        check(L.rc_case_info(h, info), h)
+        self.ne, self.nf, self.ng, self.nvar, self.nlev, self.device_bytes = [int(v) for v in info]
+        self.set_primitive(case.pa0)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().rc_case_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _call(self, name: str, *args) -> None:
+        check(getattr(lib(), name)(self.h, *args), self.h)
+
+    # -- state -------------------------------------------------------------
+    def set_primitive(self, pa: np.ndarray) -> None:
+        pa = np.ascontiguousarray(pa, dtype=np.float64)
+        if pa.shape != (self.ne, NVAR):
+            raise ValueError("PA must be (NE, Nvar)")
+        self._call("rc_set_primitive", dptr(pa))
+
+    def _get(self, name: str, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=np.float64)
+        self._call(name, dptr(out))
+        return out
+
+    def primitive(self) -> np.ndarray:
+        return self._get("rc_get_primitive", (self.ne, NVAR))
+
+    def conserved(self) -> np.ndarray:
+        return self._get("rc_get_conserved", (self.ne, NVAR))
+
+    def increment(self) -> np.ndarray:
+        """hWA: the residual after flux_sourceterm, dW after LU-SGS."""
+        return self._get("rc_get_increment", (self.ne, NVAR))
+
+    def gradient_array(self) -> np.ndarray:
+        return self._get("rc_get_gradient", (self.ne, NVAR, 2))
+
+    def ghost(self) -> np.ndarray:
+        out = np.empty((self.ng, NVAR))
+        if self.ng:
+            self._call("rc_get_ghost", dptr(out))
+        return out
+
+    def timestep_arrays(self):
+        step = np.empty(self.ne)
+        alpaf = np.empty(self.nf)
+        self._call("rc_get_timestep", dptr(step), dptr(alpaf))
+        return step, alpaf
+
+    # -- operators -----------------------------------------------------------
+    def timestep(self) -> None:
+        self._call("rc_timestep")
+
+    def gradient(self) -> None:
+        self._call("rc_gradient")
+
+    def limiter_barth(self) -> None:
+        self._call("rc_limiter_barth")
+
+    def reconstruction(self) -> None:
+        self._call("rc_reconstruction")
+
+    def flux_sourceterm(self) -> None:
+        self._call("rc_flux_sourceterm")
+
+    def lu_sgs_steady(self) -> None:
+        self._call("rc_lusgs_steady")
+
+    def lu_sgs_stage(self) -> None:
+        self._call("rc_lusgs_stage")
+
+    def residual(self) -> np.ndarray:
+        """Res(1:Nvar)/total_volume and max(rho), as step_residual reports."""
+        return self._get("rc_residual", (NVAR + 1,))
+
+    def run_steady(self, nsteps: int, every: int = 0, history: bool = True) -> Optional[np.ndarray]:
+        """``nsteps`` steady LU-SGS steps; with ``history`` the step_residual
+        norms at step 1 and every ``every`` steps, shape (nreports, Nvar+1)."""
+        every = every if every > 0 else nsteps + 1
+        nrep = 1 + nsteps // every if nsteps > 0 else 0
+        buf = np.zeros((max(nrep, 1), NVAR + 1)) if history else None
+        done = C.c_int(0)
+        rc = lib().rc_run_steady(self.h, int(nsteps), int(every), dptr(buf) if history else None, C.byref(done))
+        check(rc, self.h)
+        if not history:
+            return None
+        nrep_done = (1 if nsteps >= 1 else 0) + sum(1 for s in range(2, nsteps + 1) if s % every == 0)
+        return buf[:nrep_done]
+
+    def set_timing(self, on: bool) -> None:
+        self._call("rc_set_timing", 1 if on else 0)
+
+    TIMING_KEYS = ("timestep_face", "timestep_cell", "ghost", "gradient_limiter", "flux_face",
+                   "residual_source", "lusgs_forward", "lusgs_backward", "update")
+
+    def timing(self) -> dict:
+        """Mean device ms per step of each kernel since set_timing(True)."""
+        t = self._get("rc_get_timing", (10,))
+        n = max(t[9], 1.0)
+        out = {k: t[i] / n for i, k in enumerate(self.TIMING_KEYS)}
+        out["steps"] = int(t[9])
+        return out

@@ -1,0 +1,134 @@
+"""Writes the golden vectors under tests/golden/ by running the reference
+solver, built from its own sources by oracle/ref_build.sh, in this container.
+
+Each fixture ``<name>.npz`` holds the case inputs (mesh, BCs, decks, initial
+PA) and what the reference computed on them:
+
+* set-up arrays (faces, NeighF/NeighC, ghosts, vol, cellxy, reflen, wdis, AMLS);
+* for the first ``ndump`` steps, every per-call array of ``lu_sgs_stage``:
+  step, alpaF, GPA, unlimited and limited GradC, the residual hWA after
+  flux_sourceterm, dW after LU_SGS_steady, PA and WA;
+* the step_residual norms of every step and PA/WA after ``nsteps`` steps.
+
+Run:  python tests/golden/make_golden.py   (needs oracle/_ref/rc_ref_dump and
+the product library for the multicolour ordering utility)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from rocketcombustor_amd import deck, nozzle  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "rc_ref_dump")
+REF_DECKS = "/root/reference/src/combustion"   # transport.in / diffusion.in (read by startup, unused inviscid)
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def run_reference(case: nozzle.Case, nsteps: int, ndump: int, expect_error: bool = False):
+    tmp = tempfile.mkdtemp(prefix="rcgold_")
+    try:
+        nozzle.write_reference_case(tmp, case, nstep_max=nsteps, extra_decks=REF_DECKS)
+        od = os.path.join(tmp, "out")
+        os.makedirs(od)
+        r = subprocess.run([REF_BIN, str(nsteps), str(ndump), "out", "0"], cwd=tmp, capture_output=True, text=True)
+        ne, nv = case.mesh.ne, case.mech.nvar
+        d = {}
+        # a Fortran `stop` exits with status 0: an error is a missing final dump
+        stopped = not os.path.exists(os.path.join(od, "final_PA.bin"))
+        if r.returncode != 0 or (stopped and not expect_error):
+            raise RuntimeError("reference run failed:\n" + r.stdout[-2000:] + r.stderr[-2000:])
+
+        def rd(name, shape, dt=np.float64):
+            return np.fromfile(os.path.join(od, name), dtype=dt).reshape(shape)
+
+        nf = int(np.fromfile(os.path.join(od, "Neighbor.bin"), dtype=np.int32).size // 2)
+        ng = int(np.fromfile(os.path.join(od, "vol_Ghostcell.bin")).size)
+        for n, shp, dt in [("Neighbor", (nf, 2), np.int32), ("FNode", (nf, 2), np.int32), ("VECX", (nf, 2), np.float64),
+                           ("NeighF", (ne, 3), np.int32), ("NeighC", (ne, 4), np.int32), ("Neigh", (ne,), np.int32),
+                           ("Num_ghostcell", (ne,), np.int32), ("Ghostcell", (ne, 3), np.int32),
+                           ("Hostcell", (ng,), np.int32), ("BCtype_Ghostcell", (ng,), np.int32),
+                           ("vol", (ne,), np.float64), ("cellxy", (ne, 2), np.float64), ("reflen", (ne, 2), np.float64),
+                           ("wdis", (ne, 3), np.float64), ("AMLS", (ne, 3, 2), np.float64),
+                           ("vol_Ghostcell", (ng,), np.float64), ("CellXY_Ghostcell", (ng, 2), np.float64)]:
+            d["ref_" + n] = rd(n + ".bin", shp, dt)
+        tv = rd("total_volume.bin", (2,))
+        d["ref_total_volume"] = tv[0]
+        d["ref_p_out"] = tv[1]
+        d["ref_init_WA"] = rd("init_WA.bin", (ne, nv))
+        for s in range(1, ndump + 1):
+            for n, shp in [("step", (ne,)), ("alpaF", (nf,)), ("GPA", (ng, nv)), ("gradC0", (ne, nv, 2)),
+                           ("gradC", (ne, nv, 2)), ("res", (ne, nv)), ("dW", (ne, nv)), ("PA", (ne, nv)),
+                           ("WA", (ne, nv))]:
+                d["ref_s%d_%s" % (s, n)] = rd("s%d_%s.bin" % (s, n), shp)
+        resid = np.fromfile(os.path.join(od, "resid.bin")).reshape(-1, nv + 1)
+        d["ref_resid"] = resid
+        if stopped:
+            d["ref_error_step"] = len(resid) + 1
+            d["ref_error_message"] = np.array(r.stdout.strip().splitlines()[-2:])
+        else:
+            d["ref_final_PA"] = rd("final_PA.bin", (ne, nv))
+            d["ref_final_WA"] = rd("final_WA.bin", (ne, nv))
+        return d
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def case_arrays(case: nozzle.Case) -> dict:
+    m = case.mech
+    return {
+        "coor": case.mesh.coor, "cells": case.mesh.cells, "bedges": case.mesh.bedges, "btype": case.mesh.btype,
+        "nface": case.mesh.nface, "pa0": case.pa0,
+        "mech_mw": m.mole_weight, "mech_rconstant": m.rconstant,
+        "mech_trange": np.array([m.t_low, m.t_crit, m.t_high]),
+        "mech_low": m.low_coef, "mech_high": m.high_coef, "mech_low_b1": m.low_b1, "mech_high_b1": m.high_b1,
+        "mech_model": m.model, "mech_coef_f": m.coef_f, "mech_coef_r": m.coef_r,
+        "mech_arrh": np.array([m.a_f, m.beta, m.e_act]),
+        "ox_y": case.oxidizer.y, "ox_tpm": np.array([case.oxidizer.temp, case.oxidizer.pres, case.oxidizer.mass_flow]),
+        "fu_y": case.fuel.y, "fu_tpm": np.array([case.fuel.temp, case.fuel.pres, case.fuel.mass_flow]),
+        "info": np.array([case.info.cfl, case.info.limit_id, float(case.info.combustion), case.info.p_out_ratio]),
+    }
+
+
+def make(name: str, case: nozzle.Case, nsteps: int, ndump: int, every: int = 1, expect_error: bool = False) -> None:
+    d = run_reference(case, nsteps, ndump, expect_error)
+    if every > 1:
+        d["ref_resid"] = d["ref_resid"][::every]
+    d["resid_every"] = every
+    d["nsteps"] = nsteps
+    d.update(case_arrays(case))
+    path = os.path.join(OUT, name + ".npz")
+    np.savez_compressed(path, **d)
+    print("%-28s NE=%6d steps=%6d  %7.1f kB" % (name, case.mesh.ne, nsteps, os.path.getsize(path) / 1024), flush=True)
+
+
+def main() -> None:
+    from rocketcombustor_amd import native
+
+    base = nozzle.nozzle_case(24, 6)
+    make("n24x6_natural", base, nsteps=40, ndump=3)
+    perm, _ = native.order_multicolor(base.mesh.cells)
+    make("n24x6_multicolor", base.permuted(perm), nsteps=40, ndump=2)
+    make("n24x6_shipped_thermo", nozzle.nozzle_case(24, 6, mech=deck.default_mechanism(1, False)), nsteps=10, ndump=2)
+    make("n24x6_model0", nozzle.nozzle_case(24, 6, mech=deck.default_mechanism(0, True)), nsteps=1, ndump=1)
+    make("n24x6_cold", nozzle.nozzle_case(24, 6, combustion=False), nsteps=20, ndump=2)
+    # without the limiter T goes negative and the reference stops at step 5
+    make("n24x6_nolimiter_stop", nozzle.nozzle_case(24, 6, limit_id=0), nsteps=20, ndump=2, expect_error=True)
+    make("n24x6_bc_variants", nozzle.nozzle_case(24, 6, wall_type=1, fuel_faces=2, subsonic_exit=True,
+                                                 p_out_ratio=0.9), nsteps=20, ndump=2)
+    conv = nozzle.nozzle_case(30, 8)
+    make("n30x8_converged", conv, nsteps=8000, ndump=0, every=10)
+    perm, _ = native.order_multicolor(conv.mesh.cells)
+    make("n30x8_multicolor_converged", conv.permuted(perm), nsteps=8000, ndump=0, every=10)
+
+
+if __name__ == "__main__":
+    main()

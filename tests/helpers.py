@@ -1,0 +1,57 @@
+"""Shared test helpers: golden fixtures -> cases, normwise comparisons."""
+from __future__ import annotations
+
+import glob
+import os
+
+import numpy as np
+
+from rocketcombustor_amd import deck, nozzle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def fixture_names():
+    return sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def load(name: str) -> dict:
+    with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def case_from_fixture(d: dict) -> nozzle.Case:
+    tl, tc, th = d["mech_trange"]
+    a_f, beta, e_act = d["mech_arrh"]
+    nsp = len(d["mech_mw"])
+    mech = deck.Mechanism(names=["S%d" % k for k in range(nsp)], mole_weight=d["mech_mw"],
+                          rconstant=float(d["mech_rconstant"]), tref=298.15, t_low=tl, t_crit=tc, t_high=th,
+                          low_coef=d["mech_low"], low_b1=d["mech_low_b1"], high_coef=d["mech_high"],
+                          high_b1=d["mech_high_b1"], hof=np.zeros(nsp), model=int(d["mech_model"]),
+                          coef_f=d["mech_coef_f"], coef_r=d["mech_coef_r"], a_f=a_f, beta=beta, e_act=e_act)
+    ox = deck.Inlet(y=d["ox_y"], temp=d["ox_tpm"][0], pres=d["ox_tpm"][1], mass_flow=d["ox_tpm"][2])
+    fu = deck.Inlet(y=d["fu_y"], temp=d["fu_tpm"][0], pres=d["fu_tpm"][1], mass_flow=d["fu_tpm"][2])
+    cfl, limit_id, comb, por = d["info"]
+    info = deck.SolverInfo(cfl=cfl, limit_id=int(limit_id), combustion=bool(comb), p_out_ratio=por)
+    mesh = nozzle.Mesh(d["coor"], d["cells"].astype(np.int32), d["bedges"].astype(np.int32),
+                       d["btype"].astype(np.int32), int(d["nface"]))
+    return nozzle.Case(mesh=mesh, mech=mech, oxidizer=ox, fuel=fu, info=info, pa0=d["pa0"],
+                       geom=nozzle.NozzleGeometry(), p0=0.0, t0=0.0)
+
+
+def relerr(a, b, axis=None) -> float:
+    """max |a-b| / max |b|, per variable (last axis) when ``axis`` is -1,
+    with a guard so that identically-zero references compare absolutely."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if axis is None:
+        den = max(np.max(np.abs(b)), 1e-300)
+        return float(np.max(np.abs(a - b)) / den)
+    a2 = a.reshape(-1, a.shape[-1])
+    b2 = b.reshape(-1, b.shape[-1])
+    worst = 0.0
+    for k in range(a2.shape[1]):
+        den = np.max(np.abs(b2[:, k]))
+        num = np.max(np.abs(a2[:, k] - b2[:, k]))
+        worst = max(worst, num / den if den > 0 else num)
+    return float(worst)

@@ -1,0 +1,143 @@
+"""Parity of the HIP path (through the C-ABI) with the reference's golden
+vectors and with the C oracle.
+
+Bars (FP64): single evaluations of each operator <=1e-12 relative per
+variable; short trajectories <=1e-10; the 8000-step converged fields <=1e-8
+(the north-star bar for converged fields is 1e-6).  Every GPU step is
+announced on stderr first so a failing run names its case.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import case_from_fixture, fixture_names, load, relerr
+from rocketcombustor_amd import native, nozzle
+from rocketcombustor_amd.solver import SteadyCase
+
+oracle = pytest.importorskip("oracle.pyoracle")
+
+pytestmark = pytest.mark.gpu
+NAMES = fixture_names()
+
+
+def announce(msg: str) -> None:
+    sys.__stderr__.write("[gpu-test] " + msg + "\n")
+    sys.__stderr__.flush()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_per_call_vs_reference(name):
+    d = load(name)
+    case = case_from_fixture(d)
+    announce("per-call %s NE=%d" % (name, case.mesh.ne))
+    with SteadyCase(case) as g:
+        assert g.ne == case.mesh.ne and g.nf == len(d["ref_Neighbor"])
+        assert relerr(g.conserved(), d["ref_init_WA"], -1) < 1e-15
+        s = 1
+        while "ref_s%d_step" % s in d:
+            g.timestep()
+            step, alpaf = g.timestep_arrays()
+            assert relerr(step, d["ref_s%d_step" % s]) < 1e-13
+            assert relerr(alpaf, d["ref_s%d_alpaF" % s]) < 1e-13
+            g.gradient()
+            assert relerr(g.ghost(), d["ref_s%d_GPA" % s], -1) < 1e-13
+            assert relerr(g.gradient_array(), d["ref_s%d_gradC0" % s], -1) < 1e-12
+            if case.info.limit_id > 0:
+                g.limiter_barth()
+            assert relerr(g.gradient_array(), d["ref_s%d_gradC" % s], -1) < 1e-12
+            g.flux_sourceterm()
+            assert relerr(g.increment(), d["ref_s%d_res" % s], -1) < 1e-12
+            g.lu_sgs_steady()
+            assert relerr(g.increment(), d["ref_s%d_dW" % s], -1) < 1e-12
+            assert relerr(g.primitive(), d["ref_s%d_PA" % s], -1) < 1e-13
+            assert relerr(g.conserved(), d["ref_s%d_WA" % s], -1) < 1e-13
+            s += 1
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_trajectory_vs_reference(name):
+    d = load(name)
+    case = case_from_fixture(d)
+    nsteps, every = int(d["nsteps"]), int(d["resid_every"])
+    announce("trajectory %s NE=%d steps=%d" % (name, case.mesh.ne, nsteps))
+    with SteadyCase(case) as g:
+        if "ref_error_step" in d:
+            err_step = int(d["ref_error_step"])
+            with pytest.raises(native.RcError) as ei:
+                g.run_steady(nsteps, every=1)
+            assert ei.value.code == 2            # RC_ERR_TEMPERATURE, where the reference stops
+            assert "step %d)" % err_step in str(ei.value)
+            return
+        hist = g.run_steady(nsteps, every=every)
+        ref = d["ref_resid"]
+        assert hist.shape == ref.shape
+        big = np.abs(ref[:, :7]) > 1e-6 * np.abs(ref[0, :7])
+        rel = np.abs(hist[:, :7] - ref[:, :7]) / np.maximum(np.abs(ref[:, :7]), 1e-300)
+        assert np.max(rel[big]) < (1e-8 if nsteps > 100 else 1e-10)
+        tol = 1e-8 if nsteps > 100 else 1e-10
+        assert relerr(g.primitive(), d["ref_final_PA"], -1) < tol
+        assert relerr(g.conserved(), d["ref_final_WA"], -1) < tol
+
+
+def test_converged_across_orderings():
+    """The multicolour-ordered GPU solve converges to the natural-order
+    reference field (orderings differ only in the transient)."""
+    nat = load("n30x8_converged")
+    mc = load("n30x8_multicolor_converged")
+    case = case_from_fixture(mc)
+    announce("converged multicolour NE=%d" % case.mesh.ne)
+    perm, _ = native.order_multicolor(case_from_fixture(nat).mesh.cells)
+    with SteadyCase(case) as g:
+        g.run_steady(8000, every=0, history=False)
+        pa = g.primitive()
+    back = np.empty_like(pa)
+    back[perm] = pa
+    assert relerr(back, nat["ref_final_PA"], -1) < 1e-6
+
+
+@pytest.mark.parametrize("nx,ny,order", [(150, 40, "natural"), (150, 40, "multicolor"), (1000, 200, "multicolor")])
+def test_config_sizes_vs_oracle(nx, ny, order):
+    """The 12k-cell and 400k-cell nozzles: per-call arrays of one step, and
+    three whole steps, against the oracle on the same inputs."""
+    case = nozzle.nozzle_case(nx, ny)
+    if order == "multicolor":
+        perm, nc = native.order_multicolor(case.mesh.cells)
+        assert nc <= 5
+        case = case.permuted(perm)
+    o = oracle.OracleCase(case)
+    announce("config %dx%d %s NE=%d" % (nx, ny, order, case.mesh.ne))
+    with SteadyCase(case) as g:
+        if order == "multicolor":
+            assert g.nlev <= 5
+        g.timestep(); o.timestep()
+        assert relerr(g.timestep_arrays()[1], o.get("alpaF")) < 1e-13
+        g.reconstruction(); o.reconstruction()
+        assert relerr(g.gradient_array(), o.get("gradC"), -1) < 1e-12
+        g.flux_sourceterm(); o.flux_sourceterm()
+        assert relerr(g.increment(), o.get("hWA"), -1) < 1e-12
+        g.lu_sgs_steady(); o.lusgs_steady()
+        assert relerr(g.increment(), o.get("hWA"), -1) < 1e-12
+        assert relerr(g.primitive(), o.get("PA"), -1) < 1e-13
+        for _ in range(2):
+            g.lu_sgs_stage(); o.stage()
+        assert relerr(g.primitive(), o.get("PA"), -1) < 1e-12
+        assert relerr(g.residual()[:7], o.residual()[:7]) < 1e-11
+
+
+def test_deterministic_and_resident():
+    """Gather-only kernels: two runs give bitwise-identical states; state
+    stays resident between calls (get does not perturb it)."""
+    case = nozzle.nozzle_case(150, 40)
+    perm, _ = native.order_multicolor(case.mesh.cells)
+    case = case.permuted(perm)
+    announce("determinism 150x40")
+    outs = []
+    for _ in range(2):
+        with SteadyCase(case) as g:
+            g.run_steady(10, history=False)
+            _ = g.primitive()
+            g.run_steady(10, history=False)
+            outs.append((g.primitive(), g.conserved(), g.increment()))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)

@@ -1,0 +1,89 @@
+"""CPU checks of the C-ABI library: it loads, exports every function that
+include/rocketcombustor.h declares, and its host-side ordering utility
+returns a valid colour-major permutation."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from rocketcombustor_amd import native, nozzle
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "rocketcombustor.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rc_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_header():
+    lib = native.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(native.SIGNATURES), "ctypes signatures out of sync with the header"
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(native, "_LIB", None)
+    monkeypatch.setattr(native, "LIB_PATH", str(tmp_path / "absent.so"))
+    with pytest.raises(ImportError):
+        native.lib()
+
+
+def test_error_without_device_is_reported():
+    """Creating a case without a usable device returns an error code, never
+    falls back to a CPU path."""
+    case = nozzle.nozzle_case(6, 3)
+    from rocketcombustor_amd.solver import SteadyCase
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(native.RcError) as ei:
+        SteadyCase(case)
+    assert ei.value.code == 6
+
+
+@pytest.mark.parametrize("nx,ny", [(24, 6), (150, 40)])
+def test_multicolor_order(nx, ny):
+    case = nozzle.nozzle_case(nx, ny)
+    perm, nc = native.order_multicolor(case.mesh.cells)
+    ne = case.mesh.ne
+    assert sorted(perm.tolist()) == list(range(ne))
+    assert 2 <= nc <= 4
+    cells = case.mesh.cells[perm]
+    # face adjacency of the reordered mesh
+    edges = {}
+    adj = [[] for _ in range(ne)]
+    for i, c in enumerate(cells):
+        for k in range(3):
+            e = tuple(sorted((c[k], c[(k + 1) % 3])))
+            if e in edges:
+                j = edges[e]
+                adj[i].append(j)
+                adj[j].append(i)
+            else:
+                edges[e] = i
+    # colour-major: recover colours as wavefront levels; no two adjacent
+    # cells share a level, and the number of levels equals the colours
+    level = np.zeros(ne, dtype=int)
+    for i in range(ne):
+        lower = [level[j] for j in adj[i] if j < i]
+        level[i] = 1 + max(lower) if lower else 0
+    assert level.max() + 1 <= nc
+    for i in range(ne):
+        for j in adj[i]:
+            assert level[i] != level[j]
+
+
+def test_ctypes_structs_match_header_layout():
+    # rc_mechanism / rc_params sizes follow from the header's field list
+    assert ctypes.sizeof(native.RcParams) == 8 + 4 + 4 + 8 + 8 * 8 + 16 + 8 * 8 + 16
+    assert ctypes.sizeof(native.RcMechanism) % 8 == 0
