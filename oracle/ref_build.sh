@@ -50,3 +50,13 @@ $FC -O3 -o "$OUT/rc_ref_dump" ref_dump.o $OBJS -L"$CONDA/lib" -lfmpich -lmpich -
 $FC -O3 -o "$OUT/rc_ref_main" $(echo $OBJS | sed 's/fvm2d_lib.o/fvm2d.o/') -L"$CONDA/lib" -lfmpich -lmpich -lmkl_rt \
     -Wl,-rpath,"$CONDA/lib" -Wl,-z,execstack
 echo "built $OUT/rc_ref_dump $OUT/rc_ref_main"
+# the reference host driving the MI355X library (needs the built library)
+PKG=$(cd "$HERE/.." && pwd)/rocketcombustor_amd
+if [ -f "$PKG/librocketcombustor.so" ]; then
+    $FC $FFLAGS -c "$HERE/../include/rocketcombustor_iface.f90" -o rocketcombustor_iface.o
+    $FC $FFLAGS -I"$CONDA/include" -c "$HERE/ref_host_gpu.f90" -o ref_host_gpu.o
+    $FC -O3 -o "$OUT/rc_ref_host_gpu" ref_host_gpu.o rocketcombustor_iface.o $OBJS -L"$CONDA/lib" -lfmpich -lmpich \
+        -lmkl_rt -L"$PKG" -lrocketcombustor -Wl,-rpath,"$CONDA/lib" -Wl,-rpath,'$ORIGIN/../../rocketcombustor_amd' \
+        -Wl,-z,execstack
+    echo "built $OUT/rc_ref_host_gpu"
+fi

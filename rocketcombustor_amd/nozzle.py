@@ -222,6 +222,8 @@ class Case:
         mass_flow / (2*(ymax-ymin)) over the inlet faces
         (src/initialization.f90:22, :194)."""
         ins = self.mesh.bedges[self.mesh.btype == 3]
+        if len(ins) == 0:
+            return 0.0
         y = self.mesh.coor[ins.ravel(), 1]
         return self.oxidizer.mass_flow / (2.0 * (y.max() - y.min()))
 
