@@ -144,22 +144,29 @@ int rc_get_gradient(rc_case *h, double *gradc);         /* GradC(2,Nvar,NE) */
 int rc_get_timestep(rc_case *h, double *step, double *alpaf); /* step(NE), alpaF(NF); either may be NULL */
 int rc_get_ghost(rc_case *h, double *gpa);              /* GPA(Nvar,Nghost) */
 
-/* Per-kernel device time summed over the steps run while timing was on,
- * in ms: t[0] timestep faces, t[1] timestep cells, t[2] ghost update,
- * t[3] gradient+limiter, t[4] flux faces, t[5] residual assembly + source,
- * t[6] LU-SGS forward sweep (all wavefronts), t[7] backward sweep, t[8]
- * update; t[9] the number of timed steps.  Collected with HIP events on the
- * case's stream; enabling it synchronises after every step. */
+/* Per-kernel device time of the fused step summed over the steps run while
+ * timing was on, in ms: t[0] gradient + Barth limiter (ghost states formed
+ * in-kernel), t[1] flux faces (+ TimeStep spectral radius), t[2] residual
+ * assembly + source (+ local step and LU diagonal), t[3] LU-SGS forward
+ * sweep (all wavefronts), t[4] backward sweep, t[5] update; t[6] the number
+ * of timed steps.  Collected with HIP events recorded on the case's stream
+ * between the kernels, without synchronising; with timing off, rc_run_steady
+ * replays the step as one captured HIP graph instead. */
 int rc_set_timing(rc_case *h, int enable);
-int rc_get_timing(rc_case *h, double *t10);
+int rc_get_timing(rc_case *h, double *t7);
 
 /* No counterpart in the reference: a host-side utility that returns a
- * cell permutation (perm[k] = old 0-based index of new cell k) ordering the
- * cells colour by colour (greedy colouring of the face-adjacency graph in
- * reverse Cuthill-McKee order), so that the LU-SGS sweeps of the reordered
- * mesh run as ncolors wavefronts.  Writing the same order into grid1.in
- * makes the reference's serial sweeps perform the identical computation. */
-int rc_mesh_order_multicolor(int ne, const int *cellnodes, int *perm, int *ncolors);
+ * cell permutation (perm[k] = old 0-based index of new cell k) for block
+ * multicolour sweeps: cells are grouped into blocks of up to `block`
+ * face-connected cells grown in reverse Cuthill-McKee order, the block graph
+ * is coloured greedily, and cells are ordered by (block colour, block,
+ * RCM rank).  The LU-SGS sweeps of the reordered mesh then run as at most
+ * block*ncolors wavefronts while keeping Gauss-Seidel coupling inside each
+ * block (block = 1 is plain per-cell colouring, which stalls the steady
+ * nozzle; block >= 2 converges like the serial order).  Writing the same
+ * order into grid1.in makes the reference's serial sweeps perform the
+ * identical computation. */
+int rc_mesh_order_multicolor(int ne, const int *cellnodes, int block, int *perm, int *ncolors);
 
 #ifdef __cplusplus
 }

@@ -161,16 +161,16 @@ module rocketcombustor_iface
             integer(c_int), value :: enable
         end function
 
-        integer(c_int) function rc_get_timing(h, t10) bind(c, name="rc_get_timing")
+        integer(c_int) function rc_get_timing(h, t7) bind(c, name="rc_get_timing")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: h
-            real(c_double), intent(out) :: t10(10)
+            real(c_double), intent(out) :: t7(7)
         end function
 
-        integer(c_int) function rc_mesh_order_multicolor(ne, cellnodes, perm, ncolors) &
+        integer(c_int) function rc_mesh_order_multicolor(ne, cellnodes, block, perm, ncolors) &
                 bind(c, name="rc_mesh_order_multicolor")
             import :: c_int
-            integer(c_int), value :: ne
+            integer(c_int), value :: ne, block
             integer(c_int), intent(in) :: cellnodes(*)
             integer(c_int), intent(out) :: perm(*)
             integer(c_int), intent(out) :: ncolors

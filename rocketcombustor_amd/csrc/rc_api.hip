@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -15,9 +16,10 @@ namespace rc {
 void launch_ts_face(const DevMesh &, const DevState &, const DevThermo *, hipStream_t);
 void launch_ts_cell(const DevMesh &, const DevState &, const DevParams *, hipStream_t);
 void launch_ghost(const DevMesh &, const DevState &, const DevThermo *, const DevParams *, hipStream_t);
-void launch_gradlim(const DevMesh &, const DevState &, bool, bool, hipStream_t);
-void launch_flux(const DevMesh &, const DevState &, const DevThermo *, const DevParams *, hipStream_t);
-void launch_residual(const DevMesh &, const DevState &, const DevThermo *, const DevParams *, hipStream_t);
+void launch_gradlim(const DevMesh &, const DevState &, const DevThermo *, const DevParams *, bool, bool, hipStream_t);
+void launch_flux(const DevMesh &, const DevState &, const DevThermo *, const DevParams *, bool, hipStream_t);
+void launch_residual(const DevMesh &, const DevState &, const DevThermo *, const DevParams *, bool, hipStream_t);
+void launch_step_begin(const DevState &, hipStream_t);
 void launch_lusgs_level(const DevMesh &, const DevState &, const DevThermo *, bool, int, int, hipStream_t);
 void launch_update(const DevMesh &, const DevState &, const DevThermo *, hipStream_t);
 void launch_prim_to_cons(const DevMesh &, const DevState &, const DevThermo *, hipStream_t);
@@ -27,7 +29,7 @@ void launch_resnorm(const DevMesh &, const DevState &, double *, int, double, do
 using namespace rc;
 
 static thread_local std::string g_error;
-static constexpr int kTimeSlots = 9;
+static constexpr int kTimeSlots = 6;
 
 struct rc_case {
     int device = 0;
@@ -48,15 +50,16 @@ struct rc_case {
     double *h_res = nullptr;     // pinned
     int steps_done = 0;
     bool timing = false;
-    // timing slots: 0 timestep faces, 1 timestep cells, 2 ghost update,
-    // 3 gradient+limiter, 4 flux faces, 5 residual+source, 6 LU forward,
-    // 7 LU backward, 8 update
+    // timing slots: 0 gradient+limiter, 1 flux faces, 2 residual+source,
+    // 3 LU forward, 4 LU backward, 5 update
     // events are recorded without synchronising; each step uses kTimeSlots+1
     // consecutive pool entries, drained into tms[] at the next sync point
     std::vector<hipEvent_t> evpool;
     size_t ev_used = 0;
     double tms[kTimeSlots] = {};
     long timed_steps = 0;
+    bool use_graph = true;
+    hipGraphExec_t graph_exec = nullptr;
     std::string err;
 };
 
@@ -189,16 +192,19 @@ int layout_and_upload(rc_case *h) {
     h->nblk_res = std::min(1024, std::max(1, (m.ne + 255) / 256));
     auto plan = [&](Carver &c, DevMesh &M, DevState &S) {
         M.ne = m.ne; M.nf = m.nf; M.ng = m.ng; M.sc = sc; M.sf = sf; M.sg = sg;
-        S.pa = c.take<double>((size_t)kNvar * sc);
+        S.pa = c.take<double>((size_t)kRecPA * sc);
         S.wa = c.take<double>((size_t)kNvar * sc);
-        S.hwa = c.take<double>((size_t)kNvar * sc);
-        S.gradc = c.take<double>((size_t)2 * kNvar * sc);
+        S.hwa = c.take<double>((size_t)kRecHW * sc);
+        S.gradc = c.take<double>((size_t)kRecGR * sc);
         S.step = c.take<double>(sc);
         S.dd = c.take<double>(sc);
-        S.alpaf = c.take<double>(sf);
-        S.fflux = c.take<double>((size_t)kNvar * sf);
-        S.gpa = c.take<double>((size_t)kNvar * sg);
-        S.err = c.take<int>(4);
+        S.fflux = c.take<double>((size_t)kRecF * sf);
+        S.gpa = c.take<double>((size_t)kRecPA * sg);
+        S.err = c.take<int>(8);
+        S.lucoef = c.take<double>((size_t)kRecLC * sc);
+        S.lu_af = c.take<double>((size_t)3 * sc);
+        M.lu_nb = c.take<int>((size_t)3 * sc);
+        M.lu_n = c.take<double>((size_t)6 * sc);
         M.vol = c.take<double>(sc);
         M.wdis = c.take<double>((size_t)3 * sc);
         M.amls = c.take<double>((size_t)6 * sc);
@@ -249,6 +255,10 @@ int layout_and_upload(rc_case *h) {
     if ((rc = up(M.amls, am.data(), sizeof(double) * am.size()))) return rc;
     auto lv = soa(m.lvv, m.ne, 6, sc);
     if ((rc = up(M.lvv, lv.data(), sizeof(double) * lv.size()))) return rc;
+    auto lnb = soa(m.lu_nb, m.ne, 3, sc);
+    if ((rc = up(M.lu_nb, lnb.data(), sizeof(int) * lnb.size()))) return rc;
+    auto lnn = soa(m.lu_n, m.ne, 6, sc);
+    if ((rc = up(M.lu_n, lnn.data(), sizeof(double) * lnn.size()))) return rc;
     auto stn = soa(m.stencil, m.ne, 3, sc);
     if ((rc = up(M.stencil, stn.data(), sizeof(int) * stn.size()))) return rc;
     auto cf = soa(m.cface, m.ne, 3, sc);
@@ -304,56 +314,62 @@ int drain_timing(rc_case *h) {
     h->ev_used = 0;
     return RC_OK;
 }
-void enqueue_timestep(rc_case *h) {
+// one steady step as the fused kernel pipeline:
+//   step counter, gradient+limiter (ghost states formed in-kernel), flux
+//   (+ TimeStep face part), residual + source (+ TimeStep cell part),
+//   LU-SGS forward wavefronts, backward wavefronts, update.
+void enqueue_step_kernels(rc_case *h) {
+    launch_step_begin(h->S, h->stream);
     mark(h, 0);
-    launch_ts_face(h->M, h->S, h->d_th, h->stream);
+    launch_gradlim(h->M, h->S, h->d_th, h->d_par, true, h->par.limit_id > 0, h->stream);
     mark(h, 1);
-    launch_ts_cell(h->M, h->S, h->d_par, h->stream);
-}
-void enqueue_reconstruction(rc_case *h, bool grad, bool limit) {
+    launch_flux(h->M, h->S, h->d_th, h->d_par, true, h->stream);
     mark(h, 2);
-    if (grad) launch_ghost(h->M, h->S, h->d_th, h->d_par, h->stream);
+    launch_residual(h->M, h->S, h->d_th, h->d_par, true, h->stream);
     mark(h, 3);
-    launch_gradlim(h->M, h->S, grad, limit, h->stream);
-}
-void enqueue_flux(rc_case *h) {
-    mark(h, 4);
-    launch_flux(h->M, h->S, h->d_th, h->d_par, h->stream);
-    mark(h, 5);
-    launch_residual(h->M, h->S, h->d_th, h->d_par, h->stream);
-}
-void enqueue_fwd(rc_case *h) {
-    mark(h, 6);
     for (int l = 0; l < h->nlev; ++l)
         launch_lusgs_level(h->M, h->S, h->d_th, true, h->hm.lvl_ptr[l], h->hm.lvl_ptr[l + 1], h->stream);
-}
-void enqueue_bwd(rc_case *h) {
-    mark(h, 7);
+    mark(h, 4);
     for (int l = h->nlev - 1; l >= 0; --l)
         launch_lusgs_level(h->M, h->S, h->d_th, false, h->hm.lvl_ptr[l], h->hm.lvl_ptr[l + 1], h->stream);
-}
-void enqueue_update(rc_case *h) {
-    mark(h, 8);
+    mark(h, 5);
     launch_update(h->M, h->S, h->d_th, h->stream);
 }
 
-int enqueue_stage(rc_case *h) {
-    h->S.step_no = h->steps_done + 1;
-    if (h->timing && h->ev_used + kTimeSlots + 1 > 4096) {
-        int rc = drain_timing(h);
-        if (rc) return rc;
+// the step captured once into a HIP graph and replayed (no memset nodes:
+// every node is a kernel)
+int ensure_graph(rc_case *h) {
+    if (h->graph_exec) return RC_OK;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step_kernels(h);
+    hipGraph_t g = nullptr;
+    HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+    hipError_t e = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        h->graph_exec = nullptr;
+        return set_err(h, RC_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
-    enqueue_timestep(h);
-    enqueue_reconstruction(h, true, h->par.limit_id > 0);
-    enqueue_flux(h);
-    enqueue_fwd(h);
-    enqueue_bwd(h);
-    enqueue_update(h);
+    return RC_OK;
+}
+
+int enqueue_stage(rc_case *h, bool allow_graph) {
     if (h->timing) {
+        if (h->ev_used + kTimeSlots + 1 > 4096) {
+            int rc = drain_timing(h);
+            if (rc) return rc;
+        }
+        enqueue_step_kernels(h);
         int rc = ensure_events(h, h->ev_used + kTimeSlots + 1);
         if (rc) return rc;
         HIPCHK(h, hipEventRecord(h->evpool[h->ev_used + kTimeSlots], h->stream));
         h->ev_used += kTimeSlots + 1;
+    } else if (allow_graph && h->use_graph) {
+        int rc = ensure_graph(h);
+        if (rc) return rc;
+        HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+    } else {
+        enqueue_step_kernels(h);
     }
     h->steps_done++;
     return RC_OK;
@@ -365,15 +381,18 @@ int residual_async(rc_case *h, double *host_out) {
     return RC_OK;
 }
 
-template <class F>
-int get_cells(rc_case *h, double *out, const double *dev, int width, int stride, F) {
-    if (!out) return RC_OK;
+// copies n items of `width` values out of a device array laid out either as
+// one array per value (stride = array length) or as records (rec > 0 values
+// per item, the first `width` used), into out[i*width + c]
+int get_items(rc_case *h, double *out, const double *dev, int n, int width, int stride, int rec) {
+    if (!out || n == 0) return RC_OK;
+    const size_t len = rec > 0 ? (size_t)rec * n : (size_t)width * stride;
+    std::vector<double> tmp(len);
+    HIPCHK(h, hipMemcpyAsync(tmp.data(), dev, sizeof(double) * len, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    std::vector<double> tmp((size_t)width * stride);
-    HIPCHK(h, hipMemcpy(tmp.data(), dev, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
-    const int n = stride == h->M.sc ? h->ne : (stride == h->M.sf ? h->nf : h->ng);
     for (int i = 0; i < n; ++i)
-        for (int c = 0; c < width; ++c) out[(size_t)i * width + c] = tmp[(size_t)c * stride + i];
+        for (int c = 0; c < width; ++c)
+            out[(size_t)i * width + c] = rec > 0 ? tmp[(size_t)i * rec + c] : tmp[(size_t)c * stride + i];
     return RC_OK;
 }
 
@@ -424,6 +443,8 @@ int rc_case_create(const rc_mechanism *mech, const rc_params *par, int nnt, cons
         return bail(set_err(h, RC_ERR_MEMORY, "pinned host allocation failed"));
     rc = layout_and_upload(h);
     if (rc) return bail(rc);
+    const char *ng = std::getenv("RC_NO_GRAPH");
+    h->use_graph = !(ng && ng[0] == '1');
     *out = h;
     return RC_OK;
 }
@@ -434,6 +455,7 @@ void rc_case_destroy(rc_case *h) {
         (void)hipSetDevice(h->device);
         (void)hipStreamSynchronize(h->stream);
     }
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->arena) (void)hipFree(h->arena);
     for (auto &ev : h->evpool) (void)hipEventDestroy(ev);
     if (h->h_err) (void)hipHostFree(h->h_err);
@@ -451,10 +473,9 @@ int rc_case_info(const rc_case *h, long long *s) {
 int rc_set_primitive(rc_case *h, const double *pa) {
     if (!h || !pa) return set_err(h, RC_ERR_ARGUMENT, "NULL argument");
     HIPCHK(h, hipSetDevice(h->device));
-    const int sc = h->M.sc;
-    std::vector<double> tmp((size_t)kNvar * sc, 0.0);
+    std::vector<double> tmp((size_t)kRecPA * h->M.sc, 0.0);
     for (int i = 0; i < h->ne; ++i)
-        for (int k = 0; k < kNvar; ++k) tmp[(size_t)k * sc + i] = pa[(size_t)i * kNvar + k];
+        for (int k = 0; k < kNvar; ++k) tmp[(size_t)i * kRecPA + k] = pa[(size_t)i * kNvar + k];
     HIPCHK(h, hipMemcpyAsync(h->S.pa, tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice, h->stream));
     launch_prim_to_cons(h->M, h->S, h->d_th, h->stream);
     return sync_check(h);
@@ -463,51 +484,55 @@ int rc_set_primitive(rc_case *h, const double *pa) {
 int rc_timestep(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    enqueue_timestep(h);
+    launch_ts_face(h->M, h->S, h->d_th, h->stream);
+    launch_ts_cell(h->M, h->S, h->d_par, h->stream);
     return sync_check(h);
 }
 
 int rc_gradient(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    enqueue_reconstruction(h, true, false);
+    launch_gradlim(h->M, h->S, h->d_th, h->d_par, true, false, h->stream);
     return sync_check(h);
 }
 
 int rc_limiter_barth(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    enqueue_reconstruction(h, false, true);
+    launch_gradlim(h->M, h->S, h->d_th, h->d_par, false, true, h->stream);
     return sync_check(h);
 }
 
 int rc_reconstruction(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    enqueue_reconstruction(h, true, h->par.limit_id > 0);
+    launch_gradlim(h->M, h->S, h->d_th, h->d_par, true, h->par.limit_id > 0, h->stream);
     return sync_check(h);
 }
 
 int rc_flux_sourceterm(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    enqueue_flux(h);
+    launch_flux(h->M, h->S, h->d_th, h->d_par, false, h->stream);
+    launch_residual(h->M, h->S, h->d_th, h->d_par, false, h->stream);
     return sync_check(h);
 }
 
 int rc_lusgs_steady(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    enqueue_fwd(h);
-    enqueue_bwd(h);
-    enqueue_update(h);
+    for (int l = 0; l < h->nlev; ++l)
+        launch_lusgs_level(h->M, h->S, h->d_th, true, h->hm.lvl_ptr[l], h->hm.lvl_ptr[l + 1], h->stream);
+    for (int l = h->nlev - 1; l >= 0; --l)
+        launch_lusgs_level(h->M, h->S, h->d_th, false, h->hm.lvl_ptr[l], h->hm.lvl_ptr[l + 1], h->stream);
+    launch_update(h->M, h->S, h->d_th, h->stream);
     return sync_check(h);
 }
 
 int rc_lusgs_stage(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    int rc = enqueue_stage(h);
+    int rc = enqueue_stage(h, false);
     if (rc) return rc;
     return sync_check(h);
 }
@@ -518,7 +543,7 @@ int rc_run_steady(rc_case *h, int nsteps, int every, double *res_out, int *done)
     if (every <= 0) every = nsteps + 1;
     int nrep = 0;
     for (int s = 1; s <= nsteps; ++s) {
-        int rc = enqueue_stage(h);
+        int rc = enqueue_stage(h, true);
         if (rc) return rc;
         if (s == 1 || s % every == 0) {
             if (res_out) {
@@ -551,30 +576,35 @@ int rc_residual(rc_case *h, double *res) {
 
 int rc_get_primitive(rc_case *h, double *pa) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
-    return get_cells(h, pa, h->S.pa, kNvar, h->M.sc, 0);
+    return get_items(h, pa, h->S.pa, h->ne, kNvar, 0, kRecPA);
 }
 int rc_get_conserved(rc_case *h, double *wa) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
-    return get_cells(h, wa, h->S.wa, kNvar, h->M.sc, 0);
+    return get_items(h, wa, h->S.wa, h->ne, kNvar, h->M.sc, 0);
 }
 int rc_get_increment(rc_case *h, double *hwa) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
-    return get_cells(h, hwa, h->S.hwa, kNvar, h->M.sc, 0);
+    return get_items(h, hwa, h->S.hwa, h->ne, kNvar, 0, kRecHW);
 }
 int rc_get_gradient(rc_case *h, double *g) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
-    return get_cells(h, g, h->S.gradc, 2 * kNvar, h->M.sc, 0);
+    return get_items(h, g, h->S.gradc, h->ne, 2 * kNvar, 0, kRecGR);
 }
 int rc_get_timestep(rc_case *h, double *step, double *alpaf) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
-    int rc = get_cells(h, step, h->S.step, 1, h->M.sc, 0);
+    int rc = get_items(h, step, h->S.step, h->ne, 1, h->M.sc, 0);
     if (rc) return rc;
-    return get_cells(h, alpaf, h->S.alpaf, 1, h->M.sf, 0);
+    if (!alpaf) return RC_OK;
+    std::vector<double> rec((size_t)h->nf * kRecF);
+    rc = get_items(h, rec.data(), h->S.fflux, h->nf, kRecF, 0, kRecF);
+    if (rc) return rc;
+    for (int f = 0; f < h->nf; ++f) alpaf[f] = rec[(size_t)f * kRecF + kNvar];
+    return RC_OK;
 }
 int rc_get_ghost(rc_case *h, double *gpa) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     if (h->ng == 0) return RC_OK;
-    return get_cells(h, gpa, h->S.gpa, kNvar, h->M.sg, 0);
+    return get_items(h, gpa, h->S.gpa, h->ng, kNvar, 0, kRecPA);
 }
 
 int rc_set_timing(rc_case *h, int enable) {
@@ -587,21 +617,21 @@ int rc_set_timing(rc_case *h, int enable) {
     return RC_OK;
 }
 
-int rc_get_timing(rc_case *h, double *t10) {
-    if (!h || !t10) return set_err(h, RC_ERR_ARGUMENT, "NULL argument");
+int rc_get_timing(rc_case *h, double *t7) {
+    if (!h || !t7) return set_err(h, RC_ERR_ARGUMENT, "NULL argument");
     HIPCHK(h, hipStreamSynchronize(h->stream));
     int rc = drain_timing(h);
     if (rc) return rc;
-    for (int k = 0; k < kTimeSlots; ++k) t10[k] = h->tms[k];
-    t10[kTimeSlots] = (double)h->timed_steps;
+    for (int k = 0; k < kTimeSlots; ++k) t7[k] = h->tms[k];
+    t7[kTimeSlots] = (double)h->timed_steps;
     return RC_OK;
 }
 
-int rc_mesh_order_multicolor(int ne, const int *cellnodes, int *perm, int *ncolors) {
-    if (ne <= 0 || !cellnodes || !perm) return set_err(nullptr, RC_ERR_ARGUMENT, "bad argument");
+int rc_mesh_order_multicolor(int ne, const int *cellnodes, int block, int *perm, int *ncolors) {
+    if (ne <= 0 || !cellnodes || !perm || block < 1) return set_err(nullptr, RC_ERR_ARGUMENT, "bad argument");
     std::vector<int> c0(3 * (size_t)ne);
     for (size_t k = 0; k < c0.size(); ++k) c0[k] = cellnodes[k] - 1;
-    int rc = order_multicolor(ne, c0.data(), perm, ncolors);
+    int rc = order_multicolor(ne, c0.data(), block, perm, ncolors);
     if (rc) return set_err(nullptr, rc, "mesh is not a manifold triangle mesh");
     return RC_OK;
 }

@@ -32,21 +32,51 @@ struct DevMesh {
     const double *__restrict__ amls;
     const double *__restrict__ lvv;
     const double *__restrict__ gnrm;
+    // LU-SGS face slots of each cell, in NeighF order: the neighbour across
+    // the face (-1 on a boundary) and the face normal oriented out of the
+    // cell (Time_LUSGS_steady.f90:60-68): lu_nb[k*sc+i], lu_n[(k*2+d)*sc+i]
+    const int *__restrict__ lu_nb;
+    const double *__restrict__ lu_n;
 };
 
+// per-cell quantities of the cell-average state that the LU-SGS neighbour
+// term (Time_LUSGS_steady.f90:72-87), the source term and the ghost update
+// evaluate from PA alone; formed whenever PA is written
+enum LuCoef { LC_RR = 0, LC_ROU, LC_ROV, LC_ROEP, LC_U, LC_V, LC_T, LC_GAM, LC_H0, LC_RY0 = LC_H0 + kNsp,
+              LC_N = LC_RY0 + kNsp - 1 };
+
+// record widths (doubles) of the arrays that kernels gather by neighbour
+// index: one 64-byte or 128-byte record per cell or face, so a gather reads
+// one or two cache lines instead of one line per variable
+constexpr int kRecPA = 8;    // PA(7) and GPA(7)
+constexpr int kRecHW = 8;    // hWA(7)
+constexpr int kRecGR = 16;   // GradC(2,7)
+constexpr int kRecLC = 16;   // LU cache (LC_N)
+constexpr int kRecF = 8;     // face flux F(7) and alpaF
+static_assert(LC_N <= kRecLC, "LU cache record too small");
+
 struct DevState {
-    double *pa, *wa, *hwa, *gradc, *gpa, *alpaf, *step, *dd, *fflux;
-    int *err;        // [0] code, [1] index, [2] kernel, [3] step
-    int step_no;
+    double *pa;      // kRecPA per cell
+    double *wa;      // Nvar arrays of sc
+    double *hwa;     // kRecHW per cell
+    double *gradc;   // kRecGR per cell
+    double *gpa;     // kRecPA per ghost
+    double *step, *dd;
+    double *fflux;   // kRecF per face: F(1:7), alpaF
+    double *lucoef;  // kRecLC per cell
+    double *lu_af;   // alpaF of the cell's 3 face slots, 3 x sc
+    int *err;        // [0] code, [1] index, [2] kernel, [3] step of the error, [4] current step
 };
 
 enum KernelId { K_TS_FACE = 1, K_TS_CELL, K_GHOST, K_GRADLIM, K_FLUX, K_RES, K_FWD, K_BWD, K_UPDATE, K_P2C };
 
-__device__ inline void flag_error(int *err, int code, int index, int kernel, int step) {
+// first error wins; the step number comes from the device-side step
+// counter err[4] (advanced by k_step_begin), so captured graphs report it too
+__device__ inline void flag_error(int *err, int code, int index, int kernel) {
     if (atomicCAS(err, 0, code) == 0) {
         err[1] = index;
         err[2] = kernel;
-        err[3] = step;
+        err[3] = __atomic_load_n(err + 4, __ATOMIC_RELAXED);
     }
 }
 

@@ -41,6 +41,8 @@ struct HostMesh {
     std::vector<double> fvl, fvr;    // 2*nf  flux-point offsets of owner / neighbour
     std::vector<double> gnrm;        // 2*ng  unit normal used by the ghost update
     std::vector<int> cface;          // 3*ne  face f if cell is the owner, ~f otherwise
+    std::vector<int> lu_nb;          // 3*ne  neighbour across each face (-1: boundary)
+    std::vector<double> lu_n;        // 6*ne  face normal oriented out of the cell [i][k][d]
     std::vector<int> level;          // ne    LU-SGS wavefront level
     std::vector<int> sched;          // ne    cells sorted by level
     std::vector<int> lvl_ptr;        // nlev+1
@@ -51,7 +53,7 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
                const int *btype, HostMesh &m, std::string &msg);
 
 // Multicolour ordering (no reference counterpart).
-int order_multicolor(int ne, const int *cells0, int *perm, int *ncolors);
+int order_multicolor(int ne, const int *cells0, int block, int *perm, int *ncolors);
 
 // Thermo/kinetics tables on the device.
 struct DevThermo {

@@ -10,22 +10,24 @@
 
 namespace rc {
 
-#define PA_(k, i) S.pa[(k) * M.sc + (i)]
+// gathered arrays are records per cell / face (one or two cache lines per
+// gather); WA, only ever read by its own cell, stays one array per variable
+#define PA_(k, i) S.pa[(i) * kRecPA + (k)]
 #define WA_(k, i) S.wa[(k) * M.sc + (i)]
-#define HWA_(k, i) S.hwa[(k) * M.sc + (i)]
-#define GR_(k, r, i) S.gradc[((k) * 2 + (r)) * M.sc + (i)]
-#define GPA_(k, j) S.gpa[(k) * M.sg + (j)]
+#define HWA_(k, i) S.hwa[(i) * kRecHW + (k)]
+#define GR_(k, r, i) S.gradc[(i) * kRecGR + (k) * 2 + (r)]
+#define GPA_(k, j) S.gpa[(j) * kRecPA + (k)]
+#define LC_(c, i) S.lucoef[(i) * kRecLC + (c)]
+#define F_(k, f) S.fflux[(f) * kRecF + (k)]
+#define ALPHA_(f) S.fflux[(f) * kRecF + kNvar]
 
 constexpr int NV = kNvar;
 
-// TimeStep face loop (TimeStep.f90:16-92, inviscid): spectral radius per face
+// TimeStep face loop (TimeStep.f90:16-92, inviscid): spectral radius of face
+// f from the cell averages of its two cells (one cell on a boundary)
 template <int NSP>
-__global__ void __launch_bounds__(256) k_timestep_face(DevMesh M, DevState S, const DevThermo *__restrict__ thp) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= M.nf) return;
-    const DevThermo &th = *thp;
-    const int nl = M.fnl[f], nr = M.fnr[f];
-    const double dx = M.vecx[f], dy = M.vecx[M.sf + f];
+__device__ __forceinline__ double face_spectral_radius(const DevMesh &M, const DevState &S, const DevThermo &th,
+                                                       int f, int nl, int nr, double dx, double dy) {
     double pp, uu, vv, tt, y[NSP], s = 0.0;
     if (nr >= 0) {
         pp = 0.5 * (PA_(0, nl) + PA_(0, nr));
@@ -41,25 +43,32 @@ __global__ void __launch_bounds__(256) k_timestep_face(DevMesh M, DevState S, co
     }
     y[NSP - 1] = 1.0 - s;
     double cp, gam, r;
-    if (!gas_param<NSP>(th, y, tt, cp, gam, r)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_TS_FACE, S.step_no);
+    if (!gas_param<NSP>(th, y, tt, cp, gam, r)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_TS_FACE);
     const double rr = pp / (r * tt);
-    if (rr < 0. || pp < 0. || tt < 0.) flag_error(S.err, RC_ERR_NEGATIVE, f, K_TS_FACE, S.step_no);
+    if (rr < 0. || pp < 0. || tt < 0.) flag_error(S.err, RC_ERR_NEGATIVE, f, K_TS_FACE);
     const double un = uu * dx + vv * dy;
-    S.alpaf[f] = fabs(un) + sqrt(gam * pp / rr) * sqrt(dx * dx + dy * dy);
+    return fabs(un) + sqrt(gam * pp / rr) * sqrt(dx * dx + dy * dy);
+}
+
+template <int NSP>
+__global__ void __launch_bounds__(256) k_timestep_face(DevMesh M, DevState S, const DevThermo *__restrict__ thp) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= M.nf) return;
+    ALPHA_(f) = face_spectral_radius<NSP>(M, S, *thp, f, M.fnl[f], M.fnr[f], M.vecx[f], M.vecx[M.sf + f]);
 }
 
 // TimeStep cell loop (TimeStep.f90:94-96) and the LU-SGS diagonal
 // dd(i) = sum 0.5*alpaF + vol/step (Time_LUSGS_steady.f90:46-53)
-__global__ void __launch_bounds__(256) k_timestep_cell(DevMesh M, DevState S, const DevParams *__restrict__ pars) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M.ne) return;
+__device__ __forceinline__ void cell_step_dd(const DevMesh &M, const DevState &S, const DevParams *__restrict__ pars,
+                                             int i) {
     const double cfl = pars[M.cell_par ? M.cell_par[i] : 0].cfl;
     double a = 0.0, d = 0.0, al[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int cf = M.cface[k * M.sc + i];
-        al[k] = S.alpaf[cf >= 0 ? cf : ~cf];
+        al[k] = ALPHA_(cf >= 0 ? cf : ~cf);
         a = a + al[k];
+        S.lu_af[k * M.sc + i] = al[k];
     }
     const double v = M.vol[i];
     const double st = cfl * v / a;
@@ -69,23 +78,33 @@ __global__ void __launch_bounds__(256) k_timestep_cell(DevMesh M, DevState S, co
     S.dd[i] = d + v / st;
 }
 
-// update_ghostcell_variables (update_flow_variables.f90:294-473)
+__global__ void __launch_bounds__(256) k_timestep_cell(DevMesh M, DevState S, const DevParams *__restrict__ pars) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M.ne) return;
+    cell_step_dd(M, S, pars, i);
+}
+
+// advances the device step counter used in error reports
+__global__ void k_step_begin(int *err) {
+    if (threadIdx.x == 0) err[4] = err[4] + 1;
+}
+
+// update_ghostcell_variables (update_flow_variables.f90:294-473): the state
+// of ghost g mirrored from its host cell h's primitive state pl[]; the
+// host's gamma and density (ComputeGasParameter of its own state) come from
+// the cell-average cache
 template <int NSP>
-__global__ void __launch_bounds__(256) k_ghost(DevMesh M, DevState S, const DevThermo *__restrict__ thp,
-                                               const DevParams *__restrict__ pars) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= M.ng) return;
-    const DevThermo &th = *thp;
+__device__ __forceinline__ void ghost_state(const DevMesh &M, const DevState &S, const DevThermo &th,
+                                            const DevParams *__restrict__ pars, int g, int h, const double *pl_,
+                                            double *out) {
     const DevParams &P = pars[M.ghost_par ? M.ghost_par[g] : 0];
-    const int i = M.ghost_host[g];
-    const double pl = PA_(0, i), ul = PA_(1, i), vl = PA_(2, i), tl = PA_(3, i);
+    const double pl = pl_[0], ul = pl_[1], vl = pl_[2], tl = pl_[3];
     double yl[NSP], yr[NSP], s = 0.0;
 #pragma unroll
-    for (int k = 0; k < NSP - 1; ++k) { yl[k] = PA_(kNFlow + k, i); s = s + yl[k]; }
+    for (int k = 0; k < NSP - 1; ++k) { yl[k] = pl_[kNFlow + k]; s = s + yl[k]; }
     yl[NSP - 1] = 1.0 - s;
-    double cp, gam, r;
-    if (!gas_param<NSP>(th, yl, tl, cp, gam, r)) flag_error(S.err, RC_ERR_TEMPERATURE, g, K_GHOST, S.step_no);
-    const double rl = pl / (r * tl);
+    const double gam = LC_(LC_GAM, h);
+    const double rl = LC_(LC_RR, h);
     const double s1n = M.gnrm[g], s2n = M.gnrm[M.sg + g];
     const double ac = sqrt(gam * pl / rl);
     double pr, ur, vr, tr;
@@ -108,7 +127,7 @@ __global__ void __launch_bounds__(256) k_ghost(DevMesh M, DevState S, const DevT
         for (int k = 0; k < NSP; ++k) yr[k] = yin[k];
         tr = type == 3 ? P.ox_t : P.fu_t;
         double c2, g2, r2;
-        if (!gas_param<NSP>(th, yr, tr, c2, g2, r2)) flag_error(S.err, RC_ERR_TEMPERATURE, g, K_GHOST, S.step_no);
+        if (!gas_param<NSP>(th, yr, tr, c2, g2, r2)) flag_error(S.err, RC_ERR_TEMPERATURE, g, K_GHOST);
         pr = pl;
         const double rr = pr / (r2 * tr);
         ur = (type == 3 ? P.ox_g : P.fu_g) / rr;
@@ -120,17 +139,34 @@ __global__ void __launch_bounds__(256) k_ghost(DevMesh M, DevState S, const DevT
 #pragma unroll
         for (int k = 0; k < NSP; ++k) yr[k] = yl[k];
     }
-    GPA_(0, g) = pr; GPA_(1, g) = ur; GPA_(2, g) = vr; GPA_(3, g) = tr;
+    out[0] = pr; out[1] = ur; out[2] = vr; out[3] = tr;
 #pragma unroll
-    for (int k = 0; k < NSP - 1; ++k) GPA_(kNFlow + k, g) = yr[k];
+    for (int k = 0; k < NSP - 1; ++k) out[kNFlow + k] = yr[k];
+}
+
+template <int NSP>
+__global__ void __launch_bounds__(256) k_ghost(DevMesh M, DevState S, const DevThermo *__restrict__ thp,
+                                               const DevParams *__restrict__ pars) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= M.ng) return;
+    const int i = M.ghost_host[g];
+    double pl[NV], out[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) pl[k] = PA_(k, i);
+    ghost_state<NSP>(M, S, *thp, pars, g, i, pl, out);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) GPA_(k, g) = out[k];
 }
 
 // gradient (gradient.f90:4-45) and/or Limiter_Barth (limiter.f90:5-107),
 // one thread per cell.  GRAD: compute GradC = AMLS * (wdis * dPA) over the
-// 3 stencil slots (interior neighbours, then ghosts); LIMIT: scale each
-// variable's gradient by the Barth factor over the 3 face midpoints.
-template <bool GRAD, bool LIMIT>
-__global__ void __launch_bounds__(256) k_gradlim(DevMesh M, DevState S) {
+// 3 stencil slots (interior neighbours, then ghosts); the ghost states of a
+// boundary cell depend only on the cell itself, so they are formed here (and
+// stored for the limiter-only entry point).  LIMIT: scale each variable's
+// gradient by the Barth factor over the 3 face midpoints.
+template <int NSP, bool GRAD, bool LIMIT>
+__global__ void __launch_bounds__(256) k_gradlim(DevMesh M, DevState S, const DevThermo *__restrict__ thp,
+                                                 const DevParams *__restrict__ pars) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M.ne) return;
     double pi[NV], pmin[NV], pmax[NV], g0[NV], g1[NV];
@@ -142,9 +178,21 @@ __global__ void __launch_bounds__(256) k_gradlim(DevMesh M, DevState S) {
         const double w = M.wdis[s * M.sc + i];
         const double a0 = M.amls[(s * 2 + 0) * M.sc + i];
         const double a1 = M.amls[(s * 2 + 1) * M.sc + i];
+        double q[NV];
+        if (j >= 0) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) q[k] = PA_(k, j);
+        } else if (GRAD) {
+            ghost_state<NSP>(M, S, *thp, pars, -j - 1, i, pi, q);
+#pragma unroll
+            for (int k = 0; k < NV; ++k) GPA_(k, -j - 1) = q[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) q[k] = GPA_(k, -j - 1);
+        }
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
-            const double pj = (j >= 0) ? PA_(k, j) : GPA_(k, -j - 1);
+            const double pj = q[k];
             if (GRAD) {
                 const double cc = w * (pj - pi[k]);
                 g0[k] = g0[k] + a0 * cc;
@@ -182,8 +230,11 @@ __global__ void __launch_bounds__(256) k_gradlim(DevMesh M, DevState S) {
 }
 
 // flux_lax (flux_lax.f90:17-296), one thread per face: reconstructed L/R
-// states at the face point, boundary branch, Rusanov flux -> fflux(7,f)
-template <int NSP>
+// states at the face point, boundary branch, Rusanov flux -> fflux(7,f).
+// TS: also the face spectral radius of TimeStep, which depends only on the
+// cell averages this thread loads anyway (the values equal a separate
+// TimeStep pass: PA does not change between TimeStep and flux_lax).
+template <int NSP, bool TS>
 __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, const DevThermo *__restrict__ thp,
                                               const DevParams *__restrict__ pars) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -191,6 +242,7 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, const DevTh
     const DevThermo &th = *thp;
     const int nl = M.fnl[f], nr = M.fnr[f];
     const double dx = M.vecx[f], dy = M.vecx[M.sf + f];
+    if (TS) ALPHA_(f) = face_spectral_radius<NSP>(M, S, th, f, nl, nr, dx, dy);
     const double sav1 = dx, sav2 = dy;
     const double sav = fmax(sqrt(dx * dx + dy * dy), kSmall);
     const double s1n = sav1 / sav, s2n = sav2 / sav;
@@ -204,7 +256,7 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, const DevTh
     for (int k = 0; k < NSP - 1; ++k) { yl[k] = ql[kNFlow + k]; s = s + yl[k]; }
     yl[NSP - 1] = 1.0 - s;
     double cpl, gaml, rcl;
-    if (!gas_param<NSP>(th, yl, tl, cpl, gaml, rcl)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX, S.step_no);
+    if (!gas_param<NSP>(th, yl, tl, cpl, gaml, rcl)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX);
     double rl = pl / (rcl * tl);
     double pr, ur, vr, tr, rr, cpr, gamr, rcr;
     if (nr >= 0) {
@@ -217,7 +269,7 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, const DevTh
 #pragma unroll
         for (int k = 0; k < NSP - 1; ++k) { yr[k] = qr[kNFlow + k]; s = s + yr[k]; }
         yr[NSP - 1] = 1.0 - s;
-        if (!gas_param<NSP>(th, yr, tr, cpr, gamr, rcr)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX, S.step_no);
+        if (!gas_param<NSP>(th, yr, tr, cpr, gamr, rcr)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX);
         rr = pr / (rcr * tr);
     } else {
         const DevParams &P = pars[M.face_par ? M.face_par[f] : 0];
@@ -237,7 +289,7 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, const DevTh
 #pragma unroll
             for (int k = 0; k < NSP; ++k) yr[k] = yin[k];
             tr = type == 3 ? P.ox_t : P.fu_t;
-            if (!gas_param<NSP>(th, yr, tr, cpr, gamr, rcr)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX, S.step_no);
+            if (!gas_param<NSP>(th, yr, tr, cpr, gamr, rcr)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX);
             pr = pl;
             rr = pr / (rcr * tr);
             ur = (type == 3 ? P.ox_g : P.fu_g) / rr;
@@ -261,12 +313,12 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, const DevTh
             }
         }
     }
-    if (rl < 0. || pl < 0. || (nr >= 0 && (rr < 0. || pr < 0.))) flag_error(S.err, RC_ERR_NEGATIVE, f, K_FLUX, S.step_no);
+    if (rl < 0. || pl < 0. || (nr >= 0 && (rr < 0. || pr < 0.))) flag_error(S.err, RC_ERR_NEGATIVE, f, K_FLUX);
     double h[NSP], hl = 0.0, hr = 0.0;
-    if (!enthalpy<NSP>(th, tl, h)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX, S.step_no);
+    if (!enthalpy<NSP>(th, tl, h)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX);
 #pragma unroll
     for (int k = 0; k < NSP; ++k) hl = hl + yl[k] * h[k];
-    if (!enthalpy<NSP>(th, tr, h)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX, S.step_no);
+    if (!enthalpy<NSP>(th, tr, h)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX);
 #pragma unroll
     for (int k = 0; k < NSP; ++k) hr = hr + yr[k] * h[k];
     const double vaml = ul * ul + vl * vl, vamr = ur * ur + vr * vr;
@@ -277,25 +329,25 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, const DevTh
     const double uln = sav1 * ul + sav2 * vl, urn = sav1 * ur + sav2 * vr;
     const double coex = fmax(fabs(uln) + acl * sav, fabs(urn) + acr * sav);
     const double ruln = rl * uln, rurn = rr * urn;
-    double *F = S.fflux;
-    const int sf = M.sf;
-    F[0 * sf + f] = 0.5 * (ruln + rurn - coex * (rr - rl));
-    F[1 * sf + f] = 0.5 * (ruln * ul + rurn * ur + sav1 * (pl + pr) - coex * (rr * ur - rl * ul));
-    F[2 * sf + f] = 0.5 * (ruln * vl + rurn * vr + sav2 * (pl + pr) - coex * (rr * vr - rl * vl));
-    F[3 * sf + f] = 0.5 * (ruln * hl + rurn * hr - coex * (rr * er - rl * el));
+    F_(0, f) = 0.5 * (ruln + rurn - coex * (rr - rl));
+    F_(1, f) = 0.5 * (ruln * ul + rurn * ur + sav1 * (pl + pr) - coex * (rr * ur - rl * ul));
+    F_(2, f) = 0.5 * (ruln * vl + rurn * vr + sav2 * (pl + pr) - coex * (rr * vr - rl * vl));
+    F_(3, f) = 0.5 * (ruln * hl + rurn * hr - coex * (rr * er - rl * el));
 #pragma unroll
     for (int k = 0; k < NSP - 1; ++k)
-        F[(kNFlow + k) * sf + f] = 0.5 * (ruln * yl[k] + rurn * yr[k] - coex * (rr * yr[k] - rl * yl[k]));
+        F_(kNFlow + k, f) = 0.5 * (ruln * yl[k] + rurn * yr[k] - coex * (rr * yr[k] - rl * yl[k]));
 }
 
 // residual assembly of flux_sourceterm (fvm2d.f90:201-219): hWA = sum of
 // -F (owner) / +F (neighbour) over the cell's faces in face order, then the
-// chemical source vol*omega of sourceterm (sourceterm.f90:310-366)
-template <int NSP>
+// chemical source vol*omega of sourceterm (sourceterm.f90:310-366).
+// TS: also the cell part of TimeStep (step, dd) from the same faces.
+template <int NSP, bool TS>
 __global__ void __launch_bounds__(256) k_residual(DevMesh M, DevState S, const DevThermo *__restrict__ thp,
                                                   const DevParams *__restrict__ pars) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M.ne) return;
+    if (TS) cell_step_dd(M, S, pars, i);
     double h[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) h[k] = 0.0;
@@ -305,23 +357,22 @@ __global__ void __launch_bounds__(256) k_residual(DevMesh M, DevState S, const D
         const int f = cf >= 0 ? cf : ~cf;
         if (cf >= 0) {
 #pragma unroll
-            for (int k = 0; k < NV; ++k) h[k] = h[k] - S.fflux[k * M.sf + f];
+            for (int k = 0; k < NV; ++k) h[k] = h[k] - F_(k, f);
         } else {
 #pragma unroll
-            for (int k = 0; k < NV; ++k) h[k] = h[k] + S.fflux[k * M.sf + f];
+            for (int k = 0; k < NV; ++k) h[k] = h[k] + F_(k, f);
         }
     }
     const DevParams &P = pars[M.cell_par ? M.cell_par[i] : 0];
     if (P.combustion) {
         const DevThermo &th = *thp;
-        const double p = PA_(0, i), t = PA_(3, i);
+        const double t = PA_(3, i);
         double y[NSP], s = 0.0;
 #pragma unroll
         for (int k = 0; k < NSP - 1; ++k) { y[k] = PA_(kNFlow + k, i); s = s + y[k]; }
         y[NSP - 1] = 1.0 - s;
-        double cp, gam, r;
-        if (!gas_param<NSP>(th, y, t, cp, gam, r)) flag_error(S.err, RC_ERR_TEMPERATURE, i, K_RES, S.step_no);
-        const double rho = p / (t * r);
+        // rho = p/(t*R) of sourceterm.f90:334 is the cached p/(R*t)
+        const double rho = LC_(LC_RR, i);
         double w[NSP];
         production_rate<NSP>(th, t, rho, y, w);
         const double v = M.vol[i];
@@ -334,16 +385,13 @@ __global__ void __launch_bounds__(256) k_residual(DevMesh M, DevState S, const D
     for (int k = 0; k < NV; ++k) HWA_(k, i) = h[k];
 }
 
-// the neighbour term of both LU-SGS sweeps (Time_LUSGS_steady.f90:72-111):
-// the linearised Rusanov flux change of neighbour NR's increment
+// the cell-average quantities of Time_LUSGS_steady.f90:72-87 (and of the
+// source term and ghost update) from one cell's primitive state, written to
+// the per-cell cache; the arithmetic is that of the reference's neighbour
+// term, so the cached values equal recomputing them there
 template <int NSP>
-__device__ __forceinline__ bool lusgs_neighbor(const DevMesh &M, const DevState &S, const DevThermo &th, int nr,
-                                               double dx, double dy, double af, double *dflux) {
-    const double pp = PA_(0, nr), uu = PA_(1, nr), vv = PA_(2, nr), tt = PA_(3, nr);
-    double yy[NSP], s = 0.0;
-#pragma unroll
-    for (int k = 0; k < NSP - 1; ++k) { yy[k] = PA_(kNFlow + k, nr); s = s + yy[k]; }
-    yy[NSP - 1] = 1.0 - s;
+__device__ __forceinline__ bool write_lucoef(const DevMesh &M, const DevState &S, const DevThermo &th, int i,
+                                             double pp, double uu, double vv, double tt, const double *yy) {
     double cp, gama, rcp, h[NSP];
     bool ok = gas_param<NSP>(th, yy, tt, cp, gama, rcp);
     ok = enthalpy<NSP>(th, tt, h) && ok;
@@ -351,11 +399,36 @@ __device__ __forceinline__ bool lusgs_neighbor(const DevMesh &M, const DevState 
 #pragma unroll
     for (int k = 0; k < NSP; ++k) hh = hh + yy[k] * h[k];
     const double rr = pp / (rcp * tt);
-    const double rou = rr * uu, rov = rr * vv;
     const double roe = rr * (hh + 0.5 * (uu * uu + vv * vv)) - pp;
+    LC_(LC_RR, i) = rr;
+    LC_(LC_ROU, i) = rr * uu;
+    LC_(LC_ROV, i) = rr * vv;
+    LC_(LC_ROEP, i) = roe + pp;
+    LC_(LC_U, i) = uu;
+    LC_(LC_V, i) = vv;
+    LC_(LC_T, i) = tt;
+    LC_(LC_GAM, i) = gama;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) LC_(LC_H0 + k, i) = h[k];
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) LC_(LC_RY0 + k, i) = rr * yy[k];
+    return ok;
+}
+
+// the neighbour term of both LU-SGS sweeps (Time_LUSGS_steady.f90:88-111):
+// the linearised Rusanov flux change of neighbour nr's increment, with the
+// neighbour's cell-average quantities taken from the cache
+template <int NSP>
+__device__ __forceinline__ void lusgs_neighbor(const DevMesh &M, const DevState &S, const DevThermo &th, int nr,
+                                               double dx, double dy, double af, double *dflux) {
+    const double rr = LC_(LC_RR, nr), rou = LC_(LC_ROU, nr), rov = LC_(LC_ROV, nr);
+    const double roep = LC_(LC_ROEP, nr), uu = LC_(LC_U, nr), vv = LC_(LC_V, nr);
+    const double tt = LC_(LC_T, nr), gama = LC_(LC_GAM, nr);
+    double h[NSP];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) h[k] = LC_(LC_H0 + k, nr);
     const double drr = HWA_(0, nr), dru = HWA_(1, nr), drv = HWA_(2, nr), dre = HWA_(3, nr);
-    double dry[NSP];
-    s = 0.0;
+    double dry[NSP], s = 0.0;
 #pragma unroll
     for (int k = 0; k < NSP - 1; ++k) { dry[k] = HWA_(kNFlow + k, nr); s = s + dry[k]; }
     dry[NSP - 1] = drr - s;
@@ -373,36 +446,40 @@ __device__ __forceinline__ bool lusgs_neighbor(const DevMesh &M, const DevState 
     dflux[0] = dflux[0] + 0.5 * (-af * drr + dru * dx + drv * dy);
     dflux[1] = dflux[1] + 0.5 * (-af * dru + dru * un + rou * dun + dpp * dx);
     dflux[2] = dflux[2] + 0.5 * (-af * drv + drv * un + rov * dun + dpp * dy);
-    dflux[3] = dflux[3] + 0.5 * (-af * dre + dun * (roe + pp) + un * (dre + dpp));
+    dflux[3] = dflux[3] + 0.5 * (-af * dre + dun * roep + un * (dre + dpp));
 #pragma unroll
     for (int k = 0; k < NSP - 1; ++k)
-        dflux[kNFlow + k] = dflux[kNFlow + k] + 0.5 * (-af * dry[k] + dry[k] * un + (rr * yy[k]) * dun);
-    return ok;
+        dflux[kNFlow + k] = dflux[kNFlow + k] + 0.5 * (-af * dry[k] + dry[k] * un + LC_(LC_RY0 + k, nr) * dun);
 }
 
 // one wavefront of the forward (FWD) or backward sweep of LU_SGS_steady
-// (Time_LUSGS_steady.f90:56-120 / 127-190) over sched[lo, hi)
+// (Time_LUSGS_steady.f90:56-120 / 127-190) over sched[lo, hi): per cell the
+// coalesced face slots, then one gather of each neighbour below (FWD) or
+// above it in the sweep order
 template <int NSP, bool FWD>
 __global__ void __launch_bounds__(256) k_lusgs(DevMesh M, DevState S, const DevThermo *__restrict__ thp, int lo, int hi) {
     const int t = lo + blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= hi) return;
     const int i = M.sched[t];
     const DevThermo &th = *thp;
+    const int sc = M.sc;
+    int nb[3];
+    double nx[3], ny[3], af[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        nb[k] = M.lu_nb[k * sc + i];
+        nx[k] = M.lu_n[(k * 2) * sc + i];
+        ny[k] = M.lu_n[(k * 2 + 1) * sc + i];
+        af[k] = S.lu_af[k * sc + i];
+    }
     double dflux[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) dflux[k] = 0.0;
-    bool ok = true;
 #pragma unroll
-    for (int kf = 0; kf < 3; ++kf) {
-        const int cf = M.cface[kf * M.sc + i];
-        const int f = cf >= 0 ? cf : ~cf;
-        double dx = M.vecx[f], dy = M.vecx[M.sf + f];
-        int nr = M.fnr[f];
-        if (nr == i) { nr = M.fnl[f]; dx = -dx; dy = -dy; }
-        if (nr >= 0 && (FWD ? nr < i : nr > i))
-            ok = lusgs_neighbor<NSP>(M, S, th, nr, dx, dy, S.alpaf[f], dflux) && ok;
+    for (int k = 0; k < 3; ++k) {
+        const int nr = nb[k];
+        if (nr >= 0 && (FWD ? nr < i : nr > i)) lusgs_neighbor<NSP>(M, S, th, nr, nx[k], ny[k], af[k], dflux);
     }
-    if (!ok) flag_error(S.err, RC_ERR_TEMPERATURE, i, FWD ? K_FWD : K_BWD, S.step_no);
     const double d = S.dd[i];
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -412,7 +489,8 @@ __global__ void __launch_bounds__(256) k_lusgs(DevMesh M, DevState S, const DevT
 }
 
 // WA += dW (Time_LUSGS_steady.f90:193-197) and update_primitve_variables
-// (update_flow_variables.f90:208-252), plus the NaN check of lu_sgs_stage
+// (update_flow_variables.f90:208-252), plus the NaN check of lu_sgs_stage;
+// the new cell-average quantities go to the cache
 template <int NSP>
 __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, const DevThermo *__restrict__ thp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -429,18 +507,21 @@ __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, const Dev
     for (int k = 0; k < NSP - 1; ++k) s = s + y[k];
     y[NSP - 1] = 1.0 - s;
     double t, cp, gam, r;
-    if (!newton_temperature<NSP>(th, w[3] / rho - 0.5 * (u * u + v * v), y, PA_(3, i), t, cp, gam, r))
-        flag_error(S.err, RC_ERR_TEMPERATURE, i, K_UPDATE, S.step_no);
-    PA_(0, i) = rho * r * t;
+    bool ok = newton_temperature<NSP>(th, w[3] / rho - 0.5 * (u * u + v * v), y, PA_(3, i), t, cp, gam, r);
+    const double p = rho * r * t;
+    PA_(0, i) = p;
     PA_(1, i) = u;
     PA_(2, i) = v;
     PA_(3, i) = t;
 #pragma unroll
     for (int k = 0; k < NSP - 1; ++k) PA_(kNFlow + k, i) = y[k];
-    if (i == 0 && isnan(w[0])) flag_error(S.err, RC_ERR_NAN, 0, K_UPDATE, S.step_no);
+    ok = write_lucoef<NSP>(M, S, th, i, p, u, v, t, y) && ok;
+    if (!ok) flag_error(S.err, RC_ERR_TEMPERATURE, i, K_UPDATE);
+    if (i == 0 && isnan(w[0])) flag_error(S.err, RC_ERR_NAN, 0, K_UPDATE);
 }
 
-// primitve_to_conserved_variables (update_flow_variables.f90:254-288)
+// primitve_to_conserved_variables (update_flow_variables.f90:254-288), and
+// the cell-average cache of the new state
 template <int NSP>
 __global__ void __launch_bounds__(256) k_prim_to_cons(DevMesh M, DevState S, const DevThermo *__restrict__ thp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -455,7 +536,8 @@ __global__ void __launch_bounds__(256) k_prim_to_cons(DevMesh M, DevState S, con
     bool ok = gas_param<NSP>(th, yy, tt, cp, gam, r);
     const double rr = pp / (r * tt);
     ok = enthalpy<NSP>(th, tt, h) && ok;
-    if (!ok) flag_error(S.err, RC_ERR_TEMPERATURE, i, K_P2C, S.step_no);
+    ok = write_lucoef<NSP>(M, S, th, i, pp, uu, vv, tt, yy) && ok;
+    if (!ok) flag_error(S.err, RC_ERR_TEMPERATURE, i, K_P2C);
     double hh = 0.0;
 #pragma unroll
     for (int k = 0; k < NSP; ++k) hh = hh + yy[k] * h[k];
@@ -524,18 +606,25 @@ void launch_ghost(const DevMesh &M, const DevState &S, const DevThermo *th, cons
     if (M.ng) k_ghost<kNsp><<<grid_for(M.ng), 256, 0, st>>>(M, S, th, par);
 }
 
-void launch_gradlim(const DevMesh &M, const DevState &S, bool grad, bool limit, hipStream_t st) {
-    if (grad && limit) k_gradlim<true, true><<<grid_for(M.ne), 256, 0, st>>>(M, S);
-    else if (grad) k_gradlim<true, false><<<grid_for(M.ne), 256, 0, st>>>(M, S);
-    else if (limit) k_gradlim<false, true><<<grid_for(M.ne), 256, 0, st>>>(M, S);
+void launch_gradlim(const DevMesh &M, const DevState &S, const DevThermo *th, const DevParams *par, bool grad,
+                    bool limit, hipStream_t st) {
+    if (grad && limit) k_gradlim<kNsp, true, true><<<grid_for(M.ne), 256, 0, st>>>(M, S, th, par);
+    else if (grad) k_gradlim<kNsp, true, false><<<grid_for(M.ne), 256, 0, st>>>(M, S, th, par);
+    else if (limit) k_gradlim<kNsp, false, true><<<grid_for(M.ne), 256, 0, st>>>(M, S, th, par);
 }
 
-void launch_flux(const DevMesh &M, const DevState &S, const DevThermo *th, const DevParams *par, hipStream_t st) {
-    k_flux<kNsp><<<grid_for(M.nf), 256, 0, st>>>(M, S, th, par);
+void launch_step_begin(const DevState &S, hipStream_t st) { k_step_begin<<<1, 64, 0, st>>>(S.err); }
+
+void launch_flux(const DevMesh &M, const DevState &S, const DevThermo *th, const DevParams *par, bool ts,
+                 hipStream_t st) {
+    if (ts) k_flux<kNsp, true><<<grid_for(M.nf), 256, 0, st>>>(M, S, th, par);
+    else k_flux<kNsp, false><<<grid_for(M.nf), 256, 0, st>>>(M, S, th, par);
 }
 
-void launch_residual(const DevMesh &M, const DevState &S, const DevThermo *th, const DevParams *par, hipStream_t st) {
-    k_residual<kNsp><<<grid_for(M.ne), 256, 0, st>>>(M, S, th, par);
+void launch_residual(const DevMesh &M, const DevState &S, const DevThermo *th, const DevParams *par, bool ts,
+                     hipStream_t st) {
+    if (ts) k_residual<kNsp, true><<<grid_for(M.ne), 256, 0, st>>>(M, S, th, par);
+    else k_residual<kNsp, false><<<grid_for(M.ne), 256, 0, st>>>(M, S, th, par);
 }
 
 void launch_lusgs_level(const DevMesh &M, const DevState &S, const DevThermo *th, bool fwd, int lo, int hi, hipStream_t st) {

@@ -214,6 +214,8 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
     m.neigh.assign(ne, 0);
     m.neighc.assign(3 * (size_t)ne, -1);
     m.cface.assign(3 * (size_t)ne, 0);
+    m.lu_nb.assign(3 * (size_t)ne, -1);
+    m.lu_n.assign(6 * (size_t)ne, 0.0);
     for (int i = 0; i < ne; ++i) {
         if (cnt[i] != 3) return fail(msg, "cell %d has %d faces", i + 1, cnt[i]);
         for (int k = 0; k < 3; ++k) {
@@ -224,6 +226,14 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
                 if (ic != i && ic >= 0) m.neighc[3 * i + m.neigh[i]++] = ic;
             }
             m.cface[3 * i + k] = (m.fnl[f] == i) ? f : ~f;
+            // LU-SGS face slot (Time_LUSGS_steady.f90:60-68): the other cell
+            // and the normal VECX, negated when the cell is not the owner
+            int nr = m.fnr[f];
+            double dx = m.vecx[2 * f], dy = m.vecx[2 * f + 1];
+            if (nr == i) { nr = m.fnl[f]; dx = -dx; dy = -dy; }
+            m.lu_nb[3 * i + k] = nr >= 0 ? nr : -1;
+            m.lu_n[6 * (size_t)i + 2 * k] = dx;
+            m.lu_n[6 * (size_t)i + 2 * k + 1] = dy;
         }
     }
 
@@ -387,8 +397,8 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
     return RC_OK;
 }
 
-int order_multicolor(int ne, const int *cells0, int *perm, int *ncolors) {
-    if (ne <= 0) return RC_ERR_ARGUMENT;
+int order_multicolor(int ne, const int *cells0, int block, int *perm, int *ncolors) {
+    if (ne <= 0 || block < 1) return RC_ERR_ARGUMENT;
     // face adjacency
     std::unordered_map<uint64_t, int> first;
     first.reserve((size_t)ne * 2);
@@ -442,24 +452,58 @@ int order_multicolor(int ne, const int *cells0, int *perm, int *ncolors) {
         }
         order.insert(order.end(), q.rbegin(), q.rend());
     }
-    // greedy colouring in RCM order
-    std::vector<int> color(ne, -1), rank(ne);
+    // blocks of up to `block` face-connected cells, grown in RCM order (a
+    // block takes its seed's unassigned neighbours, lowest RCM rank first),
+    // then a greedy colouring of the block adjacency graph.  Cells of one
+    // block stay consecutive, in RCM order, so the sweeps keep Gauss-Seidel
+    // coupling inside every block; blocks of one colour share no face.
+    std::vector<int> rank(ne);
     for (int k = 0; k < ne; ++k) rank[order[k]] = k;
-    int nc = 0;
+    std::vector<int> blk(ne, -1), bseed;
     for (int k = 0; k < ne; ++k) {
         const int u = order[k];
-        unsigned used = 0;
-        for (int t = 0; t < deg[u]; ++t)
-            if (color[adj[3 * u + t]] >= 0) used |= 1u << color[adj[3 * u + t]];
+        if (blk[u] >= 0) continue;
+        const int b = (int)bseed.size();
+        bseed.push_back(k);
+        blk[u] = b;
+        std::vector<int> members{u};
+        for (size_t h = 0; h < members.size() && (int)members.size() < block; ++h) {
+            const int w = members[h];
+            int nb[3], nn = 0;
+            for (int t = 0; t < deg[w]; ++t)
+                if (blk[adj[3 * w + t]] < 0) nb[nn++] = adj[3 * w + t];
+            std::sort(nb, nb + nn, [&](int x, int y) { return rank[x] < rank[y]; });
+            for (int t = 0; t < nn && (int)members.size() < block; ++t) {
+                blk[nb[t]] = b;
+                members.push_back(nb[t]);
+            }
+        }
+    }
+    const int nblk = (int)bseed.size();
+    std::vector<std::vector<int>> badj(nblk);
+    for (int u = 0; u < ne; ++u)
+        for (int t = 0; t < deg[u]; ++t) {
+            const int v = adj[3 * u + t];
+            if (blk[u] != blk[v]) badj[blk[u]].push_back(blk[v]);
+        }
+    std::vector<int> bcol(nblk, -1);
+    int nc = 0;
+    for (int b = 0; b < nblk; ++b) {
+        uint64_t used = 0;
+        for (int v : badj[b])
+            if (bcol[v] >= 0 && bcol[v] < 64) used |= 1ull << bcol[v];
         int c = 0;
-        while (used & (1u << c)) ++c;
-        color[u] = c;
+        while (c < 64 && (used & (1ull << c))) ++c;
+        bcol[b] = c;
         nc = std::max(nc, c + 1);
     }
     std::vector<int> idx(ne);
     std::iota(idx.begin(), idx.end(), 0);
     std::sort(idx.begin(), idx.end(), [&](int a, int b) {
-        return color[a] != color[b] ? color[a] < color[b] : rank[a] < rank[b];
+        const int ba = blk[a], bb = blk[b];
+        if (bcol[ba] != bcol[bb]) return bcol[ba] < bcol[bb];
+        if (ba != bb) return bseed[ba] < bseed[bb];
+        return rank[a] < rank[b];
     });
     for (int k = 0; k < ne; ++k) perm[k] = idx[k];
     if (ncolors) *ncolors = nc;

@@ -71,7 +71,7 @@ SIGNATURES = {
     "rc_get_ghost": (C.c_int, [_VP, _DP]),
     "rc_set_timing": (C.c_int, [_VP, C.c_int]),
     "rc_get_timing": (C.c_int, [_VP, _DP]),
-    "rc_mesh_order_multicolor": (C.c_int, [C.c_int, _IP, _IP, _IP]),
+    "rc_mesh_order_multicolor": (C.c_int, [C.c_int, _IP, C.c_int, _IP, _IP]),
 }
 
 
@@ -140,12 +140,12 @@ def make_params(cfl, limit_id, combustion, p_out, ox, ox_mass_flux, fu, fu_mass_
     return p
 
 
-def order_multicolor(cells: np.ndarray):
-    """Colour-major cell permutation (perm[k] = old cell of new cell k) and
-    the number of colours; host-side, no device needed."""
+def order_multicolor(cells: np.ndarray, block: int = 4):
+    """Block-multicolour cell permutation (perm[k] = old cell of new cell k)
+    and the number of colours; host-side, no device needed."""
     cells1 = np.ascontiguousarray(cells, dtype=np.int32) + 1
     ne = len(cells1)
     perm = np.empty(ne, dtype=np.int32)
     nc = C.c_int(0)
-    check(lib().rc_mesh_order_multicolor(ne, iptr(cells1), iptr(perm), C.byref(nc)))
+    check(lib().rc_mesh_order_multicolor(ne, iptr(cells1), int(block), iptr(perm), C.byref(nc)))
     return perm, nc.value

