@@ -44,7 +44,7 @@ enum {
 
 /* Species, thermo and kinetics tables: module species (species.f90:4-12),
  * thermo_property (thermo_property.f90:4-16), chemical_kinetics
- * (chemical_kinetics.f90:186-195), filled by setup_species_info,
+ * (chemical_kinetics.f90:4-13), filled by setup_species_info,
  * setup_thermo_property and setup_chemical_kinetics. */
 typedef struct rc_mechanism {
     int nsp;                            /* species count (Nvar = 4 + nsp - 1); the kernels are built for nsp = 4 */
@@ -66,7 +66,7 @@ typedef struct rc_mechanism {
 typedef struct rc_params {
     double cfl;
     int limit_id;                       /* >0: Barth limiter after the gradient */
-    int combustion;                     /* 0: no source term (sourceterm.f90:308) */
+    int combustion;                     /* 0: no source term (sourceterm.f90:9) */
     double p_out;
     double ox_y[RC_MAXSP], ox_t, ox_mass_flux;
     double fu_y[RC_MAXSP], fu_t, fu_mass_flux;
@@ -77,7 +77,7 @@ typedef struct rc_case rc_case;
 /* Replaces ReadMesh (mesh.f90:2-115: faces by `change`, BC tags,
  * VolumeAndCenter) and compute_reconstruction_matrix
  * (reconstruction_initialization.f90:1-169: NeighF/NeighC, ghost cells of
- * ghostcell.f90:203-379, inverse-distance weights, pinv(A) of la_math.f90:1-62)
+ * ghostcell.f90:19-195, inverse-distance weights, pinv(A) of la_math.f90:1-62)
  * for a one-rank triangle mesh.
  *   coor[2*nnt]      node coordinates Coor(2,NNT)
  *   cellnodes[3*ne]  1-based counter-clockwise triangle nodes (N(1:3,NE))
@@ -98,15 +98,15 @@ const char *rc_last_error(const rc_case *h);           /* thread-local message w
  * wavefront levels, resident device bytes. */
 int rc_case_info(const rc_case *h, long long *sizes6);
 
-/* Replaces the PA load of Read_Backup (output.f90:314-327) followed by
- * primitve_to_conserved_variables (update_flow_variables.f90:254-288). */
+/* Replaces the PA load of Read_Backup (output.f90:276-338) followed by
+ * primitve_to_conserved_variables (update_flow_variables.f90:50-84). */
 int rc_set_primitive(rc_case *h, const double *pa);
 
 /* Replaces TimeStep, steady inviscid branch (TimeStep.f90:4-103): alpaF,
  * local step(i) = CFL*vol/sum(alpaF). */
 int rc_timestep(rc_case *h);
 /* Replaces gradient (gradient.f90:4-45) including
- * update_ghostcell_variables (update_flow_variables.f90:294-473). */
+ * update_ghostcell_variables (update_flow_variables.f90:90-269). */
 int rc_gradient(rc_case *h);
 /* Replaces Limiter_Barth (limiter.f90:5-107), applied to the current GradC. */
 int rc_limiter_barth(rc_case *h);
@@ -117,7 +117,7 @@ int rc_reconstruction(rc_case *h);
  * (flux_lax.f90:4-299), sourceterm (sourceterm.f90:2-70). */
 int rc_flux_sourceterm(rc_case *h);
 /* Replaces LU_SGS_steady (Time_LUSGS_steady.f90:32-204) including
- * update_primitve_variables (update_flow_variables.f90:208-252). */
+ * update_primitve_variables (update_flow_variables.f90:4-48). */
 int rc_lusgs_steady(rc_case *h);
 /* Replaces lu_sgs_stage (Time_LUSGS_steady.f90:4-30): the four calls above
  * and the NaN check on WA(1,1). */

@@ -156,7 +156,7 @@ int rco_newton(const rco_mech *m, double et, const double *y, double t0, double 
     return 0;
 }
 
-/* compute_production_rate, chemical_kinetics.f90:291-347 */
+/* compute_production_rate, chemical_kinetics.f90:109-165 */
 void rco_production_rate(const rco_mech *m, double t, double rho, const double *y2, double *w)
 {
     double y[RCO_MAXSP], kf;
@@ -375,7 +375,7 @@ int rco_create(const rco_mech *mech, const rco_params *par, int nnt, const doubl
         NEIGHC(0, i) = i;
     }
 
-    /* construct_ghostcells, ghostcell.f90:203-379 */
+    /* construct_ghostcells, ghostcell.f90:19-195 */
     int ng = 0;
     for (i = 1; i <= ne; i++) ng += 3 - c->neigh[i - 1];
     c->ng = ng;
@@ -541,7 +541,7 @@ int *rco_iarray(rco_case *c, const char *n)
 /* state                                                                    */
 /* ------------------------------------------------------------------------ */
 
-/* primitve_to_conserved_variables, update_flow_variables.f90:254-288 */
+/* primitve_to_conserved_variables, update_flow_variables.f90:50-84 */
 static int prim_to_cons(rco_case *c)
 {
     int i, k, nsp = c->nsp;
@@ -606,7 +606,7 @@ int rco_timestep(rco_case *c)
 static void inlet_state(const rco_case *c, int which, double pl, double *pr, double *ur, double *vr,
                         double *tr, double *rr, double *yr, double *g, int *bad)
 {
-    /* oxidizer_inlet_bc oxidizer.f90:130-145 / fuel_inlet_bc fuel.f90:200-215 */
+    /* oxidizer_inlet_bc oxidizer.f90:53-68 / fuel_inlet_bc fuel.f90:54-69 */
     const double *yin = which == 3 ? c->p.ox_y : c->p.fu_y;
     int k;
     for (k = 0; k < c->nsp; k++) yr[k] = yin[k];
@@ -618,7 +618,7 @@ static void inlet_state(const rco_case *c, int which, double pl, double *pr, dou
     *vr = 0.0;
 }
 
-/* update_ghostcell_variables, update_flow_variables.f90:294-473 */
+/* update_ghostcell_variables, update_flow_variables.f90:90-269 */
 static int ghost_update(rco_case *c)
 {
     int j, k, nsp = c->nsp;
@@ -889,7 +889,7 @@ static int lusgs_neighbor(rco_case *c, int NR, double dx, double dy, double af, 
     return 0;
 }
 
-/* update_primitve_variables, update_flow_variables.f90:208-252 */
+/* update_primitve_variables, update_flow_variables.f90:4-48 */
 static int update_primitive(rco_case *c)
 {
     int i, k, nsp = c->nsp, ns1 = nsp - 1;

@@ -176,7 +176,7 @@ __device__ __forceinline__ bool newton_temperature(const DevThermo &th, double e
     return ok;
 }
 
-// compute_production_rate, chemical_kinetics.f90:291-347 (one reaction;
+// compute_production_rate, chemical_kinetics.f90:109-165 (one reaction;
 // model 0: A exp(-E/RT) [H2]^0.2 [O2]^1.3, model 1: 1e4 [H2] above 710 K)
 template <int NSP>
 __device__ __forceinline__ void production_rate(const DevThermo &th, double t, double rho, const double *y2, double *w) {

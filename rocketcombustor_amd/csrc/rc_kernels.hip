@@ -89,7 +89,7 @@ __global__ void k_step_begin(int *err) {
     if (threadIdx.x == 0) err[4] = err[4] + 1;
 }
 
-// update_ghostcell_variables (update_flow_variables.f90:294-473): the state
+// update_ghostcell_variables (update_flow_variables.f90:90-269): the state
 // of ghost g mirrored from its host cell h's primitive state pl[]; the
 // host's gamma and density (ComputeGasParameter of its own state) come from
 // the cell-average cache
@@ -121,7 +121,7 @@ __device__ __forceinline__ void ghost_state(const DevMesh &M, const DevState &S,
 #pragma unroll
         for (int k = 0; k < NSP; ++k) yr[k] = yl[k];
     } else if (type == 3 || type == 4) {
-        // oxidizer_inlet_bc / fuel_inlet_bc (oxidizer.f90:130-145, fuel.f90:200-215)
+        // oxidizer_inlet_bc / fuel_inlet_bc (oxidizer.f90:53-68, fuel.f90:54-69)
         const double *yin = type == 3 ? P.ox_y : P.fu_y;
 #pragma unroll
         for (int k = 0; k < NSP; ++k) yr[k] = yin[k];
@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, const DevTh
 
 // residual assembly of flux_sourceterm (fvm2d.f90:201-219): hWA = sum of
 // -F (owner) / +F (neighbour) over the cell's faces in face order, then the
-// chemical source vol*omega of sourceterm (sourceterm.f90:310-366).
+// chemical source vol*omega of sourceterm (sourceterm.f90:2-70).
 // TS: also the cell part of TimeStep (step, dd) from the same faces.
 template <int NSP, bool TS>
 __global__ void __launch_bounds__(256) k_residual(DevMesh M, DevState S, const DevThermo *__restrict__ thp,
@@ -371,7 +371,7 @@ __global__ void __launch_bounds__(256) k_residual(DevMesh M, DevState S, const D
 #pragma unroll
         for (int k = 0; k < NSP - 1; ++k) { y[k] = PA_(kNFlow + k, i); s = s + y[k]; }
         y[NSP - 1] = 1.0 - s;
-        // rho = p/(t*R) of sourceterm.f90:334 is the cached p/(R*t)
+        // rho = p/(t*R) of sourceterm.f90:35 is the cached p/(R*t)
         const double rho = LC_(LC_RR, i);
         double w[NSP];
         production_rate<NSP>(th, t, rho, y, w);
@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(256) k_lusgs(DevMesh M, DevState S, const DevT
 }
 
 // WA += dW (Time_LUSGS_steady.f90:193-197) and update_primitve_variables
-// (update_flow_variables.f90:208-252), plus the NaN check of lu_sgs_stage;
+// (update_flow_variables.f90:4-48), plus the NaN check of lu_sgs_stage;
 // the new cell-average quantities go to the cache
 template <int NSP>
 __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, const DevThermo *__restrict__ thp) {
@@ -520,7 +520,7 @@ __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, const Dev
     if (i == 0 && isnan(w[0])) flag_error(S.err, RC_ERR_NAN, 0, K_UPDATE);
 }
 
-// primitve_to_conserved_variables (update_flow_variables.f90:254-288), and
+// primitve_to_conserved_variables (update_flow_variables.f90:50-84), and
 // the cell-average cache of the new state
 template <int NSP>
 __global__ void __launch_bounds__(256) k_prim_to_cons(DevMesh M, DevState S, const DevThermo *__restrict__ thp) {

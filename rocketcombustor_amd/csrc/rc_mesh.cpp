@@ -7,7 +7,7 @@
 //   boundary tags Neighbor(2,f) = -type                src/mesh.f90:76-93
 //   13-point triangle rule for vol / centroid          src/gaussInteg4.f90:4-50, mesh.f90:206-255
 //   NeighF ascending, NeighC, Neigh                    src/reconstruction_initialization.f90:11-50
-//   mirrored ghost cells                               src/ghostcell.f90:203-379
+//   mirrored ghost cells                               src/ghostcell.f90:19-195
 //   wdis = 1/|dc|, A = wdis*(<x_j>-<x_i>)/reflen       src/reconstruction_initialization.f90:93-150
 //   pinv(A) (dgesvd in the reference, la_math.f90:1-62; here a thin QR)
 #include <algorithm>
@@ -237,7 +237,7 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
         }
     }
 
-    // ---- ghost cells (ghostcell.f90:203-379)
+    // ---- ghost cells (ghostcell.f90:19-195)
     int ng = 0;
     for (int i = 0; i < ne; ++i) ng += 3 - m.neigh[i];
     m.ng = ng;
@@ -287,7 +287,7 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
                 m.ghost_xy[2 * j + 1] = quad_rule(fy, g, g + 2, g + 4, g + 6) / gv;
                 if (gv < 0.0) return fail(msg, "negative ghost volume %d", j + 1);
                 // unit normal as update_ghostcell_variables forms it from the
-                // first two ghost nodes (update_flow_variables.f90:318-332)
+                // first two ghost nodes (update_flow_variables.f90:114-121)
                 const double gs1 = g[3] - g[1];
                 const double gs2 = -(g[2] - g[0]);
                 const double gsav = std::max(std::sqrt(gs1 * gs1 + gs2 * gs2), kSmall);
@@ -355,7 +355,7 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
             const int f = m.neighf[3 * i + kf];
             const int n1 = m.fnode[2 * f], n2 = m.fnode[2 * f + 1];
             for (int d = 0; d < 2; ++d) {
-                const double xy = 0.5 * (X[2 * n1 + d] + X[2 * n2 + d]);       // limiter.f90:228
+                const double xy = 0.5 * (X[2 * n1 + d] + X[2 * n2 + d]);       // limiter.f90:63
                 m.lvv[6 * (size_t)i + 2 * kf + d] = (xy - m.cellxy[2 * i + d]) / m.reflen[2 * i + d];
             }
         }

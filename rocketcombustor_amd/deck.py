@@ -5,9 +5,9 @@ directory (one comment line before each value group):
 
 * ``combustion/species.in``            -- ``setup_species_info``     (src/combustion_modeling/species.f90:14-51)
 * ``combustion/thermo.in``             -- ``setup_thermo_property``  (src/combustion_modeling/thermo_property.f90:18-59)
-* ``combustion/chemical_kinetics.in``  -- ``setup_chemical_kinetics`` (src/combustion_modeling/chemical_kinetics.f90:198-239)
-* ``combustion/oxidizer.in``           -- ``setup_oxidizer_info``    (src/combustion_modeling/oxidizer.f90:90-118)
-* ``combustion/fuel.in``               -- ``setup_fuel_info``        (src/combustion_modeling/fuel.f90:159-187)
+* ``combustion/chemical_kinetics.in``  -- ``setup_chemical_kinetics`` (src/combustion_modeling/chemical_kinetics.f90:16-57)
+* ``combustion/oxidizer.in``           -- ``setup_oxidizer_info``    (src/combustion_modeling/oxidizer.f90:13-41)
+* ``combustion/fuel.in``               -- ``setup_fuel_info``        (src/combustion_modeling/fuel.f90:13-41)
 * ``flow/flow_solver_info.in``         -- ``startup``                (src/startup.f90:55-128)
 
 This module reads and writes those layouts so that the same deck drives the

@@ -87,7 +87,7 @@ def test_converged(name):
     orders on the GPU, to the field the reference reaches with the same order
     (<=1e-8).  The two orders reach two different discrete steady states
     (species fields about 3 % apart): the threshold kinetics (rate switched
-    on above 710 K, chemical_kinetics.f90:325) and the Barth limiter admit
+    on above 710 K, chemical_kinetics.f90:143) and the Barth limiter admit
     more than one, so a converged field is compared within one order."""
     d = load(name)
     case = case_from_fixture(d)
