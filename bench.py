@@ -4,13 +4,15 @@
 
 A step is one ``lu_sgs_stage`` (timestep, LSQ gradient + Barth limiter, Lax
 flux + chemical source, LU-SGS forward/backward sweeps, primitive update) of
-the 400k-triangle nozzle (1000x200 mapped grid, seeded jitter, multicolour
+the 400k-triangle nozzle (1000x200 mapped grid, seeded jitter, block-multicolour
 cell order), all state resident in device memory.  Prints one JSON line.
 
 roofline: the kernel with the largest share of the step, its algorithmic
 bytes per launch (each operand element touched once) over its mean launch
-duration, measured with HIP events on the case's stream during the timed
-steps, against the 8 TB/s HBM peak.
+duration, measured with HIP events on the case's stream over a replay of the
+timed steps, against the 8 TB/s HBM peak; traffic is that kernel's measured
+HBM bytes per launch (rocprofv3 FETCH_SIZE and WRITE_SIZE passes, summarised
+in profiles/steady_step_400k_hbm_traffic.md).
 
 cpu_baseline: the reference solver (built from its sources into
 oracle/_ref by oracle/ref_build.sh) timed on a bounded sample of the same
@@ -55,6 +57,23 @@ def kernel_bytes(ne: int, nf: int, ng: int) -> dict:
     }
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "steady_step_400k_hbm_traffic.md")
+
+
+def measured_traffic() -> dict:
+    """HBM bytes per launch of each kernel of the 1000x200 workload, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in the profiles
+    folder (each counter scaled by what it reads for a plain copy)."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return json.loads(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return {}
+
+
 def cpu_baseline(case, nsteps: int) -> dict:
     ref = os.path.join(ROOT, "oracle", "_ref", "rc_ref_dump")
     ne = case.mesh.ne
@@ -84,6 +103,30 @@ def cpu_baseline(case, nsteps: int) -> dict:
             "sample": "%d steps of the same %d-cell case, serial C restatement" % (nsteps, ne)}
 
 
+def copy_calibration(mib: int) -> int:
+    """Three device-to-device copies of `mib` MiB through the HIP runtime (its
+    copy kernel), as the reference point for the HBM byte counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMemset failed")
+        for _ in range(3):
+            if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) != 0:    # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return n
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +136,9 @@ def main() -> int:
     ap.add_argument("--ny", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device-to-device three times (a plain copy kernel of "
+                         "known size, to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one case on one GPU; --gpus %d is not supported "
@@ -129,8 +175,11 @@ def main() -> int:
     ms_step = 1000.0 * wall / args.steps
     kb = kernel_bytes(ne, nf, ng)
     kernels = {k: times[k] for k in g.TIMING_KEYS}
+    launches = {k: (g.nlev if k.startswith("lusgs") else 1) for k in kernels}
     dom = max(kernels, key=kernels.get)
-    ach = kb[dom] / (kernels[dom] * 1e-3) / 1e9
+    # per launch: algorithmic bytes / mean launch duration
+    ach = (kb[dom] / launches[dom]) / (kernels[dom] * 1e-3 / launches[dom]) / 1e9
+    traffic = measured_traffic().get(dom) if (args.nx, args.ny) == (1000, 200) else None
     b_cu = 1256.0 + 160.0 * nf / ne          # whole-step algorithmic bytes per cell-update
     out = {
         "metric": METRIC,
@@ -150,7 +199,8 @@ def main() -> int:
                                (ne, args.nx, args.ny, ncolors),
                    "cells": ne, "faces": nf, "ghosts": ng, "lusgs_levels": g.nlev},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None},
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": kb[dom] / launches[dom], "launches_per_step": launches[dom]},
         "kernel_timing": "HIP events between kernels, eager replay of the same %d steps (%.4f ms/step)"
                          % (args.steps, 1000.0 * wall_eager / args.steps),
         "kernel_ms_per_step": {k: round(v, 5) for k, v in kernels.items()},
@@ -158,6 +208,8 @@ def main() -> int:
         "step_hbm_frac_bcu": b_cu * value / (HBM_PEAK_GBS * 1e9),
     }
     g.close()
+    if args.copy_calibration > 0:
+        out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
     print(json.dumps(out), flush=True)

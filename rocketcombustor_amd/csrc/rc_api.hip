@@ -211,7 +211,7 @@ int layout_and_upload(rc_case *h) {
         M.lvv = c.take<double>((size_t)6 * sc);
         M.stencil = c.take<int>((size_t)3 * sc);
         M.cface = c.take<int>((size_t)3 * sc);
-        M.sched = c.take<int>(sc);
+        M.runs = c.take<int>(m.runs.size());
         M.vecx = c.take<double>((size_t)2 * sf);
         M.fvl = c.take<double>((size_t)2 * sf);
         M.fvr = c.take<double>((size_t)2 * sf);
@@ -263,7 +263,7 @@ int layout_and_upload(rc_case *h) {
     if ((rc = up(M.stencil, stn.data(), sizeof(int) * stn.size()))) return rc;
     auto cf = soa(m.cface, m.ne, 3, sc);
     if ((rc = up(M.cface, cf.data(), sizeof(int) * cf.size()))) return rc;
-    if ((rc = up(M.sched, m.sched.data(), sizeof(int) * m.ne))) return rc;
+    if ((rc = up(M.runs, m.runs.data(), sizeof(int) * m.runs.size()))) return rc;
     auto vx = soa(m.vecx, m.nf, 2, sf);
     if ((rc = up(M.vecx, vx.data(), sizeof(double) * vx.size()))) return rc;
     auto fl = soa(m.fvl, m.nf, 2, sf);

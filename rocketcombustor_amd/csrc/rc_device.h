@@ -23,7 +23,7 @@ struct DevMesh {
     const int *__restrict__ ghost_par;   // parameter-set index per ghost (batches)
     const int *__restrict__ face_par;    // parameter-set index per face (batches)
     const int *__restrict__ cell_par;    // parameter-set index per cell (batches)
-    const int *__restrict__ sched;
+    const int *__restrict__ runs;        // LU-SGS runs (first cell, count), by level
     const double *__restrict__ vecx;
     const double *__restrict__ fvl;
     const double *__restrict__ fvr;

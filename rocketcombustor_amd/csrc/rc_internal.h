@@ -14,6 +14,7 @@ constexpr int kNFlow = 4;          // src/module.f90:8
 constexpr int kNsp = 4;            // species count the kernels are built for
 constexpr int kNvar = kNFlow + kNsp - 1;
 constexpr double kSmall = 1.0e-16; // src/module.f90:19
+constexpr int kRunMax = 4;         // cells one thread sweeps in order (LU-SGS runs)
 
 // Host-side mesh set-up: topology and geometry in the reference's numbering
 // (0-based here), plus the derived per-face / per-cell operands the kernels read.
@@ -43,9 +44,10 @@ struct HostMesh {
     std::vector<int> cface;          // 3*ne  face f if cell is the owner, ~f otherwise
     std::vector<int> lu_nb;          // 3*ne  neighbour across each face (-1: boundary)
     std::vector<double> lu_n;        // 6*ne  face normal oriented out of the cell [i][k][d]
-    std::vector<int> level;          // ne    LU-SGS wavefront level
-    std::vector<int> sched;          // ne    cells sorted by level
-    std::vector<int> lvl_ptr;        // nlev+1
+    // LU-SGS schedule: the cell order is cut into runs of up to kRunMax
+    // consecutive, face-connected cells; one thread sweeps a run in order.
+    std::vector<int> runs;           // 2*nruns (first cell, cell count), sorted by level
+    std::vector<int> lvl_ptr;        // nlev+1  runs of level l: runs[lvl_ptr[l] .. lvl_ptr[l+1])
 };
 
 // Builds HostMesh from 0-based inputs; returns RC_OK or RC_ERR_MESH with msg.

@@ -415,54 +415,75 @@ __device__ __forceinline__ bool write_lucoef(const DevMesh &M, const DevState &S
     return ok;
 }
 
-// the neighbour term of both LU-SGS sweeps (Time_LUSGS_steady.f90:88-111):
-// the linearised Rusanov flux change of neighbour nr's increment, with the
-// neighbour's cell-average quantities taken from the cache
-template <int NSP>
-__device__ __forceinline__ void lusgs_neighbor(const DevMesh &M, const DevState &S, const DevThermo &th, int nr,
-                                               double dx, double dy, double af, double *dflux) {
-    const double rr = LC_(LC_RR, nr), rou = LC_(LC_ROU, nr), rov = LC_(LC_ROV, nr);
-    const double roep = LC_(LC_ROEP, nr), uu = LC_(LC_U, nr), vv = LC_(LC_V, nr);
-    const double tt = LC_(LC_T, nr), gama = LC_(LC_GAM, nr);
-    double h[NSP];
+// cell-average quantities of one neighbour read by the LU-SGS term
+struct LuNb {
+    double rr, rou, rov, roep, uu, vv, tt, gama, h[kNsp], ry[kNsp - 1];
+};
+
+__device__ __forceinline__ LuNb load_lunb(const DevState &S, int nr) {
+    LuNb n;
+    n.rr = LC_(LC_RR, nr); n.rou = LC_(LC_ROU, nr); n.rov = LC_(LC_ROV, nr); n.roep = LC_(LC_ROEP, nr);
+    n.uu = LC_(LC_U, nr); n.vv = LC_(LC_V, nr); n.tt = LC_(LC_T, nr); n.gama = LC_(LC_GAM, nr);
 #pragma unroll
-    for (int k = 0; k < NSP; ++k) h[k] = LC_(LC_H0 + k, nr);
-    const double drr = HWA_(0, nr), dru = HWA_(1, nr), drv = HWA_(2, nr), dre = HWA_(3, nr);
-    double dry[NSP], s = 0.0;
+    for (int k = 0; k < kNsp; ++k) n.h[k] = LC_(LC_H0 + k, nr);
 #pragma unroll
-    for (int k = 0; k < NSP - 1; ++k) { dry[k] = HWA_(kNFlow + k, nr); s = s + dry[k]; }
-    dry[NSP - 1] = drr - s;
-    const double duu = (dru - uu * drr) / rr;
-    const double dvv = (drv - vv * drr) / rr;
-    double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) s1 = s1 + h[k] * dry[k];
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) s2 = s2 + th.rsp[k] * dry[k];
-    const double dpp = (gama - 1) * (dre - s1 + tt * s2 * gama / (gama - 1)
-                                     - 0.5 * (dru * uu + duu * rou + drv * vv + dvv * rov));
-    const double un = uu * dx + vv * dy;
-    const double dun = duu * dx + dvv * dy;
-    dflux[0] = dflux[0] + 0.5 * (-af * drr + dru * dx + drv * dy);
-    dflux[1] = dflux[1] + 0.5 * (-af * dru + dru * un + rou * dun + dpp * dx);
-    dflux[2] = dflux[2] + 0.5 * (-af * drv + drv * un + rov * dun + dpp * dy);
-    dflux[3] = dflux[3] + 0.5 * (-af * dre + dun * roep + un * (dre + dpp));
-#pragma unroll
-    for (int k = 0; k < NSP - 1; ++k)
-        dflux[kNFlow + k] = dflux[kNFlow + k] + 0.5 * (-af * dry[k] + dry[k] * un + LC_(LC_RY0 + k, nr) * dun);
+    for (int k = 0; k < kNsp - 1; ++k) n.ry[k] = LC_(LC_RY0 + k, nr);
+    return n;
 }
 
-// one wavefront of the forward (FWD) or backward sweep of LU_SGS_steady
-// (Time_LUSGS_steady.f90:56-120 / 127-190) over sched[lo, hi): per cell the
-// coalesced face slots, then one gather of each neighbour below (FWD) or
-// above it in the sweep order
+// the neighbour term of both LU-SGS sweeps (Time_LUSGS_steady.f90:88-111):
+// the linearised Rusanov flux change 0.5*(-alpaF*dW + dF(dW).S) of a
+// neighbour's increment dw; the caller adds it to dflux in face order
+template <int NSP>
+__device__ __forceinline__ void lusgs_term(const DevThermo &th, const LuNb &n, const double *dw, double dx,
+                                           double dy, double af, double *c) {
+    const double drr = dw[0], dru = dw[1], drv = dw[2], dre = dw[3];
+    double dry[NSP], s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) { dry[k] = dw[kNFlow + k]; s = s + dry[k]; }
+    dry[NSP - 1] = drr - s;
+    const double duu = (dru - n.uu * drr) / n.rr;
+    const double dvv = (drv - n.vv * drr) / n.rr;
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) s1 = s1 + n.h[k] * dry[k];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) s2 = s2 + th.rsp[k] * dry[k];
+    const double dpp = (n.gama - 1) * (dre - s1 + n.tt * s2 * n.gama / (n.gama - 1)
+                                       - 0.5 * (dru * n.uu + duu * n.rou + drv * n.vv + dvv * n.rov));
+    const double un = n.uu * dx + n.vv * dy;
+    const double dun = duu * dx + dvv * dy;
+    c[0] = 0.5 * (-af * drr + dru * dx + drv * dy);
+    c[1] = 0.5 * (-af * dru + dru * un + n.rou * dun + dpp * dx);
+    c[2] = 0.5 * (-af * drv + drv * un + n.rov * dun + dpp * dy);
+    c[3] = 0.5 * (-af * dre + dun * n.roep + un * (dre + dpp));
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) c[kNFlow + k] = 0.5 * (-af * dry[k] + dry[k] * un + n.ry[k] * dun);
+}
+
+// one level of the forward (FWD) or backward sweep of LU_SGS_steady
+// (Time_LUSGS_steady.f90:56-120 / 127-190) over runs[lo, hi).  A run is up
+// to kRunMax consecutive cells swept in order; it is handled by a group of
+// kRunMax lanes, lane p owning cell first+p.  Each lane loads the records
+// of its cell's neighbours at once and forms the terms of neighbours outside
+// the run (lower levels, already final).  Then the run's cells are finished
+// one step at a time in the sweep direction: the lane of the step adds the
+// terms of neighbours inside the run, whose new increments the group has
+// received by shuffles in the steps before, and forms its increment.  Terms
+// enter dflux in face order; a face that does not take part adds +0.0, which
+// leaves the reference's sums unchanged.
 template <int NSP, bool FWD>
 __global__ void __launch_bounds__(256) k_lusgs(DevMesh M, DevState S, const DevThermo *__restrict__ thp, int lo, int hi) {
-    const int t = lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= hi) return;
-    const int i = M.sched[t];
+    constexpr int R = kRunMax;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x % R;
+    const int t = lo + gid / R;
     const DevThermo &th = *thp;
     const int sc = M.sc;
+    int first = 0, cnt = 0;
+    if (t < hi) { first = M.runs[2 * t]; cnt = M.runs[2 * t + 1]; }
+    const bool live = lane < cnt;
+    const int i = live ? first + lane : 0;
     int nb[3];
     double nx[3], ny[3], af[3];
 #pragma unroll
@@ -472,19 +493,71 @@ __global__ void __launch_bounds__(256) k_lusgs(DevMesh M, DevState S, const DevT
         ny[k] = M.lu_n[(k * 2 + 1) * sc + i];
         af[k] = S.lu_af[k * sc + i];
     }
-    double dflux[NV];
+    const double d = S.dd[i];
+    double rhs[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) dflux[k] = 0.0;
+    for (int v = 0; v < NV; ++v) rhs[v] = HWA_(v, i);
+    bool use[3];
+    int qs[3];         // sweep step of a neighbour inside the run, else -1
+    LuNb nbr[3];
+    double c[3][NV];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int nr = nb[k];
-        if (nr >= 0 && (FWD ? nr < i : nr > i)) lusgs_neighbor<NSP>(M, S, th, nr, nx[k], ny[k], af[k], dflux);
-    }
-    const double d = S.dd[i];
+        use[k] = live && nr >= 0 && (FWD ? nr < i : nr > i);
+        const bool inrun = use[k] && nr >= first && nr < first + cnt;
+        qs[k] = inrun ? (FWD ? nr - first : first + cnt - 1 - nr) : -1;
+        // faces that do not take part read cell 0's cache record, and so do
+        // increments that come from the run: one shared line, not one per
+        // lane, and never a record this launch writes
+        const int j = use[k] ? nr : 0;
+        const double *hw = (use[k] && !inrun) ? &HWA_(0, nr) : &LC_(0, 0);
+        nbr[k] = load_lunb(S, j);
+        double dw[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        if (FWD) HWA_(k, i) = (HWA_(k, i) - dflux[k]) / d;
-        else HWA_(k, i) = HWA_(k, i) - dflux[k] / d;
+        for (int v = 0; v < NV; ++v) dw[v] = hw[v];
+        lusgs_term<NSP>(th, nbr[k], dw, nx[k], ny[k], af[k], c[k]);
+    }
+    const int mystep = FWD ? lane : cnt - 1 - lane;
+    double hv[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) hv[v] = 0.0;
+    double hs[R][NV];  // increments of the run's cells, by sweep step
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        if (mystep == s) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (qs[k] >= 0) {
+                    double dw[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) dw[v] = 0.0;
+#pragma unroll
+                    for (int q = 0; q < s; ++q)
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) dw[v] = qs[k] == q ? hs[q][v] : dw[v];
+                    lusgs_term<NSP>(th, nbr[k], dw, nx[k], ny[k], af[k], c[k]);
+                }
+            }
+            double dflux[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) dflux[v] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) dflux[v] = dflux[v] + (use[k] ? c[k][v] : 0.0);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) hv[v] = FWD ? (rhs[v] - dflux[v]) / d : rhs[v] - dflux[v] / d;
+        }
+        if (s < R - 1) {
+            const int src = FWD ? s : (cnt - 1 - s > 0 ? cnt - 1 - s : 0);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) hs[s][v] = __shfl(hv[v], src, R);
+        }
+    }
+    if (live) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) HWA_(v, i) = hv[v];
     }
 }
 
@@ -629,8 +702,9 @@ void launch_residual(const DevMesh &M, const DevState &S, const DevThermo *th, c
 
 void launch_lusgs_level(const DevMesh &M, const DevState &S, const DevThermo *th, bool fwd, int lo, int hi, hipStream_t st) {
     if (hi <= lo) return;
-    if (fwd) k_lusgs<kNsp, true><<<grid_for(hi - lo), 256, 0, st>>>(M, S, th, lo, hi);
-    else k_lusgs<kNsp, false><<<grid_for(hi - lo), 256, 0, st>>>(M, S, th, lo, hi);
+    const dim3 g = grid_for((hi - lo) * kRunMax);
+    if (fwd) k_lusgs<kNsp, true><<<g, 256, 0, st>>>(M, S, th, lo, hi);
+    else k_lusgs<kNsp, false><<<g, 256, 0, st>>>(M, S, th, lo, hi);
 }
 
 void launch_update(const DevMesh &M, const DevState &S, const DevThermo *th, hipStream_t st) {

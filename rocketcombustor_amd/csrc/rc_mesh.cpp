@@ -372,28 +372,47 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
         }
     }
 
-    // ---- LU-SGS wavefront levels of the given order: level(i) = 1 + max
-    // level(j) over neighbours j < i.  Cells of one level are independent in
-    // the forward sweep (neighbours below them are in lower levels) and in
-    // the backward sweep (neighbours above are in higher levels), so a
+    // ---- LU-SGS schedule of the given order.  The order is cut into runs:
+    // a cell joins the run of the cell before it when it shares a face with a
+    // cell of that run and the run holds fewer than kRunMax cells.  A run's
+    // level is 1 + the highest level of the runs holding its cells' lower
+    // neighbours outside it (upper neighbours for the backward sweep are in
+    // higher levels by the same rule).  One thread sweeps one run cell by cell
+    // in the given order, and the runs of one level are independent, so a
     // level-by-level sweep is the reference's serial sweep.
-    m.level.assign(ne, 0);
+    std::vector<int> run_of(ne, -1), run_lev, run_first;
     int nlev = 0;
     for (int i = 0; i < ne; ++i) {
-        int l = 0;
+        int r = i > 0 ? run_of[i - 1] : -1;
+        bool join = false;
+        if (r >= 0 && i - run_first[r] < kRunMax)
+            for (int k = 0; k < m.neigh[i]; ++k) {
+                const int jc = m.neighc[3 * i + k];
+                if (jc < i && jc >= run_first[r]) join = true;
+            }
+        if (!join) {
+            r = (int)run_first.size();
+            run_first.push_back(i);
+            run_lev.push_back(0);
+        }
+        run_of[i] = r;
         for (int k = 0; k < m.neigh[i]; ++k) {
             const int jc = m.neighc[3 * i + k];
-            if (jc < i) l = std::max(l, m.level[jc] + 1);
+            if (jc < run_first[r]) run_lev[r] = std::max(run_lev[r], run_lev[run_of[jc]] + 1);
         }
-        m.level[i] = l;
-        nlev = std::max(nlev, l + 1);
+        nlev = std::max(nlev, run_lev[r] + 1);
     }
+    const int nruns = (int)run_first.size();
     m.lvl_ptr.assign(nlev + 1, 0);
-    for (int i = 0; i < ne; ++i) m.lvl_ptr[m.level[i] + 1]++;
+    for (int r = 0; r < nruns; ++r) m.lvl_ptr[run_lev[r] + 1]++;
     for (int l = 0; l < nlev; ++l) m.lvl_ptr[l + 1] += m.lvl_ptr[l];
-    m.sched.assign(ne, 0);
+    m.runs.assign(2 * (size_t)nruns, 0);
     std::vector<int> pos(m.lvl_ptr.begin(), m.lvl_ptr.end() - 1);
-    for (int i = 0; i < ne; ++i) m.sched[pos[m.level[i]]++] = i;
+    for (int r = 0; r < nruns; ++r) {
+        const int t = pos[run_lev[r]]++;
+        m.runs[2 * t] = run_first[r];
+        m.runs[2 * t + 1] = (r + 1 < nruns ? run_first[r + 1] : ne) - run_first[r];
+    }
     return RC_OK;
 }
 
@@ -455,11 +474,13 @@ int order_multicolor(int ne, const int *cells0, int block, int *perm, int *ncolo
     // blocks of up to `block` face-connected cells, grown in RCM order (a
     // block takes its seed's unassigned neighbours, lowest RCM rank first),
     // then a greedy colouring of the block adjacency graph.  Cells of one
-    // block stay consecutive, in RCM order, so the sweeps keep Gauss-Seidel
-    // coupling inside every block; blocks of one colour share no face.
+    // block stay consecutive, in the order they joined the block (each shares
+    // a face with a cell before it), so the sweeps keep Gauss-Seidel coupling
+    // inside every block and a block is one sweep run; blocks of one colour
+    // share no face.
     std::vector<int> rank(ne);
     for (int k = 0; k < ne; ++k) rank[order[k]] = k;
-    std::vector<int> blk(ne, -1), bseed;
+    std::vector<int> blk(ne, -1), bseed, gpos(ne, 0);
     for (int k = 0; k < ne; ++k) {
         const int u = order[k];
         if (blk[u] >= 0) continue;
@@ -475,6 +496,7 @@ int order_multicolor(int ne, const int *cells0, int block, int *perm, int *ncolo
             std::sort(nb, nb + nn, [&](int x, int y) { return rank[x] < rank[y]; });
             for (int t = 0; t < nn && (int)members.size() < block; ++t) {
                 blk[nb[t]] = b;
+                gpos[nb[t]] = (int)members.size();
                 members.push_back(nb[t]);
             }
         }
@@ -503,7 +525,7 @@ int order_multicolor(int ne, const int *cells0, int block, int *perm, int *ncolo
         const int ba = blk[a], bb = blk[b];
         if (bcol[ba] != bcol[bb]) return bcol[ba] < bcol[bb];
         if (ba != bb) return bseed[ba] < bseed[bb];
-        return rank[a] < rank[b];
+        return gpos[a] < gpos[b];
     });
     for (int k = 0; k < ne; ++k) perm[k] = idx[k];
     if (ncolors) *ncolors = nc;

@@ -98,7 +98,12 @@ def case_arrays(case: nozzle.Case) -> dict:
     }
 
 
+ONLY = set(sys.argv[1:])     # fixture names to (re)generate; all when empty
+
+
 def make(name: str, case: nozzle.Case, nsteps: int, ndump: int, every: int = 1, expect_error: bool = False) -> None:
+    if ONLY and name not in ONLY:
+        return
     d = run_reference(case, nsteps, ndump, expect_error)
     if every > 1:
         d["ref_resid"] = d["ref_resid"][::every]

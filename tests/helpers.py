@@ -55,3 +55,40 @@ def relerr(a, b, axis=None) -> float:
         num = np.max(np.abs(a2[:, k] - b2[:, k]))
         worst = max(worst, num / den if den > 0 else num)
     return float(worst)
+
+
+def face_adjacency(cells) -> list:
+    """Face neighbours of each triangle (cells: (ne, 3) node ids)."""
+    edges, adj = {}, [[] for _ in range(len(cells))]
+    for i, c in enumerate(cells):
+        for k in range(3):
+            e = (min(c[k], c[(k + 1) % 3]), max(c[k], c[(k + 1) % 3]))
+            if e in edges:
+                j = edges.pop(e)
+                adj[i].append(j)
+                adj[j].append(i)
+            else:
+                edges[e] = i
+    return adj
+
+
+def sweep_run_levels(adj, run_max: int = 4):
+    """The LU-SGS schedule the library builds, restated: the cell order is
+    cut into runs (a cell joins the run of the cell before it when it shares
+    a face with a cell of that run and the run has fewer than run_max cells);
+    a run's level is 1 + the highest level of the runs holding its cells'
+    lower neighbours outside it.  Returns (run first cells, run levels)."""
+    ne = len(adj)
+    run_of = np.empty(ne, dtype=int)
+    first, lev = [], []
+    for i in range(ne):
+        r = run_of[i - 1] if i else -1
+        if not (r >= 0 and i - first[r] < run_max and any(first[r] <= j < i for j in adj[i])):
+            first.append(i)
+            lev.append(0)
+            r = len(first) - 1
+        run_of[i] = r
+        for j in adj[i]:
+            if j < first[r]:
+                lev[r] = max(lev[r], lev[run_of[j]] + 1)
+    return np.array(first), np.array(lev)

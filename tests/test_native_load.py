@@ -10,6 +10,8 @@ import pytest
 
 from rocketcombustor_amd import native, nozzle
 
+from helpers import face_adjacency, sweep_run_levels
+
 HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "rocketcombustor.h")
 
 
@@ -59,21 +61,10 @@ def test_multicolor_order(nx, ny, block):
     assert sorted(perm.tolist()) == list(range(ne))
     assert 2 <= nc <= 6
     cells = case.mesh.cells[perm]
-    # face adjacency of the reordered mesh
-    edges = {}
-    adj = [[] for _ in range(ne)]
-    for i, c in enumerate(cells):
-        for k in range(3):
-            e = tuple(sorted((c[k], c[(k + 1) % 3])))
-            if e in edges:
-                j = edges[e]
-                adj[i].append(j)
-                adj[j].append(i)
-            else:
-                edges[e] = i
-    # wavefront levels of the new order (what the library schedules): no two
-    # adjacent cells share a level, and a colour of blocks of up to `block`
-    # cells needs at most `block` levels
+    adj = face_adjacency(cells)
+    # cell wavefront levels of the new order: no two adjacent cells share a
+    # level, and a colour of blocks of up to `block` cells needs at most
+    # `block` levels
     level = np.zeros(ne, dtype=int)
     for i in range(ne):
         lower = [level[j] for j in adj[i] if j < i]
@@ -82,11 +73,14 @@ def test_multicolor_order(nx, ny, block):
     for i in range(ne):
         for j in adj[i]:
             assert level[i] != level[j]
-    # blocks of one colour never touch: runs of consecutive equal-colour
-    # cells are separated by a face only inside a block (checked through
-    # the level bound above); with block=1 the levels are the colours
     if block == 1:
         assert level.max() + 1 <= nc
+    # each block is consecutive and face-connected in the order, so the
+    # library's sweep runs recover the blocks (block <= 4): one level per colour
+    if block <= 4:
+        first, lev = sweep_run_levels(adj, 4)
+        assert lev.max() + 1 == nc
+        assert len(first) <= ne // block + ne // 10
 
 
 def test_ctypes_structs_match_header_layout():
