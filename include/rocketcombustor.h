@@ -73,6 +73,7 @@ typedef struct rc_params {
 } rc_params;
 
 typedef struct rc_case rc_case;
+typedef struct rc_batch rc_batch;
 
 /* Replaces ReadMesh (mesh.f90:2-115: faces by `change`, BC tags,
  * VolumeAndCenter) and compute_reconstruction_matrix
@@ -167,6 +168,27 @@ int rc_get_timing(rc_case *h, double *t7);
  * order into grid1.in makes the reference's serial sweeps perform the
  * identical computation. */
 int rc_mesh_order_multicolor(int ne, const int *cellnodes, int block, int *perm, int *ncolors);
+
+/* Batches of resident cases on one device, stepped together.  No
+ * counterpart in the reference (one case per run there); for hosts that
+ * solve many cases at once (parameter sweeps, serving).  Every launch of a
+ * batch step covers all members, so many small cases fill the GPU; each
+ * member's arithmetic is its own single-case step, so its results are
+ * bitwise the same alone and in any batch.  Between batch calls the members
+ * are ordinary cases (rc_get_*, rc_set_primitive); a case belongs to at most
+ * one batch and is removed from it when destroyed. */
+int rc_batch_create(int device, rc_batch **out);
+void rc_batch_destroy(rc_batch *b);                    /* members stay valid */
+int rc_batch_add(rc_batch *b, rc_case *h);
+int rc_batch_remove(rc_batch *b, rc_case *h);
+/* info3[0..2] = members, total cells, most LU-SGS levels of a member. */
+int rc_batch_info(const rc_batch *b, long long *info3);
+/* rc_run_steady for every member at once: nsteps steps; every `every` steps
+ * and at step 1, res_out[(n*members + c)*(nvar+1) + k] = the step_residual
+ * norms of member c (member order = order of rc_batch_add, after removals).
+ * On a device error of a member the run stops at that report with the
+ * member's code and *err_case = its index. */
+int rc_batch_run_steady(rc_batch *b, int nsteps, int every, double *res_out, int *done, int *err_case);
 
 #ifdef __cplusplus
 }

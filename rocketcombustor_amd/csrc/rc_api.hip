@@ -1,5 +1,6 @@
 // C-ABI of the library (include/rocketcombustor.h): case set-up, resident
-// device state, the per-operator entry points and the steady step loop.
+// device state, the per-operator entry points, the steady step loop, and
+// batches of resident cases stepped by the same launches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,17 +14,17 @@
 #include "rc_device.h"
 
 namespace rc {
-void launch_ts_face(const DevMesh &, const DevState &, const DevThermo *, hipStream_t);
-void launch_ts_cell(const DevMesh &, const DevState &, const DevParams *, hipStream_t);
-void launch_ghost(const DevMesh &, const DevState &, const DevThermo *, const DevParams *, hipStream_t);
-void launch_gradlim(const DevMesh &, const DevState &, const DevThermo *, const DevParams *, bool, bool, hipStream_t);
-void launch_flux(const DevMesh &, const DevState &, const DevThermo *, const DevParams *, bool, hipStream_t);
-void launch_residual(const DevMesh &, const DevState &, const DevThermo *, const DevParams *, bool, hipStream_t);
-void launch_step_begin(const DevState &, hipStream_t);
-void launch_lusgs_level(const DevMesh &, const DevState &, const DevThermo *, bool, int, int, hipStream_t);
-void launch_update(const DevMesh &, const DevState &, const DevThermo *, hipStream_t);
-void launch_prim_to_cons(const DevMesh &, const DevState &, const DevThermo *, hipStream_t);
-void launch_resnorm(const DevMesh &, const DevState &, double *, int, double, double *, hipStream_t);
+void launch_ts_face(const LaunchSet &, hipStream_t);
+void launch_ts_cell(const LaunchSet &, hipStream_t);
+void launch_ghost(const LaunchSet &, hipStream_t);
+void launch_gradlim(const LaunchSet &, bool, int, hipStream_t);
+void launch_flux(const LaunchSet &, bool, hipStream_t);
+void launch_residual(const LaunchSet &, bool, hipStream_t);
+void launch_step_begin(const LaunchSet &, hipStream_t);
+void launch_lusgs_level(const LaunchSet &, bool, int, hipStream_t);
+void launch_update(const LaunchSet &, hipStream_t);
+void launch_prim_to_cons(const LaunchSet &, hipStream_t);
+void launch_resnorm(const LaunchSet &, hipStream_t);
 }  // namespace rc
 
 using namespace rc;
@@ -42,8 +43,14 @@ struct rc_case {
     DevState S{};
     DevThermo *d_th = nullptr;
     DevParams *d_par = nullptr;
-    double *d_partial = nullptr, *d_resout = nullptr;
-    int nblk_res = 0;
+    CaseDev hc{};                  // host copy of the case's descriptor
+    CaseDev *d_case = nullptr;
+    // work lists of the case alone (case index 0), kept on the host too so
+    // that batches can concatenate them
+    std::vector<int4> w_cell, w_face, w_ghost;
+    std::vector<std::vector<int4>> w_level;
+    LaunchSet L;                   // the case alone
+    rc_batch *batch = nullptr;     // batch the case belongs to, if any
     void *arena = nullptr;
     size_t arena_bytes = 0;
     int *h_err = nullptr;        // pinned
@@ -58,6 +65,22 @@ struct rc_case {
     size_t ev_used = 0;
     double tms[kTimeSlots] = {};
     long timed_steps = 0;
+    bool use_graph = true;
+    hipGraphExec_t graph_exec = nullptr;
+    std::string err;
+};
+
+// resident cases stepped together: every launch covers all members
+struct rc_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<rc_case *> members;
+    void *arena = nullptr;         // CaseDev array, work lists, residual output
+    LaunchSet L;
+    int nlev = 0;
+    double *h_res = nullptr;       // pinned, members x (kNvar+1)
+    size_t h_res_n = 0;
+    bool dirty = true;             // membership changed since the last build
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
     std::string err;
@@ -186,10 +209,30 @@ std::vector<T> soa(const std::vector<T> &aos, int n, int w, int stride) {
     return out;
 }
 
+// thread-block work items over [lo, hi) in chunks, case index c, partial
+// slots counted from slot0
+void chunk_list(std::vector<int4> &out, int c, int lo, int hi, int chunk, int slot0 = 0) {
+    int slot = slot0;
+    for (int a = lo; a < hi; a += chunk) out.push_back(make_int4(c, a, std::min(hi, a + chunk), slot++));
+}
+
 int layout_and_upload(rc_case *h) {
     const HostMesh &m = h->hm;
     const int sc = pad64(m.ne), sf = pad64(m.nf), sg = pad64(m.ng);
-    h->nblk_res = std::min(1024, std::max(1, (m.ne + 255) / 256));
+    h->w_cell.clear(); h->w_face.clear(); h->w_ghost.clear();
+    chunk_list(h->w_cell, 0, 0, m.ne, kChunk);
+    chunk_list(h->w_face, 0, 0, m.nf, kChunk);
+    chunk_list(h->w_ghost, 0, 0, m.ng, kChunk);
+    const int nlev = (int)m.lvl_ptr.size() - 1;
+    h->w_level.assign(nlev, {});
+    size_t nlw = 0;
+    for (int l = 0; l < nlev; ++l) {
+        chunk_list(h->w_level[l], 0, m.lvl_ptr[l], m.lvl_ptr[l + 1], kRunChunk);
+        nlw += h->w_level[l].size();
+    }
+    const int npartial = (int)h->w_cell.size();
+    int4 *d_wc = nullptr, *d_wf = nullptr, *d_wg = nullptr, *d_wl = nullptr;
+    double *d_partial = nullptr, *d_resout = nullptr;
     auto plan = [&](Carver &c, DevMesh &M, DevState &S) {
         M.ne = m.ne; M.nf = m.nf; M.ng = m.ng; M.sc = sc; M.sf = sf; M.sg = sg;
         S.pa = c.take<double>((size_t)kRecPA * sc);
@@ -220,13 +263,15 @@ int layout_and_upload(rc_case *h) {
         M.gnrm = c.take<double>((size_t)2 * sg);
         M.ghost_host = c.take<int>(sg);
         M.ghost_type = c.take<int>(sg);
-        M.ghost_par = nullptr;
-        M.face_par = nullptr;
-        M.cell_par = nullptr;
         h->d_th = c.take<DevThermo>(1);
         h->d_par = c.take<DevParams>(1);
-        h->d_partial = c.take<double>((size_t)h->nblk_res * (kNvar + 1));
-        h->d_resout = c.take<double>(kNvar + 1);
+        h->d_case = c.take<CaseDev>(1);
+        d_partial = c.take<double>((size_t)npartial * (kNvar + 1));
+        d_resout = c.take<double>(kNvar + 1);
+        d_wc = c.take<int4>(h->w_cell.size());
+        d_wf = c.take<int4>(h->w_face.size());
+        d_wg = c.take<int4>(h->w_ghost.size());
+        d_wl = c.take<int4>(nlw);
     };
     Carver dry;
     DevMesh M{};
@@ -282,6 +327,34 @@ int layout_and_upload(rc_case *h) {
     DevParams dp = make_params(h->par, h->mech.nsp);
     if ((rc = up(h->d_th, &th, sizeof th))) return rc;
     if ((rc = up(h->d_par, &dp, sizeof dp))) return rc;
+    CaseDev &hc = h->hc;
+    hc.M = M;
+    hc.S = S;
+    hc.th = h->d_th;
+    hc.par = h->d_par;
+    hc.partial = d_partial;
+    hc.npartial = npartial;
+    hc.inv_volume = 1.0 / m.total_volume;
+    if ((rc = up(h->d_case, &hc, sizeof hc))) return rc;
+    if (!h->w_cell.empty() && (rc = up(d_wc, h->w_cell.data(), sizeof(int4) * h->w_cell.size()))) return rc;
+    if (!h->w_face.empty() && (rc = up(d_wf, h->w_face.data(), sizeof(int4) * h->w_face.size()))) return rc;
+    if (!h->w_ghost.empty() && (rc = up(d_wg, h->w_ghost.data(), sizeof(int4) * h->w_ghost.size()))) return rc;
+    LaunchSet &L = h->L;
+    L.cases = h->d_case;
+    L.ncases = 1;
+    L.single = true;
+    L.cell = {d_wc, (int)h->w_cell.size(), 0, m.ne};
+    L.face = {d_wf, (int)h->w_face.size(), 0, m.nf};
+    L.ghost = {d_wg, (int)h->w_ghost.size(), 0, m.ng};
+    L.level.assign(nlev, {});
+    size_t off = 0;
+    for (int l = 0; l < nlev; ++l) {
+        const size_t n = h->w_level[l].size();
+        if ((rc = up(d_wl + off, h->w_level[l].data(), sizeof(int4) * n))) return rc;
+        L.level[l] = {d_wl + off, (int)n, m.lvl_ptr[l], m.lvl_ptr[l + 1]};
+        off += n;
+    }
+    L.resout = d_resout;
     return RC_OK;
 }
 
@@ -319,21 +392,20 @@ int drain_timing(rc_case *h) {
 //   (+ TimeStep face part), residual + source (+ TimeStep cell part),
 //   LU-SGS forward wavefronts, backward wavefronts, update.
 void enqueue_step_kernels(rc_case *h) {
-    launch_step_begin(h->S, h->stream);
+    const LaunchSet &L = h->L;
+    launch_step_begin(L, h->stream);
     mark(h, 0);
-    launch_gradlim(h->M, h->S, h->d_th, h->d_par, true, h->par.limit_id > 0, h->stream);
+    launch_gradlim(L, true, 2, h->stream);
     mark(h, 1);
-    launch_flux(h->M, h->S, h->d_th, h->d_par, true, h->stream);
+    launch_flux(L, true, h->stream);
     mark(h, 2);
-    launch_residual(h->M, h->S, h->d_th, h->d_par, true, h->stream);
+    launch_residual(L, true, h->stream);
     mark(h, 3);
-    for (int l = 0; l < h->nlev; ++l)
-        launch_lusgs_level(h->M, h->S, h->d_th, true, h->hm.lvl_ptr[l], h->hm.lvl_ptr[l + 1], h->stream);
+    for (int l = 0; l < h->nlev; ++l) launch_lusgs_level(L, true, l, h->stream);
     mark(h, 4);
-    for (int l = h->nlev - 1; l >= 0; --l)
-        launch_lusgs_level(h->M, h->S, h->d_th, false, h->hm.lvl_ptr[l], h->hm.lvl_ptr[l + 1], h->stream);
+    for (int l = h->nlev - 1; l >= 0; --l) launch_lusgs_level(L, false, l, h->stream);
     mark(h, 5);
-    launch_update(h->M, h->S, h->d_th, h->stream);
+    launch_update(L, h->stream);
 }
 
 // the step captured once into a HIP graph and replayed (no memset nodes:
@@ -376,8 +448,8 @@ int enqueue_stage(rc_case *h, bool allow_graph) {
 }
 
 int residual_async(rc_case *h, double *host_out) {
-    launch_resnorm(h->M, h->S, h->d_partial, h->nblk_res, 1.0 / h->hm.total_volume, h->d_resout, h->stream);
-    HIPCHK(h, hipMemcpyAsync(host_out, h->d_resout, sizeof(double) * (kNvar + 1), hipMemcpyDeviceToHost, h->stream));
+    launch_resnorm(h->L, h->stream);
+    HIPCHK(h, hipMemcpyAsync(host_out, h->L.resout, sizeof(double) * (kNvar + 1), hipMemcpyDeviceToHost, h->stream));
     return RC_OK;
 }
 
@@ -393,6 +465,148 @@ int get_items(rc_case *h, double *out, const double *dev, int n, int width, int 
     for (int i = 0; i < n; ++i)
         for (int c = 0; c < width; ++c)
             out[(size_t)i * width + c] = rec > 0 ? tmp[(size_t)i * rec + c] : tmp[(size_t)c * stride + i];
+    return RC_OK;
+}
+
+int bset_err(rc_batch *b, int code, const std::string &msg) {
+    if (b) b->err = msg;
+    g_error = msg;
+    return code;
+}
+
+#define BHIPCHK(b, x)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            return bset_err(b, RC_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+    } while (0)
+
+void batch_release(rc_batch *b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    b->graph_exec = nullptr;
+    if (b->arena) {
+        (void)hipStreamSynchronize(b->stream);
+        (void)hipFree(b->arena);
+    }
+    b->arena = nullptr;
+}
+
+// concatenates the members' descriptors and work lists (case index =
+// position in the batch) into one device allocation
+int batch_build(rc_batch *b) {
+    if (!b->dirty) return RC_OK;
+    batch_release(b);
+    const int n = (int)b->members.size();
+    std::vector<CaseDev> hcs(n);
+    std::vector<int4> wc, wf, wg;
+    std::vector<std::vector<int4>> wl;
+    for (int c = 0; c < n; ++c) {
+        const rc_case *h = b->members[c];
+        hcs[c] = h->hc;
+        auto add = [c](std::vector<int4> &dst, const std::vector<int4> &src) {
+            for (int4 w : src) dst.push_back(make_int4(c, w.y, w.z, w.w));
+        };
+        add(wc, h->w_cell);
+        add(wf, h->w_face);
+        add(wg, h->w_ghost);
+        if (h->w_level.size() > wl.size()) wl.resize(h->w_level.size());
+        for (size_t l = 0; l < h->w_level.size(); ++l) add(wl[l], h->w_level[l]);
+    }
+    b->nlev = (int)wl.size();
+    size_t nlw = 0;
+    for (auto &v : wl) nlw += v.size();
+    Carver dry, real;
+    auto plan = [&](Carver &c, CaseDev *&dc, int4 *&dwc, int4 *&dwf, int4 *&dwg, int4 *&dwl, double *&dres) {
+        dc = c.take<CaseDev>(std::max(n, 1));
+        dwc = c.take<int4>(wc.size());
+        dwf = c.take<int4>(wf.size());
+        dwg = c.take<int4>(wg.size());
+        dwl = c.take<int4>(nlw);
+        dres = c.take<double>((size_t)std::max(n, 1) * (kNvar + 1));
+    };
+    CaseDev *dc;
+    int4 *dwc, *dwf, *dwg, *dwl;
+    double *dres;
+    plan(dry, dc, dwc, dwf, dwg, dwl, dres);
+    BHIPCHK(b, hipSetDevice(b->device));
+    BHIPCHK(b, hipMalloc(&b->arena, align_up(dry.off, 256)));
+    real.base = static_cast<char *>(b->arena);
+    plan(real, dc, dwc, dwf, dwg, dwl, dres);
+    auto up = [&](void *dst, const void *src, size_t bytes) -> int {
+        if (bytes) BHIPCHK(b, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b->stream));
+        return RC_OK;
+    };
+    int rc;
+    if ((rc = up(dc, hcs.data(), sizeof(CaseDev) * n))) return rc;
+    if ((rc = up(dwc, wc.data(), sizeof(int4) * wc.size()))) return rc;
+    if ((rc = up(dwf, wf.data(), sizeof(int4) * wf.size()))) return rc;
+    if ((rc = up(dwg, wg.data(), sizeof(int4) * wg.size()))) return rc;
+    LaunchSet &L = b->L;
+    L.cases = dc;
+    L.ncases = n;
+    L.single = false;
+    L.cell = {dwc, (int)wc.size(), 0, 0};
+    L.face = {dwf, (int)wf.size(), 0, 0};
+    L.ghost = {dwg, (int)wg.size(), 0, 0};
+    L.level.assign(b->nlev, {});
+    size_t off = 0;
+    for (int l = 0; l < b->nlev; ++l) {
+        if ((rc = up(dwl + off, wl[l].data(), sizeof(int4) * wl[l].size()))) return rc;
+        L.level[l] = {dwl + off, (int)wl[l].size(), 0, 0};
+        off += wl[l].size();
+    }
+    L.resout = dres;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    const size_t need = (size_t)std::max(n, 1) * (kNvar + 1);
+    if (b->h_res_n < need) {
+        if (b->h_res) (void)hipHostFree(b->h_res);
+        b->h_res = nullptr;
+        b->h_res_n = 0;
+        BHIPCHK(b, hipHostMalloc(&b->h_res, need * sizeof(double)));
+        b->h_res_n = need;
+    }
+    b->dirty = false;
+    return RC_OK;
+}
+
+// one steady step of every member: the same kernels as a single case's step
+void enqueue_batch_step(rc_batch *b) {
+    const LaunchSet &L = b->L;
+    launch_step_begin(L, b->stream);
+    launch_gradlim(L, true, 2, b->stream);
+    launch_flux(L, true, b->stream);
+    launch_residual(L, true, b->stream);
+    for (int l = 0; l < b->nlev; ++l) launch_lusgs_level(L, true, l, b->stream);
+    for (int l = b->nlev - 1; l >= 0; --l) launch_lusgs_level(L, false, l, b->stream);
+    launch_update(L, b->stream);
+}
+
+int batch_graph(rc_batch *b) {
+    if (b->graph_exec) return RC_OK;
+    BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_batch_step(b);
+    hipGraph_t g = nullptr;
+    BHIPCHK(b, hipStreamEndCapture(b->stream, &g));
+    hipError_t e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        b->graph_exec = nullptr;
+        return bset_err(b, RC_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    return RC_OK;
+}
+
+// device error flags of every member; the first failing member's code
+int batch_check(rc_batch *b, int *err_case) {
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    for (size_t c = 0; c < b->members.size(); ++c) {
+        rc_case *h = b->members[c];
+        const int rc = check_device_error(h);
+        if (rc) {
+            if (err_case) *err_case = (int)c;
+            return bset_err(b, rc, "case " + std::to_string(c) + ": " + h->err);
+        }
+    }
     return RC_OK;
 }
 
@@ -451,6 +665,7 @@ int rc_case_create(const rc_mechanism *mech, const rc_params *par, int nnt, cons
 
 void rc_case_destroy(rc_case *h) {
     if (!h) return;
+    if (h->batch) rc_batch_remove(h->batch, h);
     if (h->stream) {
         (void)hipSetDevice(h->device);
         (void)hipStreamSynchronize(h->stream);
@@ -477,55 +692,53 @@ int rc_set_primitive(rc_case *h, const double *pa) {
     for (int i = 0; i < h->ne; ++i)
         for (int k = 0; k < kNvar; ++k) tmp[(size_t)i * kRecPA + k] = pa[(size_t)i * kNvar + k];
     HIPCHK(h, hipMemcpyAsync(h->S.pa, tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice, h->stream));
-    launch_prim_to_cons(h->M, h->S, h->d_th, h->stream);
+    launch_prim_to_cons(h->L, h->stream);
     return sync_check(h);
 }
 
 int rc_timestep(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    launch_ts_face(h->M, h->S, h->d_th, h->stream);
-    launch_ts_cell(h->M, h->S, h->d_par, h->stream);
+    launch_ts_face(h->L, h->stream);
+    launch_ts_cell(h->L, h->stream);
     return sync_check(h);
 }
 
 int rc_gradient(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    launch_gradlim(h->M, h->S, h->d_th, h->d_par, true, false, h->stream);
+    launch_gradlim(h->L, true, 0, h->stream);
     return sync_check(h);
 }
 
 int rc_limiter_barth(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    launch_gradlim(h->M, h->S, h->d_th, h->d_par, false, true, h->stream);
+    launch_gradlim(h->L, false, 1, h->stream);
     return sync_check(h);
 }
 
 int rc_reconstruction(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    launch_gradlim(h->M, h->S, h->d_th, h->d_par, true, h->par.limit_id > 0, h->stream);
+    launch_gradlim(h->L, true, 2, h->stream);
     return sync_check(h);
 }
 
 int rc_flux_sourceterm(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    launch_flux(h->M, h->S, h->d_th, h->d_par, false, h->stream);
-    launch_residual(h->M, h->S, h->d_th, h->d_par, false, h->stream);
+    launch_flux(h->L, false, h->stream);
+    launch_residual(h->L, false, h->stream);
     return sync_check(h);
 }
 
 int rc_lusgs_steady(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    for (int l = 0; l < h->nlev; ++l)
-        launch_lusgs_level(h->M, h->S, h->d_th, true, h->hm.lvl_ptr[l], h->hm.lvl_ptr[l + 1], h->stream);
-    for (int l = h->nlev - 1; l >= 0; --l)
-        launch_lusgs_level(h->M, h->S, h->d_th, false, h->hm.lvl_ptr[l], h->hm.lvl_ptr[l + 1], h->stream);
-    launch_update(h->M, h->S, h->d_th, h->stream);
+    for (int l = 0; l < h->nlev; ++l) launch_lusgs_level(h->L, true, l, h->stream);
+    for (int l = h->nlev - 1; l >= 0; --l) launch_lusgs_level(h->L, false, l, h->stream);
+    launch_update(h->L, h->stream);
     return sync_check(h);
 }
 
@@ -634,6 +847,110 @@ int rc_mesh_order_multicolor(int ne, const int *cellnodes, int block, int *perm,
     int rc = order_multicolor(ne, c0.data(), block, perm, ncolors);
     if (rc) return set_err(nullptr, rc, "mesh is not a manifold triangle mesh");
     return RC_OK;
+}
+
+int rc_batch_create(int device, rc_batch **out) {
+    if (!out) return set_err(nullptr, RC_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    rc_batch *b = new rc_batch();
+    b->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete b;
+        return set_err(nullptr, RC_ERR_DEVICE, std::string("batch stream: ") + hipGetErrorString(e));
+    }
+    const char *ng = std::getenv("RC_NO_GRAPH");
+    b->use_graph = !(ng && ng[0] == '1');
+    *out = b;
+    return RC_OK;
+}
+
+void rc_batch_destroy(rc_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    for (rc_case *h : b->members) h->batch = nullptr;
+    batch_release(b);
+    if (b->h_res) (void)hipHostFree(b->h_res);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int rc_batch_add(rc_batch *b, rc_case *h) {
+    if (!b || !h) return set_err(nullptr, RC_ERR_ARGUMENT, "NULL argument");
+    if (h->batch) return bset_err(b, RC_ERR_ARGUMENT, "the case already belongs to a batch");
+    if (h->device != b->device) return bset_err(b, RC_ERR_ARGUMENT, "the case is on another device");
+    BHIPCHK(b, hipSetDevice(b->device));
+    BHIPCHK(b, hipStreamSynchronize(h->stream));
+    b->members.push_back(h);
+    h->batch = b;
+    b->dirty = true;
+    return RC_OK;
+}
+
+int rc_batch_remove(rc_batch *b, rc_case *h) {
+    if (!b || !h) return set_err(nullptr, RC_ERR_ARGUMENT, "NULL argument");
+    auto it = std::find(b->members.begin(), b->members.end(), h);
+    if (it == b->members.end()) return bset_err(b, RC_ERR_ARGUMENT, "the case is not in the batch");
+    BHIPCHK(b, hipSetDevice(b->device));
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    b->members.erase(it);
+    h->batch = nullptr;
+    b->dirty = true;
+    return RC_OK;
+}
+
+int rc_batch_info(const rc_batch *b, long long *info3) {
+    if (!b || !info3) return set_err(nullptr, RC_ERR_ARGUMENT, "NULL argument");
+    long long cells = 0;
+    int nlev = 0;
+    for (const rc_case *h : b->members) {
+        cells += h->ne;
+        nlev = std::max(nlev, h->nlev);
+    }
+    info3[0] = (long long)b->members.size();
+    info3[1] = cells;
+    info3[2] = nlev;
+    return RC_OK;
+}
+
+int rc_batch_run_steady(rc_batch *b, int nsteps, int every, double *res_out, int *done, int *err_case) {
+    if (done) *done = 0;
+    if (err_case) *err_case = -1;
+    if (!b || nsteps < 0) return set_err(nullptr, RC_ERR_ARGUMENT, "bad argument");
+    if (b->members.empty()) return RC_OK;
+    BHIPCHK(b, hipSetDevice(b->device));
+    int rc = batch_build(b);
+    if (rc) return rc;
+    const size_t n = b->members.size();
+    if (every <= 0) every = nsteps + 1;
+    int nrep = 0;
+    for (int s = 1; s <= nsteps; ++s) {
+        if (b->use_graph) {
+            if ((rc = batch_graph(b))) return rc;
+            BHIPCHK(b, hipGraphLaunch(b->graph_exec, b->stream));
+        } else {
+            enqueue_batch_step(b);
+        }
+        for (rc_case *h : b->members) h->steps_done++;
+        if (s == 1 || s % every == 0) {
+            if (res_out) {
+                launch_resnorm(b->L, b->stream);
+                BHIPCHK(b, hipMemcpyAsync(b->h_res, b->L.resout, sizeof(double) * n * (kNvar + 1),
+                                          hipMemcpyDeviceToHost, b->stream));
+            }
+            if ((rc = batch_check(b, err_case))) {
+                if (done) *done = s;
+                return rc;
+            }
+            if (res_out)
+                std::memcpy(res_out + (size_t)nrep * n * (kNvar + 1), b->h_res, sizeof(double) * n * (kNvar + 1));
+            ++nrep;
+        }
+    }
+    rc = batch_check(b, err_case);
+    if (done) *done = nsteps;
+    return rc;
 }
 
 }  // extern "C"

@@ -3,14 +3,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "rc_internal.h"
 
 namespace rc {
 
-// Structure-of-arrays views of one case (or of a packed batch of cases).
-// Cell arrays have stride sc, face arrays sf, ghost arrays sg:
-//   pa[k*sc+i], gradc[(k*2+r)*sc+i], stencil[s*sc+i], amls[(s*2+r)*sc+i],
-//   lvv[(kf*2+d)*sc+i], cface[kf*sc+i], vecx[d*sf+f], fflux[k*sf+f].
+// Device views of one case.  Arrays a cell reads only for itself are one
+// array per value with stride sc (faces sf, ghosts sg):
+//   stencil[s*sc+i], amls[(s*2+r)*sc+i], lvv[(kf*2+d)*sc+i], cface[kf*sc+i],
+//   vecx[d*sf+f]; gathered arrays are records (see kRec* below).
 struct DevMesh {
     int ne, nf, ng;
     int sc, sf, sg;
@@ -20,9 +22,6 @@ struct DevMesh {
     const int *__restrict__ cface;
     const int *__restrict__ ghost_host;
     const int *__restrict__ ghost_type;
-    const int *__restrict__ ghost_par;   // parameter-set index per ghost (batches)
-    const int *__restrict__ face_par;    // parameter-set index per face (batches)
-    const int *__restrict__ cell_par;    // parameter-set index per cell (batches)
     const int *__restrict__ runs;        // LU-SGS runs (first cell, count), by level
     const double *__restrict__ vecx;
     const double *__restrict__ fvl;
@@ -68,12 +67,88 @@ struct DevState {
     int *err;        // [0] code, [1] index, [2] kernel, [3] step of the error, [4] current step
 };
 
+// one case as the kernels see it; an array of these (one per resident case)
+// is what every launch reads, so one launch serves any set of cases
+struct CaseDev {
+    DevMesh M;
+    DevState S;
+    const DevThermo *th;
+    const DevParams *par;
+    double *partial;     // residual partial sums, (kNvar+1) per cell chunk
+    int npartial;
+    double inv_volume;   // 1 / total volume
+};
+
+// work of one launch: one int4 per thread block, (case index, first item,
+// end item, partial-sum slot); items are cells, faces, ghosts or LU-SGS runs.
+// [lo, hi) is the same range for a set of one case, launched without the
+// table.
+struct WorkList {
+    const int4 *w = nullptr;
+    int n = 0;
+    int lo = 0, hi = 0;
+};
+
+// everything one step launches over a set of cases
+struct LaunchSet {
+    const CaseDev *cases = nullptr;
+    int ncases = 0;
+    bool single = false;           // one case: cases[0], launched over [lo, hi)
+    WorkList cell, face, ghost;
+    std::vector<WorkList> level;   // LU-SGS runs of each level
+    double *resout = nullptr;      // ncases x (kNvar+1) residual norms
+};
+
+// Device-side views of a case whose pointers are typed as global memory.
+// The host-side structs above hold plain pointers; a kernel reads them from
+// the case table and casts them here, so every access compiles to a global
+// (not flat) load or store.
+#define RC_G __attribute__((address_space(1)))
+struct GMesh {
+    int ne, nf, ng;
+    int sc, sf, sg;
+    const RC_G int *fnl, *fnr, *stencil, *cface, *ghost_host, *ghost_type, *runs, *lu_nb;
+    const RC_G double *vecx, *fvl, *fvr, *vol, *wdis, *amls, *lvv, *gnrm, *lu_n;
+};
+struct GState {
+    RC_G double *pa, *wa, *hwa, *gradc, *gpa, *step, *dd, *fflux, *lucoef, *lu_af;
+    RC_G int *err;
+};
+typedef const RC_G DevThermo GThermo;
+typedef const RC_G DevParams GParams;
+
+template <class T>
+__device__ __forceinline__ RC_G T *gp(T *p) {
+    return (RC_G T *)p;
+}
+
+__device__ __forceinline__ GMesh gview(const RC_G DevMesh &m) {
+    GMesh g;
+    g.ne = m.ne; g.nf = m.nf; g.ng = m.ng; g.sc = m.sc; g.sf = m.sf; g.sg = m.sg;
+    g.fnl = gp(m.fnl); g.fnr = gp(m.fnr); g.stencil = gp(m.stencil); g.cface = gp(m.cface);
+    g.ghost_host = gp(m.ghost_host); g.ghost_type = gp(m.ghost_type); g.runs = gp(m.runs); g.lu_nb = gp(m.lu_nb);
+    g.vecx = gp(m.vecx); g.fvl = gp(m.fvl); g.fvr = gp(m.fvr); g.vol = gp(m.vol); g.wdis = gp(m.wdis);
+    g.amls = gp(m.amls); g.lvv = gp(m.lvv); g.gnrm = gp(m.gnrm); g.lu_n = gp(m.lu_n);
+    return g;
+}
+
+__device__ __forceinline__ GState gview(const RC_G DevState &s) {
+    GState g;
+    g.pa = gp(s.pa); g.wa = gp(s.wa); g.hwa = gp(s.hwa); g.gradc = gp(s.gradc); g.gpa = gp(s.gpa);
+    g.step = gp(s.step); g.dd = gp(s.dd); g.fflux = gp(s.fflux); g.lucoef = gp(s.lucoef); g.lu_af = gp(s.lu_af);
+    g.err = gp(s.err);
+    return g;
+}
+
+constexpr int kChunk = 256;          // cells / faces / ghosts per thread block
+constexpr int kRunChunk = kChunk / kRunMax;   // LU-SGS runs per thread block
+
 enum KernelId { K_TS_FACE = 1, K_TS_CELL, K_GHOST, K_GRADLIM, K_FLUX, K_RES, K_FWD, K_BWD, K_UPDATE, K_P2C };
 
 // first error wins; the step number comes from the device-side step
 // counter err[4] (advanced by k_step_begin), so captured graphs report it too
-__device__ inline void flag_error(int *err, int code, int index, int kernel) {
-    if (atomicCAS(err, 0, code) == 0) {
+__device__ inline void flag_error(RC_G int *err, int code, int index, int kernel) {
+    if (atomicCAS((int *)err, 0, code) == 0) {
         err[1] = index;
         err[2] = kernel;
         err[3] = __atomic_load_n(err + 4, __ATOMIC_RELAXED);
@@ -100,7 +175,7 @@ __device__ __forceinline__ double powi_d(double a, int b) {
 // outside [t_low, t_high] (the reference stops there); the high-range
 // coefficients are then used so that the arithmetic stays finite.
 template <int NSP>
-__device__ __forceinline__ bool gas_param(const DevThermo &th, const double *y, double t,
+__device__ __forceinline__ bool gas_param(GThermo &th, const double *y, double t,
                                           double &cp, double &gam, double &r) {
     double basis[7];
 #pragma unroll
@@ -108,7 +183,7 @@ __device__ __forceinline__ bool gas_param(const DevThermo &th, const double *y, 
     bool ok = true;
     const bool low = (th.t_low <= t && t <= th.t_crit);
     if (!low && !(th.t_crit < t && t <= th.t_high)) ok = false;
-    const double(*coef)[7] = low ? th.low : th.high;
+    const RC_G double(*coef)[7] = low ? th.low : th.high;
     double c = 0.0, rr = 0.0;
     double cps[NSP];
 #pragma unroll
@@ -130,7 +205,7 @@ __device__ __forceinline__ bool gas_param(const DevThermo &th, const double *y, 
 
 // compute_enthalpy, thermo_property.f90:114-144
 template <int NSP>
-__device__ __forceinline__ bool enthalpy(const DevThermo &th, double t, double *h) {
+__device__ __forceinline__ bool enthalpy(GThermo &th, double t, double *h) {
     double basis[7];
     basis[0] = -powi_d(t, -2);
     basis[1] = log(t) / t;
@@ -139,8 +214,8 @@ __device__ __forceinline__ bool enthalpy(const DevThermo &th, double t, double *
     bool ok = true;
     const bool low = (th.t_low <= t && t <= th.t_crit);
     if (!low && !(th.t_crit < t && t <= th.t_high)) ok = false;
-    const double(*coef)[7] = low ? th.low : th.high;
-    const double *b1 = low ? th.low_b1 : th.high_b1;
+    const RC_G double(*coef)[7] = low ? th.low : th.high;
+    const RC_G double *b1 = low ? th.low_b1 : th.high_b1;
 #pragma unroll
     for (int i = 0; i < NSP; ++i) {
         double s = 0.0;
@@ -153,7 +228,7 @@ __device__ __forceinline__ bool enthalpy(const DevThermo &th, double t, double *
 
 // Newton_temperature, thermo_property.f90:146-180
 template <int NSP>
-__device__ __forceinline__ bool newton_temperature(const DevThermo &th, double et, const double *y, double t0,
+__device__ __forceinline__ bool newton_temperature(GThermo &th, double et, const double *y, double t0,
                                                    double &t, double &cp, double &gam, double &r) {
     t = t0;
     bool ok = true;
@@ -179,7 +254,7 @@ __device__ __forceinline__ bool newton_temperature(const DevThermo &th, double e
 // compute_production_rate, chemical_kinetics.f90:109-165 (one reaction;
 // model 0: A exp(-E/RT) [H2]^0.2 [O2]^1.3, model 1: 1e4 [H2] above 710 K)
 template <int NSP>
-__device__ __forceinline__ void production_rate(const DevThermo &th, double t, double rho, const double *y2, double *w) {
+__device__ __forceinline__ void production_rate(GThermo &th, double t, double rho, const double *y2, double *w) {
     double y[NSP];
 #pragma unroll
     for (int k = 0; k < NSP; ++k) y[k] = fmin(1.0, fmax(0.0, y2[k]));

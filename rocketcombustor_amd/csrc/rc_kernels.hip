@@ -23,10 +23,34 @@ namespace rc {
 
 constexpr int NV = kNvar;
 
+// Every per-item kernel has two forms.  B (batch): thread block b takes
+// work item work[b] = (case, first, end, slot) of the case table.  !B (one
+// case): cases[0] is the case and block b takes items lo + b*blockDim ...
+// below hi, so no table load precedes the case's fields.
+#define ITEM_ARGS const CaseDev *__restrict__ cases, const int4 *__restrict__ work, int lo, int hi
+#define ITEM_PROLOGUE(idx)                                              \
+    const RC_G CaseDev *Cp_;                                            \
+    int idx;                                                            \
+    if constexpr (B) {                                                  \
+        const int4 w_ = work[blockIdx.x];                           \
+        Cp_ = gp(cases) + w_.x;                                         \
+        idx = w_.y + (int)threadIdx.x;                                  \
+        if (idx >= w_.z) return;                                        \
+    } else {                                                            \
+        Cp_ = gp(cases);                                                \
+        idx = lo + (int)(blockIdx.x * blockDim.x + threadIdx.x);        \
+        if (idx >= hi) return;                                          \
+    }                                                                   \
+    const RC_G CaseDev &C = *Cp_;                                       \
+    const GMesh M = gview(C.M);                                         \
+    const GState S = gview(C.S);                                        \
+    GThermo &TH = *gp(C.th);                                            \
+    GParams &PAR = *gp(C.par);
+
 // TimeStep face loop (TimeStep.f90:16-92, inviscid): spectral radius of face
 // f from the cell averages of its two cells (one cell on a boundary)
 template <int NSP>
-__device__ __forceinline__ double face_spectral_radius(const DevMesh &M, const DevState &S, const DevThermo &th,
+__device__ __forceinline__ double face_spectral_radius(const GMesh &M, const GState &S, GThermo &th,
                                                        int f, int nl, int nr, double dx, double dy) {
     double pp, uu, vv, tt, y[NSP], s = 0.0;
     if (nr >= 0) {
@@ -50,18 +74,16 @@ __device__ __forceinline__ double face_spectral_radius(const DevMesh &M, const D
     return fabs(un) + sqrt(gam * pp / rr) * sqrt(dx * dx + dy * dy);
 }
 
-template <int NSP>
-__global__ void __launch_bounds__(256) k_timestep_face(DevMesh M, DevState S, const DevThermo *__restrict__ thp) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= M.nf) return;
-    ALPHA_(f) = face_spectral_radius<NSP>(M, S, *thp, f, M.fnl[f], M.fnr[f], M.vecx[f], M.vecx[M.sf + f]);
+template <int NSP, bool B>
+__global__ void __launch_bounds__(256) k_timestep_face(ITEM_ARGS) {
+    ITEM_PROLOGUE(f);
+    ALPHA_(f) = face_spectral_radius<NSP>(M, S, TH, f, M.fnl[f], M.fnr[f], M.vecx[f], M.vecx[M.sf + f]);
 }
 
 // TimeStep cell loop (TimeStep.f90:94-96) and the LU-SGS diagonal
 // dd(i) = sum 0.5*alpaF + vol/step (Time_LUSGS_steady.f90:46-53)
-__device__ __forceinline__ void cell_step_dd(const DevMesh &M, const DevState &S, const DevParams *__restrict__ pars,
-                                             int i) {
-    const double cfl = pars[M.cell_par ? M.cell_par[i] : 0].cfl;
+__device__ __forceinline__ void cell_step_dd(const GMesh &M, const GState &S, GParams &P, int i) {
+    const double cfl = P.cfl;
     double a = 0.0, d = 0.0, al[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -78,15 +100,19 @@ __device__ __forceinline__ void cell_step_dd(const DevMesh &M, const DevState &S
     S.dd[i] = d + v / st;
 }
 
-__global__ void __launch_bounds__(256) k_timestep_cell(DevMesh M, DevState S, const DevParams *__restrict__ pars) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M.ne) return;
-    cell_step_dd(M, S, pars, i);
+template <int NSP, bool B>
+__global__ void __launch_bounds__(256) k_timestep_cell(ITEM_ARGS) {
+    ITEM_PROLOGUE(i);
+    cell_step_dd(M, S, PAR, i);
 }
 
-// advances the device step counter used in error reports
-__global__ void k_step_begin(int *err) {
-    if (threadIdx.x == 0) err[4] = err[4] + 1;
+// advances each case's device step counter used in error reports
+__global__ void k_step_begin(const CaseDev *__restrict__ cases, int n) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) {
+        RC_G int *err = gp(gp(cases)[c].S.err);
+        err[4] = err[4] + 1;
+    }
 }
 
 // update_ghostcell_variables (update_flow_variables.f90:90-269): the state
@@ -94,10 +120,8 @@ __global__ void k_step_begin(int *err) {
 // host's gamma and density (ComputeGasParameter of its own state) come from
 // the cell-average cache
 template <int NSP>
-__device__ __forceinline__ void ghost_state(const DevMesh &M, const DevState &S, const DevThermo &th,
-                                            const DevParams *__restrict__ pars, int g, int h, const double *pl_,
-                                            double *out) {
-    const DevParams &P = pars[M.ghost_par ? M.ghost_par[g] : 0];
+__device__ __forceinline__ void ghost_state(const GMesh &M, const GState &S, GThermo &th,
+                                            GParams &P, int g, int h, const double *pl_, double *out) {
     const double pl = pl_[0], ul = pl_[1], vl = pl_[2], tl = pl_[3];
     double yl[NSP], yr[NSP], s = 0.0;
 #pragma unroll
@@ -122,7 +146,7 @@ __device__ __forceinline__ void ghost_state(const DevMesh &M, const DevState &S,
         for (int k = 0; k < NSP; ++k) yr[k] = yl[k];
     } else if (type == 3 || type == 4) {
         // oxidizer_inlet_bc / fuel_inlet_bc (oxidizer.f90:53-68, fuel.f90:54-69)
-        const double *yin = type == 3 ? P.ox_y : P.fu_y;
+        const RC_G double *yin = type == 3 ? P.ox_y : P.fu_y;
 #pragma unroll
         for (int k = 0; k < NSP; ++k) yr[k] = yin[k];
         tr = type == 3 ? P.ox_t : P.fu_t;
@@ -144,16 +168,14 @@ __device__ __forceinline__ void ghost_state(const DevMesh &M, const DevState &S,
     for (int k = 0; k < NSP - 1; ++k) out[kNFlow + k] = yr[k];
 }
 
-template <int NSP>
-__global__ void __launch_bounds__(256) k_ghost(DevMesh M, DevState S, const DevThermo *__restrict__ thp,
-                                               const DevParams *__restrict__ pars) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= M.ng) return;
+template <int NSP, bool B>
+__global__ void __launch_bounds__(256) k_ghost(ITEM_ARGS) {
+    ITEM_PROLOGUE(g);
     const int i = M.ghost_host[g];
     double pl[NV], out[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) pl[k] = PA_(k, i);
-    ghost_state<NSP>(M, S, *thp, pars, g, i, pl, out);
+    ghost_state<NSP>(M, S, TH, PAR, g, i, pl, out);
 #pragma unroll
     for (int k = 0; k < NV; ++k) GPA_(k, g) = out[k];
 }
@@ -162,13 +184,14 @@ __global__ void __launch_bounds__(256) k_ghost(DevMesh M, DevState S, const DevT
 // one thread per cell.  GRAD: compute GradC = AMLS * (wdis * dPA) over the
 // 3 stencil slots (interior neighbours, then ghosts); the ghost states of a
 // boundary cell depend only on the cell itself, so they are formed here (and
-// stored for the limiter-only entry point).  LIMIT: scale each variable's
-// gradient by the Barth factor over the 3 face midpoints.
-template <int NSP, bool GRAD, bool LIMIT>
-__global__ void __launch_bounds__(256) k_gradlim(DevMesh M, DevState S, const DevThermo *__restrict__ thp,
-                                                 const DevParams *__restrict__ pars) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M.ne) return;
+// stored for the limiter-only entry point).  LIM: scale each variable's
+// gradient by the Barth factor over the 3 face midpoints (LIM_ALWAYS), or
+// when the case's limit_ID > 0 (LIM_CASE), as reconstruction does.
+enum { LIM_NONE = 0, LIM_ALWAYS = 1, LIM_CASE = 2 };
+template <int NSP, bool GRAD, int LIM, bool B>
+__global__ void __launch_bounds__(256) k_gradlim(ITEM_ARGS) {
+    ITEM_PROLOGUE(i);
+    const bool LIMIT = LIM == LIM_ALWAYS || (LIM == LIM_CASE && PAR.limit_id > 0);
     double pi[NV], pmin[NV], pmax[NV], g0[NV], g1[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) { pi[k] = PA_(k, i); pmin[k] = pi[k]; pmax[k] = pi[k]; g0[k] = 0.0; g1[k] = 0.0; }
@@ -183,7 +206,7 @@ __global__ void __launch_bounds__(256) k_gradlim(DevMesh M, DevState S, const De
 #pragma unroll
             for (int k = 0; k < NV; ++k) q[k] = PA_(k, j);
         } else if (GRAD) {
-            ghost_state<NSP>(M, S, *thp, pars, -j - 1, i, pi, q);
+            ghost_state<NSP>(M, S, TH, PAR, -j - 1, i, pi, q);
 #pragma unroll
             for (int k = 0; k < NV; ++k) GPA_(k, -j - 1) = q[k];
         } else {
@@ -234,12 +257,10 @@ __global__ void __launch_bounds__(256) k_gradlim(DevMesh M, DevState S, const De
 // TS: also the face spectral radius of TimeStep, which depends only on the
 // cell averages this thread loads anyway (the values equal a separate
 // TimeStep pass: PA does not change between TimeStep and flux_lax).
-template <int NSP, bool TS>
-__global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, const DevThermo *__restrict__ thp,
-                                              const DevParams *__restrict__ pars) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= M.nf) return;
-    const DevThermo &th = *thp;
+template <int NSP, bool TS, bool B>
+__global__ void __launch_bounds__(256) k_flux(ITEM_ARGS) {
+    ITEM_PROLOGUE(f);
+    GThermo &th = TH;
     const int nl = M.fnl[f], nr = M.fnr[f];
     const double dx = M.vecx[f], dy = M.vecx[M.sf + f];
     if (TS) ALPHA_(f) = face_spectral_radius<NSP>(M, S, th, f, nl, nr, dx, dy);
@@ -272,7 +293,7 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, const DevTh
         if (!gas_param<NSP>(th, yr, tr, cpr, gamr, rcr)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX);
         rr = pr / (rcr * tr);
     } else {
-        const DevParams &P = pars[M.face_par ? M.face_par[f] : 0];
+        GParams &P = PAR;
         const int type = -nr;
         cpr = cpl; gamr = gaml; rcr = rcl;
 #pragma unroll
@@ -285,7 +306,7 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, const DevTh
             ur = ul - 2 * vn * s1n;
             vr = vl - 2 * vn * s2n;
         } else if (type == 3 || type == 4) {
-            const double *yin = type == 3 ? P.ox_y : P.fu_y;
+            const RC_G double *yin = type == 3 ? P.ox_y : P.fu_y;
 #pragma unroll
             for (int k = 0; k < NSP; ++k) yr[k] = yin[k];
             tr = type == 3 ? P.ox_t : P.fu_t;
@@ -342,12 +363,11 @@ __global__ void __launch_bounds__(256) k_flux(DevMesh M, DevState S, const DevTh
 // -F (owner) / +F (neighbour) over the cell's faces in face order, then the
 // chemical source vol*omega of sourceterm (sourceterm.f90:2-70).
 // TS: also the cell part of TimeStep (step, dd) from the same faces.
-template <int NSP, bool TS>
-__global__ void __launch_bounds__(256) k_residual(DevMesh M, DevState S, const DevThermo *__restrict__ thp,
-                                                  const DevParams *__restrict__ pars) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M.ne) return;
-    if (TS) cell_step_dd(M, S, pars, i);
+template <int NSP, bool TS, bool B>
+__global__ void __launch_bounds__(256) k_residual(ITEM_ARGS) {
+    ITEM_PROLOGUE(i);
+    GParams &P = PAR;
+    if (TS) cell_step_dd(M, S, P, i);
     double h[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) h[k] = 0.0;
@@ -363,9 +383,8 @@ __global__ void __launch_bounds__(256) k_residual(DevMesh M, DevState S, const D
             for (int k = 0; k < NV; ++k) h[k] = h[k] + F_(k, f);
         }
     }
-    const DevParams &P = pars[M.cell_par ? M.cell_par[i] : 0];
     if (P.combustion) {
-        const DevThermo &th = *thp;
+        GThermo &th = TH;
         const double t = PA_(3, i);
         double y[NSP], s = 0.0;
 #pragma unroll
@@ -390,7 +409,7 @@ __global__ void __launch_bounds__(256) k_residual(DevMesh M, DevState S, const D
 // the per-cell cache; the arithmetic is that of the reference's neighbour
 // term, so the cached values equal recomputing them there
 template <int NSP>
-__device__ __forceinline__ bool write_lucoef(const DevMesh &M, const DevState &S, const DevThermo &th, int i,
+__device__ __forceinline__ bool write_lucoef(const GMesh &M, const GState &S, GThermo &th, int i,
                                              double pp, double uu, double vv, double tt, const double *yy) {
     double cp, gama, rcp, h[NSP];
     bool ok = gas_param<NSP>(th, yy, tt, cp, gama, rcp);
@@ -420,7 +439,7 @@ struct LuNb {
     double rr, rou, rov, roep, uu, vv, tt, gama, h[kNsp], ry[kNsp - 1];
 };
 
-__device__ __forceinline__ LuNb load_lunb(const DevState &S, int nr) {
+__device__ __forceinline__ LuNb load_lunb(const GState &S, int nr) {
     LuNb n;
     n.rr = LC_(LC_RR, nr); n.rou = LC_(LC_ROU, nr); n.rov = LC_(LC_ROV, nr); n.roep = LC_(LC_ROEP, nr);
     n.uu = LC_(LC_U, nr); n.vv = LC_(LC_V, nr); n.tt = LC_(LC_T, nr); n.gama = LC_(LC_GAM, nr);
@@ -435,7 +454,7 @@ __device__ __forceinline__ LuNb load_lunb(const DevState &S, int nr) {
 // the linearised Rusanov flux change 0.5*(-alpaF*dW + dF(dW).S) of a
 // neighbour's increment dw; the caller adds it to dflux in face order
 template <int NSP>
-__device__ __forceinline__ void lusgs_term(const DevThermo &th, const LuNb &n, const double *dw, double dx,
+__device__ __forceinline__ void lusgs_term(GThermo &th, const LuNb &n, const double *dw, double dx,
                                            double dy, double af, double *c) {
     const double drr = dw[0], dru = dw[1], drv = dw[2], dre = dw[3];
     double dry[NSP], s = 0.0;
@@ -462,7 +481,8 @@ __device__ __forceinline__ void lusgs_term(const DevThermo &th, const LuNb &n, c
 }
 
 // one level of the forward (FWD) or backward sweep of LU_SGS_steady
-// (Time_LUSGS_steady.f90:56-120 / 127-190) over runs[lo, hi).  A run is up
+// (Time_LUSGS_steady.f90:56-120 / 127-190) over the runs of each work item
+// (kRunChunk runs of one case per thread block).  A run is up
 // to kRunMax consecutive cells swept in order; it is handled by a group of
 // kRunMax lanes, lane p owning cell first+p.  Each lane loads the records
 // of its cell's neighbours at once and forms the terms of neighbours outside
@@ -472,16 +492,31 @@ __device__ __forceinline__ void lusgs_term(const DevThermo &th, const LuNb &n, c
 // received by shuffles in the steps before, and forms its increment.  Terms
 // enter dflux in face order; a face that does not take part adds +0.0, which
 // leaves the reference's sums unchanged.
-template <int NSP, bool FWD>
-__global__ void __launch_bounds__(256) k_lusgs(DevMesh M, DevState S, const DevThermo *__restrict__ thp, int lo, int hi) {
+template <int NSP, bool FWD, bool B>
+__global__ void __launch_bounds__(256) k_lusgs(ITEM_ARGS) {
     constexpr int R = kRunMax;
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const RC_G CaseDev *Cp_;
+    int t, tend;
+    if constexpr (B) {
+        const int4 w_ = work[blockIdx.x];
+        Cp_ = gp(cases) + w_.x;
+        t = w_.y + (int)threadIdx.x / R;
+        tend = w_.z;
+    } else {
+        Cp_ = gp(cases);
+        t = lo + (int)(blockIdx.x * blockDim.x + threadIdx.x) / R;
+        tend = hi;
+    }
+    const RC_G CaseDev &C = *Cp_;
+    const GMesh M = gview(C.M);
+    const GState S = gview(C.S);
+    GThermo &TH = *gp(C.th);
     const int lane = threadIdx.x % R;
-    const int t = lo + gid / R;
-    const DevThermo &th = *thp;
+    GThermo &th = TH;
     const int sc = M.sc;
+    // every lane stays to the end: the group's shuffles need all four
     int first = 0, cnt = 0;
-    if (t < hi) { first = M.runs[2 * t]; cnt = M.runs[2 * t + 1]; }
+    if (t < tend) { first = M.runs[2 * t]; cnt = M.runs[2 * t + 1]; }
     const bool live = lane < cnt;
     const int i = live ? first + lane : 0;
     int nb[3];
@@ -511,7 +546,7 @@ __global__ void __launch_bounds__(256) k_lusgs(DevMesh M, DevState S, const DevT
         // increments that come from the run: one shared line, not one per
         // lane, and never a record this launch writes
         const int j = use[k] ? nr : 0;
-        const double *hw = (use[k] && !inrun) ? &HWA_(0, nr) : &LC_(0, 0);
+        const RC_G double *hw = (use[k] && !inrun) ? &HWA_(0, nr) : &LC_(0, 0);
         nbr[k] = load_lunb(S, j);
         double dw[NV];
 #pragma unroll
@@ -564,11 +599,10 @@ __global__ void __launch_bounds__(256) k_lusgs(DevMesh M, DevState S, const DevT
 // WA += dW (Time_LUSGS_steady.f90:193-197) and update_primitve_variables
 // (update_flow_variables.f90:4-48), plus the NaN check of lu_sgs_stage;
 // the new cell-average quantities go to the cache
-template <int NSP>
-__global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, const DevThermo *__restrict__ thp) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M.ne) return;
-    const DevThermo &th = *thp;
+template <int NSP, bool B>
+__global__ void __launch_bounds__(256) k_update(ITEM_ARGS) {
+    ITEM_PROLOGUE(i);
+    GThermo &th = TH;
     double w[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) { w[k] = WA_(k, i) + HWA_(k, i); WA_(k, i) = w[k]; }
@@ -595,11 +629,10 @@ __global__ void __launch_bounds__(256) k_update(DevMesh M, DevState S, const Dev
 
 // primitve_to_conserved_variables (update_flow_variables.f90:50-84), and
 // the cell-average cache of the new state
-template <int NSP>
-__global__ void __launch_bounds__(256) k_prim_to_cons(DevMesh M, DevState S, const DevThermo *__restrict__ thp) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M.ne) return;
-    const DevThermo &th = *thp;
+template <int NSP, bool B>
+__global__ void __launch_bounds__(256) k_prim_to_cons(ITEM_ARGS) {
+    ITEM_PROLOGUE(i);
+    GThermo &th = TH;
     const double pp = PA_(0, i), uu = PA_(1, i), vv = PA_(2, i), tt = PA_(3, i);
     double yy[NSP], s = 0.0;
 #pragma unroll
@@ -622,103 +655,144 @@ __global__ void __launch_bounds__(256) k_prim_to_cons(DevMesh M, DevState S, con
     for (int k = 0; k < NSP - 1; ++k) WA_(kNFlow + k, i) = rr * yy[k];
 }
 
-// Residual (residual.f90:4-21): per-block partial sums of vol*|hWA| and
-// max(WA(1)); a second pass folds the partials in block order, so the
-// result is deterministic.
-__global__ void __launch_bounds__(256) k_resnorm_partial(DevMesh M, DevState S, double *partial) {
+// Residual (residual.f90:4-21): partial sums of vol*|hWA| and max(WA(1))
+// per cell chunk (one work item per block, slot w.w of the case's partials),
+// then one block per case folds its partials in a fixed order, so results
+// are deterministic (the summation order is not the reference's serial one:
+// the norms agree to rounding).
+template <int NSP, bool B>
+__global__ void __launch_bounds__(256) k_resnorm_partial(ITEM_ARGS) {
     __shared__ double sh[256][NV + 1];
+    const RC_G CaseDev *Cp_;
+    int i, iend, slot;
+    if constexpr (B) {
+        const int4 w_ = work[blockIdx.x];
+        Cp_ = gp(cases) + w_.x;
+        i = w_.y + (int)threadIdx.x;
+        iend = w_.z;
+        slot = w_.w;
+    } else {
+        Cp_ = gp(cases);
+        i = lo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+        iend = hi;
+        slot = blockIdx.x;
+    }
+    const RC_G CaseDev &C = *Cp_;
+    const GMesh M = gview(C.M);
+    const GState S = gview(C.S);
     const int tid = threadIdx.x;
     double acc[NV + 1];
 #pragma unroll
     for (int k = 0; k <= NV; ++k) acc[k] = 0.0;
-    for (int i = blockIdx.x * blockDim.x + tid; i < M.ne; i += gridDim.x * blockDim.x) {
+    if (i < iend) {
         const double v = M.vol[i];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) acc[k] += v * fabs(HWA_(k, i));
-        acc[NV] = fmax(acc[NV], WA_(0, i));
+        for (int k = 0; k < NV; ++k) acc[k] = v * fabs(HWA_(k, i));
+        acc[NV] = WA_(0, i);
+    } else {
+        acc[NV] = -INFINITY;
     }
 #pragma unroll
     for (int k = 0; k <= NV; ++k) sh[tid][k] = acc[k];
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w) {
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) {
 #pragma unroll
-            for (int k = 0; k < NV; ++k) sh[tid][k] += sh[tid + w][k];
-            sh[tid][NV] = fmax(sh[tid][NV], sh[tid + w][NV]);
+            for (int k = 0; k < NV; ++k) sh[tid][k] += sh[tid + h][k];
+            sh[tid][NV] = fmax(sh[tid][NV], sh[tid + h][NV]);
         }
         __syncthreads();
     }
-    if (tid <= NV) partial[blockIdx.x * (NV + 1) + tid] = sh[0][tid];
+    if (tid <= NV) gp(C.partial)[slot * (NV + 1) + tid] = sh[0][tid];
 }
 
-__global__ void k_resnorm_final(const double *partial, int nblk, double inv_volume, double *out) {
-    const int k = threadIdx.x;
-    if (k > NV) return;
-    double a = 0.0;
-    for (int b = 0; b < nblk; ++b) {
-        const double x = partial[b * (NV + 1) + k];
-        a = (k == NV) ? fmax(a, x) : a + x;
+__global__ void __launch_bounds__(256) k_resnorm_final(const CaseDev *__restrict__ cases, double *out) {
+    __shared__ double sh[256][NV + 1];
+    const RC_G CaseDev &C = gp(cases)[blockIdx.x];
+    const RC_G double *partial = gp(C.partial);
+    const int tid = threadIdx.x;
+    double acc[NV + 1];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    acc[NV] = -INFINITY;
+    for (int b = tid; b < C.npartial; b += 256) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += partial[b * (NV + 1) + k];
+        acc[NV] = fmax(acc[NV], partial[b * (NV + 1) + NV]);
     }
-    out[k] = (k == NV) ? a : a * inv_volume;
+#pragma unroll
+    for (int k = 0; k <= NV; ++k) sh[tid][k] = acc[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) sh[tid][k] += sh[tid + h][k];
+            sh[tid][NV] = fmax(sh[tid][NV], sh[tid + h][NV]);
+        }
+        __syncthreads();
+    }
+    if (tid < NV) out[blockIdx.x * (NV + 1) + tid] = sh[0][tid] * C.inv_volume;
+    if (tid == NV) out[blockIdx.x * (NV + 1) + tid] = sh[0][NV];
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static inline dim3 grid_for(int n, int bs = 256) { return dim3((unsigned)((n + bs - 1) / bs)); }
+// the one-case form over [lo, hi) (per items per block) or the batch form
+// over the work list (one block per work item)
+#define LAUNCH(list, per, st, KERN, ...)                                                              \
+    do {                                                                                              \
+        if (L.single) {                                                                               \
+            const int n_ = (list).hi - (list).lo;                                                     \
+            if (n_ > 0)                                                                               \
+                KERN<__VA_ARGS__, false><<<(n_ + (per) - 1) / (per), kChunk, 0, st>>>(L.cases, nullptr, \
+                                                                                     (list).lo, (list).hi); \
+        } else if ((list).n) {                                                                        \
+            KERN<__VA_ARGS__, true><<<(list).n, kChunk, 0, st>>>(L.cases, (list).w, 0, 0);            \
+        }                                                                                             \
+    } while (0)
 
-void launch_ts_face(const DevMesh &M, const DevState &S, const DevThermo *th, hipStream_t st) {
-    if (M.nf) k_timestep_face<kNsp><<<grid_for(M.nf), 256, 0, st>>>(M, S, th);
+void launch_ts_face(const LaunchSet &L, hipStream_t st) { LAUNCH(L.face, kChunk, st, k_timestep_face, kNsp); }
+
+void launch_ts_cell(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_timestep_cell, kNsp); }
+
+void launch_ghost(const LaunchSet &L, hipStream_t st) { LAUNCH(L.ghost, kChunk, st, k_ghost, kNsp); }
+
+// grad: gradient; limit: 0 none, 1 Barth limiter always, 2 when the
+// case's limit_ID > 0
+void launch_gradlim(const LaunchSet &L, bool grad, int limit, hipStream_t st) {
+    if (grad && limit == 2) LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, true, LIM_CASE);
+    else if (grad && limit == 1) LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, true, LIM_ALWAYS);
+    else if (grad) LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, true, LIM_NONE);
+    else if (limit == 1) LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, false, LIM_ALWAYS);
 }
 
-void launch_ts_cell(const DevMesh &M, const DevState &S, const DevParams *par, hipStream_t st) {
-    k_timestep_cell<<<grid_for(M.ne), 256, 0, st>>>(M, S, par);
+void launch_step_begin(const LaunchSet &L, hipStream_t st) {
+    k_step_begin<<<(L.ncases + 63) / 64, 64, 0, st>>>(L.cases, L.ncases);
 }
 
-void launch_ghost(const DevMesh &M, const DevState &S, const DevThermo *th, const DevParams *par, hipStream_t st) {
-    if (M.ng) k_ghost<kNsp><<<grid_for(M.ng), 256, 0, st>>>(M, S, th, par);
+void launch_flux(const LaunchSet &L, bool ts, hipStream_t st) {
+    if (ts) LAUNCH(L.face, kChunk, st, k_flux, kNsp, true);
+    else LAUNCH(L.face, kChunk, st, k_flux, kNsp, false);
 }
 
-void launch_gradlim(const DevMesh &M, const DevState &S, const DevThermo *th, const DevParams *par, bool grad,
-                    bool limit, hipStream_t st) {
-    if (grad && limit) k_gradlim<kNsp, true, true><<<grid_for(M.ne), 256, 0, st>>>(M, S, th, par);
-    else if (grad) k_gradlim<kNsp, true, false><<<grid_for(M.ne), 256, 0, st>>>(M, S, th, par);
-    else if (limit) k_gradlim<kNsp, false, true><<<grid_for(M.ne), 256, 0, st>>>(M, S, th, par);
+void launch_residual(const LaunchSet &L, bool ts, hipStream_t st) {
+    if (ts) LAUNCH(L.cell, kChunk, st, k_residual, kNsp, true);
+    else LAUNCH(L.cell, kChunk, st, k_residual, kNsp, false);
 }
 
-void launch_step_begin(const DevState &S, hipStream_t st) { k_step_begin<<<1, 64, 0, st>>>(S.err); }
-
-void launch_flux(const DevMesh &M, const DevState &S, const DevThermo *th, const DevParams *par, bool ts,
-                 hipStream_t st) {
-    if (ts) k_flux<kNsp, true><<<grid_for(M.nf), 256, 0, st>>>(M, S, th, par);
-    else k_flux<kNsp, false><<<grid_for(M.nf), 256, 0, st>>>(M, S, th, par);
+void launch_lusgs_level(const LaunchSet &L, bool fwd, int l, hipStream_t st) {
+    if (fwd) LAUNCH(L.level[l], kRunChunk, st, k_lusgs, kNsp, true);
+    else LAUNCH(L.level[l], kRunChunk, st, k_lusgs, kNsp, false);
 }
 
-void launch_residual(const DevMesh &M, const DevState &S, const DevThermo *th, const DevParams *par, bool ts,
-                     hipStream_t st) {
-    if (ts) k_residual<kNsp, true><<<grid_for(M.ne), 256, 0, st>>>(M, S, th, par);
-    else k_residual<kNsp, false><<<grid_for(M.ne), 256, 0, st>>>(M, S, th, par);
-}
+void launch_update(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_update, kNsp); }
 
-void launch_lusgs_level(const DevMesh &M, const DevState &S, const DevThermo *th, bool fwd, int lo, int hi, hipStream_t st) {
-    if (hi <= lo) return;
-    const dim3 g = grid_for((hi - lo) * kRunMax);
-    if (fwd) k_lusgs<kNsp, true><<<g, 256, 0, st>>>(M, S, th, lo, hi);
-    else k_lusgs<kNsp, false><<<g, 256, 0, st>>>(M, S, th, lo, hi);
-}
+void launch_prim_to_cons(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_prim_to_cons, kNsp); }
 
-void launch_update(const DevMesh &M, const DevState &S, const DevThermo *th, hipStream_t st) {
-    k_update<kNsp><<<grid_for(M.ne), 256, 0, st>>>(M, S, th);
-}
-
-void launch_prim_to_cons(const DevMesh &M, const DevState &S, const DevThermo *th, hipStream_t st) {
-    k_prim_to_cons<kNsp><<<grid_for(M.ne), 256, 0, st>>>(M, S, th);
-}
-
-void launch_resnorm(const DevMesh &M, const DevState &S, double *partial, int nblk, double inv_volume, double *out,
-                    hipStream_t st) {
-    k_resnorm_partial<<<nblk, 256, 0, st>>>(M, S, partial);
-    k_resnorm_final<<<1, 64, 0, st>>>(partial, nblk, inv_volume, out);
+void launch_resnorm(const LaunchSet &L, hipStream_t st) {
+    LAUNCH(L.cell, kChunk, st, k_resnorm_partial, kNsp);
+    k_resnorm_final<<<L.ncases, 256, 0, st>>>(L.cases, L.resout);
 }
 
 }  // namespace rc

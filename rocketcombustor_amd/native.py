@@ -72,6 +72,12 @@ SIGNATURES = {
     "rc_set_timing": (C.c_int, [_VP, C.c_int]),
     "rc_get_timing": (C.c_int, [_VP, _DP]),
     "rc_mesh_order_multicolor": (C.c_int, [C.c_int, _IP, C.c_int, _IP, _IP]),
+    "rc_batch_create": (C.c_int, [C.c_int, P(_VP)]),
+    "rc_batch_destroy": (None, [_VP]),
+    "rc_batch_add": (C.c_int, [_VP, _VP]),
+    "rc_batch_remove": (C.c_int, [_VP, _VP]),
+    "rc_batch_info": (C.c_int, [_VP, P(C.c_longlong)]),
+    "rc_batch_run_steady": (C.c_int, [_VP, C.c_int, C.c_int, _DP, _IP, _IP]),
 }
 
 

@@ -316,3 +316,38 @@ def write_reference_case(case_dir: str, case: Case, nstep_max: Optional[int] = N
                 g.write(f.read())
     else:
         deck.write_transport_placeholders(case_dir, case.mech.nsp)
+
+
+@dataclasses.dataclass
+class CaseSpec:
+    """Parameters of one sampled nozzle/chamber case."""
+    nx: int
+    ny: int
+    p0: float
+    t0: float
+    phi: float
+    throat_ratio: float
+
+    @property
+    def cells(self) -> int:
+        return 2 * self.nx * self.ny
+
+    def build(self, **kw) -> Case:
+        g = NozzleGeometry(h_throat=self.throat_ratio * NozzleGeometry.h_chamber)
+        return nozzle_case(self.nx, self.ny, p0=self.p0, t0=self.t0, phi=self.phi, geom=g, **kw)
+
+
+def sample_cases(n: int, seed: int = 3, cells_lo: float = 5e3, cells_hi: float = 8e4, first: int = 0) -> list:
+    """Cases ``first .. first+n-1`` of the seeded family of independent
+    nozzle/chamber cases: cell count log-uniform in [cells_lo, cells_hi]
+    (mapped grid with nx = 5 ny), chamber pressure U[1, 10] MPa, T0
+    U[2500, 3500] K, equivalence ratio U[0.7, 1.3], throat-to-chamber
+    height ratio U[0.3, 0.5]; five SplitMix64 draws per case."""
+    out = []
+    for k in range(first, first + n):
+        u = uniform01(seed, 5 * k + np.arange(5))
+        cells = math.exp(math.log(cells_lo) + u[0] * (math.log(cells_hi) - math.log(cells_lo)))
+        ny = max(2, int(round(math.sqrt(cells / 10.0))))
+        out.append(CaseSpec(nx=5 * ny, ny=ny, p0=float(1e6 + 9e6 * u[1]), t0=float(2500.0 + 1000.0 * u[2]),
+                            phi=float(0.7 + 0.6 * u[3]), throat_ratio=float(0.3 + 0.2 * u[4])))
+    return out

@@ -16,6 +16,9 @@ methods keep the reference's names and call sequence:
     residual()          Residual/step_residual  src/residual.f90:4-21,61-79
 
 Errors where the reference stops raise :class:`native.RcError`.
+
+:class:`SteadyBatch` steps many resident cases with the same launches (no
+reference counterpart); each member computes exactly what it computes alone.
 """
 from __future__ import annotations
 
@@ -162,3 +165,64 @@ class SteadyCase:
         out = {k: t[i] / n for i, k in enumerate(self.TIMING_KEYS)}
         out["steps"] = int(t[6])
         return out
+
+
+class SteadyBatch:
+    """Resident :class:`SteadyCase` objects of one device stepped together:
+    every kernel launch of a step covers all members.  Members can be added
+    and removed between calls (continuous batching); a member's results are
+    bitwise the same alone and in any batch."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().rc_batch_create(int(device), C.byref(h)), None)
+        self.h = h
+        self.members = []
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().rc_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, case: SteadyCase) -> None:
+        check(lib().rc_batch_add(self.h, case.h), None)
+        self.members.append(case)
+
+    def remove(self, case: SteadyCase) -> None:
+        check(lib().rc_batch_remove(self.h, case.h), None)
+        self.members.remove(case)
+
+    def info(self) -> dict:
+        v = (C.c_longlong * 3)()
+        check(lib().rc_batch_info(self.h, v), None)
+        return {"members": int(v[0]), "cells": int(v[1]), "levels": int(v[2])}
+
+    def run_steady(self, nsteps: int, every: int = 0, history: bool = True) -> Optional[np.ndarray]:
+        """``nsteps`` steps of every member; with ``history`` the step_residual
+        norms at step 1 and every ``every`` steps, shape (nreports, members,
+        Nvar+1).  A member's device error raises RcError naming the member."""
+        n = len(self.members)
+        every = every if every > 0 else nsteps + 1
+        nrep = (1 if nsteps >= 1 else 0) + sum(1 for s in range(2, nsteps + 1) if s % every == 0)
+        buf = np.zeros((max(nrep, 1), max(n, 1), NVAR + 1)) if history else None
+        done, bad = C.c_int(0), C.c_int(-1)
+        rc = lib().rc_batch_run_steady(self.h, int(nsteps), int(every), dptr(buf) if history else None,
+                                       C.byref(done), C.byref(bad))
+        if rc != 0:
+            err = native.RcError(rc, lib().rc_last_error(None).decode())
+            err.done, err.member = int(done.value), int(bad.value)
+            raise err
+        return buf[:nrep, :n] if history else None

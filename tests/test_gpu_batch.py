@@ -1,0 +1,124 @@
+"""Batches of resident cases (rc_batch_*): every member's results are bitwise
+the same alone and in a batch, whatever the other members are, including
+when members join and leave between calls, and a member's error leaves the
+others intact."""
+import sys
+
+import numpy as np
+import pytest
+
+from rocketcombustor_amd import native, nozzle
+from rocketcombustor_amd.solver import SteadyBatch, SteadyCase
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg: str) -> None:
+    sys.__stderr__.write("[gpu-test] " + msg + "\n")
+    sys.__stderr__.flush()
+
+
+def mixed_cases():
+    """Different sizes, orders, inlet states, throat ratios, CFL, outlet
+    branch and combustion switch (the limiter switch is mixed in the error
+    test: without the limiter these nozzles stop within a few steps)."""
+    out = [nozzle.nozzle_case(24, 6),
+           nozzle.nozzle_case(36, 9, p0=8e6, t0=3400.0, phi=1.2),
+           nozzle.nozzle_case(40, 10, combustion=False, cfl=3.0),
+           nozzle.nozzle_case(30, 8, p_out_ratio=0.9, subsonic_exit=True)]
+    out += [s.build() for s in nozzle.sample_cases(2, cells_lo=800, cells_hi=3000)]
+    for k in (1, 4):
+        perm, _ = native.order_multicolor(out[k].mesh.cells)
+        out[k] = out[k].permuted(perm)
+    return out
+
+
+def alone(case, nsteps, every):
+    with SteadyCase(case) as g:
+        h = g.run_steady(nsteps, every=every)
+        return h, g.primitive(), g.increment()
+
+
+def test_batch_equals_alone():
+    cases = mixed_cases()
+    n, every = 60, 10
+    announce("batch of %d mixed cases, %d steps" % (len(cases), n))
+    ref = [alone(c, n, every) for c in cases]
+    gs = [SteadyCase(c) for c in cases]
+    with SteadyBatch() as b:
+        for g in gs:
+            b.add(g)
+        info = b.info()
+        assert info["members"] == len(cases) and info["cells"] == sum(c.mesh.ne for c in cases)
+        hist = b.run_steady(n, every=every)
+        for k, g in enumerate(gs):
+            assert np.array_equal(hist[:, k], ref[k][0]), k
+            assert np.array_equal(g.primitive(), ref[k][1]), k
+            assert np.array_equal(g.increment(), ref[k][2]), k
+    for g in gs:
+        g.close()
+
+
+def test_continuous_batching_join_and_leave():
+    """A, B from the start; C joins after 30 steps; A leaves after 50; each
+    ends bitwise where it ends alone after the same number of steps."""
+    ca, cb, cc = mixed_cases()[:3]
+    announce("continuous batching join/leave")
+    ga, gb, gc = SteadyCase(ca), SteadyCase(cb), SteadyCase(cc)
+    with SteadyBatch() as b:
+        b.add(ga)
+        b.add(gb)
+        b.run_steady(30, history=False)
+        b.add(gc)
+        b.run_steady(20, history=False)
+        b.remove(ga)
+        b.run_steady(25, history=False)
+        assert b.info()["members"] == 2
+    for g, c, steps in ((ga, ca, 50), (gb, cb, 75), (gc, cc, 45)):
+        assert np.array_equal(g.primitive(), alone(c, steps, 0)[1])
+        g.close()
+
+
+def test_member_error_is_isolated():
+    """The no-limiter case stops at step 5 (as the reference does); the
+    error names the member, and the healthy member continues to the same
+    state as alone."""
+    bad = nozzle.nozzle_case(24, 6, limit_id=0)
+    good = nozzle.nozzle_case(30, 8)
+    announce("batch member error")
+    gbad, ggood = SteadyCase(bad), SteadyCase(good)
+    with SteadyBatch() as b:
+        b.add(ggood)
+        b.add(gbad)
+        with pytest.raises(native.RcError) as ei:
+            b.run_steady(20, every=1)
+        assert "case 1" in str(ei.value)
+        done = ei.value.done
+        assert ei.value.member == 1 and 1 <= done < 20
+        b.remove(gbad)
+        b.run_steady(20 - done, history=False)
+    assert np.array_equal(ggood.primitive(), alone(good, 20, 0)[1])
+    gbad.close()
+    ggood.close()
+
+
+def test_batch_graph_replay_matches_eager(monkeypatch):
+    """The batch step as a replayed graph and as eager launches agree bitwise
+    (several replays with host work between them)."""
+    cases = mixed_cases()[:3]
+    announce("batch graph vs eager")
+    outs = []
+    for no_graph in ("0", "1"):
+        monkeypatch.setenv("RC_NO_GRAPH", no_graph)
+        gs = [SteadyCase(c) for c in cases]
+        with SteadyBatch() as b:
+            for g in gs:
+                b.add(g)
+            for _ in range(4):
+                b.run_steady(7, history=False)
+                _ = [g.primitive() for g in gs]
+            outs.append([g.primitive() for g in gs])
+        for g in gs:
+            g.close()
+    for a, e in zip(*outs):
+        assert np.array_equal(a, e)

@@ -3,7 +3,7 @@ vectors and with the C oracle.
 
 Bars (FP64): single evaluations of each operator <=1e-12 relative per
 variable; short trajectories <=1e-10; the 8000-step converged fields <=1e-8
-(the north-star bar for converged fields is 1e-6).  Every GPU step is
+(the required bar for converged fields is 1e-6).  Every GPU step is
 announced on stderr first so a failing run names its case.
 """
 import sys
