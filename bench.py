@@ -103,6 +103,30 @@ def cpu_baseline(case, nsteps: int) -> dict:
             "sample": "%d steps of the same %d-cell case, serial C restatement" % (nsteps, ne)}
 
 
+SERVE_RATE = 20.0     # requests per second
+SERVE_ORDERS = 2.0    # converged: RoRes dropped two orders from its first step
+SERVE_ROUND = 200     # steps between convergence checks
+
+
+def serving_run(n: int, plain: bool) -> dict:
+    """Continuous-batching serving of n seeded requests (Poisson arrivals,
+    5k-400k cells, parameters of the batched family), and optionally the
+    same requests served one at a time through the single-case path."""
+    from rocketcombustor_amd import serving
+    reqs = serving.poisson_requests(n, rate=SERVE_RATE)
+    srv = serving.Server(round_steps=SERVE_ROUND, orders=SERVE_ORDERS, max_steps=20000, workers=4)
+    rep = srv.run(reqs, time_limit=600.0)
+    out = {"requests": n, "rate_req_per_s": SERVE_RATE, "cells": "log-uniform 5e3-4e5",
+           "converged_when": "RoRes <= 1e-2 x its step-1 value (max 20000 steps)", "batched": rep}
+    if plain:
+        seq = serving.serve_sequential(serving.poisson_requests(n, rate=SERVE_RATE), round_steps=SERVE_ROUND,
+                                       orders=SERVE_ORDERS, max_steps=20000, time_limit=900.0)
+        out["plain"] = seq
+        if seq["cases_per_hour"] > 0:
+            out["speedup_cases_per_hour"] = rep["cases_per_hour"] / seq["cases_per_hour"]
+    return out
+
+
 def copy_calibration(mib: int) -> int:
     """Three device-to-device copies of `mib` MiB through the HIP runtime (its
     copy kernel), as the reference point for the HBM byte counters."""
@@ -136,6 +160,10 @@ def main() -> int:
     ap.add_argument("--ny", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve", type=int, default=24, metavar="N",
+                    help="requests of the serving run (Poisson 20/s, 5k-400k cells); 0 skips it")
+    ap.add_argument("--serve-plain", type=int, default=1,
+                    help="also serve the same requests one case at a time (plain path) for the speedup")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device three times (a plain copy kernel of "
                          "known size, to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -210,6 +238,8 @@ def main() -> int:
     g.close()
     if args.copy_calibration > 0:
         out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
+    if args.serve > 0:
+        out["serving"] = serving_run(args.serve, args.serve_plain)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
     print(json.dumps(out), flush=True)
