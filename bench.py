@@ -127,6 +127,46 @@ def serving_run(n: int, plain: bool) -> dict:
     return out
 
 
+def batched_run(n: int, steps: int) -> dict:
+    """n seeded independent cases (cells log-uniform 5k-80k, chamber
+    pressure, T0, equivalence and throat ratio sampled): `steps` steady steps
+    of all of them as one batch (graph replay, one launch per kernel for all
+    cases), then the same steps case by case through the single-case path."""
+    import concurrent.futures as cf
+    from rocketcombustor_amd import nozzle, serving
+    from rocketcombustor_amd.solver import SteadyBatch
+    specs = nozzle.sample_cases(n, seed=3)
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=8) as pool:
+        cases = list(pool.map(serving.build_case, specs))
+    t_build = time.perf_counter() - t0
+    cells = sum(g.ne for g in cases)
+    nbytes = sum(g.device_bytes for g in cases)
+    with SteadyBatch() as b:
+        for g in cases:
+            b.add(g)
+        b.run_steady(5, history=False)          # warm-up (graph capture)
+        t0 = time.perf_counter()
+        b.run_steady(steps, history=False)
+        t_batch = time.perf_counter() - t0
+        nlev = b.info()["levels"]
+        for g in cases:
+            b.remove(g)
+    for g in cases:
+        g.run_steady(2, history=False)          # warm-up of each case's own graph
+    t0 = time.perf_counter()
+    for g in cases:
+        g.run_steady(steps, history=False)
+    t_plain = time.perf_counter() - t0
+    for g in cases:
+        g.close()
+    return {"cases": n, "cells": cells, "device_bytes": nbytes, "steps": steps, "levels": nlev,
+            "host_build_s": t_build,
+            "batched_cell_updates_per_s": cells * steps / t_batch, "batched_ms_per_step": 1e3 * t_batch / steps,
+            "plain_cell_updates_per_s": cells * steps / t_plain, "plain_ms_per_step_all": 1e3 * t_plain / steps,
+            "speedup": t_plain / t_batch}
+
+
 def copy_calibration(mib: int) -> int:
     """Three device-to-device copies of `mib` MiB through the HIP runtime (its
     copy kernel), as the reference point for the HBM byte counters."""
@@ -164,6 +204,10 @@ def main() -> int:
                     help="requests of the serving run (Poisson 20/s, 5k-400k cells); 0 skips it")
     ap.add_argument("--serve-plain", type=int, default=1,
                     help="also serve the same requests one case at a time (plain path) for the speedup")
+    ap.add_argument("--batched", type=int, default=0, metavar="N",
+                    help="also time N independent sampled cases (5k-80k cells) stepped as one batch "
+                         "against the same cases stepped one at a time")
+    ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device three times (a plain copy kernel of "
                          "known size, to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -240,6 +284,8 @@ def main() -> int:
         out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
     if args.serve > 0:
         out["serving"] = serving_run(args.serve, args.serve_plain)
+    if args.batched > 0:
+        out["batched"] = batched_run(args.batched, args.batched_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -51,6 +51,7 @@ class Request:
     res: float = math.nan           # RoRes at the last report
     device_bytes: int = 0
     cells: int = 0
+    error: str = ""
 
     @property
     def latency(self) -> float:
@@ -145,9 +146,9 @@ class Server:
                     r = building.pop(fut)
                     try:
                         case = fut.result()
-                    except RcError:
+                    except RcError as e:
                         reserved -= r.device_bytes
-                        r.status, r.finished = "failed", self.clock() - t0
+                        r.status, r.finished, r.error = "failed", self.clock() - t0, str(e)
                         continue
                     reserved += case.device_bytes - r.device_bytes
                     r.device_bytes, r.cells = int(case.device_bytes), int(case.ne)
@@ -163,11 +164,11 @@ class Server:
                     continue
                 # one round of batched steps
                 n = self.round_steps
-                done_steps, failed = n, None
+                done_steps, failed, err = n, None, ""
                 try:
                     hist = self.batch.run_steady(n, every=n)
                 except RcError as e:
-                    hist, done_steps, failed = None, getattr(e, "done", 0), getattr(e, "member", -1)
+                    hist, done_steps, failed, err = None, getattr(e, "done", 0), getattr(e, "member", -1), str(e)
                 rounds += 1
                 members_sum += len(active)
                 cell_steps += done_steps * sum(r.cells for r, _ in active)
@@ -176,7 +177,7 @@ class Server:
                 for k, (r, case) in enumerate(active):
                     r.steps += done_steps
                     if k == failed:
-                        r.status = "failed"
+                        r.status, r.error = "failed", err
                         leave.append(k)
                         continue
                     if hist is None:
@@ -232,6 +233,8 @@ class Server:
             "mean_members": members_sum / rounds if rounds else 0.0,
             "cell_updates_per_s": cell_steps / elapsed if elapsed > 0 else 0.0,
             "steps_to_converge_mean": float(np.mean([r.steps for r in conv])) if conv else math.nan,
+            "failures": [{"rid": r.rid, "cells": r.cells, "steps": r.steps, "error": r.error}
+                         for r in reqs if r.status == "failed"],
         }
 
 
@@ -269,8 +272,8 @@ def serve_sequential(requests: List[Request], round_steps: int = 200, orders: fl
                     if r.steps >= max_steps:
                         r.status = "capped"
                         break
-            except RcError:
-                r.status = "failed"
+            except RcError as e:
+                r.status, r.error = "failed", str(e)
         r.finished = clock() - t0
     elapsed = clock() - t0
     srv = Server.__new__(Server)
