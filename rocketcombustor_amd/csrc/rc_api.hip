@@ -21,7 +21,7 @@ void launch_gradlim(const LaunchSet &, bool, int, hipStream_t);
 void launch_flux(const LaunchSet &, bool, hipStream_t);
 void launch_residual(const LaunchSet &, bool, hipStream_t);
 void launch_step_begin(const LaunchSet &, hipStream_t);
-void launch_lusgs_level(const LaunchSet &, bool, int, hipStream_t);
+void launch_lusgs_level(const LaunchSet &, bool, int, bool, hipStream_t);
 void launch_update(const LaunchSet &, hipStream_t);
 void launch_prim_to_cons(const LaunchSet &, hipStream_t);
 void launch_resnorm(const LaunchSet &, hipStream_t);
@@ -389,8 +389,8 @@ int drain_timing(rc_case *h) {
 }
 // one steady step as the fused kernel pipeline:
 //   step counter, gradient+limiter (ghost states formed in-kernel), flux
-//   (+ TimeStep face part), residual + source (+ TimeStep cell part),
-//   LU-SGS forward wavefronts, backward wavefronts, update.
+//   (+ TimeStep face part), LU-SGS forward levels that first form their
+//   cells' residual + source (+ TimeStep cell part), backward levels, update.
 void enqueue_step_kernels(rc_case *h) {
     const LaunchSet &L = h->L;
     launch_step_begin(L, h->stream);
@@ -399,11 +399,10 @@ void enqueue_step_kernels(rc_case *h) {
     mark(h, 1);
     launch_flux(L, true, h->stream);
     mark(h, 2);
-    launch_residual(L, true, h->stream);
-    mark(h, 3);
-    for (int l = 0; l < h->nlev; ++l) launch_lusgs_level(L, true, l, h->stream);
+    mark(h, 3);                 // the residual is formed in the forward sweep
+    for (int l = 0; l < h->nlev; ++l) launch_lusgs_level(L, true, l, true, h->stream);
     mark(h, 4);
-    for (int l = h->nlev - 1; l >= 0; --l) launch_lusgs_level(L, false, l, h->stream);
+    for (int l = h->nlev - 1; l >= 0; --l) launch_lusgs_level(L, false, l, false, h->stream);
     mark(h, 5);
     launch_update(L, h->stream);
 }
@@ -575,9 +574,8 @@ void enqueue_batch_step(rc_batch *b) {
     launch_step_begin(L, b->stream);
     launch_gradlim(L, true, 2, b->stream);
     launch_flux(L, true, b->stream);
-    launch_residual(L, true, b->stream);
-    for (int l = 0; l < b->nlev; ++l) launch_lusgs_level(L, true, l, b->stream);
-    for (int l = b->nlev - 1; l >= 0; --l) launch_lusgs_level(L, false, l, b->stream);
+    for (int l = 0; l < b->nlev; ++l) launch_lusgs_level(L, true, l, true, b->stream);
+    for (int l = b->nlev - 1; l >= 0; --l) launch_lusgs_level(L, false, l, false, b->stream);
     launch_update(L, b->stream);
 }
 
@@ -736,8 +734,8 @@ int rc_flux_sourceterm(rc_case *h) {
 int rc_lusgs_steady(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
-    for (int l = 0; l < h->nlev; ++l) launch_lusgs_level(h->L, true, l, h->stream);
-    for (int l = h->nlev - 1; l >= 0; --l) launch_lusgs_level(h->L, false, l, h->stream);
+    for (int l = 0; l < h->nlev; ++l) launch_lusgs_level(h->L, true, l, false, h->stream);
+    for (int l = h->nlev - 1; l >= 0; --l) launch_lusgs_level(h->L, false, l, false, h->stream);
     launch_update(h->L, h->stream);
     return sync_check(h);
 }

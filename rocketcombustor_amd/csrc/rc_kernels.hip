@@ -82,28 +82,35 @@ __global__ void __launch_bounds__(256) k_timestep_face(ITEM_ARGS) {
 
 // TimeStep cell loop (TimeStep.f90:94-96) and the LU-SGS diagonal
 // dd(i) = sum 0.5*alpaF + vol/step (Time_LUSGS_steady.f90:46-53)
-__device__ __forceinline__ void cell_step_dd(const GMesh &M, const GState &S, GParams &P, int i) {
+// (stored when `store`; the face alpaF values and dd are also returned)
+__device__ __forceinline__ double cell_step_dd(const GMesh &M, const GState &S, GParams &P, int i, double *al,
+                                               bool store = true) {
     const double cfl = P.cfl;
-    double a = 0.0, d = 0.0, al[3];
+    double a = 0.0, d = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int cf = M.cface[k * M.sc + i];
         al[k] = ALPHA_(cf >= 0 ? cf : ~cf);
         a = a + al[k];
-        S.lu_af[k * M.sc + i] = al[k];
+        if (store) S.lu_af[k * M.sc + i] = al[k];
     }
     const double v = M.vol[i];
     const double st = cfl * v / a;
-    S.step[i] = st;
 #pragma unroll
     for (int k = 0; k < 3; ++k) d = d + 0.5 * al[k];
-    S.dd[i] = d + v / st;
+    d = d + v / st;
+    if (store) {
+        S.step[i] = st;
+        S.dd[i] = d;
+    }
+    return d;
 }
 
 template <int NSP, bool B>
 __global__ void __launch_bounds__(256) k_timestep_cell(ITEM_ARGS) {
     ITEM_PROLOGUE(i);
-    cell_step_dd(M, S, PAR, i);
+    double al[3];
+    cell_step_dd(M, S, PAR, i, al);
 }
 
 // advances each case's device step counter used in error reports
@@ -359,16 +366,12 @@ __global__ void __launch_bounds__(256) k_flux(ITEM_ARGS) {
         F_(kNFlow + k, f) = 0.5 * (ruln * yl[k] + rurn * yr[k] - coex * (rr * yr[k] - rl * yl[k]));
 }
 
-// residual assembly of flux_sourceterm (fvm2d.f90:201-219): hWA = sum of
-// -F (owner) / +F (neighbour) over the cell's faces in face order, then the
-// chemical source vol*omega of sourceterm (sourceterm.f90:2-70).
-// TS: also the cell part of TimeStep (step, dd) from the same faces.
-template <int NSP, bool TS, bool B>
-__global__ void __launch_bounds__(256) k_residual(ITEM_ARGS) {
-    ITEM_PROLOGUE(i);
-    GParams &P = PAR;
-    if (TS) cell_step_dd(M, S, P, i);
-    double h[NV];
+// residual assembly of flux_sourceterm (fvm2d.f90:201-219) of cell i: h =
+// sum of -F (owner) / +F (neighbour) over the cell's faces in face order,
+// then the chemical source vol*omega of sourceterm (sourceterm.f90:2-70)
+template <int NSP>
+__device__ __forceinline__ void cell_residual(const GMesh &M, const GState &S, GThermo &th, GParams &P, int i,
+                                              double *h) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) h[k] = 0.0;
 #pragma unroll
@@ -384,7 +387,6 @@ __global__ void __launch_bounds__(256) k_residual(ITEM_ARGS) {
         }
     }
     if (P.combustion) {
-        GThermo &th = TH;
         const double t = PA_(3, i);
         double y[NSP], s = 0.0;
 #pragma unroll
@@ -400,6 +402,19 @@ __global__ void __launch_bounds__(256) k_residual(ITEM_ARGS) {
 #pragma unroll
         for (int k = 0; k < NSP - 1; ++k) h[kNFlow + k] = h[kNFlow + k] + v * w[k];
     }
+}
+
+// the residual of flux_sourceterm into hWA; TS: also the cell part of
+// TimeStep (step, dd) from the same faces
+template <int NSP, bool TS, bool B>
+__global__ void __launch_bounds__(256) k_residual(ITEM_ARGS) {
+    ITEM_PROLOGUE(i);
+    if (TS) {
+        double al[3];
+        cell_step_dd(M, S, PAR, i, al);
+    }
+    double h[NV];
+    cell_residual<NSP>(M, S, TH, PAR, i, h);
 #pragma unroll
     for (int k = 0; k < NV; ++k) HWA_(k, i) = h[k];
 }
@@ -492,7 +507,10 @@ __device__ __forceinline__ void lusgs_term(GThermo &th, const LuNb &n, const dou
 // received by shuffles in the steps before, and forms its increment.  Terms
 // enter dflux in face order; a face that does not take part adds +0.0, which
 // leaves the reference's sums unchanged.
-template <int NSP, bool FWD, bool B>
+// RES (forward sweep of the fused step): each lane first forms its cell's
+// TimeStep cell part and residual (what k_residual stores) and sweeps with
+// them from registers.
+template <int NSP, bool FWD, bool RES, bool B>
 __global__ void __launch_bounds__(256) k_lusgs(ITEM_ARGS) {
     constexpr int R = kRunMax;
     const RC_G CaseDev *Cp_;
@@ -511,6 +529,7 @@ __global__ void __launch_bounds__(256) k_lusgs(ITEM_ARGS) {
     const GMesh M = gview(C.M);
     const GState S = gview(C.S);
     GThermo &TH = *gp(C.th);
+    GParams &PAR = *gp(C.par);
     const int lane = threadIdx.x % R;
     GThermo &th = TH;
     const int sc = M.sc;
@@ -526,12 +545,18 @@ __global__ void __launch_bounds__(256) k_lusgs(ITEM_ARGS) {
         nb[k] = M.lu_nb[k * sc + i];
         nx[k] = M.lu_n[(k * 2) * sc + i];
         ny[k] = M.lu_n[(k * 2 + 1) * sc + i];
-        af[k] = S.lu_af[k * sc + i];
     }
-    const double d = S.dd[i];
-    double rhs[NV];
+    double d, rhs[NV];
+    if (RES) {
+        d = cell_step_dd(M, S, PAR, i, af, live);
+        cell_residual<NSP>(M, S, th, PAR, i, rhs);
+    } else {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) rhs[v] = HWA_(v, i);
+        for (int k = 0; k < 3; ++k) af[k] = S.lu_af[k * sc + i];
+        d = S.dd[i];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) rhs[v] = HWA_(v, i);
+    }
     bool use[3];
     int qs[3];         // sweep step of a neighbour inside the run, else -1
     LuNb nbr[3];
@@ -781,9 +806,11 @@ void launch_residual(const LaunchSet &L, bool ts, hipStream_t st) {
     else LAUNCH(L.cell, kChunk, st, k_residual, kNsp, false);
 }
 
-void launch_lusgs_level(const LaunchSet &L, bool fwd, int l, hipStream_t st) {
-    if (fwd) LAUNCH(L.level[l], kRunChunk, st, k_lusgs, kNsp, true);
-    else LAUNCH(L.level[l], kRunChunk, st, k_lusgs, kNsp, false);
+// res: forward levels that also form their cells' residual (the fused step)
+void launch_lusgs_level(const LaunchSet &L, bool fwd, int l, bool res, hipStream_t st) {
+    if (fwd && res) LAUNCH(L.level[l], kRunChunk, st, k_lusgs, kNsp, true, true);
+    else if (fwd) LAUNCH(L.level[l], kRunChunk, st, k_lusgs, kNsp, true, false);
+    else LAUNCH(L.level[l], kRunChunk, st, k_lusgs, kNsp, false, false);
 }
 
 void launch_update(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_update, kNsp); }

@@ -166,3 +166,25 @@ def test_graph_replay_matches_eager():
         for _ in range(5):
             eager.lu_sgs_stage()
         assert np.array_equal(graph.conserved(), eager.conserved())
+
+
+def test_operator_calls_after_fused_steps():
+    """The fused step leaves the cell-average cache of the pre-update state;
+    operator-level calls made right after it refresh the cache first, so
+    LU_SGS_steady after two lu_sgs_stage calls matches the oracle."""
+    case = nozzle.nozzle_case(30, 8)
+    perm, _ = native.order_multicolor(case.mesh.cells)
+    case = case.permuted(perm)
+    o = oracle.OracleCase(case)
+    announce("operator calls after fused steps NE=%d" % case.mesh.ne)
+    with SteadyCase(case) as g:
+        for _ in range(2):
+            g.lu_sgs_stage()
+            o.stage()
+        g.lu_sgs_steady()
+        o.lusgs_steady()
+        assert relerr(g.increment(), o.get("hWA"), -1) < 1e-12
+        assert relerr(g.primitive(), o.get("PA"), -1) < 1e-12
+        g.flux_sourceterm()
+        o.flux_sourceterm()
+        assert relerr(g.increment(), o.get("hWA"), -1) < 1e-12
