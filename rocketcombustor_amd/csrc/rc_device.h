@@ -174,8 +174,8 @@ __device__ __forceinline__ double powi_d(double a, int b) {
 // ComputeGasParameter, thermo_property.f90:78-112.  Returns false when T is
 // outside [t_low, t_high] (the reference stops there); the high-range
 // coefficients are then used so that the arithmetic stays finite.
-template <int NSP>
-__device__ __forceinline__ bool gas_param(GThermo &th, const double *y, double t,
+template <int NSP, class TT>
+__device__ __forceinline__ bool gas_param(TT &th, const double *y, double t,
                                           double &cp, double &gam, double &r) {
     double basis[7];
 #pragma unroll
@@ -183,7 +183,7 @@ __device__ __forceinline__ bool gas_param(GThermo &th, const double *y, double t
     bool ok = true;
     const bool low = (th.t_low <= t && t <= th.t_crit);
     if (!low && !(th.t_crit < t && t <= th.t_high)) ok = false;
-    const RC_G double(*coef)[7] = low ? th.low : th.high;
+    const auto *coef = low ? th.low : th.high;
     double c = 0.0, rr = 0.0;
     double cps[NSP];
 #pragma unroll
@@ -204,8 +204,8 @@ __device__ __forceinline__ bool gas_param(GThermo &th, const double *y, double t
 }
 
 // compute_enthalpy, thermo_property.f90:114-144
-template <int NSP>
-__device__ __forceinline__ bool enthalpy(GThermo &th, double t, double *h) {
+template <int NSP, class TT>
+__device__ __forceinline__ bool enthalpy(TT &th, double t, double *h) {
     double basis[7];
     basis[0] = -powi_d(t, -2);
     basis[1] = log(t) / t;
@@ -214,8 +214,8 @@ __device__ __forceinline__ bool enthalpy(GThermo &th, double t, double *h) {
     bool ok = true;
     const bool low = (th.t_low <= t && t <= th.t_crit);
     if (!low && !(th.t_crit < t && t <= th.t_high)) ok = false;
-    const RC_G double(*coef)[7] = low ? th.low : th.high;
-    const RC_G double *b1 = low ? th.low_b1 : th.high_b1;
+    const auto *coef = low ? th.low : th.high;
+    const auto *b1 = low ? th.low_b1 : th.high_b1;
 #pragma unroll
     for (int i = 0; i < NSP; ++i) {
         double s = 0.0;
@@ -227,8 +227,8 @@ __device__ __forceinline__ bool enthalpy(GThermo &th, double t, double *h) {
 }
 
 // Newton_temperature, thermo_property.f90:146-180
-template <int NSP>
-__device__ __forceinline__ bool newton_temperature(GThermo &th, double et, const double *y, double t0,
+template <int NSP, class TT>
+__device__ __forceinline__ bool newton_temperature(TT &th, double et, const double *y, double t0,
                                                    double &t, double &cp, double &gam, double &r) {
     t = t0;
     bool ok = true;
@@ -253,8 +253,8 @@ __device__ __forceinline__ bool newton_temperature(GThermo &th, double et, const
 
 // compute_production_rate, chemical_kinetics.f90:109-165 (one reaction;
 // model 0: A exp(-E/RT) [H2]^0.2 [O2]^1.3, model 1: 1e4 [H2] above 710 K)
-template <int NSP>
-__device__ __forceinline__ void production_rate(GThermo &th, double t, double rho, const double *y2, double *w) {
+template <int NSP, class TT>
+__device__ __forceinline__ void production_rate(TT &th, double t, double rho, const double *y2, double *w) {
     double y[NSP];
 #pragma unroll
     for (int k = 0; k < NSP; ++k) y[k] = fmin(1.0, fmax(0.0, y2[k]));

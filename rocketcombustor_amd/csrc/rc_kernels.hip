@@ -47,10 +47,41 @@ constexpr int NV = kNvar;
     GThermo &TH = *gp(C.th);                                            \
     GParams &PAR = *gp(C.par);
 
+// the same with the case's thermo table staged in LDS by the whole block
+// first (the thermo functions then read their coefficients with LDS
+// latency); TH refers to that copy
+#define ITEM_PROLOGUE_LDS(idx)                                          \
+    __shared__ DevThermo th_lds_;                                       \
+    const RC_G CaseDev *Cp_;                                            \
+    int idx, end_;                                                      \
+    if constexpr (B) {                                                  \
+        const int4 w_ = work[blockIdx.x];                               \
+        Cp_ = gp(cases) + w_.x;                                         \
+        idx = w_.y + (int)threadIdx.x;                                  \
+        end_ = w_.z;                                                    \
+    } else {                                                            \
+        Cp_ = gp(cases);                                                \
+        idx = lo + (int)(blockIdx.x * blockDim.x + threadIdx.x);        \
+        end_ = hi;                                                      \
+    }                                                                   \
+    const RC_G CaseDev &C = *Cp_;                                       \
+    {                                                                   \
+        const RC_G double *src_ = (const RC_G double *)gp(C.th);        \
+        double *dst_ = reinterpret_cast<double *>(&th_lds_);            \
+        for (int k_ = threadIdx.x; k_ < (int)(sizeof(DevThermo) / 8); k_ += blockDim.x) dst_[k_] = src_[k_]; \
+    }                                                                   \
+    __syncthreads();                                                    \
+    if (idx >= end_) return;                                            \
+    const GMesh M = gview(C.M);                                         \
+    const GState S = gview(C.S);                                        \
+    const DevThermo &TH = th_lds_;                                      \
+    GParams &PAR = *gp(C.par);
+static_assert(sizeof(DevThermo) % 8 == 0, "thermo table is copied as doubles");
+
 // TimeStep face loop (TimeStep.f90:16-92, inviscid): spectral radius of face
 // f from the cell averages of its two cells (one cell on a boundary)
-template <int NSP>
-__device__ __forceinline__ double face_spectral_radius(const GMesh &M, const GState &S, GThermo &th,
+template <int NSP, class TT>
+__device__ __forceinline__ double face_spectral_radius(const GMesh &M, const GState &S, TT &th,
                                                        int f, int nl, int nr, double dx, double dy) {
     double pp, uu, vv, tt, y[NSP], s = 0.0;
     if (nr >= 0) {
@@ -126,8 +157,8 @@ __global__ void k_step_begin(const CaseDev *__restrict__ cases, int n) {
 // of ghost g mirrored from its host cell h's primitive state pl[]; the
 // host's gamma and density (ComputeGasParameter of its own state) come from
 // the cell-average cache
-template <int NSP>
-__device__ __forceinline__ void ghost_state(const GMesh &M, const GState &S, GThermo &th,
+template <int NSP, class TT>
+__device__ __forceinline__ void ghost_state(const GMesh &M, const GState &S, TT &th,
                                             GParams &P, int g, int h, const double *pl_, double *out) {
     const double pl = pl_[0], ul = pl_[1], vl = pl_[2], tl = pl_[3];
     double yl[NSP], yr[NSP], s = 0.0;
@@ -266,8 +297,8 @@ __global__ void __launch_bounds__(256) k_gradlim(ITEM_ARGS) {
 // TimeStep pass: PA does not change between TimeStep and flux_lax).
 template <int NSP, bool TS, bool B>
 __global__ void __launch_bounds__(256) k_flux(ITEM_ARGS) {
-    ITEM_PROLOGUE(f);
-    GThermo &th = TH;
+    ITEM_PROLOGUE_LDS(f);
+    const DevThermo &th = TH;
     const int nl = M.fnl[f], nr = M.fnr[f];
     const double dx = M.vecx[f], dy = M.vecx[M.sf + f];
     if (TS) ALPHA_(f) = face_spectral_radius<NSP>(M, S, th, f, nl, nr, dx, dy);
@@ -369,8 +400,8 @@ __global__ void __launch_bounds__(256) k_flux(ITEM_ARGS) {
 // residual assembly of flux_sourceterm (fvm2d.f90:201-219) of cell i: h =
 // sum of -F (owner) / +F (neighbour) over the cell's faces in face order,
 // then the chemical source vol*omega of sourceterm (sourceterm.f90:2-70)
-template <int NSP>
-__device__ __forceinline__ void cell_residual(const GMesh &M, const GState &S, GThermo &th, GParams &P, int i,
+template <int NSP, class TT>
+__device__ __forceinline__ void cell_residual(const GMesh &M, const GState &S, TT &th, GParams &P, int i,
                                               double *h) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) h[k] = 0.0;
@@ -423,8 +454,8 @@ __global__ void __launch_bounds__(256) k_residual(ITEM_ARGS) {
 // source term and ghost update) from one cell's primitive state, written to
 // the per-cell cache; the arithmetic is that of the reference's neighbour
 // term, so the cached values equal recomputing them there
-template <int NSP>
-__device__ __forceinline__ bool write_lucoef(const GMesh &M, const GState &S, GThermo &th, int i,
+template <int NSP, class TT>
+__device__ __forceinline__ bool write_lucoef(const GMesh &M, const GState &S, TT &th, int i,
                                              double pp, double uu, double vv, double tt, const double *yy) {
     double cp, gama, rcp, h[NSP];
     bool ok = gas_param<NSP>(th, yy, tt, cp, gama, rcp);
@@ -468,8 +499,8 @@ __device__ __forceinline__ LuNb load_lunb(const GState &S, int nr) {
 // the neighbour term of both LU-SGS sweeps (Time_LUSGS_steady.f90:88-111):
 // the linearised Rusanov flux change 0.5*(-alpaF*dW + dF(dW).S) of a
 // neighbour's increment dw; the caller adds it to dflux in face order
-template <int NSP>
-__device__ __forceinline__ void lusgs_term(GThermo &th, const LuNb &n, const double *dw, double dx,
+template <int NSP, class TT>
+__device__ __forceinline__ void lusgs_term(TT &th, const LuNb &n, const double *dw, double dx,
                                            double dy, double af, double *c) {
     const double drr = dw[0], dru = dw[1], drv = dw[2], dre = dw[3];
     double dry[NSP], s = 0.0;
@@ -626,8 +657,8 @@ __global__ void __launch_bounds__(256) k_lusgs(ITEM_ARGS) {
 // the new cell-average quantities go to the cache
 template <int NSP, bool B>
 __global__ void __launch_bounds__(256) k_update(ITEM_ARGS) {
-    ITEM_PROLOGUE(i);
-    GThermo &th = TH;
+    ITEM_PROLOGUE_LDS(i);
+    const DevThermo &th = TH;
     double w[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) { w[k] = WA_(k, i) + HWA_(k, i); WA_(k, i) = w[k]; }
