@@ -496,6 +496,26 @@ __device__ __forceinline__ LuNb load_lunb(const GState &S, int nr) {
     return n;
 }
 
+// record k of three (k = 0, 1, 2), selected field by field so that the
+// three stay in registers
+__device__ __forceinline__ double sel3(int k, double a, double b, double c) { return k == 0 ? a : (k == 1 ? b : c); }
+__device__ __forceinline__ LuNb select_lunb(int k, const LuNb *n) {
+    LuNb r;
+    r.rr = sel3(k, n[0].rr, n[1].rr, n[2].rr);
+    r.rou = sel3(k, n[0].rou, n[1].rou, n[2].rou);
+    r.rov = sel3(k, n[0].rov, n[1].rov, n[2].rov);
+    r.roep = sel3(k, n[0].roep, n[1].roep, n[2].roep);
+    r.uu = sel3(k, n[0].uu, n[1].uu, n[2].uu);
+    r.vv = sel3(k, n[0].vv, n[1].vv, n[2].vv);
+    r.tt = sel3(k, n[0].tt, n[1].tt, n[2].tt);
+    r.gama = sel3(k, n[0].gama, n[1].gama, n[2].gama);
+#pragma unroll
+    for (int j = 0; j < kNsp; ++j) r.h[j] = sel3(k, n[0].h[j], n[1].h[j], n[2].h[j]);
+#pragma unroll
+    for (int j = 0; j < kNsp - 1; ++j) r.ry[j] = sel3(k, n[0].ry[j], n[1].ry[j], n[2].ry[j]);
+    return r;
+}
+
 // the neighbour term of both LU-SGS sweeps (Time_LUSGS_steady.f90:88-111):
 // the linearised Rusanov flux change 0.5*(-alpaF*dW + dF(dW).S) of a
 // neighbour's increment dw; the caller adds it to dflux in face order
@@ -613,23 +633,11 @@ __global__ void __launch_bounds__(256) k_lusgs(ITEM_ARGS) {
     double hv[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) hv[v] = 0.0;
-    double hs[R][NV];  // increments of the run's cells, by sweep step
 #pragma unroll
     for (int s = 0; s < R; ++s) {
+        // the lane of step s: all its terms are ready (in-run neighbours were
+        // swept at earlier steps)
         if (mystep == s) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                if (qs[k] >= 0) {
-                    double dw[NV];
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) dw[v] = 0.0;
-#pragma unroll
-                    for (int q = 0; q < s; ++q)
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) dw[v] = qs[k] == q ? hs[q][v] : dw[v];
-                    lusgs_term<NSP>(th, nbr[k], dw, nx[k], ny[k], af[k], c[k]);
-                }
-            }
             double dflux[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) dflux[v] = 0.0;
@@ -641,9 +649,24 @@ __global__ void __launch_bounds__(256) k_lusgs(ITEM_ARGS) {
             for (int v = 0; v < NV; ++v) hv[v] = FWD ? (rhs[v] - dflux[v]) / d : rhs[v] - dflux[v] / d;
         }
         if (s < R - 1) {
+            // the increment of the cell swept at step s, to the whole group
             const int src = FWD ? s : (cnt - 1 - s > 0 ? cnt - 1 - s : 0);
+            double dw[NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) hs[s][v] = __shfl(hv[v], src, R);
+            for (int v = 0; v < NV; ++v) dw[v] = __shfl(hv[v], src, R);
+            // every lane with that cell among its neighbours (one face at
+            // most) forms the term of that face now
+            const int ks = qs[0] == s ? 0 : (qs[1] == s ? 1 : (qs[2] == s ? 2 : -1));
+            if (ks >= 0) {
+                const LuNb nb_s = select_lunb(ks, nbr);
+                double cs[NV];
+                lusgs_term<NSP>(th, nb_s, dw, sel3(ks, nx[0], nx[1], nx[2]), sel3(ks, ny[0], ny[1], ny[2]),
+                                sel3(ks, af[0], af[1], af[2]), cs);
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) c[k][v] = ks == k ? cs[v] : c[k][v];
+            }
         }
     }
     if (live) {
