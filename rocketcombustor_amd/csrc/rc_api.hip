@@ -216,20 +216,94 @@ void chunk_list(std::vector<int4> &out, int c, int lo, int hi, int chunk, int sl
     for (int a = lo; a < hi; a += chunk) out.push_back(make_int4(c, a, std::min(hi, a + chunk), slot++));
 }
 
+// Work items laid out for the 8 XCDs (thread blocks are dispatched to the
+// XCDs round-robin, block b to XCD b % 8): XCD x takes the x-th eighth of
+// every segment, in chunks; item p of the list goes to XCD p % 8, shorter
+// lists padded with empty items.  With segments that are the colours of a
+// block-multicolour order (each sorted along the mesh), an XCD sweeps one
+// band of the domain in every colour, so the neighbours it gathers are
+// mostly in its own L2.
+constexpr int kXcd = 8;
+void xcd_list(std::vector<int4> &out, int c, const std::vector<std::pair<int, int>> &segs, int chunk) {
+    std::vector<std::vector<int2>> per(kXcd);
+    for (const auto &sg : segs) {
+        const long a = sg.first, len = (long)sg.second - sg.first;
+        for (int x = 0; x < kXcd; ++x) {
+            const int lo = (int)(a + len * x / kXcd), hi = (int)(a + len * (x + 1) / kXcd);
+            for (int q = lo; q < hi; q += chunk) per[x].push_back(make_int2(q, std::min(hi, q + chunk)));
+        }
+    }
+    size_t n = 0;
+    for (auto &v : per) n = std::max(n, v.size());
+    int slot = 0;
+    for (size_t j = 0; j < n; ++j)
+        for (int x = 0; x < kXcd; ++x) {
+            const int2 r = j < per[x].size() ? per[x][j] : make_int2(0, 0);
+            out.push_back(make_int4(c, r.x, r.y, slot++));
+        }
+}
+
+bool use_xcd_lists() {
+    const char *e = std::getenv("RC_XCD");
+    return !(e && e[0] == '0');
+}
+
+// cell index ranges of the sweep levels when every level's runs are one
+// contiguous range (a colour of a block-multicolour order); else the whole
+// range
+std::vector<std::pair<int, int>> level_cell_segments(const HostMesh &m) {
+    const int nlev = (int)m.lvl_ptr.size() - 1;
+    std::vector<std::pair<int, int>> segs;
+    long covered = 0;
+    for (int l = 0; l < nlev; ++l) {
+        int a = m.ne, b = 0;
+        long cells = 0;
+        for (int r = m.lvl_ptr[l]; r < m.lvl_ptr[l + 1]; ++r) {
+            a = std::min(a, m.runs[2 * r]);
+            b = std::max(b, m.runs[2 * r] + m.runs[2 * r + 1]);
+            cells += m.runs[2 * r + 1];
+        }
+        if (cells != (long)b - a) return {{0, m.ne}};
+        segs.push_back({a, b});
+        covered += cells;
+    }
+    std::sort(segs.begin(), segs.end());
+    for (size_t k = 1; k < segs.size(); ++k)
+        if (segs[k].first != segs[k - 1].second) return {{0, m.ne}};
+    if (covered != m.ne) return {{0, m.ne}};
+    return segs;
+}
+
 int layout_and_upload(rc_case *h) {
     const HostMesh &m = h->hm;
     const int sc = pad64(m.ne), sf = pad64(m.nf), sg = pad64(m.ng);
     h->w_cell.clear(); h->w_face.clear(); h->w_ghost.clear();
-    chunk_list(h->w_cell, 0, 0, m.ne, kChunk);
-    chunk_list(h->w_face, 0, 0, m.nf, kChunk);
-    chunk_list(h->w_ghost, 0, 0, m.ng, kChunk);
     const int nlev = (int)m.lvl_ptr.size() - 1;
     h->w_level.assign(nlev, {});
-    size_t nlw = 0;
-    for (int l = 0; l < nlev; ++l) {
-        chunk_list(h->w_level[l], 0, m.lvl_ptr[l], m.lvl_ptr[l + 1], kRunChunk);
-        nlw += h->w_level[l].size();
+    const bool xcd = use_xcd_lists();
+    if (xcd) {
+        auto cs = level_cell_segments(m);
+        if (!std::is_sorted(m.fnl.begin(), m.fnl.end())) cs = {{0, m.ne}};
+        // faces are numbered in order of first appearance over the cells, so
+        // each cell segment owns one contiguous face range
+        std::vector<std::pair<int, int>> fs;
+        for (const auto &c : cs) {
+            const int fa = (int)(std::lower_bound(m.fnl.begin(), m.fnl.end(), c.first) - m.fnl.begin());
+            const int fb = (int)(std::lower_bound(m.fnl.begin(), m.fnl.end(), c.second) - m.fnl.begin());
+            fs.push_back({fa, fb});
+        }
+        xcd_list(h->w_cell, 0, cs, kChunk);
+        xcd_list(h->w_face, 0, fs, kChunk);
+        chunk_list(h->w_ghost, 0, 0, m.ng, kChunk);
+        for (int l = 0; l < nlev; ++l) xcd_list(h->w_level[l], 0, {{m.lvl_ptr[l], m.lvl_ptr[l + 1]}}, kRunChunk);
+    } else {
+        chunk_list(h->w_cell, 0, 0, m.ne, kChunk);
+        chunk_list(h->w_face, 0, 0, m.nf, kChunk);
+        chunk_list(h->w_ghost, 0, 0, m.ng, kChunk);
+        for (int l = 0; l < nlev; ++l) chunk_list(h->w_level[l], 0, m.lvl_ptr[l], m.lvl_ptr[l + 1], kRunChunk);
     }
+    size_t nlw = 0;
+    for (int l = 0; l < nlev; ++l) nlw += h->w_level[l].size();
     const int npartial = (int)h->w_cell.size();
     int4 *d_wc = nullptr, *d_wf = nullptr, *d_wg = nullptr, *d_wl = nullptr;
     double *d_partial = nullptr, *d_resout = nullptr;
@@ -342,7 +416,7 @@ int layout_and_upload(rc_case *h) {
     LaunchSet &L = h->L;
     L.cases = h->d_case;
     L.ncases = 1;
-    L.single = true;
+    L.single = !xcd;
     L.cell = {d_wc, (int)h->w_cell.size(), 0, m.ne};
     L.face = {d_wf, (int)h->w_face.size(), 0, m.nf};
     L.ghost = {d_wg, (int)h->w_ghost.size(), 0, m.ng};
