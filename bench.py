@@ -86,11 +86,18 @@ def cpu_baseline(case, nsteps: int) -> dict:
             r = subprocess.run([ref, str(nsteps), "0", "out", "1"], cwd=tmp, capture_output=True, text=True,
                                timeout=600)
             line = [ln for ln in r.stdout.splitlines() if ln.startswith("steps")]
+            stage = [ln for ln in r.stdout.splitlines() if ln.startswith("stage_s")]
             if r.returncode == 0 and line:
                 wall = float(line[-1].split()[3])
-                return {"value": ne * nsteps / wall, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
-                        "sample": "%d steps of the same %d-cell case, reference Fortran (amdflang -O3, 1 rank), "
-                                  "step loop wall time %.2f s" % (nsteps, ne, wall)}
+                out = {"value": ne * nsteps / wall, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                       "sample": "%d steps of the same %d-cell case, reference Fortran (amdflang -O3, 1 rank), "
+                                 "step loop wall time %.2f s" % (nsteps, ne, wall)}
+                if stage:
+                    t = [float(x) for x in stage[-1].split()[2:6]]
+                    out["ms_per_step_by_call"] = {k: 1e3 * v / nsteps for k, v in
+                                                  zip(("timestep", "reconstruction", "flux_sourceterm",
+                                                       "lu_sgs_steady"), t)}
+                return out
         finally:
             shutil.rmtree(tmp, ignore_errors=True)
     from oracle import pyoracle

@@ -144,6 +144,10 @@ int rc_get_increment(rc_case *h, double *hwa);          /* hWA(Nvar,NE): residua
 int rc_get_gradient(rc_case *h, double *gradc);         /* GradC(2,Nvar,NE) */
 int rc_get_timestep(rc_case *h, double *step, double *alpaf); /* step(NE), alpaF(NF); either may be NULL */
 int rc_get_ghost(rc_case *h, double *gpa);              /* GPA(Nvar,Nghost) */
+/* The derived cell fields of output_vtk (output.f90:5-227) for the current
+ * state: rho(NE), Ma(NE) and Production_rate(Nsp-1,NE) as sourceterm forms
+ * it (sourceterm.f90:17-43; zero without combustion).  Any may be NULL. */
+int rc_get_derived(rc_case *h, double *rho, double *mach, double *prod);
 
 /* Per-kernel device time of the fused step summed over the steps run while
  * timing was on, in ms: t[0] gradient + Barth limiter (ghost states formed

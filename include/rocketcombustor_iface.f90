@@ -155,6 +155,12 @@ module rocketcombustor_iface
             real(c_double), intent(out) :: gpa(*)
         end function
 
+        integer(c_int) function rc_get_derived(h, rho, mach, prod) bind(c, name="rc_get_derived")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: rho(*), mach(*), prod(*)
+        end function
+
         integer(c_int) function rc_set_timing(h, enable) bind(c, name="rc_set_timing")
             import :: c_int, c_ptr
             type(c_ptr), value :: h
@@ -174,6 +180,42 @@ module rocketcombustor_iface
             integer(c_int), intent(in) :: cellnodes(*)
             integer(c_int), intent(out) :: perm(*)
             integer(c_int), intent(out) :: ncolors
+        end function
+
+        integer(c_int) function rc_batch_create(device, b) bind(c, name="rc_batch_create")
+            import :: c_int, c_ptr
+            integer(c_int), value :: device
+            type(c_ptr), intent(out) :: b
+        end function
+
+        subroutine rc_batch_destroy(b) bind(c, name="rc_batch_destroy")
+            import :: c_ptr
+            type(c_ptr), value :: b
+        end subroutine
+
+        integer(c_int) function rc_batch_add(b, h) bind(c, name="rc_batch_add")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b, h
+        end function
+
+        integer(c_int) function rc_batch_remove(b, h) bind(c, name="rc_batch_remove")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b, h
+        end function
+
+        integer(c_int) function rc_batch_info(b, info3) bind(c, name="rc_batch_info")
+            import :: c_int, c_ptr, c_long_long
+            type(c_ptr), value :: b
+            integer(c_long_long), intent(out) :: info3(3)
+        end function
+
+        integer(c_int) function rc_batch_run_steady(b, nsteps, every, res_out, done, err_case) &
+                bind(c, name="rc_batch_run_steady")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: b
+            integer(c_int), value :: nsteps, every
+            real(c_double), intent(out) :: res_out(*)
+            integer(c_int), intent(out) :: done, err_case
         end function
 
         integer(c_size_t) function c_strlen(s) bind(c, name="strlen")

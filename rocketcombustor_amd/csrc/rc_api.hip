@@ -25,6 +25,7 @@ void launch_lusgs_level(const LaunchSet &, bool, int, bool, hipStream_t);
 void launch_update(const LaunchSet &, hipStream_t);
 void launch_prim_to_cons(const LaunchSet &, hipStream_t);
 void launch_resnorm(const LaunchSet &, hipStream_t);
+void launch_derived(const LaunchSet &, double *, hipStream_t);
 }  // namespace rc
 
 using namespace rc;
@@ -890,6 +891,29 @@ int rc_get_ghost(rc_case *h, double *gpa) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     if (h->ng == 0) return RC_OK;
     return get_items(h, gpa, h->S.gpa, h->ng, kNvar, 0, kRecPA);
+}
+
+int rc_get_derived(rc_case *h, double *rho, double *mach, double *prod) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    HIPCHK(h, hipSetDevice(h->device));
+    const int w = kNsp + 1;
+    double *d = nullptr;
+    HIPCHK(h, hipMalloc(&d, sizeof(double) * w * std::max(h->ne, 1)));
+    launch_derived(h->L, d, h->stream);
+    std::vector<double> tmp((size_t)w * h->ne);
+    hipError_t e = hipMemcpyAsync(tmp.data(), d, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return set_err(h, RC_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e));
+    int rc = check_device_error(h);
+    if (rc) return rc;
+    for (int i = 0; i < h->ne; ++i) {
+        if (rho) rho[i] = tmp[(size_t)i * w];
+        if (mach) mach[i] = tmp[(size_t)i * w + 1];
+        if (prod)
+            for (int k = 0; k < kNsp - 1; ++k) prod[(size_t)i * (kNsp - 1) + k] = tmp[(size_t)i * w + 2 + k];
+    }
+    return RC_OK;
 }
 
 int rc_set_timing(rc_case *h, int enable) {

@@ -734,6 +734,33 @@ __global__ void __launch_bounds__(256) k_prim_to_cons(ITEM_ARGS) {
     for (int k = 0; k < NSP - 1; ++k) WA_(kNFlow + k, i) = rr * yy[k];
 }
 
+// the derived cell fields of output_vtk (output.f90:26-40): density,
+// Mach number, and the production rates of the current state as sourceterm
+// forms them (sourceterm.f90:17-43; zero without combustion); out holds
+// rho, Ma and NSP-1 rates per cell
+template <int NSP, bool B>
+__global__ void __launch_bounds__(256) k_derived(ITEM_ARGS, double *__restrict__ out) {
+    ITEM_PROLOGUE(i);
+    const double pre = PA_(0, i), u = PA_(1, i), v = PA_(2, i), tem = PA_(3, i);
+    double y[NSP], s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) { y[k] = PA_(kNFlow + k, i); s = s + y[k]; }
+    y[NSP - 1] = 1.0 - s;
+    double cp, gama, rcp;
+    if (!gas_param<NSP>(TH, y, tem, cp, gama, rcp)) flag_error(S.err, RC_ERR_TEMPERATURE, i, K_P2C);
+    const double rho = pre / (rcp * tem);
+    const double ac = sqrt(gama * rcp * tem);
+    RC_G double *o = gp(out) + (size_t)i * (NSP + 1);
+    o[0] = rho;
+    o[1] = sqrt(u * u + v * v) / ac;
+    double w[NSP];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) w[k] = 0.0;
+    if (PAR.combustion) production_rate<NSP>(TH, tem, pre / (tem * rcp), y, w);
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) o[2 + k] = w[k];
+}
+
 // Residual (residual.f90:4-21): partial sums of vol*|hWA| and max(WA(1))
 // per cell chunk (one work item per block, slot w.w of the case's partials),
 // then one block per case folds its partials in a fixed order, so results
@@ -870,6 +897,17 @@ void launch_lusgs_level(const LaunchSet &L, bool fwd, int l, bool res, hipStream
 void launch_update(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_update, kNsp); }
 
 void launch_prim_to_cons(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_prim_to_cons, kNsp); }
+
+void launch_derived(const LaunchSet &L, double *out, hipStream_t st) {
+    if (L.single) {
+        const int n = L.cell.hi - L.cell.lo;
+        if (n > 0)
+            k_derived<kNsp, false><<<(n + kChunk - 1) / kChunk, kChunk, 0, st>>>(L.cases, nullptr, L.cell.lo, L.cell.hi,
+                                                                                 out);
+    } else if (L.cell.n) {
+        k_derived<kNsp, true><<<L.cell.n, kChunk, 0, st>>>(L.cases, L.cell.w, 0, 0, out);
+    }
+}
 
 void launch_resnorm(const LaunchSet &L, hipStream_t st) {
     LAUNCH(L.cell, kChunk, st, k_resnorm_partial, kNsp);

@@ -346,3 +346,151 @@ def read_backup(path: str, nvar: int):
         data = np.loadtxt(f, ndmin=2)
     data = data.reshape(ne, 2 + nvar)
     return ttime, data[:, :2], data[:, 2:]
+
+
+def read_grid(path: str, unit: float = 1.0):
+    """``grid<N>.in`` as mesh.f90:10-65 reads it (free-format tokens):
+    returns (coor, cells, bedges, btype, nface) with 0-based node numbers.
+    Multi-rank connectivity lists and periodic pairs are not supported."""
+    with open(path) as f:
+        tok = f.read().split()
+    pos = 0
+
+    def take(n=1, conv=int):
+        nonlocal pos
+        out = [conv(_f(t)) if conv is float else int(t) for t in tok[pos:pos + n]]
+        pos += n
+        return out
+
+    ne, ne_vir, nnt, nnt_vir, nf, _ = take(6)
+    cells = np.empty((ne_vir, 3), dtype=np.int32)
+    for i in range(ne_vir):
+        k = take()[0]
+        nodes = take(k)
+        if k != 3:
+            raise ValueError("cell %d has %d nodes: only triangles are supported" % (i + 1, k))
+        cells[i] = nodes
+    xy = np.array(take(4 * nnt_vir, float), dtype=np.float64).reshape(nnt_vir, 4)
+    ncone = take()[0]
+    if ncone:
+        raise ValueError("partition connectivity lists (multi-rank grids) are not supported")
+    nbound, _ = take(2)
+    bedges, btype = [], []
+    for _ in range(nbound):
+        t, cnt = take(2)
+        pairs = np.array(take(2 * cnt), dtype=np.int32).reshape(cnt, 2)
+        bedges.append(pairs)
+        btype.append(np.full(cnt, t, dtype=np.int32))
+    ncon = take()[0] if pos < len(tok) else 0
+    if ncon:
+        raise ValueError("periodic face pairs are not supported")
+    be = np.concatenate(bedges) - 1 if bedges else np.zeros((0, 2), dtype=np.int32)
+    bt = np.concatenate(btype) if btype else np.zeros(0, dtype=np.int32)
+    return xy[:, :2] / unit, cells - 1, be.astype(np.int32), bt, nf
+
+
+def read_solver_info(case_dir: str) -> SolverInfo:
+    """``flow/flow_solver_info.in`` (startup.f90:55-128): the value lines in
+    the order write_solver_info writes them (comment lines start with #)."""
+    with open(os.path.join(case_dir, "flow", "flow_solver_info.in")) as f:
+        vals = [ln.replace(",", " ").split() for ln in f if ln.strip() and not ln.lstrip().startswith("#")]
+    it = iter(vals)
+    s = SolverInfo()
+    s.combustion = _b(next(it)[0])
+    s.if_right_domain = _b(next(it)[0])
+    s.n_previous_par = int(next(it)[0])
+    s.npar = int(next(it)[0])
+    s.unit = _f(next(it)[0])
+    v = next(it)
+    s.steady, s.viscous, s.read_backup, s.explicit = (_b(x) for x in v[:4])
+    s.implicit_time_method = int(next(it)[0])
+    v = next(it)
+    s.cfl, s.totaltime = _f(v[0]), _f(v[1])
+    s.nstep_max = int(v[2])
+    s.save_start_time, s.save_end_time = _f(v[3]), _f(v[4])
+    s.fix_dt = _b(next(it)[0])
+    v = next(it)
+    s.inloop, s.innerloop, s.reltol = int(v[0]), int(v[1]), _f(v[2])
+    v = next(it)
+    s.n_screen, s.n_file_output, s.n_file_backup, s.n_write_monitor = (int(x) for x in v[:4])
+    s.rc_detector = _f(next(it)[0])
+    v = next(it)
+    s.n_wbap, s.limit_id = int(v[0]), int(v[1])
+    nmon = int(next(it)[0])
+    for _ in range(nmon):
+        next(it)
+    v = next(it)
+    s.p_out_ratio, s.a_p, s.f_p = _f(v[0]), _f(v[1]), _f(v[2])
+    return s
+
+
+def write_converge(path: str, rows, append: bool = False) -> None:
+    """``converge.dat`` as step_residual writes it (residual.f90:83-93): a
+    header line, then ``istep cputime Res(1:Nvar) Romax`` per report.
+    ``rows`` holds (istep, cputime, res[Nvar+1]) with res[Nvar] = Romax."""
+    with open(path, "a" if append else "w") as f:
+        if not append:
+            f.write(' VARIABLES = "Time step","CPU time (Sec)","L1(Res)","Romax"\n')
+        for istep, cpu, res in rows:
+            f.write(" %d %.17g %s\n" % (istep, cpu, " ".join("%.17g" % x for x in res)))
+
+
+def read_converge(path: str) -> np.ndarray:
+    """The rows of a ``converge.dat`` (reference's or ours): (n, 2+Nvar+1)."""
+    with open(path) as f:
+        lines = [ln for ln in f if ln.strip() and "VARIABLES" not in ln]
+    return np.array([[_f(t) for t in ln.split()] for ln in lines], dtype=np.float64)
+
+
+def write_vtk(path: str, coor: np.ndarray, cells: np.ndarray, pa: np.ndarray, rho: np.ndarray, mach: np.ndarray,
+              prod: np.ndarray) -> None:
+    """Legacy-VTK cell data in the layout of output_vtk (output.f90:59-223):
+    points, triangles, then rho, u, v, p, T, Ma, Y1..Ynsp and
+    Production_rate_1..nsp-1; values below 1e-15 in magnitude written as 0
+    for Y and the rates as there (:45-53)."""
+    ne, nn = len(cells), len(coor)
+    nsp = pa.shape[1] - 3
+    y = np.column_stack([pa[:, 4:], 1.0 - pa[:, 4:].sum(axis=1)])
+    y = np.where(np.abs(y) < 1e-15, 0.0, y)
+    prod = np.where(np.abs(prod[:, :1]) < 1e-15, 0.0, prod)
+    with open(path, "w") as f:
+        f.write("# vtk DataFile Version 1.0\nUnstructured Grid Format\nASCII\nDATASET UNSTRUCTURED_GRID\n")
+        f.write("POINTS %d double\n" % nn)
+        np.savetxt(f, np.column_stack([coor, np.zeros(nn)]), fmt="%.8E")
+        f.write("CELLS %d %d\n" % (ne, 4 * ne))
+        np.savetxt(f, np.column_stack([np.full(ne, 3), cells]), fmt="%d")
+        f.write("CELL_TYPES %d\n" % ne)
+        np.savetxt(f, np.full(ne, 5), fmt="%d")
+        f.write("CELL_DATA %d\n" % ne)
+        fields = [("rho", rho, "%.8E"), ("u", pa[:, 1], "%.8E"), ("v", pa[:, 2], "%.8E"), ("p", pa[:, 0], "%.8E"),
+                  ("T", pa[:, 3], "%.8E"), ("Ma", mach, "%.8E")]
+        fields += [("Y%d" % (k + 1), y[:, k], "%.12E") for k in range(nsp)]
+        fields += [("Production_rate_%d" % (k + 1), prod[:, k], "%.12E") for k in range(nsp - 1)]
+        for name, vals, fmt in fields:
+            f.write("SCALARS %s double\nLOOKUP_TABLE default\n" % name)
+            np.savetxt(f, vals, fmt=fmt)
+
+
+def read_vtk_cell_data(path: str) -> dict:
+    """The CELL_DATA scalars of a legacy-VTK file written by write_vtk or
+    output_vtk."""
+    out, name, vals = {}, None, []
+    in_data = False
+    with open(path) as f:
+        for ln in f:
+            t = ln.split()
+            if not t:
+                continue
+            if t[0] == "CELL_DATA":
+                in_data = True
+            elif in_data and t[0] == "SCALARS":
+                if name:
+                    out[name] = np.array(vals)
+                name, vals = t[1], []
+            elif in_data and t[0] == "LOOKUP_TABLE":
+                continue
+            elif in_data and name:
+                vals.append(_f(t[0]))
+    if name:
+        out[name] = np.array(vals)
+    return out

@@ -351,3 +351,18 @@ def sample_cases(n: int, seed: int = 3, cells_lo: float = 5e3, cells_hi: float =
         out.append(CaseSpec(nx=5 * ny, ny=ny, p0=float(1e6 + 9e6 * u[1]), t0=float(2500.0 + 1000.0 * u[2]),
                             phi=float(0.7 + 0.6 * u[3]), throat_ratio=float(0.3 + 0.2 * u[4])))
     return out
+
+
+def case_from_dir(case_dir: str) -> Case:
+    """A case from a reference case directory (grid1.in, fvp1.sav,
+    combustion/*.in, flow/flow_solver_info.in), as the reference's start-up
+    reads it (startup.f90, mesh.f90:10-65, output.f90:276-338)."""
+    info = deck.read_solver_info(case_dir)
+    mech = deck.read_mechanism(case_dir)
+    oxid = deck.read_inlet(os.path.join(case_dir, "combustion", "oxidizer.in"), mech.nsp)
+    fuel = deck.read_inlet(os.path.join(case_dir, "combustion", "fuel.in"), mech.nsp)
+    coor, cells, bedges, btype, nface = deck.read_grid(os.path.join(case_dir, "grid1.in"), info.unit)
+    mesh = Mesh(coor=coor, cells=cells, bedges=bedges, btype=btype, nface=nface)
+    _, _, pa = deck.read_backup(os.path.join(case_dir, "fvp1.sav"), 3 + mech.nsp)
+    return Case(mesh=mesh, mech=mech, oxidizer=oxid, fuel=fuel, info=info, pa0=pa, geom=NozzleGeometry(),
+                p0=oxid.pres, t0=oxid.temp)

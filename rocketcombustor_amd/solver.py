@@ -152,6 +152,38 @@ class SteadyCase:
         nrep_done = (1 if nsteps >= 1 else 0) + sum(1 for s in range(2, nsteps + 1) if s % every == 0)
         return buf[:nrep_done]
 
+    # -- output and restart in the reference's formats ---------------------
+    def derived(self) -> dict:
+        """rho, Ma and the production rates of the current state, as
+        output_vtk writes them (computed on the device)."""
+        rho, ma = np.empty(self.ne), np.empty(self.ne)
+        prod = np.empty((self.ne, NVAR - 4))
+        check(lib().rc_get_derived(self.h, dptr(rho), dptr(ma), dptr(prod)), self.h)
+        return {"rho": rho, "Ma": ma, "production_rate": prod}
+
+    def write_vtk(self, path: str) -> None:
+        """The current field as output_vtk writes it (output.f90:5-227)."""
+        from . import deck
+        d = self.derived()
+        deck.write_vtk(path, self.case.mesh.coor, self.case.mesh.cells, self.primitive(), d["rho"], d["Ma"],
+                       d["production_rate"])
+
+    def backup(self, path: str, ttime: float = 0.0) -> None:
+        """``fvp1.sav`` of the current PA, as BackupData writes it
+        (output.f90:256-273); the reference restarts from it."""
+        from . import deck
+        deck.write_backup(path, self.case.mesh.centroids(), self.primitive(), ttime)
+
+    def restart(self, path: str) -> float:
+        """PA from a backup file, then WA from PA, as Read_Backup and
+        primitve_to_conserved_variables do; returns the stored time."""
+        from . import deck
+        ttime, _, pa = deck.read_backup(path, NVAR)
+        if len(pa) != self.ne:
+            raise ValueError("backup holds %d cells, the case has %d" % (len(pa), self.ne))
+        self.set_primitive(pa)
+        return ttime
+
     def set_timing(self, on: bool) -> None:
         self._call("rc_set_timing", 1 if on else 0)
 

@@ -1,0 +1,71 @@
+"""The reference's on-disk formats: grid1.in, fvp1.sav, flow_solver_info.in,
+converge.dat and the legacy-VTK output; a full case directory written for the
+reference reads back into the same case."""
+import os
+
+import numpy as np
+
+from rocketcombustor_amd import deck, nozzle
+
+
+def test_case_directory_round_trip(tmp_path):
+    case = nozzle.nozzle_case(24, 6, p0=3e6, t0=2800.0, fuel_faces=2, wall_type=1)
+    nozzle.write_reference_case(str(tmp_path), case, nstep_max=123)
+    back = nozzle.case_from_dir(str(tmp_path))
+    m, b = case.mesh, back.mesh
+    assert np.array_equal(m.cells, b.cells) and np.array_equal(m.coor, b.coor)
+    assert np.array_equal(m.bedges[np.argsort(m.btype, kind="stable")], b.bedges)
+    assert sorted(m.btype.tolist()) == sorted(b.btype.tolist())
+    assert m.nface == b.nface
+    assert np.array_equal(case.pa0, back.pa0)
+    assert back.info.nstep_max == 123 and back.info.cfl == case.info.cfl
+    assert back.info.limit_id == case.info.limit_id and back.info.combustion == case.info.combustion
+    assert back.info.p_out_ratio == case.info.p_out_ratio
+    assert np.array_equal(back.oxidizer.y, case.oxidizer.y) and back.oxidizer.mass_flow == case.oxidizer.mass_flow
+    assert np.array_equal(back.mech.low_coef, case.mech.low_coef)
+    assert back.p_out == case.p_out
+    assert back.inlet_mass_flux == case.inlet_mass_flux and back.fuel_mass_flux == case.fuel_mass_flux
+
+
+def test_grid_reader_rejects_what_is_unsupported(tmp_path):
+    p = tmp_path / "grid1.in"
+    p.write_text("1 1 4 4 4 4\n4 1 2 3 4\n0 0 0 0\n1 0 0 0\n1 1 0 0\n0 1 0 0\n0\n0 0\n0\n")
+    try:
+        deck.read_grid(str(p))
+    except ValueError as e:
+        assert "triangles" in str(e)
+    else:
+        raise AssertionError("a quadrilateral cell must be rejected")
+
+
+def test_converge_dat(tmp_path):
+    rows = [(1, 0.5, np.arange(8) + 0.25), (10, 1.5, np.arange(8) * 1e-3)]
+    p = str(tmp_path / "converge.dat")
+    deck.write_converge(p, rows[:1])
+    deck.write_converge(p, rows[1:], append=True)
+    with open(p) as f:
+        assert f.readline().strip().startswith("VARIABLES")
+    a = deck.read_converge(p)
+    assert a.shape == (2, 10)
+    assert np.array_equal(a[:, 0], [1, 10]) and np.array_equal(a[1, 2:], rows[1][2])
+
+
+def test_vtk_layout(tmp_path):
+    case = nozzle.nozzle_case(12, 4)
+    ne, nn = case.mesh.ne, len(case.mesh.coor)
+    rho, ma = np.linspace(1, 2, ne), np.linspace(0.1, 3, ne)
+    prod = np.zeros((ne, 3))
+    prod[0] = [1e-17, 2.0, 3.0]          # zeroed: |rate 1| < 1e-15
+    prod[1] = [4.0, 5.0, 6.0]
+    p = str(tmp_path / "out.vtk")
+    deck.write_vtk(p, case.mesh.coor, case.mesh.cells, case.pa0, rho, ma, prod)
+    txt = open(p).read().split("\n")
+    assert txt[0].startswith("# vtk DataFile") and "POINTS %d double" % nn in txt
+    assert "CELLS %d %d" % (ne, 4 * ne) in txt and "CELL_TYPES %d" % ne in txt
+    d = deck.read_vtk_cell_data(p)
+    assert list(d) == ["rho", "u", "v", "p", "T", "Ma", "Y1", "Y2", "Y3", "Y4",
+                       "Production_rate_1", "Production_rate_2", "Production_rate_3"]
+    assert np.allclose(d["rho"], rho, rtol=1e-8) and np.allclose(d["p"], case.pa0[:, 0], rtol=1e-8)
+    y4 = 1.0 - case.pa0[:, 4:].sum(axis=1)
+    assert np.allclose(d["Y4"], y4, rtol=1e-11)
+    assert np.all(d["Production_rate_2"][:1] == 0.0) and d["Production_rate_2"][1] == 5.0
