@@ -58,6 +58,12 @@ def lib():
         L.rco_enthalpy.argtypes = [P(RcoMech), C.c_double, P(C.c_double)]
         L.rco_newton.argtypes = [P(RcoMech), C.c_double, P(C.c_double), C.c_double, P(C.c_double)]
         L.rco_production_rate.argtypes = [P(RcoMech), C.c_double, C.c_double, P(C.c_double), P(C.c_double)]
+        L.rco_con_to_pri_jaco.argtypes = [P(RcoMech), P(C.c_double), P(C.c_double)]
+        L.rco_dwdp.argtypes = [P(RcoMech), P(C.c_double), P(C.c_double)]
+        L.rco_solve_dgesvx.argtypes = [C.c_int, P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.rco_be_lusgs.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.rco_be_stage.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, P(C.c_int), P(C.c_double)]
+        L.rco_set_p_out.argtypes = [C.c_void_p, C.c_double]
         _LIB = L
     return _LIB
 
@@ -139,7 +145,7 @@ class OracleCase:
             raise RuntimeError(_LIB.rco_last_error(self.h).decode())
 
     _DSHAPE = {"PA": "nv", "WA": "nv", "hWA": "nv", "gradC": "g", "GPA": "gv", "alpaF": "f", "step": "e",
-               "dd": "e", "VECX": "f2", "vol": "e", "cellxy": "e2", "reflen": "e2", "wdis": "e3",
+               "dd": "e", "fixed": "nv", "VECX": "f2", "vol": "e", "cellxy": "e2", "reflen": "e2", "wdis": "e3",
                "AMLS": "a", "vol_Ghostcell": "gg", "CellXY_Ghostcell": "gg2", "production_rate": "ps"}
     _ISHAPE = {"Neighbor": "f2", "FNode": "f2", "NeighF": "e3", "NeighC": "e4", "Neigh": "e",
                "Num_ghostcell": "e", "Ghostcell": "e3", "Hostcell": "gg", "BCtype_Ghostcell": "gg"}
@@ -196,6 +202,20 @@ class OracleCase:
         _LIB.rco_residual(self.h, _dp(out))
         return out
 
+    # dual-time backward Euler (implicit_time_method = 3)
+    def set_p_out(self, p_out: float):
+        _LIB.rco_set_p_out(self.h, float(p_out))
+
+    def be_lusgs(self, dt: float, inner: int):
+        self._check(_LIB.rco_be_lusgs(self.h, float(dt), int(inner)))
+
+    def be_stage(self, dt: float, innerloop: int, reltol: float) -> np.ndarray:
+        """One physical step's inner iterations; returns their RoRes."""
+        n = C.c_int(0)
+        ro = np.zeros(max(int(innerloop), 1))
+        self._check(_LIB.rco_be_stage(self.h, float(dt), int(innerloop), float(reltol), C.byref(n), _dp(ro)))
+        return ro[:n.value]
+
 
 def gas_param(mech, y, t):
     m = make_mech(mech)
@@ -232,3 +252,32 @@ def production_rate(mech, t, rho, y):
     out = np.zeros(mech.nsp)
     lib().rco_production_rate(C.byref(m), float(t), float(rho), _dp(y), _dp(out))
     return out
+
+
+def con_to_pri_jaco(mech, pri):
+    """dU/dPA at one primitive state (nvar x nvar)."""
+    m = make_mech(mech)
+    pri = np.ascontiguousarray(pri, dtype=np.float64)
+    out = np.zeros((len(pri), len(pri)))
+    if lib().rco_con_to_pri_jaco(C.byref(m), _dp(pri), _dp(out)):
+        raise ValueError("temperature out of range")
+    return out
+
+
+def dwdp(mech, pri):
+    """d(omega_i)/dPA for the first nsp-1 species ((nsp-1) x nvar)."""
+    m = make_mech(mech)
+    pri = np.ascontiguousarray(pri, dtype=np.float64)
+    out = np.zeros((mech.nsp - 1, len(pri)))
+    if lib().rco_dwdp(C.byref(m), _dp(pri), _dp(out)):
+        raise ValueError("temperature out of range")
+    return out
+
+
+def solve_dgesvx(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    if lib().rco_solve_dgesvx(len(b), _dp(a), _dp(b), _dp(x)):
+        raise np.linalg.LinAlgError("singular matrix")
+    return x

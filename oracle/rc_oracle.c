@@ -38,6 +38,7 @@ struct rco_case {
     double *wdis, *amls, *xyh;     /* wdis(3,NE), AMLS(2,3,NE), XYH(2,NE) */
     double total_volume;
     double *pa, *wa, *hwa, *gradc, *gpa, *alpaf, *step, *prod, *dd;
+    double *fixed;                 /* fixed(Nvar,NE) of the dual-time iteration */
     char err[256];
 };
 
@@ -481,6 +482,7 @@ int rco_create(const rco_mech *mech, const rco_params *par, int nnt, const doubl
     c->step = (double *)xcalloc(ne, sizeof(double));
     c->prod = (double *)xcalloc((size_t)c->nsp * ne, sizeof(double));
     c->dd = (double *)xcalloc(ne, sizeof(double));
+    c->fixed = (double *)xcalloc((size_t)nv * ne, sizeof(double));
     return 0;
 }
 
@@ -490,7 +492,7 @@ void rco_destroy(rco_case *c)
     void *ps[] = {c->coor, c->n4, c->fnode, c->neighbor, c->vecx, c->vol, c->cellxy, c->reflen,
                   c->neighf, c->neighc, c->neigh, c->hostcell, c->hostface, c->bctype_g,
                   c->ghostcell, c->num_ghost, c->coor_gn, c->vol_g, c->cellxy_g, c->wdis, c->amls,
-                  c->xyh, c->pa, c->wa, c->hwa, c->gradc, c->gpa, c->alpaf, c->step, c->prod, c->dd};
+                  c->xyh, c->pa, c->wa, c->hwa, c->gradc, c->gpa, c->alpaf, c->step, c->prod, c->dd, c->fixed};
     size_t i;
     for (i = 0; i < sizeof ps / sizeof ps[0]; i++) free(ps[i]);
     free(c);
@@ -510,6 +512,7 @@ double *rco_darray(rco_case *c, const char *n)
     if (!strcmp(n, "alpaF")) return c->alpaf;
     if (!strcmp(n, "step")) return c->step;
     if (!strcmp(n, "dd")) return c->dd;
+    if (!strcmp(n, "fixed")) return c->fixed;
     if (!strcmp(n, "production_rate")) return c->prod;
     if (!strcmp(n, "VECX")) return c->vecx;
     if (!strcmp(n, "vol")) return c->vol;
@@ -973,3 +976,341 @@ void rco_residual(rco_case *c, double *res)
     for (k = 0; k < nvar; k++) res[k] = res[k] / c->total_volume;
     res[nvar] = romax;
 }
+
+
+/* ------------------------------------------------------------------------ */
+/* dual-time backward Euler with the point-implicit LU-SGS                  */
+/* (implicit_time_method = 3): Time_BackwardEuler.f90                       */
+/* ------------------------------------------------------------------------ */
+
+/* Compute_Con_to_Pri_Jaco, Con_to_Pri_Jacobian.f90:1-87: dU/dPA, row-major
+ * jaco[r][c] = J(r+1,c+1) */
+int rco_con_to_pri_jaco(const rco_mech *m, const double *pri, double *jaco)
+{
+    int i, j, nsp = m->nsp, ns1 = nsp - 1, nv = NFLOW + ns1;
+    double p = pri[0], u = pri[1], v = pri[2], t = pri[3], y[RCO_MAXSP], y2[RCO_MAXSP], s = 0.0, g[3], entha[RCO_MAXSP];
+    for (i = 0; i < ns1; i++) { y[i] = pri[NFLOW + i]; y2[i] = y[i]; s = s + y[i]; }
+    y2[nsp - 1] = 1.0 - s;
+    if (rco_gas_param(m, y2, t, g)) return 1;
+    double cp = g[0], rcp = g[2];
+    double r = p / (rcp * t);
+    if (rco_enthalpy(m, t, entha)) return 1;
+    double h = entha[nsp - 1], dhdy[RCO_MAXSP];
+    for (i = 0; i < ns1; i++) {
+        h = h + y[i] * (entha[i] - entha[nsp - 1]);
+        dhdy[i] = entha[i] - entha[nsp - 1];
+    }
+    double drdp = 1. / (rcp * t), drdt = -p / (rcp * t * t), drdy[RCO_MAXSP];
+    const double rn = m->rconstant / m->mw[nsp - 1];
+    for (i = 0; i < ns1; i++) drdy[i] = -(m->rconstant / m->mw[i] - rn) * p / (t * rcp * rcp);
+    const double e2 = h + 0.5 * (u * u + v * v);
+    for (i = 0; i < nv * nv; i++) jaco[i] = 0.0;
+#define J(a, b) jaco[(a) * nv + (b)]
+    J(0, 0) = drdp; J(0, 3) = drdt;
+    J(1, 0) = u * drdp; J(1, 1) = r; J(1, 3) = u * drdt;
+    J(2, 0) = v * drdp; J(2, 2) = r; J(2, 3) = v * drdt;
+    J(3, 0) = e2 * drdp - 1.; J(3, 1) = r * u; J(3, 2) = r * v; J(3, 3) = r * cp + e2 * drdt;
+    for (j = 0; j < ns1; j++) {
+        J(0, NFLOW + j) = drdy[j];
+        J(1, NFLOW + j) = u * drdy[j];
+        J(2, NFLOW + j) = v * drdy[j];
+        J(3, NFLOW + j) = e2 * drdy[j] + r * dhdy[j];
+    }
+    for (i = 0; i < ns1; i++) {
+        J(NFLOW + i, 0) = drdp * y[i];
+        J(NFLOW + i, 3) = drdt * y[i];
+        for (j = 0; j < ns1; j++) J(NFLOW + i, NFLOW + j) = (i == j) ? r + y[i] * drdy[j] : y[i] * drdy[j];
+    }
+#undef J
+    return 0;
+}
+
+/* compute_derivative_production_rate, chemical_kinetics.f90:167-265:
+ * dwdp[i][k] = d omega_i / d PA_k, i < nsp-1.  In the model-1 branch the
+ * reference never assigns dkfdy(3) (:213-236); it is taken as 0 here. */
+int rco_dwdp(const rco_mech *m, const double *pri, double *dwdp)
+{
+    int i, nsp = m->nsp, ns1 = nsp - 1, nv = NFLOW + ns1;
+    double p = pri[0], T = pri[3], y[RCO_MAXSP], y2[RCO_MAXSP], s = 0.0, g[3];
+    for (i = 0; i < ns1; i++) { y[i] = pri[NFLOW + i]; y2[i] = y[i]; s = s + y[i]; }
+    y2[nsp - 1] = 1.0 - s;
+    if (rco_gas_param(m, y2, T, g)) return 1;
+    double rcp = g[2];
+    double rho = p / (rcp * T);
+    for (i = 0; i < ns1; i++) y[i] = fmin(1., fmax(0., y[i]));
+    double dkf[RCO_MAXSP + NFLOW];
+    for (i = 0; i < nv; i++) dkf[i] = 0.0;
+    if (!(y[0] < 1.0e-12)) {
+        double af = m->a_f * exp(-m->e_act / (m->rconstant * T));
+        double r1 = y[0] * rho / (m->mw[0] * 1000), dr1dr = y[0] / (m->mw[0] * 1000), dr1dy1 = rho / (m->mw[0] * 1000);
+        double r2 = y[1] * rho / (m->mw[1] * 1000), dr2dr = y[1] / (m->mw[1] * 1000), dr2dy2 = rho / (m->mw[1] * 1000);
+        double drdp = rho / p, drdT = -rho / T, drdy[RCO_MAXSP];
+        const double rn = m->rconstant / m->mw[nsp - 1];
+        for (i = 0; i < ns1; i++) drdy[i] = -(m->rconstant / m->mw[i] - rn) * rho / rcp;
+        double dkfdr, dkfdp, dkfdT, dkfdy[RCO_MAXSP];
+        for (i = 0; i < ns1; i++) dkfdy[i] = 0.0;
+        if (m->model == 0) {
+            double dter1_dr = 0.2 * pow(rho, -0.8) * pow(dr1dr, 0.2);
+            dkfdr = af * (pow(r1, 0.2) * 1.3 * pow(r2, 0.3) * dr2dr + dter1_dr * pow(r2, 1.3));
+            dkfdp = dkfdr * drdp;
+            dkfdT = af * pow(r1, 0.2) * pow(r2, 1.3) * m->e_act / (m->rconstant * T * T) + dkfdr * drdT;
+            dkfdy[0] = af * 0.2 * pow(r1, -0.8) * dr1dy1 * pow(r2, 1.3) + dkfdr * drdy[0];
+            dkfdy[1] = af * pow(r1, 0.2) * 1.3 * pow(r2, 0.3) * dr2dy2 + dkfdr * drdy[1];
+            for (i = 2; i < ns1; i++) dkfdy[i] = dkfdr * drdy[i];
+        } else {
+            af = 0.0;
+            if (T > 710.0) af = 1e4;
+            double dter1_dr = 1 * powi(rho, 0) * powi(dr1dr, 1);
+            dkfdr = af * (powi(r1, 1) * 0 * powi(r2, 0) * dr2dr + dter1_dr * powi(r2, 0));
+            dkfdp = dkfdr * drdp;
+            dkfdT = 0.;
+            dkfdy[0] = af * 1 * powi(r1, 0) * dr1dy1 * powi(r2, 0) + dkfdr * drdy[0];
+            dkfdy[1] = af * powi(r1, 1) * 0 * powi(r2, 0) * dr2dy2 + dkfdr * drdy[1];
+        }
+        dkf[0] = dkfdp; dkf[1] = 0.; dkf[2] = 0.; dkf[3] = dkfdT;
+        for (i = 0; i < ns1; i++) dkf[NFLOW + i] = dkfdy[i];
+    }
+    for (i = 0; i < ns1; i++) {
+        int k;
+        for (k = 0; k < nv; k++) dwdp[i * nv + k] = 1000 * m->mw[i] * (m->coef_r[i] - m->coef_f[i]) * dkf[k];
+    }
+    return 0;
+}
+
+/* advanced_linear_solve_lapack (la_math.f90:120-213) calls LAPACK dgesvx
+ * with FACT='E': equilibrate (dgeequ, applied as dlaqge decides), LU with
+ * partial pivoting (dgetrf), solve (dgetrs), iterative refinement (dgerfs),
+ * unscale.  Restated on the published algorithm of those routines for an
+ * n x n system, a row-major. */
+int rco_solve_dgesvx(int n, const double *a_in, const double *b_in, double *x)
+{
+    double a[64], af[64], b[8], r[8], cs[8], w[8], dx[8];
+    int ipiv[8], i, j, k;
+    const double eps = 1.1102230246251565e-16, sfmin = 2.2250738585072014e-308;
+    const double smlnum = sfmin, bignum = 1.0 / smlnum;
+    for (i = 0; i < n * n; i++) a[i] = a_in[i];
+    for (i = 0; i < n; i++) b[i] = b_in[i];
+    /* dgeequ */
+    double rcmin = bignum, rcmax = 0.0;
+    for (i = 0; i < n; i++) {
+        r[i] = 0.0;
+        for (j = 0; j < n; j++) r[i] = fmax(r[i], fabs(a[i * n + j]));
+        rcmax = fmax(rcmax, r[i]);
+        rcmin = fmin(rcmin, r[i]);
+    }
+    const double amax = rcmax;
+    if (rcmin == 0.0) return 1;
+    for (i = 0; i < n; i++) r[i] = 1.0 / fmin(fmax(r[i], smlnum), bignum);
+    const double rowcnd = fmax(rcmin, smlnum) / fmin(rcmax, bignum);
+    rcmin = bignum; rcmax = 0.0;
+    for (j = 0; j < n; j++) {
+        cs[j] = 0.0;
+        for (i = 0; i < n; i++) cs[j] = fmax(cs[j], fabs(a[i * n + j]) * r[i]);
+        rcmin = fmin(rcmin, cs[j]);
+        rcmax = fmax(rcmax, cs[j]);
+    }
+    if (rcmin == 0.0) return 1;
+    for (j = 0; j < n; j++) cs[j] = 1.0 / fmin(fmax(cs[j], smlnum), bignum);
+    const double colcnd = fmax(rcmin, smlnum) / fmin(rcmax, bignum);
+    /* dlaqge */
+    const double thresh = 0.1, small = sfmin / (2 * eps), large = 1.0 / small;
+    int rows = 0, cols = 0;
+    if (rowcnd >= thresh && amax >= small && amax <= large) {
+        cols = colcnd < thresh;
+    } else {
+        rows = 1;
+        cols = colcnd < thresh;
+    }
+    for (j = 0; j < n; j++)
+        for (i = 0; i < n; i++) {
+            if (rows && cols) a[i * n + j] = cs[j] * r[i] * a[i * n + j];
+            else if (rows) a[i * n + j] = r[i] * a[i * n + j];
+            else if (cols) a[i * n + j] = cs[j] * a[i * n + j];
+        }
+    if (rows)
+        for (i = 0; i < n; i++) b[i] = r[i] * b[i];
+    /* dgetf2 */
+    for (i = 0; i < n * n; i++) af[i] = a[i];
+    for (j = 0; j < n; j++) {
+        int p = j;
+        for (i = j + 1; i < n; i++)
+            if (fabs(af[i * n + j]) > fabs(af[p * n + j])) p = i;
+        ipiv[j] = p;
+        if (af[p * n + j] == 0.0) return 1;
+        if (p != j)
+            for (k = 0; k < n; k++) { double t = af[j * n + k]; af[j * n + k] = af[p * n + k]; af[p * n + k] = t; }
+        if (fabs(af[j * n + j]) >= sfmin) {
+            const double rp = 1.0 / af[j * n + j];
+            for (i = j + 1; i < n; i++) af[i * n + j] = rp * af[i * n + j];
+        } else {
+            for (i = j + 1; i < n; i++) af[i * n + j] = af[i * n + j] / af[j * n + j];
+        }
+        for (k = j + 1; k < n; k++) {
+            const double t = -af[j * n + k];
+            for (i = j + 1; i < n; i++) af[i * n + k] = af[i * n + k] + t * af[i * n + j];
+        }
+    }
+    /* dgetrs on a right-hand side v (in place) */
+#define GETRS(v)                                                                   \
+    do {                                                                           \
+        for (i = 0; i < n; i++)                                                    \
+            if (ipiv[i] != i) { double t = v[i]; v[i] = v[ipiv[i]]; v[ipiv[i]] = t; } \
+        for (k = 0; k < n; k++)                                                    \
+            if (v[k] != 0.0)                                                       \
+                for (i = k + 1; i < n; i++) v[i] = v[i] - v[k] * af[i * n + k];    \
+        for (k = n - 1; k >= 0; k--)                                               \
+            if (v[k] != 0.0) {                                                     \
+                v[k] = v[k] / af[k * n + k];                                       \
+                for (i = 0; i < k; i++) v[i] = v[i] - v[k] * af[i * n + k];        \
+            }                                                                      \
+    } while (0)
+    for (i = 0; i < n; i++) x[i] = b[i];
+    GETRS(x);
+    /* dgerfs: at most 5 refinement steps */
+    double lstres = 3.0;
+    for (int count = 1; count <= 5; count++) {
+        for (i = 0; i < n; i++) w[i] = b[i];
+        for (j = 0; j < n; j++) {
+            const double t = -x[j];
+            for (i = 0; i < n; i++) w[i] = w[i] + t * a[i * n + j];
+        }
+        double berr = 0.0;
+        for (i = 0; i < n; i++) {
+            double den = fabs(b[i]);
+            for (j = 0; j < n; j++) den = den + fabs(a[i * n + j]) * fabs(x[j]);
+            const double safe1 = (n + 1) * sfmin, safe2 = safe1 / eps;
+            berr = fmax(berr, den > safe2 ? fabs(w[i]) / den : (fabs(w[i]) + safe1) / (den + safe1));
+        }
+        if (!(berr > eps && 2.0 * berr <= lstres)) break;
+        for (i = 0; i < n; i++) dx[i] = w[i];
+        GETRS(dx);
+        for (i = 0; i < n; i++) x[i] = x[i] + dx[i];
+        lstres = berr;
+    }
+#undef GETRS
+    if (cols)
+        for (j = 0; j < n; j++) x[j] = cs[j] * x[j];
+    return 0;
+}
+
+/* the neighbour term of backwardEuler_LU_SGS (Time_BackwardEuler.f90:
+ * 131-176): the neighbour's primitive increment hWA(:,NR) -> conserved
+ * increment, then the linearised Rusanov flux change */
+static int be_neighbor(rco_case *c, int NR, double dx, double dy, double af, double *dflux)
+{
+    int k, nsp = c->nsp, ns1 = nsp - 1;
+    double pp = PA(1, NR), uu = PA(2, NR), vv = PA(3, NR), tt = PA(4, NR), yy[RCO_MAXSP], s = 0.0, g[3],
+           entha[RCO_MAXSP];
+    for (k = 0; k < ns1; k++) { yy[k] = PA(NFLOW + 1 + k, NR); s = s + yy[k]; }
+    yy[nsp - 1] = 1.0 - s;
+    if (rco_gas_param(&c->m, yy, tt, g)) return fail(c, "backward Euler: temperature %g out of range", tt);
+    if (rco_enthalpy(&c->m, tt, entha)) return fail(c, "backward Euler: temperature %g out of range", tt);
+    double cp = g[0], rcp = g[2], hh = 0.0;
+    for (k = 0; k < nsp; k++) hh = hh + yy[k] * entha[k];
+    double rr = pp / (rcp * tt), rou = rr * uu, rov = rr * vv;
+    double roe = rr * (hh + 0.5 * (uu * uu + vv * vv)) - pp;
+    double ry[RCO_MAXSP], dyy[RCO_MAXSP], dry[RCO_MAXSP];
+    for (k = 0; k < nsp; k++) ry[k] = rr * yy[k];
+    double dpp = HWA(1, NR), duu = HWA(2, NR), dvv = HWA(3, NR), dtt = HWA(4, NR);
+    s = 0.0;
+    for (k = 0; k < ns1; k++) { dyy[k] = HWA(NFLOW + 1 + k, NR); s = s + dyy[k]; }
+    dyy[nsp - 1] = -s;
+    double s1 = 0.0, s2 = 0.0;
+    for (k = 0; k < nsp; k++) s1 = s1 + (c->m.rconstant / c->m.mw[k]) * dyy[k];
+    for (k = 0; k < nsp; k++) s2 = s2 + entha[k] * dyy[k];
+    double drr = (dpp - rr * rcp * dtt - rr * tt * s1) / (rcp * tt);
+    double dru = rr * duu + uu * drr;
+    double drv = rr * dvv + vv * drr;
+    double dre = rr * cp * dtt + rr * s2 + hh * drr + rr * (uu * duu + vv * dvv) + 0.5 * (uu * uu + vv * vv) * drr - dpp;
+    for (k = 0; k < nsp; k++) dry[k] = rr * dyy[k] + yy[k] * drr;
+    double un = uu * dx + vv * dy, dun = duu * dx + dvv * dy;
+    dflux[0] = dflux[0] + 0.5 * (-af * drr + dru * dx + drv * dy);
+    dflux[1] = dflux[1] + 0.5 * (-af * dru + dru * un + rou * dun + dpp * dx);
+    dflux[2] = dflux[2] + 0.5 * (-af * drv + drv * un + rov * dun + dpp * dy);
+    dflux[3] = dflux[3] + 0.5 * (-af * dre + dun * (roe + pp) + un * (dre + dpp));
+    for (k = 0; k < ns1; k++) dflux[NFLOW + k] = dflux[NFLOW + k] + 0.5 * (-af * dry[k] + dry[k] * un + ry[k] * dun);
+    return 0;
+}
+
+/* the block of cell i: dU/dPA*dd(i) - vol(i)*d(omega)/dPA in the species
+ * rows (Time_BackwardEuler.f90:106-116) */
+static int be_block(rco_case *c, int i, double *jm)
+{
+    int nv = c->nvar, j, k;
+    double dw[RCO_MAXSP * (RCO_MAXSP + NFLOW)];
+    if (rco_con_to_pri_jaco(&c->m, &PA(1, i), jm)) return fail(c, "backward Euler: temperature out of range (cell %d)", i);
+    for (k = 0; k < nv * nv; k++) jm[k] = jm[k] * c->dd[i - 1];
+    if (rco_dwdp(&c->m, &PA(1, i), dw)) return fail(c, "backward Euler: temperature out of range (cell %d)", i);
+    for (j = 0; j < c->nsp - 1; j++)
+        for (k = 0; k < nv; k++) jm[(NFLOW + j) * nv + k] = jm[(NFLOW + j) * nv + k] - c->vol[i - 1] * dw[j * nv + k];
+    return 0;
+}
+
+/* backwardEuler_LU_SGS, Time_BackwardEuler.f90:58-324 */
+int rco_be_lusgs(rco_case *c, double dt, int inner)
+{
+    int i, k, nei, nvar = c->nvar;
+    double dflux[RCO_MAXSP + NFLOW], jm[(RCO_MAXSP + NFLOW) * (RCO_MAXSP + NFLOW)], rhs[RCO_MAXSP + NFLOW],
+           sol[RCO_MAXSP + NFLOW];
+    if (inner == 1)
+        for (i = 1; i <= c->ne; i++)
+            for (k = 1; k <= nvar; k++) c->fixed[(i - 1) * nvar + k - 1] = c->vol[i - 1] / dt * WA(k, i);
+    for (i = 1; i <= c->ne; i++) {
+        double d = 0.0;
+        for (nei = 1; nei <= 3; nei++) d = d + 0.5 * c->alpaf[NEIGHF(nei, i) - 1];
+        c->dd[i - 1] = d + c->vol[i - 1] * (1. / c->step[i - 1] + 1. / dt);
+    }
+    for (i = 1; i <= c->ne; i++)
+        for (k = 1; k <= nvar; k++)
+            HWA(k, i) = HWA(k, i) - c->vol[i - 1] / dt * WA(k, i) + c->fixed[(i - 1) * nvar + k - 1];
+    for (int pass = 0; pass < 2; pass++) {
+        for (int ii = 1; ii <= c->ne; ii++) {
+            i = pass == 0 ? ii : c->ne + 1 - ii;
+            if (be_block(c, i, jm)) return 1;
+            for (k = 0; k < nvar; k++) dflux[k] = 0.0;
+            for (nei = 1; nei <= 3; nei++) {
+                int iface = NEIGHF(nei, i);
+                double dx = VECX(1, iface), dy = VECX(2, iface);
+                int NR = NEIGHBOR(2, iface);
+                if (i == NR) { NR = NEIGHBOR(1, iface); dx = -dx; dy = -dy; }
+                if (NR > 0 && (pass == 0 ? NR < i : NR > i))
+                    if (be_neighbor(c, NR, dx, dy, c->alpaf[iface - 1], dflux)) return 1;
+            }
+            for (k = 0; k < nvar; k++) rhs[k] = pass == 0 ? HWA(k + 1, i) - dflux[k] : dflux[k];
+            if (rco_solve_dgesvx(nvar, jm, rhs, sol)) return fail(c, "error in routine dgesvx (cell %d)", i);
+            for (k = 0; k < nvar; k++) HWA(k + 1, i) = pass == 0 ? sol[k] : HWA(k + 1, i) - sol[k];
+        }
+    }
+    for (i = 1; i <= c->ne; i++)
+        for (k = 1; k <= nvar; k++) PA(k, i) = PA(k, i) + HWA(k, i);
+    return prim_to_cons(c);
+}
+
+/* backwardEuler_stage, Time_BackwardEuler.f90:4-56 (without its closing
+ * diagnostic reconstruction/flux_sourceterm): inner iterations until RoRes
+ * drops below relTol times its first value or InnerLoop is reached; the
+ * number of inner iterations and their RoRes go to *ninner, roress[] */
+int rco_be_stage(rco_case *c, double dt, int innerloop, double reltol, int *ninner, double *roress)
+{
+    double res[RCO_MAXSP + NFLOW + 1], ro1 = 0.0;
+    int inner;
+    for (inner = 1; inner <= innerloop; inner++) {
+        if (rco_timestep(c)) return 1;
+        if (rco_gradient(c)) return 1;
+        if (c->p.limit_id > 0 && rco_limiter(c)) return 1;
+        if (rco_flux_sourceterm(c)) return 1;
+        if (rco_be_lusgs(c, dt, inner)) return 1;
+        rco_residual(c, res);
+        double ro = 0.0;
+        for (int k = NFLOW; k < c->nvar; k++) ro = ro + res[k];
+        if (roress) roress[inner - 1] = ro;
+        if (ninner) *ninner = inner;
+        if (inner == 1) ro1 = ro;
+        if (ro < ro1 * reltol || inner == innerloop) break;
+        if (isnan(WA(1, 1))) return fail(c, "there is NAN in WA(1,1)");
+    }
+    return 0;
+}
+
+/* p_out as compute_p_out sets it for the next evaluations (fvm2d.f90:229-239) */
+void rco_set_p_out(rco_case *c, double p_out) { c->p.p_out = p_out; }

@@ -63,6 +63,13 @@ int  rco_gas_param(const rco_mech *m, const double *y, double t, double *cp_gam_
 int  rco_enthalpy(const rco_mech *m, double t, double *h);
 int  rco_newton(const rco_mech *m, double et, const double *y, double t0, double *t_cp_gam_r);
 void rco_production_rate(const rco_mech *m, double t, double rho, const double *y, double *w);
+/* dual-time backward Euler (implicit_time_method = 3) */
+int  rco_con_to_pri_jaco(const rco_mech *m, const double *pri, double *jaco_rowmajor);
+int  rco_dwdp(const rco_mech *m, const double *pri, double *dwdp_rowmajor);
+int  rco_solve_dgesvx(int n, const double *a_rowmajor, const double *b, double *x);
+int  rco_be_lusgs(rco_case *c, double dt, int inner);
+int  rco_be_stage(rco_case *c, double dt, int innerloop, double reltol, int *ninner, double *roress);
+void rco_set_p_out(rco_case *c, double p_out);
 
 #ifdef __cplusplus
 }

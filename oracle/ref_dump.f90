@@ -13,14 +13,22 @@
 !! OUTDIR/resid.bin; the final PA/WA are written to OUTDIR/final_*.bin.
 !! With TIMING=1 nothing but resid.bin/final is written and per-stage wall
 !! times are printed.
+!!
+!! Unsteady decks with implicit_time_method=3 run the dual-time backward
+!! Euler steps instead (fvm2d.f90:60-120, backwardEuler_stage of
+!! Time_BackwardEuler.f90:4-56 without its closing diagnostic
+!! reconstruction/flux_sourceterm, which change neither PA nor WA): inner
+!! iteration k overall (counted across steps) dumps s<k>_res, s<k>_dPA,
+!! s<k>_PA, s<k>_WA for k <= NDUMP, and resid.bin holds rows
+!! (istep, inner, Res(1:Nvar)/total_volume, Romax).
 program rc_ref_dump
     use mainvar
     use Parallel_Var
     implicit none
     include 'mpif.h'
-    integer :: istep, nsteps, ndump, ierr, timing, k
+    integer :: istep, nsteps, ndump, ierr, timing, k, inner, kin
     character(len=256) :: arg, outdir
-    double precision :: res(64), romax, t0, t1, tstage(6)
+    double precision :: res(64), romax, t0, t1, tstage(6), roress, roress1, ttime_n
     integer(8) :: c0, c1, crate
 
     call get_command_argument(1, arg); read(arg,*) nsteps
@@ -44,7 +52,44 @@ program rc_ref_dump
     open(91, file=trim(outdir)//'/resid.bin', access='stream', form='unformatted', status='replace')
     tstage = 0.
     call system_clock(c0, crate)
-    do istep = 1, nsteps
+    if (.not. IfSteady) then
+        if (IfExplicit .or. implicit_time_method /= 3) then
+            write(*,*) 'rc_ref_dump: unsteady runs are dumped for implicit_time_method=3 only'
+            stop 2
+        endif
+        kin = 0
+        do istep = 1, nsteps
+            ttime_n = ttime
+            ttime = ttime_n + cs(1)*dt
+            call compute_p_out
+            do inner = 1, InnerLoop
+                kin = kin + 1
+                call TimeStep
+                call reconstruction
+                call flux_sourceterm
+                if (kin <= ndump) call dump_step_d(outdir, kin, 'res', hWA(:,1:NE))
+                call backwardEuler_LU_SGS(inner)
+                if (kin <= ndump) then
+                    call dump_step_d(outdir, kin, 'dPA', hWA(:,1:NE))
+                    call dump_step_d(outdir, kin, 'PA', PA(:,1:NE))
+                    call dump_step_d(outdir, kin, 'WA', WA(:,1:NE))
+                endif
+                call Residual(res(1:Nvar), romax)
+                res(1:Nvar) = res(1:Nvar) / total_volume
+                write(91) dble(istep), dble(inner), res(1:Nvar), romax
+                roress = sum(res(Nflow+1:Nvar))
+                if (inner == 1) roress1 = roress
+                if (roress < roress1*relTol .or. inner == InnerLoop) exit
+                if (isnan(WA(1,1))) then
+                    write(*,*) 'NaN at step', istep, 'inner', inner
+                    stop 3
+                endif
+            enddo
+            ttime = ttime_n + dt
+            call compute_p_out
+        enddo
+    endif
+    do istep = 1, merge(nsteps, 0, IfSteady)
         if (istep <= ndump .and. timing == 0) then
             call TimeStep
             call dump_step_d(outdir, istep, 'step', step(1:NE))

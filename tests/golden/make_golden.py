@@ -15,6 +15,7 @@ the product library for the multicolour ordering utility)
 """
 from __future__ import annotations
 
+import dataclasses
 import os
 import shutil
 import subprocess
@@ -64,12 +65,16 @@ def run_reference(case: nozzle.Case, nsteps: int, ndump: int, expect_error: bool
         d["ref_total_volume"] = tv[0]
         d["ref_p_out"] = tv[1]
         d["ref_init_WA"] = rd("init_WA.bin", (ne, nv))
+        unsteady = not case.info.steady
+        names = ([("res", (ne, nv)), ("dPA", (ne, nv)), ("PA", (ne, nv)), ("WA", (ne, nv))] if unsteady else
+                 [("step", (ne,)), ("alpaF", (nf,)), ("GPA", (ng, nv)), ("gradC0", (ne, nv, 2)),
+                  ("gradC", (ne, nv, 2)), ("res", (ne, nv)), ("dW", (ne, nv)), ("PA", (ne, nv)), ("WA", (ne, nv))])
         for s in range(1, ndump + 1):
-            for n, shp in [("step", (ne,)), ("alpaF", (nf,)), ("GPA", (ng, nv)), ("gradC0", (ne, nv, 2)),
-                           ("gradC", (ne, nv, 2)), ("res", (ne, nv)), ("dW", (ne, nv)), ("PA", (ne, nv)),
-                           ("WA", (ne, nv))]:
-                d["ref_s%d_%s" % (s, n)] = rd("s%d_%s.bin" % (s, n), shp)
-        resid = np.fromfile(os.path.join(od, "resid.bin")).reshape(-1, nv + 1)
+            for n, shp in names:
+                if os.path.exists(os.path.join(od, "s%d_%s.bin" % (s, n))):
+                    d["ref_s%d_%s" % (s, n)] = rd("s%d_%s.bin" % (s, n), shp)
+        # unsteady rows: (istep, inner, Res(1:Nvar), Romax)
+        resid = np.fromfile(os.path.join(od, "resid.bin")).reshape(-1, nv + (3 if unsteady else 1))
         d["ref_resid"] = resid
         if stopped:
             d["ref_error_step"] = len(resid) + 1
@@ -95,6 +100,9 @@ def case_arrays(case: nozzle.Case) -> dict:
         "ox_y": case.oxidizer.y, "ox_tpm": np.array([case.oxidizer.temp, case.oxidizer.pres, case.oxidizer.mass_flow]),
         "fu_y": case.fuel.y, "fu_tpm": np.array([case.fuel.temp, case.fuel.pres, case.fuel.mass_flow]),
         "info": np.array([case.info.cfl, case.info.limit_id, float(case.info.combustion), case.info.p_out_ratio]),
+        # dual-time backward Euler: steady flag, dt, InnerLoop, relTol, A_p, f_p
+        "unsteady": np.array([float(case.info.steady), case.info.totaltime / case.info.nstep_max,
+                              case.info.innerloop, case.info.reltol, case.info.a_p, case.info.f_p]),
     }
 
 
@@ -129,6 +137,18 @@ def main() -> None:
     make("n24x6_nolimiter_stop", nozzle.nozzle_case(24, 6, limit_id=0), nsteps=20, ndump=2, expect_error=True)
     make("n24x6_bc_variants", nozzle.nozzle_case(24, 6, wall_type=1, fuel_faces=2, subsonic_exit=True,
                                                  p_out_ratio=0.9), nsteps=20, ndump=2)
+    # dual-time backward Euler with the point-implicit LU-SGS
+    # (implicit_time_method=3); totaltime is set so that dt = totaltime/nstep_max
+    def be(case, nsteps, dt, inner, reltol, a_p=0.0, f_p=0.0):
+        case.info = dataclasses.replace(case.info, steady=False, implicit_time_method=3, innerloop=inner,
+                                        reltol=reltol, totaltime=dt * nsteps, nstep_max=nsteps, a_p=a_p, f_p=f_p)
+        return case
+    make("n24x6_be_bc_variants", be(nozzle.nozzle_case(24, 6, wall_type=1, fuel_faces=2, subsonic_exit=True,
+                                                       p_out_ratio=0.9), 8, 1e-5, 4, 0.7, 0.1, 5000.0),
+         nsteps=8, ndump=4)
+    make("n24x6_be_model1", be(nozzle.nozzle_case(24, 6), 30, 1e-4, 3, 0.5), nsteps=30, ndump=2)
+    make("n24x6_be_model0_stop", be(nozzle.nozzle_case(24, 6, mech=deck.default_mechanism(0, True)), 8, 1e-6, 3, 0.5),
+         nsteps=8, ndump=2, expect_error=True)
     conv = nozzle.nozzle_case(30, 8)
     make("n30x8_converged", conv, nsteps=8000, ndump=0, every=10)
     perm, _ = native.order_multicolor(conv.mesh.cells)
