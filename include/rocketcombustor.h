@@ -39,7 +39,8 @@ enum {
     RC_ERR_NAN = 4,           /* NaN in WA(1,1): Time_LUSGS_steady.f90:24-27 */
     RC_ERR_MESH = 5,          /* mesh set-up checks of mesh.f90 / ghostcell.f90 / reconstruction_initialization.f90 */
     RC_ERR_DEVICE = 6,        /* HIP runtime error */
-    RC_ERR_MEMORY = 7         /* device memory exhausted / budget exceeded */
+    RC_ERR_MEMORY = 7,        /* device memory exhausted / budget exceeded */
+    RC_ERR_SINGULAR = 8       /* zero pivot / zero row or column in a point-implicit block: la_math.f90:177-209 */
 };
 
 /* Species, thermo and kinetics tables: module species (species.f90:4-12),
@@ -132,6 +133,33 @@ int rc_lusgs_stage(rc_case *h);
  * asynchronously between reports; `res_out` may be NULL.  Returns the
  * number of completed steps in *done. */
 int rc_run_steady(rc_case *h, int nsteps, int every, double *res_out, int *done);
+
+/* ---- dual-time backward Euler with the point-implicit LU-SGS
+ * (implicit_time_method = 3; fvm2d.f90:60-110 unsteady branch). */
+
+/* Replaces compute_p_out (fvm2d.f90:229-239): the exit pressure the ghost
+ * update uses from the next call on (the host evaluates
+ * p_static*(1 + A_p*sin(2*pi*f_p*ttime)) and passes the value). */
+int rc_set_outlet_pressure(rc_case *h, double p_out);
+
+/* Replaces backwardEuler_LU_SGS(Inner_rk) (Time_BackwardEuler.f90:58-324),
+ * after rc_timestep, rc_reconstruction and rc_flux_sourceterm: at inner == 1
+ * fixed = vol/dt*WA; dd = sum 0.5*alpaF + vol*(1/step + 1/dt); hWA = hWA -
+ * vol/dt*WA + fixed; forward and backward point-implicit sweeps solving each
+ * cell's block dU/dPA*dd - vol*d(omega)/dPA as dgesvx('E') does; PA += hWA;
+ * WA = U(PA).  hWA holds the primitive increment afterwards. */
+int rc_backward_euler_lusgs(rc_case *h, double dt, int inner);
+
+/* Replaces backwardEuler_stage(istep) (Time_BackwardEuler.f90:4-56): inner
+ * iterations of TimeStep, reconstruction, flux_sourceterm,
+ * backwardEuler_LU_SGS and inner_iteration_residual (residual.f90:23-59)
+ * until RoRes = sum of the species norms drops below relTol times its value
+ * at the first inner iteration, or innerloop iterations.  res_out (may be
+ * NULL) receives one row per inner iteration: Res(k)/total_volume, k <
+ * nvar, then max(rho); *ninner the number of inner iterations.  The closing
+ * reconstruction + flux_sourceterm of the reference (:37-38) change neither
+ * PA nor WA and are not run.  The exit pressure is the last one set. */
+int rc_backward_euler_stage(rc_case *h, double dt, int innerloop, double reltol, int *ninner, double *res_out);
 
 /* Replaces Residual + the normalisation of step_residual (residual.f90:4-21,
  * 61-79): res[k] = sum_i vol_i |hWA(k,i)| / total_volume, res[nvar] = max rho. */

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -26,6 +27,10 @@ void launch_update(const LaunchSet &, hipStream_t);
 void launch_prim_to_cons(const LaunchSet &, hipStream_t);
 void launch_resnorm(const LaunchSet &, hipStream_t);
 void launch_derived(const LaunchSet &, double *, hipStream_t);
+void launch_be_prepare(const LaunchSet &, double, bool, hipStream_t);
+void launch_be_level(const LaunchSet &, bool, int, hipStream_t);
+void launch_be_update(const LaunchSet &, hipStream_t);
+int pi_chunk();
 }  // namespace rc
 
 using namespace rc;
@@ -40,6 +45,7 @@ struct rc_case {
     rc_mechanism mech{};
     rc_params par{};
     int ne = 0, nf = 0, ng = 0, nvar = kNvar, nlev = 0;
+    int npl = 0;                   // point-implicit sweep levels
     DevMesh M{};
     DevState S{};
     DevThermo *d_th = nullptr;
@@ -50,6 +56,7 @@ struct rc_case {
     // that batches can concatenate them
     std::vector<int4> w_cell, w_face, w_ghost;
     std::vector<std::vector<int4>> w_level;
+    std::vector<std::vector<int4>> w_pi;   // point-implicit sweep levels
     LaunchSet L;                   // the case alone
     rc_batch *batch = nullptr;     // batch the case belongs to, if any
     void *arena = nullptr;
@@ -117,6 +124,9 @@ const char *kernel_name(int k) {
     case K_BWD: return "LU-SGS backward sweep";
     case K_UPDATE: return "update_primitve_variables";
     case K_P2C: return "primitve_to_conserved_variables";
+    case K_BE_FWD: return "backward Euler forward sweep";
+    case K_BE_BWD: return "backward Euler backward sweep";
+    case K_BE_UPDATE: return "backward Euler update";
     default: return "?";
     }
 }
@@ -130,7 +140,8 @@ int check_device_error(rc_case *h) {
     char buf[256];
     const char *what = code == RC_ERR_TEMPERATURE ? "temperature outside the thermo polynomial range"
                      : code == RC_ERR_NEGATIVE ? "negative density or pressure"
-                     : code == RC_ERR_NAN ? "NaN in WA(1,1)" : "error";
+                     : code == RC_ERR_NAN ? "NaN in WA(1,1)"
+                     : code == RC_ERR_SINGULAR ? "error in routine dgesvx (singular block)" : "error";
     std::snprintf(buf, sizeof buf, "%s in %s (index %d, step %d)", what, kernel_name(h->h_err[2]), h->h_err[1] + 1,
                   h->h_err[3]);
     HIPCHK(h, hipMemsetAsync(h->S.err, 0, 4 * sizeof(int), h->stream));
@@ -303,10 +314,15 @@ int layout_and_upload(rc_case *h) {
         chunk_list(h->w_ghost, 0, 0, m.ng, kChunk);
         for (int l = 0; l < nlev; ++l) chunk_list(h->w_level[l], 0, m.lvl_ptr[l], m.lvl_ptr[l + 1], kRunChunk);
     }
+    const int npl = (int)m.pi_lvl_ptr.size() - 1;
+    h->w_pi.assign(npl, {});
+    for (int l = 0; l < npl; ++l) chunk_list(h->w_pi[l], 0, m.pi_lvl_ptr[l], m.pi_lvl_ptr[l + 1], pi_chunk());
     size_t nlw = 0;
     for (int l = 0; l < nlev; ++l) nlw += h->w_level[l].size();
+    size_t npw = 0;
+    for (int l = 0; l < npl; ++l) npw += h->w_pi[l].size();
     const int npartial = (int)h->w_cell.size();
-    int4 *d_wc = nullptr, *d_wf = nullptr, *d_wg = nullptr, *d_wl = nullptr;
+    int4 *d_wc = nullptr, *d_wf = nullptr, *d_wg = nullptr, *d_wl = nullptr, *d_wp = nullptr;
     double *d_partial = nullptr, *d_resout = nullptr;
     auto plan = [&](Carver &c, DevMesh &M, DevState &S) {
         M.ne = m.ne; M.nf = m.nf; M.ng = m.ng; M.sc = sc; M.sf = sf; M.sg = sg;
@@ -321,6 +337,7 @@ int layout_and_upload(rc_case *h) {
         S.err = c.take<int>(8);
         S.lucoef = c.take<double>((size_t)kRecLC * sc);
         S.lu_af = c.take<double>((size_t)3 * sc);
+        S.fixed = c.take<double>((size_t)kNvar * sc);
         M.lu_nb = c.take<int>((size_t)3 * sc);
         M.lu_n = c.take<double>((size_t)6 * sc);
         M.vol = c.take<double>(sc);
@@ -330,6 +347,7 @@ int layout_and_upload(rc_case *h) {
         M.stencil = c.take<int>((size_t)3 * sc);
         M.cface = c.take<int>((size_t)3 * sc);
         M.runs = c.take<int>(m.runs.size());
+        M.pi_cells = c.take<int>(m.pi_cells.size());
         M.vecx = c.take<double>((size_t)2 * sf);
         M.fvl = c.take<double>((size_t)2 * sf);
         M.fvr = c.take<double>((size_t)2 * sf);
@@ -347,6 +365,7 @@ int layout_and_upload(rc_case *h) {
         d_wf = c.take<int4>(h->w_face.size());
         d_wg = c.take<int4>(h->w_ghost.size());
         d_wl = c.take<int4>(nlw);
+        d_wp = c.take<int4>(npw);
     };
     Carver dry;
     DevMesh M{};
@@ -384,6 +403,7 @@ int layout_and_upload(rc_case *h) {
     auto cf = soa(m.cface, m.ne, 3, sc);
     if ((rc = up(M.cface, cf.data(), sizeof(int) * cf.size()))) return rc;
     if ((rc = up(M.runs, m.runs.data(), sizeof(int) * m.runs.size()))) return rc;
+    if ((rc = up(M.pi_cells, m.pi_cells.data(), sizeof(int) * m.pi_cells.size()))) return rc;
     auto vx = soa(m.vecx, m.nf, 2, sf);
     if ((rc = up(M.vecx, vx.data(), sizeof(double) * vx.size()))) return rc;
     auto fl = soa(m.fvl, m.nf, 2, sf);
@@ -427,6 +447,14 @@ int layout_and_upload(rc_case *h) {
         const size_t n = h->w_level[l].size();
         if ((rc = up(d_wl + off, h->w_level[l].data(), sizeof(int4) * n))) return rc;
         L.level[l] = {d_wl + off, (int)n, m.lvl_ptr[l], m.lvl_ptr[l + 1]};
+        off += n;
+    }
+    L.pi_level.assign(npl, {});
+    off = 0;
+    for (int l = 0; l < npl; ++l) {
+        const size_t n = h->w_pi[l].size();
+        if ((rc = up(d_wp + off, h->w_pi[l].data(), sizeof(int4) * n))) return rc;
+        L.pi_level[l] = {d_wp + off, (int)n, m.pi_lvl_ptr[l], m.pi_lvl_ptr[l + 1]};
         off += n;
     }
     L.resout = d_resout;
@@ -525,6 +553,16 @@ int residual_async(rc_case *h, double *host_out) {
     launch_resnorm(h->L, h->stream);
     HIPCHK(h, hipMemcpyAsync(host_out, h->L.resout, sizeof(double) * (kNvar + 1), hipMemcpyDeviceToHost, h->stream));
     return RC_OK;
+}
+
+// backwardEuler_LU_SGS: dual-time right-hand side and diagonal, the
+// point-implicit sweeps level by level, PA += dPA and WA = U(PA)
+void enqueue_be_lusgs(rc_case *h, double dt, bool first) {
+    const LaunchSet &L = h->L;
+    launch_be_prepare(L, dt, first, h->stream);
+    for (int l = 0; l < h->npl; ++l) launch_be_level(L, true, l, h->stream);
+    for (int l = h->npl - 1; l >= 0; --l) launch_be_level(L, false, l, h->stream);
+    launch_be_update(L, h->stream);
 }
 
 // copies n items of `width` values out of a device array laid out either as
@@ -716,6 +754,7 @@ int rc_case_create(const rc_mechanism *mech, const rc_params *par, int nnt, cons
     h->nf = h->hm.nf;
     h->ng = h->hm.ng;
     h->nlev = (int)h->hm.lvl_ptr.size() - 1;
+    h->npl = (int)h->hm.pi_lvl_ptr.size() - 1;
     auto bail = [&](int code) {
         g_error = h->err;
         rc_case_destroy(h);
@@ -848,6 +887,60 @@ int rc_run_steady(rc_case *h, int nsteps, int every, double *res_out, int *done)
     int rc = sync_check(h);
     if (done) *done = nsteps;
     return rc;
+}
+
+int rc_set_outlet_pressure(rc_case *h, double p_out) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    HIPCHK(h, hipSetDevice(h->device));
+    h->par.p_out = p_out;
+    HIPCHK(h, hipMemcpyAsync(reinterpret_cast<char *>(h->d_par) + offsetof(DevParams, p_out), &h->par.p_out,
+                             sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return RC_OK;
+}
+
+int rc_backward_euler_lusgs(rc_case *h, double dt, int inner) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    if (!(dt > 0.0) || inner < 1) return set_err(h, RC_ERR_ARGUMENT, "dt must be > 0 and inner >= 1");
+    HIPCHK(h, hipSetDevice(h->device));
+    enqueue_be_lusgs(h, dt, inner == 1);
+    return sync_check(h);
+}
+
+int rc_backward_euler_stage(rc_case *h, double dt, int innerloop, double reltol, int *ninner, double *res_out) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    if (!(dt > 0.0) || innerloop < 1) return set_err(h, RC_ERR_ARGUMENT, "dt must be > 0 and innerloop >= 1");
+    HIPCHK(h, hipSetDevice(h->device));
+    const LaunchSet &L = h->L;
+    launch_step_begin(L, h->stream);
+    double ro1 = 0.0;
+    int inner = 1;
+    for (;; ++inner) {
+        launch_ts_face(L, h->stream);
+        launch_ts_cell(L, h->stream);
+        launch_gradlim(L, true, 2, h->stream);
+        launch_flux(L, false, h->stream);
+        launch_residual(L, false, h->stream);
+        enqueue_be_lusgs(h, dt, inner == 1);
+        int rc = residual_async(h, h->h_res);
+        if (rc) return rc;
+        rc = sync_check(h);
+        if (ninner) *ninner = inner;
+        if (rc) return rc;
+        if (res_out) std::memcpy(res_out + (size_t)(inner - 1) * (kNvar + 1), h->h_res, sizeof(double) * (kNvar + 1));
+        double ro = 0.0;
+        for (int k = kNFlow; k < kNvar; ++k) ro = ro + h->h_res[k];
+        if (inner == 1) ro1 = ro;
+        if (ro < ro1 * reltol || inner == innerloop) break;
+        // the NaN check of backwardEuler_stage (Time_BackwardEuler.f90:44-47)
+        double wa11 = 0.0;
+        HIPCHK(h, hipMemcpyAsync(&wa11, h->S.wa, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (std::isnan(wa11)) return set_err(h, RC_ERR_NAN, "there is NAN in WA(1,1) (inner iteration " +
+                                                                std::to_string(inner) + ")");
+    }
+    h->steps_done++;
+    return RC_OK;
 }
 
 int rc_residual(rc_case *h, double *res) {

@@ -23,6 +23,7 @@ struct DevMesh {
     const int *__restrict__ ghost_host;
     const int *__restrict__ ghost_type;
     const int *__restrict__ runs;        // LU-SGS runs (first cell, count), by level
+    const int *__restrict__ pi_cells;    // point-implicit sweep: cells by level
     const double *__restrict__ vecx;
     const double *__restrict__ fvl;
     const double *__restrict__ fvr;
@@ -64,6 +65,7 @@ struct DevState {
     double *fflux;   // kRecF per face: F(1:7), alpaF
     double *lucoef;  // kRecLC per cell
     double *lu_af;   // alpaF of the cell's 3 face slots, 3 x sc
+    double *fixed;   // backward Euler: vol/dt*WA at the first inner iteration, Nvar arrays of sc
     int *err;        // [0] code, [1] index, [2] kernel, [3] step of the error, [4] current step
 };
 
@@ -96,6 +98,7 @@ struct LaunchSet {
     bool single = false;           // one case: cases[0], launched over [lo, hi)
     WorkList cell, face, ghost;
     std::vector<WorkList> level;   // LU-SGS runs of each level
+    std::vector<WorkList> pi_level;   // point-implicit sweep: cells (positions in pi_cells) of each level
     double *resout = nullptr;      // ncases x (kNvar+1) residual norms
 };
 
@@ -107,11 +110,11 @@ struct LaunchSet {
 struct GMesh {
     int ne, nf, ng;
     int sc, sf, sg;
-    const RC_G int *fnl, *fnr, *stencil, *cface, *ghost_host, *ghost_type, *runs, *lu_nb;
+    const RC_G int *fnl, *fnr, *stencil, *cface, *ghost_host, *ghost_type, *runs, *lu_nb, *pi_cells;
     const RC_G double *vecx, *fvl, *fvr, *vol, *wdis, *amls, *lvv, *gnrm, *lu_n;
 };
 struct GState {
-    RC_G double *pa, *wa, *hwa, *gradc, *gpa, *step, *dd, *fflux, *lucoef, *lu_af;
+    RC_G double *pa, *wa, *hwa, *gradc, *gpa, *step, *dd, *fflux, *lucoef, *lu_af, *fixed;
     RC_G int *err;
 };
 typedef const RC_G DevThermo GThermo;
@@ -127,6 +130,7 @@ __device__ __forceinline__ GMesh gview(const RC_G DevMesh &m) {
     g.ne = m.ne; g.nf = m.nf; g.ng = m.ng; g.sc = m.sc; g.sf = m.sf; g.sg = m.sg;
     g.fnl = gp(m.fnl); g.fnr = gp(m.fnr); g.stencil = gp(m.stencil); g.cface = gp(m.cface);
     g.ghost_host = gp(m.ghost_host); g.ghost_type = gp(m.ghost_type); g.runs = gp(m.runs); g.lu_nb = gp(m.lu_nb);
+    g.pi_cells = gp(m.pi_cells);
     g.vecx = gp(m.vecx); g.fvl = gp(m.fvl); g.fvr = gp(m.fvr); g.vol = gp(m.vol); g.wdis = gp(m.wdis);
     g.amls = gp(m.amls); g.lvv = gp(m.lvv); g.gnrm = gp(m.gnrm); g.lu_n = gp(m.lu_n);
     return g;
@@ -136,6 +140,7 @@ __device__ __forceinline__ GState gview(const RC_G DevState &s) {
     GState g;
     g.pa = gp(s.pa); g.wa = gp(s.wa); g.hwa = gp(s.hwa); g.gradc = gp(s.gradc); g.gpa = gp(s.gpa);
     g.step = gp(s.step); g.dd = gp(s.dd); g.fflux = gp(s.fflux); g.lucoef = gp(s.lucoef); g.lu_af = gp(s.lu_af);
+    g.fixed = gp(s.fixed);
     g.err = gp(s.err);
     return g;
 }
@@ -143,7 +148,8 @@ __device__ __forceinline__ GState gview(const RC_G DevState &s) {
 constexpr int kChunk = 256;          // cells / faces / ghosts per thread block
 constexpr int kRunChunk = kChunk / kRunMax;   // LU-SGS runs per thread block
 
-enum KernelId { K_TS_FACE = 1, K_TS_CELL, K_GHOST, K_GRADLIM, K_FLUX, K_RES, K_FWD, K_BWD, K_UPDATE, K_P2C };
+enum KernelId { K_TS_FACE = 1, K_TS_CELL, K_GHOST, K_GRADLIM, K_FLUX, K_RES, K_FWD, K_BWD, K_UPDATE, K_P2C,
+                K_BE_FWD, K_BE_BWD, K_BE_UPDATE };
 
 // first error wins; the step number comes from the device-side step
 // counter err[4] (advanced by k_step_begin), so captured graphs report it too

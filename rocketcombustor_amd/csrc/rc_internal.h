@@ -48,6 +48,11 @@ struct HostMesh {
     // consecutive, face-connected cells; one thread sweeps a run in order.
     std::vector<int> runs;           // 2*nruns (first cell, cell count), sorted by level
     std::vector<int> lvl_ptr;        // nlev+1  runs of level l: runs[lvl_ptr[l] .. lvl_ptr[l+1])
+    // point-implicit (backward Euler) sweep schedule: a cell's level is 1 +
+    // the highest level of its lower neighbours; one lane group solves one
+    // cell, so the cells of a level are independent
+    std::vector<int> pi_cells;       // ne      cells sorted by level (ascending within a level)
+    std::vector<int> pi_lvl_ptr;     // npl+1   cells of level l: pi_cells[pi_lvl_ptr[l] .. pi_lvl_ptr[l+1])
 };
 
 // Builds HostMesh from 0-based inputs; returns RC_OK or RC_ERR_MESH with msg.

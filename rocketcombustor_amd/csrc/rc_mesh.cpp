@@ -413,6 +413,26 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
         m.runs[2 * t] = run_first[r];
         m.runs[2 * t + 1] = (r + 1 < nruns ? run_first[r + 1] : ne) - run_first[r];
     }
+
+    // ---- point-implicit sweep schedule (backward Euler): cell levels.  A
+    // cell's lower neighbours are in lower levels, its upper neighbours in
+    // higher ones, so the forward sweep runs the levels up and the backward
+    // sweep down, each level's cells at once.
+    std::vector<int> clev(ne, 0);
+    int npl = 0;
+    for (int i = 0; i < ne; ++i) {
+        for (int k = 0; k < m.neigh[i]; ++k) {
+            const int jc = m.neighc[3 * i + k];
+            if (jc < i) clev[i] = std::max(clev[i], clev[jc] + 1);
+        }
+        npl = std::max(npl, clev[i] + 1);
+    }
+    m.pi_lvl_ptr.assign(npl + 1, 0);
+    for (int i = 0; i < ne; ++i) m.pi_lvl_ptr[clev[i] + 1]++;
+    for (int l = 0; l < npl; ++l) m.pi_lvl_ptr[l + 1] += m.pi_lvl_ptr[l];
+    m.pi_cells.assign(ne, 0);
+    std::vector<int> cpos(m.pi_lvl_ptr.begin(), m.pi_lvl_ptr.end() - 1);
+    for (int i = 0; i < ne; ++i) m.pi_cells[cpos[clev[i]]++] = i;
     return RC_OK;
 }
 

@@ -14,6 +14,9 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "librocketcombustor.so")
 MAXSP = 8
 NVAR = 7
+# return codes (include/rocketcombustor.h)
+RC_OK, RC_ERR_ARGUMENT, RC_ERR_TEMPERATURE, RC_ERR_NEGATIVE, RC_ERR_NAN = 0, 1, 2, 3, 4
+RC_ERR_MESH, RC_ERR_DEVICE, RC_ERR_MEMORY, RC_ERR_SINGULAR = 5, 6, 7, 8
 
 _LIB = None
 
@@ -63,6 +66,9 @@ SIGNATURES = {
     "rc_lusgs_stage": (C.c_int, [_VP]),
     "rc_run_steady": (C.c_int, [_VP, C.c_int, C.c_int, _DP, _IP]),
     "rc_residual": (C.c_int, [_VP, _DP]),
+    "rc_set_outlet_pressure": (C.c_int, [_VP, C.c_double]),
+    "rc_backward_euler_lusgs": (C.c_int, [_VP, C.c_double, C.c_int]),
+    "rc_backward_euler_stage": (C.c_int, [_VP, C.c_double, C.c_int, C.c_double, _IP, _DP]),
     "rc_get_primitive": (C.c_int, [_VP, _DP]),
     "rc_get_conserved": (C.c_int, [_VP, _DP]),
     "rc_get_increment": (C.c_int, [_VP, _DP]),

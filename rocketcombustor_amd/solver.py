@@ -1,5 +1,6 @@
 """Host-side mirror of the reference's operator interface for the steady
-path, driving the HIP library through its C-ABI.
+path and the dual-time backward-Euler path, driving the HIP library
+through its C-ABI.
 
 The reference's operators are argument-less subroutines over module state;
 here that state lives in a device-resident :class:`SteadyCase`, and the
@@ -15,6 +16,11 @@ methods keep the reference's names and call sequence:
     run_steady(n)       steady main loop    src/fvm2d.f90:48-150
     residual()          Residual/step_residual  src/residual.f90:4-21,61-79
 
+    set_outlet_pressure(p)        compute_p_out         src/fvm2d.f90:229-239
+    backward_euler_lusgs(dt, k)   backwardEuler_LU_SGS  src/Time_BackwardEuler.f90:58
+    backward_euler_stage(dt, ..)  backwardEuler_stage   src/Time_BackwardEuler.f90:4
+    run_unsteady(n)     unsteady main loop  src/fvm2d.f90:48-120 (implicit_time_method=3)
+
 Errors where the reference stops raise :class:`native.RcError`.
 
 :class:`SteadyBatch` steps many resident cases with the same launches (no
@@ -23,6 +29,7 @@ reference counterpart); each member computes exactly what it computes alone.
 from __future__ import annotations
 
 import ctypes as C
+import math
 from typing import Optional
 
 import numpy as np
@@ -31,8 +38,14 @@ from . import native
 from .native import NVAR, check, dptr, iptr, lib
 
 
+def outlet_pressure(p_static: float, a_p: float, f_p: float, ttime: float) -> float:
+    """compute_p_out (src/fvm2d.f90:229-239): p_static*(1 + A_p sin(2 pi f_p t))."""
+    return p_static * (1. + a_p * math.sin(2 * math.pi * f_p * ttime))
+
+
 class SteadyCase:
-    """One nozzle/chamber case resident on a GPU."""
+    """One nozzle/chamber case resident on a GPU (steady LU-SGS steps and
+    dual-time backward-Euler steps act on the same resident state)."""
 
     def __init__(self, case, device: int = 0):
         L = lib()
@@ -151,6 +164,58 @@ class SteadyCase:
             return None
         nrep_done = (1 if nsteps >= 1 else 0) + sum(1 for s in range(2, nsteps + 1) if s % every == 0)
         return buf[:nrep_done]
+
+    # -- dual-time backward Euler (implicit_time_method = 3) ----------------
+    def set_outlet_pressure(self, p_out: float) -> None:
+        self._call("rc_set_outlet_pressure", float(p_out))
+
+    def backward_euler_lusgs(self, dt: float, inner: int) -> None:
+        """backwardEuler_LU_SGS(inner) after timestep, reconstruction and
+        flux_sourceterm; increment() is then the primitive increment."""
+        self._call("rc_backward_euler_lusgs", float(dt), int(inner))
+
+    def backward_euler_stage(self, dt: float, innerloop: int, reltol: float) -> np.ndarray:
+        """backwardEuler_stage: the inner iterations of one physical step;
+        returns their inner_iteration_residual rows, shape (ninner, Nvar+1)."""
+        buf = np.zeros((max(int(innerloop), 1), NVAR + 1))
+        n = C.c_int(0)
+        rc = lib().rc_backward_euler_stage(self.h, float(dt), int(innerloop), float(reltol), C.byref(n), dptr(buf))
+        if rc:
+            try:
+                check(rc, self.h)
+            except native.RcError as e:
+                e.inner = n.value
+                raise
+        return buf[:n.value]
+
+    def run_unsteady(self, nsteps: int, dt: Optional[float] = None, innerloop: Optional[int] = None,
+                     reltol: Optional[float] = None, a_p: Optional[float] = None, f_p: Optional[float] = None,
+                     ttime: float = 0.0) -> np.ndarray:
+        """The unsteady branch of the main loop with backward Euler
+        (src/fvm2d.f90:48-120): per step ttime += dt, compute_p_out, the
+        stage, and compute_p_out at the step's end time.  Parameters default
+        to the case's solver settings (dt = totaltime/nstep_max, InnerLoop,
+        relTol, A_p, f_p); p_static is the case's exit pressure.  Returns
+        rows (istep, inner, Res/total_volume, max rho) of every inner
+        iteration; self.ttime holds the time reached."""
+        info = self.case.info
+        dt = info.totaltime / info.nstep_max if dt is None else dt
+        innerloop = info.innerloop if innerloop is None else innerloop
+        reltol = info.reltol if reltol is None else reltol
+        a_p = info.a_p if a_p is None else a_p
+        f_p = info.f_p if f_p is None else f_p
+        p_static = self.case.p_out
+        rows = []
+        for istep in range(1, int(nsteps) + 1):
+            tn = ttime
+            ttime = tn + 1. * dt
+            self.set_outlet_pressure(outlet_pressure(p_static, a_p, f_p, ttime))
+            h = self.backward_euler_stage(dt, innerloop, reltol)
+            rows += [np.r_[istep, k + 1, h[k]] for k in range(len(h))]
+            ttime = tn + dt
+        self.set_outlet_pressure(outlet_pressure(p_static, a_p, f_p, ttime))
+        self.ttime = ttime
+        return np.array(rows).reshape(-1, NVAR + 3)
 
     # -- output and restart in the reference's formats ---------------------
     def derived(self) -> dict:

@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import glob
+import math
 import os
 
 import numpy as np
@@ -100,3 +101,52 @@ def sweep_run_levels(adj, run_max: int = 4):
             if j < first[r]:
                 lev[r] = max(lev[r], lev[run_of[j]] + 1)
     return np.array(first), np.array(lev)
+
+
+def be_params(case):
+    """dt, InnerLoop, relTol, A_p, f_p of a backward-Euler case's solver
+    settings (dt = totaltime/nstep_max as initParam sets it)."""
+    i = case.info
+    return i.totaltime / i.nstep_max, int(i.innerloop), i.reltol, i.a_p, i.f_p
+
+
+def outlet_pressure(p_static, a_p, f_p, t):
+    """compute_p_out, src/fvm2d.f90:229-239."""
+    return p_static * (1. + a_p * math.sin(2 * math.pi * f_p * t))
+
+
+def run_be(o, case, nsteps, on_inner=None, rows=None, lusgs="be_lusgs", set_p="set_p_out"):
+    """The reference's time loop (src/fvm2d.f90:60-110) around
+    backwardEuler_stage (src/Time_BackwardEuler.f90:4-56), driven call by
+    call on ``o`` (the oracle, or a device case with ``lusgs`` /
+    ``set_p`` naming its methods): rows (istep, inner, Res/total_volume,
+    Romax) of every inner iteration, appended to ``rows`` as they come;
+    ``on_inner(kin, "res" | "lu")`` after flux_sourceterm and after the
+    point-implicit update of the kin-th inner iteration overall."""
+    dt, inner, reltol, a_p, f_p = be_params(case)
+    ttime, kin = 0.0, 0
+    rows = [] if rows is None else rows
+    for istep in range(1, nsteps + 1):
+        tn = ttime
+        ttime = tn + 1.0 * dt
+        getattr(o, set_p)(outlet_pressure(case.p_out, a_p, f_p, ttime))
+        ro1 = None
+        for it in range(1, inner + 1):
+            kin += 1
+            o.timestep()
+            o.reconstruction()
+            o.flux_sourceterm()
+            if on_inner:
+                on_inner(kin, "res")
+            getattr(o, lusgs)(dt, it)
+            if on_inner:
+                on_inner(kin, "lu")
+            r = o.residual()
+            rows.append(np.r_[istep, it, r])
+            ro = float(np.sum(r[4:case.mech.nsp + 3]))
+            if it == 1:
+                ro1 = ro
+            if ro < ro1 * reltol or it == inner:
+                break
+        ttime = tn + dt
+    return np.array(rows)

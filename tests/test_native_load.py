@@ -30,6 +30,14 @@ def test_library_exports_header():
     assert set(names) == set(native.SIGNATURES), "ctypes signatures out of sync with the header"
 
 
+def test_return_codes_match_header():
+    text = open(HEADER).read()
+    codes = dict((k, int(v)) for k, v in re.findall(r"\b(RC_(?:OK|ERR_[A-Z]+))\s*=\s*(\d+)", text))
+    assert len(codes) >= 9
+    for k, v in codes.items():
+        assert getattr(native, k) == v, k
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(native, "_LIB", None)
     monkeypatch.setattr(native, "LIB_PATH", str(tmp_path / "absent.so"))

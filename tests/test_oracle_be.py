@@ -18,12 +18,10 @@ term dkfdr*drdT) and leaves dkfdy(3) unassigned
 both as 0, which is what the golden vectors show, so the model-1
 finite-difference check skips those two columns and asserts they are zero.
 """
-import math
-
 import numpy as np
 import pytest
 
-from helpers import case_from_fixture, fixture_names, load, relerr
+from helpers import be_params, case_from_fixture, fixture_names, load, outlet_pressure, relerr, run_be
 from rocketcombustor_amd import deck
 
 oracle = pytest.importorskip("oracle.pyoracle")
@@ -125,45 +123,6 @@ def test_dgesvx_solve_matches_numpy():
         oracle.solve_dgesvx(a, np.ones(7))
 
 
-def outlet_pressure(p_static, a_p, f_p, t):
-    """compute_p_out, src/fvm2d.f90:229-239."""
-    return p_static * (1. + a_p * math.sin(2 * math.pi * f_p * t))
-
-
-def run_be(o, case, d, nsteps, on_inner=None, rows=None):
-    """The reference's time loop (src/fvm2d.f90:60-110) around
-    backwardEuler_stage (src/Time_BackwardEuler.f90:4-56), inner iteration by
-    inner iteration: rows (istep, inner, Res/total_volume, Romax), appended
-    to ``rows`` as they come."""
-    _, dt, inner, reltol, a_p, f_p = d["unsteady"]
-    ttime, kin = 0.0, 0
-    rows = [] if rows is None else rows
-    for istep in range(1, nsteps + 1):
-        tn = ttime
-        ttime = tn + 1.0 * dt
-        o.set_p_out(outlet_pressure(case.p_out, a_p, f_p, ttime))
-        ro1 = None
-        for it in range(1, int(inner) + 1):
-            kin += 1
-            o.timestep()
-            o.reconstruction()
-            o.flux_sourceterm()
-            if on_inner:
-                on_inner(kin, "res")
-            o.be_lusgs(dt, it)
-            if on_inner:
-                on_inner(kin, "lu")
-            r = o.residual()
-            rows.append(np.r_[istep, it, r])
-            ro = float(np.sum(r[NFLOW:case.mech.nsp + NFLOW - 1]))
-            if it == 1:
-                ro1 = ro
-            if ro < ro1 * reltol or it == inner:
-                break
-        ttime = tn + dt
-    return np.array(rows)
-
-
 @pytest.fixture(scope="module", params=NAMES)
 def fx(request):
     d = load(request.param)
@@ -192,9 +151,9 @@ def test_inner_iterations_match_reference(fx):
     if "ref_error_step" in d:
         # the reference stops in the inner iteration after its last row
         with pytest.raises(RuntimeError, match="temperature"):
-            run_be(o, case, d, nsteps, check)
+            run_be(o, case, nsteps, check)
         return
-    rows = run_be(o, case, d, nsteps, check)
+    rows = run_be(o, case, nsteps, check)
     assert seen, "no dumped inner iteration"
     assert rows.shape == ref.shape
     assert np.array_equal(rows[:, :2], ref[:, :2])      # same inner-iteration counts
@@ -211,7 +170,7 @@ def test_stopping_case_rows_before_the_stop(fx):
     ref = d["ref_resid"]
     rows = []
     with pytest.raises(RuntimeError, match="temperature"):
-        run_be(o, case, d, int(d["nsteps"]), rows=rows)
+        run_be(o, case, int(d["nsteps"]), rows=rows)
     got = np.array(rows)
     assert got.shape == ref.shape
     assert np.array_equal(got[:, :2], ref[:, :2])
@@ -224,9 +183,9 @@ def test_stage_entry_matches_inner_loop(fx):
     name, d, case = fx
     if "ref_error_step" in d:
         pytest.skip("stops")
-    _, dt, inner, reltol, a_p, f_p = d["unsteady"]
+    dt, inner, reltol, a_p, f_p = be_params(case)
     a, b = oracle.OracleCase(case), oracle.OracleCase(case)
-    rows = run_be(a, case, d, 3)
+    rows = run_be(a, case, 3)
     ttime, ro_all = 0.0, []
     for istep in range(1, 4):
         tn, ttime = ttime, ttime + dt
