@@ -174,6 +174,56 @@ def batched_run(n: int, steps: int) -> dict:
             "speedup": t_plain / t_batch}
 
 
+def backward_euler_run(case, nsteps: int, dt: float = 1e-7, inner: int = 4, cpu: bool = True) -> dict:
+    """Dual-time backward Euler (implicit_time_method=3) on the same case:
+    ``nsteps`` physical steps of exactly ``inner`` inner iterations (relTol
+    0), each inner iteration TimeStep, reconstruction, flux_sourceterm, the
+    point-implicit LU-SGS (a 7x7 block solve per cell and sweep) and the
+    inner-iteration residual, after one untimed warm-up step."""
+    from rocketcombustor_amd.solver import SteadyCase
+    with SteadyCase(case, device=0) as g:
+        g.run_unsteady(1, dt=dt, innerloop=inner, reltol=0.0, a_p=0.0, f_p=0.0)
+        t0 = time.perf_counter()
+        rows = g.run_unsteady(nsteps, dt=dt, innerloop=inner, reltol=0.0, a_p=0.0, f_p=0.0)
+        wall = time.perf_counter() - t0
+        n_inner = len(rows)
+        assert all(x == x for x in rows[-1]), "non-finite residual"
+        out = {"workload": "same case, dt %g, %d steps x %d inner iterations" % (dt, nsteps, inner),
+               "inner_iterations": n_inner, "ms_per_inner_iteration": 1000.0 * wall / n_inner,
+               "cell_inner_iterations_per_s": g.ne * n_inner / wall}
+    if cpu:
+        out["cpu_baseline"] = backward_euler_cpu(case, dt)
+    return out
+
+
+def backward_euler_cpu(case, dt: float, inner: int = 2) -> dict:
+    """The reference's backward-Euler stage on one host core: one step of
+    ``inner`` inner iterations of the same case (oracle/_ref/rc_ref_dump)."""
+    import dataclasses
+    ref = os.path.join(ROOT, "oracle", "_ref", "rc_ref_dump")
+    if not os.path.exists(ref):
+        return None
+    from rocketcombustor_amd import nozzle
+    c = dataclasses.replace(case, info=dataclasses.replace(case.info, steady=False, implicit_time_method=3,
+                                                           innerloop=inner, reltol=0.0, a_p=0.0, f_p=0.0,
+                                                           nstep_max=1, totaltime=dt))
+    tmp = tempfile.mkdtemp(prefix="rcbench_be_")
+    try:
+        nozzle.write_reference_case(tmp, c, nstep_max=1)
+        os.makedirs(os.path.join(tmp, "out"))
+        r = subprocess.run([ref, "1", "0", "out", "1"], cwd=tmp, capture_output=True, text=True, timeout=900)
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("steps")]
+        if r.returncode != 0 or not line:
+            return None
+        wall = float(line[-1].split()[3])
+        ne = case.mesh.ne
+        return {"value": ne * inner / wall, "unit": "cell-inner-iterations/s", "cores": 1, "kind": "reference",
+                "sample": "1 step x %d inner iterations of the same %d-cell case, reference Fortran (amdflang "
+                          "-O3, MKL dgesvx, 1 rank), wall time %.2f s" % (inner, ne, wall)}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def copy_calibration(mib: int) -> int:
     """Three device-to-device copies of `mib` MiB through the HIP runtime (its
     copy kernel), as the reference point for the HBM byte counters."""
@@ -215,6 +265,9 @@ def main() -> int:
                     help="also time N independent sampled cases (5k-80k cells) stepped as one batch "
                          "against the same cases stepped one at a time")
     ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--backward-euler", type=int, default=0, metavar="N",
+                    help="also time N dual-time backward-Euler steps (point-implicit LU-SGS, 4 inner "
+                         "iterations each) of the same 400k-cell case")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device three times (a plain copy kernel of "
                          "known size, to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -292,6 +345,8 @@ def main() -> int:
     g.close()
     if args.copy_calibration > 0:
         out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
+    if args.backward_euler > 0:
+        out["backward_euler"] = backward_euler_run(case, args.backward_euler, cpu=not args.no_cpu_baseline)
     if args.serve > 0:
         out["serving"] = serving_run(args.serve, args.serve_plain)
     if args.batched > 0:

@@ -12,7 +12,7 @@ module rocketcombustor_iface
     integer(c_int), parameter :: RC_MAXSP = 8
     integer(c_int), parameter :: RC_OK = 0, RC_ERR_ARGUMENT = 1, RC_ERR_TEMPERATURE = 2, &
                                  RC_ERR_NEGATIVE = 3, RC_ERR_NAN = 4, RC_ERR_MESH = 5, &
-                                 RC_ERR_DEVICE = 6, RC_ERR_MEMORY = 7
+                                 RC_ERR_DEVICE = 6, RC_ERR_MEMORY = 7, RC_ERR_SINGULAR = 8
 
     type, bind(c) :: rc_mechanism
         integer(c_int) :: nsp
@@ -111,6 +111,31 @@ module rocketcombustor_iface
             integer(c_int), value :: nsteps, every
             real(c_double), intent(out) :: res_out(*)
             integer(c_int), intent(out) :: done
+        end function
+
+        !! dual-time backward Euler (implicit_time_method = 3): compute_p_out,
+        !! backwardEuler_LU_SGS(Inner_rk), backwardEuler_stage(istep)
+        integer(c_int) function rc_set_outlet_pressure(h, p_out) bind(c, name="rc_set_outlet_pressure")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), value :: p_out
+        end function
+
+        integer(c_int) function rc_backward_euler_lusgs(h, dt, inner) bind(c, name="rc_backward_euler_lusgs")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), value :: dt
+            integer(c_int), value :: inner
+        end function
+
+        integer(c_int) function rc_backward_euler_stage(h, dt, innerloop, reltol, ninner, res_out) &
+                bind(c, name="rc_backward_euler_stage")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), value :: dt, reltol
+            integer(c_int), value :: innerloop
+            integer(c_int), intent(out) :: ninner
+            real(c_double), intent(out) :: res_out(*)
         end function
 
         integer(c_int) function rc_residual(h, res) bind(c, name="rc_residual")

@@ -5,7 +5,13 @@
 !! (src/output.f90:256) -- runs unchanged, linked from the reference's
 !! objects; the only change is the steady step of the main loop
 !! (src/fvm2d.f90:76), where `call lu_sgs_stage(istep)` becomes a call of
-!! the MI355X library through include/rocketcombustor_iface.f90.
+!! the MI355X library through include/rocketcombustor_iface.f90, and, for
+!! unsteady decks with implicit_time_method=3, the backward-Euler stage
+!! (src/fvm2d.f90:93), where `call backwardEuler_stage(istep)` becomes
+!! rc_backward_euler_stage after the reference's own compute_p_out hands
+!! its exit pressure to the library.  Unsteady runs write the inner-iteration
+!! rows (istep, inner, Res/total_volume, Romax) to OUTDIR/resid.bin and
+!! converge_uns.dat.
 !!
 !! usage (in a case directory): rc_ref_host_gpu OUTDIR
 !! writes converge.dat and fvp1.sav as the reference does, and OUTDIR/resid.bin,
@@ -26,8 +32,10 @@ program rc_ref_host_gpu
     type(rc_params) :: par
     type(c_ptr) :: h
     integer(c_int), allocatable :: cells(:), bn(:), bt(:)
-    integer :: i, ib, np, nb, istep, ierr
-    double precision :: time_all, RoRes, res(64), romax
+    integer :: i, ib, np, nb, istep, ierr, k, istop
+    integer(c_int) :: ninner
+    double precision :: time_all, RoRes, res(64), romax, ttime_n, totaltime_num
+    double precision, allocatable :: rows(:, :)
     character(len=256) :: outdir
 
     call get_command_argument(1, outdir)
@@ -83,6 +91,39 @@ program rc_ref_host_gpu
 
     open(91, file=trim(outdir)//'/resid.bin', access='stream', form='unformatted', status='replace')
     time_all = 0.
+    if (.not. IfSteady) then
+        !! the main loop, unsteady implicit branch with backward Euler
+        !! (fvm2d.f90:48-120)
+        if (IfExplicit .or. implicit_time_method /= 3) then
+            write(*,*) 'rc_ref_host_gpu: unsteady runs need implicit_time_method=3'
+            stop 2
+        endif
+        allocate(rows(Nvar + 1, InnerLoop))
+        open(93, file='converge_uns.dat')
+        write(93,*) 'VARIABLES = "Time step","Inner iteration step", "L1(Res)","Romax"'
+        totaltime_num = totaltime*(1.-1.e-10)
+        istop = 0
+        do istep = 1, nstep_max
+            if (ttime + dt > totaltime_num) then
+                dt = totaltime - ttime
+                istop = 1
+            endif
+            ttime_n = ttime
+            ttime = ttime_n + cs(1)*dt
+            call compute_p_out
+            call rc_check(rc_set_outlet_pressure(h, p_out), h, 'compute_p_out')
+            call rc_check(rc_backward_euler_stage(h, dt, InnerLoop, relTol, ninner, rows), h, &
+                          'backwardEuler_stage')                        ! was: call backwardEuler_stage(istep)
+            do k = 1, ninner
+                write(91) dble(istep), dble(k), rows(:, k)
+                write(93,*) istep, k, rows(1:Nvar, k), rows(Nvar + 1, k)
+            enddo
+            ttime = ttime_n + dt
+            call compute_p_out
+            if (istop == 1) exit
+        enddo
+        close(93)
+    else
     !! the main loop, steady implicit branch (fvm2d.f90:48-150)
     do istep = 1, nstep_max
         call rc_check(rc_lusgs_stage(h), h, 'lu_sgs_stage')            ! was: call lu_sgs_stage(istep)
@@ -95,6 +136,7 @@ program rc_ref_host_gpu
             write(91) res(1:Nvar), romax
         endif
     enddo
+    endif
     close(91)
 
     call rc_check(rc_get_primitive(h, PA), h, 'rc_get_primitive')
@@ -104,6 +146,6 @@ program rc_ref_host_gpu
     write(92) PA(:, 1:NE)
     close(92)
     call rc_case_destroy(h)
-    write(*,*) 'rc_ref_host_gpu finished', nstep_max, 'steps'
+    write(*,*) 'rc_ref_host_gpu finished', min(istep, nstep_max), 'steps'
     call MPI_Finalize(ierr)
 end program

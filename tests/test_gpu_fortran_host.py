@@ -1,8 +1,9 @@
 """The drop-in: the reference's own Fortran host (start-up, mesh, restart,
-residual monitor, backup) with its steady step replaced by the MI355X library
-through include/rocketcombustor_iface.f90 (oracle/ref_host_gpu.f90, linked
-from the reference's objects by oracle/ref_build.sh).  Its step_residual
-history and final field must match the reference's own run (golden vectors).
+residual monitor, backup) with its steady step -- or, for unsteady decks,
+its backward-Euler stage -- replaced by the MI355X library through
+include/rocketcombustor_iface.f90 (oracle/ref_host_gpu.f90, linked from the
+reference's objects by oracle/ref_build.sh).  Its residual history and final
+field must match the reference's own run (golden vectors).
 """
 import os
 import subprocess
@@ -20,7 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "oracle", "_ref", "rc_ref_host_gpu")
 
 
-@pytest.mark.parametrize("name", ["n24x6_natural", "n24x6_bc_variants"])
+@pytest.mark.parametrize("name", ["n24x6_natural", "n24x6_bc_variants", "n24x6_be_bc_variants", "n24x6_be_model1"])
 def test_reference_host_with_gpu_step(name):
     if not os.path.exists(BIN):
         pytest.skip("reference host binary not built (oracle/ref_build.sh needs the reference sources)")
@@ -34,11 +35,15 @@ def test_reference_host_with_gpu_step(name):
         os.makedirs(os.path.join(tmp, "out"))
         r = subprocess.run([BIN, "out"], cwd=tmp, capture_output=True, text=True, timeout=120)
         assert "rc_ref_host_gpu finished" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
-        res = np.fromfile(os.path.join(tmp, "out", "resid.bin")).reshape(-1, 8)
+        unsteady = not case.info.steady
+        res = np.fromfile(os.path.join(tmp, "out", "resid.bin")).reshape(-1, 10 if unsteady else 8)
         pa = np.fromfile(os.path.join(tmp, "out", "final_PA.bin")).reshape(case.mesh.ne, 7)
-        assert os.path.exists(os.path.join(tmp, "converge.dat"))
+        assert os.path.exists(os.path.join(tmp, "converge_uns.dat" if unsteady else "converge.dat"))
         assert os.path.exists(os.path.join(tmp, "fvp1.sav"))
     ref = d["ref_resid"]
     assert res.shape == ref.shape
+    if unsteady:
+        assert np.array_equal(res[:, :2], ref[:, :2])   # same inner-iteration counts
+        res, ref = res[:, 2:], ref[:, 2:]
     assert relerr(res[:, :7], ref[:, :7], -1) < 1e-10
     assert relerr(pa, d["ref_final_PA"], -1) < 1e-10
