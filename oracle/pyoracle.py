@@ -31,6 +31,24 @@ class RcoParams(C.Structure):
                 ("fu_y", C.c_double * MAXSP), ("fu_t", C.c_double), ("fu_g", C.c_double)]
 
 
+class RcoTransport(C.Structure):
+    _fields_ = [("t_low", C.c_double), ("t_crit", C.c_double), ("t_high", C.c_double),
+                ("mu_low", (C.c_double * 4) * MAXSP), ("mu_high", (C.c_double * 4) * MAXSP),
+                ("k_low", (C.c_double * 4) * MAXSP), ("k_high", (C.c_double * 4) * MAXSP),
+                ("k_b", C.c_double), ("sigma", C.c_double * MAXSP), ("eps", C.c_double * MAXSP)]
+
+
+def make_transport(tr) -> RcoTransport:
+    t = RcoTransport()
+    t.t_low, t.t_crit, t.t_high, t.k_b = tr.t_low, tr.t_crit, tr.t_high, tr.k_b
+    for i in range(len(tr.sigma)):
+        t.sigma[i], t.eps[i] = tr.sigma[i], tr.eps[i]
+        for k in range(4):
+            t.mu_low[i][k], t.mu_high[i][k] = tr.mu_low[i, k], tr.mu_high[i, k]
+            t.k_low[i][k], t.k_high[i][k] = tr.k_low[i, k], tr.k_high[i, k]
+    return t
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -64,6 +82,11 @@ def lib():
         L.rco_be_lusgs.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.rco_be_stage.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, P(C.c_int), P(C.c_double)]
         L.rco_set_p_out.argtypes = [C.c_void_p, C.c_double]
+        L.rco_set_transport.argtypes = [C.c_void_p, P(RcoTransport), C.c_int]
+        L.rco_transport_property.argtypes = [P(RcoMech), P(RcoTransport), C.c_double, P(C.c_double),
+                                             P(C.c_double), P(C.c_double)]
+        L.rco_diffusion_velocity.argtypes = [P(RcoMech), P(RcoTransport), C.c_double, C.c_double, P(C.c_double),
+                                             P(C.c_double)]
         _LIB = L
     return _LIB
 
@@ -133,6 +156,11 @@ class OracleCase:
         s = (C.c_int * 4)()
         L.rco_sizes(h, s)
         self.ne, self.nf, self.ng, self.nvar = s[0], s[1], s[2], s[3]
+        if getattr(case.info, "viscous", False):
+            if case.transport is None:
+                raise ValueError("viscous case without transport tables")
+            self._tr = make_transport(case.transport)
+            L.rco_set_transport(h, C.byref(self._tr), 1)
         self.set_pa(case.pa0)
 
     def __del__(self):
@@ -281,3 +309,22 @@ def solve_dgesvx(a, b):
     if lib().rco_solve_dgesvx(len(b), _dp(a), _dp(b), _dp(x)):
         raise np.linalg.LinAlgError("singular matrix")
     return x
+
+
+def transport_property(mech, tr, t, y):
+    """(mu, K) of the mixture: compute_transport_property."""
+    m, T = make_mech(mech), make_transport(tr)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    mu, k = C.c_double(), C.c_double()
+    if lib().rco_transport_property(C.byref(m), C.byref(T), float(t), _dp(y), C.byref(mu), C.byref(k)):
+        raise ValueError("temperature out of the transport range")
+    return mu.value, k.value
+
+
+def diffusion_velocity(mech, tr, p, t, y):
+    """Mixture-averaged diffusion coefficients: compute_diffusion_velocity."""
+    m, T = make_mech(mech), make_transport(tr)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(mech.nsp)
+    lib().rco_diffusion_velocity(C.byref(m), C.byref(T), float(p), float(t), _dp(y), _dp(out))
+    return out

@@ -39,6 +39,8 @@ struct rco_case {
     double total_volume;
     double *pa, *wa, *hwa, *gradc, *gpa, *alpaf, *step, *prod, *dd;
     double *fixed;                 /* fixed(Nvar,NE) of the dual-time iteration */
+    rco_transport tr;              /* transport / diffusion tables */
+    int viscous;                   /* IfViscous */
     char err[256];
 };
 
@@ -573,7 +575,7 @@ int rco_set_pa(rco_case *c, const double *pa)
     return prim_to_cons(c);
 }
 
-/* TimeStep (inviscid, steady), TimeStep.f90:4-103 */
+/* TimeStep (steady; viscous branch when IfViscous), TimeStep.f90:4-103 */
 int rco_timestep(rco_case *c)
 {
     int i, k, nsp = c->nsp, nvar = c->nvar;
@@ -596,6 +598,21 @@ int rco_timestep(rco_case *c)
         double rr = pp / (g[2] * tt);
         if (rr < 0. || pp < 0. || tt < 0.) { free(alpac); return fail(c, "error in computing timestep, cells %d %d", NL, NR); }
         double un = uu * dx + vv * dy;
+        if (c->viscous) {
+            /* TimeStep.f90:73-85 */
+            double vmul, akmu;
+            if (rco_transport_property(&c->m, &c->tr, tt, yy, &vmul, &akmu)) {
+                free(alpac);
+                return fail(c, "error: temperature %g is not in range of the transport table", tt);
+            }
+            double prl = vmul * g[0] / akmu;
+            double alpa = fabs(un) + sqrt(g[1] * pp / rr) * sqrt(dx * dx + dy * dy);
+            double alpavis = 2. * (dx * dx + dy * dy) * fmax(4. / 3., g[1] / prl) * vmul / rr;
+            c->alpaf[i - 1] = alpa + alpavis / c->vol[NL - 1];
+            alpac[NL] = alpac[NL] + alpa + alpavis / c->vol[NL - 1];
+            if (NR > 0) alpac[NR] = alpac[NR] + alpa + alpavis / c->vol[NR - 1];
+            continue;
+        }
         double alpa = fabs(un) + sqrt(g[1] * pp / rr) * sqrt(dx * dx + dy * dy);
         c->alpaf[i - 1] = alpa;
         alpac[NL] = alpac[NL] + alpa;
@@ -736,6 +753,258 @@ int rco_limiter(rco_case *c)
     return 0;
 }
 
+/* compute_transport_property, transport_property.f90:78-125: mixture
+ * viscosity (Wilke) and conductivity at (T, Y); 1 if T is above the table */
+int rco_transport_property(const rco_mech *m, const rco_transport *tr, double tem2, const double *yy,
+                           double *vmul, double *akmu)
+{
+    int i, j, nsp = m->nsp;
+    double tem = fmax(tr->t_low, tem2);
+    double xx[RCO_MAXSP], mul[RCO_MAXSP], kl[RCO_MAXSP], phil[RCO_MAXSP], s = 0.0;
+    for (i = 0; i < nsp; i++) s = s + yy[i] / m->mw[i];
+    double ww = 1. / s;
+    for (i = 0; i < nsp; i++) xx[i] = ww * yy[i] / m->mw[i];
+    const double(*mu)[4], (*kk)[4];
+    if (tr->t_low <= tem && tem <= tr->t_crit) {
+        mu = tr->mu_low; kk = tr->k_low;
+    } else if (tr->t_crit < tem && tem <= tr->t_high) {
+        mu = tr->mu_high; kk = tr->k_high;
+    } else {
+        return 1;
+    }
+    for (i = 0; i < nsp; i++) {
+        mul[i] = exp(mu[i][0] * log(tem) + mu[i][1] / tem + mu[i][2] / (tem * tem) + mu[i][3]);
+        kl[i] = exp(kk[i][0] * log(tem) + kk[i][1] / tem + kk[i][2] / (tem * tem) + kk[i][3]);
+    }
+    for (i = 0; i < nsp; i++) mul[i] = mul[i] * 1.0e-7;
+    for (i = 0; i < nsp; i++) kl[i] = kl[i] * 1.0e-4;
+    for (i = 0; i < nsp; i++) {
+        phil[i] = 0.;
+        for (j = 0; j < nsp; j++) {
+            double t = 1. + sqrt(mul[i] / mul[j]) * pow(m->mw[j] / m->mw[i], 0.25);
+            phil[i] = phil[i] + xx[j] * (t * t) / sqrt(1. + m->mw[i] / m->mw[j]);
+        }
+        phil[i] = 0.25 * sqrt(2.) * phil[i];
+    }
+    double v = 0.0, a1 = 0.0, a2 = 0.0;
+    for (i = 0; i < nsp; i++) v = v + xx[i] * mul[i] / phil[i];
+    for (i = 0; i < nsp; i++) a1 = a1 + xx[i] * kl[i];
+    for (i = 0; i < nsp; i++) a2 = a2 + xx[i] / kl[i];
+    *vmul = v;
+    *akmu = 0.5 * (a1 + 1. / a2);
+    return 0;
+}
+
+/* compute_diffusion_velocity, diffusion_velocity.f90:55-117: mixture-
+ * averaged diffusion coefficients from binary ones (Chapman-Enskog with the
+ * Neufeld collision integral) */
+void rco_diffusion_velocity(const rco_mech *m, const rco_transport *tr, double pp, double tem, const double *yy,
+                            double *dk)
+{
+    static const double A = 1.06036, B = 0.15610, C = 0.19300, D = 0.47635, E = 1.03587, F = 1.52996,
+                        G = 1.76474, H = 3.89411;
+    int i, j, nsp = m->nsp;
+    double xx[RCO_MAXSP], dij[RCO_MAXSP], s = 0.0;
+    for (i = 0; i < nsp; i++) s = s + yy[i] / m->mw[i];
+    double ww = 1. / s;
+    for (i = 0; i < nsp; i++) xx[i] = ww * yy[i] / m->mw[i];
+    for (i = 0; i < nsp; i++) {
+        for (j = 0; j < nsp; j++) {
+            double mw = 2. / (1. / m->mw[i] + 1. / m->mw[j]) * 1000;
+            double cd = 0.5 * (tr->sigma[i] + tr->sigma[j]);
+            double tstar = tem / sqrt(tr->eps[i] * tr->eps[j]);
+            double om = A * pow(tstar, -B) + C * exp(-D * tstar) + E * exp(-F * tstar) + G * exp(-H * tstar);
+            dij[j] = 0.0266 * sqrt(tem * tem * tem) / (pp * sqrt(mw) * (cd * cd) * om);
+        }
+        double xd = 0.;
+        for (j = 0; j < i; j++) xd = xd + xx[j] / dij[j];
+        for (j = i + 1; j < nsp; j++) xd = xd + xx[j] / dij[j];
+        dk[i] = (1. - xx[i]) / (xd + 1.0e-16);
+    }
+}
+
+static void removedirection(double *vx, double *vy, double nx, double ny)
+{
+    /* flux_viscous.f90:354-362 */
+    double vn = *vx * nx + *vy * ny;
+    *vx = *vx - vn * nx;
+    *vy = *vy - vn * ny;
+}
+
+/* flux_viscous, flux_viscous.f90:4-350 (one Gauss point, wgg = 1): the
+ * viscous, heat-conduction and species-diffusion fluxes at the face
+ * midpoint from the reconstructed states and the averaged gradients with
+ * the face-normal jump term, accumulated into hWA in face order */
+static int flux_viscous(rco_case *c)
+{
+    int i, k, nsp = c->nsp, nvar = c->nvar, ns1 = nsp - 1;
+    const double yita = 4. / 3;
+    for (i = 1; i <= c->nf; i++) {
+        int NL = NEIGHBOR(1, i), NR = NEIGHBOR(2, i);
+        double dx = VECX(1, i), dy = VECX(2, i);
+        int n1 = FNODE(1, i), n2 = FNODE(2, i);
+        double sav1 = dx, sav2 = dy;
+        double sav = fmax(sqrt(dx * dx + dy * dy), SMALL);
+        double s1n = sav1 / sav, s2n = sav2 / sav;
+        double xy0 = COOR(1, n1) + 0.5 * (COOR(1, n2) - COOR(1, n1));
+        double xy1 = COOR(2, n1) + 0.5 * (COOR(2, n2) - COOR(2, n1));
+        double v0 = (xy0 - CELLXY(1, NL)) / REFLEN(1, NL);
+        double v1 = (xy1 - CELLXY(2, NL)) / REFLEN(2, NL);
+        double u_l[RCO_MAXSP + NFLOW], u_r[RCO_MAXSP + NFLOW];
+        for (k = 1; k <= nvar; k++) u_l[k - 1] = PA(k, NL) + (GRADC(1, k, NL) * v0 + GRADC(2, k, NL) * v1);
+        double pl = u_l[0], ul = u_l[1], vl = u_l[2], tl = u_l[3];
+        double yl[RCO_MAXSP], yr[RCO_MAXSP], gl[3], gr[3], s = 0.0;
+        for (k = 0; k < ns1; k++) yl[k] = u_l[NFLOW + k];
+        for (k = 0; k < ns1; k++) s = s + yl[k];
+        yl[nsp - 1] = 1. - s;
+        if (rco_gas_param(&c->m, yl, tl, gl)) return fail(c, "flux_viscous: temperature %g out of range", tl);
+        double rl = pl / (gl[2] * tl);
+        double pr, ur, vr, tr, rr;
+        if (NR > 0) {
+            double w0 = (xy0 - (CELLXY(1, NR) - 0.0)) / REFLEN(1, NR);
+            double w1 = (xy1 - (CELLXY(2, NR) - 0.0)) / REFLEN(2, NR);
+            for (k = 1; k <= nvar; k++) u_r[k - 1] = PA(k, NR) + (GRADC(1, k, NR) * w0 + GRADC(2, k, NR) * w1);
+            pr = u_r[0]; ur = u_r[1]; vr = u_r[2]; tr = u_r[3];
+            s = 0.0;
+            for (k = 0; k < ns1; k++) yr[k] = u_r[NFLOW + k];
+            for (k = 0; k < ns1; k++) s = s + yr[k];
+            yr[nsp - 1] = 1. - s;
+            if (rco_gas_param(&c->m, yr, tr, gr)) return fail(c, "flux_viscous: temperature %g out of range", tr);
+            rr = pr / (gr[2] * tr);
+        } else if (NR == -1) {
+            rr = rl; pr = pl; tr = tl;
+            ul = 0.; vl = 0.;
+            ur = 0.; vr = 0.;
+            for (k = 0; k < nsp; k++) yr[k] = yl[k];
+            gr[0] = gl[0]; gr[1] = gl[1]; gr[2] = gl[2];
+        } else if (NR == -2 || NR == -6) {
+            removedirection(&ul, &vl, s1n, s2n);
+            rr = rl; pr = pl; tr = tl;
+            ur = ul; vr = vl;
+            for (k = 0; k < nsp; k++) yr[k] = yl[k];
+            gr[0] = gl[0]; gr[1] = gl[1]; gr[2] = gl[2];
+        } else if (NR == -3 || NR == -4) {
+            int bad = 0;
+            inlet_state(c, -NR, pl, &pr, &ur, &vr, &tr, &rr, yr, gr, &bad);
+            if (bad) return fail(c, "flux_viscous inlet: temperature out of range");
+            pl = pr; ul = ur; vl = vr; tl = tr; rl = rr;
+            for (k = 0; k < nsp; k++) yl[k] = yr[k];
+            gl[0] = gr[0]; gl[1] = gr[1]; gl[2] = gr[2];
+        } else if (NR == -5) {
+            double ac = sqrt(gl[1] * pl / rl);
+            double vn = ul * s1n + vl * s2n;
+            if (fabs(vn) > ac) {
+                rr = rl; ur = ul; vr = vl; pr = pl; tr = tl;
+            } else {
+                pr = c->p.p_out;
+                ur = ul + s1n * (pl - pr) / (rl * ac);
+                vr = vl + s2n * (pl - pr) / (rl * ac);
+                rr = rl + (pr - pl) / (ac * ac);
+                tr = pr / (gl[2] * rr);
+                pl = pr; ul = ur; vl = vr; tl = tr; rl = rr;
+            }
+            for (k = 0; k < nsp; k++) yr[k] = yl[k];
+            gr[0] = gl[0]; gr[1] = gl[1]; gr[2] = gl[2];
+        } else {
+            rr = rl; pr = pl; tr = tl;
+            ur = ul; vr = vl;
+            for (k = 0; k < nsp; k++) yr[k] = yl[k];
+            gr[0] = gl[0]; gr[1] = gl[1]; gr[2] = gl[2];
+        }
+        double pp = 0.5 * (pl + pr), uu = 0.5 * (ul + ur), vv = 0.5 * (vl + vr), tem = 0.5 * (tl + tr);
+        double yy[RCO_MAXSP], g[3];
+        for (k = 0; k < nsp; k++) yy[k] = 0.5 * (yl[k] + yr[k]);
+        if (rco_gas_param(&c->m, yy, tem, g)) return fail(c, "flux_viscous: temperature %g out of range", tem);
+        double rho = pp / (g[2] * tem);
+        /* gradients: fbaseGrad_dxdy (module.f90:124-137) scales by 1/reflen */
+        double tgd[2][RCO_MAXSP + NFLOW];
+        double ix = 1. / REFLEN(1, NL), iy = 1. / REFLEN(2, NL);
+        for (k = 1; k <= nvar; k++) {
+            tgd[0][k - 1] = GRADC(1, k, NL) * ix + GRADC(2, k, NL) * 0.;
+            tgd[1][k - 1] = GRADC(1, k, NL) * 0. + GRADC(2, k, NL) * iy;
+        }
+        if (NR > 0) {
+            double dlrx = CELLXY(1, NL) - (CELLXY(1, NR) - 0.0);
+            double dlry = CELLXY(2, NL) - (CELLXY(2, NR) - 0.0);
+            double dxk = fabs(dlrx * s1n + dlry * s2n);
+            double jx = 1. / REFLEN(1, NR), jy = 1. / REFLEN(2, NR);
+            for (k = 1; k <= nvar; k++) {
+                tgd[0][k - 1] = 0.5 * (tgd[0][k - 1] + (GRADC(1, k, NR) * jx + GRADC(2, k, NR) * 0.)) +
+                                yita * s1n * (u_r[k - 1] - u_l[k - 1]) / dxk;
+                tgd[1][k - 1] = 0.5 * (tgd[1][k - 1] + (GRADC(1, k, NR) * 0. + GRADC(2, k, NR) * jy)) +
+                                yita * s2n * (u_r[k - 1] - u_l[k - 1]) / dxk;
+            }
+        }
+        double dpdx = tgd[0][0], dpdy = tgd[1][0], dudx = tgd[0][1], dudy = tgd[1][1];
+        double dvdx = tgd[0][2], dvdy = tgd[1][2], dtdx = tgd[0][3], dtdy = tgd[1][3];
+        double dydx[RCO_MAXSP], dydy[RCO_MAXSP];
+        for (k = 0; k < ns1; k++) { dydx[k] = tgd[0][NFLOW + k]; dydy[k] = tgd[1][NFLOW + k]; }
+        if (NR == -1 || NR == -6) removedirection(&dtdx, &dtdy, s1n, s2n);
+        if (NR == -2) {
+            double snx = s1n, sny = s2n, stx = -sny, sty = snx;
+            removedirection(&dpdx, &dpdy, snx, sny);
+            removedirection(&dtdx, &dtdy, snx, sny);
+            for (k = 0; k < ns1; k++) removedirection(&dydx[k], &dydy[k], snx, sny);
+            double dutdx = dudx * stx + dvdx * sty, dutdy = dudy * stx + dvdy * sty;
+            double dundx = dudx * snx + dvdx * sny, dundy = dudy * snx + dvdy * sny;
+            removedirection(&dutdx, &dutdy, snx, sny);
+            removedirection(&dundx, &dundy, stx, sty);
+            dudx = dutdx * stx + dundx * snx;
+            dudy = dutdy * stx + dundy * snx;
+            dvdx = dutdx * sty + dundx * sny;
+            dvdy = dutdy * sty + dundy * sny;
+        }
+        (void)dpdx; (void)dpdy;
+        s = 0.0;
+        for (k = 0; k < ns1; k++) s = s + dydx[k];
+        dydx[nsp - 1] = -s;
+        s = 0.0;
+        for (k = 0; k < ns1; k++) s = s + dydy[k];
+        dydy[nsp - 1] = -s;
+        double entha[RCO_MAXSP], vmulc, akmu, dk[RCO_MAXSP];
+        if (rco_enthalpy(&c->m, tem, entha)) return fail(c, "flux_viscous: temperature %g out of range", tem);
+        if (rco_transport_property(&c->m, &c->tr, tem, yy, &vmulc, &akmu))
+            return fail(c, "error: temperature %g is not in range of the transport table", tem);
+        rco_diffusion_velocity(&c->m, &c->tr, pp, tem, yy, dk);
+        double uicx = 0.0, uicy = 0.0;
+        for (k = 0; k < nsp; k++) uicx = uicx + dk[k] * dydx[k];
+        for (k = 0; k < nsp; k++) uicy = uicy + dk[k] * dydy[k];
+        double vlac = -2. / 3. * vmulc;
+        double div = dudx + dvdy;
+        double txx = 2. * vmulc * dudx + vlac * div;
+        double tyy = 2. * vmulc * dvdy + vlac * div;
+        double txy = vmulc * (dudy + dvdx);
+        double sx = 0.0, sy = 0.0;
+        for (k = 0; k < nsp; k++) sx = sx + (dk[k] * dydx[k] - yy[k] * uicx) * entha[k];
+        for (k = 0; k < nsp; k++) sy = sy + (dk[k] * dydy[k] - yy[k] * uicy) * entha[k];
+        double qx = -akmu * dtdx - rho * sx;
+        double qy = -akmu * dtdy - rho * sy;
+        double btx = uu * txx + vv * txy - qx;
+        double bty = uu * txy + vv * tyy - qy;
+        double flu[RCO_MAXSP + NFLOW];
+        flu[0] = 0.;
+        flu[1] = -(sav1 * txx + sav2 * txy);
+        flu[2] = -(sav1 * txy + sav2 * tyy);
+        flu[3] = -(sav1 * btx + sav2 * bty);
+        for (k = 0; k < ns1; k++) {
+            double spx = rho * (dk[k] * dydx[k] - yy[k] * uicx);
+            double spy = rho * (dk[k] * dydy[k] - yy[k] * uicy);
+            flu[NFLOW + k] = -(sav1 * spx + sav2 * spy);
+        }
+        for (k = 1; k <= nvar; k++) {
+            HWA(k, NL) = HWA(k, NL) - flu[k - 1] * 1.0;
+            if (NR > 0) HWA(k, NR) = HWA(k, NR) + flu[k - 1] * 1.0;
+        }
+    }
+    return 0;
+}
+
+void rco_set_transport(rco_case *c, const rco_transport *tr, int viscous)
+{
+    c->tr = *tr;
+    c->viscous = viscous;
+}
+
 /* flux_lax, flux_lax.f90:4-299, and sourceterm, sourceterm.f90:2-70,
  * composed as flux_sourceterm, fvm2d.f90:196-224 */
 int rco_flux_sourceterm(rco_case *c)
@@ -839,6 +1108,7 @@ int rco_flux_sourceterm(rco_case *c)
             if (NR > 0) HWA(k, NR) = HWA(k, NR) + flu[k - 1] * 1.0;
         }
     }
+    if (c->viscous && flux_viscous(c)) return 1;
     if (!c->p.combustion) return 0;
     for (i = 1; i <= c->ne; i++) {
         double p = PA(1, i), t = PA(4, i), y[RCO_MAXSP], s = 0.0, g[3], w[RCO_MAXSP];

@@ -38,6 +38,14 @@ typedef struct rco_params {
     double fu_y[RCO_MAXSP], fu_t, fu_g;   /* fuel inlet */
 } rco_params;
 
+/* transport.in / diffusion.in (transport_property.f90:31-42,
+ * diffusion_velocity.f90:30-35); mu, K = exp(A ln T + B/T + C/T^2 + D) */
+typedef struct rco_transport {
+    double t_low, t_crit, t_high;
+    double mu_low[RCO_MAXSP][4], mu_high[RCO_MAXSP][4], k_low[RCO_MAXSP][4], k_high[RCO_MAXSP][4];
+    double k_b, sigma[RCO_MAXSP], eps[RCO_MAXSP];
+} rco_transport;
+
 typedef struct rco_case rco_case;
 
 int  rco_create(const rco_mech *mech, const rco_params *par, int nnt, const double *coor,
@@ -70,6 +78,13 @@ int  rco_solve_dgesvx(int n, const double *a_rowmajor, const double *b, double *
 int  rco_be_lusgs(rco_case *c, double dt, int inner);
 int  rco_be_stage(rco_case *c, double dt, int innerloop, double reltol, int *ninner, double *roress);
 void rco_set_p_out(rco_case *c, double p_out);
+/* viscous path (IfViscous): transport tables, then TimeStep's viscous
+ * branch and flux_viscous inside flux_sourceterm */
+void rco_set_transport(rco_case *c, const rco_transport *tr, int viscous);
+int  rco_transport_property(const rco_mech *m, const rco_transport *tr, double t, const double *y,
+                            double *vmul, double *akmu);
+void rco_diffusion_velocity(const rco_mech *m, const rco_transport *tr, double p, double t, const double *y,
+                            double *dk);
 
 #ifdef __cplusplus
 }

@@ -79,6 +79,90 @@ class Inlet:
 
 
 @dataclasses.dataclass
+class Transport:
+    """``transport.in`` and ``diffusion.in`` (modules ``transport_property``
+    and ``diffusion_velocity``): per-species viscosity and conductivity fits
+    mu, K = exp(A ln T + B/T + C/T^2 + D) (micropoise, microW/cm/K) on two
+    temperature ranges, and Lennard-Jones collision data for the binary
+    diffusion coefficients."""
+    t_low: float
+    t_crit: float
+    t_high: float
+    mu_low: np.ndarray               # (Nsp, 4) A, B, C, D
+    mu_high: np.ndarray
+    k_low: np.ndarray
+    k_high: np.ndarray
+    k_b: float                       # read, unused by the reference's formulas
+    sigma: np.ndarray                # (Nsp,) collision diameter, Angstrom
+    eps: np.ndarray                  # (Nsp,) Lennard-Jones energy / k_B, K
+
+
+def default_transport() -> Transport:
+    """The reference's shipped ``combustion/transport.in`` and
+    ``combustion/diffusion.in`` tables (four species, in the deck's order)."""
+    rows = np.array([
+        [0.57388074E0, -0.98544160E2, 0.20012204E4, 0.17536015E1],
+        [0.65074534E0, 0.23936771E2, -0.22020183E5, 0.11244058E1],
+        [0.11770360E1, -0.17422121E3, 0.22865563E5, -0.55146852E0],
+        [0.49214767E0, -0.91598343E3, 0.87265127E5, 0.48489412E1],
+        [0.61936357E0, -0.44608607E2, -0.13460714E4, 0.19597562E1],
+        [0.63839563E0, -0.12344438E1, -0.22885810E5, 0.18056937E1],
+        [0.81595343E0, -0.34366856E2, 0.22785080E4, 0.10050999E1],
+        [0.80805788E0, 0.11982181E3, -0.47335931E5, 0.95189193E0],
+        [0.54330318E0, -0.18823898E3, 0.88726567E4, 0.24499362E1],
+        [0.65318879E0, 0.51738759E2, -0.62834882E5, 0.15227045E1],
+        [0.53726173E0, -0.49928331E3, 0.37397504E5, 0.32903619E1],
+        [0.66068182E0, -0.12741845E3, -0.81580328E5, 0.21817907E1],
+        [0.78387780E0, -0.38260408E3, 0.49040158E5, 0.85222785E0],
+        [0.50714993E0, -0.68966913E3, 0.87454750E5, 0.30285155E1],
+        [0.15541443E1, 0.66106305E2, 0.55969886E4, -0.39259598E1],
+        [0.79349503E0, -0.13340063E4, 0.37864327E6, 0.23591474E1]]).reshape(4, 4, 4)
+    return Transport(t_low=200.0, t_crit=1000.0, t_high=50000.0, mu_low=rows[:, 0].copy(),
+                     mu_high=rows[:, 1].copy(), k_low=rows[:, 2].copy(), k_high=rows[:, 3].copy(),
+                     k_b=1.38064852E-23, sigma=np.array([3.758, 3.467, 3.941, 2.641]),
+                     eps=np.array([148.6, 106.7, 195.2, 809.1]))
+
+
+def read_transport(case_dir: str, nsp: int) -> Transport:
+    """The layouts ``setup_transport_property`` (transport_property.f90:18-45)
+    and ``setup_diffusion_velocity`` (diffusion_velocity.f90:15-38) read."""
+    comb = os.path.join(case_dir, "combustion")
+    r = _Reader(os.path.join(comb, "transport.in"))
+    r.skip(); tl, tc, th = [_f(t) for t in r.values()[:3]]
+    r.skip()
+    tab = np.zeros((nsp, 4, 4))
+    for i in range(nsp):
+        for k in range(4):
+            tab[i, k] = [_f(t) for t in r.values()[:4]]
+    r = _Reader(os.path.join(comb, "diffusion.in"))
+    r.skip(); kb = _f(r.values()[0])
+    r.skip()
+    sig, eps = [], []
+    for _ in range(nsp):
+        v = r.values(); sig.append(_f(v[0])); eps.append(_f(v[1]))
+    return Transport(t_low=tl, t_crit=tc, t_high=th, mu_low=tab[:, 0].copy(), mu_high=tab[:, 1].copy(),
+                     k_low=tab[:, 2].copy(), k_high=tab[:, 3].copy(), k_b=kb, sigma=np.array(sig),
+                     eps=np.array(eps))
+
+
+def write_transport(case_dir: str, tr: Transport) -> None:
+    comb = os.path.join(case_dir, "combustion")
+    os.makedirs(comb, exist_ok=True)
+    with open(os.path.join(comb, "transport.in"), "w") as f:
+        f.write("# transport_lowest_tem,transport_critical_tem,transport_highest_tem\n")
+        f.write("%.17g, %.17g, %.17g\n" % (tr.t_low, tr.t_crit, tr.t_high))
+        f.write("# low_mu_A,low_mu_B,low_mu_C,low_mu_D\n")
+        for i in range(len(tr.sigma)):
+            for tab in (tr.mu_low, tr.mu_high, tr.k_low, tr.k_high):
+                f.write(", ".join("%.17g" % x for x in tab[i]) + "\n")
+    with open(os.path.join(comb, "diffusion.in"), "w") as f:
+        f.write("# Boltzmann's constant k_B (m2*kg*s-2*K-1)\n%.17g\n" % tr.k_b)
+        f.write("# hard-sphere collision diameter sigma (A), Lennard-Jones energy epsilon/k_B (K)\n")
+        for sg, ep in zip(tr.sigma, tr.eps):
+            f.write("%.17g, %.17g\n" % (sg, ep))
+
+
+@dataclasses.dataclass
 class SolverInfo:
     """The fields of ``flow/flow_solver_info.in`` (src/startup.f90:60-120)."""
     combustion: bool = True

@@ -215,6 +215,7 @@ class Case:
     geom: NozzleGeometry
     p0: float
     t0: float
+    transport: Optional[deck.Transport] = None   # used when info.viscous
 
     @property
     def inlet_mass_flux(self) -> float:
@@ -249,11 +250,13 @@ def nozzle_case(nx: int, ny: int, p0: float = 5.0e6, t0: float = 3000.0, phi: fl
                 geom: Optional[NozzleGeometry] = None, mech: Optional[deck.Mechanism] = None,
                 cfl: float = 5.0, mesh_seed: int = 1, y_seed: int = 2, y_perturb: float = 0.01,
                 limit_id: int = 1, combustion: bool = True, p_out_ratio: float = 0.02,
-                wall_type: int = 6, fuel_faces: int = 0, subsonic_exit: bool = False) -> Case:
+                wall_type: int = 6, fuel_faces: int = 0, subsonic_exit: bool = False,
+                viscous: bool = False) -> Case:
     """A nozzle case.  ``subsonic_exit`` starts from the subsonic branch of
     the area-Mach relation in the divergent part too (the outlet then takes
     the fixed-pressure branch); the default mechanism is the reference's with
-    kinetics model 1 and the formation constant in K."""
+    kinetics model 1 and the formation constant in K.  ``viscous`` switches
+    on the viscous path (IfViscous) with the shipped transport tables."""
     geom = geom or NozzleGeometry()
     mech = mech or deck.default_mechanism(model=1, b1_in_kelvin=True)
     mesh = nozzle_mesh(nx, ny, geom, seed=mesh_seed, wall_type=wall_type, fuel_faces=fuel_faces)
@@ -291,9 +294,10 @@ def nozzle_case(nx: int, ny: int, p0: float = 5.0e6, t0: float = 3000.0, phi: fl
         yy[:, s] = y_in[s] * (1.0 + y_perturb * (2.0 * uniform01(y_seed, nsp * k + s) - 1.0))
     yy /= yy.sum(axis=1, keepdims=True)
     pa0 = np.column_stack([p, u, v, t, yy[:, :nsp - 1]])
-    info = deck.SolverInfo(cfl=cfl, limit_id=limit_id, combustion=combustion, p_out_ratio=p_out_ratio)
+    info = deck.SolverInfo(cfl=cfl, limit_id=limit_id, combustion=combustion, p_out_ratio=p_out_ratio,
+                           viscous=viscous)
     return Case(mesh=mesh, mech=mech, oxidizer=oxid, fuel=fuel, info=info, pa0=pa0,
-                geom=geom, p0=p0, t0=t0)
+                geom=geom, p0=p0, t0=t0, transport=deck.default_transport() if viscous else None)
 
 
 def write_reference_case(case_dir: str, case: Case, nstep_max: Optional[int] = None,
@@ -309,7 +313,9 @@ def write_reference_case(case_dir: str, case: Case, nstep_max: Optional[int] = N
     deck.write_grid(os.path.join(case_dir, "grid1.in"), case.mesh.coor, case.mesh.cells,
                     case.mesh.bedges, case.mesh.btype, case.mesh.nface)
     deck.write_backup(os.path.join(case_dir, "fvp1.sav"), case.mesh.centroids(), case.pa0)
-    if extra_decks:
+    if case.transport is not None:
+        deck.write_transport(case_dir, case.transport)
+    elif extra_decks:
         for name in ("transport.in", "diffusion.in"):
             src = os.path.join(extra_decks, name)
             with open(src) as f, open(os.path.join(case_dir, "combustion", name), "w") as g:

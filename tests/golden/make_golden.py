@@ -103,7 +103,22 @@ def case_arrays(case: nozzle.Case) -> dict:
         # dual-time backward Euler: steady flag, dt, InnerLoop, relTol, A_p, f_p
         "unsteady": np.array([float(case.info.steady), case.info.totaltime / case.info.nstep_max,
                               case.info.innerloop, case.info.reltol, case.info.a_p, case.info.f_p]),
+        "viscous": np.array([float(case.info.viscous)]),
+        **(transport_arrays(case.transport) if case.info.viscous else {}),
     }
+
+
+def transport_arrays(tr: deck.Transport) -> dict:
+    return {"tr_range": np.array([tr.t_low, tr.t_crit, tr.t_high, tr.k_b]), "tr_mu_low": tr.mu_low,
+            "tr_mu_high": tr.mu_high, "tr_k_low": tr.k_low, "tr_k_high": tr.k_high, "tr_sigma": tr.sigma,
+            "tr_eps": tr.eps}
+
+
+def viscous(case: nozzle.Case) -> nozzle.Case:
+    """IfViscous with the shipped transport.in / diffusion.in tables."""
+    case.info = dataclasses.replace(case.info, viscous=True)
+    case.transport = deck.read_transport(os.path.dirname(REF_DECKS), case.mech.nsp)
+    return case
 
 
 ONLY = set(sys.argv[1:])     # fixture names to (re)generate; all when empty
@@ -149,6 +164,14 @@ def main() -> None:
     make("n24x6_be_model1", be(nozzle.nozzle_case(24, 6), 30, 1e-4, 3, 0.5), nsteps=30, ndump=2)
     make("n24x6_be_model0_stop", be(nozzle.nozzle_case(24, 6, mech=deck.default_mechanism(0, True)), 8, 1e-6, 3, 0.5),
          nsteps=8, ndump=2, expect_error=True)
+    # the viscous path (IfViscous): TimeStep's viscous branch and flux_viscous
+    make("n24x6_viscous", viscous(nozzle.nozzle_case(24, 6)), nsteps=20, ndump=2)
+    make("n24x6_viscous_bc_variants", viscous(nozzle.nozzle_case(24, 6, wall_type=1, fuel_faces=2,
+                                                                 subsonic_exit=True, p_out_ratio=0.9)),
+         nsteps=20, ndump=2)
+    make("n24x6_be_viscous", be(viscous(nozzle.nozzle_case(24, 6, wall_type=1, fuel_faces=2, subsonic_exit=True,
+                                                           p_out_ratio=0.9)), 6, 1e-5, 3, 0.5),
+         nsteps=6, ndump=3)
     conv = nozzle.nozzle_case(30, 8)
     make("n30x8_converged", conv, nsteps=8000, ndump=0, every=10)
     perm, _ = native.order_multicolor(conv.mesh.cells)

@@ -42,10 +42,17 @@ def case_from_fixture(d: dict) -> nozzle.Case:
         info.steady, info.implicit_time_method, info.innerloop, info.reltol = False, 3, int(inner), reltol
         info.a_p, info.f_p, info.nstep_max = a_p, f_p, int(d["nsteps"])
         info.totaltime = dt * info.nstep_max
+    transport = None
+    if "viscous" in d and d["viscous"][0]:
+        info.viscous = True
+        t_low, t_crit, t_high, k_b = d["tr_range"]
+        transport = deck.Transport(t_low=t_low, t_crit=t_crit, t_high=t_high, mu_low=d["tr_mu_low"],
+                                   mu_high=d["tr_mu_high"], k_low=d["tr_k_low"], k_high=d["tr_k_high"], k_b=k_b,
+                                   sigma=d["tr_sigma"], eps=d["tr_eps"])
     mesh = nozzle.Mesh(d["coor"], d["cells"].astype(np.int32), d["bedges"].astype(np.int32),
                        d["btype"].astype(np.int32), int(d["nface"]))
     return nozzle.Case(mesh=mesh, mech=mech, oxidizer=ox, fuel=fu, info=info, pa0=d["pa0"],
-                       geom=nozzle.NozzleGeometry(), p0=0.0, t0=0.0)
+                       geom=nozzle.NozzleGeometry(), p0=0.0, t0=0.0, transport=transport)
 
 
 def relerr(a, b, axis=None) -> float:

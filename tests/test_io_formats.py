@@ -69,3 +69,19 @@ def test_vtk_layout(tmp_path):
     y4 = 1.0 - case.pa0[:, 4:].sum(axis=1)
     assert np.allclose(d["Y4"], y4, rtol=1e-11)
     assert np.all(d["Production_rate_2"][:1] == 0.0) and d["Production_rate_2"][1] == 5.0
+
+
+def test_transport_tables_round_trip_and_match_shipped(tmp_path):
+    """default_transport() holds the shipped transport.in / diffusion.in
+    tables (stored in the viscous golden fixture from the reference's own
+    files), and write_transport / read_transport round-trip them exactly."""
+    from helpers import load
+    d = load("n24x6_viscous")
+    tr = deck.default_transport()
+    assert np.array_equal(np.array([tr.t_low, tr.t_crit, tr.t_high, tr.k_b]), d["tr_range"])
+    for f in ("mu_low", "mu_high", "k_low", "k_high", "sigma", "eps"):
+        assert np.array_equal(getattr(tr, f), d["tr_" + f]), f
+    deck.write_transport(str(tmp_path), tr)
+    back = deck.read_transport(str(tmp_path), 4)
+    for f in ("t_low", "t_crit", "t_high", "k_b", "mu_low", "mu_high", "k_low", "k_high", "sigma", "eps"):
+        assert np.array_equal(getattr(back, f), getattr(tr, f)), f
