@@ -134,6 +134,29 @@ int rc_lusgs_stage(rc_case *h);
  * number of completed steps in *done. */
 int rc_run_steady(rc_case *h, int nsteps, int every, double *res_out, int *done);
 
+/* ---- viscous path (IfViscous, startup.f90:76).
+ * The tables of transport.in and diffusion.in as setup_transport_property
+ * (transport_property.f90:18-45) and setup_diffusion_velocity
+ * (diffusion_velocity.f90:15-38) read them, species in the mechanism's
+ * order. */
+typedef struct rc_transport {
+    double t_low, t_crit, t_high;          /* transport table ranges (K) */
+    double mu_low[RC_MAXSP][4];            /* ln mu[micropoise] = A ln T + B/T + C/T^2 + D, T <= t_crit */
+    double mu_high[RC_MAXSP][4];           /* the same above t_crit */
+    double k_low[RC_MAXSP][4];             /* ln K[microW/cm/K], same form */
+    double k_high[RC_MAXSP][4];
+    double k_b;                            /* Boltzmann's constant as read (unused by the formulas) */
+    double sigma[RC_MAXSP];                /* collision diameter (Angstrom) */
+    double eps[RC_MAXSP];                  /* Lennard-Jones energy / k_B (K) */
+} rc_transport;
+
+/* Switches the case to the viscous path with these tables: from the next
+ * call on, TimeStep adds the viscous spectral radius (TimeStep.f90:73-85)
+ * and flux_sourceterm adds flux_viscous (flux_viscous.f90:4-350) after
+ * flux_lax, in the steady step, the batch step and the backward-Euler
+ * stage alike. */
+int rc_set_transport(rc_case *h, const rc_transport *tr);
+
 /* ---- dual-time backward Euler with the point-implicit LU-SGS
  * (implicit_time_method = 3; fvm2d.f90:60-110 unsteady branch). */
 

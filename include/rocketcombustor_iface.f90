@@ -36,6 +36,15 @@ module rocketcombustor_iface
         real(c_double) :: fu_y(RC_MAXSP), fu_t, fu_mass_flux
     end type
 
+    !! transport.in / diffusion.in tables (C mu_low[k][j] = mu_low(j+1,k+1))
+    type, bind(c) :: rc_transport
+        real(c_double) :: t_low, t_crit, t_high
+        real(c_double) :: mu_low(4, RC_MAXSP), mu_high(4, RC_MAXSP)
+        real(c_double) :: k_low(4, RC_MAXSP), k_high(4, RC_MAXSP)
+        real(c_double) :: k_b
+        real(c_double) :: sigma(RC_MAXSP), eps(RC_MAXSP)
+    end type
+
     interface
         integer(c_int) function rc_case_create(mech, par, nnt, coor, ne, cellnodes, nb, bnodes, btype, &
                                                device, handle) bind(c, name="rc_case_create")
@@ -111,6 +120,13 @@ module rocketcombustor_iface
             integer(c_int), value :: nsteps, every
             real(c_double), intent(out) :: res_out(*)
             integer(c_int), intent(out) :: done
+        end function
+
+        !! viscous path (IfViscous): the transport and diffusion tables
+        integer(c_int) function rc_set_transport(h, tr) bind(c, name="rc_set_transport")
+            import :: c_int, c_ptr, rc_transport
+            type(c_ptr), value :: h
+            type(rc_transport), intent(in) :: tr
         end function
 
         !! dual-time backward Euler (implicit_time_method = 3): compute_p_out,

@@ -230,6 +230,11 @@ class OracleCase:
         _LIB.rco_residual(self.h, _dp(out))
         return out
 
+    def set_transport(self, transport, viscous: bool = True):
+        """Switches the viscous path on (IfViscous) with these tables."""
+        self._tr = make_transport(transport)
+        _LIB.rco_set_transport(self.h, C.byref(self._tr), 1 if viscous else 0)
+
     # dual-time backward Euler (implicit_time_method = 3)
     def set_p_out(self, p_out: float):
         _LIB.rco_set_p_out(self.h, float(p_out))

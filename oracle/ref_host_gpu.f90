@@ -11,7 +11,8 @@
 !! rc_backward_euler_stage after the reference's own compute_p_out hands
 !! its exit pressure to the library.  Unsteady runs write the inner-iteration
 !! rows (istep, inner, Res/total_volume, Romax) to OUTDIR/resid.bin and
-!! converge_uns.dat.
+!! converge_uns.dat.  Viscous decks hand the reference's transport and
+!! diffusion tables to the library (rc_set_transport).
 !!
 !! usage (in a case directory): rc_ref_host_gpu OUTDIR
 !! writes converge.dat and fvp1.sav as the reference does, and OUTDIR/resid.bin,
@@ -25,11 +26,14 @@ program rc_ref_host_gpu
     use chemical_kinetics
     use oxidizer
     use fuel
+    use transport_property
+    use diffusion_velocity
     use rocketcombustor_iface
     implicit none
     include 'mpif.h'
     type(rc_mechanism) :: mech
     type(rc_params) :: par
+    type(rc_transport) :: trp
     type(c_ptr) :: h
     integer(c_int), allocatable :: cells(:), bn(:), bt(:)
     integer :: i, ib, np, nb, istep, ierr, k, istop
@@ -87,6 +91,24 @@ program rc_ref_host_gpu
     enddo
 
     call rc_check(rc_case_create(mech, par, NNT, Coor, NE, cells, nb, bn, bt, 0, h), c_null_ptr, 'rc_case_create')
+    if (IfViscous) then
+        !! module transport_property / diffusion_velocity -> the library's tables
+        trp%t_low = transport_lowest_tem; trp%t_crit = transport_critical_tem
+        trp%t_high = transport_highest_tem
+        trp%mu_low = 0.; trp%mu_high = 0.; trp%k_low = 0.; trp%k_high = 0.
+        trp%mu_low(1, 1:Nsp) = low_mu_A(1:Nsp); trp%mu_low(2, 1:Nsp) = low_mu_B(1:Nsp)
+        trp%mu_low(3, 1:Nsp) = low_mu_C(1:Nsp); trp%mu_low(4, 1:Nsp) = low_mu_D(1:Nsp)
+        trp%mu_high(1, 1:Nsp) = high_mu_A(1:Nsp); trp%mu_high(2, 1:Nsp) = high_mu_B(1:Nsp)
+        trp%mu_high(3, 1:Nsp) = high_mu_C(1:Nsp); trp%mu_high(4, 1:Nsp) = high_mu_D(1:Nsp)
+        trp%k_low(1, 1:Nsp) = low_K_A(1:Nsp); trp%k_low(2, 1:Nsp) = low_K_B(1:Nsp)
+        trp%k_low(3, 1:Nsp) = low_K_C(1:Nsp); trp%k_low(4, 1:Nsp) = low_K_D(1:Nsp)
+        trp%k_high(1, 1:Nsp) = high_K_A(1:Nsp); trp%k_high(2, 1:Nsp) = high_K_B(1:Nsp)
+        trp%k_high(3, 1:Nsp) = high_K_C(1:Nsp); trp%k_high(4, 1:Nsp) = high_K_D(1:Nsp)
+        trp%k_b = k_B
+        trp%sigma = 0.; trp%eps = 0.
+        trp%sigma(1:Nsp) = collision_diameter(1:Nsp); trp%eps(1:Nsp) = LJ_energy(1:Nsp)
+        call rc_check(rc_set_transport(h, trp), h, 'rc_set_transport')
+    endif
     call rc_check(rc_set_primitive(h, PA), h, 'rc_set_primitive')
 
     open(91, file=trim(outdir)//'/resid.bin', access='stream', form='unformatted', status='replace')

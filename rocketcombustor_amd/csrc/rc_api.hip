@@ -31,6 +31,7 @@ void launch_be_prepare(const LaunchSet &, double, bool, hipStream_t);
 void launch_be_level(const LaunchSet &, bool, int, hipStream_t);
 void launch_be_update(const LaunchSet &, hipStream_t);
 int pi_chunk();
+void launch_viscous(const LaunchSet &, bool, bool, hipStream_t);
 }  // namespace rc
 
 using namespace rc;
@@ -50,6 +51,8 @@ struct rc_case {
     DevState S{};
     DevThermo *d_th = nullptr;
     DevParams *d_par = nullptr;
+    DevTransport *d_tr = nullptr;  // viscous path tables (uploaded by rc_set_transport)
+    bool viscous = false;
     CaseDev hc{};                  // host copy of the case's descriptor
     CaseDev *d_case = nullptr;
     // work lists of the case alone (case index 0), kept on the host too so
@@ -127,6 +130,7 @@ const char *kernel_name(int k) {
     case K_BE_FWD: return "backward Euler forward sweep";
     case K_BE_BWD: return "backward Euler backward sweep";
     case K_BE_UPDATE: return "backward Euler update";
+    case K_VISC: return "flux_viscous";
     default: return "?";
     }
 }
@@ -338,6 +342,7 @@ int layout_and_upload(rc_case *h) {
         S.lucoef = c.take<double>((size_t)kRecLC * sc);
         S.lu_af = c.take<double>((size_t)3 * sc);
         S.fixed = c.take<double>((size_t)kNvar * sc);
+        S.fvis = c.take<double>((size_t)kRecF * sf);
         M.lu_nb = c.take<int>((size_t)3 * sc);
         M.lu_n = c.take<double>((size_t)6 * sc);
         M.vol = c.take<double>(sc);
@@ -353,11 +358,13 @@ int layout_and_upload(rc_case *h) {
         M.fvr = c.take<double>((size_t)2 * sf);
         M.fnl = c.take<int>(sf);
         M.fnr = c.take<int>(sf);
+        M.vgeo = c.take<double>((size_t)6 * sf);
         M.gnrm = c.take<double>((size_t)2 * sg);
         M.ghost_host = c.take<int>(sg);
         M.ghost_type = c.take<int>(sg);
         h->d_th = c.take<DevThermo>(1);
         h->d_par = c.take<DevParams>(1);
+        h->d_tr = c.take<DevTransport>(1);
         h->d_case = c.take<CaseDev>(1);
         d_partial = c.take<double>((size_t)npartial * (kNvar + 1));
         d_resout = c.take<double>(kNvar + 1);
@@ -412,6 +419,8 @@ int layout_and_upload(rc_case *h) {
     if ((rc = up(M.fvr, fr.data(), sizeof(double) * fr.size()))) return rc;
     if ((rc = up(M.fnl, m.fnl.data(), sizeof(int) * m.nf))) return rc;
     if ((rc = up(M.fnr, m.fnr.data(), sizeof(int) * m.nf))) return rc;
+    auto vg = soa(m.vgeo, m.nf, 6, sf);
+    if ((rc = up(M.vgeo, vg.data(), sizeof(double) * vg.size()))) return rc;
     if (m.ng) {
         auto gn = soa(m.gnrm, m.ng, 2, sg);
         if ((rc = up(M.gnrm, gn.data(), sizeof(double) * gn.size()))) return rc;
@@ -427,6 +436,7 @@ int layout_and_upload(rc_case *h) {
     hc.S = S;
     hc.th = h->d_th;
     hc.par = h->d_par;
+    hc.tr = nullptr;
     hc.partial = d_partial;
     hc.npartial = npartial;
     hc.inv_volume = 1.0 / m.total_volume;
@@ -501,6 +511,7 @@ void enqueue_step_kernels(rc_case *h) {
     launch_gradlim(L, true, 2, h->stream);
     mark(h, 1);
     launch_flux(L, true, h->stream);
+    if (h->viscous) launch_viscous(L, true, true, h->stream);
     mark(h, 2);
     mark(h, 3);                 // the residual is formed in the forward sweep
     for (int l = 0; l < h->nlev; ++l) launch_lusgs_level(L, true, l, true, h->stream);
@@ -687,6 +698,9 @@ void enqueue_batch_step(rc_batch *b) {
     launch_step_begin(L, b->stream);
     launch_gradlim(L, true, 2, b->stream);
     launch_flux(L, true, b->stream);
+    bool visc = false;
+    for (const rc_case *m : b->members) visc = visc || m->viscous;
+    if (visc) launch_viscous(L, true, true, b->stream);
     for (int l = 0; l < b->nlev; ++l) launch_lusgs_level(L, true, l, true, b->stream);
     for (int l = b->nlev - 1; l >= 0; --l) launch_lusgs_level(L, false, l, false, b->stream);
     launch_update(L, b->stream);
@@ -812,6 +826,7 @@ int rc_timestep(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
     launch_ts_face(h->L, h->stream);
+    if (h->viscous) launch_viscous(h->L, true, false, h->stream);
     launch_ts_cell(h->L, h->stream);
     return sync_check(h);
 }
@@ -841,6 +856,7 @@ int rc_flux_sourceterm(rc_case *h) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
     launch_flux(h->L, false, h->stream);
+    if (h->viscous) launch_viscous(h->L, false, true, h->stream);
     launch_residual(h->L, false, h->stream);
     return sync_check(h);
 }
@@ -889,6 +905,54 @@ int rc_run_steady(rc_case *h, int nsteps, int every, double *res_out, int *done)
     return rc;
 }
 
+int rc_set_transport(rc_case *h, const rc_transport *t) {
+    if (!h || !t) return set_err(h, RC_ERR_ARGUMENT, "NULL argument");
+    const rc_mechanism &m = h->mech;
+    if (!(t->t_low < t->t_crit && t->t_crit < t->t_high)) return set_err(h, RC_ERR_ARGUMENT, "bad transport ranges");
+    DevTransport d{};
+    d.t_low = t->t_low;
+    d.t_crit = t->t_crit;
+    d.t_high = t->t_high;
+    for (int i = 0; i < kNsp; ++i) {
+        for (int j = 0; j < 4; ++j) {
+            d.mu_low[i][j] = t->mu_low[i][j];
+            d.mu_high[i][j] = t->mu_high[i][j];
+            d.k_low[i][j] = t->k_low[i][j];
+            d.k_high[i][j] = t->k_high[i][j];
+        }
+        // the per-pair constants of compute_transport_property (:113-116)
+        // and compute_diffusion_velocity (:72-80)
+        for (int j = 0; j < kNsp; ++j) {
+            const double mi = m.mole_weight[i], mj = m.mole_weight[j];
+            d.wa[i][j] = std::pow(mj / mi, 0.25);
+            d.wb[i][j] = std::sqrt(1. + mi / mj);
+            d.sqmw[i][j] = std::sqrt(2. / (1. / mi + 1. / mj) * 1000);
+            const double cd = 0.5 * (t->sigma[i] + t->sigma[j]);
+            d.cd2[i][j] = cd * cd;
+            d.sqeps[i][j] = std::sqrt(t->eps[i] * t->eps[j]);
+        }
+    }
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_tr, &d, sizeof d, hipMemcpyHostToDevice, h->stream));
+    h->hc.tr = h->d_tr;
+    HIPCHK(h, hipMemcpyAsync(h->d_case, &h->hc, sizeof h->hc, hipMemcpyHostToDevice, h->stream));
+    DevParams dp = make_params(h->par, h->mech.nsp);
+    dp.viscous = 1;
+    HIPCHK(h, hipMemcpyAsync(h->d_par, &dp, sizeof dp, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->viscous = true;
+    // the captured step and any batch holding the case change shape
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+    if (h->batch) {
+        h->batch->dirty = true;
+        if (h->batch->graph_exec) (void)hipGraphExecDestroy(h->batch->graph_exec);
+        h->batch->graph_exec = nullptr;
+    }
+    return RC_OK;
+}
+
 int rc_set_outlet_pressure(rc_case *h, double p_out) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     HIPCHK(h, hipSetDevice(h->device));
@@ -917,9 +981,11 @@ int rc_backward_euler_stage(rc_case *h, double dt, int innerloop, double reltol,
     int inner = 1;
     for (;; ++inner) {
         launch_ts_face(L, h->stream);
+        if (h->viscous) launch_viscous(L, true, false, h->stream);
         launch_ts_cell(L, h->stream);
         launch_gradlim(L, true, 2, h->stream);
         launch_flux(L, false, h->stream);
+        if (h->viscous) launch_viscous(L, false, true, h->stream);
         launch_residual(L, false, h->stream);
         enqueue_be_lusgs(h, dt, inner == 1);
         int rc = residual_async(h, h->h_res);

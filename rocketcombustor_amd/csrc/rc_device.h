@@ -32,6 +32,7 @@ struct DevMesh {
     const double *__restrict__ amls;
     const double *__restrict__ lvv;
     const double *__restrict__ gnrm;
+    const double *__restrict__ vgeo;     // viscous face operands, vgeo[k*sf+f], k < 6
     // LU-SGS face slots of each cell, in NeighF order: the neighbour across
     // the face (-1 on a boundary) and the face normal oriented out of the
     // cell (Time_LUSGS_steady.f90:60-68): lu_nb[k*sc+i], lu_n[(k*2+d)*sc+i]
@@ -66,6 +67,7 @@ struct DevState {
     double *lucoef;  // kRecLC per cell
     double *lu_af;   // alpaF of the cell's 3 face slots, 3 x sc
     double *fixed;   // backward Euler: vol/dt*WA at the first inner iteration, Nvar arrays of sc
+    double *fvis;    // viscous path, kRecF per face: alpa, Fv(2:Nvar), alpaVis
     int *err;        // [0] code, [1] index, [2] kernel, [3] step of the error, [4] current step
 };
 
@@ -76,6 +78,7 @@ struct CaseDev {
     DevState S;
     const DevThermo *th;
     const DevParams *par;
+    const DevTransport *tr;   // viscous path: transport tables (null when inviscid)
     double *partial;     // residual partial sums, (kNvar+1) per cell chunk
     int npartial;
     double inv_volume;   // 1 / total volume
@@ -111,10 +114,10 @@ struct GMesh {
     int ne, nf, ng;
     int sc, sf, sg;
     const RC_G int *fnl, *fnr, *stencil, *cface, *ghost_host, *ghost_type, *runs, *lu_nb, *pi_cells;
-    const RC_G double *vecx, *fvl, *fvr, *vol, *wdis, *amls, *lvv, *gnrm, *lu_n;
+    const RC_G double *vecx, *fvl, *fvr, *vol, *wdis, *amls, *lvv, *gnrm, *lu_n, *vgeo;
 };
 struct GState {
-    RC_G double *pa, *wa, *hwa, *gradc, *gpa, *step, *dd, *fflux, *lucoef, *lu_af, *fixed;
+    RC_G double *pa, *wa, *hwa, *gradc, *gpa, *step, *dd, *fflux, *lucoef, *lu_af, *fixed, *fvis;
     RC_G int *err;
 };
 typedef const RC_G DevThermo GThermo;
@@ -132,7 +135,7 @@ __device__ __forceinline__ GMesh gview(const RC_G DevMesh &m) {
     g.ghost_host = gp(m.ghost_host); g.ghost_type = gp(m.ghost_type); g.runs = gp(m.runs); g.lu_nb = gp(m.lu_nb);
     g.pi_cells = gp(m.pi_cells);
     g.vecx = gp(m.vecx); g.fvl = gp(m.fvl); g.fvr = gp(m.fvr); g.vol = gp(m.vol); g.wdis = gp(m.wdis);
-    g.amls = gp(m.amls); g.lvv = gp(m.lvv); g.gnrm = gp(m.gnrm); g.lu_n = gp(m.lu_n);
+    g.amls = gp(m.amls); g.lvv = gp(m.lvv); g.gnrm = gp(m.gnrm); g.lu_n = gp(m.lu_n); g.vgeo = gp(m.vgeo);
     return g;
 }
 
@@ -141,6 +144,7 @@ __device__ __forceinline__ GState gview(const RC_G DevState &s) {
     g.pa = gp(s.pa); g.wa = gp(s.wa); g.hwa = gp(s.hwa); g.gradc = gp(s.gradc); g.gpa = gp(s.gpa);
     g.step = gp(s.step); g.dd = gp(s.dd); g.fflux = gp(s.fflux); g.lucoef = gp(s.lucoef); g.lu_af = gp(s.lu_af);
     g.fixed = gp(s.fixed);
+    g.fvis = gp(s.fvis);
     g.err = gp(s.err);
     return g;
 }
@@ -149,7 +153,7 @@ constexpr int kChunk = 256;          // cells / faces / ghosts per thread block
 constexpr int kRunChunk = kChunk / kRunMax;   // LU-SGS runs per thread block
 
 enum KernelId { K_TS_FACE = 1, K_TS_CELL, K_GHOST, K_GRADLIM, K_FLUX, K_RES, K_FWD, K_BWD, K_UPDATE, K_P2C,
-                K_BE_FWD, K_BE_BWD, K_BE_UPDATE };
+                K_BE_FWD, K_BE_BWD, K_BE_UPDATE, K_VISC };
 
 // first error wins; the step number comes from the device-side step
 // counter err[4] (advanced by k_step_begin), so captured graphs report it too

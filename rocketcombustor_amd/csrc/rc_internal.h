@@ -40,6 +40,7 @@ struct HostMesh {
     std::vector<int> stencil;        // 3*ne  slot -> cell (>=0) or -(ghost+1)
     std::vector<double> lvv;         // 6*ne  limiter face-midpoint offsets [i][kf][d]
     std::vector<double> fvl, fvr;    // 2*nf  flux-point offsets of owner / neighbour
+    std::vector<double> vgeo;        // 6*nf  viscous face operands: 1/reflen of NL (x,y), of NR (x,y), cellxy(NL)-cellxy(NR)
     std::vector<double> gnrm;        // 2*ng  unit normal used by the ghost update
     std::vector<int> cface;          // 3*ne  face f if cell is the owner, ~f otherwise
     std::vector<int> lu_nb;          // 3*ne  neighbour across each face (-1: boundary)
@@ -75,6 +76,19 @@ struct DevThermo {
     double a_f, e_act, rconstant;
 };
 
+// Transport and diffusion tables on the device (viscous path), with the
+// per-pair constants of the Wilke rule and of the binary diffusion
+// coefficients formed once on the host.
+struct DevTransport {
+    double t_low, t_crit, t_high;
+    double mu_low[kNsp][4], mu_high[kNsp][4], k_low[kNsp][4], k_high[kNsp][4];
+    double wa[kNsp][kNsp];     // (mw_j/mw_i)^0.25
+    double wb[kNsp][kNsp];     // sqrt(1 + mw_i/mw_j)
+    double sqmw[kNsp][kNsp];   // sqrt(2/(1/mw_i + 1/mw_j)*1000)
+    double cd2[kNsp][kNsp];    // (0.5*(sigma_i + sigma_j))^2
+    double sqeps[kNsp][kNsp];  // sqrt(eps_i*eps_j)
+};
+
 // Run parameters on the device.
 struct DevParams {
     double cfl;
@@ -83,6 +97,7 @@ struct DevParams {
     double fu_y[RC_MAXSP], fu_t, fu_g;
     int limit_id;
     int combustion;
+    int viscous;                   // IfViscous
 };
 
 }  // namespace rc

@@ -18,6 +18,8 @@ namespace rc {
 #define LC_(c, i) S.lucoef[(i) * kRecLC + (c)]
 #define F_(k, f) S.fflux[(f) * kRecF + (k)]
 #define ALPHA_(f) S.fflux[(f) * kRecF + kNvar]
+// viscous face record: alpa (inviscid spectral radius), Fv(2:Nvar), alpaVis
+#define FV_(k, f) S.fvis[(f) * kRecF + (k)]
 
 constexpr int NV = kNvar;
 

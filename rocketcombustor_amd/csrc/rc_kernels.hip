@@ -43,21 +43,24 @@ __global__ void __launch_bounds__(256) k_timestep_face(ITEM_ARGS) {
     ALPHA_(f) = face_spectral_radius<NSP>(M, S, TH, f, M.fnl[f], M.fnr[f], M.vecx[f], M.vecx[M.sf + f]);
 }
 
-// TimeStep cell loop (TimeStep.f90:94-96) and the LU-SGS diagonal
+// TimeStep cell loop (TimeStep.f90:94-96; viscous: :77-81) and the LU-SGS diagonal
 // dd(i) = sum 0.5*alpaF + vol/step (Time_LUSGS_steady.f90:46-53)
 // (stored when `store`; the face alpaF values and dd are also returned)
 __device__ __forceinline__ double cell_step_dd(const GMesh &M, const GState &S, GParams &P, int i, double *al,
                                                bool store = true) {
     const double cfl = P.cfl;
+    const double v = M.vol[i];
     double a = 0.0, d = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int cf = M.cface[k * M.sc + i];
-        al[k] = ALPHA_(cf >= 0 ? cf : ~cf);
-        a = a + al[k];
+        const int f = cf >= 0 ? cf : ~cf;
+        al[k] = ALPHA_(f);
+        // viscous: alpaC(i) += alpa + alpaVis/vol(i) for either side of the
+        // face (TimeStep.f90:79-81); alpaF stays alpa + alpaVis/vol(NL)
+        a = P.viscous ? a + FV_(0, f) + FV_(kRecF - 1, f) / v : a + al[k];
         if (store) S.lu_af[k * M.sc + i] = al[k];
     }
-    const double v = M.vol[i];
     const double st = cfl * v / a;
 #pragma unroll
     for (int k = 0; k < 3; ++k) d = d + 0.5 * al[k];
@@ -347,6 +350,24 @@ __device__ __forceinline__ void cell_residual(const GMesh &M, const GState &S, T
         } else {
 #pragma unroll
             for (int k = 0; k < NV; ++k) h[k] = h[k] + F_(k, f);
+        }
+    }
+    if (P.viscous) {
+        // flux_viscous after flux_lax (fvm2d.f90:212), face order again;
+        // its mass flux is zero
+#pragma unroll
+        for (int kf = 0; kf < 3; ++kf) {
+            const int cf = M.cface[kf * M.sc + i];
+            const int f = cf >= 0 ? cf : ~cf;
+            if (cf >= 0) {
+                h[0] = h[0] - 0.0;
+#pragma unroll
+                for (int k = 1; k < NV; ++k) h[k] = h[k] - FV_(k, f);
+            } else {
+                h[0] = h[0] + 0.0;
+#pragma unroll
+                for (int k = 1; k < NV; ++k) h[k] = h[k] + FV_(k, f);
+            }
         }
     }
     if (P.combustion) {

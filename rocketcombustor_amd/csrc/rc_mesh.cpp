@@ -371,6 +371,22 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
             if (nr >= 0) m.fvr[2 * f + d] = (xy - (m.cellxy[2 * nr + d] - 0.0)) / m.reflen[2 * nr + d];
         }
     }
+    // viscous face operands (flux_viscous.f90:187-209, fbaseGrad_dxdy
+    // module.f90:124-137): 1/reflen of the two cells and the centroid
+    // difference NL - NR
+    m.vgeo.assign(6 * (size_t)m.nf, 0.0);
+    for (int f = 0; f < m.nf; ++f) {
+        const int nl = m.fnl[f], nr = m.fnr[f];
+        double *g = &m.vgeo[6 * (size_t)f];
+        g[0] = 1. / m.reflen[2 * nl];
+        g[1] = 1. / m.reflen[2 * nl + 1];
+        if (nr >= 0) {
+            g[2] = 1. / m.reflen[2 * nr];
+            g[3] = 1. / m.reflen[2 * nr + 1];
+            g[4] = m.cellxy[2 * nl] - (m.cellxy[2 * nr] - 0.0);
+            g[5] = m.cellxy[2 * nl + 1] - (m.cellxy[2 * nr + 1] - 0.0);
+        }
+    }
 
     // ---- LU-SGS schedule of the given order.  The order is cut into runs:
     // a cell joins the run of the cell before it when it shares a face with a

@@ -36,6 +36,25 @@ class RcParams(C.Structure):
                 ("fu_y", C.c_double * MAXSP), ("fu_t", C.c_double), ("fu_mass_flux", C.c_double)]
 
 
+class RcTransport(C.Structure):
+    _fields_ = [("t_low", C.c_double), ("t_crit", C.c_double), ("t_high", C.c_double),
+                ("mu_low", (C.c_double * 4) * MAXSP), ("mu_high", (C.c_double * 4) * MAXSP),
+                ("k_low", (C.c_double * 4) * MAXSP), ("k_high", (C.c_double * 4) * MAXSP),
+                ("k_b", C.c_double), ("sigma", C.c_double * MAXSP), ("eps", C.c_double * MAXSP)]
+
+
+def make_transport(tr) -> RcTransport:
+    """rc_transport from a deck.Transport."""
+    t = RcTransport()
+    t.t_low, t.t_crit, t.t_high, t.k_b = tr.t_low, tr.t_crit, tr.t_high, tr.k_b
+    for i in range(len(tr.sigma)):
+        t.sigma[i], t.eps[i] = tr.sigma[i], tr.eps[i]
+        for k in range(4):
+            t.mu_low[i][k], t.mu_high[i][k] = tr.mu_low[i, k], tr.mu_high[i, k]
+            t.k_low[i][k], t.k_high[i][k] = tr.k_low[i, k], tr.k_high[i, k]
+    return t
+
+
 class RcError(RuntimeError):
     """A nonzero return code of the library; ``code`` is the RC_ERR_* value."""
 
@@ -67,6 +86,7 @@ SIGNATURES = {
     "rc_run_steady": (C.c_int, [_VP, C.c_int, C.c_int, _DP, _IP]),
     "rc_residual": (C.c_int, [_VP, _DP]),
     "rc_set_outlet_pressure": (C.c_int, [_VP, C.c_double]),
+    "rc_set_transport": (C.c_int, [_VP, P(RcTransport)]),
     "rc_backward_euler_lusgs": (C.c_int, [_VP, C.c_double, C.c_int]),
     "rc_backward_euler_stage": (C.c_int, [_VP, C.c_double, C.c_int, C.c_double, _IP, _DP]),
     "rc_get_primitive": (C.c_int, [_VP, _DP]),

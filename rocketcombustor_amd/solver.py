@@ -16,6 +16,7 @@ methods keep the reference's names and call sequence:
     run_steady(n)       steady main loop    src/fvm2d.f90:48-150
     residual()          Residual/step_residual  src/residual.f90:4-21,61-79
 
+    set_transport(tables)         IfViscous + transport/diffusion decks  src/startup.f90:76
     set_outlet_pressure(p)        compute_p_out         src/fvm2d.f90:229-239
     backward_euler_lusgs(dt, k)   backwardEuler_LU_SGS  src/Time_BackwardEuler.f90:58
     backward_euler_stage(dt, ..)  backwardEuler_stage   src/Time_BackwardEuler.f90:4
@@ -66,6 +67,8 @@ class SteadyCase:
         info = (C.c_longlong * 6)()
         check(L.rc_case_info(h, info), h)
         self.ne, self.nf, self.ng, self.nvar, self.nlev, self.device_bytes = [int(v) for v in info]
+        if getattr(case.info, "viscous", False):
+            self.set_transport(case.transport)
         self.set_primitive(case.pa0)
 
     def close(self) -> None:
@@ -164,6 +167,15 @@ class SteadyCase:
             return None
         nrep_done = (1 if nsteps >= 1 else 0) + sum(1 for s in range(2, nsteps + 1) if s % every == 0)
         return buf[:nrep_done]
+
+    def set_transport(self, transport) -> None:
+        """The viscous path (IfViscous) with these transport.in /
+        diffusion.in tables (setup_transport_property,
+        setup_diffusion_velocity)."""
+        if transport is None:
+            raise ValueError("the viscous path needs transport tables")
+        self._tr = native.make_transport(transport)
+        check(lib().rc_set_transport(self.h, C.byref(self._tr)), self.h)
 
     # -- dual-time backward Euler (implicit_time_method = 3) ----------------
     def set_outlet_pressure(self, p_out: float) -> None:

@@ -21,7 +21,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "oracle", "_ref", "rc_ref_host_gpu")
 
 
-@pytest.mark.parametrize("name", ["n24x6_natural", "n24x6_bc_variants", "n24x6_be_bc_variants", "n24x6_be_model1"])
+@pytest.mark.parametrize("name", ["n24x6_natural", "n24x6_bc_variants", "n24x6_be_bc_variants", "n24x6_be_model1",
+                                  "n24x6_viscous_bc_variants", "n24x6_be_viscous"])
 def test_reference_host_with_gpu_step(name):
     if not os.path.exists(BIN):
         pytest.skip("reference host binary not built (oracle/ref_build.sh needs the reference sources)")

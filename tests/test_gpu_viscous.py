@@ -1,0 +1,110 @@
+"""The device viscous path (rc_set_transport: TimeStep's viscous branch and
+flux_viscous) against the reference and the oracle.  The per-call arrays and
+trajectories of the viscous golden fixtures run in tests/test_gpu_parity.py
+and tests/test_gpu_backward_euler.py with every other fixture; here the
+viscous part itself (~4e-5 of the residual), larger meshes in the
+multicolour order, and batches mixing viscous and inviscid members."""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import case_from_fixture, load, relerr
+from rocketcombustor_amd import native, nozzle
+from rocketcombustor_amd.solver import SteadyBatch, SteadyCase
+
+oracle = pytest.importorskip("oracle.pyoracle")
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg: str) -> None:
+    sys.__stderr__.write("[gpu-test] " + msg + "\n")
+    sys.__stderr__.flush()
+
+
+def residual_and_alpaf(case):
+    with SteadyCase(case) as g:
+        g.timestep()
+        _, alpaf = g.timestep_arrays()
+        g.reconstruction()
+        g.flux_sourceterm()
+        return g.increment(), alpaf
+
+
+def test_viscous_part_matches_reference():
+    dv, dn = load("n24x6_viscous"), load("n24x6_natural")
+    case = case_from_fixture(dv)
+    announce("viscous part of the residual vs reference")
+    hv, av = residual_and_alpaf(case)
+    hi, ai = residual_and_alpaf(dataclasses.replace(case, info=dataclasses.replace(case.info, viscous=False)))
+    assert relerr(hv - hi, dv["ref_s1_res"] - dn["ref_s1_res"], -1) < 1e-9
+    assert relerr(av - ai, dv["ref_s1_alpaF"] - dn["ref_s1_alpaF"]) < 1e-9
+
+
+@pytest.mark.parametrize("kw", [{}, {"wall_type": 1, "fuel_faces": 2, "subsonic_exit": True, "p_out_ratio": 0.9}])
+def test_multicolour_viscous_steps_vs_oracle(kw):
+    case = nozzle.nozzle_case(80, 20, viscous=True, **kw)
+    perm, _ = native.order_multicolor(case.mesh.cells)
+    case = case.permuted(perm)
+    announce("viscous multicolour NE=%d %s" % (case.mesh.ne, kw))
+    o = oracle.OracleCase(case)
+    with SteadyCase(case) as g:
+        g.timestep()
+        o.timestep()
+        step, alpaf = g.timestep_arrays()
+        assert relerr(step, o.get("step")) < 1e-13 and relerr(alpaf, o.get("alpaF")) < 1e-13
+        g.reconstruction()
+        o.reconstruction()
+        g.flux_sourceterm()
+        o.flux_sourceterm()
+        assert relerr(g.increment(), o.get("hWA"), -1) < 1e-12
+        g.lu_sgs_steady()
+        o.lusgs_steady()
+        assert relerr(g.primitive(), o.get("PA"), -1) < 1e-12
+        # fused, graph-replayed steps
+        g.run_steady(5, history=False)
+        for _ in range(5):
+            o.stage()
+        assert relerr(g.primitive(), o.get("PA"), -1) < 1e-10
+
+
+def test_batch_mixing_viscous_and_inviscid_equals_alone():
+    cases = [nozzle.nozzle_case(24, 6, viscous=True, wall_type=1), nozzle.nozzle_case(30, 8),
+             nozzle.nozzle_case(36, 9, viscous=True)]
+    announce("batch of viscous and inviscid members")
+    ref = []
+    for c in cases:
+        with SteadyCase(c) as g:
+            ref.append((g.run_steady(20, every=5), g.primitive()))
+    gs = [SteadyCase(c) for c in cases]
+    with SteadyBatch() as b:
+        for g in gs:
+            b.add(g)
+        hist = b.run_steady(20, every=5)
+        for k, g in enumerate(gs):
+            assert np.array_equal(hist[:, k], ref[k][0]), k
+            assert np.array_equal(g.primitive(), ref[k][1]), k
+    for g in gs:
+        g.close()
+
+
+def test_transport_set_after_steps_switches_the_captured_step():
+    """Switching a case to the viscous path re-captures its step graph: the
+    steps after the switch equal the oracle switched at the same step."""
+    inv = nozzle.nozzle_case(24, 6)
+    vis = nozzle.nozzle_case(24, 6, viscous=True)
+    announce("switch to viscous after graph-replayed steps")
+    with SteadyCase(inv) as a:
+        a.run_steady(3, history=False)
+        a.set_transport(vis.transport)
+        a.run_steady(4, history=False)
+        pa = a.primitive()
+    o = oracle.OracleCase(inv)
+    for _ in range(3):
+        o.stage()
+    o.set_transport(vis.transport)
+    for _ in range(4):
+        o.stage()
+    assert relerr(pa, o.get("PA"), -1) < 1e-10
