@@ -174,6 +174,32 @@ def batched_run(n: int, steps: int) -> dict:
             "speedup": t_plain / t_batch}
 
 
+def viscous_run(case, nsteps: int, cpu: bool = True) -> dict:
+    """Steady steps of the same case on the viscous path (IfViscous: the
+    viscous spectral radius in TimeStep, flux_viscous in flux_sourceterm),
+    graph-replayed like the inviscid ones, after 5 untimed steps."""
+    import dataclasses
+    from rocketcombustor_amd import deck
+    from rocketcombustor_amd.solver import SteadyCase
+    c = dataclasses.replace(case, info=dataclasses.replace(case.info, viscous=True),
+                            transport=deck.default_transport())
+    with SteadyCase(c, device=0) as g:
+        g.run_steady(5, history=False)
+        t0 = time.perf_counter()
+        g.run_steady(nsteps, history=False)
+        wall = time.perf_counter() - t0
+        res = g.residual()
+        assert all(x == x for x in res), "non-finite residual"
+        out = {"workload": "same case with IfViscous, %d steps" % nsteps, "ms_per_step": 1000.0 * wall / nsteps,
+               "cell_updates_per_s": g.ne * nsteps / wall}
+    if cpu:
+        ref = cpu_baseline(c, 3)
+        if ref is not None:
+            ref["sample"] = ref["sample"].replace("steps of the same", "viscous steps of the same")
+            out["cpu_baseline"] = ref
+    return out
+
+
 def backward_euler_run(case, nsteps: int, dt: float = 1e-7, inner: int = 4, cpu: bool = True) -> dict:
     """Dual-time backward Euler (implicit_time_method=3) on the same case:
     ``nsteps`` physical steps of exactly ``inner`` inner iterations (relTol
@@ -268,6 +294,9 @@ def main() -> int:
     ap.add_argument("--backward-euler", type=int, default=0, metavar="N",
                     help="also time N dual-time backward-Euler steps (point-implicit LU-SGS, 4 inner "
                          "iterations each) of the same 400k-cell case")
+    ap.add_argument("--viscous", type=int, default=0, metavar="K",
+                    help="also time K steady steps of the same case on the viscous path (IfViscous, the "
+                         "shipped transport tables)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device three times (a plain copy kernel of "
                          "known size, to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -345,6 +374,8 @@ def main() -> int:
     g.close()
     if args.copy_calibration > 0:
         out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
+    if args.viscous > 0:
+        out["viscous"] = viscous_run(case, args.viscous, cpu=not args.no_cpu_baseline)
     if args.backward_euler > 0:
         out["backward_euler"] = backward_euler_run(case, args.backward_euler, cpu=not args.no_cpu_baseline)
     if args.serve > 0:
