@@ -28,7 +28,11 @@ void launch_prim_to_cons(const LaunchSet &, hipStream_t);
 void launch_resnorm(const LaunchSet &, hipStream_t);
 void launch_derived(const LaunchSet &, double *, hipStream_t);
 void launch_be_prepare(const LaunchSet &, double, bool, hipStream_t);
-void launch_be_level(const LaunchSet &, bool, int, hipStream_t);
+void launch_be_factor(const LaunchSet &, double *, int *, double *, hipStream_t);
+void launch_be_level(const LaunchSet &, bool, int, const double *, const int *, const double *, hipStream_t);
+size_t be_factor_doubles();
+size_t be_factor_ints();
+size_t be_cache_doubles();
 void launch_be_update(const LaunchSet &, hipStream_t);
 int pi_chunk();
 void launch_viscous(const LaunchSet &, bool, bool, hipStream_t);
@@ -52,6 +56,9 @@ struct rc_case {
     DevThermo *d_th = nullptr;
     DevParams *d_par = nullptr;
     DevTransport *d_tr = nullptr;  // viscous path tables (uploaded by rc_set_transport)
+    double *be_fac = nullptr;      // backward Euler: per-cell block factors (allocated at first use)
+    int *be_piv = nullptr;
+    double *be_nbc = nullptr;      // backward Euler: per-cell thermo records of the neighbour terms
     bool viscous = false;
     CaseDev hc{};                  // host copy of the case's descriptor
     CaseDev *d_case = nullptr;
@@ -566,13 +573,26 @@ int residual_async(rc_case *h, double *host_out) {
     return RC_OK;
 }
 
-// backwardEuler_LU_SGS: dual-time right-hand side and diagonal, the
-// point-implicit sweeps level by level, PA += dPA and WA = U(PA)
+// the per-cell block factors of the backward-Euler sweeps (allocated once,
+// at the first backward-Euler call of the case)
+int ensure_be_storage(rc_case *h) {
+    if (h->be_fac) return RC_OK;
+    HIPCHK(h, hipMalloc(&h->be_fac, sizeof(double) * be_factor_doubles() * h->M.sc));
+    HIPCHK(h, hipMalloc(&h->be_piv, sizeof(int) * be_factor_ints() * h->M.sc));
+    HIPCHK(h, hipMalloc(&h->be_nbc, sizeof(double) * be_cache_doubles() * h->M.sc));
+    return RC_OK;
+}
+
+// backwardEuler_LU_SGS: dual-time right-hand side and diagonal, every
+// cell's block factored, the point-implicit sweeps level by level, PA +=
+// dPA and WA = U(PA)
 void enqueue_be_lusgs(rc_case *h, double dt, bool first) {
     const LaunchSet &L = h->L;
     launch_be_prepare(L, dt, first, h->stream);
-    for (int l = 0; l < h->npl; ++l) launch_be_level(L, true, l, h->stream);
-    for (int l = h->npl - 1; l >= 0; --l) launch_be_level(L, false, l, h->stream);
+    launch_be_factor(L, h->be_fac, h->be_piv, h->be_nbc, h->stream);
+    for (int l = 0; l < h->npl; ++l) launch_be_level(L, true, l, h->be_fac, h->be_piv, h->be_nbc, h->stream);
+    for (int l = h->npl - 1; l >= 0; --l)
+        launch_be_level(L, false, l, h->be_fac, h->be_piv, h->be_nbc, h->stream);
     launch_be_update(L, h->stream);
 }
 
@@ -798,6 +818,9 @@ void rc_case_destroy(rc_case *h) {
     }
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->arena) (void)hipFree(h->arena);
+    if (h->be_fac) (void)hipFree(h->be_fac);
+    if (h->be_piv) (void)hipFree(h->be_piv);
+    if (h->be_nbc) (void)hipFree(h->be_nbc);
     for (auto &ev : h->evpool) (void)hipEventDestroy(ev);
     if (h->h_err) (void)hipHostFree(h->h_err);
     if (h->h_res) (void)hipHostFree(h->h_res);
@@ -967,6 +990,8 @@ int rc_backward_euler_lusgs(rc_case *h, double dt, int inner) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     if (!(dt > 0.0) || inner < 1) return set_err(h, RC_ERR_ARGUMENT, "dt must be > 0 and inner >= 1");
     HIPCHK(h, hipSetDevice(h->device));
+    int rc = ensure_be_storage(h);
+    if (rc) return rc;
     enqueue_be_lusgs(h, dt, inner == 1);
     return sync_check(h);
 }
@@ -975,6 +1000,8 @@ int rc_backward_euler_stage(rc_case *h, double dt, int innerloop, double reltol,
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     if (!(dt > 0.0) || innerloop < 1) return set_err(h, RC_ERR_ARGUMENT, "dt must be > 0 and innerloop >= 1");
     HIPCHK(h, hipSetDevice(h->device));
+    int rc0 = ensure_be_storage(h);
+    if (rc0) return rc0;
     const LaunchSet &L = h->L;
     launch_step_begin(L, h->stream);
     double ro1 = 0.0;

@@ -6,31 +6,40 @@
 //   k_be_prepare  the diagonal dd = sum 0.5*alpaF + vol*(1/step + 1/dt), the
 //                 dual-time right-hand side hWA - vol/dt*WA + fixed (fixed =
 //                 vol/dt*WA saved at the first inner iteration)
-//   k_be_sweep    forward levels, then backward levels: each cell solves its
-//                 Nvar x Nvar block  dU/dPA*dd - vol*d(omega)/dPA  for the
-//                 primitive increment, with the linearised Rusanov terms of
-//                 its lower (forward) or upper (backward) neighbours
+//   k_be_factor   every cell's Nvar x Nvar block dU/dPA*dd - vol*d(omega)/dPA,
+//                 equilibrated and LU-factored as LAPACK's dgesvx with
+//                 FACT='E' does (advanced_linear_solve_lapack,
+//                 la_math.f90:120-213: dgeequ, dlaqge, dgetf2), all cells at
+//                 once: the block depends only on the cell's own state, so
+//                 both sweeps solve with the same factors
+//   k_be_sweep    forward levels, then backward levels: one thread per cell
+//                 adds the linearised Rusanov terms of its lower (forward) or
+//                 upper (backward) neighbours, solves with the stored factors
+//                 (dgetrs) and refines (dgerfs: at most 5 steps while the
+//                 componentwise backward error exceeds eps and halves)
 //   k_be_update   PA += dPA, WA = U(PA), the cell-average cache
 //
-// A cell of the sweep is solved by a group of 8 lanes, lane r owning row r
-// of the block: lanes 0-2 form the terms of the cell's three face slots at
-// once, every lane forms its row of the block, and the group solves the
-// system as LAPACK's dgesvx with FACT='E' does (advanced_linear_solve_lapack,
-// la_math.f90:120-213): equilibration (dgeequ, dlaqge), LU with partial
-// pivoting, triangular solves and iterative refinement (dgerfs), unscaling.
-// Row operations are lane-local; pivot search, row exchanges, pivot-row
-// broadcasts and the solution vector go by shuffles within the group.  Each
-// value is formed with the operations, in the order, of the restated
-// routines, so the result is that of the serial solve.
+// Each value is formed with the operations, in the order, of the restated
+// routines, so each cell's solution is that of the serial dgesvx call.
 #include "rc_items.h"
 
 namespace rc {
 
 #define FIX_(k, i) S.fixed[(k) * M.sc + (i)]
 
-constexpr int kPiLanes = 8;                       // lanes per cell
-constexpr int kPiChunk = kChunk / kPiLanes;       // cells per thread block
-static_assert(kNvar <= kPiLanes, "one lane per row of the block");
+constexpr int kPiChunk = 64;        // cells per thread block of a sweep level
+
+// stored factors of cell c (arrays of sc): the equilibrated block AE, its LU
+// factors (rows in pivoted order), row and column scales, pivots and flags
+// (bit 0 rows scaled, bit 1 columns scaled, bit 2 singular)
+constexpr int kBeAE = 0, kBeLU = kNvar * kNvar, kBeRS = 2 * kNvar * kNvar, kBeCS = kBeRS + kNvar;
+constexpr int kBeFac = kBeCS + kNvar;          // doubles per cell
+constexpr int kBePiv = kNvar + 1;              // ints per cell: pivots, flags
+constexpr int kBeNb = 8;                        // cache record per cell: rr, Rcpcv, Cp, hh, h_k (4 species)
+static_assert(kBeNb >= 4 + kNsp, "backward-Euler cache record too small");
+static_assert(kBeLU == kBeAE + kNvar * kNvar, "AE and LU are contiguous");
+#define BEF_(k, c) fac[(size_t)(k) * sc + (c)]
+#define BEP_(k, c) piv[(size_t)(k) * sc + (c)]
 
 // LAPACK machine constants (dlamch): eps = 2^-53, safe minimum = 2^-1022
 constexpr double kEps = 1.1102230246251565e-16;
@@ -40,193 +49,24 @@ constexpr double kSmallEq = kSfmin / (2 * kEps);  // dlaqge SMALL = safe minimum
 constexpr double kLargeEq = 1.0 / kSmallEq;
 constexpr double kThresh = 0.1;
 
-__device__ __forceinline__ double gsh(double v, int src) { return __shfl(v, src, kPiLanes); }
-__device__ __forceinline__ int gsh(int v, int src) { return __shfl(v, src, kPiLanes); }
-__device__ __forceinline__ double gmax(double v) {
-#pragma unroll
-    for (int o = 1; o < kPiLanes; o <<= 1) v = fmax(v, __shfl_xor(v, o, kPiLanes));
-    return v;
-}
-__device__ __forceinline__ double gmin(double v) {
-#pragma unroll
-    for (int o = 1; o < kPiLanes; o <<= 1) v = fmin(v, __shfl_xor(v, o, kPiLanes));
-    return v;
-}
-template <int N>
-__device__ __forceinline__ double pick(const double (&a)[N], int k) {
-    double v = a[0];
-#pragma unroll
-    for (int j = 1; j < N; ++j) v = k == j ? a[j] : v;
-    return v;
-}
-
-// dgetrs of the group's LU factors on one right-hand side held by row
-// (lane r: entry r of the original row order): the row interchanges as one
-// permutation (src = original row now in lane r), unit-lower then upper
-// triangular solves.  Returns entry r of the solution.
-template <int N>
-__device__ __forceinline__ double group_getrs(int r, const double (&lu)[N], const double (&ukk)[N], int src,
-                                              double v) {
-    v = gsh(v, src);
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        const double vk = gsh(v, k);
-        if (vk != 0.0 && r > k && r < N) v = v - vk * lu[k];
-    }
-#pragma unroll
-    for (int k = N - 1; k >= 0; --k) {
-        double vk = gsh(v, k);
-        if (vk != 0.0) {
-            vk = vk / ukk[k];
-            if (r == k) v = vk;
-            else if (r < k) v = v - vk * lu[k];
-        }
-    }
-    return v;
-}
-
-// dgesvx('E', 'N', N, 1, A, ...) by a group of 8 lanes: lane r < N holds row
-// r of A (arow) and entry r of b; returns entry r of x in x.  False when A
-// has a zero row or column or a zero pivot (dgesvx INFO in 1..N); the
-// result is then not formed.
-template <int N>
-__device__ bool group_dgesvx(int r, const double (&arow)[N], double b, double &x) {
-    const bool row = r < N;
-    double a[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) a[k] = row ? arow[k] : 0.0;
-    b = row ? b : 0.0;
-    // dgeequ: row scale factors, then column scale factors of the row-scaled A
-    double rs = 0.0;
-#pragma unroll
-    for (int k = 0; k < N; ++k) rs = fmax(rs, fabs(a[k]));
-    const double rcmax = gmax(row ? rs : 0.0);
-    const double rcmin = gmin(row ? rs : kBignum);
-    const double amax = rcmax;
-    if (rcmin == 0.0) return false;
-    rs = 1.0 / fmin(fmax(rs, kSfmin), kBignum);
-    const double rowcnd = fmax(rcmin, kSfmin) / fmin(rcmax, kBignum);
-    double cs[N];
-    double ccmin = kBignum, ccmax = 0.0;
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        cs[j] = gmax(row ? fabs(a[j]) * rs : 0.0);
-        ccmin = fmin(ccmin, cs[j]);
-        ccmax = fmax(ccmax, cs[j]);
-    }
-    if (ccmin == 0.0) return false;
-#pragma unroll
-    for (int j = 0; j < N; ++j) cs[j] = 1.0 / fmin(fmax(cs[j], kSfmin), kBignum);
-    const double colcnd = fmax(ccmin, kSfmin) / fmin(ccmax, kBignum);
-    // dlaqge
-    bool rows = false, cols;
-    if (rowcnd >= kThresh && amax >= kSmallEq && amax <= kLargeEq) {
-        cols = colcnd < kThresh;
-    } else {
-        rows = true;
-        cols = colcnd < kThresh;
-    }
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        if (rows && cols) a[j] = cs[j] * rs * a[j];
-        else if (rows) a[j] = rs * a[j];
-        else if (cols) a[j] = cs[j] * a[j];
-    }
-    if (rows) b = rs * b;
-    // dgetf2: the group's rows move with the interchanges; src tracks the
-    // original row held by each lane
-    double lu[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) lu[k] = a[k];
-    int src = r;
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        // pivot: the first row (from j on) of largest |a(i,j)|
-        double pv = (r >= j && row) ? fabs(lu[j]) : -1.0;
-        int pi = r;
-#pragma unroll
-        for (int o = 1; o < kPiLanes; o <<= 1) {
-            const double ov = __shfl_xor(pv, o, kPiLanes);
-            const int oi = __shfl_xor(pi, o, kPiLanes);
-            if (ov > pv || (ov == pv && oi < pi)) { pv = ov; pi = oi; }
-        }
-        const int p = pi;
-        if (gsh(lu[j], p) == 0.0) return false;
-        if (p != j) {
-            const int partner = r == j ? p : (r == p ? j : r);
-#pragma unroll
-            for (int k = 0; k < N; ++k) lu[k] = gsh(lu[k], partner);
-            src = gsh(src, partner);
-        }
-        double prow[N];
-#pragma unroll
-        for (int k = j; k < N; ++k) prow[k] = gsh(lu[k], j);
-        if (r > j && row) {
-            double l;
-            if (fabs(prow[j]) >= kSfmin) {
-                const double rp = 1.0 / prow[j];
-                l = rp * lu[j];
-            } else {
-                l = lu[j] / prow[j];
-            }
-            lu[j] = l;
-#pragma unroll
-            for (int k = j + 1; k < N; ++k) lu[k] = lu[k] + (-prow[k]) * l;
-        }
-    }
-    double ukk[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) ukk[k] = gsh(lu[k], k);
-    // dgetrs, then dgerfs: at most 5 refinement steps while the componentwise
-    // backward error exceeds eps and halves
-    double xv = group_getrs<N>(r, lu, ukk, src, b);
-    const double safe1 = (N + 1) * kSfmin, safe2 = safe1 / kEps;
-    double lstres = 3.0;
-    for (int count = 1; count <= 5; ++count) {
-        double w = b, den = fabs(b);
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            const double xj = gsh(xv, j);
-            w = w + (-xj) * a[j];
-        }
-#pragma unroll
-        for (int j = 0; j < N; ++j) den = den + fabs(a[j]) * fabs(gsh(xv, j));
-        const double e = !row ? 0.0 : (den > safe2 ? fabs(w) / den : (fabs(w) + safe1) / (den + safe1));
-        const double berr = gmax(e);
-        if (!(berr > kEps && 2.0 * berr <= lstres)) break;
-        const double dx = group_getrs<N>(r, lu, ukk, src, w);
-        xv = xv + dx;
-        lstres = berr;
-    }
-    if (cols) xv = pick(cs, r) * xv;
-    x = xv;
-    return true;
-}
-
 // the neighbour term of backwardEuler_LU_SGS (Time_BackwardEuler.f90:
 // 131-176 forward, 222-267 backward): the neighbour's primitive increment
 // dpa -> conserved increment at its state pa, then the linearised Rusanov
-// flux change 0.5*(-alpaF*dU + dF(dU).S); false if pa's temperature is out
-// of range
+// flux change 0.5*(-alpaF*dU + dF(dU).S).  The neighbour's gas parameters
+// and enthalpies come from its cache record nc (rr, Rcpcv, Cp, hh, h_k),
+// formed from the same state by k_be_factor with the same arithmetic.
 template <int NSP, class TT>
-__device__ __forceinline__ bool be_term(TT &th, const double *pa, const double *dpa, double dx, double dy,
-                                        double af, double *c) {
+__device__ __forceinline__ void be_term(TT &th, const double *nc, const double *pa, const double *dpa, double dx,
+                                        double dy, double af, double *c) {
     const double pp = pa[0], uu = pa[1], vv = pa[2], tt = pa[3];
-    double yy[NSP], s = 0.0;
+    double yy[NSP];
 #pragma unroll
-    for (int k = 0; k < NSP - 1; ++k) { yy[k] = pa[kNFlow + k]; s = s + yy[k]; }
-    yy[NSP - 1] = 1.0 - s;
-    double cp, gam, rcp, entha[NSP];
-    bool ok = gas_param<NSP>(th, yy, tt, cp, gam, rcp);
-    ok = enthalpy<NSP>(th, tt, entha) && ok;
-    double hh = 0.0;
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) hh = hh + yy[k] * entha[k];
-    const double rr = pp / (rcp * tt), rou = rr * uu, rov = rr * vv;
+    for (int k = 0; k < NSP - 1; ++k) yy[k] = pa[kNFlow + k];
+    const double rr = nc[0], rcp = nc[1], cp = nc[2], hh = nc[3];
+    const double rou = rr * uu, rov = rr * vv;
     const double roe = rr * (hh + 0.5 * (uu * uu + vv * vv)) - pp;
     const double dpp = dpa[0], duu = dpa[1], dvv = dpa[2], dtt = dpa[3];
-    double dyy[NSP];
-    s = 0.0;
+    double dyy[NSP], s = 0.0;
 #pragma unroll
     for (int k = 0; k < NSP - 1; ++k) { dyy[k] = dpa[kNFlow + k]; s = s + dyy[k]; }
     dyy[NSP - 1] = -s;
@@ -234,7 +74,7 @@ __device__ __forceinline__ bool be_term(TT &th, const double *pa, const double *
 #pragma unroll
     for (int k = 0; k < NSP; ++k) s1 = s1 + th.rsp[k] * dyy[k];
 #pragma unroll
-    for (int k = 0; k < NSP; ++k) s2 = s2 + entha[k] * dyy[k];
+    for (int k = 0; k < NSP; ++k) s2 = s2 + nc[4 + k] * dyy[k];
     const double drr = (dpp - rr * rcp * dtt - rr * tt * s1) / (rcp * tt);
     const double dru = rr * duu + uu * drr;
     const double drv = rr * dvv + vv * drr;
@@ -250,7 +90,6 @@ __device__ __forceinline__ bool be_term(TT &th, const double *pa, const double *
         const double dry = rr * dyy[k] + yy[k] * drr;
         c[kNFlow + k] = 0.5 * (-af * dry + dry * un + rr * yy[k] * dun);
     }
-    return ok;
 }
 
 // dkf/dPA of compute_derivative_production_rate (chemical_kinetics.f90:
@@ -327,74 +166,14 @@ __global__ void __launch_bounds__(256) k_be_prepare(ITEM_ARGS, double dt, int fi
     }
 }
 
-// one level of the forward (FWD) or backward point-implicit sweep
-// (Time_BackwardEuler.f90:102-193 / 195-285) over cells pi_cells[t] of the
-// work item, one 8-lane group per cell
-template <int NSP, bool FWD, bool B>
-__global__ void __launch_bounds__(256) k_be_sweep(ITEM_ARGS) {
-    constexpr int G = kPiLanes;
-    __shared__ DevThermo th_lds_;
-    const RC_G CaseDev *Cp_;
-    int t, tend;
-    if constexpr (B) {
-        const int4 w_ = work[blockIdx.x];
-        Cp_ = gp(cases) + w_.x;
-        t = w_.y + (int)threadIdx.x / G;
-        tend = w_.z;
-    } else {
-        Cp_ = gp(cases);
-        t = lo + (int)(blockIdx.x * blockDim.x + threadIdx.x) / G;
-        tend = hi;
-    }
-    const RC_G CaseDev &C = *Cp_;
-    {
-        const RC_G double *src_ = (const RC_G double *)gp(C.th);
-        double *dst_ = reinterpret_cast<double *>(&th_lds_);
-        for (int k_ = threadIdx.x; k_ < (int)(sizeof(DevThermo) / 8); k_ += blockDim.x) dst_[k_] = src_[k_];
-    }
-    __syncthreads();
-    if (t >= tend) return;             // the whole group
-    const GMesh M = gview(C.M);
-    const GState S = gview(C.S);
-    const DevThermo &th = th_lds_;
-    const int r = threadIdx.x % G;
-    const int sc = M.sc;
-    const int i = M.pi_cells[t];
-    const int kerr = FWD ? K_BE_FWD : K_BE_BWD;
-
-    // lanes 0-2: the term of face slot r, if its neighbour is on the swept side
-    double c[NV];
-    int use = 0;
-    {
-        const int ks = r < 3 ? r : 0;
-        const int nr = M.lu_nb[ks * sc + i];
-        use = (r < 3 && nr >= 0 && (FWD ? nr < i : nr > i)) ? 1 : 0;
-        const int j = use ? nr : i;
-        double pj[NV], dj[NV];
-#pragma unroll
-        for (int k = 0; k < NV; ++k) { pj[k] = PA_(k, j); dj[k] = HWA_(k, j); }
-        const bool ok = be_term<NSP>(th, pj, dj, M.lu_n[(ks * 2) * sc + i], M.lu_n[(ks * 2 + 1) * sc + i],
-                                     S.lu_af[ks * sc + i], c);
-        if (use && !ok) flag_error(S.err, RC_ERR_TEMPERATURE, i, kerr);
-    }
-    // dflux(r): the terms in face order
-    double dflux = 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        double mine = 0.0;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const double x = gsh(c[v], k);
-            mine = r == v ? x : mine;
-        }
-        dflux = dflux + (gsh(use, k) ? mine : 0.0);
-    }
-
-    // row r of the block: dU/dPA (Con_to_Pri_Jacobian.f90:1-87) * dd, minus
-    // vol * d(omega)/dPA in the species rows
-    double pa[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) pa[k] = PA_(k, i);
+// the block of cell i: dU/dPA (Compute_Con_to_Pri_Jaco,
+// Con_to_Pri_Jacobian.f90:1-87) * dd, minus vol * d(omega)/dPA in the
+// species rows (Time_BackwardEuler.f90:106-116); false if the temperature is
+// out of range
+template <int NSP, class TT>
+__device__ __forceinline__ bool cell_block(TT &th, const double *pa, double d, double vol, double (&J)[kNvar][kNvar],
+                                           double *nc) {
+    constexpr int NV = kNvar;
     const double p = pa[0], u = pa[1], v = pa[2], T = pa[3];
     double y[NSP], yc[NSP], s = 0.0;
 #pragma unroll
@@ -403,8 +182,20 @@ __global__ void __launch_bounds__(256) k_be_sweep(ITEM_ARGS) {
     double cp, gam, rcp, entha[NSP];
     bool ok = gas_param<NSP>(th, y, T, cp, gam, rcp);
     ok = enthalpy<NSP>(th, T, entha) && ok;
-    if (!ok) flag_error(S.err, RC_ERR_TEMPERATURE, i, kerr);
     const double rho = p / (rcp * T);
+    // the cache record the neighbour terms read (Time_BackwardEuler.f90:
+    // 139-152): rr, Rcpcv, Cp, hh = sum Y_k h_k, h_k
+    {
+        double hh = 0.0;
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) hh = hh + y[k] * entha[k];
+        nc[0] = rho;
+        nc[1] = rcp;
+        nc[2] = cp;
+        nc[3] = hh;
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) nc[4 + k] = entha[k];
+    }
     double h = entha[NSP - 1], dhdy[NSP - 1];
 #pragma unroll
     for (int k = 0; k < NSP - 1; ++k) {
@@ -416,69 +207,294 @@ __global__ void __launch_bounds__(256) k_be_sweep(ITEM_ARGS) {
 #pragma unroll
     for (int k = 0; k < NSP - 1; ++k) drdy[k] = -(th.rsp[k] - th.rsp[NSP - 1]) * p / (T * rcp * rcp);
     const double e2 = h + 0.5 * (u * u + v * v);
-    double jr[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) jr[k] = 0.0;
-    if (r == 0) {
-        jr[0] = drdp;
-        jr[3] = drdt;
+    for (int r = 0; r < NV; ++r)
 #pragma unroll
-        for (int k = 0; k < NSP - 1; ++k) jr[kNFlow + k] = drdy[k];
-    } else if (r == 1) {
-        jr[0] = u * drdp;
-        jr[1] = rho;
-        jr[3] = u * drdt;
+        for (int k = 0; k < NV; ++k) J[r][k] = 0.0;
+    J[0][0] = drdp; J[0][3] = drdt;
+    J[1][0] = u * drdp; J[1][1] = rho; J[1][3] = u * drdt;
+    J[2][0] = v * drdp; J[2][2] = rho; J[2][3] = v * drdt;
+    J[3][0] = e2 * drdp - 1.; J[3][1] = rho * u; J[3][2] = rho * v; J[3][3] = rho * cp + e2 * drdt;
 #pragma unroll
-        for (int k = 0; k < NSP - 1; ++k) jr[kNFlow + k] = u * drdy[k];
-    } else if (r == 2) {
-        jr[0] = v * drdp;
-        jr[2] = rho;
-        jr[3] = v * drdt;
-#pragma unroll
-        for (int k = 0; k < NSP - 1; ++k) jr[kNFlow + k] = v * drdy[k];
-    } else if (r == 3) {
-        jr[0] = e2 * drdp - 1.;
-        jr[1] = rho * u;
-        jr[2] = rho * v;
-        jr[3] = rho * cp + e2 * drdt;
-#pragma unroll
-        for (int k = 0; k < NSP - 1; ++k) jr[kNFlow + k] = e2 * drdy[k] + rho * dhdy[k];
-    } else if (r < NV) {
-        const int sp = r - kNFlow;
-        double ys = y[0];
-#pragma unroll
-        for (int k = 1; k < NSP - 1; ++k) ys = sp == k ? y[k] : ys;
-        jr[0] = drdp * ys;
-        jr[3] = drdt * ys;
-#pragma unroll
-        for (int k = 0; k < NSP - 1; ++k) jr[kNFlow + k] = sp == k ? rho + ys * drdy[k] : ys * drdy[k];
+    for (int j = 0; j < NSP - 1; ++j) {
+        J[0][kNFlow + j] = drdy[j];
+        J[1][kNFlow + j] = u * drdy[j];
+        J[2][kNFlow + j] = v * drdy[j];
+        J[3][kNFlow + j] = e2 * drdy[j] + rho * dhdy[j];
     }
-    const double d = S.dd[i];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) jr[k] = jr[k] * d;
+    for (int i = 0; i < NSP - 1; ++i) {
+        J[kNFlow + i][0] = drdp * y[i];
+        J[kNFlow + i][3] = drdt * y[i];
+#pragma unroll
+        for (int j = 0; j < NSP - 1; ++j) J[kNFlow + i][kNFlow + j] = i == j ? rho + y[i] * drdy[j] : y[i] * drdy[j];
+    }
+#pragma unroll
+    for (int r = 0; r < NV; ++r)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) J[r][k] = J[r][k] * d;
     // compute_derivative_production_rate's state: rho of the unclipped
     // fractions, then the fractions clipped to [0, 1]
 #pragma unroll
     for (int k = 0; k < NSP - 1; ++k) yc[k] = fmin(1., fmax(0., y[k]));
     double dkf[NV];
     dkf_dpa<NSP>(th, p, T, rho, rcp, yc, dkf);
-    if (r >= kNFlow && r < NV) {
-        const int sp = r - kNFlow;
-        const double wfac = 1000 * th.mw[sp] * th.dnu[sp];
-        const double vol = M.vol[i];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) jr[k] = jr[k] - vol * (wfac * dkf[k]);
+    for (int sp = 0; sp < NSP - 1; ++sp) {
+        const double wfac = 1000 * th.mw[sp] * th.dnu[sp];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) J[kNFlow + sp][k] = J[kNFlow + sp][k] - vol * (wfac * dkf[k]);
     }
+    return ok;
+}
 
-    // right-hand side and solve
-    const double hw = r < NV ? HWA_(r, i) : 0.0;
-    const double rhs = FWD ? hw - dflux : dflux;
-    double sol = 0.0;
-    if (!group_dgesvx<NV>(r, jr, rhs, sol)) {
-        if (r == 0) flag_error(S.err, RC_ERR_SINGULAR, i, kerr);
-        return;
+// k_be_factor: dgeequ + dlaqge + dgetf2 of every cell's block
+template <int NSP, bool B>
+__global__ void __launch_bounds__(256) k_be_factor(ITEM_ARGS, double *__restrict__ fac_, int *__restrict__ piv_,
+                                                   double *__restrict__ nbc_) {
+    ITEM_PROLOGUE_LDS(i);
+    constexpr int N = NV;
+    const DevThermo &th = TH;
+    const int sc = M.sc;
+    RC_G double *fac = gp(fac_);
+    RC_G int *piv = gp(piv_);
+    double pa[NV], a[N][N], nc[kBeNb];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) pa[k] = PA_(k, i);
+    if (!cell_block<NSP>(th, pa, S.dd[i], M.vol[i], a, nc)) flag_error(S.err, RC_ERR_TEMPERATURE, i, K_BE_FWD);
+    RC_G double *nbc = gp(nbc_) + (size_t)i * kBeNb;
+#pragma unroll
+    for (int k = 0; k < kBeNb; ++k) nbc[k] = nc[k];
+    int flags = 0;
+    // dgeequ
+    double r[N], c[N];
+    double rcmax = 0.0, rcmin = kBignum;
+#pragma unroll
+    for (int ii = 0; ii < N; ++ii) {
+        r[ii] = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) r[ii] = fmax(r[ii], fabs(a[ii][j]));
+        rcmax = fmax(rcmax, r[ii]);
+        rcmin = fmin(rcmin, r[ii]);
     }
-    if (r < NV) HWA_(r, i) = FWD ? sol : hw - sol;
+    const double amax = rcmax;
+    bool singular = rcmin == 0.0;
+#pragma unroll
+    for (int ii = 0; ii < N; ++ii) r[ii] = 1.0 / fmin(fmax(r[ii], kSfmin), kBignum);
+    const double rowcnd = fmax(rcmin, kSfmin) / fmin(rcmax, kBignum);
+    double ccmin = kBignum, ccmax = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        c[j] = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < N; ++ii) c[j] = fmax(c[j], fabs(a[ii][j]) * r[ii]);
+        ccmin = fmin(ccmin, c[j]);
+        ccmax = fmax(ccmax, c[j]);
+    }
+    singular = singular || ccmin == 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) c[j] = 1.0 / fmin(fmax(c[j], kSfmin), kBignum);
+    const double colcnd = fmax(ccmin, kSfmin) / fmin(ccmax, kBignum);
+    // dlaqge
+    bool rows = false, cols;
+    if (rowcnd >= kThresh && amax >= kSmallEq && amax <= kLargeEq) {
+        cols = colcnd < kThresh;
+    } else {
+        rows = true;
+        cols = colcnd < kThresh;
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+#pragma unroll
+        for (int ii = 0; ii < N; ++ii) {
+            if (rows && cols) a[ii][j] = c[j] * r[ii] * a[ii][j];
+            else if (rows) a[ii][j] = r[ii] * a[ii][j];
+            else if (cols) a[ii][j] = c[j] * a[ii][j];
+        }
+#pragma unroll
+    for (int ii = 0; ii < N; ++ii)
+#pragma unroll
+        for (int j = 0; j < N; ++j) BEF_(kBeAE + ii * N + j, i) = a[ii][j];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        BEF_(kBeRS + k, i) = r[k];
+        BEF_(kBeCS + k, i) = c[k];
+    }
+    flags = (rows ? 1 : 0) | (cols ? 2 : 0);
+    // dgetf2: partial pivoting (the first row of largest |a(i,j)|), rows
+    // exchanged in registers by selects
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        int p = j;
+        double pmax = fabs(a[j][j]);
+#pragma unroll
+        for (int ii = j + 1; ii < N; ++ii) {
+            const double t = fabs(a[ii][j]);
+            if (t > pmax) { pmax = t; p = ii; }
+        }
+        BEP_(j, i) = p;
+        double pval = a[j][j];
+#pragma unroll
+        for (int ii = j + 1; ii < N; ++ii) pval = p == ii ? a[ii][j] : pval;
+        singular = singular || pval == 0.0;
+#pragma unroll
+        for (int ii = j + 1; ii < N; ++ii) {
+            const bool sw = p == ii;
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                const double t = a[j][k];
+                a[j][k] = sw ? a[ii][k] : t;
+                a[ii][k] = sw ? t : a[ii][k];
+            }
+        }
+        if (fabs(a[j][j]) >= kSfmin) {
+            const double rp = 1.0 / a[j][j];
+#pragma unroll
+            for (int ii = j + 1; ii < N; ++ii) a[ii][j] = rp * a[ii][j];
+        } else {
+#pragma unroll
+            for (int ii = j + 1; ii < N; ++ii) a[ii][j] = a[ii][j] / a[j][j];
+        }
+#pragma unroll
+        for (int k = j + 1; k < N; ++k) {
+            const double t = -a[j][k];
+#pragma unroll
+            for (int ii = j + 1; ii < N; ++ii) a[ii][k] = a[ii][k] + t * a[ii][j];
+        }
+    }
+#pragma unroll
+    for (int ii = 0; ii < N; ++ii)
+#pragma unroll
+        for (int j = 0; j < N; ++j) BEF_(kBeLU + ii * N + j, i) = a[ii][j];
+    if (singular) {
+        flags |= 4;
+        flag_error(S.err, RC_ERR_SINGULAR, i, K_BE_FWD);
+    }
+    BEP_(N, i) = flags;
+}
+
+// dgetrs on v with the factors of cell i: pivots from memory, the LU
+// factors from registers
+template <int N, class P>
+__device__ __forceinline__ void be_getrs(const P &piv, int sc, int i, const double (&lu)[N][N], double (&v)[N]) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int p = BEP_(j, i);
+#pragma unroll
+        for (int ii = j + 1; ii < N; ++ii) {
+            const bool sw = p == ii;
+            const double t = v[j];
+            v[j] = sw ? v[ii] : t;
+            v[ii] = sw ? t : v[ii];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+        if (v[k] != 0.0)
+#pragma unroll
+            for (int ii = k + 1; ii < N; ++ii) v[ii] = v[ii] - v[k] * lu[ii][k];
+#pragma unroll
+    for (int k = N - 1; k >= 0; --k)
+        if (v[k] != 0.0) {
+            v[k] = v[k] / lu[k][k];
+#pragma unroll
+            for (int ii = 0; ii < k; ++ii) v[ii] = v[ii] - v[k] * lu[ii][k];
+        }
+}
+
+// one level of the forward (FWD) or backward point-implicit sweep
+// (Time_BackwardEuler.f90:102-193 / 195-285) over cells pi_cells[t] of the
+// work item, one thread per cell
+template <int NSP, bool FWD, bool B>
+__global__ void __launch_bounds__(kPiChunk) k_be_sweep(ITEM_ARGS, const double *__restrict__ fac_,
+                                                      const int *__restrict__ piv_,
+                                                      const double *__restrict__ nbc_) {
+    ITEM_PROLOGUE_LDS(t);
+    constexpr int N = NV;
+    const DevThermo &th = TH;
+    const int sc = M.sc;
+    const RC_G double *fac = gp(fac_);
+    const RC_G int *piv = gp(piv_);
+    const RC_G double *nbc = gp(nbc_);
+    const int i = M.pi_cells[t];
+    const int flags = BEP_(N, i);
+    if (flags & 4) return;               // singular block: reported by k_be_factor
+    double lu[N][N];
+#pragma unroll
+    for (int ii = 0; ii < N; ++ii)
+#pragma unroll
+        for (int j = 0; j < N; ++j) lu[ii][j] = BEF_(kBeLU + ii * N + j, i);
+    // the three neighbour records at once (faces that take no part read the
+    // cell's own records; their terms are not added)
+    int nbr[3];
+    bool use[3];
+#pragma unroll
+    for (int ks = 0; ks < 3; ++ks) {
+        const int nr = M.lu_nb[ks * sc + i];
+        use[ks] = nr >= 0 && (FWD ? nr < i : nr > i);
+        nbr[ks] = use[ks] ? nr : i;
+    }
+    double pj[3][N], dj[3][N], nc[3][kBeNb];
+#pragma unroll
+    for (int ks = 0; ks < 3; ++ks) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) { pj[ks][k] = PA_(k, nbr[ks]); dj[ks][k] = HWA_(k, nbr[ks]); }
+#pragma unroll
+        for (int k = 0; k < kBeNb; ++k) nc[ks][k] = nbc[(size_t)nbr[ks] * kBeNb + k];
+    }
+    double dflux[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) dflux[k] = 0.0;
+#pragma unroll
+    for (int ks = 0; ks < 3; ++ks) {
+        double c[N];
+        be_term<NSP>(th, nc[ks], pj[ks], dj[ks], M.lu_n[(ks * 2) * sc + i], M.lu_n[(ks * 2 + 1) * sc + i],
+                     S.lu_af[ks * sc + i], c);
+#pragma unroll
+        for (int k = 0; k < N; ++k) dflux[k] = use[ks] ? dflux[k] + c[k] : dflux[k];
+    }
+    double hw[N], b[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        hw[k] = HWA_(k, i);
+        const double rhs = FWD ? hw[k] - dflux[k] : dflux[k];
+        b[k] = (flags & 1) ? BEF_(kBeRS + k, i) * rhs : rhs;
+    }
+    double x[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) x[k] = b[k];
+    be_getrs<N>(piv, sc, i, lu, x);
+    // dgerfs
+    constexpr double safe1 = (N + 1) * kSfmin, safe2 = safe1 / kEps;
+    double lstres = 3.0;
+    for (int count = 1; count <= 5; ++count) {
+        double w[N], berr = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < N; ++ii) w[ii] = b[ii];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const double tj = -x[j];
+#pragma unroll
+            for (int ii = 0; ii < N; ++ii) w[ii] = w[ii] + tj * BEF_(kBeAE + ii * N + j, i);
+        }
+#pragma unroll
+        for (int ii = 0; ii < N; ++ii) {
+            double den = fabs(b[ii]);
+#pragma unroll
+            for (int j = 0; j < N; ++j) den = den + fabs(BEF_(kBeAE + ii * N + j, i)) * fabs(x[j]);
+            berr = fmax(berr, den > safe2 ? fabs(w[ii]) / den : (fabs(w[ii]) + safe1) / (den + safe1));
+        }
+        if (!(berr > kEps && 2.0 * berr <= lstres)) break;
+        be_getrs<N>(piv, sc, i, lu, w);
+#pragma unroll
+        for (int k = 0; k < N; ++k) x[k] = x[k] + w[k];
+        lstres = berr;
+    }
+    if (flags & 2)
+#pragma unroll
+        for (int k = 0; k < N; ++k) x[k] = BEF_(kBeCS + k, i) * x[k];
+#pragma unroll
+    for (int k = 0; k < N; ++k) HWA_(k, i) = FWD ? x[k] : hw[k] - x[k];
 }
 
 // PA += dPA (Time_BackwardEuler.f90:288-292), primitve_to_conserved_variables
@@ -508,13 +524,37 @@ void launch_be_prepare(const LaunchSet &L, double dt, bool first, hipStream_t st
     }
 }
 
-void launch_be_level(const LaunchSet &L, bool fwd, int l, hipStream_t st) {
-    if (fwd) LAUNCH(L.pi_level[l], kPiChunk, st, k_be_sweep, kNsp, true);
-    else LAUNCH(L.pi_level[l], kPiChunk, st, k_be_sweep, kNsp, false);
+void launch_be_factor(const LaunchSet &L, double *fac, int *piv, double *nbc, hipStream_t st) {
+    if (L.single) {
+        const int n = L.cell.hi - L.cell.lo;
+        if (n > 0)
+            k_be_factor<kNsp, false><<<(n + kChunk - 1) / kChunk, kChunk, 0, st>>>(L.cases, nullptr, L.cell.lo,
+                                                                                   L.cell.hi, fac, piv, nbc);
+    } else if (L.cell.n) {
+        k_be_factor<kNsp, true><<<L.cell.n, kChunk, 0, st>>>(L.cases, L.cell.w, 0, 0, fac, piv, nbc);
+    }
+}
+
+void launch_be_level(const LaunchSet &L, bool fwd, int l, const double *fac, const int *piv, const double *nbc,
+                     hipStream_t st) {
+    const WorkList &w = L.pi_level[l];
+    if (L.single) {
+        const int n = w.hi - w.lo;
+        if (n <= 0) return;
+        const int nb = (n + kPiChunk - 1) / kPiChunk;
+        if (fwd) k_be_sweep<kNsp, true, false><<<nb, kPiChunk, 0, st>>>(L.cases, nullptr, w.lo, w.hi, fac, piv, nbc);
+        else k_be_sweep<kNsp, false, false><<<nb, kPiChunk, 0, st>>>(L.cases, nullptr, w.lo, w.hi, fac, piv, nbc);
+    } else if (w.n) {
+        if (fwd) k_be_sweep<kNsp, true, true><<<w.n, kPiChunk, 0, st>>>(L.cases, w.w, 0, 0, fac, piv, nbc);
+        else k_be_sweep<kNsp, false, true><<<w.n, kPiChunk, 0, st>>>(L.cases, w.w, 0, 0, fac, piv, nbc);
+    }
 }
 
 void launch_be_update(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_be_update, kNsp); }
 
 int pi_chunk() { return kPiChunk; }
+size_t be_factor_doubles() { return kBeFac; }
+size_t be_factor_ints() { return kBePiv; }
+size_t be_cache_doubles() { return kBeNb; }
 
 }  // namespace rc
