@@ -157,6 +157,30 @@ typedef struct rc_transport {
  * stage alike. */
 int rc_set_transport(rc_case *h, const rc_transport *tr);
 
+/* ---- learned chemistry (no reference counterpart): an FP32 MLP for the
+ * species source term, omega_1..omega_{Nsp-1} from (T, p, Y_1..Y_Nsp), with
+ * `hidden` tanh layers of `width`, trained on the device by the library's own
+ * forward, backward and Adam kernels.  Parameters are one vector: per layer
+ * W (fan_in x fan_out, row-major) then b. */
+typedef struct rc_mlp rc_mlp;
+int  rc_mlp_create(int device, int n_in, int width, int hidden, int n_out, rc_mlp **out);
+void rc_mlp_destroy(rc_mlp *m);
+const char *rc_mlp_last_error(const rc_mlp *m);
+long long rc_mlp_num_params(const rc_mlp *m);
+int  rc_mlp_set_params(rc_mlp *m, const float *params);      /* also resets the Adam moments */
+int  rc_mlp_get_params(rc_mlp *m, float *params);
+/* y[n x n_out] = f(x[n x n_in]) */
+int  rc_mlp_forward(rc_mlp *m, const float *x, int n, float *y);
+/* mean squared error of f(x) against t over the n x n_out outputs, and its
+ * gradient with respect to the parameter vector (grad may be NULL) */
+int  rc_mlp_gradient(rc_mlp *m, const float *x, const float *t, int n, float *grad, float *loss);
+/* the training set, resident on the device (n rows) */
+int  rc_mlp_set_data(rc_mlp *m, const float *x, const float *t, long long n);
+/* nsteps Adam steps on minibatches of `batch` rows drawn with replacement
+ * (SplitMix64 stream `seed`); every `every` steps (and the last) the batch
+ * loss goes to loss_out (may be NULL) */
+int  rc_mlp_train(rc_mlp *m, int nsteps, int batch, float lr, unsigned long long seed, int every, float *loss_out);
+
 /* ---- dual-time backward Euler with the point-implicit LU-SGS
  * (implicit_time_method = 3; fvm2d.f90:60-110 unsteady branch). */
 

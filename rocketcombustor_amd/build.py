@@ -17,7 +17,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "librocketcombustor.so")
 SOURCES = ["csrc/rc_api.hip", "csrc/rc_kernels.hip", "csrc/rc_backward_euler.hip", "csrc/rc_viscous.hip",
-           "csrc/rc_mesh.cpp"]
+           "csrc/rc_mlp.hip", "csrc/rc_mesh.cpp"]
 ARCH = os.environ.get("RC_OFFLOAD_ARCH", "gfx950")
 
 

@@ -1,0 +1,412 @@
+// FP32 MLP for the species source term (no reference counterpart): a
+// network (T, p, Y_1..Y_Nsp) -> omega_1..omega_{Nsp-1} with three tanh hidden
+// layers, trained on the device by the kernels below (forward, backward,
+// Adam) and evaluated per cell inside the steady step with a per-cell gate
+// and the exact FP64 kinetics as fallback.
+//
+// All dense products go through one LDS-tiled FP32 GEMM (64x64 tiles, 256
+// threads, 4x4 outputs per thread) with the layer's epilogue fused: bias +
+// tanh (hidden), bias (output), or the tanh derivative (backward).  Column
+// sums (bias gradients) and all other reductions are block reductions in a
+// fixed order, so training is deterministic run to run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rc_items.h"
+
+namespace rc {
+
+constexpr int kTile = 64, kTk = 16;
+
+enum Epi { EPI_NONE = 0, EPI_BIAS = 1, EPI_BIAS_TANH = 2, EPI_DTANH = 3 };
+
+// C[M x N] = op(A) op(B) (+ epilogue).  op(A) is M x K: A[m*lda + k], or
+// A[k*lda + m] when TA; op(B) is K x N: B[k*ldb + n], or B[n*ldb + k] when TB.
+// EPI_BIAS(_TANH): + bias[n] (then tanh); EPI_DTANH: * (1 - aux[m*ldc+n]^2).
+template <bool TA, bool TB, int EPI>
+__global__ void __launch_bounds__(256) k_gemm(const float *__restrict__ A, const float *__restrict__ B,
+                                              float *__restrict__ C, int M, int N, int K, int lda, int ldb,
+                                              int ldc, const float *__restrict__ bias,
+                                              const float *__restrict__ aux) {
+    __shared__ float As[kTk][kTile + 1];
+    __shared__ float Bs[kTk][kTile + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+    float acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.f;
+    for (int k0 = 0; k0 < K; k0 += kTk) {
+        // 64 x 16 of op(A) and 16 x 64 of op(B), 4 elements per thread each
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int e = threadIdx.x + r * 256;     // 0..1023
+            const int mm = e % kTile, kk = e / kTile;
+            const int m = m0 + mm, k = k0 + kk;
+            float v = 0.f;
+            if (m < M && k < K) v = TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
+            As[kk][mm] = v;
+            const int nn = e % kTile;
+            const int n = n0 + nn;
+            float w = 0.f;
+            if (n < N && k < K) w = TB ? B[(size_t)n * ldb + k] : B[(size_t)k * ldb + n];
+            Bs[kk][nn] = w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kTk; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { a[q] = As[kk][ty + 16 * q]; b[q] = Bs[kk][tx + 16 * q]; }
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[p][q] = fmaf(a[p], b[q], acc[p][q]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int m = m0 + ty + 16 * p;
+        if (m >= M) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int n = n0 + tx + 16 * q;
+            if (n >= N) continue;
+            float v = acc[p][q];
+            if (EPI == EPI_BIAS || EPI == EPI_BIAS_TANH) v = v + bias[n];
+            if (EPI == EPI_BIAS_TANH) v = tanhf(v);
+            if (EPI == EPI_DTANH) {
+                const float h = aux[(size_t)m * ldc + n];
+                v = v * (1.f - h * h);
+            }
+            C[(size_t)m * ldc + n] = v;
+        }
+    }
+}
+
+template <bool TA, bool TB, int EPI>
+void gemm(hipStream_t st, const float *A, const float *B, float *C, int M, int N, int K, int lda, int ldb, int ldc,
+          const float *bias = nullptr, const float *aux = nullptr) {
+    dim3 grid((N + kTile - 1) / kTile, (M + kTile - 1) / kTile);
+    k_gemm<TA, TB, EPI><<<grid, 256, 0, st>>>(A, B, C, M, N, K, lda, ldb, ldc, bias, aux);
+}
+
+// out[n] = sum_m X[m*ld + n] over m < M: one block per column, fixed order
+__global__ void __launch_bounds__(256) k_colsum(const float *__restrict__ X, float *__restrict__ out, int M, int ld) {
+    __shared__ float sh[256];
+    const int n = blockIdx.x;
+    float s = 0.f;
+    for (int m = threadIdx.x; m < M; m += 256) s += X[(size_t)m * ld + n];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[n] = sh[0];
+}
+
+// G = 2 (Y - T) / (M*N), and per-block partial sums of (Y - T)^2
+__global__ void __launch_bounds__(256) k_mse_grad(const float *__restrict__ Y, const float *__restrict__ T,
+                                                  float *__restrict__ G, float *__restrict__ partial, int count,
+                                                  float scale) {
+    __shared__ float sh[256];
+    float s = 0.f;
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < count; e += gridDim.x * 256) {
+        const float r = Y[e] - T[e];
+        G[e] = 2.f * r * scale;
+        s += r * r;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void k_sum_partials(const float *__restrict__ partial, int n, float scale, float *__restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        float s = 0.f;
+        for (int k = 0; k < n; ++k) s += partial[k];
+        *out = s * scale;
+    }
+}
+
+// Adam (Kingma & Ba) with bias correction; step t from 1
+__global__ void __launch_bounds__(256) k_adam(float *__restrict__ p, const float *__restrict__ g,
+                                              float *__restrict__ m1, float *__restrict__ m2, int n, float lr,
+                                              float b1, float b2, float eps, float c1, float c2) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const float gi = g[e];
+    const float a = b1 * m1[e] + (1.f - b1) * gi;
+    const float b = b2 * m2[e] + (1.f - b2) * gi * gi;
+    m1[e] = a;
+    m2[e] = b;
+    p[e] = p[e] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014)
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// a minibatch of rows drawn with replacement from the resident data set
+__global__ void __launch_bounds__(256) k_gather_batch(const float *__restrict__ X, const float *__restrict__ Y,
+                                                      float *__restrict__ xb, float *__restrict__ yb, int batch,
+                                                      long long n, int nin, int nout, uint64_t seed) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= batch) return;
+    const long long src = (long long)(splitmix64(seed + (uint64_t)r) % (uint64_t)n);
+    for (int k = 0; k < nin; ++k) xb[(size_t)r * nin + k] = X[(size_t)src * nin + k];
+    for (int k = 0; k < nout; ++k) yb[(size_t)r * nout + k] = Y[(size_t)src * nout + k];
+}
+
+}  // namespace rc
+
+using namespace rc;
+
+// device-resident network, data set and work buffers
+struct rc_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nin = 0, nout = 0, width = 0, nlayer = 0;   // nlayer = hidden layers + 1
+    std::vector<int> dims;                           // nin, width x hidden, nout
+    std::vector<size_t> woff, boff;                  // offsets of W_l, b_l in the parameter vector
+    size_t nparam = 0;
+    float *params = nullptr, *grads = nullptr, *m1 = nullptr, *m2 = nullptr;
+    long long adam_t = 0;
+    float *data_x = nullptr, *data_y = nullptr;
+    long long ndata = 0;
+    int cap = 0;                                     // rows of the work buffers
+    std::vector<float *> act;                        // act[0] = inputs, act[l] = layer l outputs (cap rows)
+    std::vector<float *> dz;                         // gradients of the layer pre-activations (cap rows)
+    float *ybuf = nullptr, *partial = nullptr, *loss = nullptr;
+    std::string err;
+};
+
+namespace {
+thread_local std::string g_mlp_error;
+int mlp_fail(rc_mlp *m, int code, const std::string &msg) {
+    if (m) m->err = msg;
+    g_mlp_error = msg;
+    return code;
+}
+#define MCHK(m, x)                                                                                       \
+    do {                                                                                                 \
+        hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess) return mlp_fail(m, RC_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kPartials = 256;
+
+int ensure_cap(rc_mlp *m, int rows) {
+    if (rows <= m->cap) return RC_OK;
+    for (float *p : m->act) (void)hipFree(p);
+    for (float *p : m->dz) (void)hipFree(p);
+    if (m->ybuf) (void)hipFree(m->ybuf);
+    m->act.assign(m->nlayer + 1, nullptr);
+    m->dz.assign(m->nlayer + 1, nullptr);
+    for (int l = 0; l <= m->nlayer; ++l) {
+        MCHK(m, hipMalloc(&m->act[l], sizeof(float) * (size_t)rows * m->dims[l]));
+        MCHK(m, hipMalloc(&m->dz[l], sizeof(float) * (size_t)rows * m->dims[l]));
+    }
+    MCHK(m, hipMalloc(&m->ybuf, sizeof(float) * (size_t)rows * m->nout));
+    m->cap = rows;
+    return RC_OK;
+}
+
+// forward of rows act[0][0..n): hidden layers bias+tanh, output layer bias
+void forward(rc_mlp *m, int n, const float *params) {
+    for (int l = 0; l < m->nlayer; ++l) {
+        const float *W = params + m->woff[l], *b = params + m->boff[l];
+        const int fi = m->dims[l], fo = m->dims[l + 1];
+        if (l < m->nlayer - 1)
+            gemm<false, false, EPI_BIAS_TANH>(m->stream, m->act[l], W, m->act[l + 1], n, fo, fi, fi, fo, fo, b);
+        else
+            gemm<false, false, EPI_BIAS>(m->stream, m->act[l], W, m->act[l + 1], n, fo, fi, fi, fo, fo, b);
+    }
+}
+
+// loss and parameter gradient of rows 0..n against targets ybuf
+void backward(rc_mlp *m, int n) {
+    const int L = m->nlayer;
+    const int count = n * m->nout;
+    const int nb = std::min(kPartials, (count + 255) / 256);
+    k_mse_grad<<<nb, 256, 0, m->stream>>>(m->act[L], m->ybuf, m->dz[L], m->partial, count, 1.f / (float)count);
+    k_sum_partials<<<1, 64, 0, m->stream>>>(m->partial, nb, 1.f / (float)count, m->loss);
+    for (int l = L - 1; l >= 0; --l) {
+        const int fi = m->dims[l], fo = m->dims[l + 1];
+        // dW_l = act_l^T dz_{l+1}; db_l = column sums of dz_{l+1}
+        gemm<true, false, EPI_NONE>(m->stream, m->act[l], m->dz[l + 1], m->grads + m->woff[l], fi, fo, n, fi, fo,
+                                    fo);
+        k_colsum<<<fo, 256, 0, m->stream>>>(m->dz[l + 1], m->grads + m->boff[l], n, fo);
+        if (l > 0)   // dz_l = (dz_{l+1} W_l^T) * (1 - act_l^2)
+            gemm<false, true, EPI_DTANH>(m->stream, m->dz[l + 1], m->params + m->woff[l], m->dz[l], n, fi, fo, fo,
+                                         fo, fi, nullptr, m->act[l]);
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char *rc_mlp_last_error(const rc_mlp *m) { return m ? m->err.c_str() : g_mlp_error.c_str(); }
+
+int rc_mlp_create(int device, int n_in, int width, int hidden, int n_out, rc_mlp **out) {
+    if (!out || n_in < 1 || width < 1 || hidden < 1 || n_out < 1) return mlp_fail(nullptr, RC_ERR_ARGUMENT, "bad argument");
+    *out = nullptr;
+    rc_mlp *m = new rc_mlp();
+    m->device = device;
+    m->nin = n_in;
+    m->nout = n_out;
+    m->width = width;
+    m->nlayer = hidden + 1;
+    m->dims.push_back(n_in);
+    for (int k = 0; k < hidden; ++k) m->dims.push_back(width);
+    m->dims.push_back(n_out);
+    for (int l = 0; l < m->nlayer; ++l) {
+        m->woff.push_back(m->nparam);
+        m->nparam += (size_t)m->dims[l] * m->dims[l + 1];
+        m->boff.push_back(m->nparam);
+        m->nparam += m->dims[l + 1];
+    }
+    auto bail = [&](int code) { rc_mlp_destroy(m); return code; };
+    if (hipSetDevice(device) != hipSuccess) return bail(mlp_fail(nullptr, RC_ERR_DEVICE, "hipSetDevice failed"));
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(mlp_fail(nullptr, RC_ERR_DEVICE, "stream creation failed"));
+    for (float **p : {&m->params, &m->grads, &m->m1, &m->m2})
+        if (hipMalloc(p, sizeof(float) * m->nparam) != hipSuccess)
+            return bail(mlp_fail(nullptr, RC_ERR_MEMORY, "device allocation failed"));
+    if (hipMalloc(&m->partial, sizeof(float) * kPartials) != hipSuccess ||
+        hipMalloc(&m->loss, sizeof(float)) != hipSuccess)
+        return bail(mlp_fail(nullptr, RC_ERR_MEMORY, "device allocation failed"));
+    if (hipMemsetAsync(m->m1, 0, sizeof(float) * m->nparam, m->stream) != hipSuccess ||
+        hipMemsetAsync(m->m2, 0, sizeof(float) * m->nparam, m->stream) != hipSuccess ||
+        hipMemsetAsync(m->params, 0, sizeof(float) * m->nparam, m->stream) != hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+        return bail(mlp_fail(nullptr, RC_ERR_DEVICE, "device initialisation failed"));
+    *out = m;
+    return RC_OK;
+}
+
+void rc_mlp_destroy(rc_mlp *m) {
+    if (!m) return;
+    if (m->stream) {
+        (void)hipSetDevice(m->device);
+        (void)hipStreamSynchronize(m->stream);
+    }
+    for (float *p : {m->params, m->grads, m->m1, m->m2, m->data_x, m->data_y, m->ybuf, m->partial, m->loss})
+        if (p) (void)hipFree(p);
+    for (float *p : m->act) if (p) (void)hipFree(p);
+    for (float *p : m->dz) if (p) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+long long rc_mlp_num_params(const rc_mlp *m) { return m ? (long long)m->nparam : -1; }
+
+int rc_mlp_set_params(rc_mlp *m, const float *p) {
+    if (!m || !p) return mlp_fail(m, RC_ERR_ARGUMENT, "NULL argument");
+    MCHK(m, hipSetDevice(m->device));
+    MCHK(m, hipMemcpyAsync(m->params, p, sizeof(float) * m->nparam, hipMemcpyHostToDevice, m->stream));
+    MCHK(m, hipMemsetAsync(m->m1, 0, sizeof(float) * m->nparam, m->stream));
+    MCHK(m, hipMemsetAsync(m->m2, 0, sizeof(float) * m->nparam, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    m->adam_t = 0;
+    return RC_OK;
+}
+
+int rc_mlp_get_params(rc_mlp *m, float *p) {
+    if (!m || !p) return mlp_fail(m, RC_ERR_ARGUMENT, "NULL argument");
+    MCHK(m, hipSetDevice(m->device));
+    MCHK(m, hipMemcpyAsync(p, m->params, sizeof(float) * m->nparam, hipMemcpyDeviceToHost, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    return RC_OK;
+}
+
+int rc_mlp_forward(rc_mlp *m, const float *x, int n, float *y) {
+    if (!m || !x || !y || n < 0) return mlp_fail(m, RC_ERR_ARGUMENT, "bad argument");
+    if (n == 0) return RC_OK;
+    MCHK(m, hipSetDevice(m->device));
+    int rc = ensure_cap(m, n);
+    if (rc) return rc;
+    MCHK(m, hipMemcpyAsync(m->act[0], x, sizeof(float) * (size_t)n * m->nin, hipMemcpyHostToDevice, m->stream));
+    forward(m, n, m->params);
+    MCHK(m, hipMemcpyAsync(y, m->act[m->nlayer], sizeof(float) * (size_t)n * m->nout, hipMemcpyDeviceToHost,
+                           m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    return RC_OK;
+}
+
+int rc_mlp_gradient(rc_mlp *m, const float *x, const float *t, int n, float *grad, float *loss) {
+    if (!m || !x || !t || n < 1) return mlp_fail(m, RC_ERR_ARGUMENT, "bad argument");
+    MCHK(m, hipSetDevice(m->device));
+    int rc = ensure_cap(m, n);
+    if (rc) return rc;
+    MCHK(m, hipMemcpyAsync(m->act[0], x, sizeof(float) * (size_t)n * m->nin, hipMemcpyHostToDevice, m->stream));
+    MCHK(m, hipMemcpyAsync(m->ybuf, t, sizeof(float) * (size_t)n * m->nout, hipMemcpyHostToDevice, m->stream));
+    forward(m, n, m->params);
+    backward(m, n);
+    if (grad)
+        MCHK(m, hipMemcpyAsync(grad, m->grads, sizeof(float) * m->nparam, hipMemcpyDeviceToHost, m->stream));
+    if (loss) MCHK(m, hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    return RC_OK;
+}
+
+int rc_mlp_set_data(rc_mlp *m, const float *x, const float *t, long long n) {
+    if (!m || !x || !t || n < 1) return mlp_fail(m, RC_ERR_ARGUMENT, "bad argument");
+    MCHK(m, hipSetDevice(m->device));
+    if (m->data_x) (void)hipFree(m->data_x);
+    if (m->data_y) (void)hipFree(m->data_y);
+    m->data_x = m->data_y = nullptr;
+    MCHK(m, hipMalloc(&m->data_x, sizeof(float) * (size_t)n * m->nin));
+    MCHK(m, hipMalloc(&m->data_y, sizeof(float) * (size_t)n * m->nout));
+    MCHK(m, hipMemcpyAsync(m->data_x, x, sizeof(float) * (size_t)n * m->nin, hipMemcpyHostToDevice, m->stream));
+    MCHK(m, hipMemcpyAsync(m->data_y, t, sizeof(float) * (size_t)n * m->nout, hipMemcpyHostToDevice, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    m->ndata = n;
+    return RC_OK;
+}
+
+int rc_mlp_train(rc_mlp *m, int nsteps, int batch, float lr, unsigned long long seed, int every, float *loss_out) {
+    if (!m || nsteps < 0 || batch < 1) return mlp_fail(m, RC_ERR_ARGUMENT, "bad argument");
+    if (!m->data_x) return mlp_fail(m, RC_ERR_ARGUMENT, "no training data (rc_mlp_set_data)");
+    MCHK(m, hipSetDevice(m->device));
+    int rc = ensure_cap(m, batch);
+    if (rc) return rc;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    int nrep = 0;
+    for (int s = 1; s <= nsteps; ++s) {
+        k_gather_batch<<<(batch + 255) / 256, 256, 0, m->stream>>>(m->data_x, m->data_y, m->act[0], m->ybuf, batch,
+                                                                   m->ndata, m->nin, m->nout,
+                                                                   seed + (uint64_t)(m->adam_t + 1) * 0x100000000ull);
+        forward(m, batch, m->params);
+        backward(m, batch);
+        m->adam_t++;
+        const float c1 = 1.f - std::pow(b1, (float)m->adam_t), c2 = 1.f - std::pow(b2, (float)m->adam_t);
+        k_adam<<<(int)((m->nparam + 255) / 256), 256, 0, m->stream>>>(m->params, m->grads, m->m1, m->m2,
+                                                                      (int)m->nparam, lr, b1, b2, eps, c1, c2);
+        if (loss_out && every > 0 && (s % every == 0 || s == nsteps)) {
+            MCHK(m, hipMemcpyAsync(loss_out + nrep, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+            ++nrep;
+        }
+    }
+    MCHK(m, hipStreamSynchronize(m->stream));
+    return RC_OK;
+}
+
+}  // extern "C"

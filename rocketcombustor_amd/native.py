@@ -67,6 +67,7 @@ P = C.POINTER
 _VP = C.c_void_p
 _DP = P(C.c_double)
 _IP = P(C.c_int)
+_FP = P(C.c_float)
 
 # name -> (restype, argtypes); every function of include/rocketcombustor.h
 SIGNATURES = {
@@ -87,6 +88,16 @@ SIGNATURES = {
     "rc_residual": (C.c_int, [_VP, _DP]),
     "rc_set_outlet_pressure": (C.c_int, [_VP, C.c_double]),
     "rc_set_transport": (C.c_int, [_VP, P(RcTransport)]),
+    "rc_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(_VP)]),
+    "rc_mlp_destroy": (None, [_VP]),
+    "rc_mlp_last_error": (C.c_char_p, [_VP]),
+    "rc_mlp_num_params": (C.c_longlong, [_VP]),
+    "rc_mlp_set_params": (C.c_int, [_VP, _FP]),
+    "rc_mlp_get_params": (C.c_int, [_VP, _FP]),
+    "rc_mlp_forward": (C.c_int, [_VP, _FP, C.c_int, _FP]),
+    "rc_mlp_gradient": (C.c_int, [_VP, _FP, _FP, C.c_int, _FP, _FP]),
+    "rc_mlp_set_data": (C.c_int, [_VP, _FP, _FP, C.c_longlong]),
+    "rc_mlp_train": (C.c_int, [_VP, C.c_int, C.c_int, C.c_float, C.c_ulonglong, C.c_int, _FP]),
     "rc_backward_euler_lusgs": (C.c_int, [_VP, C.c_double, C.c_int]),
     "rc_backward_euler_stage": (C.c_int, [_VP, C.c_double, C.c_int, C.c_double, _IP, _DP]),
     "rc_get_primitive": (C.c_int, [_VP, _DP]),

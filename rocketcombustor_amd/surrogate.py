@@ -1,0 +1,173 @@
+"""Learned chemistry (no reference counterpart): an FP32 MLP for the species
+source term, trained on the GPU by the library's own kernels
+(``rc_mlp_*``), and its use in the steady step with a per-cell gate and the
+exact FP64 kinetics as fallback (``rc_set_surrogate``).
+
+Network: (T, p, Y_1..Y_Nsp) -> omega_1..omega_{Nsp-1}, three tanh hidden
+layers of 128 (the configuration of the learned-chemistry workload).
+Inputs are standardised (log p for the pressure), outputs divided by a
+per-species scale; both transforms travel with the network.
+
+Training data are states the kinetics itself produces: snapshots of the
+primitive field of steady runs with the production rates the device forms
+for them (``SteadyCase.derived``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+from typing import List, Optional
+
+import numpy as np
+
+from . import native
+from .native import NVAR, check, lib
+
+N_IN = 6           # T, p, Y_1..Y_4
+N_OUT = 3          # omega_1..omega_3
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class MLP:
+    """Device-resident network (``rc_mlp``)."""
+
+    def __init__(self, n_in: int = N_IN, width: int = 128, hidden: int = 3, n_out: int = N_OUT, device: int = 0,
+                 seed: int = 5):
+        h = C.c_void_p()
+        rc = lib().rc_mlp_create(int(device), n_in, width, hidden, n_out, C.byref(h))
+        if rc:
+            raise native.RcError(rc, lib().rc_mlp_last_error(None).decode())
+        self.h = h
+        self.n_in, self.n_out, self.width, self.hidden = n_in, n_out, width, hidden
+        dims = [n_in] + [width] * hidden + [n_out]
+        self.shapes = [(dims[k], dims[k + 1]) for k in range(len(dims) - 1)]
+        self.nparam = int(lib().rc_mlp_num_params(h))
+        self.params = glorot(self.shapes, seed)
+
+    def _check(self, rc):
+        if rc:
+            raise native.RcError(rc, lib().rc_mlp_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rc_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.empty(self.nparam, dtype=np.float32)
+        self._check(lib().rc_mlp_get_params(self.h, _fp(p)))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        self._check(lib().rc_mlp_set_params(self.h, _fp(p)))
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty((len(x), self.n_out), dtype=np.float32)
+        self._check(lib().rc_mlp_forward(self.h, _fp(x), len(x), _fp(y)))
+        return y
+
+    def gradient(self, x: np.ndarray, t: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(self.nparam, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self._check(lib().rc_mlp_gradient(self.h, _fp(x), _fp(t), len(x), _fp(g), _fp(loss)))
+        return float(loss[0]), g
+
+    def set_data(self, x: np.ndarray, t: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        self._check(lib().rc_mlp_set_data(self.h, _fp(x), _fp(t), len(x)))
+
+    def train(self, nsteps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 5, every: int = 100) -> np.ndarray:
+        nrep = (nsteps // every + 1) if every > 0 else 0
+        out = np.zeros(max(nrep, 1), dtype=np.float32)
+        self._check(lib().rc_mlp_train(self.h, int(nsteps), int(batch), float(lr), int(seed), int(every), _fp(out)))
+        return out[:nrep]
+
+
+def glorot(shapes, seed: int) -> np.ndarray:
+    """Glorot-uniform weights, zero biases (seeded NumPy generator)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fi, fo in shapes:
+        a = np.sqrt(6.0 / (fi + fo))
+        parts += [rng.uniform(-a, a, size=fi * fo), np.zeros(fo)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+@dataclasses.dataclass
+class Transform:
+    """Input standardisation (log p) and output scales."""
+    mean: np.ndarray          # (N_IN,)
+    std: np.ndarray           # (N_IN,)
+    out_scale: np.ndarray     # (N_OUT,)
+    lo: np.ndarray            # (N_IN,) training range of the raw inputs (T, p, Y)
+    hi: np.ndarray
+
+    @staticmethod
+    def fit(raw: np.ndarray, omega: np.ndarray) -> "Transform":
+        z = features(raw)
+        return Transform(mean=z.mean(axis=0), std=np.maximum(z.std(axis=0), 1e-12),
+                         out_scale=np.maximum(np.abs(omega).max(axis=0), 1e-30),
+                         lo=raw.min(axis=0), hi=raw.max(axis=0))
+
+    def x(self, raw: np.ndarray) -> np.ndarray:
+        return ((features(raw) - self.mean) / self.std).astype(np.float32)
+
+    def y(self, omega: np.ndarray) -> np.ndarray:
+        return (omega / self.out_scale).astype(np.float32)
+
+    def packed(self) -> np.ndarray:
+        """The layout rc_set_surrogate reads: mean, std, out_scale, lo, hi."""
+        return np.concatenate([self.mean, self.std, self.out_scale, self.lo, self.hi]).astype(np.float64)
+
+
+def features(raw: np.ndarray) -> np.ndarray:
+    """(T, p, Y_1..Y_4) -> (T, log p, Y_1..Y_4)."""
+    z = np.array(raw, dtype=np.float64, copy=True)
+    z[:, 1] = np.log(z[:, 1])
+    return z
+
+
+def raw_inputs(pa: np.ndarray) -> np.ndarray:
+    """(T, p, Y_1..Y_4) of primitive states PA(Nvar) (Y_4 = 1 - sum)."""
+    y = pa[:, 4:NVAR]
+    return np.column_stack([pa[:, 3], pa[:, 0], y, 1.0 - y.sum(axis=1)])
+
+
+def sample_states(cases, steps: List[int], per_snapshot: int, seed: int = 5, device: int = 0):
+    """States the kinetics produces: for each case, steady steps on the
+    device with snapshots after each count in ``steps``; each snapshot
+    contributes ``per_snapshot`` random cells (inputs and the device's exact
+    FP64 production rates)."""
+    from .solver import SteadyCase
+    rng = np.random.default_rng(seed)
+    xs, ws = [], []
+    for case in cases:
+        with SteadyCase(case, device=device) as g:
+            done = 0
+            for s in steps:
+                if s > done:
+                    g.run_steady(s - done, history=False)
+                    done = s
+                pa = g.primitive()
+                w = g.derived()["production_rate"]
+                idx = rng.choice(len(pa), size=min(per_snapshot, len(pa)), replace=False)
+                xs.append(raw_inputs(pa[idx]))
+                ws.append(w[idx])
+    return np.concatenate(xs), np.concatenate(ws)

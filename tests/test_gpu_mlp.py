@@ -1,0 +1,52 @@
+"""The device's FP32 MLP kernels (rc_mlp_*) against the float64 reference
+(oracle/mlp_oracle.py): forward and parameter gradient of the mean-squared
+error on seeded inputs (FP32 bar: 1e-4 relative to the largest entry), and
+training: the loss falls, and two runs with one seed are bitwise equal."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+from rocketcombustor_amd.surrogate import MLP
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg: str) -> None:
+    sys.__stderr__.write("[gpu-test] " + msg + "\n")
+    sys.__stderr__.flush()
+
+
+@pytest.mark.parametrize("n", [1, 37, 300])
+def test_forward_and_gradient_match_reference(n):
+    announce("mlp forward/gradient n=%d" % n)
+    m = MLP(seed=7)
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=(n, 6)).astype(np.float32)
+    t = rng.normal(size=(n, 3)).astype(np.float32)
+    p = m.params.astype(np.float64)
+    y_ref, _ = mo.forward(p, m.shapes, x.astype(np.float64))
+    y = m.forward(x)
+    assert np.max(np.abs(y - y_ref)) <= 1e-5 * max(1.0, np.max(np.abs(y_ref)))
+    loss, g = m.gradient(x, t)
+    loss_ref, g_ref = mo.loss_and_grad(p, m.shapes, x.astype(np.float64), t.astype(np.float64))
+    assert abs(loss - loss_ref) <= 1e-5 * loss_ref
+    assert np.max(np.abs(g - g_ref)) <= 1e-4 * np.max(np.abs(g_ref))
+    m.close()
+
+
+def test_training_reduces_loss_and_is_deterministic():
+    announce("mlp training")
+    rng = np.random.default_rng(11)
+    x = rng.uniform(-1, 1, size=(20000, 6)).astype(np.float32)
+    t = np.column_stack([np.sin(2 * x[:, 0]) * x[:, 1], x[:, 2] * x[:, 3], np.tanh(x[:, 4] - x[:, 5])]).astype(np.float32)
+    runs = []
+    for _ in range(2):
+        m = MLP(seed=3)
+        m.set_data(x, t)
+        losses = m.train(600, batch=1024, lr=2e-3, seed=9, every=100)
+        runs.append((losses, m.params))
+        m.close()
+    assert runs[0][0][-1] < 0.1 * runs[0][0][0]
+    assert np.array_equal(runs[0][1], runs[1][1])
