@@ -181,6 +181,21 @@ int  rc_mlp_set_data(rc_mlp *m, const float *x, const float *t, long long n);
  * loss goes to loss_out (may be NULL) */
 int  rc_mlp_train(rc_mlp *m, int nsteps, int batch, float lr, unsigned long long seed, int every, float *loss_out);
 
+/* The species source term of the case's steady step (and of
+ * rc_flux_sourceterm / the backward-Euler stage) from the network m: per
+ * cell the inputs (T, log p, Y_1..Y_Nsp) standardised with xform, the
+ * outputs scaled back, and the result accepted when the raw inputs lie in
+ * the training range and the rates keep the reaction's mass balance,
+ * |sum omega| <= tol (sum |omega| + 1e-3 max scale); otherwise the exact FP64
+ * rate (sourceterm.f90:17-43).  xform = input mean (2+Nsp), input std
+ * (2+Nsp), output scales (Nsp-1), raw input range low and high (2+Nsp each,
+ * on T, p, Y).  m = NULL restores the exact source.  The network stays owned
+ * by the caller and must outlive its use; such a case steps alone (not in a
+ * batch). */
+int rc_set_surrogate(rc_case *h, rc_mlp *m, const double *xform, double tol);
+/* accepted and evaluated cell source evaluations since rc_set_surrogate */
+int rc_get_surrogate_stats(rc_case *h, long long *accepted_evaluated2);
+
 /* ---- dual-time backward Euler with the point-implicit LU-SGS
  * (implicit_time_method = 3; fvm2d.f90:60-110 unsteady branch). */
 

@@ -32,10 +32,16 @@ void launch_be_factor(const LaunchSet &, double *, int *, double *, hipStream_t)
 void launch_be_level(const LaunchSet &, bool, int, const double *, const int *, const double *, hipStream_t);
 size_t be_factor_doubles();
 size_t be_factor_ints();
-size_t be_cache_doubles();
 void launch_be_update(const LaunchSet &, hipStream_t);
 int pi_chunk();
 void launch_viscous(const LaunchSet &, bool, bool, hipStream_t);
+size_t be_cache_doubles();
+struct Surrogate;
+int surrogate_xform_len();
+int surrogate_create(rc_mlp *, int, const double *, double, Surrogate **, std::string &);
+void surrogate_destroy(Surrogate *);
+void surrogate_enqueue(Surrogate *, const CaseDev *, int, hipStream_t);
+int surrogate_accepted(Surrogate *, hipStream_t, long long &);
 }  // namespace rc
 
 using namespace rc;
@@ -59,6 +65,9 @@ struct rc_case {
     double *be_fac = nullptr;      // backward Euler: per-cell block factors (allocated at first use)
     int *be_piv = nullptr;
     double *be_nbc = nullptr;      // backward Euler: per-cell thermo records of the neighbour terms
+    Surrogate *sur = nullptr;      // learned-chemistry source (null: exact kinetics)
+    double *sur_src = nullptr;     // its per-cell rates, (Nsp-1) arrays of sc
+    long long sur_evals = 0;       // cell evaluations since the surrogate was set
     bool viscous = false;
     CaseDev hc{};                  // host copy of the case's descriptor
     CaseDev *d_case = nullptr;
@@ -519,6 +528,7 @@ void enqueue_step_kernels(rc_case *h) {
     mark(h, 1);
     launch_flux(L, true, h->stream);
     if (h->viscous) launch_viscous(L, true, true, h->stream);
+    if (h->sur) surrogate_enqueue(h->sur, h->d_case, h->ne, h->stream);
     mark(h, 2);
     mark(h, 3);                 // the residual is formed in the forward sweep
     for (int l = 0; l < h->nlev; ++l) launch_lusgs_level(L, true, l, true, h->stream);
@@ -564,6 +574,7 @@ int enqueue_stage(rc_case *h, bool allow_graph) {
         enqueue_step_kernels(h);
     }
     h->steps_done++;
+    if (h->sur) h->sur_evals += h->ne;
     return RC_OK;
 }
 
@@ -821,6 +832,8 @@ void rc_case_destroy(rc_case *h) {
     if (h->be_fac) (void)hipFree(h->be_fac);
     if (h->be_piv) (void)hipFree(h->be_piv);
     if (h->be_nbc) (void)hipFree(h->be_nbc);
+    if (h->sur) surrogate_destroy(h->sur);
+    if (h->sur_src) (void)hipFree(h->sur_src);
     for (auto &ev : h->evpool) (void)hipEventDestroy(ev);
     if (h->h_err) (void)hipHostFree(h->h_err);
     if (h->h_res) (void)hipHostFree(h->h_res);
@@ -880,6 +893,10 @@ int rc_flux_sourceterm(rc_case *h) {
     HIPCHK(h, hipSetDevice(h->device));
     launch_flux(h->L, false, h->stream);
     if (h->viscous) launch_viscous(h->L, false, true, h->stream);
+    if (h->sur) {
+        surrogate_enqueue(h->sur, h->d_case, h->ne, h->stream);
+        h->sur_evals += h->ne;
+    }
     launch_residual(h->L, false, h->stream);
     return sync_check(h);
 }
@@ -962,6 +979,7 @@ int rc_set_transport(rc_case *h, const rc_transport *t) {
     HIPCHK(h, hipMemcpyAsync(h->d_case, &h->hc, sizeof h->hc, hipMemcpyHostToDevice, h->stream));
     DevParams dp = make_params(h->par, h->mech.nsp);
     dp.viscous = 1;
+    dp.surrogate = h->sur ? 1 : 0;
     HIPCHK(h, hipMemcpyAsync(h->d_par, &dp, sizeof dp, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->viscous = true;
@@ -973,6 +991,46 @@ int rc_set_transport(rc_case *h, const rc_transport *t) {
         if (h->batch->graph_exec) (void)hipGraphExecDestroy(h->batch->graph_exec);
         h->batch->graph_exec = nullptr;
     }
+    return RC_OK;
+}
+
+int rc_set_surrogate(rc_case *h, rc_mlp *m, const double *xform, double tol) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    if (m && !xform) return set_err(h, RC_ERR_ARGUMENT, "NULL transform");
+    if (h->batch) return set_err(h, RC_ERR_ARGUMENT, "a case with the learned source term steps alone");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->sur) surrogate_destroy(h->sur);
+    h->sur = nullptr;
+    h->sur_evals = 0;
+    if (m) {
+        std::string msg;
+        int rc = surrogate_create(m, h->ne, xform, tol, &h->sur, msg);
+        if (rc) return set_err(h, rc, msg);
+        if (!h->sur_src) HIPCHK(h, hipMalloc(&h->sur_src, sizeof(double) * (kNsp - 1) * h->M.sc));
+    }
+    h->hc.S.src = m ? h->sur_src : nullptr;
+    h->S.src = h->hc.S.src;
+    HIPCHK(h, hipMemcpyAsync(h->d_case, &h->hc, sizeof h->hc, hipMemcpyHostToDevice, h->stream));
+    DevParams dp = make_params(h->par, h->mech.nsp);
+    dp.viscous = h->viscous ? 1 : 0;
+    dp.surrogate = m ? 1 : 0;
+    HIPCHK(h, hipMemcpyAsync(h->d_par, &dp, sizeof dp, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+    return RC_OK;
+}
+
+int rc_get_surrogate_stats(rc_case *h, long long *acc_eval) {
+    if (!h || !acc_eval) return set_err(h, RC_ERR_ARGUMENT, "NULL argument");
+    acc_eval[0] = acc_eval[1] = 0;
+    if (!h->sur) return RC_OK;
+    HIPCHK(h, hipSetDevice(h->device));
+    long long acc = 0;
+    if (surrogate_accepted(h->sur, h->stream, acc)) return set_err(h, RC_ERR_DEVICE, "reading the counters failed");
+    acc_eval[0] = acc;
+    acc_eval[1] = h->sur_evals;
     return RC_OK;
 }
 
@@ -1013,6 +1071,10 @@ int rc_backward_euler_stage(rc_case *h, double dt, int innerloop, double reltol,
         launch_gradlim(L, true, 2, h->stream);
         launch_flux(L, false, h->stream);
         if (h->viscous) launch_viscous(L, false, true, h->stream);
+        if (h->sur) {
+            surrogate_enqueue(h->sur, h->d_case, h->ne, h->stream);
+            h->sur_evals += h->ne;
+        }
         launch_residual(L, false, h->stream);
         enqueue_be_lusgs(h, dt, inner == 1);
         int rc = residual_async(h, h->h_res);
@@ -1161,6 +1223,7 @@ void rc_batch_destroy(rc_batch *b) {
 int rc_batch_add(rc_batch *b, rc_case *h) {
     if (!b || !h) return set_err(nullptr, RC_ERR_ARGUMENT, "NULL argument");
     if (h->batch) return bset_err(b, RC_ERR_ARGUMENT, "the case already belongs to a batch");
+    if (h->sur) return bset_err(b, RC_ERR_ARGUMENT, "a case with the learned source term steps alone");
     if (h->device != b->device) return bset_err(b, RC_ERR_ARGUMENT, "the case is on another device");
     BHIPCHK(b, hipSetDevice(b->device));
     BHIPCHK(b, hipStreamSynchronize(h->stream));

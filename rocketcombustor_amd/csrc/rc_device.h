@@ -68,6 +68,7 @@ struct DevState {
     double *lu_af;   // alpaF of the cell's 3 face slots, 3 x sc
     double *fixed;   // backward Euler: vol/dt*WA at the first inner iteration, Nvar arrays of sc
     double *fvis;    // viscous path, kRecF per face: alpa, Fv(2:Nvar), alpaVis
+    double *src;     // learned-chemistry source: omega_1..omega_{Nsp-1}, arrays of sc (null when exact)
     int *err;        // [0] code, [1] index, [2] kernel, [3] step of the error, [4] current step
 };
 
@@ -117,7 +118,7 @@ struct GMesh {
     const RC_G double *vecx, *fvl, *fvr, *vol, *wdis, *amls, *lvv, *gnrm, *lu_n, *vgeo;
 };
 struct GState {
-    RC_G double *pa, *wa, *hwa, *gradc, *gpa, *step, *dd, *fflux, *lucoef, *lu_af, *fixed, *fvis;
+    RC_G double *pa, *wa, *hwa, *gradc, *gpa, *step, *dd, *fflux, *lucoef, *lu_af, *fixed, *fvis, *src;
     RC_G int *err;
 };
 typedef const RC_G DevThermo GThermo;
@@ -145,6 +146,7 @@ __device__ __forceinline__ GState gview(const RC_G DevState &s) {
     g.step = gp(s.step); g.dd = gp(s.dd); g.fflux = gp(s.fflux); g.lucoef = gp(s.lucoef); g.lu_af = gp(s.lu_af);
     g.fixed = gp(s.fixed);
     g.fvis = gp(s.fvis);
+    g.src = gp(s.src);
     g.err = gp(s.err);
     return g;
 }

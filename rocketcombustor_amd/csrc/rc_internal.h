@@ -98,6 +98,7 @@ struct DevParams {
     int limit_id;
     int combustion;
     int viscous;                   // IfViscous
+    int surrogate;                 // source term from the learned-chemistry array S.src
 };
 
 }  // namespace rc

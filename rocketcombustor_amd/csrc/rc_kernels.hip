@@ -370,7 +370,15 @@ __device__ __forceinline__ void cell_residual(const GMesh &M, const GState &S, T
             }
         }
     }
-    if (P.combustion) {
+    if (P.combustion && P.surrogate) {
+        // the learned-chemistry source of the cell (accepted surrogate or
+        // its exact fallback), formed before the sweep
+        const double v = M.vol[i];
+#pragma unroll
+        for (int k = 0; k < kNFlow; ++k) h[k] = h[k] + v * 0.0;
+#pragma unroll
+        for (int k = 0; k < NSP - 1; ++k) h[kNFlow + k] = h[kNFlow + k] + v * S.src[k * M.sc + i];
+    } else if (P.combustion) {
         const double t = PA_(3, i);
         double y[NSP], s = 0.0;
 #pragma unroll

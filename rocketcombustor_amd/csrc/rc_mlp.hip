@@ -175,6 +175,91 @@ __global__ void __launch_bounds__(256) k_gather_batch(const float *__restrict__ 
     for (int k = 0; k < nout; ++k) yb[(size_t)r * nout + k] = Y[(size_t)src * nout + k];
 }
 
+// ---------------------------------------------------------------------------
+// the learned source term of a case: inputs of cells [c0, c0+n) from PA,
+// the network, then per cell the gate and the exact FP64 fallback
+// ---------------------------------------------------------------------------
+// xf: input mean (6), input std (6), output scales (Nsp-1), raw input range
+// lo (6), hi (6) -- inputs (T, log p, Y_1..Y_Nsp), raw range on (T, p, Y)
+constexpr int kSurIn = 2 + kNsp;
+constexpr int kXfMean = 0, kXfStd = kSurIn, kXfScale = 2 * kSurIn, kXfLo = kXfScale + kNsp - 1,
+              kXfHi = kXfLo + kSurIn, kXfLen = kXfHi + kSurIn;
+
+template <int NSP>
+__global__ void __launch_bounds__(256) k_sur_inputs(const CaseDev *__restrict__ cases, int c0, int n,
+                                                    const double *__restrict__ xf, float *__restrict__ x) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    const int i = c0 + r;
+    const RC_G CaseDev &C = *gp(cases);
+    const GState S = gview(C.S);
+    double raw[kSurIn];
+    raw[0] = PA_(3, i);
+    raw[1] = log(PA_(0, i));
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) { raw[2 + k] = PA_(kNFlow + k, i); s = s + raw[2 + k]; }
+    raw[1 + NSP] = 1.0 - s;
+#pragma unroll
+    for (int k = 0; k < kSurIn; ++k) x[(size_t)r * kSurIn + k] = (float)((raw[k] - xf[kXfMean + k]) / xf[kXfStd + k]);
+}
+
+// accept the network's source for a cell when its raw inputs lie in the
+// training range and its rates keep the reaction's mass balance (the
+// species rates of one reaction sum to zero): |sum w| <= tol (sum |w| +
+// 1e-3 max scale); else the exact FP64 rate of the source term.  Accepted
+// cells are counted per block of 256 cells (acc, cumulative).
+template <int NSP>
+__global__ void __launch_bounds__(256) k_sur_gate(const CaseDev *__restrict__ cases, int c0, int n,
+                                                  const double *__restrict__ xf, const float *__restrict__ y,
+                                                  double tol, long long *__restrict__ acc) {
+    __shared__ int cnt[256];
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    int ok = 0;
+    if (r < n) {
+        const int i = c0 + r;
+        const RC_G CaseDev &C = *gp(cases);
+        const GMesh M = gview(C.M);
+        const GState S = gview(C.S);
+        GThermo &th = *gp(C.th);
+        double raw[kSurIn];
+        raw[0] = PA_(3, i);
+        raw[1] = PA_(0, i);
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NSP - 1; ++k) { raw[2 + k] = PA_(kNFlow + k, i); s = s + raw[2 + k]; }
+        raw[1 + NSP] = 1.0 - s;
+        bool inside = true;
+#pragma unroll
+        for (int k = 0; k < kSurIn; ++k) inside = inside && raw[k] >= xf[kXfLo + k] && raw[k] <= xf[kXfHi + k];
+        double w[NSP], wsum = 0.0, wabs = 0.0, smax = 0.0;
+#pragma unroll
+        for (int k = 0; k < NSP - 1; ++k) {
+            w[k] = (double)y[(size_t)r * (NSP - 1) + k] * xf[kXfScale + k];
+            wsum += w[k];
+            wabs += fabs(w[k]);
+            smax = fmax(smax, xf[kXfScale + k]);
+        }
+        ok = (inside && fabs(wsum) <= tol * (wabs + 1e-3 * smax)) ? 1 : 0;
+        if (!ok) {
+            double yy[NSP];
+#pragma unroll
+            for (int k = 0; k < NSP - 1; ++k) yy[k] = raw[2 + k];
+            yy[NSP - 1] = raw[1 + NSP];
+            production_rate<NSP>(th, raw[0], LC_(LC_RR, i), yy, w);
+        }
+#pragma unroll
+        for (int k = 0; k < NSP - 1; ++k) S.src[k * M.sc + i] = w[k];
+    }
+    cnt[threadIdx.x] = ok;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) cnt[threadIdx.x] += cnt[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[c0 / 256 + blockIdx.x] += cnt[0];
+}
+
 }  // namespace rc
 
 using namespace rc;
@@ -410,3 +495,92 @@ int rc_mlp_train(rc_mlp *m, int nsteps, int batch, float lr, unsigned long long 
 }
 
 }  // extern "C"
+
+namespace rc {
+
+// per-case learned-source state: the network (not owned), the transform,
+// the chunk buffers of the forward pass and the acceptance counters
+struct Surrogate {
+    rc_mlp *m = nullptr;
+    double tol = 0.0;
+    double *xf = nullptr;        // device, kXfLen
+    float *buf[8] = {};          // activations of one chunk: dims[0..nlayer]
+    int chunk = 0;
+    long long *acc = nullptr;    // per block of 256 cells
+    int nacc = 0;
+};
+
+int surrogate_xform_len() { return kXfLen; }
+
+// the network's dims must be (2+Nsp) -> ... -> Nsp-1
+int surrogate_create(rc_mlp *m, int ne, const double *xform, double tol, Surrogate **out, std::string &msg) {
+    if (m->nin != kSurIn || m->nout != kNsp - 1) {
+        msg = "the network must map " + std::to_string(kSurIn) + " inputs to " + std::to_string(kNsp - 1) + " rates";
+        return RC_ERR_ARGUMENT;
+    }
+    if (m->nlayer + 1 > 8) {
+        msg = "at most 6 hidden layers";
+        return RC_ERR_ARGUMENT;
+    }
+    Surrogate *s = new Surrogate();
+    s->m = m;
+    s->tol = tol;
+    s->chunk = std::min(((ne + 255) / 256) * 256, 131072);
+    s->nacc = (ne + 255) / 256;
+    bool bad = hipMalloc(&s->xf, sizeof(double) * kXfLen) != hipSuccess ||
+               hipMalloc(&s->acc, sizeof(long long) * s->nacc) != hipSuccess;
+    for (int l = 0; l <= m->nlayer && !bad; ++l)
+        bad = hipMalloc(&s->buf[l], sizeof(float) * (size_t)s->chunk * m->dims[l]) != hipSuccess;
+    if (!bad)
+        bad = hipMemcpy(s->xf, xform, sizeof(double) * kXfLen, hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemset(s->acc, 0, sizeof(long long) * s->nacc) != hipSuccess;
+    if (bad) {
+        msg = "device allocation for the learned source failed";
+        for (float *p : s->buf) if (p) (void)hipFree(p);
+        if (s->xf) (void)hipFree(s->xf);
+        if (s->acc) (void)hipFree(s->acc);
+        delete s;
+        return RC_ERR_MEMORY;
+    }
+    *out = s;
+    return RC_OK;
+}
+
+void surrogate_destroy(Surrogate *s) {
+    if (!s) return;
+    for (float *p : s->buf) if (p) (void)hipFree(p);
+    if (s->xf) (void)hipFree(s->xf);
+    if (s->acc) (void)hipFree(s->acc);
+    delete s;
+}
+
+// the learned source of every cell of the case (before the forward sweep)
+void surrogate_enqueue(Surrogate *s, const CaseDev *dcase, int ne, hipStream_t st) {
+    rc_mlp *m = s->m;
+    for (int c0 = 0; c0 < ne; c0 += s->chunk) {
+        const int n = std::min(s->chunk, ne - c0);
+        k_sur_inputs<kNsp><<<(n + 255) / 256, 256, 0, st>>>(dcase, c0, n, s->xf, s->buf[0]);
+        for (int l = 0; l < m->nlayer; ++l) {
+            const float *W = m->params + m->woff[l], *b = m->params + m->boff[l];
+            const int fi = m->dims[l], fo = m->dims[l + 1];
+            if (l < m->nlayer - 1)
+                gemm<false, false, EPI_BIAS_TANH>(st, s->buf[l], W, s->buf[l + 1], n, fo, fi, fi, fo, fo, b);
+            else
+                gemm<false, false, EPI_BIAS>(st, s->buf[l], W, s->buf[l + 1], n, fo, fi, fi, fo, fo, b);
+        }
+        k_sur_gate<kNsp><<<(n + 255) / 256, 256, 0, st>>>(dcase, c0, n, s->xf, s->buf[m->nlayer], s->tol, s->acc);
+    }
+}
+
+// accepted cell evaluations since the surrogate was set
+int surrogate_accepted(Surrogate *s, hipStream_t st, long long &acc) {
+    std::vector<long long> h(s->nacc);
+    if (hipMemcpyAsync(h.data(), s->acc, sizeof(long long) * s->nacc, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return RC_ERR_DEVICE;
+    acc = 0;
+    for (long long v : h) acc += v;
+    return RC_OK;
+}
+
+}  // namespace rc

@@ -177,6 +177,25 @@ class SteadyCase:
         self._tr = native.make_transport(transport)
         check(lib().rc_set_transport(self.h, C.byref(self._tr)), self.h)
 
+    # -- learned chemistry (no reference counterpart) ----------------------
+    def set_surrogate(self, mlp, transform=None, tol: float = 0.05) -> None:
+        """The source term from the network ``mlp`` (surrogate.MLP) with the
+        per-cell gate (training range, mass balance within ``tol``) and the
+        exact FP64 rate as fallback; ``mlp=None`` restores the exact source."""
+        if mlp is None:
+            check(lib().rc_set_surrogate(self.h, None, None, 0.0), self.h)
+            self._sur = None
+            return
+        xf = np.ascontiguousarray(transform.packed(), dtype=np.float64)
+        check(lib().rc_set_surrogate(self.h, mlp.h, dptr(xf), float(tol)), self.h)
+        self._sur = (mlp, xf)
+
+    def surrogate_stats(self):
+        """(accepted, evaluated) cell source evaluations since set_surrogate."""
+        out = (C.c_longlong * 2)()
+        check(lib().rc_get_surrogate_stats(self.h, out), self.h)
+        return int(out[0]), int(out[1])
+
     # -- dual-time backward Euler (implicit_time_method = 3) ----------------
     def set_outlet_pressure(self, p_out: float) -> None:
         self._call("rc_set_outlet_pressure", float(p_out))
