@@ -174,6 +174,91 @@ def batched_run(n: int, steps: int) -> dict:
             "speedup": t_plain / t_batch}
 
 
+def learned_run(train_steps: int, batch: int = 4096, lr: float = 2e-3, nsteps: int = 50, tol: float = 0.05,
+                rtol: float = 1e-3) -> dict:
+    """The learned-chemistry workload: ~10M states of the 12k and 400k
+    steady runs (snapshots with the device's exact FP64 rates), the MLP
+    (6 -> 3 x 128 tanh -> 3, FP32) trained on the device, its error on
+    held-out states, then the 1.6M-triangle nozzle stepped with exact
+    kinetics, with the surrogate behind the plausibility gate (training
+    range, mass balance) and behind the verified gate (also within rtol of
+    the exact FP64 rates)."""
+    import numpy as np
+    from rocketcombustor_amd import native, nozzle, surrogate
+    from rocketcombustor_amd.solver import SteadyCase
+
+    def ordered(nx, ny):
+        c = nozzle.nozzle_case(nx, ny)
+        perm, _ = native.order_multicolor(c.mesh.cells)
+        return c.permuted(perm)
+
+    def rel_err(w, we, tf):
+        return np.abs(w - we) / np.maximum(np.abs(we), 1e-3 * tf.out_scale)
+
+    t0 = time.perf_counter()
+    snaps = [1] + list(range(50, 1201, 50))
+    raw, omega = surrogate.sample_states([ordered(150, 40), ordered(1000, 200)], steps=snaps,
+                                         per_snapshot=400000, seed=5)
+    t_sample = time.perf_counter() - t0
+    rng = np.random.default_rng(5)
+    hold = rng.choice(len(raw), size=100000, replace=False)
+    keep = np.ones(len(raw), dtype=bool)
+    keep[hold] = False
+    tf = surrogate.Transform.fit(raw[keep], omega[keep], knee=1e-3)
+    m = surrogate.MLP(seed=5)
+    m.set_data(tf.x(raw[keep]), tf.y(omega[keep]))
+    t1 = time.perf_counter()
+    losses = surrogate.train_schedule(m, train_steps, batch=batch, lr=lr, seed=5)
+    t_train = time.perf_counter() - t1
+    flops_step = 3 * 2 * batch * sum(fi * fo for fi, fo in m.shapes)   # forward + 2x backward
+    w_hold = tf.omega(m.forward(tf.x(raw[hold])))
+    rel = rel_err(w_hold, omega[hold], tf)
+    out = {"states": int(len(raw)), "sample_s": t_sample,
+           "network": "6 -> 3 x 128 tanh -> 3, FP32, outputs omega_k = 1e-3 s_k sinh(y_k)",
+           "train": {"rows": int(keep.sum()), "steps": train_steps, "batch": batch, "lr": lr,
+                     "schedule": "Adam, 4 equal phases, lr x0.3 per phase", "loss_per_phase": losses.tolist(),
+                     "seconds": t_train, "tflops": flops_step * train_steps / t_train / 1e12},
+           "holdout": {"rows": int(len(hold)), "rel_err_median": float(np.median(rel)),
+                       "rel_err_p90": float(np.percentile(rel, 90)), "rel_err_p99": float(np.percentile(rel, 99)),
+                       "rel_err_definition": "|w - w_exact| / max(|w_exact|, 1e-3 s_k), per species",
+                       "pass_rtol_1e-3": float(surrogate.gate(tf, raw[hold], w_hold, omega[hold], 1e9, 1e-3).mean())}}
+    big_case = ordered(2000, 400)
+    res, fields = {}, {}
+    for mode in ("exact", "plausibility_gate", "verified_gate"):
+        with SteadyCase(big_case, device=0) as g:
+            if mode != "exact":
+                g.set_surrogate(m, tf, tol=tol, rtol=rtol if mode == "verified_gate" else 0.0)
+            g.run_steady(5, history=False)
+            t2 = time.perf_counter()
+            g.run_steady(nsteps, history=False)
+            wall = time.perf_counter() - t2
+            res[mode] = {"ms_per_step": 1000.0 * wall / nsteps, "cell_updates_per_s": g.ne * nsteps / wall}
+            pa = g.primitive()
+            fields[mode] = pa
+            if mode != "exact":
+                acc, ev = g.surrogate_stats()
+                res[mode]["fallback_fraction"] = 1.0 - acc / max(ev, 1)
+                # the final state: the gate restated on the host splits the
+                # cells; the accepted cells' rates against the exact ones
+                w_exact = g.derived()["production_rate"]
+                r = surrogate.raw_inputs(pa)
+                w_net = tf.omega(m.forward(tf.x(r)))
+                ok = surrogate.gate(tf, r, w_net, w_exact, tol, rtol if mode == "verified_gate" else 0.0)
+                e = rel_err(w_net, w_exact, tf)
+                res[mode]["final_state"] = {
+                    "accepted_fraction": float(ok.mean()),
+                    "accepted_rel_err_median": float(np.median(e[ok])) if ok.any() else None,
+                    "accepted_rel_err_max": float(e[ok].max()) if ok.any() else None}
+                dT = np.abs(pa[:, 3] - fields["exact"][:, 3]) / fields["exact"][:, 3]
+                res[mode]["T_rel_diff_vs_exact_after_%d_steps" % (nsteps + 5)] = float(dT.max())
+                res[mode]["network_ns_per_cell"] = (res[mode]["ms_per_step"] - res["exact"]["ms_per_step"]) * 1e6 / g.ne
+            res["cells"] = g.ne
+    out["nozzle_1p6M"] = res
+    out["gate"] = {"tol": tol, "rtol": rtol}
+    m.close()
+    return out
+
+
 def viscous_run(case, nsteps: int, cpu: bool = True) -> dict:
     """Steady steps of the same case on the viscous path (IfViscous: the
     viscous spectral radius in TimeStep, flux_viscous in flux_sourceterm),
@@ -297,6 +382,11 @@ def main() -> int:
     ap.add_argument("--viscous", type=int, default=0, metavar="K",
                     help="also time K steady steps of the same case on the viscous path (IfViscous, the "
                          "shipped transport tables)")
+    ap.add_argument("--learned", action="store_true",
+                    help="also run the learned-chemistry workload: sample states from the 12k and 400k cases, "
+                         "train the source-term MLP on the device, then step the 1.6M-triangle nozzle with the "
+                         "gated surrogate and with exact kinetics")
+    ap.add_argument("--learned-steps", type=int, default=60000, help="training steps of --learned")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device three times (a plain copy kernel of "
                          "known size, to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -376,6 +466,8 @@ def main() -> int:
         out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
     if args.viscous > 0:
         out["viscous"] = viscous_run(case, args.viscous, cpu=not args.no_cpu_baseline)
+    if args.learned:
+        out["learned"] = learned_run(args.learned_steps)
     if args.backward_euler > 0:
         out["backward_euler"] = backward_euler_run(case, args.backward_euler, cpu=not args.no_cpu_baseline)
     if args.serve > 0:

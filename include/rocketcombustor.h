@@ -184,15 +184,17 @@ int  rc_mlp_train(rc_mlp *m, int nsteps, int batch, float lr, unsigned long long
 /* The species source term of the case's steady step (and of
  * rc_flux_sourceterm / the backward-Euler stage) from the network m: per
  * cell the inputs (T, log p, Y_1..Y_Nsp) standardised with xform, the
- * outputs scaled back, and the result accepted when the raw inputs lie in
- * the training range and the rates keep the reaction's mass balance,
- * |sum omega| <= tol (sum |omega| + 1e-3 max scale); otherwise the exact FP64
- * rate (sourceterm.f90:17-43).  xform = input mean (2+Nsp), input std
- * (2+Nsp), output scales (Nsp-1), raw input range low and high (2+Nsp each,
- * on T, p, Y).  m = NULL restores the exact source.  The network stays owned
- * by the caller and must outlive its use; such a case steps alone (not in a
- * batch). */
-int rc_set_surrogate(rc_case *h, rc_mlp *m, const double *xform, double tol);
+ * outputs mapped back by omega_k = kappa s_k sinh(y_k), and the result
+ * accepted when the raw inputs lie in the training range and the rates keep
+ * the reaction's mass balance, |sum omega| <= tol (sum |omega| + 1e-3 max
+ * s_k); with rtol > 0 the rates must also lie within rtol of the exact FP64
+ * rates, |omega_k - exact_k| <= rtol max(|exact_k|, 1e-3 s_k).  A rejected
+ * cell takes the exact FP64 rate (sourceterm.f90:17-43).  xform = input mean
+ * (2+Nsp), input std (2+Nsp), output scales s_k (Nsp-1), kappa (1), raw input
+ * range low and high (2+Nsp each, on T, p, Y).  m = NULL restores the exact
+ * source.  The network stays owned by the caller and must outlive its use;
+ * such a case steps alone (not in a batch).  No reference counterpart. */
+int rc_set_surrogate(rc_case *h, rc_mlp *m, const double *xform, double tol, double rtol);
 /* accepted and evaluated cell source evaluations since rc_set_surrogate */
 int rc_get_surrogate_stats(rc_case *h, long long *accepted_evaluated2);
 

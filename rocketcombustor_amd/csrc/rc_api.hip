@@ -38,7 +38,7 @@ void launch_viscous(const LaunchSet &, bool, bool, hipStream_t);
 size_t be_cache_doubles();
 struct Surrogate;
 int surrogate_xform_len();
-int surrogate_create(rc_mlp *, int, const double *, double, Surrogate **, std::string &);
+int surrogate_create(rc_mlp *, int, const double *, double, double, Surrogate **, std::string &);
 void surrogate_destroy(Surrogate *);
 void surrogate_enqueue(Surrogate *, const CaseDev *, int, hipStream_t);
 int surrogate_accepted(Surrogate *, hipStream_t, long long &);
@@ -994,7 +994,7 @@ int rc_set_transport(rc_case *h, const rc_transport *t) {
     return RC_OK;
 }
 
-int rc_set_surrogate(rc_case *h, rc_mlp *m, const double *xform, double tol) {
+int rc_set_surrogate(rc_case *h, rc_mlp *m, const double *xform, double tol, double rtol) {
     if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
     if (m && !xform) return set_err(h, RC_ERR_ARGUMENT, "NULL transform");
     if (h->batch) return set_err(h, RC_ERR_ARGUMENT, "a case with the learned source term steps alone");
@@ -1005,7 +1005,7 @@ int rc_set_surrogate(rc_case *h, rc_mlp *m, const double *xform, double tol) {
     h->sur_evals = 0;
     if (m) {
         std::string msg;
-        int rc = surrogate_create(m, h->ne, xform, tol, &h->sur, msg);
+        int rc = surrogate_create(m, h->ne, xform, tol, rtol, &h->sur, msg);
         if (rc) return set_err(h, rc, msg);
         if (!h->sur_src) HIPCHK(h, hipMalloc(&h->sur_src, sizeof(double) * (kNsp - 1) * h->M.sc));
     }

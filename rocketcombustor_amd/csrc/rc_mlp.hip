@@ -6,9 +6,10 @@
 //
 // All dense products go through one LDS-tiled FP32 GEMM (64x64 tiles, 256
 // threads, 4x4 outputs per thread) with the layer's epilogue fused: bias +
-// tanh (hidden), bias (output), or the tanh derivative (backward).  Column
-// sums (bias gradients) and all other reductions are block reductions in a
-// fixed order, so training is deterministic run to run.
+// tanh (hidden), bias (output), or the tanh derivative (backward).  The
+// weight and bias gradients are split over the batch into fixed slices whose
+// partials are added in slice order; all other reductions are block
+// reductions in a fixed order, so training is deterministic run to run.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -99,19 +100,87 @@ void gemm(hipStream_t st, const float *A, const float *B, float *C, int M, int N
     k_gemm<TA, TB, EPI><<<grid, 256, 0, st>>>(A, B, C, M, N, K, lda, ldb, ldc, bias, aux);
 }
 
-// out[n] = sum_m X[m*ld + n] over m < M: one block per column, fixed order
-__global__ void __launch_bounds__(256) k_colsum(const float *__restrict__ X, float *__restrict__ out, int M, int ld) {
-    __shared__ float sh[256];
-    const int n = blockIdx.x;
-    float s = 0.f;
-    for (int m = threadIdx.x; m < M; m += 256) s += X[(size_t)m * ld + n];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+// Weight and bias gradient of one layer, split over the batch:
+// P[s][m][n] = sum over rows k of slice s of A[k*lda + m] B[k*ldb + n]
+// (m < M), and row M of P[s] = sum over the slice of B[k*ldb + n] (the
+// bias gradient).  Slices are `slice` rows (a multiple of kTk); blocks of
+// grid row 0 form the bias row.  k_splitk_sum adds the slices in order.
+__global__ void __launch_bounds__(256) k_gemm_at_splitk(const float *__restrict__ A, const float *__restrict__ B,
+                                                        float *__restrict__ P, int M, int N, int K, int lda,
+                                                        int ldb, int slice) {
+    __shared__ float As[kTk][kTile + 1];
+    __shared__ float Bs[kTk][kTile + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+    const int kb = blockIdx.z * slice, ke = min(K, kb + slice);
+    const bool bias_row = blockIdx.y == 0 && threadIdx.x < kTile;
+    float acc[4][4], cs = 0.f;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.f;
+    for (int k0 = kb; k0 < ke; k0 += kTk) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int e = threadIdx.x + r * 256;
+            const int mm = e % kTile, kk = e / kTile;
+            const int m = m0 + mm, k = k0 + kk, n = n0 + mm;
+            As[kk][mm] = (m < M && k < ke) ? A[(size_t)k * lda + m] : 0.f;
+            Bs[kk][mm] = (n < N && k < ke) ? B[(size_t)k * ldb + n] : 0.f;
+        }
+        __syncthreads();
+        if (bias_row) {
+#pragma unroll
+            for (int kk = 0; kk < kTk; ++kk) cs += Bs[kk][threadIdx.x];
+        }
+#pragma unroll
+        for (int kk = 0; kk < kTk; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { a[q] = As[kk][ty + 16 * q]; b[q] = Bs[kk][tx + 16 * q]; }
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[p][q] = fmaf(a[p], b[q], acc[p][q]);
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[n] = sh[0];
+    float *Ps = P + (size_t)blockIdx.z * (M + 1) * N;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int m = m0 + ty + 16 * p;
+        if (m >= M) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int n = n0 + tx + 16 * q;
+            if (n < N) Ps[(size_t)m * N + n] = acc[p][q];
+        }
+    }
+    if (bias_row && n0 + (int)threadIdx.x < N) Ps[(size_t)M * N + n0 + threadIdx.x] = cs;
+}
+
+// out[e] = sum over s < ns of P[s*len + e], in slice order
+__global__ void __launch_bounds__(256) k_splitk_sum(const float *__restrict__ P, float *__restrict__ out, int len,
+                                                    int ns) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= len) return;
+    float s = 0.f;
+    for (int k = 0; k < ns; ++k) s += P[(size_t)k * len + e];
+    out[e] = s;
+}
+
+// slices of the batch for a layer gradient: about 512 blocks in all, slices
+// of at least 64 rows
+inline int splitk_slices(int M, int N, int K, int &slice) {
+    const int tiles = ((M + kTile - 1) / kTile) * ((N + kTile - 1) / kTile);
+    int ns = std::max(1, std::min(512 / tiles, (K + 63) / 64));
+    slice = (((K + ns - 1) / ns) + kTk - 1) / kTk * kTk;
+    ns = (K + slice - 1) / slice;
+    return ns;
+}
+inline int splitk_max_slices(int M, int N) {
+    const int tiles = ((M + kTile - 1) / kTile) * ((N + kTile - 1) / kTile);
+    return std::max(1, 512 / tiles);
 }
 
 // G = 2 (Y - T) / (M*N), and per-block partial sums of (Y - T)^2
@@ -179,11 +248,14 @@ __global__ void __launch_bounds__(256) k_gather_batch(const float *__restrict__ 
 // the learned source term of a case: inputs of cells [c0, c0+n) from PA,
 // the network, then per cell the gate and the exact FP64 fallback
 // ---------------------------------------------------------------------------
-// xf: input mean (6), input std (6), output scales (Nsp-1), raw input range
-// lo (6), hi (6) -- inputs (T, log p, Y_1..Y_Nsp), raw range on (T, p, Y)
+// xf: input mean (6), input std (6), output scales s_k (Nsp-1), the knee
+// kappa of the output transform, raw input range lo (6), hi (6) -- inputs
+// (T, log p, Y_1..Y_Nsp), raw range on (T, p, Y).  The network's output y_k
+// stands for omega_k = kappa s_k sinh(y_k): linear below kappa s_k,
+// logarithmic above, so a squared-error fit weighs relative errors.
 constexpr int kSurIn = 2 + kNsp;
-constexpr int kXfMean = 0, kXfStd = kSurIn, kXfScale = 2 * kSurIn, kXfLo = kXfScale + kNsp - 1,
-              kXfHi = kXfLo + kSurIn, kXfLen = kXfHi + kSurIn;
+constexpr int kXfMean = 0, kXfStd = kSurIn, kXfScale = 2 * kSurIn, kXfKnee = kXfScale + kNsp - 1,
+              kXfLo = kXfKnee + 1, kXfHi = kXfLo + kSurIn, kXfLen = kXfHi + kSurIn;
 
 template <int NSP>
 __global__ void __launch_bounds__(256) k_sur_inputs(const CaseDev *__restrict__ cases, int c0, int n,
@@ -207,12 +279,14 @@ __global__ void __launch_bounds__(256) k_sur_inputs(const CaseDev *__restrict__ 
 // accept the network's source for a cell when its raw inputs lie in the
 // training range and its rates keep the reaction's mass balance (the
 // species rates of one reaction sum to zero): |sum w| <= tol (sum |w| +
-// 1e-3 max scale); else the exact FP64 rate of the source term.  Accepted
+// 1e-3 max scale); with rtol > 0 the rates must also lie within rtol of the
+// exact FP64 rates, |w_k - w_k,exact| <= rtol max(|w_k,exact|, 1e-3 s_k).
+// A rejected cell takes the exact FP64 rate of the source term.  Accepted
 // cells are counted per block of 256 cells (acc, cumulative).
 template <int NSP>
 __global__ void __launch_bounds__(256) k_sur_gate(const CaseDev *__restrict__ cases, int c0, int n,
                                                   const double *__restrict__ xf, const float *__restrict__ y,
-                                                  double tol, long long *__restrict__ acc) {
+                                                  double tol, double rtol, long long *__restrict__ acc) {
     __shared__ int cnt[256];
     const int r = blockIdx.x * 256 + threadIdx.x;
     int ok = 0;
@@ -233,20 +307,30 @@ __global__ void __launch_bounds__(256) k_sur_gate(const CaseDev *__restrict__ ca
 #pragma unroll
         for (int k = 0; k < kSurIn; ++k) inside = inside && raw[k] >= xf[kXfLo + k] && raw[k] <= xf[kXfHi + k];
         double w[NSP], wsum = 0.0, wabs = 0.0, smax = 0.0;
+        const double knee = xf[kXfKnee];
 #pragma unroll
         for (int k = 0; k < NSP - 1; ++k) {
-            w[k] = (double)y[(size_t)r * (NSP - 1) + k] * xf[kXfScale + k];
+            w[k] = knee * xf[kXfScale + k] * sinh((double)y[(size_t)r * (NSP - 1) + k]);
             wsum += w[k];
             wabs += fabs(w[k]);
             smax = fmax(smax, xf[kXfScale + k]);
         }
         ok = (inside && fabs(wsum) <= tol * (wabs + 1e-3 * smax)) ? 1 : 0;
-        if (!ok) {
-            double yy[NSP];
+        if (!ok || rtol > 0.0) {
+            double yy[NSP], we[NSP];
 #pragma unroll
             for (int k = 0; k < NSP - 1; ++k) yy[k] = raw[2 + k];
             yy[NSP - 1] = raw[1 + NSP];
-            production_rate<NSP>(th, raw[0], LC_(LC_RR, i), yy, w);
+            production_rate<NSP>(th, raw[0], LC_(LC_RR, i), yy, we);
+            if (ok) {
+#pragma unroll
+                for (int k = 0; k < NSP - 1; ++k)
+                    ok = ok && fabs(w[k] - we[k]) <= rtol * fmax(fabs(we[k]), 1e-3 * xf[kXfScale + k]);
+            }
+            if (!ok) {
+#pragma unroll
+                for (int k = 0; k < NSP - 1; ++k) w[k] = we[k];
+            }
         }
 #pragma unroll
         for (int k = 0; k < NSP - 1; ++k) S.src[k * M.sc + i] = w[k];
@@ -280,6 +364,7 @@ struct rc_mlp {
     std::vector<float *> act;                        // act[0] = inputs, act[l] = layer l outputs (cap rows)
     std::vector<float *> dz;                         // gradients of the layer pre-activations (cap rows)
     float *ybuf = nullptr, *partial = nullptr, *loss = nullptr;
+    float *splitk = nullptr;                         // slice partials of the layer gradients
     std::string err;
 };
 
@@ -335,10 +420,15 @@ void backward(rc_mlp *m, int n) {
     k_sum_partials<<<1, 64, 0, m->stream>>>(m->partial, nb, 1.f / (float)count, m->loss);
     for (int l = L - 1; l >= 0; --l) {
         const int fi = m->dims[l], fo = m->dims[l + 1];
-        // dW_l = act_l^T dz_{l+1}; db_l = column sums of dz_{l+1}
-        gemm<true, false, EPI_NONE>(m->stream, m->act[l], m->dz[l + 1], m->grads + m->woff[l], fi, fo, n, fi, fo,
-                                    fo);
-        k_colsum<<<fo, 256, 0, m->stream>>>(m->dz[l + 1], m->grads + m->boff[l], n, fo);
+        // dW_l = act_l^T dz_{l+1}, db_l = column sums of dz_{l+1}: b_l
+        // follows W_l in the parameter vector, so both are one (fi+1) x fo
+        // block, formed in batch slices and summed in slice order
+        int slice = 0;
+        const int ns = splitk_slices(fi, fo, n, slice);
+        dim3 grid((fo + kTile - 1) / kTile, (fi + kTile - 1) / kTile, ns);
+        k_gemm_at_splitk<<<grid, 256, 0, m->stream>>>(m->act[l], m->dz[l + 1], m->splitk, fi, fo, n, fi, fo, slice);
+        const int len = (fi + 1) * fo;
+        k_splitk_sum<<<(len + 255) / 256, 256, 0, m->stream>>>(m->splitk, m->grads + m->woff[l], len, ns);
         if (l > 0)   // dz_l = (dz_{l+1} W_l^T) * (1 - act_l^2)
             gemm<false, true, EPI_DTANH>(m->stream, m->dz[l + 1], m->params + m->woff[l], m->dz[l], n, fi, fo, fo,
                                          fo, fi, nullptr, m->act[l]);
@@ -375,6 +465,12 @@ int rc_mlp_create(int device, int n_in, int width, int hidden, int n_out, rc_mlp
     for (float **p : {&m->params, &m->grads, &m->m1, &m->m2})
         if (hipMalloc(p, sizeof(float) * m->nparam) != hipSuccess)
             return bail(mlp_fail(nullptr, RC_ERR_MEMORY, "device allocation failed"));
+    size_t nsplit = 0;
+    for (int l = 0; l < m->nlayer; ++l)
+        nsplit = std::max(nsplit, (size_t)splitk_max_slices(m->dims[l], m->dims[l + 1]) * (m->dims[l] + 1) *
+                                      m->dims[l + 1]);
+    if (hipMalloc(&m->splitk, sizeof(float) * nsplit) != hipSuccess)
+        return bail(mlp_fail(nullptr, RC_ERR_MEMORY, "device allocation failed"));
     if (hipMalloc(&m->partial, sizeof(float) * kPartials) != hipSuccess ||
         hipMalloc(&m->loss, sizeof(float)) != hipSuccess)
         return bail(mlp_fail(nullptr, RC_ERR_MEMORY, "device allocation failed"));
@@ -393,7 +489,7 @@ void rc_mlp_destroy(rc_mlp *m) {
         (void)hipSetDevice(m->device);
         (void)hipStreamSynchronize(m->stream);
     }
-    for (float *p : {m->params, m->grads, m->m1, m->m2, m->data_x, m->data_y, m->ybuf, m->partial, m->loss})
+    for (float *p : {m->params, m->grads, m->m1, m->m2, m->data_x, m->data_y, m->ybuf, m->partial, m->loss, m->splitk})
         if (p) (void)hipFree(p);
     for (float *p : m->act) if (p) (void)hipFree(p);
     for (float *p : m->dz) if (p) (void)hipFree(p);
@@ -502,7 +598,7 @@ namespace rc {
 // the chunk buffers of the forward pass and the acceptance counters
 struct Surrogate {
     rc_mlp *m = nullptr;
-    double tol = 0.0;
+    double tol = 0.0, rtol = 0.0;
     double *xf = nullptr;        // device, kXfLen
     float *buf[8] = {};          // activations of one chunk: dims[0..nlayer]
     int chunk = 0;
@@ -513,7 +609,8 @@ struct Surrogate {
 int surrogate_xform_len() { return kXfLen; }
 
 // the network's dims must be (2+Nsp) -> ... -> Nsp-1
-int surrogate_create(rc_mlp *m, int ne, const double *xform, double tol, Surrogate **out, std::string &msg) {
+int surrogate_create(rc_mlp *m, int ne, const double *xform, double tol, double rtol, Surrogate **out,
+                     std::string &msg) {
     if (m->nin != kSurIn || m->nout != kNsp - 1) {
         msg = "the network must map " + std::to_string(kSurIn) + " inputs to " + std::to_string(kNsp - 1) + " rates";
         return RC_ERR_ARGUMENT;
@@ -525,6 +622,7 @@ int surrogate_create(rc_mlp *m, int ne, const double *xform, double tol, Surroga
     Surrogate *s = new Surrogate();
     s->m = m;
     s->tol = tol;
+    s->rtol = rtol;
     s->chunk = std::min(((ne + 255) / 256) * 256, 131072);
     s->nacc = (ne + 255) / 256;
     bool bad = hipMalloc(&s->xf, sizeof(double) * kXfLen) != hipSuccess ||
@@ -568,7 +666,8 @@ void surrogate_enqueue(Surrogate *s, const CaseDev *dcase, int ne, hipStream_t s
             else
                 gemm<false, false, EPI_BIAS>(st, s->buf[l], W, s->buf[l + 1], n, fo, fi, fi, fo, fo, b);
         }
-        k_sur_gate<kNsp><<<(n + 255) / 256, 256, 0, st>>>(dcase, c0, n, s->xf, s->buf[m->nlayer], s->tol, s->acc);
+        k_sur_gate<kNsp><<<(n + 255) / 256, 256, 0, st>>>(dcase, c0, n, s->xf, s->buf[m->nlayer], s->tol, s->rtol,
+                                                             s->acc);
     }
 }
 

@@ -178,16 +178,17 @@ class SteadyCase:
         check(lib().rc_set_transport(self.h, C.byref(self._tr)), self.h)
 
     # -- learned chemistry (no reference counterpart) ----------------------
-    def set_surrogate(self, mlp, transform=None, tol: float = 0.05) -> None:
+    def set_surrogate(self, mlp, transform=None, tol: float = 0.05, rtol: float = 0.0) -> None:
         """The source term from the network ``mlp`` (surrogate.MLP) with the
-        per-cell gate (training range, mass balance within ``tol``) and the
+        per-cell gate (training range, mass balance within ``tol``; with
+        ``rtol`` > 0 also within ``rtol`` of the exact FP64 rates) and the
         exact FP64 rate as fallback; ``mlp=None`` restores the exact source."""
         if mlp is None:
-            check(lib().rc_set_surrogate(self.h, None, None, 0.0), self.h)
+            check(lib().rc_set_surrogate(self.h, None, None, 0.0, 0.0), self.h)
             self._sur = None
             return
         xf = np.ascontiguousarray(transform.packed(), dtype=np.float64)
-        check(lib().rc_set_surrogate(self.h, mlp.h, dptr(xf), float(tol)), self.h)
+        check(lib().rc_set_surrogate(self.h, mlp.h, dptr(xf), float(tol), float(rtol)), self.h)
         self._sur = (mlp, xf)
 
     def surrogate_stats(self):

@@ -94,7 +94,8 @@ class MLP:
         self._check(lib().rc_mlp_set_data(self.h, _fp(x), _fp(t), len(x)))
 
     def train(self, nsteps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 5, every: int = 100) -> np.ndarray:
-        nrep = (nsteps // every + 1) if every > 0 else 0
+        # the loss is recorded after every ``every`` steps and after the last
+        nrep = (nsteps // every + (1 if nsteps % every else 0)) if every > 0 else 0
         out = np.zeros(max(nrep, 1), dtype=np.float32)
         self._check(lib().rc_mlp_train(self.h, int(nsteps), int(batch), float(lr), int(seed), int(every), _fp(out)))
         return out[:nrep]
@@ -112,29 +113,59 @@ def glorot(shapes, seed: int) -> np.ndarray:
 
 @dataclasses.dataclass
 class Transform:
-    """Input standardisation (log p) and output scales."""
+    """Input standardisation (log p) and the output map: the network's y_k
+    stands for omega_k = knee s_k sinh(y_k) (linear below knee s_k,
+    logarithmic above, so the squared error weighs relative errors)."""
     mean: np.ndarray          # (N_IN,)
     std: np.ndarray           # (N_IN,)
-    out_scale: np.ndarray     # (N_OUT,)
+    out_scale: np.ndarray     # (N_OUT,) s_k: the largest |omega_k| of the data
+    knee: float
     lo: np.ndarray            # (N_IN,) training range of the raw inputs (T, p, Y)
     hi: np.ndarray
 
     @staticmethod
-    def fit(raw: np.ndarray, omega: np.ndarray) -> "Transform":
+    def fit(raw: np.ndarray, omega: np.ndarray, knee: float = 1e-3) -> "Transform":
         z = features(raw)
         return Transform(mean=z.mean(axis=0), std=np.maximum(z.std(axis=0), 1e-12),
-                         out_scale=np.maximum(np.abs(omega).max(axis=0), 1e-30),
+                         out_scale=np.maximum(np.abs(omega).max(axis=0), 1e-30), knee=float(knee),
                          lo=raw.min(axis=0), hi=raw.max(axis=0))
 
     def x(self, raw: np.ndarray) -> np.ndarray:
         return ((features(raw) - self.mean) / self.std).astype(np.float32)
 
     def y(self, omega: np.ndarray) -> np.ndarray:
-        return (omega / self.out_scale).astype(np.float32)
+        return np.arcsinh(omega / (self.knee * self.out_scale)).astype(np.float32)
+
+    def omega(self, y: np.ndarray) -> np.ndarray:
+        """Rates of network outputs y (float64, as the device maps them)."""
+        return self.knee * self.out_scale * np.sinh(np.asarray(y, dtype=np.float64))
 
     def packed(self) -> np.ndarray:
-        """The layout rc_set_surrogate reads: mean, std, out_scale, lo, hi."""
-        return np.concatenate([self.mean, self.std, self.out_scale, self.lo, self.hi]).astype(np.float64)
+        """The layout rc_set_surrogate reads: mean, std, out_scale, knee, lo, hi."""
+        return np.concatenate([self.mean, self.std, self.out_scale, [self.knee], self.lo,
+                               self.hi]).astype(np.float64)
+
+
+def train_schedule(m: "MLP", nsteps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 5,
+                   phases: int = 4, decay: float = 0.3) -> np.ndarray:
+    """Adam over ``phases`` equal parts of ``nsteps`` with the learning rate
+    multiplied by ``decay`` from one part to the next; returns the batch loss
+    at the end of each part."""
+    out = []
+    per = max(nsteps // phases, 1)
+    for k in range(phases):
+        out.append(m.train(per, batch=batch, lr=lr * decay ** k, seed=seed + k, every=per)[-1])
+    return np.array(out)
+
+
+def gate(tf: Transform, raw: np.ndarray, w_net: np.ndarray, w_exact: np.ndarray, tol: float, rtol: float):
+    """The per-cell acceptance the device gate forms (training range, mass
+    balance within tol, and with rtol > 0 agreement with the exact rates)."""
+    inside = np.all((raw >= tf.lo) & (raw <= tf.hi), axis=1)
+    ok = inside & (np.abs(w_net.sum(axis=1)) <= tol * (np.abs(w_net).sum(axis=1) + 1e-3 * tf.out_scale.max()))
+    if rtol > 0:
+        ok &= np.all(np.abs(w_net - w_exact) <= rtol * np.maximum(np.abs(w_exact), 1e-3 * tf.out_scale), axis=1)
+    return ok
 
 
 def features(raw: np.ndarray) -> np.ndarray:
