@@ -134,18 +134,27 @@ def serving_run(n: int, plain: bool) -> dict:
     return out
 
 
-def batched_run(n: int, steps: int) -> dict:
+def batched_run(n: int, steps: int, tune: bool = True) -> dict:
     """n seeded independent cases (cells log-uniform 5k-80k, chamber
-    pressure, T0, equivalence and throat ratio sampled): `steps` steady steps
-    of all of them as one batch (graph replay, one launch per kernel for all
-    cases), then the same steps case by case through the single-case path."""
+    pressure, T0, equivalence and throat ratio sampled): the autotuner picks
+    the colouring granularity and the launch shape on these cases, then
+    `steps` steady steps of all of them as one batch (graph replay, one
+    launch per kernel for all cases) with the tuned setting and with the
+    default shape, then the same steps case by case through the single-case
+    path."""
     import concurrent.futures as cf
-    from rocketcombustor_amd import nozzle, serving
+    from rocketcombustor_amd import autotune, nozzle, serving
     from rocketcombustor_amd.solver import SteadyBatch
     specs = nozzle.sample_cases(n, seed=3)
+    tuned = None
+    if tune:
+        t0 = time.perf_counter()
+        tuned = autotune.tune(specs, steps=10, warmup=2, workers=8)
+        tuned["seconds"] = time.perf_counter() - t0
+    block = tuned["block"] if tuned else 4
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(max_workers=8) as pool:
-        cases = list(pool.map(serving.build_case, specs))
+        cases = list(pool.map(lambda sp: serving.build_case(sp, block=block), specs))
     t_build = time.perf_counter() - t0
     cells = sum(g.ne for g in cases)
     nbytes = sum(g.device_bytes for g in cases)
@@ -155,7 +164,15 @@ def batched_run(n: int, steps: int) -> dict:
         b.run_steady(5, history=False)          # warm-up (graph capture)
         t0 = time.perf_counter()
         b.run_steady(steps, history=False)
-        t_batch = time.perf_counter() - t0
+        t_default = time.perf_counter() - t0
+        if tuned:
+            b.set_launch(tuned["cell_threads"], tuned["run_threads"], tuned["xcd"])
+            b.run_steady(2, history=False)
+            t0 = time.perf_counter()
+            b.run_steady(steps, history=False)
+            t_batch = time.perf_counter() - t0
+        else:
+            t_batch = t_default
         nlev = b.info()["levels"]
         for g in cases:
             b.remove(g)
@@ -167,11 +184,19 @@ def batched_run(n: int, steps: int) -> dict:
     t_plain = time.perf_counter() - t0
     for g in cases:
         g.close()
-    return {"cases": n, "cells": cells, "device_bytes": nbytes, "steps": steps, "levels": nlev,
-            "host_build_s": t_build,
-            "batched_cell_updates_per_s": cells * steps / t_batch, "batched_ms_per_step": 1e3 * t_batch / steps,
-            "plain_cell_updates_per_s": cells * steps / t_plain, "plain_ms_per_step_all": 1e3 * t_plain / steps,
-            "speedup": t_plain / t_batch}
+    out = {"cases": n, "cells": cells, "device_bytes": nbytes, "steps": steps, "levels": nlev,
+           "ordering_block": block, "host_build_s": t_build,
+           "batched_cell_updates_per_s": cells * steps / t_batch, "batched_ms_per_step": 1e3 * t_batch / steps,
+           "batched_default_shape_ms_per_step": 1e3 * t_default / steps,
+           "plain_cell_updates_per_s": cells * steps / t_plain, "plain_ms_per_step_all": 1e3 * t_plain / steps,
+           "speedup": t_plain / t_batch}
+    if tuned:
+        trials = tuned.pop("trials")
+        tuned["trials_fastest5"] = sorted(trials, key=lambda t: t["ms_per_step"])[:5]
+        tuned["trials_slowest"] = max(trials, key=lambda t: t["ms_per_step"])
+        tuned["ntrials"] = len(trials)
+        out["autotune"] = tuned
+    return out
 
 
 def learned_run(train_steps: int, batch: int = 4096, lr: float = 2e-3, nsteps: int = 50, tol: float = 0.05,

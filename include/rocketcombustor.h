@@ -279,6 +279,14 @@ int rc_batch_add(rc_batch *b, rc_case *h);
 int rc_batch_remove(rc_batch *b, rc_case *h);
 /* info3[0..2] = members, total cells, most LU-SGS levels of a member. */
 int rc_batch_info(const rc_batch *b, long long *info3);
+/* Launch shape of the batch's step (applied at its next build): threads per
+ * block of the cell, face and ghost kernels and of the LU-SGS level kernels
+ * (64, 128 or 256; kRunMax = 4 lanes per run), and XCD banding of the work
+ * lists (1 on, 0 off, -1 as the members were laid out).  Results do not
+ * depend on it: every cell and face is computed by one thread in the same
+ * order whatever the shape.  The autotuner's knob; no reference counterpart
+ * (the reference steps one case with serial loops). */
+int rc_batch_set_launch(rc_batch *b, int cell_threads, int run_threads, int xcd);
 /* rc_run_steady for every member at once: nsteps steps; every `every` steps
  * and at step 1, res_out[(n*members + c)*(nvar+1) + k] = the step_residual
  * norms of member c (member order = order of rc_batch_add, after removals).

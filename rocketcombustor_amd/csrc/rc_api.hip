@@ -76,6 +76,9 @@ struct rc_case {
     std::vector<int4> w_cell, w_face, w_ghost;
     std::vector<std::vector<int4>> w_level;
     std::vector<std::vector<int4>> w_pi;   // point-implicit sweep levels
+    // the cell and face segments the lists band over the XCDs (the colours
+    // of a block-multicolour order), for batches that re-cut the lists
+    std::vector<std::pair<int, int>> seg_cell, seg_face;
     LaunchSet L;                   // the case alone
     rc_batch *batch = nullptr;     // batch the case belongs to, if any
     void *arena = nullptr;
@@ -107,9 +110,13 @@ struct rc_batch {
     int nlev = 0;
     double *h_res = nullptr;       // pinned, members x (kNvar+1)
     size_t h_res_n = 0;
-    bool dirty = true;             // membership changed since the last build
+    bool dirty = true;             // membership or launch shape changed since the last build
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
+    // launch shape: threads per block of the cell / face / ghost kernels and
+    // of the LU-SGS level kernels (kRunMax lanes per run), XCD banding of the
+    // work lists (-1: as the members were laid out)
+    int cell_bs = kChunk, run_bs = kChunk, xcd = -1;
     std::string err;
 };
 
@@ -275,6 +282,12 @@ void xcd_list(std::vector<int4> &out, int c, const std::vector<std::pair<int, in
         }
 }
 
+// empty items up to a multiple of kXcd, so that the next XCD-banded list
+// starts on XCD 0
+void pad_xcd(std::vector<int4> &out, int c) {
+    while (out.size() % kXcd) out.push_back(make_int4(c, 0, 0, 0));
+}
+
 bool use_xcd_lists() {
     const char *e = std::getenv("RC_XCD");
     return !(e && e[0] == '0');
@@ -326,6 +339,8 @@ int layout_and_upload(rc_case *h) {
         }
         xcd_list(h->w_cell, 0, cs, kChunk);
         xcd_list(h->w_face, 0, fs, kChunk);
+        h->seg_cell = cs;
+        h->seg_face = fs;
         chunk_list(h->w_ghost, 0, 0, m.ng, kChunk);
         for (int l = 0; l < nlev; ++l) xcd_list(h->w_level[l], 0, {{m.lvl_ptr[l], m.lvl_ptr[l + 1]}}, kRunChunk);
     } else {
@@ -333,6 +348,8 @@ int layout_and_upload(rc_case *h) {
         chunk_list(h->w_face, 0, 0, m.nf, kChunk);
         chunk_list(h->w_ghost, 0, 0, m.ng, kChunk);
         for (int l = 0; l < nlev; ++l) chunk_list(h->w_level[l], 0, m.lvl_ptr[l], m.lvl_ptr[l + 1], kRunChunk);
+        h->seg_cell = {{0, m.ne}};
+        h->seg_face = {{0, m.nf}};
     }
     const int npl = (int)m.pi_lvl_ptr.size() - 1;
     h->w_pi.assign(npl, {});
@@ -465,6 +482,7 @@ int layout_and_upload(rc_case *h) {
     L.ncases = 1;
     L.single = !xcd;
     L.cell = {d_wc, (int)h->w_cell.size(), 0, m.ne};
+    L.norm = L.cell;
     L.face = {d_wf, (int)h->w_face.size(), 0, m.nf};
     L.ghost = {d_wg, (int)h->w_ghost.size(), 0, m.ng};
     L.level.assign(nlev, {});
@@ -652,26 +670,48 @@ int batch_build(rc_batch *b) {
     batch_release(b);
     const int n = (int)b->members.size();
     std::vector<CaseDev> hcs(n);
-    std::vector<int4> wc, wf, wg;
+    std::vector<int4> wc, wf, wg, wn;
     std::vector<std::vector<int4>> wl;
+    const int rchunk = b->run_bs / kRunMax;
     for (int c = 0; c < n; ++c) {
         const rc_case *h = b->members[c];
         hcs[c] = h->hc;
-        auto add = [c](std::vector<int4> &dst, const std::vector<int4> &src) {
-            for (int4 w : src) dst.push_back(make_int4(c, w.y, w.z, w.w));
-        };
-        add(wc, h->w_cell);
-        add(wf, h->w_face);
-        add(wg, h->w_ghost);
-        if (h->w_level.size() > wl.size()) wl.resize(h->w_level.size());
-        for (size_t l = 0; l < h->w_level.size(); ++l) add(wl[l], h->w_level[l]);
+        const bool xcd = b->xcd < 0 ? h->L.single == false : b->xcd == 1;
+        // residual partial sums: the member's own chunks own its slots
+        for (int4 w : h->w_cell) wn.push_back(make_int4(c, w.y, w.z, w.w));
+        // the step's lists, cut to the batch's launch shape (item lists of
+        // XCD-banded members stay multiples of 8 long, so the banding holds
+        // in the concatenation)
+        if (xcd) {
+            pad_xcd(wc, c);
+            pad_xcd(wf, c);
+            xcd_list(wc, c, h->seg_cell, b->cell_bs);
+            xcd_list(wf, c, h->seg_face, b->cell_bs);
+        } else {
+            chunk_list(wc, c, 0, h->ne, b->cell_bs);
+            chunk_list(wf, c, 0, h->nf, b->cell_bs);
+        }
+        chunk_list(wg, c, 0, h->ng, b->cell_bs);
+        const int nl = (int)h->hm.lvl_ptr.size() - 1;
+        if (nl > (int)wl.size()) wl.resize(nl);
+        for (int l = 0; l < nl; ++l) {
+            const int a = h->hm.lvl_ptr[l], e = h->hm.lvl_ptr[l + 1];
+            if (xcd) {
+                pad_xcd(wl[l], c);
+                xcd_list(wl[l], c, {{a, e}}, rchunk);
+            } else {
+                chunk_list(wl[l], c, a, e, rchunk);
+            }
+        }
     }
     b->nlev = (int)wl.size();
     size_t nlw = 0;
     for (auto &v : wl) nlw += v.size();
     Carver dry, real;
+    int4 *dwn = nullptr;
     auto plan = [&](Carver &c, CaseDev *&dc, int4 *&dwc, int4 *&dwf, int4 *&dwg, int4 *&dwl, double *&dres) {
         dc = c.take<CaseDev>(std::max(n, 1));
+        dwn = c.take<int4>(wn.size());
         dwc = c.take<int4>(wc.size());
         dwf = c.take<int4>(wf.size());
         dwg = c.take<int4>(wg.size());
@@ -695,18 +735,20 @@ int batch_build(rc_batch *b) {
     if ((rc = up(dwc, wc.data(), sizeof(int4) * wc.size()))) return rc;
     if ((rc = up(dwf, wf.data(), sizeof(int4) * wf.size()))) return rc;
     if ((rc = up(dwg, wg.data(), sizeof(int4) * wg.size()))) return rc;
+    if ((rc = up(dwn, wn.data(), sizeof(int4) * wn.size()))) return rc;
     LaunchSet &L = b->L;
     L.cases = dc;
     L.ncases = n;
     L.single = false;
-    L.cell = {dwc, (int)wc.size(), 0, 0};
-    L.face = {dwf, (int)wf.size(), 0, 0};
-    L.ghost = {dwg, (int)wg.size(), 0, 0};
+    L.cell = {dwc, (int)wc.size(), 0, 0, b->cell_bs};
+    L.face = {dwf, (int)wf.size(), 0, 0, b->cell_bs};
+    L.ghost = {dwg, (int)wg.size(), 0, 0, b->cell_bs};
+    L.norm = {dwn, (int)wn.size(), 0, 0, kChunk};
     L.level.assign(b->nlev, {});
     size_t off = 0;
     for (int l = 0; l < b->nlev; ++l) {
         if ((rc = up(dwl + off, wl[l].data(), sizeof(int4) * wl[l].size()))) return rc;
-        L.level[l] = {dwl + off, (int)wl[l].size(), 0, 0};
+        L.level[l] = {dwl + off, (int)wl[l].size(), 0, 0, b->run_bs};
         off += wl[l].size();
     }
     L.resout = dres;
@@ -1256,6 +1298,19 @@ int rc_batch_info(const rc_batch *b, long long *info3) {
     info3[0] = (long long)b->members.size();
     info3[1] = cells;
     info3[2] = nlev;
+    return RC_OK;
+}
+
+int rc_batch_set_launch(rc_batch *b, int cell_threads, int run_threads, int xcd) {
+    if (!b) return set_err(nullptr, RC_ERR_ARGUMENT, "NULL batch");
+    auto ok = [](int t) { return t == 64 || t == 128 || t == 256; };
+    if (!ok(cell_threads) || !ok(run_threads) || xcd < -1 || xcd > 1)
+        return bset_err(b, RC_ERR_ARGUMENT, "threads per block must be 64, 128 or 256 and xcd -1, 0 or 1");
+    if (b->cell_bs == cell_threads && b->run_bs == run_threads && b->xcd == xcd) return RC_OK;
+    b->cell_bs = cell_threads;
+    b->run_bs = run_threads;
+    b->xcd = xcd;
+    b->dirty = true;
     return RC_OK;
 }
 

@@ -93,6 +93,7 @@ struct WorkList {
     const int4 *w = nullptr;
     int n = 0;
     int lo = 0, hi = 0;
+    int bs = 256;                  // threads per block of the table form
 };
 
 // everything one step launches over a set of cases
@@ -101,6 +102,7 @@ struct LaunchSet {
     int ncases = 0;
     bool single = false;           // one case: cases[0], launched over [lo, hi)
     WorkList cell, face, ghost;
+    WorkList norm;                 // cell chunks owning the residual partial slots
     std::vector<WorkList> level;   // LU-SGS runs of each level
     std::vector<WorkList> pi_level;   // point-implicit sweep: cells (positions in pi_cells) of each level
     double *resout = nullptr;      // ncases x (kNvar+1) residual norms

@@ -811,7 +811,7 @@ void launch_derived(const LaunchSet &L, double *out, hipStream_t st) {
 }
 
 void launch_resnorm(const LaunchSet &L, hipStream_t st) {
-    LAUNCH(L.cell, kChunk, st, k_resnorm_partial, kNsp);
+    LAUNCH(L.norm, kChunk, st, k_resnorm_partial, kNsp);
     k_resnorm_final<<<L.ncases, 256, 0, st>>>(L.cases, L.resout);
 }
 

@@ -118,6 +118,7 @@ SIGNATURES = {
     "rc_batch_remove": (C.c_int, [_VP, _VP]),
     "rc_batch_info": (C.c_int, [_VP, P(C.c_longlong)]),
     "rc_batch_run_steady": (C.c_int, [_VP, C.c_int, C.c_int, _DP, _IP, _IP]),
+    "rc_batch_set_launch": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
 }
 
 

@@ -307,6 +307,7 @@ class SteadyBatch:
         check(lib().rc_batch_create(int(device), C.byref(h)), None)
         self.h = h
         self.members = []
+        self.launch = (256, 256, -1)
 
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -333,6 +334,14 @@ class SteadyBatch:
     def remove(self, case: SteadyCase) -> None:
         check(lib().rc_batch_remove(self.h, case.h), None)
         self.members.remove(case)
+
+    def set_launch(self, cell_threads: int = 256, run_threads: int = 256, xcd: int = -1) -> None:
+        """Launch shape of the step (rc_batch_set_launch): threads per block
+        of the cell/face/ghost kernels and of the LU-SGS level kernels, and
+        XCD banding of the work lists (-1: as the members were laid out).
+        Results do not depend on it."""
+        check(lib().rc_batch_set_launch(self.h, int(cell_threads), int(run_threads), int(xcd)), None)
+        self.launch = (int(cell_threads), int(run_threads), int(xcd))
 
     def info(self) -> dict:
         v = (C.c_longlong * 3)()

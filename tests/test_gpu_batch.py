@@ -122,3 +122,35 @@ def test_batch_graph_replay_matches_eager(monkeypatch):
             g.close()
     for a, e in zip(*outs):
         assert np.array_equal(a, e)
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 0), (128, 256, 1), (64, 128, 1), (256, 64, -1)])
+def test_launch_shape_does_not_change_results(shape):
+    """The autotuner's knob (threads per block, XCD banding) re-cuts the work
+    lists only: every member stays bitwise its alone result, also when the
+    shape changes between calls."""
+    cases = mixed_cases()
+    n, every = 20, 5
+    announce("batch launch shape %s, %d mixed cases" % (shape, len(cases)))
+    ref = [alone(c, n, every) for c in cases]
+    gs = [SteadyCase(c) for c in cases]
+    with SteadyBatch() as b:
+        for g in gs:
+            b.add(g)
+        b.set_launch(*shape)
+        h1 = b.run_steady(10, every=every)
+        b.set_launch(64, 64, 1)
+        h2 = b.run_steady(10, every=every)
+        for k, g in enumerate(gs):
+            assert np.array_equal(h1[:, k], ref[k][0][:3]), k
+            assert np.array_equal(h2[1:, k], ref[k][0][3:]), k
+            assert np.array_equal(g.primitive(), ref[k][1]), k
+    for g in gs:
+        g.close()
+
+
+def test_launch_shape_rejects_bad_values():
+    with SteadyBatch() as b:
+        for bad in [(32, 256, 1), (256, 512, 1), (256, 256, 2)]:
+            with pytest.raises(native.RcError):
+                b.set_launch(*bad)
