@@ -250,6 +250,13 @@ module rocketcombustor_iface
             integer(c_long_long), intent(out) :: info3(3)
         end function
 
+        integer(c_int) function rc_batch_set_launch(b, cell_threads, run_threads, xcd) &
+                bind(c, name="rc_batch_set_launch")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b
+            integer(c_int), value :: cell_threads, run_threads, xcd
+        end function
+
         integer(c_int) function rc_batch_run_steady(b, nsteps, every, res_out, done, err_case) &
                 bind(c, name="rc_batch_run_steady")
             import :: c_int, c_ptr, c_double

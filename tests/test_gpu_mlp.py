@@ -18,7 +18,7 @@ def announce(msg: str) -> None:
     sys.__stderr__.flush()
 
 
-@pytest.mark.parametrize("n", [1, 37, 300])
+@pytest.mark.parametrize("n", [1, 37, 300, 4099])
 def test_forward_and_gradient_match_reference(n):
     announce("mlp forward/gradient n=%d" % n)
     m = MLP(seed=7)
