@@ -609,8 +609,9 @@ __global__ void __launch_bounds__(256) k_lusgs(ITEM_ARGS) {
 // WA += dW (Time_LUSGS_steady.f90:193-197) and update_primitve_variables
 // (update_flow_variables.f90:4-48), plus the NaN check of lu_sgs_stage;
 // the new cell-average quantities go to the cache
+// (four waves per SIMD: 127 VGPRs, no spill; three at the compiler's own choice)
 template <int NSP, bool B>
-__global__ void __launch_bounds__(256) k_update(ITEM_ARGS) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_update(ITEM_ARGS) {
     ITEM_PROLOGUE_LDS(i);
     const DevThermo &th = TH;
     double w[NV];

@@ -103,9 +103,10 @@ __device__ __forceinline__ void removedirection(double &vx, double &vy, double n
 
 // TS: the viscous part of TimeStep's face loop (alpaVis; alpaF = alpa +
 // alpaVis/vol(NL), alpa as the inviscid face kernel left it in the face
-// record).  FLUX: the viscous flux of the face.
+// record).  FLUX: the viscous flux of the face.  Two waves per SIMD (one
+// at the compiler's own choice of 256 VGPRs), with a 108-byte spill.
 template <int NSP, bool TS, bool FLUX, bool B>
-__global__ void __launch_bounds__(256) k_viscous(ITEM_ARGS) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_viscous(ITEM_ARGS) {
     __shared__ DevThermo th_lds_;
     __shared__ DevTransport tr_lds_;
     const RC_G CaseDev *Cp_;
