@@ -183,6 +183,92 @@ inline int splitk_max_slices(int M, int N) {
     return std::max(1, 512 / tiles);
 }
 
+// ---------------------------------------------------------------------------
+// fused forward pass of a network of width 128 (inputs <= 8, outputs <= 32):
+// one block of 256 threads takes 64 rows through every layer with the
+// activations in LDS (never in HBM); weights are staged in LDS 32 rows at a
+// time; all products are FP32 MFMA 32x32x2.  Wave w holds rows 32 (w & 1) ..
+// +31 and columns 64 (w >> 1) .. +63 (two 32x32 accumulator tiles).  The
+// MFMA operand layout: lane l gives A[l % 32][l / 32] and B[l / 32][l % 32];
+// accumulator register r of lane l is D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+// ---------------------------------------------------------------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+constexpr int kFW = 128;        // hidden width
+constexpr int kFR = 64;         // rows per block
+constexpr int kFS = kFW + 1;    // LDS row stride (odd: conflict-free column reads)
+constexpr int kFK = 32;         // weight rows per LDS chunk
+
+// tanh x = 1 - 2 / (exp(2x) + 1): exp overflows to +inf for large x (-> 1)
+// and underflows to 0 for very negative x (-> -1)
+__device__ __forceinline__ float fast_tanh(float x) { return 1.f - __fdividef(2.f, __expf(2.f * x) + 1.f); }
+
+__global__ void __launch_bounds__(256) k_mlp_infer(const float *__restrict__ X, float *__restrict__ Y,
+                                                   const float *__restrict__ P, int n, int nin, int nout,
+                                                   int hidden) {
+    __shared__ float Hs[kFR * kFS];
+    __shared__ float Ws[kFK * kFS];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int sl = w & 1, ch = w >> 1;
+    const int li = lane & 31, lk = lane >> 5;
+    const int row0 = blockIdx.x * kFR;
+    // inputs, zero past nin (to 8 columns) and past the last row
+    for (int e = t; e < kFR * 8; e += 256) {
+        const int r = e >> 3, k = e & 7;
+        Hs[r * kFS + k] = (k < nin && row0 + r < n) ? X[(size_t)(row0 + r) * nin + k] : 0.f;
+    }
+    const float *p = P;
+    for (int l = 0; l <= hidden; ++l) {
+        const bool last = l == hidden;
+        const int fi = l == 0 ? nin : kFW, fo = last ? nout : kFW;
+        const int kpad = l == 0 ? 8 : kFW;
+        const float *Wl = p, *bl = p + (size_t)fi * fo;
+        p = bl + fo;
+        f32x16 acc0 = {}, acc1 = {};
+        for (int k0 = 0; k0 < kpad; k0 += kFK) {
+            const int kc = kpad - k0 < kFK ? kpad - k0 : kFK;
+            __syncthreads();   // the previous chunk's (or layer's) reads of Ws and writes of Hs are done
+            for (int e = t; e < kc * kFW; e += 256) {
+                const int k = e >> 7, nn = e & (kFW - 1), kk = k0 + k;
+                Ws[k * kFS + nn] = (kk < fi && nn < fo) ? Wl[(size_t)kk * fo + nn] : 0.f;
+            }
+            __syncthreads();
+            const float *ha = Hs + (32 * sl + li) * kFS + k0 + lk;
+            const float *wb = Ws + lk * kFS + 64 * ch + li;
+            if (!last) {
+#pragma unroll 4
+                for (int k = 0; k < kc; k += 2) {
+                    const float a = ha[k];
+                    const float b0 = wb[k * kFS], b1 = wb[k * kFS + 32];
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1, acc1, 0, 0, 0);
+                }
+            } else if (ch == 0) {
+#pragma unroll 4
+                for (int k = 0; k < kc; k += 2)
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(ha[k], wb[k * kFS], acc0, 0, 0, 0);
+            }
+        }
+        __syncthreads();       // every wave has read its rows of this layer's input
+        if (!last) {
+            const int j0 = 64 * ch + li;
+            const float b0 = bl[j0], b1 = bl[j0 + 32];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = 32 * sl + 8 * (r >> 2) + 4 * lk + (r & 3);
+                Hs[i * kFS + j0] = fast_tanh(acc0[r] + b0);
+                Hs[i * kFS + j0 + 32] = fast_tanh(acc1[r] + b1);
+            }
+        } else if (ch == 0 && li < nout) {
+            const float b0 = bl[li];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = 32 * sl + 8 * (r >> 2) + 4 * lk + (r & 3);
+                if (row0 + i < n) Y[(size_t)(row0 + i) * nout + li] = acc0[r] + b0;
+            }
+        }
+    }
+}
+
 // G = 2 (Y - T) / (M*N), and per-block partial sums of (Y - T)^2
 __global__ void __launch_bounds__(256) k_mse_grad(const float *__restrict__ Y, const float *__restrict__ T,
                                                   float *__restrict__ G, float *__restrict__ partial, int count,
@@ -399,7 +485,31 @@ int ensure_cap(rc_mlp *m, int rows) {
     return RC_OK;
 }
 
+// networks the fused forward kernel takes
+bool fused_forward(const rc_mlp *m) { return m->width == kFW && m->nin <= 8 && m->nout <= 32; }
+
+// y = f(x) for rows [0, n): the fused kernel where the shape allows, else
+// the layer-by-layer GEMM chain through the activation buffers bufs
+void infer(rc_mlp *m, hipStream_t st, const float *x, float *y, int n, float *const *bufs) {
+    if (n <= 0) return;
+    if (fused_forward(m)) {
+        k_mlp_infer<<<(n + kFR - 1) / kFR, 256, 0, st>>>(x, y, m->params, n, m->nin, m->nout, m->nlayer - 1);
+        return;
+    }
+    for (int l = 0; l < m->nlayer; ++l) {
+        const float *W = m->params + m->woff[l], *b = m->params + m->boff[l];
+        const int fi = m->dims[l], fo = m->dims[l + 1];
+        const float *in = l == 0 ? x : bufs[l];
+        float *out = l == m->nlayer - 1 ? y : bufs[l + 1];
+        if (l < m->nlayer - 1)
+            gemm<false, false, EPI_BIAS_TANH>(st, in, W, out, n, fo, fi, fi, fo, fo, b);
+        else
+            gemm<false, false, EPI_BIAS>(st, in, W, out, n, fo, fi, fi, fo, fo, b);
+    }
+}
+
 // forward of rows act[0][0..n): hidden layers bias+tanh, output layer bias
+// (the training pass: keeps every layer's activations for the backward pass)
 void forward(rc_mlp *m, int n, const float *params) {
     for (int l = 0; l < m->nlayer; ++l) {
         const float *W = params + m->woff[l], *b = params + m->boff[l];
@@ -525,7 +635,7 @@ int rc_mlp_forward(rc_mlp *m, const float *x, int n, float *y) {
     int rc = ensure_cap(m, n);
     if (rc) return rc;
     MCHK(m, hipMemcpyAsync(m->act[0], x, sizeof(float) * (size_t)n * m->nin, hipMemcpyHostToDevice, m->stream));
-    forward(m, n, m->params);
+    infer(m, m->stream, m->act[0], m->act[m->nlayer], n, m->act.data());
     MCHK(m, hipMemcpyAsync(y, m->act[m->nlayer], sizeof(float) * (size_t)n * m->nout, hipMemcpyDeviceToHost,
                            m->stream));
     MCHK(m, hipStreamSynchronize(m->stream));
@@ -600,7 +710,7 @@ struct Surrogate {
     rc_mlp *m = nullptr;
     double tol = 0.0, rtol = 0.0;
     double *xf = nullptr;        // device, kXfLen
-    float *buf[8] = {};          // activations of one chunk: dims[0..nlayer]
+    float *buf[8] = {};          // inputs, (GEMM chain) activations, outputs of one chunk: dims[0..nlayer]
     int chunk = 0;
     long long *acc = nullptr;    // per block of 256 cells
     int nacc = 0;
@@ -627,8 +737,10 @@ int surrogate_create(rc_mlp *m, int ne, const double *xform, double tol, double 
     s->nacc = (ne + 255) / 256;
     bool bad = hipMalloc(&s->xf, sizeof(double) * kXfLen) != hipSuccess ||
                hipMalloc(&s->acc, sizeof(long long) * s->nacc) != hipSuccess;
+    // inputs and outputs; the hidden activations only for the GEMM chain
     for (int l = 0; l <= m->nlayer && !bad; ++l)
-        bad = hipMalloc(&s->buf[l], sizeof(float) * (size_t)s->chunk * m->dims[l]) != hipSuccess;
+        if (l == 0 || l == m->nlayer || !fused_forward(m))
+            bad = hipMalloc(&s->buf[l], sizeof(float) * (size_t)s->chunk * m->dims[l]) != hipSuccess;
     if (!bad)
         bad = hipMemcpy(s->xf, xform, sizeof(double) * kXfLen, hipMemcpyHostToDevice) != hipSuccess ||
               hipMemset(s->acc, 0, sizeof(long long) * s->nacc) != hipSuccess;
@@ -658,14 +770,7 @@ void surrogate_enqueue(Surrogate *s, const CaseDev *dcase, int ne, hipStream_t s
     for (int c0 = 0; c0 < ne; c0 += s->chunk) {
         const int n = std::min(s->chunk, ne - c0);
         k_sur_inputs<kNsp><<<(n + 255) / 256, 256, 0, st>>>(dcase, c0, n, s->xf, s->buf[0]);
-        for (int l = 0; l < m->nlayer; ++l) {
-            const float *W = m->params + m->woff[l], *b = m->params + m->boff[l];
-            const int fi = m->dims[l], fo = m->dims[l + 1];
-            if (l < m->nlayer - 1)
-                gemm<false, false, EPI_BIAS_TANH>(st, s->buf[l], W, s->buf[l + 1], n, fo, fi, fi, fo, fo, b);
-            else
-                gemm<false, false, EPI_BIAS>(st, s->buf[l], W, s->buf[l + 1], n, fo, fi, fi, fo, fo, b);
-        }
+        infer(m, st, s->buf[0], s->buf[m->nlayer], n, s->buf);
         k_sur_gate<kNsp><<<(n + 255) / 256, 256, 0, st>>>(dcase, c0, n, s->xf, s->buf[m->nlayer], s->tol, s->rtol,
                                                              s->acc);
     }

@@ -18,10 +18,12 @@ def announce(msg: str) -> None:
     sys.__stderr__.flush()
 
 
-@pytest.mark.parametrize("n", [1, 37, 300, 4099])
-def test_forward_and_gradient_match_reference(n):
-    announce("mlp forward/gradient n=%d" % n)
-    m = MLP(seed=7)
+@pytest.mark.parametrize("n,width", [(1, 128), (37, 128), (300, 128), (4099, 128), (37, 40), (300, 40)])
+def test_forward_and_gradient_match_reference(n, width):
+    """Width 128 runs the fused MFMA forward kernel, width 40 the GEMM chain;
+    the gradient always goes through the training GEMMs."""
+    announce("mlp forward/gradient n=%d width=%d" % (n, width))
+    m = MLP(seed=7, width=width)
     rng = np.random.default_rng(n)
     x = rng.normal(size=(n, 6)).astype(np.float32)
     t = rng.normal(size=(n, 3)).astype(np.float32)
