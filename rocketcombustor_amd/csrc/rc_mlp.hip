@@ -188,7 +188,8 @@ inline int splitk_max_slices(int M, int N) {
 // one block of 256 threads takes 64 rows through every layer with the
 // activations in LDS (never in HBM); weights are staged in LDS 32 rows at a
 // time; all products are FP32 MFMA 32x32x2.  Wave w holds rows 32 (w & 1) ..
-// +31 and columns 64 (w >> 1) .. +63 (two 32x32 accumulator tiles).  The
+// +31 and columns 64 (w >> 1) .. +63 (two 32x32 accumulator tiles); in the
+// output layer (one tile) the two waves of a row slab split K instead.  The
 // MFMA operand layout: lane l gives A[l % 32][l / 32] and B[l / 32][l % 32];
 // accumulator register r of lane l is D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
 // ---------------------------------------------------------------------------
@@ -202,9 +203,12 @@ constexpr int kFK = 32;         // weight rows per LDS chunk
 // and underflows to 0 for very negative x (-> -1)
 __device__ __forceinline__ float fast_tanh(float x) { return 1.f - __fdividef(2.f, __expf(2.f * x) + 1.f); }
 
-__global__ void __launch_bounds__(256) k_mlp_infer(const float *__restrict__ X, float *__restrict__ Y,
-                                                   const float *__restrict__ P, int n, int nin, int nout,
-                                                   int hidden) {
+// (three waves per SIMD, as the LDS allows three blocks per CU: 115 VGPRs,
+// no spill; the compiler's own choice keeps the accumulators in AGPRs and
+// drops to two)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_mlp_infer(const float *__restrict__ X, float *__restrict__ Y, const float *__restrict__ P, int n, int nin, int nout,
+            int hidden) {
     __shared__ float Hs[kFR * kFS];
     __shared__ float Ws[kFK * kFS];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -227,14 +231,15 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float *__restrict__ X, 
         for (int k0 = 0; k0 < kpad; k0 += kFK) {
             const int kc = kpad - k0 < kFK ? kpad - k0 : kFK;
             __syncthreads();   // the previous chunk's (or layer's) reads of Ws and writes of Hs are done
-            for (int e = t; e < kc * kFW; e += 256) {
-                const int k = e >> 7, nn = e & (kFW - 1), kk = k0 + k;
+            const int csh = last ? 5 : 7;        // columns staged: 32 (output layer) or 128
+            for (int e = t; e < (kc << csh); e += 256) {
+                const int k = e >> csh, nn = e & ((1 << csh) - 1), kk = k0 + k;
                 Ws[k * kFS + nn] = (kk < fi && nn < fo) ? Wl[(size_t)kk * fo + nn] : 0.f;
             }
             __syncthreads();
             const float *ha = Hs + (32 * sl + li) * kFS + k0 + lk;
-            const float *wb = Ws + lk * kFS + 64 * ch + li;
             if (!last) {
+                const float *wb = Ws + lk * kFS + 64 * ch + li;
 #pragma unroll 4
                 for (int k = 0; k < kc; k += 2) {
                     const float a = ha[k];
@@ -242,14 +247,40 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float *__restrict__ X, 
                     acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0, acc0, 0, 0, 0);
                     acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1, acc1, 0, 0, 0);
                 }
-            } else if (ch == 0) {
+            } else {
+                // one 32-column tile: wave ch takes rows [ch kc/2, (ch+1) kc/2) of
+                // the chunk, two accumulators alternating (kc/2 is 16 or 4)
+                const int half = kc >> 1;
+                const float *ha2 = ha + ch * half;
+                const float *wb = Ws + (lk + ch * half) * kFS + li;
 #pragma unroll 4
-                for (int k = 0; k < kc; k += 2)
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(ha[k], wb[k * kFS], acc0, 0, 0, 0);
+                for (int k = 0; k < half; k += 4) {
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(ha2[k], wb[k * kFS], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(ha2[k + 2], wb[(k + 2) * kFS], acc1, 0, 0, 0);
+                }
             }
         }
         __syncthreads();       // every wave has read its rows of this layer's input
-        if (!last) {
+        if (last) {
+            // the two column-half waves of a slab hold partial sums over the
+            // two halves of K: ch 1 leaves its part in LDS (Hs is free now),
+            // ch 0 adds it (fixed order) and writes the outputs
+            const f32x16 part = acc0 + acc1;
+            float *pp = Hs + sl * 32 * 33;
+            if (ch == 1) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) pp[(8 * (r >> 2) + 4 * lk + (r & 3)) * 33 + li] = part[r];
+            }
+            __syncthreads();
+            if (ch == 0 && li < nout) {
+                const float b0 = bl[li];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int il = 8 * (r >> 2) + 4 * lk + (r & 3), i = 32 * sl + il;
+                    if (row0 + i < n) Y[(size_t)(row0 + i) * nout + li] = (part[r] + pp[il * 33 + li]) + b0;
+                }
+            }
+        } else {
             const int j0 = 64 * ch + li;
             const float b0 = bl[j0], b1 = bl[j0 + 32];
 #pragma unroll
@@ -257,13 +288,6 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float *__restrict__ X, 
                 const int i = 32 * sl + 8 * (r >> 2) + 4 * lk + (r & 3);
                 Hs[i * kFS + j0] = fast_tanh(acc0[r] + b0);
                 Hs[i * kFS + j0 + 32] = fast_tanh(acc1[r] + b1);
-            }
-        } else if (ch == 0 && li < nout) {
-            const float b0 = bl[li];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int i = 32 * sl + 8 * (r >> 2) + 4 * lk + (r & 3);
-                if (row0 + i < n) Y[(size_t)(row0 + i) * nout + li] = acc0[r] + b0;
             }
         }
     }
@@ -733,7 +757,9 @@ int surrogate_create(rc_mlp *m, int ne, const double *xform, double tol, double 
     s->m = m;
     s->tol = tol;
     s->rtol = rtol;
-    s->chunk = std::min(((ne + 255) / 256) * 256, 131072);
+    // the fused forward pass keeps no activations in HBM: the whole mesh in
+    // one pass (up to 4M cells); the GEMM chain in chunks of 128k cells
+    s->chunk = std::min(((ne + 255) / 256) * 256, fused_forward(m) ? (1 << 22) : 131072);
     s->nacc = (ne + 255) / 256;
     bool bad = hipMalloc(&s->xf, sizeof(double) * kXfLen) != hipSuccess ||
                hipMalloc(&s->acc, sizeof(long long) * s->nacc) != hipSuccess;
