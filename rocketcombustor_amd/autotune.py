@@ -105,7 +105,7 @@ def tune(specs: Iterable, blocks: Sequence[int] = BLOCKS, shapes: Sequence[Tuple
             for g in cases:
                 g.close()
     best = min(trials, key=lambda t: t.ms_per_step)
-    default = [t for t in trials if (t.block, t.cell_threads, t.run_threads, t.xcd) == (4, 256, 256, 1)]
+    default = [t for t in trials if (t.block, t.cell_threads, t.run_threads, t.xcd) == (4, 256, 64, 1)]
     return {"block": best.block, "cell_threads": best.cell_threads, "run_threads": best.run_threads,
             "xcd": best.xcd, "ms_per_step": best.ms_per_step,
             "default_ms_per_step": default[0].ms_per_step if default else None,

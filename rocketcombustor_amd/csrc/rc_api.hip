@@ -116,7 +116,7 @@ struct rc_batch {
     // launch shape: threads per block of the cell / face / ghost kernels and
     // of the LU-SGS level kernels (kRunMax lanes per run), XCD banding of the
     // work lists (-1: as the members were laid out)
-    int cell_bs = kChunk, run_bs = kChunk, xcd = -1;
+    int cell_bs = kChunk, run_bs = kRunThreads, xcd = -1;
     std::string err;
 };
 
@@ -490,7 +490,7 @@ int layout_and_upload(rc_case *h) {
     for (int l = 0; l < nlev; ++l) {
         const size_t n = h->w_level[l].size();
         if ((rc = up(d_wl + off, h->w_level[l].data(), sizeof(int4) * n))) return rc;
-        L.level[l] = {d_wl + off, (int)n, m.lvl_ptr[l], m.lvl_ptr[l + 1]};
+        L.level[l] = {d_wl + off, (int)n, m.lvl_ptr[l], m.lvl_ptr[l + 1], kRunThreads};
         off += n;
     }
     L.pi_level.assign(npl, {});

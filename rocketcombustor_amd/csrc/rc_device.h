@@ -154,7 +154,8 @@ __device__ __forceinline__ GState gview(const RC_G DevState &s) {
 }
 
 constexpr int kChunk = 256;          // cells / faces / ghosts per thread block
-constexpr int kRunChunk = kChunk / kRunMax;   // LU-SGS runs per thread block
+constexpr int kRunThreads = 64;       // threads per block of the LU-SGS level kernels (default shape)
+constexpr int kRunChunk = kRunThreads / kRunMax;   // LU-SGS runs per thread block
 
 enum KernelId { K_TS_FACE = 1, K_TS_CELL, K_GHOST, K_GRADLIM, K_FLUX, K_RES, K_FWD, K_BWD, K_UPDATE, K_P2C,
                 K_BE_FWD, K_BE_BWD, K_BE_UPDATE, K_VISC };

@@ -137,14 +137,15 @@ __device__ __forceinline__ bool cell_prim_to_cons(const GMesh &M, const GState &
     return ok;
 }
 
-// the one-case form over [lo, hi) (per items per block) or the batch form
-// over the work list (one block of the list's size per work item)
+// the one-case form over [lo, hi) (per items per block of the list's size)
+// or the batch form over the work list (one block of the list's size per
+// work item)
 #define LAUNCH(list, per, st, KERN, ...)                                                              \
     do {                                                                                              \
         if (L.single) {                                                                               \
             const int n_ = (list).hi - (list).lo;                                                     \
             if (n_ > 0)                                                                               \
-                KERN<__VA_ARGS__, false><<<(n_ + (per) - 1) / (per), kChunk, 0, st>>>(L.cases, nullptr, \
+                KERN<__VA_ARGS__, false><<<(n_ + (per) - 1) / (per), (list).bs, 0, st>>>(L.cases, nullptr, \
                                                                                      (list).lo, (list).hi); \
         } else if ((list).n) {                                                                        \
             KERN<__VA_ARGS__, true><<<(list).n, (list).bs, 0, st>>>(L.cases, (list).w, 0, 0);          \

@@ -307,7 +307,7 @@ class SteadyBatch:
         check(lib().rc_batch_create(int(device), C.byref(h)), None)
         self.h = h
         self.members = []
-        self.launch = (256, 256, -1)
+        self.launch = (256, 64, -1)
 
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -335,7 +335,7 @@ class SteadyBatch:
         check(lib().rc_batch_remove(self.h, case.h), None)
         self.members.remove(case)
 
-    def set_launch(self, cell_threads: int = 256, run_threads: int = 256, xcd: int = -1) -> None:
+    def set_launch(self, cell_threads: int = 256, run_threads: int = 64, xcd: int = -1) -> None:
         """Launch shape of the step (rc_batch_set_launch): threads per block
         of the cell/face/ghost kernels and of the LU-SGS level kernels, and
         XCD banding of the work lists (-1: as the members were laid out).

@@ -29,7 +29,7 @@ class FakeBatch:
 
     def __init__(self, clock, cost):
         self.clock, self.cost = clock, cost
-        self.members, self.launch, self.calls = [], (256, 256, -1), []
+        self.members, self.launch, self.calls = [], (256, 64, -1), []
 
     def add(self, g):
         self.members.append(g)
@@ -54,8 +54,8 @@ class FakeBatch:
 
 def cost(shape, block):
     c, r, x = shape
-    # cheapest: 128 cell threads, 64 run threads, XCD banding on, block 4
-    return 1e-3 * (1.0 + abs(c - 128) / 256 + abs(r - 64) / 512 + (0.1 if x == 0 else 0.0) + abs(block - 4) / 8)
+    # cheapest: 128 cell threads, 128 run threads, XCD banding on, block 4
+    return 1e-3 * (1.0 + abs(c - 128) / 256 + abs(r - 128) / 512 + (0.1 if x == 0 else 0.0) + abs(block - 4) / 8)
 
 
 def test_tune_launch_picks_the_cheapest_shape_and_sets_it():
@@ -64,8 +64,8 @@ def test_tune_launch_picks_the_cheapest_shape_and_sets_it():
     b.add(FakeCase(None, 4))
     best, trials = autotune.tune_launch(b, steps=5, warmup=1, timer=clock, block=4)
     assert len(trials) == len(autotune.SHAPES) == 18
-    assert (best.cell_threads, best.run_threads, best.xcd) == (128, 64, 1)
-    assert b.launch == (128, 64, 1)
+    assert (best.cell_threads, best.run_threads, best.xcd) == (128, 128, 1)
+    assert b.launch == (128, 128, 1)
     for t in trials:   # every trial timed its own shape
         assert abs(t.ms_per_step - 1e3 * cost((t.cell_threads, t.run_threads, t.xcd), 4)) < 1e-9
 
@@ -81,9 +81,9 @@ def test_tune_over_blocks_and_cache(tmp_path):
 
     res = autotune.tune(specs, blocks=(2, 4, 8), steps=3, warmup=1, make_case=make_case,
                         make_batch=lambda: FakeBatch(clock, cost), timer=clock)
-    assert (res["block"], res["cell_threads"], res["run_threads"], res["xcd"]) == (4, 128, 64, 1)
+    assert (res["block"], res["cell_threads"], res["run_threads"], res["xcd"]) == (4, 128, 128, 1)
     assert len(res["trials"]) == 3 * 18
-    assert abs(res["default_ms_per_step"] - 1e3 * cost((256, 256, 1), 4)) < 1e-9
+    assert abs(res["default_ms_per_step"] - 1e3 * cost((256, 64, 1), 4)) < 1e-9
     assert res["ms_per_step"] < res["default_ms_per_step"]
     assert all(g.closed for g in made) and len(made) == 9
     path = os.path.join(str(tmp_path), "tune.json")
