@@ -12,11 +12,12 @@
 //                 la_math.f90:120-213: dgeequ, dlaqge, dgetf2), all cells at
 //                 once: the block depends only on the cell's own state, so
 //                 both sweeps solve with the same factors
-//   k_be_sweep    forward levels, then backward levels: one thread per cell
-//                 adds the linearised Rusanov terms of its lower (forward) or
-//                 upper (backward) neighbours, solves with the stored factors
-//                 (dgetrs) and refines (dgerfs: at most 5 steps while the
-//                 componentwise backward error exceeds eps and halves)
+//   k_be_sweep    forward levels, then backward levels: eight lanes per
+//                 cell (one per row of its system) add the linearised Rusanov
+//                 terms of its lower (forward) or upper (backward)
+//                 neighbours, solve with the stored factors (dgetrs) and
+//                 refine (dgerfs: at most 5 steps while the componentwise
+//                 backward error exceeds eps and halves)
 //   k_be_update   PA += dPA, WA = U(PA), the cell-average cache
 //
 // Each value is formed with the operations, in the order, of the restated
@@ -27,7 +28,9 @@ namespace rc {
 
 #define FIX_(k, i) S.fixed[(k) * M.sc + (i)]
 
-constexpr int kPiChunk = 64;        // cells per thread block of a sweep level
+constexpr int kBeLanes = 8;                       // lanes per cell in the sweeps (one per row)
+constexpr int kPiCells = 16;                      // cells per thread block of a sweep level
+constexpr int kPiThreads = kPiCells * kBeLanes;
 
 // stored factors of cell c (arrays of sc): the equilibrated block AE, its LU
 // factors (rows in pivoted order), row and column scales, pivots and flags
@@ -373,45 +376,70 @@ __global__ void __launch_bounds__(256) k_be_factor(ITEM_ARGS, double *__restrict
     BEP_(N, i) = flags;
 }
 
-// dgetrs on v with the factors of cell i: pivots from memory, the LU
-// factors from registers
-template <int N, class P>
-__device__ __forceinline__ void be_getrs(const P &piv, int sc, int i, const double (&lu)[N][N], double (&v)[N]) {
+// dgetrs on a vector held one element per lane (lane r of the cell's group
+// holds element r; src = the lane whose element the pivot interchanges
+// bring to row r; lu = row r of the LU factors).  Every element is formed
+// with the operations, in the order, of the serial routine: the
+// interchanges, then column-oriented unit-lower and upper substitution,
+// a column skipped where its solution element is zero.
+template <int N>
+__device__ __forceinline__ double be_getrs_rows(double v, int r, int src, const double (&lu)[N]) {
+    v = __shfl(v, src, kBeLanes);
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const int p = BEP_(j, i);
-#pragma unroll
-        for (int ii = j + 1; ii < N; ++ii) {
-            const bool sw = p == ii;
-            const double t = v[j];
-            v[j] = sw ? v[ii] : t;
-            v[ii] = sw ? t : v[ii];
-        }
+    for (int k = 0; k < N; ++k) {
+        const double vk = __shfl(v, k, kBeLanes);
+        if (vk != 0.0 && r > k) v = v - vk * lu[k];
     }
 #pragma unroll
-    for (int k = 0; k < N; ++k)
-        if (v[k] != 0.0)
-#pragma unroll
-            for (int ii = k + 1; ii < N; ++ii) v[ii] = v[ii] - v[k] * lu[ii][k];
-#pragma unroll
-    for (int k = N - 1; k >= 0; --k)
-        if (v[k] != 0.0) {
-            v[k] = v[k] / lu[k][k];
-#pragma unroll
-            for (int ii = 0; ii < k; ++ii) v[ii] = v[ii] - v[k] * lu[ii][k];
-        }
+    for (int k = N - 1; k >= 0; --k) {
+        const int nz = v != 0.0;
+        if (r == k && nz) v = v / lu[k];
+        const double vk = __shfl(v, k, kBeLanes);
+        if (__shfl(nz, k, kBeLanes) && r < k) v = v - vk * lu[k];
+    }
+    return v;
 }
 
 // one level of the forward (FWD) or backward point-implicit sweep
 // (Time_BackwardEuler.f90:102-193 / 195-285) over cells pi_cells[t] of the
-// work item, one thread per cell
+// work item: kBeLanes lanes per cell, lane r owning row r of the cell's
+// Nvar x Nvar system (lane Nvar has no row and writes nothing).  Lanes 0..2
+// form the terms of the cell's three faces; the rows gather them in face
+// order, then solve with the stored factors (dgetrs) and refine (dgerfs)
+// row-parallel, the broadcasts by shuffles within the group.
 template <int NSP, bool FWD, bool B>
-__global__ void __launch_bounds__(kPiChunk) k_be_sweep(ITEM_ARGS, const double *__restrict__ fac_,
-                                                      const int *__restrict__ piv_,
-                                                      const double *__restrict__ nbc_) {
-    ITEM_PROLOGUE_LDS(t);
+__global__ void __launch_bounds__(kPiThreads) k_be_sweep(ITEM_ARGS, const double *__restrict__ fac_,
+                                                        const int *__restrict__ piv_,
+                                                        const double *__restrict__ nbc_) {
+    __shared__ DevThermo th_lds_;
+    const RC_G CaseDev *Cp_;
+    int t, tend;
+    if constexpr (B) {
+        const int4 w_ = work[blockIdx.x];
+        Cp_ = gp(cases) + w_.x;
+        t = w_.y + (int)threadIdx.x / kBeLanes;
+        tend = w_.z;
+    } else {
+        Cp_ = gp(cases);
+        t = lo + (int)(blockIdx.x * blockDim.x + threadIdx.x) / kBeLanes;
+        tend = hi;
+    }
+    const RC_G CaseDev &C = *Cp_;
+    {
+        const RC_G double *src_ = (const RC_G double *)gp(C.th);
+        double *dst_ = reinterpret_cast<double *>(&th_lds_);
+        for (int k_ = threadIdx.x; k_ < (int)(sizeof(DevThermo) / 8); k_ += blockDim.x) dst_[k_] = src_[k_];
+    }
+    __syncthreads();
+    if (t >= tend) return;               // the group's lanes share t: groups leave whole
+    const GMesh M = gview(C.M);
+    const GState S = gview(C.S);
+    const DevThermo &th = th_lds_;
     constexpr int N = NV;
-    const DevThermo &th = TH;
+    static_assert(N < kBeLanes, "one lane per row");
+    const int r = (int)threadIdx.x % kBeLanes;
+    const bool row = r < N;
+    const int rr = row ? r : N - 1;      // the row-less lane mirrors row N-1's loads
     const int sc = M.sc;
     const RC_G double *fac = gp(fac_);
     const RC_G int *piv = gp(piv_);
@@ -419,82 +447,89 @@ __global__ void __launch_bounds__(kPiChunk) k_be_sweep(ITEM_ARGS, const double *
     const int i = M.pi_cells[t];
     const int flags = BEP_(N, i);
     if (flags & 4) return;               // singular block: reported by k_be_factor
-    double lu[N][N];
+    double lu[N], ae[N];
 #pragma unroll
-    for (int ii = 0; ii < N; ++ii)
+    for (int j = 0; j < N; ++j) {
+        lu[j] = BEF_(kBeLU + rr * N + j, i);
+        ae[j] = BEF_(kBeAE + rr * N + j, i);
+    }
+    // the pivot interchanges as a lane permutation: row r of the pivoted
+    // vector is element src of the unpivoted one
+    int src = r;
+    {
+        int idx[N];
 #pragma unroll
-        for (int j = 0; j < N; ++j) lu[ii][j] = BEF_(kBeLU + ii * N + j, i);
-    // the three neighbour records at once (faces that take no part read the
-    // cell's own records; their terms are not added)
-    int nbr[3];
-    bool use[3];
+        for (int k = 0; k < N; ++k) idx[k] = k;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const int p = BEP_(j, i);
+#pragma unroll
+            for (int ii = j + 1; ii < N; ++ii) {
+                const bool sw = p == ii;
+                const int tj = idx[j];
+                idx[j] = sw ? idx[ii] : tj;
+                idx[ii] = sw ? tj : idx[ii];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) src = r == k ? idx[k] : src;
+    }
+    // lane ks < 3: the term of face ks when that neighbour takes part
+    double c[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) c[k] = 0.0;
+    int use = 0;
+    if (r < 3) {
+        const int nr = M.lu_nb[r * sc + i];
+        use = nr >= 0 && (FWD ? nr < i : nr > i);
+        if (use) {
+            double pj[N], dj[N], nc[kBeNb];
+#pragma unroll
+            for (int k = 0; k < N; ++k) { pj[k] = PA_(k, nr); dj[k] = HWA_(k, nr); }
+#pragma unroll
+            for (int k = 0; k < kBeNb; ++k) nc[k] = nbc[(size_t)nr * kBeNb + k];
+            be_term<NSP>(th, nc, pj, dj, M.lu_n[(r * 2) * sc + i], M.lu_n[(r * 2 + 1) * sc + i],
+                         S.lu_af[r * sc + i], c);
+        }
+    }
+    double dflux = 0.0;
 #pragma unroll
     for (int ks = 0; ks < 3; ++ks) {
-        const int nr = M.lu_nb[ks * sc + i];
-        use[ks] = nr >= 0 && (FWD ? nr < i : nr > i);
-        nbr[ks] = use[ks] ? nr : i;
+        const int u = __shfl(use, ks, kBeLanes);
+        double mine = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const double v = __shfl(c[k], ks, kBeLanes);
+            mine = r == k ? v : mine;
+        }
+        dflux = u ? dflux + mine : dflux;
     }
-    double pj[3][N], dj[3][N], nc[3][kBeNb];
-#pragma unroll
-    for (int ks = 0; ks < 3; ++ks) {
-#pragma unroll
-        for (int k = 0; k < N; ++k) { pj[ks][k] = PA_(k, nbr[ks]); dj[ks][k] = HWA_(k, nbr[ks]); }
-#pragma unroll
-        for (int k = 0; k < kBeNb; ++k) nc[ks][k] = nbc[(size_t)nbr[ks] * kBeNb + k];
-    }
-    double dflux[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) dflux[k] = 0.0;
-#pragma unroll
-    for (int ks = 0; ks < 3; ++ks) {
-        double c[N];
-        be_term<NSP>(th, nc[ks], pj[ks], dj[ks], M.lu_n[(ks * 2) * sc + i], M.lu_n[(ks * 2 + 1) * sc + i],
-                     S.lu_af[ks * sc + i], c);
-#pragma unroll
-        for (int k = 0; k < N; ++k) dflux[k] = use[ks] ? dflux[k] + c[k] : dflux[k];
-    }
-    double hw[N], b[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        hw[k] = HWA_(k, i);
-        const double rhs = FWD ? hw[k] - dflux[k] : dflux[k];
-        b[k] = (flags & 1) ? BEF_(kBeRS + k, i) * rhs : rhs;
-    }
-    double x[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) x[k] = b[k];
-    be_getrs<N>(piv, sc, i, lu, x);
+    const double hw = HWA_(rr, i);
+    const double rhs = FWD ? hw - dflux : dflux;
+    const double b = (flags & 1) ? BEF_(kBeRS + rr, i) * rhs : rhs;
+    double x = be_getrs_rows<N>(b, r, src, lu);
     // dgerfs
     constexpr double safe1 = (N + 1) * kSfmin, safe2 = safe1 / kEps;
     double lstres = 3.0;
     for (int count = 1; count <= 5; ++count) {
-        double w[N], berr = 0.0;
-#pragma unroll
-        for (int ii = 0; ii < N; ++ii) w[ii] = b[ii];
+        double w = b, den = fabs(b);
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            const double tj = -x[j];
-#pragma unroll
-            for (int ii = 0; ii < N; ++ii) w[ii] = w[ii] + tj * BEF_(kBeAE + ii * N + j, i);
+            const double xj = __shfl(x, j, kBeLanes);
+            w = w + (-xj) * ae[j];
         }
 #pragma unroll
-        for (int ii = 0; ii < N; ++ii) {
-            double den = fabs(b[ii]);
+        for (int j = 0; j < N; ++j) den = den + fabs(ae[j]) * fabs(__shfl(x, j, kBeLanes));
+        double berr = row ? (den > safe2 ? fabs(w) / den : (fabs(w) + safe1) / (den + safe1)) : 0.0;
 #pragma unroll
-            for (int j = 0; j < N; ++j) den = den + fabs(BEF_(kBeAE + ii * N + j, i)) * fabs(x[j]);
-            berr = fmax(berr, den > safe2 ? fabs(w[ii]) / den : (fabs(w[ii]) + safe1) / (den + safe1));
-        }
+        for (int m = 1; m < kBeLanes; m <<= 1) berr = fmax(berr, __shfl_xor(berr, m, kBeLanes));
         if (!(berr > kEps && 2.0 * berr <= lstres)) break;
-        be_getrs<N>(piv, sc, i, lu, w);
-#pragma unroll
-        for (int k = 0; k < N; ++k) x[k] = x[k] + w[k];
+        w = be_getrs_rows<N>(w, r, src, lu);
+        x = x + w;
         lstres = berr;
     }
-    if (flags & 2)
-#pragma unroll
-        for (int k = 0; k < N; ++k) x[k] = BEF_(kBeCS + k, i) * x[k];
-#pragma unroll
-    for (int k = 0; k < N; ++k) HWA_(k, i) = FWD ? x[k] : hw[k] - x[k];
+    if (flags & 2) x = BEF_(kBeCS + rr, i) * x;
+    if (row) HWA_(r, i) = FWD ? x : hw - x;
 }
 
 // PA += dPA (Time_BackwardEuler.f90:288-292), primitve_to_conserved_variables
@@ -541,18 +576,18 @@ void launch_be_level(const LaunchSet &L, bool fwd, int l, const double *fac, con
     if (L.single) {
         const int n = w.hi - w.lo;
         if (n <= 0) return;
-        const int nb = (n + kPiChunk - 1) / kPiChunk;
-        if (fwd) k_be_sweep<kNsp, true, false><<<nb, kPiChunk, 0, st>>>(L.cases, nullptr, w.lo, w.hi, fac, piv, nbc);
-        else k_be_sweep<kNsp, false, false><<<nb, kPiChunk, 0, st>>>(L.cases, nullptr, w.lo, w.hi, fac, piv, nbc);
+        const int nb = (n + kPiCells - 1) / kPiCells;
+        if (fwd) k_be_sweep<kNsp, true, false><<<nb, kPiThreads, 0, st>>>(L.cases, nullptr, w.lo, w.hi, fac, piv, nbc);
+        else k_be_sweep<kNsp, false, false><<<nb, kPiThreads, 0, st>>>(L.cases, nullptr, w.lo, w.hi, fac, piv, nbc);
     } else if (w.n) {
-        if (fwd) k_be_sweep<kNsp, true, true><<<w.n, kPiChunk, 0, st>>>(L.cases, w.w, 0, 0, fac, piv, nbc);
-        else k_be_sweep<kNsp, false, true><<<w.n, kPiChunk, 0, st>>>(L.cases, w.w, 0, 0, fac, piv, nbc);
+        if (fwd) k_be_sweep<kNsp, true, true><<<w.n, kPiThreads, 0, st>>>(L.cases, w.w, 0, 0, fac, piv, nbc);
+        else k_be_sweep<kNsp, false, true><<<w.n, kPiThreads, 0, st>>>(L.cases, w.w, 0, 0, fac, piv, nbc);
     }
 }
 
 void launch_be_update(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_be_update, kNsp); }
 
-int pi_chunk() { return kPiChunk; }
+int pi_chunk() { return kPiCells; }
 size_t be_factor_doubles() { return kBeFac; }
 size_t be_factor_ints() { return kBePiv; }
 size_t be_cache_doubles() { return kBeNb; }
