@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -321,12 +322,27 @@ __global__ void k_sum_partials(const float *__restrict__ partial, int n, float s
     }
 }
 
-// Adam (Kingma & Ba) with bias correction; step t from 1
+// the training step counter (Adam's t, from 1) lives on the device, so a
+// captured graph of several steps replays with the right counter
+__global__ void k_step_count(long long *__restrict__ t) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *t = *t + 1;
+}
+
+// the batch loss of the current step into the call's history (step t0 + 1
+// of the call at index 0)
+__global__ void k_record_loss(const float *__restrict__ loss, float *__restrict__ hist,
+                              const long long *__restrict__ t, long long t0) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) hist[*t - t0 - 1] = *loss;
+}
+
+// Adam (Kingma & Ba) with bias correction 1 - beta^t, step t from 1
 __global__ void __launch_bounds__(256) k_adam(float *__restrict__ p, const float *__restrict__ g,
                                               float *__restrict__ m1, float *__restrict__ m2, int n, float lr,
-                                              float b1, float b2, float eps, float c1, float c2) {
+                                              float b1, float b2, float eps, const long long *__restrict__ t) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= n) return;
+    const float tf = (float)*t;
+    const float c1 = 1.f - powf(b1, tf), c2 = 1.f - powf(b2, tf);
     const float gi = g[e];
     const float a = b1 * m1[e] + (1.f - b1) * gi;
     const float b = b2 * m2[e] + (1.f - b2) * gi * gi;
@@ -343,13 +359,16 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
-// a minibatch of rows drawn with replacement from the resident data set
+// a minibatch of rows drawn with replacement from the resident data set:
+// row r of step t from SplitMix64 at seed + t 2^32 + r
 __global__ void __launch_bounds__(256) k_gather_batch(const float *__restrict__ X, const float *__restrict__ Y,
                                                       float *__restrict__ xb, float *__restrict__ yb, int batch,
-                                                      long long n, int nin, int nout, uint64_t seed) {
+                                                      long long n, int nin, int nout, uint64_t seed,
+                                                      const long long *__restrict__ t) {
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= batch) return;
-    const long long src = (long long)(splitmix64(seed + (uint64_t)r) % (uint64_t)n);
+    const uint64_t s = seed + (uint64_t)*t * 0x100000000ull;
+    const long long src = (long long)(splitmix64(s + (uint64_t)r) % (uint64_t)n);
     for (int k = 0; k < nin; ++k) xb[(size_t)r * nin + k] = X[(size_t)src * nin + k];
     for (int k = 0; k < nout; ++k) yb[(size_t)r * nout + k] = Y[(size_t)src * nout + k];
 }
@@ -467,7 +486,8 @@ struct rc_mlp {
     std::vector<size_t> woff, boff;                  // offsets of W_l, b_l in the parameter vector
     size_t nparam = 0;
     float *params = nullptr, *grads = nullptr, *m1 = nullptr, *m2 = nullptr;
-    long long adam_t = 0;
+    long long adam_t = 0;                            // Adam steps taken (host mirror of *d_t)
+    long long *d_t = nullptr;                        // the same on the device
     float *data_x = nullptr, *data_y = nullptr;
     long long ndata = 0;
     int cap = 0;                                     // rows of the work buffers
@@ -606,9 +626,10 @@ int rc_mlp_create(int device, int n_in, int width, int hidden, int n_out, rc_mlp
     if (hipMalloc(&m->splitk, sizeof(float) * nsplit) != hipSuccess)
         return bail(mlp_fail(nullptr, RC_ERR_MEMORY, "device allocation failed"));
     if (hipMalloc(&m->partial, sizeof(float) * kPartials) != hipSuccess ||
-        hipMalloc(&m->loss, sizeof(float)) != hipSuccess)
+        hipMalloc(&m->loss, sizeof(float)) != hipSuccess || hipMalloc(&m->d_t, sizeof(long long)) != hipSuccess)
         return bail(mlp_fail(nullptr, RC_ERR_MEMORY, "device allocation failed"));
-    if (hipMemsetAsync(m->m1, 0, sizeof(float) * m->nparam, m->stream) != hipSuccess ||
+    if (hipMemsetAsync(m->d_t, 0, sizeof(long long), m->stream) != hipSuccess ||
+        hipMemsetAsync(m->m1, 0, sizeof(float) * m->nparam, m->stream) != hipSuccess ||
         hipMemsetAsync(m->m2, 0, sizeof(float) * m->nparam, m->stream) != hipSuccess ||
         hipMemsetAsync(m->params, 0, sizeof(float) * m->nparam, m->stream) != hipSuccess ||
         hipStreamSynchronize(m->stream) != hipSuccess)
@@ -625,6 +646,7 @@ void rc_mlp_destroy(rc_mlp *m) {
     }
     for (float *p : {m->params, m->grads, m->m1, m->m2, m->data_x, m->data_y, m->ybuf, m->partial, m->loss, m->splitk})
         if (p) (void)hipFree(p);
+    if (m->d_t) (void)hipFree(m->d_t);
     for (float *p : m->act) if (p) (void)hipFree(p);
     for (float *p : m->dz) if (p) (void)hipFree(p);
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -639,6 +661,7 @@ int rc_mlp_set_params(rc_mlp *m, const float *p) {
     MCHK(m, hipMemcpyAsync(m->params, p, sizeof(float) * m->nparam, hipMemcpyHostToDevice, m->stream));
     MCHK(m, hipMemsetAsync(m->m1, 0, sizeof(float) * m->nparam, m->stream));
     MCHK(m, hipMemsetAsync(m->m2, 0, sizeof(float) * m->nparam, m->stream));
+    MCHK(m, hipMemsetAsync(m->d_t, 0, sizeof(long long), m->stream));
     MCHK(m, hipStreamSynchronize(m->stream));
     m->adam_t = 0;
     return RC_OK;
@@ -700,27 +723,66 @@ int rc_mlp_set_data(rc_mlp *m, const float *x, const float *t, long long n) {
 int rc_mlp_train(rc_mlp *m, int nsteps, int batch, float lr, unsigned long long seed, int every, float *loss_out) {
     if (!m || nsteps < 0 || batch < 1) return mlp_fail(m, RC_ERR_ARGUMENT, "bad argument");
     if (!m->data_x) return mlp_fail(m, RC_ERR_ARGUMENT, "no training data (rc_mlp_set_data)");
+    if (nsteps == 0) return RC_OK;
     MCHK(m, hipSetDevice(m->device));
     int rc = ensure_cap(m, batch);
     if (rc) return rc;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    int nrep = 0;
-    for (int s = 1; s <= nsteps; ++s) {
-        k_gather_batch<<<(batch + 255) / 256, 256, 0, m->stream>>>(m->data_x, m->data_y, m->act[0], m->ybuf, batch,
-                                                                   m->ndata, m->nin, m->nout,
-                                                                   seed + (uint64_t)(m->adam_t + 1) * 0x100000000ull);
+    float *hist = nullptr;
+    MCHK(m, hipMalloc(&hist, sizeof(float) * (size_t)nsteps));
+    const long long t0 = m->adam_t;
+    hipStream_t st = m->stream;
+    // one Adam step; every argument is fixed for the call, the step counter
+    // is read on the device
+    auto step = [&]() {
+        k_step_count<<<1, 1, 0, st>>>(m->d_t);
+        k_gather_batch<<<(batch + 255) / 256, 256, 0, st>>>(m->data_x, m->data_y, m->act[0], m->ybuf, batch,
+                                                            m->ndata, m->nin, m->nout, seed, m->d_t);
         forward(m, batch, m->params);
         backward(m, batch);
-        m->adam_t++;
-        const float c1 = 1.f - std::pow(b1, (float)m->adam_t), c2 = 1.f - std::pow(b2, (float)m->adam_t);
-        k_adam<<<(int)((m->nparam + 255) / 256), 256, 0, m->stream>>>(m->params, m->grads, m->m1, m->m2,
-                                                                      (int)m->nparam, lr, b1, b2, eps, c1, c2);
-        if (loss_out && every > 0 && (s % every == 0 || s == nsteps)) {
-            MCHK(m, hipMemcpyAsync(loss_out + nrep, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
-            ++nrep;
+        k_record_loss<<<1, 1, 0, st>>>(m->loss, hist, m->d_t, t0);
+        k_adam<<<(int)((m->nparam + 255) / 256), 256, 0, st>>>(m->params, m->grads, m->m1, m->m2, (int)m->nparam, lr,
+                                                               b1, b2, eps, m->d_t);
+    };
+    // graphs of kTrainGraph steps replayed (launch-bound otherwise), the rest
+    // launched directly: the same kernels with the same arguments
+    constexpr int kTrainGraph = 50;
+    const char *ng = std::getenv("RC_NO_GRAPH");
+    const bool graphs = !(ng && ng[0] == '1') && nsteps >= kTrainGraph;
+    int done = 0;
+    hipError_t err = hipSuccess;
+    if (graphs) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        err = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (err == hipSuccess) {
+            for (int k = 0; k < kTrainGraph; ++k) step();
+            err = hipStreamEndCapture(st, &g);
         }
+        if (err == hipSuccess) err = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        for (; err == hipSuccess && done + kTrainGraph <= nsteps; done += kTrainGraph) err = hipGraphLaunch(ge, st);
+        if (ge) (void)hipGraphExecDestroy(ge);
     }
-    MCHK(m, hipStreamSynchronize(m->stream));
+    for (; err == hipSuccess && done < nsteps; ++done) step();
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    if (err != hipSuccess) {
+        (void)hipFree(hist);
+        return mlp_fail(m, RC_ERR_DEVICE, std::string("HIP error in training: ") + hipGetErrorString(err));
+    }
+    m->adam_t += nsteps;
+    std::vector<float> h;
+    if (loss_out && every > 0) {
+        h.resize((size_t)nsteps);
+        err = hipMemcpy(h.data(), hist, sizeof(float) * (size_t)nsteps, hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(hist);
+    if (err != hipSuccess) return mlp_fail(m, RC_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(err));
+    if (!h.empty()) {
+        int nrep = 0;
+        for (int s = 1; s <= nsteps; ++s)
+            if (s % every == 0 || s == nsteps) loss_out[nrep++] = h[(size_t)s - 1];
+    }
     return RC_OK;
 }
 

@@ -52,3 +52,26 @@ def test_training_reduces_loss_and_is_deterministic():
         m.close()
     assert runs[0][0][-1] < 0.1 * runs[0][0][0]
     assert np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_graph_training_equals_direct_launches(monkeypatch):
+    """Training replays captured graphs of 50 steps (the Adam step counter
+    and the minibatch stream read on the device); with graphs off the same
+    steps launched directly give bitwise the same parameters and losses,
+    also across calls that continue the counter."""
+    announce("mlp training: graph replay vs direct launches")
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, size=(5000, 6)).astype(np.float32)
+    t = np.column_stack([x[:, 0] * x[:, 1], np.sin(x[:, 2]), x[:, 3] - x[:, 4]]).astype(np.float32)
+    runs = []
+    for no_graph in ("0", "1"):
+        monkeypatch.setenv("RC_NO_GRAPH", no_graph)
+        m = MLP(seed=3)
+        m.set_data(x, t)
+        l1 = m.train(120, batch=256, lr=1e-3, seed=9, every=40)
+        l2 = m.train(37, batch=256, lr=5e-4, seed=10, every=10)
+        runs.append((l1, l2, m.params))
+        m.close()
+    assert len(runs[0][0]) == 3 and len(runs[0][1]) == 4
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
