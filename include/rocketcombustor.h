@@ -192,8 +192,10 @@ int  rc_mlp_train(rc_mlp *m, int nsteps, int batch, float lr, unsigned long long
  * cell takes the exact FP64 rate (sourceterm.f90:17-43).  xform = input mean
  * (2+Nsp), input std (2+Nsp), output scales s_k (Nsp-1), kappa (1), raw input
  * range low and high (2+Nsp each, on T, p, Y).  m = NULL restores the exact
- * source.  The network stays owned by the caller and must outlive its use;
- * such a case steps alone (not in a batch).  No reference counterpart. */
+ * source.  The network lives on the case's device, stays owned by the
+ * caller, must outlive its use and is not trained while a case uses it
+ * (the step reads its parameters on the case's stream); such a case steps
+ * alone (not in a batch).  No reference counterpart. */
 int rc_set_surrogate(rc_case *h, rc_mlp *m, const double *xform, double tol, double rtol);
 /* accepted and evaluated cell source evaluations since rc_set_surrogate */
 int rc_get_surrogate_stats(rc_case *h, long long *accepted_evaluated2);

@@ -494,6 +494,8 @@ struct rc_mlp {
     std::vector<float *> act;                        // act[0] = inputs, act[l] = layer l outputs (cap rows)
     std::vector<float *> dz;                         // gradients of the layer pre-activations (cap rows)
     float *ybuf = nullptr, *partial = nullptr, *loss = nullptr;
+    float *fx = nullptr, *fy = nullptr;              // fused forward pass: inputs, outputs (fcap rows)
+    int fcap = 0;
     float *splitk = nullptr;                         // slice partials of the layer gradients
     std::string err;
 };
@@ -644,7 +646,8 @@ void rc_mlp_destroy(rc_mlp *m) {
         (void)hipSetDevice(m->device);
         (void)hipStreamSynchronize(m->stream);
     }
-    for (float *p : {m->params, m->grads, m->m1, m->m2, m->data_x, m->data_y, m->ybuf, m->partial, m->loss, m->splitk})
+    for (float *p : {m->params, m->grads, m->m1, m->m2, m->data_x, m->data_y, m->ybuf, m->partial, m->loss, m->splitk,
+                     m->fx, m->fy})
         if (p) (void)hipFree(p);
     if (m->d_t) (void)hipFree(m->d_t);
     for (float *p : m->act) if (p) (void)hipFree(p);
@@ -679,12 +682,29 @@ int rc_mlp_forward(rc_mlp *m, const float *x, int n, float *y) {
     if (!m || !x || !y || n < 0) return mlp_fail(m, RC_ERR_ARGUMENT, "bad argument");
     if (n == 0) return RC_OK;
     MCHK(m, hipSetDevice(m->device));
-    int rc = ensure_cap(m, n);
-    if (rc) return rc;
-    MCHK(m, hipMemcpyAsync(m->act[0], x, sizeof(float) * (size_t)n * m->nin, hipMemcpyHostToDevice, m->stream));
-    infer(m, m->stream, m->act[0], m->act[m->nlayer], n, m->act.data());
-    MCHK(m, hipMemcpyAsync(y, m->act[m->nlayer], sizeof(float) * (size_t)n * m->nout, hipMemcpyDeviceToHost,
-                           m->stream));
+    float *dx, *dy;
+    if (fused_forward(m)) {
+        // the fused pass keeps the activations on chip: inputs and outputs only
+        if (n > m->fcap) {
+            if (m->fx) (void)hipFree(m->fx);
+            if (m->fy) (void)hipFree(m->fy);
+            m->fx = m->fy = nullptr;
+            m->fcap = 0;
+            MCHK(m, hipMalloc(&m->fx, sizeof(float) * (size_t)n * m->nin));
+            MCHK(m, hipMalloc(&m->fy, sizeof(float) * (size_t)n * m->nout));
+            m->fcap = n;
+        }
+        dx = m->fx;
+        dy = m->fy;
+    } else {
+        int rc = ensure_cap(m, n);
+        if (rc) return rc;
+        dx = m->act[0];
+        dy = m->act[m->nlayer];
+    }
+    MCHK(m, hipMemcpyAsync(dx, x, sizeof(float) * (size_t)n * m->nin, hipMemcpyHostToDevice, m->stream));
+    infer(m, m->stream, dx, dy, n, m->act.data());
+    MCHK(m, hipMemcpyAsync(y, dy, sizeof(float) * (size_t)n * m->nout, hipMemcpyDeviceToHost, m->stream));
     MCHK(m, hipStreamSynchronize(m->stream));
     return RC_OK;
 }
@@ -807,6 +827,11 @@ int surrogate_xform_len() { return kXfLen; }
 // the network's dims must be (2+Nsp) -> ... -> Nsp-1
 int surrogate_create(rc_mlp *m, int ne, const double *xform, double tol, double rtol, Surrogate **out,
                      std::string &msg) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != m->device) {
+        msg = "the network lives on device " + std::to_string(m->device) + ", the case on another";
+        return RC_ERR_ARGUMENT;
+    }
     if (m->nin != kSurIn || m->nout != kNsp - 1) {
         msg = "the network must map " + std::to_string(kSurIn) + " inputs to " + std::to_string(kNsp - 1) + " rates";
         return RC_ERR_ARGUMENT;
