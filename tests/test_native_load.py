@@ -6,6 +6,8 @@ import os
 import re
 
 import numpy as np
+import ctypes as C
+
 import pytest
 
 from rocketcombustor_amd import native, nozzle
@@ -59,6 +61,25 @@ def test_error_without_device_is_reported():
     with pytest.raises(native.RcError) as ei:
         SteadyCase(case)
     assert ei.value.code == 6
+
+
+def test_argument_errors_need_no_device():
+    """The entry points reject missing handles and bad arguments with
+    RC_ERR_ARGUMENT before touching a device (batch launch shape, network,
+    learned source)."""
+    L = native.lib()
+    arg = native.RC_ERR_ARGUMENT
+    assert L.rc_batch_set_launch(None, 256, 64, 1) == arg
+    assert L.rc_set_surrogate(None, None, None, 0.0, 0.0) == arg
+    out = (C.c_longlong * 2)()
+    assert L.rc_get_surrogate_stats(None, out) == arg
+    h = C.c_void_p()
+    assert L.rc_mlp_create(0, 0, 128, 3, 3, C.byref(h)) == arg          # no inputs
+    assert L.rc_mlp_create(0, 6, 128, 0, 3, C.byref(h)) == arg          # no hidden layer
+    assert L.rc_mlp_train(None, 10, 64, 1e-3, 1, 1, None) == arg
+    assert L.rc_mlp_forward(None, None, 1, None) == arg
+    assert L.rc_mlp_num_params(None) == -1
+    assert L.rc_mlp_last_error(None)
 
 
 @pytest.mark.parametrize("nx,ny,block", [(24, 6, 1), (24, 6, 4), (150, 40, 1), (150, 40, 4)])
