@@ -240,7 +240,7 @@ def learned_run(train_steps: int, batch: int = 4096, lr: float = 2e-3, nsteps: i
     m = surrogate.MLP(seed=5)
     m.set_data(tf.x(raw[keep]), tf.y(omega[keep]))
     t1 = time.perf_counter()
-    losses = surrogate.train_schedule(m, train_steps, batch=batch, lr=lr, seed=5)
+    losses = surrogate.train_schedule(m, train_steps, batch=batch, lr=lr, seed=5, phases=6, decay=0.4)
     t_train = time.perf_counter() - t1
     flops_step = 3 * 2 * batch * sum(fi * fo for fi, fo in m.shapes)   # forward + 2x backward
     w_hold = tf.omega(m.forward(tf.x(raw[hold])))
@@ -248,7 +248,7 @@ def learned_run(train_steps: int, batch: int = 4096, lr: float = 2e-3, nsteps: i
     out = {"states": int(len(raw)), "sample_s": t_sample,
            "network": "6 -> 3 x 128 tanh -> 3, FP32, outputs omega_k = 1e-3 s_k sinh(y_k)",
            "train": {"rows": int(keep.sum()), "steps": train_steps, "batch": batch, "lr": lr,
-                     "schedule": "Adam, 4 equal phases, lr x0.3 per phase", "loss_per_phase": losses.tolist(),
+                     "schedule": "Adam, 6 equal phases, lr x0.4 per phase", "loss_per_phase": losses.tolist(),
                      "seconds": t_train, "tflops": flops_step * train_steps / t_train / 1e12},
            "holdout": {"rows": int(len(hold)), "rel_err_median": float(np.median(rel)),
                        "rel_err_p90": float(np.percentile(rel, 90)), "rel_err_p99": float(np.percentile(rel, 99)),
@@ -418,7 +418,7 @@ def main() -> int:
                     help="skip the learned-chemistry workload (by default: sample states from the 12k and 400k "
                          "cases, train the source-term MLP on the device, then step the 1.6M-triangle nozzle with "
                          "the gated surrogate and with exact kinetics)")
-    ap.add_argument("--learned-steps", type=int, default=60000, help="training steps of --learned")
+    ap.add_argument("--learned-steps", type=int, default=100000, help="training steps of --learned")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device three times (a plain copy kernel of "
                          "known size, to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
