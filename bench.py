@@ -42,6 +42,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FP64 cell-updates/s (residual+LU-SGS); served cases/hour + p50/p99 to converge"
 HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per cell-update of the steady step on triangles (NF/NE = 1.5)
+B_CU_TRI = 1256.0 + 160.0 * 1.5
 
 
 def kernel_bytes(ne: int, nf: int, ng: int) -> dict:
@@ -130,11 +132,17 @@ def serving_run(n: int, plain: bool) -> dict:
     reqs = serving.poisson_requests(n, rate=SERVE_RATE)
     srv = serving.Server(round_steps=SERVE_ROUND, orders=SERVE_ORDERS, max_steps=20000, workers=4)
     rep = srv.run(reqs, time_limit=600.0)
+    if rep.get("cell_updates_per_s"):
+        rep["step_hbm_frac_bcu"] = B_CU_TRI * rep["cell_updates_per_s"] / (HBM_PEAK_GBS * 1e9)
     out = {"requests": n, "rate_req_per_s": SERVE_RATE, "cells": "log-uniform 5e3-4e5",
-           "converged_when": "RoRes <= 1e-2 x its step-1 value (max 20000 steps)", "batched": rep}
+           "converged_when": "RoRes <= 1e-2 x its step-1 value (max 20000 steps)", "batched": rep,
+           "hbm_frac_definition": "%.0f B per cell-update (1256 + 160 NF/NE at NF/NE = 1.5) x cell-updates/s "
+                                  "/ 8 TB/s" % B_CU_TRI}
     if plain:
         seq = serving.serve_sequential(serving.poisson_requests(n, rate=SERVE_RATE), round_steps=SERVE_ROUND,
                                        orders=SERVE_ORDERS, max_steps=20000, time_limit=900.0)
+        if seq.get("cell_updates_per_s"):
+            seq["step_hbm_frac_bcu"] = B_CU_TRI * seq["cell_updates_per_s"] / (HBM_PEAK_GBS * 1e9)
         out["plain"] = seq
         if seq["cases_per_hour"] > 0:
             out["speedup_cases_per_hour"] = rep["cases_per_hour"] / seq["cases_per_hour"]
@@ -196,7 +204,8 @@ def batched_run(n: int, steps: int, tune: bool = True) -> dict:
            "batched_cell_updates_per_s": cells * steps / t_batch, "batched_ms_per_step": 1e3 * t_batch / steps,
            "batched_default_shape_ms_per_step": 1e3 * t_default / steps,
            "plain_cell_updates_per_s": cells * steps / t_plain, "plain_ms_per_step_all": 1e3 * t_plain / steps,
-           "speedup": t_plain / t_batch}
+           "speedup": t_plain / t_batch,
+           "batched_step_hbm_frac_bcu": B_CU_TRI * cells * steps / t_batch / (HBM_PEAK_GBS * 1e9)}
     if tuned:
         trials = tuned.pop("trials")
         tuned["trials_fastest5"] = sorted(trials, key=lambda t: t["ms_per_step"])[:5]
