@@ -20,7 +20,8 @@ workload on one host core; when that binary is absent, the C oracle (a
 serial restatement) instead.
 
 Side workloads, each in its own field of the same line (a failing one is
-reported there and leaves the headline standing): serving (continuous
+reported there and leaves the headline standing): small_case (the 12k
+triangles of the reference's own runs, 2000 steps), serving (continuous
 batching of Poisson requests), batched (256 sampled cases, autotuned launch
 shape and colouring, against the same cases one at a time), learned (the
 FP32 source-term network trained on the device and used behind its gate on
@@ -117,6 +118,35 @@ def cpu_baseline(case, nsteps: int) -> dict:
     wall = time.perf_counter() - t0
     return {"value": ne * nsteps / wall, "unit": "cell-updates/s", "cores": 1, "kind": "port",
             "sample": "%d steps of the same %d-cell case, serial C restatement" % (nsteps, ne)}
+
+
+def small_run(nx: int = 150, ny: int = 40, nsteps: int = 2000, cpu_steps: int = 200, cpu: bool = True) -> dict:
+    """The reference's own CPU-sized case: the 12k-triangle nozzle (150x40),
+    2000 steady steps from the initial field, graph-replayed on the device
+    (small meshes are latency-bound: us per step is the figure), and the
+    reference binary on the first cpu_steps of the same case."""
+    from rocketcombustor_amd import native, nozzle
+    from rocketcombustor_amd.solver import SteadyCase
+    case = nozzle.nozzle_case(nx, ny)
+    perm, _ = native.order_multicolor(case.mesh.cells)
+    case = case.permuted(perm)
+    with SteadyCase(case, device=0) as g:
+        g.run_steady(20, history=False)                 # warm-up, graph capture
+        g.set_primitive(case.pa0)                       # back to the initial field
+        t0 = time.perf_counter()
+        hist = g.run_steady(nsteps, every=nsteps)
+        wall = time.perf_counter() - t0
+        out = {"workload": "steady nozzle %dx%d, %d triangles, block-4 order, %d steps from the initial field"
+                           % (nx, ny, g.ne, nsteps),
+               "cells": g.ne, "steps": nsteps, "seconds": wall, "us_per_step": 1e6 * wall / nsteps,
+               "cell_updates_per_s": g.ne * nsteps / wall,
+               "residual_drop_rho": float(hist[-1][0] / hist[0][0])}
+    if cpu:
+        c = cpu_baseline(case, cpu_steps)
+        c["seconds_for_%d_steps_at_that_rate" % nsteps] = case.mesh.ne * nsteps / c["value"]
+        out["cpu_baseline"] = c
+        out["speedup_time_to_solution"] = c["seconds_for_%d_steps_at_that_rate" % nsteps] / wall
+    return out
 
 
 SERVE_RATE = 20.0     # requests per second
@@ -423,6 +453,8 @@ def main() -> int:
     ap.add_argument("--viscous", type=int, default=0, metavar="K",
                     help="also time K steady steps of the same case on the viscous path (IfViscous, the "
                          "shipped transport tables)")
+    ap.add_argument("--no-small", dest="small", action="store_false",
+                    help="skip the 12k-triangle case (2000 steps on the device, the reference binary on a sample)")
     ap.add_argument("--no-learned", dest="learned", action="store_false",
                     help="skip the learned-chemistry workload (by default: sample states from the 12k and 400k "
                          "cases, train the source-term MLP on the device, then step the 1.6M-triangle nozzle with "
@@ -514,6 +546,8 @@ def main() -> int:
             out[name] = {"error": "%s: %s" % (type(e).__name__, e)}
             print("bench.py: %s failed: %s" % (name, e), file=sys.stderr, flush=True)
 
+    if args.small:
+        leg("small_case", small_run, cpu=not args.no_cpu_baseline)
     if args.viscous > 0:
         leg("viscous", viscous_run, case, args.viscous, cpu=not args.no_cpu_baseline)
     if args.learned:
