@@ -98,10 +98,11 @@ def test_converged(name):
         assert relerr(g.primitive(), d["ref_final_PA"], -1) < 1e-8
 
 
-@pytest.mark.parametrize("nx,ny,order", [(150, 40, "natural"), (150, 40, "multicolor"), (1000, 200, "multicolor")])
+@pytest.mark.parametrize("nx,ny,order", [(150, 40, "natural"), (150, 40, "multicolor"), (1000, 200, "multicolor"),
+                                         (2000, 400, "multicolor")])
 def test_config_sizes_vs_oracle(nx, ny, order):
-    """The 12k-cell and 400k-cell nozzles: per-call arrays of one step, and
-    three whole steps, against the oracle on the same inputs."""
+    """The 12k-, 400k- and 1.6M-cell nozzles: per-call arrays of one step,
+    and three whole steps, against the oracle on the same inputs."""
     case = nozzle.nozzle_case(nx, ny)
     if order == "multicolor":
         perm, nc = native.order_multicolor(case.mesh.cells)
