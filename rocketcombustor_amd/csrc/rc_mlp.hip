@@ -232,10 +232,25 @@ k_mlp_infer(const float *__restrict__ X, float *__restrict__ Y, const float *__r
         for (int k0 = 0; k0 < kpad; k0 += kFK) {
             const int kc = kpad - k0 < kFK ? kpad - k0 : kFK;
             __syncthreads();   // the previous chunk's (or layer's) reads of Ws and writes of Hs are done
-            const int csh = last ? 5 : 7;        // columns staged: 32 (output layer) or 128
-            for (int e = t; e < (kc << csh); e += 256) {
-                const int k = e >> csh, nn = e & ((1 << csh) - 1), kk = k0 + k;
-                Ws[k * kFS + nn] = (kk < fi && nn < fo) ? Wl[(size_t)kk * fo + nn] : 0.f;
+            if (!last) {
+                // 128-column rows, 16-byte loads (rows of a 128-wide W start on
+                // 16-byte boundaries: every offset before one in the parameter
+                // vector is a multiple of 4 floats)
+                for (int e = t; e < kc * (kFW / 4); e += 256) {
+                    const int k = e >> 5, n4 = (e & 31) * 4, kk = k0 + k;
+                    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (kk < fi) v = *reinterpret_cast<const float4 *>(Wl + (size_t)kk * kFW + n4);
+                    float *w = Ws + k * kFS + n4;
+                    w[0] = v.x;
+                    w[1] = v.y;
+                    w[2] = v.z;
+                    w[3] = v.w;
+                }
+            } else {
+                for (int e = t; e < (kc << 5); e += 256) {     // 32 columns, zero past fo
+                    const int k = e >> 5, nn = e & 31, kk = k0 + k;
+                    Ws[k * kFS + nn] = (kk < fi && nn < fo) ? Wl[(size_t)kk * fo + nn] : 0.f;
+                }
             }
             __syncthreads();
             const float *ha = Hs + (32 * sl + li) * kFS + k0 + lk;
