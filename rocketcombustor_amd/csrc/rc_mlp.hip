@@ -4,12 +4,15 @@
 // Adam) and evaluated per cell inside the steady step with a per-cell gate
 // and the exact FP64 kinetics as fallback.
 //
-// All dense products go through one LDS-tiled FP32 GEMM (64x64 tiles, 256
-// threads, 4x4 outputs per thread) with the layer's epilogue fused: bias +
-// tanh (hidden), bias (output), or the tanh derivative (backward).  The
-// weight and bias gradients are split over the batch into fixed slices whose
-// partials are added in slice order; all other reductions are block
-// reductions in a fixed order, so training is deterministic run to run.
+// Training: the dense products go through one LDS-tiled FP32 GEMM (64x64
+// tiles, 256 threads, 4x4 outputs per thread) with the layer's epilogue
+// fused: bias + tanh (hidden), bias (output), or the tanh derivative
+// (backward).  The weight and bias gradients are split over the batch into
+// fixed slices whose partials are added in slice order; all other reductions
+// are block reductions in a fixed order, so training is deterministic run to
+// run.  Inference (rc_mlp_forward and the step) of a width-128 network: one
+// fused kernel per 64-row tile with the activations in LDS and FP32 MFMA
+// products (k_mlp_infer); other widths take the GEMM chain.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

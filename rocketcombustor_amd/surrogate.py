@@ -5,8 +5,10 @@ exact FP64 kinetics as fallback (``rc_set_surrogate``).
 
 Network: (T, p, Y_1..Y_Nsp) -> omega_1..omega_{Nsp-1}, three tanh hidden
 layers of 128 (the configuration of the learned-chemistry workload).
-Inputs are standardised (log p for the pressure), outputs divided by a
-per-species scale; both transforms travel with the network.
+Inputs are standardised (log p for the pressure); the outputs stand for
+omega_k = knee s_k sinh(y_k) with s_k the largest |omega_k| of the data (a
+squared-error fit then weighs relative errors); both transforms travel with
+the network.
 
 Training data are states the kinetics itself produces: snapshots of the
 primitive field of steady runs with the production rates the device forms
