@@ -29,7 +29,7 @@ namespace rc {
 #define FIX_(k, i) S.fixed[(k) * M.sc + (i)]
 
 constexpr int kBeLanes = 8;                       // lanes per cell in the sweeps (one per row)
-constexpr int kPiCells = 16;                      // cells per thread block of a sweep level
+constexpr int kPiCells = 32;                      // cells per thread block of a sweep level
 constexpr int kPiThreads = kPiCells * kBeLanes;
 
 // stored factors of cell c (arrays of sc): the equilibrated block AE, its LU
