@@ -188,16 +188,18 @@ inline int splitk_max_slices(int M, int N) {
 }
 
 // ---------------------------------------------------------------------------
-// fused forward pass of a network of width 128 (inputs <= 8, outputs <= 32):
+// fused forward pass of a network of width 128 (inputs <= 8, outputs <= 16):
 // one block of 256 threads takes 64 rows through every layer with the
 // activations in LDS (never in HBM); weights are staged in LDS 32 rows at a
 // time; all products are FP32 MFMA 32x32x2.  Wave w holds rows 32 (w & 1) ..
-// +31 and columns 64 (w >> 1) .. +63 (two 32x32 accumulator tiles); in the
-// output layer (one tile) the two waves of a row slab split K instead.  The
-// MFMA operand layout: lane l gives A[l % 32][l / 32] and B[l / 32][l % 32];
-// accumulator register r of lane l is D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+// +31 and columns 64 (w >> 1) .. +63 (two 32x32 accumulator tiles); the
+// output layer (at most 16 outputs) takes 16x16x4 tiles, and the two waves of
+// a row slab split its K.  The 32x32x2 operand layout: lane l gives
+// A[l % 32][l / 32] and B[l / 32][l % 32]; accumulator register r of lane l
+// is D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
 // ---------------------------------------------------------------------------
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kFW = 128;        // hidden width
 constexpr int kFR = 64;         // rows per block
 constexpr int kFS = kFW + 1;    // LDS row stride (odd: conflict-free column reads)
@@ -232,6 +234,7 @@ k_mlp_infer(const float *__restrict__ X, float *__restrict__ Y, const float *__r
         const float *Wl = p, *bl = p + (size_t)fi * fo;
         p = bl + fo;
         f32x16 acc0 = {}, acc1 = {};
+        f32x4 out0 = {}, out1 = {};
         for (int k0 = 0; k0 < kpad; k0 += kFK) {
             const int kc = kpad - k0 < kFK ? kpad - k0 : kFK;
             __syncthreads();   // the previous chunk's (or layer's) reads of Ws and writes of Hs are done
@@ -267,36 +270,46 @@ k_mlp_infer(const float *__restrict__ X, float *__restrict__ Y, const float *__r
                     acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1, acc1, 0, 0, 0);
                 }
             } else {
-                // one 32-column tile: wave ch takes rows [ch kc/2, (ch+1) kc/2) of
-                // the chunk, two accumulators alternating (kc/2 is 16 or 4)
+                // one 16-column tile (MFMA 16x16x4: lane l gives A[l % 16][l / 16]
+                // and B[l / 16][l % 16]) for each 16-row half of the slab; wave
+                // ch takes rows [ch kc/2, (ch+1) kc/2) of the chunk's K (kc/2 is
+                // 16, or 4 for a network without hidden layers)
                 const int half = kc >> 1;
-                const float *ha2 = ha + ch * half;
-                const float *wb = Ws + (lk + ch * half) * kFS + li;
+                const int l16 = lane & 15, lq = lane >> 4;
+                const float *h0 = Hs + (32 * sl + l16) * kFS + k0 + ch * half + lq;
+                const float *wq = Ws + (ch * half + lq) * kFS + l16;
 #pragma unroll 4
                 for (int k = 0; k < half; k += 4) {
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(ha2[k], wb[k * kFS], acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(ha2[k + 2], wb[(k + 2) * kFS], acc1, 0, 0, 0);
+                    const float b = wq[k * kFS];
+                    out0 = __builtin_amdgcn_mfma_f32_16x16x4f32(h0[k], b, out0, 0, 0, 0);
+                    out1 = __builtin_amdgcn_mfma_f32_16x16x4f32(h0[k + 16 * kFS], b, out1, 0, 0, 0);
                 }
             }
         }
         __syncthreads();       // every wave has read its rows of this layer's input
         if (last) {
-            // the two column-half waves of a slab hold partial sums over the
-            // two halves of K: ch 1 leaves its part in LDS (Hs is free now),
-            // ch 0 adds it (fixed order) and writes the outputs
-            const f32x16 part = acc0 + acc1;
-            float *pp = Hs + sl * 32 * 33;
+            // the two waves of a slab hold partial sums over the two halves of
+            // K: ch 1 leaves its part in LDS (Hs is free now), ch 0 adds it
+            // (fixed order) and writes the outputs.  Accumulator register r
+            // of lane l is D[4 (l / 16) + r][l % 16].
+            const int l16 = lane & 15, lq = lane >> 4;
+            float *pp = Hs + sl * 32 * 17;
             if (ch == 1) {
 #pragma unroll
-                for (int r = 0; r < 16; ++r) pp[(8 * (r >> 2) + 4 * lk + (r & 3)) * 33 + li] = part[r];
+                for (int r = 0; r < 4; ++r) {
+                    pp[(4 * lq + r) * 17 + l16] = out0[r];
+                    pp[(16 + 4 * lq + r) * 17 + l16] = out1[r];
+                }
             }
             __syncthreads();
-            if (ch == 0 && li < nout) {
-                const float b0 = bl[li];
+            if (ch == 0 && l16 < nout) {
+                const float b0 = bl[l16];
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int il = 8 * (r >> 2) + 4 * lk + (r & 3), i = 32 * sl + il;
-                    if (row0 + i < n) Y[(size_t)(row0 + i) * nout + li] = (part[r] + pp[il * 33 + li]) + b0;
+                for (int r = 0; r < 4; ++r) {
+                    const int il = 4 * lq + r, i = 32 * sl + il;
+                    if (row0 + i < n) Y[(size_t)(row0 + i) * nout + l16] = (out0[r] + pp[il * 17 + l16]) + b0;
+                    if (row0 + i + 16 < n)
+                        Y[(size_t)(row0 + i + 16) * nout + l16] = (out1[r] + pp[(il + 16) * 17 + l16]) + b0;
                 }
             }
         } else {
@@ -550,7 +563,7 @@ int ensure_cap(rc_mlp *m, int rows) {
 }
 
 // networks the fused forward kernel takes
-bool fused_forward(const rc_mlp *m) { return m->width == kFW && m->nin <= 8 && m->nout <= 32; }
+bool fused_forward(const rc_mlp *m) { return m->width == kFW && m->nin <= 8 && m->nout <= 16; }
 
 // y = f(x) for rows [0, n): the fused kernel where the shape allows, else
 // the layer-by-layer GEMM chain through the activation buffers bufs
