@@ -189,3 +189,48 @@ def test_operator_calls_after_fused_steps():
         g.flux_sourceterm()
         o.flux_sourceterm()
         assert relerr(g.increment(), o.get("hWA"), -1) < 1e-12
+
+
+@pytest.mark.parametrize("nx,ny", [(1, 1), (2, 1), (3, 2)])
+def test_smallest_meshes_vs_oracle(nx, ny):
+    """Meshes of 2, 4 and 12 triangles (every cell on the boundary, a single
+    work item, runs and levels of one or two cells): one step per call and
+    three whole steps against the oracle, graph-replayed and eager."""
+    case = nozzle.nozzle_case(nx, ny)
+    announce("smallest mesh %dx%d NE=%d" % (nx, ny, case.mesh.ne))
+    o = oracle.OracleCase(case)
+    with SteadyCase(case) as g:
+        g.timestep(); o.timestep()
+        g.reconstruction(); o.reconstruction()
+        g.flux_sourceterm(); o.flux_sourceterm()
+        assert relerr(g.increment(), o.get("hWA"), -1) < 1e-12
+        g.lu_sgs_steady(); o.lusgs_steady()
+        assert relerr(g.primitive(), o.get("PA"), -1) < 1e-13
+        g.run_steady(3, history=False)
+        for _ in range(3):
+            o.stage()
+        assert relerr(g.primitive(), o.get("PA"), -1) < 1e-12
+
+
+def test_empty_and_tiny_batches():
+    """A batch without members steps without error; members of 2 and 4
+    triangles in a batch are bitwise what they are alone."""
+    from rocketcombustor_amd.solver import SteadyBatch
+    announce("empty batch, batch of the smallest meshes")
+    cases = [nozzle.nozzle_case(1, 1), nozzle.nozzle_case(2, 1)]
+    ref = []
+    for c in cases:
+        with SteadyCase(c) as g:
+            g.run_steady(5, history=False)
+            ref.append(g.primitive())
+    with SteadyBatch() as b:
+        b.run_steady(3, history=False)
+        gs = [SteadyCase(c) for c in cases]
+        for g in gs:
+            b.add(g)
+        b.run_steady(5, history=False)
+        for g, r in zip(gs, ref):
+            assert np.array_equal(g.primitive(), r)
+        for g in gs:
+            b.remove(g)
+            g.close()
