@@ -170,3 +170,18 @@ def test_steady_after_backward_euler():
         g.lu_sgs_stage()
         assert relerr(g.primitive(), o.get("PA"), -1) < 1e-12
         assert relerr(g.increment(), o.get("hWA"), -1) < 1e-12
+
+
+@pytest.mark.parametrize("ni,nj", [(1, 1), (3, 2)])
+def test_smallest_meshes_vs_oracle(ni, nj):
+    """Backward Euler on 2 and 12 triangles (one or two cells per sweep
+    level): three steps against the oracle, rows and states."""
+    case = be_case(ni, nj, 2e-5, 4, 0.5, multicolor=False)
+    announce("backward Euler smallest mesh %dx%d NE=%d" % (ni, nj, case.mesh.ne))
+    o = oracle.OracleCase(case)
+    ref = run_be(o, case, 3)
+    with SteadyCase(case) as g:
+        rows = g.run_unsteady(3)
+        assert np.array_equal(rows[:, :2], ref[:, :2])
+        assert relerr(rows[:, 2:], ref[:, 2:], -1) < 1e-10
+        assert relerr(g.primitive(), o.get("PA"), -1) < 1e-12
