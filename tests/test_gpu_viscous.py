@@ -43,9 +43,15 @@ def test_viscous_part_matches_reference():
     assert relerr(av - ai, dv["ref_s1_alpaF"] - dn["ref_s1_alpaF"]) < 1e-9
 
 
-@pytest.mark.parametrize("kw", [{}, {"wall_type": 1, "fuel_faces": 2, "subsonic_exit": True, "p_out_ratio": 0.9}])
-def test_multicolour_viscous_steps_vs_oracle(kw):
-    case = nozzle.nozzle_case(80, 20, viscous=True, **kw)
+@pytest.mark.parametrize("size,kw", [((80, 20), {}),
+                                     ((80, 20), {"wall_type": 1, "fuel_faces": 2, "subsonic_exit": True,
+                                                 "p_out_ratio": 0.9}),
+                                     ((1, 1), {}), ((3, 2), {"wall_type": 1})])
+def test_multicolour_viscous_steps_vs_oracle(size, kw):
+    """Viscous steps against the oracle, per call and fused, from 1600
+    triangles down to the smallest meshes (2 and 12 triangles, every face of
+    a cell on the boundary or shared with one neighbour)."""
+    case = nozzle.nozzle_case(*size, viscous=True, **kw)
     perm, _ = native.order_multicolor(case.mesh.cells)
     case = case.permuted(perm)
     announce("viscous multicolour NE=%d %s" % (case.mesh.ne, kw))
