@@ -223,14 +223,16 @@ def test_empty_and_tiny_batches():
         with SteadyCase(c) as g:
             g.run_steady(5, history=False)
             ref.append(g.primitive())
-    with SteadyBatch() as b:
-        b.run_steady(3, history=False)
-        gs = [SteadyCase(c) for c in cases]
-        for g in gs:
-            b.add(g)
-        b.run_steady(5, history=False)
-        for g, r in zip(gs, ref):
-            assert np.array_equal(g.primitive(), r)
-        for g in gs:
-            b.remove(g)
-            g.close()
+    for shape in [(256, 64, -1), (64, 64, 0)]:
+        with SteadyBatch() as b:
+            b.set_launch(*shape)
+            b.run_steady(3, history=False)
+            gs = [SteadyCase(c) for c in cases]
+            for g in gs:
+                b.add(g)
+            b.run_steady(5, history=False)
+            for g, r in zip(gs, ref):
+                assert np.array_equal(g.primitive(), r), shape
+            for g in gs:
+                b.remove(g)
+                g.close()

@@ -43,9 +43,10 @@ def exact_run(case, n):
         return g.primitive()
 
 
-def test_gate_rejecting_all_is_the_exact_step(trained):
+@pytest.mark.parametrize("nx,ny", [(40, 10), (1, 1)])
+def test_gate_rejecting_all_is_the_exact_step(trained, nx, ny):
     m, tf = trained
-    case = nozzle.nozzle_case(40, 10)
+    case = nozzle.nozzle_case(nx, ny)
     perm, _ = native.order_multicolor(case.mesh.cells)
     case = case.permuted(perm)
     announce("surrogate with an all-rejecting gate")
