@@ -512,14 +512,13 @@ __global__ void __launch_bounds__(kPiThreads) k_be_sweep(ITEM_ARGS, const double
     constexpr double safe1 = (N + 1) * kSfmin, safe2 = safe1 / kEps;
     double lstres = 3.0;
     for (int count = 1; count <= 5; ++count) {
-        double w = b, den = fabs(b);
+        double xs[N], w = b, den = fabs(b);
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-            const double xj = __shfl(x, j, kBeLanes);
-            w = w + (-xj) * ae[j];
-        }
+        for (int j = 0; j < N; ++j) xs[j] = __shfl(x, j, kBeLanes);
 #pragma unroll
-        for (int j = 0; j < N; ++j) den = den + fabs(ae[j]) * fabs(__shfl(x, j, kBeLanes));
+        for (int j = 0; j < N; ++j) w = w + (-xs[j]) * ae[j];
+#pragma unroll
+        for (int j = 0; j < N; ++j) den = den + fabs(ae[j]) * fabs(xs[j]);
         double berr = row ? (den > safe2 ? fabs(w) / den : (fabs(w) + safe1) / (den + safe1)) : 0.0;
 #pragma unroll
         for (int m = 1; m < kBeLanes; m <<= 1) berr = fmax(berr, __shfl_xor(berr, m, kBeLanes));
