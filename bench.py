@@ -527,6 +527,10 @@ def main() -> int:
                    "cells": ne, "faces": nf, "ghosts": ng, "lusgs_levels": g.nlev},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "rocprofv3 FETCH_SIZE / WRITE_SIZE passes in %s (the sweep kernels "
+                                       "then ran in 256-thread blocks; the profiler stops at the first "
+                                       "dispatch with the current library)" % os.path.relpath(TRAFFIC_PROFILE, ROOT)
+                     if traffic else None,
                      "algorithmic_bytes_per_launch": kb[dom] / launches[dom], "launches_per_step": launches[dom]},
         "kernel_timing": "HIP events between kernels, eager replay of the same %d steps (%.4f ms/step)"
                          % (args.steps, 1000.0 * wall_eager / args.steps),
