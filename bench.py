@@ -84,6 +84,55 @@ def measured_traffic() -> dict:
     return {}
 
 
+def host_cores() -> dict:
+    """What the host offers this process: nproc, the CPUs this process may
+    run on, and the share used for all-core runs (at most 16 per GPU)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    return {"nproc": os.cpu_count() or 1, "affinity": aff, "share": max(1, min(16, aff))}
+
+
+def cpu_baseline_all_cores(case, nsteps: int, procs: int) -> dict:
+    """The reference binary on every core of the host share at once: procs
+    independent runs of the same case (the reference's own multi-core mode is
+    MPI domain decomposition, which needs mpmetis and is not built), started
+    together; aggregate cell-updates/s = procs x cells x steps / the slowest
+    run's step-loop wall time."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "rc_ref_dump")
+    if not os.path.exists(ref):
+        return {"error": "reference binary not built"}
+    from rocketcombustor_amd import nozzle
+    tmp = tempfile.mkdtemp(prefix="rcbench_all_")
+    try:
+        dirs = []
+        for k in range(procs):
+            d = os.path.join(tmp, "c%d" % k)
+            nozzle.write_reference_case(d, case, nstep_max=nsteps)
+            os.makedirs(os.path.join(d, "out"))
+            dirs.append(d)
+        t0 = time.perf_counter()
+        ps = [subprocess.Popen([ref, str(nsteps), "0", "out", "1"], cwd=d, stdout=subprocess.PIPE,
+                               stderr=subprocess.DEVNULL, text=True) for d in dirs]
+        walls = []
+        for p in ps:
+            out, _ = p.communicate(timeout=900)
+            line = [ln for ln in out.splitlines() if ln.startswith("steps")]
+            if p.returncode != 0 or not line:
+                return {"error": "a reference run failed"}
+            walls.append(float(line[-1].split()[3]))
+        elapsed = time.perf_counter() - t0
+        ne = case.mesh.ne
+        return {"value": procs * ne * nsteps / max(walls), "unit": "cell-updates/s", "cores": procs,
+                "kind": "reference",
+                "sample": "%d concurrent runs of %d steps of the same %d-cell case, reference Fortran (amdflang -O3, "
+                          "1 rank each); slowest step loop %.2f s, process wall %.2f s"
+                          % (procs, nsteps, ne, max(walls), elapsed)}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def cpu_baseline(case, nsteps: int) -> dict:
     ref = os.path.join(ROOT, "oracle", "_ref", "rc_ref_dump")
     ne = case.mesh.ne
@@ -439,6 +488,8 @@ def main() -> int:
     ap.add_argument("--ny", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-all-cores", dest="cpu_all_cores", action="store_false",
+                    help="skip the all-core CPU figure (the reference on every core of the host share at once)")
     ap.add_argument("--serve", type=int, default=24, metavar="N",
                     help="requests of the serving run (Poisson 20/s, 5k-400k cells); 0 skips it")
     ap.add_argument("--serve-plain", type=int, default=1,
@@ -564,6 +615,13 @@ def main() -> int:
         leg("batched", batched_run, args.batched, args.batched_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+        hc = host_cores()
+        out["cpu_baseline"].update(hc)
+        if args.cpu_all_cores:
+            try:
+                out["cpu_baseline"]["all_cores"] = cpu_baseline_all_cores(case, args.cpu_steps, hc["share"])
+            except Exception as e:  # noqa: BLE001
+                out["cpu_baseline"]["all_cores"] = {"error": "%s: %s" % (type(e).__name__, e)}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -222,9 +222,11 @@ int rc_backward_euler_lusgs(rc_case *h, double dt, int inner);
  * until RoRes = sum of the species norms drops below relTol times its value
  * at the first inner iteration, or innerloop iterations.  res_out (may be
  * NULL) receives one row per inner iteration: Res(k)/total_volume, k <
- * nvar, then max(rho); *ninner the number of inner iterations.  The closing
- * reconstruction + flux_sourceterm of the reference (:37-38) change neither
- * PA nor WA and are not run.  The exit pressure is the last one set. */
+ * nvar, then max(rho); *ninner the number of inner iterations.  After the
+ * last inner iteration the closing reconstruction + flux_sourceterm
+ * (:36-38) run as in the reference, so hWA and GradC hold the residual and
+ * limited gradient of the new state (what step_residual reads, fvm2d.f90:137;
+ * rc_residual returns its norms).  The exit pressure is the last one set. */
 int rc_backward_euler_stage(rc_case *h, double dt, int innerloop, double reltol, int *ninner, double *res_out);
 
 /* Replaces Residual + the normalisation of step_residual (residual.f90:4-21,

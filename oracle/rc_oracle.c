@@ -1576,7 +1576,16 @@ int rco_be_stage(rco_case *c, double dt, int innerloop, double reltol, int *ninn
         if (roress) roress[inner - 1] = ro;
         if (ninner) *ninner = inner;
         if (inner == 1) ro1 = ro;
-        if (ro < ro1 * reltol || inner == innerloop) break;
+        if (ro < ro1 * reltol || inner == innerloop) {
+            /* the closing reconstruction + flux_sourceterm of the converged
+               inner loop (Time_BackwardEuler.f90:36-38): hWA and GradC then
+               hold the residual and gradient of the new state, which
+               step_residual reads (fvm2d.f90:137) */
+            if (rco_gradient(c)) return 1;
+            if (c->p.limit_id > 0 && rco_limiter(c)) return 1;
+            if (rco_flux_sourceterm(c)) return 1;
+            break;
+        }
         if (isnan(WA(1, 1))) return fail(c, "there is NAN in WA(1,1)");
     }
     return 0;

@@ -16,8 +16,8 @@
 !!
 !! Unsteady decks with implicit_time_method=3 run the dual-time backward
 !! Euler steps instead (fvm2d.f90:60-120, backwardEuler_stage of
-!! Time_BackwardEuler.f90:4-56 without its closing diagnostic
-!! reconstruction/flux_sourceterm, which change neither PA nor WA): inner
+!! Time_BackwardEuler.f90:4-56 with its closing reconstruction/flux_sourceterm,
+!! then step_residual's norms into OUTDIR/resid_step.bin): inner
 !! iteration k overall (counted across steps) dumps s<k>_res, s<k>_dPA,
 !! s<k>_PA, s<k>_WA for k <= NDUMP, and resid.bin holds rows
 !! (istep, inner, Res(1:Nvar)/total_volume, Romax).
@@ -58,6 +58,7 @@ program rc_ref_dump
             stop 2
         endif
         kin = 0
+        open(94, file=trim(outdir)//'/resid_step.bin', access='stream', form='unformatted', status='replace')
         do istep = 1, nsteps
             ttime_n = ttime
             ttime = ttime_n + cs(1)*dt
@@ -79,7 +80,13 @@ program rc_ref_dump
                 write(91) dble(istep), dble(inner), res(1:Nvar), romax
                 roress = sum(res(Nflow+1:Nvar))
                 if (inner == 1) roress1 = roress
-                if (roress < roress1*relTol .or. inner == InnerLoop) exit
+                if (roress < roress1*relTol .or. inner == InnerLoop) then
+                    ! the closing reconstruction + flux_sourceterm of
+                    ! backwardEuler_stage (Time_BackwardEuler.f90:36-38)
+                    call reconstruction
+                    call flux_sourceterm
+                    exit
+                endif
                 if (isnan(WA(1,1))) then
                     write(*,*) 'NaN at step', istep, 'inner', inner
                     stop 3
@@ -87,7 +94,13 @@ program rc_ref_dump
             enddo
             ttime = ttime_n + dt
             call compute_p_out
+            ! step_residual after the stage (fvm2d.f90:137): norms of the
+            ! residual the closing flux_sourceterm left in hWA
+            call Residual(res(1:Nvar), romax)
+            res(1:Nvar) = res(1:Nvar) / total_volume
+            write(94) dble(istep), res(1:Nvar), romax
         enddo
+        close(94)
     endif
     do istep = 1, merge(nsteps, 0, IfSteady)
         if (istep <= ndump .and. timing == 0) then

@@ -11,7 +11,10 @@
 !! rc_backward_euler_stage after the reference's own compute_p_out hands
 !! its exit pressure to the library.  Unsteady runs write the inner-iteration
 !! rows (istep, inner, Res/total_volume, Romax) to OUTDIR/resid.bin and
-!! converge_uns.dat.  Viscous decks hand the reference's transport and
+!! converge_uns.dat, and, at the main loop's screen-output steps, the
+!! reference's step_residual (converge.dat) of the residual the stage leaves
+!! in hWA, with its rows (istep, Res/total_volume, Romax) in
+!! OUTDIR/resid_step.bin.  Viscous decks hand the reference's transport and
 !! diffusion tables to the library (rc_set_transport).
 !!
 !! usage (in a case directory): rc_ref_host_gpu OUTDIR
@@ -121,6 +124,7 @@ program rc_ref_host_gpu
             stop 2
         endif
         allocate(rows(Nvar + 1, InnerLoop))
+        open(94, file=trim(outdir)//'/resid_step.bin', access='stream', form='unformatted', status='replace')
         open(93, file='converge_uns.dat')
         write(93,*) 'VARIABLES = "Time step","Inner iteration step", "L1(Res)","Romax"'
         totaltime_num = totaltime*(1.-1.e-10)
@@ -142,9 +146,19 @@ program rc_ref_host_gpu
             enddo
             ttime = ttime_n + dt
             call compute_p_out
+            if (istep == 1 .or. mod(istep, NScreenOutput) == 0) then
+                ! hWA is the residual the stage's closing flux_sourceterm left
+                call rc_check(rc_get_increment(h, hWA), h, 'rc_get_increment')
+                call rc_check(rc_get_conserved(h, WA), h, 'rc_get_conserved')
+                call step_residual(istep, time_all, RoRes)             ! the reference's converge.dat
+                call Residual(res(1:Nvar), romax)
+                res(1:Nvar) = res(1:Nvar) / total_volume
+                write(94) dble(istep), res(1:Nvar), romax
+            endif
             if (istop == 1) exit
         enddo
         close(93)
+        close(94)
     else
     !! the main loop, steady implicit branch (fvm2d.f90:48-150)
     do istep = 1, nstep_max

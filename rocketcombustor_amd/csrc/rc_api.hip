@@ -1134,7 +1134,23 @@ int rc_backward_euler_stage(rc_case *h, double dt, int innerloop, double reltol,
         double ro = 0.0;
         for (int k = kNFlow; k < kNvar; ++k) ro = ro + h->h_res[k];
         if (inner == 1) ro1 = ro;
-        if (ro < ro1 * reltol || inner == innerloop) break;
+        if (ro < ro1 * reltol || inner == innerloop) {
+            // the closing reconstruction + flux_sourceterm of the converged
+            // inner loop (Time_BackwardEuler.f90:36-38): hWA and GradC then
+            // hold the residual and gradient of the new state, which the
+            // main loop's step_residual reads (fvm2d.f90:137)
+            launch_gradlim(L, true, 2, h->stream);
+            launch_flux(L, false, h->stream);
+            if (h->viscous) launch_viscous(L, false, true, h->stream);
+            if (h->sur) {
+                surrogate_enqueue(h->sur, h->d_case, h->ne, h->stream);
+                h->sur_evals += h->ne;
+            }
+            launch_residual(L, false, h->stream);
+            rc = sync_check(h);
+            if (rc) return rc;
+            break;
+        }
         // the NaN check of backwardEuler_stage (Time_BackwardEuler.f90:44-47)
         double wa11 = 0.0;
         HIPCHK(h, hipMemcpyAsync(&wa11, h->S.wa, sizeof(double), hipMemcpyDeviceToHost, h->stream));

@@ -222,14 +222,17 @@ class SteadyCase:
 
     def run_unsteady(self, nsteps: int, dt: Optional[float] = None, innerloop: Optional[int] = None,
                      reltol: Optional[float] = None, a_p: Optional[float] = None, f_p: Optional[float] = None,
-                     ttime: float = 0.0) -> np.ndarray:
+                     ttime: float = 0.0, step_rows: Optional[list] = None) -> np.ndarray:
         """The unsteady branch of the main loop with backward Euler
         (src/fvm2d.f90:48-120): per step ttime += dt, compute_p_out, the
         stage, and compute_p_out at the step's end time.  Parameters default
         to the case's solver settings (dt = totaltime/nstep_max, InnerLoop,
         relTol, A_p, f_p); p_static is the case's exit pressure.  Returns
         rows (istep, inner, Res/total_volume, max rho) of every inner
-        iteration; self.ttime holds the time reached."""
+        iteration; self.ttime holds the time reached.  With ``step_rows``
+        the step_residual norms after each step (fvm2d.f90:137, of the
+        residual the stage's closing flux_sourceterm leaves) are appended
+        to it as (istep, Res/total_volume, max rho)."""
         info = self.case.info
         dt = info.totaltime / info.nstep_max if dt is None else dt
         innerloop = info.innerloop if innerloop is None else innerloop
@@ -245,6 +248,8 @@ class SteadyCase:
             h = self.backward_euler_stage(dt, innerloop, reltol)
             rows += [np.r_[istep, k + 1, h[k]] for k in range(len(h))]
             ttime = tn + dt
+            if step_rows is not None:
+                step_rows.append(np.r_[istep, self.residual()])
         self.set_outlet_pressure(outlet_pressure(p_static, a_p, f_p, ttime))
         self.ttime = ttime
         return np.array(rows).reshape(-1, NVAR + 3)

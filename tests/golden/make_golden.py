@@ -8,7 +8,9 @@ PA) and what the reference computed on them:
 * for the first ``ndump`` steps, every per-call array of ``lu_sgs_stage``:
   step, alpaF, GPA, unlimited and limited GradC, the residual hWA after
   flux_sourceterm, dW after LU_SGS_steady, PA and WA;
-* the step_residual norms of every step and PA/WA after ``nsteps`` steps.
+* the step_residual norms of every step and PA/WA after ``nsteps`` steps
+  (backward-Euler fixtures: every inner iteration's norms, and the
+  step_residual norms after each step's closing flux_sourceterm).
 
 Run:  python tests/golden/make_golden.py   (needs oracle/_ref/rc_ref_dump and
 the product library for the multicolour ordering utility)
@@ -76,6 +78,10 @@ def run_reference(case: nozzle.Case, nsteps: int, ndump: int, expect_error: bool
         # unsteady rows: (istep, inner, Res(1:Nvar), Romax)
         resid = np.fromfile(os.path.join(od, "resid.bin")).reshape(-1, nv + (3 if unsteady else 1))
         d["ref_resid"] = resid
+        if unsteady:
+            # step_residual rows (istep, Res(1:Nvar)/total_volume, Romax) after
+            # each step's closing reconstruction + flux_sourceterm
+            d["ref_step_resid"] = np.fromfile(os.path.join(od, "resid_step.bin")).reshape(-1, nv + 2)
         if stopped:
             d["ref_error_step"] = len(resid) + 1
             d["ref_error_message"] = np.array(r.stdout.strip().splitlines()[-2:])
@@ -122,6 +128,33 @@ def viscous(case: nozzle.Case) -> nozzle.Case:
 
 
 ONLY = set(sys.argv[1:])     # fixture names to (re)generate; all when empty
+
+
+def make_trajectory(name: str, case: nozzle.Case, nsteps: int) -> None:
+    """A whole-run fixture: the case inputs, the reference's step_residual
+    rows of every step and its final PA after ``nsteps`` steady steps of
+    ``lu_sgs_stage`` (no per-call or set-up dumps, so it stays small at
+    config sizes)."""
+    if ONLY and name not in ONLY:
+        return
+    tmp = tempfile.mkdtemp(prefix="rcgold_")
+    try:
+        nozzle.write_reference_case(tmp, case, nstep_max=nsteps, extra_decks=REF_DECKS)
+        od = os.path.join(tmp, "out")
+        os.makedirs(od)
+        r = subprocess.run([REF_BIN, str(nsteps), "0", "out", "2"], cwd=tmp, capture_output=True, text=True)
+        if r.returncode != 0 or not os.path.exists(os.path.join(od, "final_PA.bin")):
+            raise RuntimeError("reference run failed:\n" + r.stdout[-2000:] + r.stderr[-2000:])
+        nv = case.mech.nvar
+        d = {"ref_resid": np.fromfile(os.path.join(od, "resid.bin")).reshape(-1, nv + 1),
+             "ref_final_PA": np.fromfile(os.path.join(od, "final_PA.bin")).reshape(case.mesh.ne, nv),
+             "resid_every": 1, "nsteps": nsteps}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    d.update(case_arrays(case))
+    path = os.path.join(OUT, name + ".npz")
+    np.savez_compressed(path, **d)
+    print("%-28s NE=%6d steps=%6d  %7.1f kB" % (name, case.mesh.ne, nsteps, os.path.getsize(path) / 1024), flush=True)
 
 
 def make(name: str, case: nozzle.Case, nsteps: int, ndump: int, every: int = 1, expect_error: bool = False) -> None:
@@ -176,6 +209,11 @@ def main() -> None:
     make("n30x8_converged", conv, nsteps=8000, ndump=0, every=10)
     perm, _ = native.order_multicolor(conv.mesh.cells)
     make("n30x8_multicolor_converged", conv.permuted(perm), nsteps=8000, ndump=0, every=10)
+    # the plain-method workload at its size: the 12k-triangle nozzle (150x40),
+    # block-4 order, CFL 5, 2000 steps from the initial field
+    c1 = nozzle.nozzle_case(150, 40)
+    perm, _ = native.order_multicolor(c1.mesh.cells)
+    make_trajectory("n150x40_config1_2000", c1.permuted(perm), nsteps=2000)
 
 
 if __name__ == "__main__":

@@ -13,10 +13,23 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def fixture_names(kind: str = "steady"):
-    """Golden fixtures of the steady path (``steady``) or of the dual-time
-    backward Euler path (``be``)."""
+    """Golden fixtures by kind: ``be`` the dual-time backward Euler path;
+    ``steady`` steady fixtures with the reference's set-up arrays;
+    ``percall`` those of them that also hold per-call dumps (not the
+    8000-step converged runs); ``trajectory`` every steady whole-run fixture,
+    including the config-size runs that hold only residual rows and the
+    final state."""
     names = sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
-    return [n for n in names if n.startswith("n24x6_be_") == (kind == "be")]
+    be = [n for n in names if n.startswith("n24x6_be_")]
+    if kind == "be":
+        return be
+    steady = [n for n in names if n not in be]
+    if kind == "trajectory":
+        return steady
+    steady = [n for n in steady if "_config" not in n]
+    if kind == "percall":
+        return [n for n in steady if "converged" not in n]
+    return steady
 
 
 def load(name: str) -> dict:
@@ -122,14 +135,16 @@ def outlet_pressure(p_static, a_p, f_p, t):
     return p_static * (1. + a_p * math.sin(2 * math.pi * f_p * t))
 
 
-def run_be(o, case, nsteps, on_inner=None, rows=None, lusgs="be_lusgs", set_p="set_p_out"):
+def run_be(o, case, nsteps, on_inner=None, rows=None, lusgs="be_lusgs", set_p="set_p_out", step_rows=None):
     """The reference's time loop (src/fvm2d.f90:60-110) around
     backwardEuler_stage (src/Time_BackwardEuler.f90:4-56), driven call by
     call on ``o`` (the oracle, or a device case with ``lusgs`` /
     ``set_p`` naming its methods): rows (istep, inner, Res/total_volume,
     Romax) of every inner iteration, appended to ``rows`` as they come;
     ``on_inner(kin, "res" | "lu")`` after flux_sourceterm and after the
-    point-implicit update of the kin-th inner iteration overall."""
+    point-implicit update of the kin-th inner iteration overall; rows
+    (istep, Res/total_volume, Romax) of step_residual after every step are
+    appended to ``step_rows``."""
     dt, inner, reltol, a_p, f_p = be_params(case)
     ttime, kin = 0.0, 0
     rows = [] if rows is None else rows
@@ -154,6 +169,14 @@ def run_be(o, case, nsteps, on_inner=None, rows=None, lusgs="be_lusgs", set_p="s
             if it == 1:
                 ro1 = ro
             if ro < ro1 * reltol or it == inner:
+                # the closing reconstruction + flux_sourceterm of
+                # backwardEuler_stage (Time_BackwardEuler.f90:36-38)
+                o.reconstruction()
+                o.flux_sourceterm()
                 break
         ttime = tn + dt
+        if step_rows is not None:
+            # step_residual after the stage (fvm2d.f90:137): norms of the
+            # residual the closing flux_sourceterm left in hWA
+            step_rows.append(np.r_[istep, o.residual()])
     return np.array(rows)

@@ -120,12 +120,18 @@ def test_run_unsteady_matches_reference(name):
     case = case_from_fixture(d)
     announce("backward Euler run_unsteady %s" % name)
     with SteadyCase(case) as g:
-        rows = g.run_unsteady(int(d["nsteps"]))
+        step_rows = []
+        rows = g.run_unsteady(int(d["nsteps"]), step_rows=step_rows)
         ref = d["ref_resid"]
         assert rows.shape == ref.shape
         assert np.array_equal(rows[:, :2], ref[:, :2])
         assert relerr(rows[:, 2:], ref[:, 2:], -1) < 1e-12
         assert relerr(g.primitive(), d["ref_final_PA"], -1) < 1e-12
+        # step_residual after each stage (the residual of the new state the
+        # stage's closing flux_sourceterm leaves in hWA)
+        sr, ref_sr = np.array(step_rows), d["ref_step_resid"]
+        assert sr.shape == ref_sr.shape
+        assert relerr(sr[:, 1:], ref_sr[:, 1:], -1) < 1e-12
 
 
 def test_stopping_case_raises_at_the_reference_step():

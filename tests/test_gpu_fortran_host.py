@@ -39,12 +39,18 @@ def test_reference_host_with_gpu_step(name):
         unsteady = not case.info.steady
         res = np.fromfile(os.path.join(tmp, "out", "resid.bin")).reshape(-1, 10 if unsteady else 8)
         pa = np.fromfile(os.path.join(tmp, "out", "final_PA.bin")).reshape(case.mesh.ne, 7)
-        assert os.path.exists(os.path.join(tmp, "converge_uns.dat" if unsteady else "converge.dat"))
+        assert os.path.exists(os.path.join(tmp, "converge.dat"))
+        if unsteady:
+            assert os.path.exists(os.path.join(tmp, "converge_uns.dat"))
+            # step_residual after every stage (NScreenOutput = 1)
+            sr = np.fromfile(os.path.join(tmp, "out", "resid_step.bin")).reshape(-1, 9)
         assert os.path.exists(os.path.join(tmp, "fvp1.sav"))
     ref = d["ref_resid"]
     assert res.shape == ref.shape
     if unsteady:
         assert np.array_equal(res[:, :2], ref[:, :2])   # same inner-iteration counts
         res, ref = res[:, 2:], ref[:, 2:]
+        assert sr.shape == d["ref_step_resid"].shape
+        assert relerr(sr[:, 1:], d["ref_step_resid"][:, 1:], -1) < 1e-10
     assert relerr(res[:, :7], ref[:, :7], -1) < 1e-10
     assert relerr(pa, d["ref_final_PA"], -1) < 1e-10

@@ -18,7 +18,8 @@ from rocketcombustor_amd.solver import SteadyCase
 oracle = pytest.importorskip("oracle.pyoracle")
 
 pytestmark = pytest.mark.gpu
-NAMES = fixture_names()
+NAMES = fixture_names("percall")
+TRAJ = fixture_names("trajectory")
 
 
 def announce(msg: str) -> None:
@@ -55,7 +56,7 @@ def test_per_call_vs_reference(name):
             s += 1
 
 
-@pytest.mark.parametrize("name", NAMES)
+@pytest.mark.parametrize("name", TRAJ)
 def test_trajectory_vs_reference(name):
     d = load(name)
     case = case_from_fixture(d)

@@ -153,8 +153,15 @@ def test_inner_iterations_match_reference(fx):
         with pytest.raises(RuntimeError, match="temperature"):
             run_be(o, case, nsteps, check)
         return
-    rows = run_be(o, case, nsteps, check)
+    step_rows = []
+    rows = run_be(o, case, nsteps, check, step_rows=step_rows)
     assert seen, "no dumped inner iteration"
+    # step_residual after each stage: the residual of the new state that the
+    # closing reconstruction + flux_sourceterm leave (Time_BackwardEuler.f90:36-38)
+    sr, ref_sr = np.array(step_rows), d["ref_step_resid"]
+    assert sr.shape == ref_sr.shape
+    assert np.array_equal(sr[:, 0], ref_sr[:, 0])
+    assert relerr(sr[:, 1:], ref_sr[:, 1:], -1) < 1e-12
     assert rows.shape == ref.shape
     assert np.array_equal(rows[:, :2], ref[:, :2])      # same inner-iteration counts
     assert relerr(rows[:, 2:], ref[:, 2:], -1) < 1e-12
