@@ -70,6 +70,8 @@ struct DevState {
     double *fvis;    // viscous path, kRecF per face: alpa, Fv(2:Nvar), alpaVis
     double *src;     // learned-chemistry source: omega_1..omega_{Nsp-1}, arrays of sc (null when exact)
     int *err;        // [0] code, [1] index, [2] kernel, [3] step of the error, [4] current step
+    int *flag;       // per cell: sweep progress of the dataflow sweep, 2e+1 after the forward
+                     // and 2e+2 after the backward sweep of step e (step counter err[4])
 };
 
 // one case as the kernels see it; an array of these (one per resident case)
@@ -96,6 +98,25 @@ struct WorkList {
     int bs = 256;                  // threads per block of the table form
 };
 
+// The dataflow sweep: every LU-SGS item of a step (forward levels in order,
+// then backward levels in reverse order) in one launch.  Items are split into
+// kXcdLists sub-lists, each in increasing order of the whole list (a
+// topological order of the sweep); thread block b takes the next item of
+// sub-list b % kXcdLists from an atomic counter, so a block waits only for
+// items handed out before its own, and the blocks that serve every counter
+// are the first ones dispatched: the launch cannot deadlock, whatever number
+// of blocks is resident.
+constexpr int kXcdLists = 8;
+struct FlowSpan {
+    int off[kXcdLists + 1];        // sub-list x: items[off[x] .. off[x+1])
+};
+struct FlowList {
+    const int4 *w = nullptr;       // (case, first run, end run, 1 forward / 0 backward)
+    FlowSpan span{};
+    int *ctr = nullptr;            // kXcdLists counters, zeroed by k_step_begin
+    int grid = 0;                  // thread blocks of the launch (0: no dataflow sweep)
+};
+
 // everything one step launches over a set of cases
 struct LaunchSet {
     const CaseDev *cases = nullptr;
@@ -105,6 +126,7 @@ struct LaunchSet {
     WorkList norm;                 // cell chunks owning the residual partial slots
     std::vector<WorkList> level;   // LU-SGS runs of each level
     std::vector<WorkList> pi_level;   // point-implicit sweep: cells (positions in pi_cells) of each level
+    FlowList flow;                 // the whole LU-SGS sweep of the fused step as one launch
     double *resout = nullptr;      // ncases x (kNvar+1) residual norms
 };
 
@@ -121,7 +143,7 @@ struct GMesh {
 };
 struct GState {
     RC_G double *pa, *wa, *hwa, *gradc, *gpa, *step, *dd, *fflux, *lucoef, *lu_af, *fixed, *fvis, *src;
-    RC_G int *err;
+    RC_G int *err, *flag;
 };
 typedef const RC_G DevThermo GThermo;
 typedef const RC_G DevParams GParams;
@@ -150,6 +172,7 @@ __device__ __forceinline__ GState gview(const RC_G DevState &s) {
     g.fvis = gp(s.fvis);
     g.src = gp(s.src);
     g.err = gp(s.err);
+    g.flag = gp(s.flag);
     return g;
 }
 
@@ -158,7 +181,7 @@ constexpr int kRunThreads = 64;       // threads per block of the LU-SGS level k
 constexpr int kRunChunk = kRunThreads / kRunMax;   // LU-SGS runs per thread block
 
 enum KernelId { K_TS_FACE = 1, K_TS_CELL, K_GHOST, K_GRADLIM, K_FLUX, K_RES, K_FWD, K_BWD, K_UPDATE, K_P2C,
-                K_BE_FWD, K_BE_BWD, K_BE_UPDATE, K_VISC };
+                K_BE_FWD, K_BE_BWD, K_BE_UPDATE, K_VISC, K_FLOW };
 
 // first error wins; the step number comes from the device-side step
 // counter err[4] (advanced by k_step_begin), so captured graphs report it too

@@ -6,6 +6,8 @@
 // ascending face number (NeighF order), which is the order in which the
 // reference's face loops accumulate into the same cell; the LU-SGS sweeps
 // run level by level over wavefronts of the given cell order.
+#include <algorithm>
+
 #include "rc_items.h"
 
 namespace rc {
@@ -80,12 +82,14 @@ __global__ void __launch_bounds__(256) k_timestep_cell(ITEM_ARGS) {
 }
 
 // advances each case's device step counter used in error reports
-__global__ void k_step_begin(const CaseDev *__restrict__ cases, int n) {
+// and clears the item counters of the dataflow sweep that follows
+__global__ void k_step_begin(const CaseDev *__restrict__ cases, int n, int *__restrict__ ctr) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < n) {
         RC_G int *err = gp(gp(cases)[c].S.err);
         err[4] = err[4] + 1;
     }
+    if (ctr && c < kXcdLists) gp(ctr)[c] = 0;
 }
 
 // update_ghostcell_variables (update_flow_variables.f90:90-269): the state
@@ -477,43 +481,58 @@ __device__ __forceinline__ void lusgs_term(TT &th, const LuNb &n, const double *
     for (int k = 0; k < NSP - 1; ++k) c[kNFlow + k] = 0.5 * (-af * dry[k] + dry[k] * un + n.ry[k] * dun);
 }
 
-// one level of the forward (FWD) or backward sweep of LU_SGS_steady
-// (Time_LUSGS_steady.f90:56-120 / 127-190) over the runs of each work item
-// (kRunChunk runs of one case per thread block).  A run is up
-// to kRunMax consecutive cells swept in order; it is handled by a group of
-// kRunMax lanes, lane p owning cell first+p.  Each lane loads the records
-// of its cell's neighbours at once and forms the terms of neighbours outside
-// the run (lower levels, already final).  Then the run's cells are finished
-// one step at a time in the sweep direction: the lane of the step adds the
-// terms of neighbours inside the run, whose new increments the group has
-// received by shuffles in the steps before, and forms its increment.  Terms
-// enter dflux in face order; a face that does not take part adds +0.0, which
+// Loads and stores of the dataflow sweep: increments and progress flags
+// pass between thread blocks of one launch, which may run on different XCDs
+// (whose L2 caches are not coherent with each other), so they are
+// agent-scope atomic accesses (the L2 of the reading XCD does not answer
+// them from a stale line); plain arrays written by earlier launches are
+// read as usual.
+__device__ __forceinline__ double ld_agent(const RC_G double *p) {
+    return __hip_atomic_load((RC_G double *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(RC_G double *p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_agent(const RC_G int *p) {
+    return __hip_atomic_load((RC_G int *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(RC_G int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// polls of one dependency wait before the sweep gives up with an error
+// (a wait normally ends within the sweep's own duration, tens of us)
+constexpr int kSpinMax = 1 << 18;
+
+// The forward (FWD) or backward sweep of LU_SGS_steady
+// (Time_LUSGS_steady.f90:56-120 / 127-190) over runs t .. tend-1 of case C.
+// A run is up to kRunMax consecutive cells swept in order; it is handled by
+// a group of kRunMax lanes, lane p owning cell first+p.  Each lane loads the
+// records of its cell's neighbours at once and forms the terms of
+// neighbours outside the run (swept before: lower levels in the forward
+// sweep, higher in the backward one).  Then the run's cells are finished one
+// step at a time in the sweep direction: the lane of the step adds the terms
+// of neighbours inside the run, whose new increments the group has received
+// by shuffles in the steps before, and forms its increment.  Terms enter
+// dflux in face order; a face that does not take part adds +0.0, which
 // leaves the reference's sums unchanged.
 // RES (forward sweep of the fused step): each lane first forms its cell's
 // TimeStep cell part and residual (what k_residual stores) and sweeps with
 // them from registers.
-template <int NSP, bool FWD, bool RES, bool B>
-__global__ void __launch_bounds__(256) k_lusgs(ITEM_ARGS) {
+// FLOW (the dataflow sweep, all levels in one launch): before it reads a
+// neighbour's increment (and, in the backward sweep, its own forward one) a
+// lane waits until that cell's progress flag shows the sweep that wrote it
+// done in step `epoch`; increments are written before the flag (a wait for
+// the stores in between).  The backward sweep forms the TimeStep cell part
+// again from the face records instead of reading what the forward sweep of
+// another block stored (same arithmetic, same values).
+template <int NSP, bool FWD, bool RES, bool FLOW>
+__device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int tend, int epoch) {
     constexpr int R = kRunMax;
-    const RC_G CaseDev *Cp_;
-    int t, tend;
-    if constexpr (B) {
-        const int4 w_ = work[blockIdx.x];
-        Cp_ = gp(cases) + w_.x;
-        t = w_.y + (int)threadIdx.x / R;
-        tend = w_.z;
-    } else {
-        Cp_ = gp(cases);
-        t = lo + (int)(blockIdx.x * blockDim.x + threadIdx.x) / R;
-        tend = hi;
-    }
-    const RC_G CaseDev &C = *Cp_;
     const GMesh M = gview(C.M);
     const GState S = gview(C.S);
-    GThermo &TH = *gp(C.th);
+    GThermo &th = *gp(C.th);
     GParams &PAR = *gp(C.par);
     const int lane = threadIdx.x % R;
-    GThermo &th = TH;
     const int sc = M.sc;
     // every lane stays to the end: the group's shuffles need all four
     int first = 0, cnt = 0;
@@ -532,12 +551,38 @@ __global__ void __launch_bounds__(256) k_lusgs(ITEM_ARGS) {
     if (RES) {
         d = cell_step_dd(M, S, PAR, i, af, live);
         cell_residual<NSP>(M, S, th, PAR, i, rhs);
+    } else if (FLOW) {
+        d = cell_step_dd(M, S, PAR, i, af, false);
     } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k) af[k] = S.lu_af[k * sc + i];
         d = S.dd[i];
 #pragma unroll
         for (int v = 0; v < NV; ++v) rhs[v] = HWA_(v, i);
+    }
+    if (FLOW) {
+        // wait for the increments this lane reads from outside its run
+        const int need = FWD ? 2 * epoch + 1 : 2 * epoch + 2;
+        for (int it = 0;; ++it) {
+            bool ready = true;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int nr = nb[k];
+                const bool ext = live && nr >= 0 && (FWD ? nr < i : nr > i) && !(nr >= first && nr < first + cnt);
+                if (ext) ready = ld_agent(S.flag + nr) >= need && ready;
+            }
+            if (!FWD && live) ready = ld_agent(S.flag + i) >= 2 * epoch + 1 && ready;
+            if (ready) break;
+            if (it >= kSpinMax || ld_agent(S.err) != 0) {
+                if (it >= kSpinMax) flag_error(S.err, RC_ERR_DEVICE, i, K_FLOW);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (!FWD) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) rhs[v] = live ? ld_agent(&HWA_(v, i)) : 0.0;
+        }
     }
     bool use[3];
     int qs[3];         // sweep step of a neighbour inside the run, else -1
@@ -553,11 +598,16 @@ __global__ void __launch_bounds__(256) k_lusgs(ITEM_ARGS) {
         // increments that come from the run: one shared line, not one per
         // lane, and never a record this launch writes
         const int j = use[k] ? nr : 0;
-        const RC_G double *hw = (use[k] && !inrun) ? &HWA_(0, nr) : &LC_(0, 0);
         nbr[k] = load_lunb(S, j);
         double dw[NV];
+        if (FLOW && use[k] && !inrun) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) dw[v] = hw[v];
+            for (int v = 0; v < NV; ++v) dw[v] = ld_agent(&HWA_(v, nr));
+        } else {
+            const RC_G double *hw = (use[k] && !inrun) ? &HWA_(0, nr) : &LC_(0, 0);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) dw[v] = hw[v];
+        }
         lusgs_term<NSP>(th, nbr[k], dw, nx[k], ny[k], af[k], c[k]);
     }
     const int mystep = FWD ? lane : cnt - 1 - lane;
@@ -601,8 +651,62 @@ __global__ void __launch_bounds__(256) k_lusgs(ITEM_ARGS) {
         }
     }
     if (live) {
+        if (FLOW) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) HWA_(v, i) = hv[v];
+            for (int v = 0; v < NV; ++v) st_agent(&HWA_(v, i), hv[v]);
+            // the increment's stores complete before the flag is raised
+            __builtin_amdgcn_s_waitcnt(0x0f70);   // s_waitcnt vmcnt(0)
+            st_agent(S.flag + i, FWD ? 2 * epoch + 1 : 2 * epoch + 2);
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) HWA_(v, i) = hv[v];
+        }
+    }
+}
+
+// one level of the sweep: kRunChunk runs of one case per thread block (the
+// work item), or runs lo + ... below hi of the one case
+template <int NSP, bool FWD, bool RES, bool B>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_lusgs(ITEM_ARGS) {
+    constexpr int R = kRunMax;
+    const RC_G CaseDev *Cp_;
+    int t, tend;
+    if constexpr (B) {
+        const int4 w_ = work[blockIdx.x];
+        Cp_ = gp(cases) + w_.x;
+        t = w_.y + (int)threadIdx.x / R;
+        tend = w_.z;
+    } else {
+        Cp_ = gp(cases);
+        t = lo + (int)(blockIdx.x * blockDim.x + threadIdx.x) / R;
+        tend = hi;
+    }
+    lusgs_runs<NSP, FWD, RES, false>(*Cp_, t, tend, 0);
+}
+
+// The dataflow sweep: forward sweep (with the residual) and backward sweep
+// of every case of the launch set in one launch.  Each thread block (one
+// wave, kRunChunk runs) takes items from its counter until its sub-list is
+// done; the counter's next value is fetched while the current item runs.
+template <int NSP>
+__global__ void __launch_bounds__(kRunThreads) __attribute__((amdgpu_waves_per_eu(2))) k_lusgs_flow(const CaseDev *__restrict__ cases,
+                                                            const int4 *__restrict__ items, FlowSpan span,
+                                                            int *__restrict__ ctr) {
+    const int x = blockIdx.x % kXcdLists;
+    const int beg = span.off[x], len = span.off[x + 1] - beg;
+    RC_G int *cx = gp(ctr) + x;
+    int k = __builtin_amdgcn_readfirstlane(
+        threadIdx.x == 0 ? __hip_atomic_fetch_add(cx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0);
+    while (k < len) {
+        const int nk = threadIdx.x == 0 ? __hip_atomic_fetch_add(cx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                        : 0;
+        const int4 w = items[beg + k];
+        const RC_G CaseDev &C = gp(cases)[w.x];
+        const int epoch = gp(C.S.err)[4];
+        const int t = w.y + (int)threadIdx.x / kRunMax;
+        if (w.w) lusgs_runs<NSP, true, true, true>(C, t, w.z, epoch);
+        else lusgs_runs<NSP, false, false, true>(C, t, w.z, epoch);
+        k = __builtin_amdgcn_readfirstlane(nk);
     }
 }
 
@@ -776,7 +880,28 @@ void launch_gradlim(const LaunchSet &L, bool grad, int limit, hipStream_t st) {
 }
 
 void launch_step_begin(const LaunchSet &L, hipStream_t st) {
-    k_step_begin<<<(L.ncases + 63) / 64, 64, 0, st>>>(L.cases, L.ncases);
+    k_step_begin<<<(std::max(L.ncases, kXcdLists) + 63) / 64, 64, 0, st>>>(L.cases, L.ncases, L.flow.ctr);
+}
+
+// the whole LU-SGS sweep of the fused step (forward with the residual, then
+// backward) as one launch; needs k_step_begin before it in the same stream
+void launch_lusgs_flow(const LaunchSet &L, hipStream_t st) {
+    if (L.flow.grid > 0)
+        k_lusgs_flow<kNsp><<<L.flow.grid, kRunThreads, 0, st>>>(L.cases, L.flow.w, L.flow.span, L.flow.ctr);
+}
+
+// blocks of the dataflow sweep resident at once on the device (its
+// occupancy times the CU count), at least kXcdLists
+int flow_max_blocks(int device) {
+    static int cache[64] = {};
+    if (device < 0 || device >= 64) return kXcdLists;
+    if (cache[device]) return cache[device];
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lusgs_flow<kNsp>, kRunThreads, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return kXcdLists;
+    cache[device] = std::max(kXcdLists, per_cu * cus / kXcdLists * kXcdLists);
+    return cache[device];
 }
 
 void launch_flux(const LaunchSet &L, bool ts, hipStream_t st) {

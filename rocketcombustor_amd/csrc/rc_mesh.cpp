@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <numeric>
 #include <queue>
+#include <tuple>
 #include <unordered_map>
 
 #include "rc_internal.h"
@@ -544,16 +545,169 @@ int order_multicolor(int ne, const int *cells0, int block, int *perm, int *ncolo
             const int v = adj[3 * u + t];
             if (blk[u] != blk[v]) badj[blk[u]].push_back(blk[v]);
         }
-    std::vector<int> bcol(nblk, -1);
-    int nc = 0;
-    for (int b = 0; b < nblk; ++b) {
-        uint64_t used = 0;
-        for (int v : badj[b])
-            if (bcol[v] >= 0 && bcol[v] < 64) used |= 1ull << bcol[v];
-        int c = 0;
-        while (c < 64 && (used & (1ull << c))) ++c;
-        bcol[b] = c;
-        nc = std::max(nc, c + 1);
+    for (auto &v : badj) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    // Colouring of the block graph: the number of colours is the number of
+    // sweep levels, each a chain of dependent launches, so fewer is faster
+    // (a planar block graph needs 4 at least).  Greedy in RCM order first;
+    // then DSatur (the most-constrained block next); then iterated greedy
+    // passes (Culberson: re-colour greedily taking the colour classes in
+    // reverse order, which never needs more colours), keeping the best.
+    auto greedy = [&](const std::vector<int> &seq, std::vector<int> &col) {
+        col.assign(nblk, -1);
+        int n = 0;
+        for (int b : seq) {
+            uint64_t used = 0;
+            for (int v : badj[b])
+                if (col[v] >= 0 && col[v] < 64) used |= 1ull << col[v];
+            int c = 0;
+            while (c < 64 && (used & (1ull << c))) ++c;
+            col[b] = c;
+            n = std::max(n, c + 1);
+        }
+        return n;
+    };
+    std::vector<int> seq(nblk);
+    std::iota(seq.begin(), seq.end(), 0);
+    std::vector<int> bcol;
+    int nc = greedy(seq, bcol);
+    {
+        // DSatur with a lazy max-heap keyed by (saturation, degree, -index)
+        std::vector<int> col(nblk, -1), sat(nblk, 0);
+        std::vector<uint64_t> seen_c(nblk, 0);
+        std::priority_queue<std::tuple<int, int, int>> pq;
+        for (int b = 0; b < nblk; ++b) pq.emplace(0, (int)badj[b].size(), -b);
+        int n = 0;
+        while (!pq.empty()) {
+            const auto [sb, db, nbb] = pq.top();
+            pq.pop();
+            const int b = -nbb;
+            if (col[b] >= 0 || sb != sat[b]) continue;
+            int c = 0;
+            while (c < 64 && (seen_c[b] & (1ull << c))) ++c;
+            col[b] = c;
+            n = std::max(n, c + 1);
+            for (int v : badj[b])
+                if (col[v] < 0 && c < 64 && !(seen_c[v] & (1ull << c))) {
+                    seen_c[v] |= 1ull << c;
+                    pq.emplace(++sat[v], (int)badj[v].size(), -v);
+                }
+        }
+        if (n < nc) {
+            nc = n;
+            bcol = col;
+        }
+    }
+    for (int pass = 0; pass < 32 && nc > 4; ++pass) {
+        // colour classes in reverse order, blocks of a class in RCM order
+        std::vector<int> s2(nblk);
+        std::iota(s2.begin(), s2.end(), 0);
+        std::stable_sort(s2.begin(), s2.end(), [&](int a, int b) { return bcol[a] > bcol[b]; });
+        std::vector<int> col;
+        const int n = greedy(s2, col);
+        if (n > nc) break;
+        nc = n;
+        bcol = col;
+    }
+    if (nc > 4) {
+        // four colours by local search (tabu min-conflicts, Hertz and de
+        // Werra's TabuCol): blocks of colour >= 4 take the colour with
+        // fewest conflicts, then a conflicting block moves to the colour
+        // with fewest conflicts that is not tabu, until none conflicts;
+        // deterministic (SplitMix64 stream), kept only if it succeeds
+        constexpr int K = 4;
+        std::vector<int> col(bcol);
+        std::vector<int> conf((size_t)nblk * K, 0);
+        for (int b = 0; b < nblk; ++b)
+            if (col[b] >= K) col[b] = -1;
+        auto cnt = [&](int b, int c) -> int & { return conf[(size_t)b * K + c]; };
+        for (int b = 0; b < nblk; ++b)
+            if (col[b] >= 0)
+                for (int v : badj[b]) ++cnt(v, col[b]);
+        for (int b = 0; b < nblk; ++b)
+            if (col[b] < 0) {
+                int best = 0;
+                for (int c = 1; c < K; ++c)
+                    if (cnt(b, c) < cnt(b, best)) best = c;
+                col[b] = best;
+                for (int v : badj[b]) ++cnt(v, best);
+            }
+        std::vector<int> bad, pos(nblk, -1);
+        auto refresh = [&](int b) {
+            const bool is_bad = cnt(b, col[b]) > 0;
+            if (is_bad && pos[b] < 0) {
+                pos[b] = (int)bad.size();
+                bad.push_back(b);
+            } else if (!is_bad && pos[b] >= 0) {
+                const int last = bad.back();
+                bad[pos[b]] = last;
+                pos[last] = pos[b];
+                bad.pop_back();
+                pos[b] = -1;
+            }
+        };
+        for (int b = 0; b < nblk; ++b) refresh(b);
+        std::vector<long> tabu((size_t)nblk * K, -1);
+        uint64_t rng = 0x9e3779b97f4a7c15ull;
+        auto next = [&]() {
+            uint64_t z = (rng += 0x9e3779b97f4a7c15ull);
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+            return z ^ (z >> 31);
+        };
+        const long max_iter = 200L * nblk + 100000;
+        for (long it = 0; it < max_iter && !bad.empty(); ++it) {
+            const int b = bad[next() % bad.size()];
+            const int old = col[b];
+            int best = -1;
+            for (int c = 0; c < K; ++c) {
+                if (c == old) continue;
+                const bool allowed = tabu[(size_t)b * K + c] < it || cnt(b, c) == 0;
+                if (allowed && (best < 0 || cnt(b, c) < cnt(b, best))) best = c;
+            }
+            if (best < 0) continue;
+            col[b] = best;
+            tabu[(size_t)b * K + old] = it + 7 + (long)(next() % 10);
+            for (int v : badj[b]) {
+                --cnt(v, old);
+                ++cnt(v, best);
+                refresh(v);
+            }
+            refresh(b);
+        }
+        if (bad.empty()) {
+            bcol = col;
+            nc = K;
+        }
+    }
+    {
+        // balance the colour classes (cells per sweep level): a level's
+        // blocks run as one round of waves while they fit on the device, so
+        // the largest class sets the cost; a block moves to the smallest
+        // class none of its neighbours is in while that evens the sizes
+        std::vector<long> wsz(nblk, 0), csz(nc, 0);
+        for (int u = 0; u < ne; ++u) ++wsz[blk[u]];
+        for (int b = 0; b < nblk; ++b) csz[bcol[b]] += wsz[b];
+        for (int pass = 0; pass < 8; ++pass) {
+            bool moved = false;
+            for (int b = 0; b < nblk; ++b) {
+                const int c0 = bcol[b];
+                uint64_t used = 0;
+                for (int v : badj[b]) used |= 1ull << bcol[v];
+                int best = -1;
+                for (int c = 0; c < nc; ++c)
+                    if (c != c0 && !(used & (1ull << c)) && (best < 0 || csz[c] < csz[best])) best = c;
+                if (best >= 0 && csz[best] + wsz[b] < csz[c0]) {
+                    csz[c0] -= wsz[b];
+                    csz[best] += wsz[b];
+                    bcol[b] = best;
+                    moved = true;
+                }
+            }
+            if (!moved) break;
+        }
     }
     std::vector<int> idx(ne);
     std::iota(idx.begin(), idx.end(), 0);

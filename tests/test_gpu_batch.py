@@ -59,6 +59,31 @@ def test_batch_equals_alone():
         g.close()
 
 
+def test_batch_dataflow_sweep_equals_alone(monkeypatch):
+    """The batch's whole LU-SGS sweep as one dataflow launch (RC_FLOW=1, all
+    members, forward and backward) gives every member bitwise what it
+    computes alone with level launches."""
+    cases = mixed_cases()
+    n, every = 30, 10
+    announce("batch dataflow sweep, %d mixed cases, %d steps" % (len(cases), n))
+    monkeypatch.setenv("RC_FLOW", "0")
+    ref = [alone(c, n, every) for c in cases]
+    gs = [SteadyCase(c) for c in cases]
+    monkeypatch.setenv("RC_FLOW", "1")
+    try:
+        with SteadyBatch() as b:
+            for g in gs:
+                b.add(g)
+            hist = b.run_steady(n, every=every)
+            for k, g in enumerate(gs):
+                assert np.array_equal(hist[:, k], ref[k][0]), k
+                assert np.array_equal(g.primitive(), ref[k][1]), k
+    finally:
+        monkeypatch.delenv("RC_FLOW")
+        for g in gs:
+            g.close()
+
+
 def test_continuous_batching_join_and_leave():
     """A, B from the start; C joins after 30 steps; A leaves after 50; each
     ends bitwise where it ends alone after the same number of steps."""

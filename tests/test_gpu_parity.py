@@ -79,7 +79,8 @@ def test_trajectory_vs_reference(name):
         assert np.max(rel[big]) < (1e-8 if nsteps > 100 else 1e-10)
         tol = 1e-8 if nsteps > 100 else 1e-10
         assert relerr(g.primitive(), d["ref_final_PA"], -1) < tol
-        assert relerr(g.conserved(), d["ref_final_WA"], -1) < tol
+        if "ref_final_WA" in d:
+            assert relerr(g.conserved(), d["ref_final_WA"], -1) < tol
 
 
 @pytest.mark.parametrize("name", ["n30x8_converged", "n30x8_multicolor_converged"])
@@ -168,6 +169,49 @@ def test_graph_replay_matches_eager():
         for _ in range(5):
             eager.lu_sgs_stage()
         assert np.array_equal(graph.conserved(), eager.conserved())
+
+
+@pytest.mark.parametrize("nx,ny,order", [(1, 1, "natural"), (24, 6, "natural"), (150, 40, "natural"),
+                                         (150, 40, "multicolor"), (1000, 200, "multicolor"),
+                                         (1000, 200, "natural")])
+def test_dataflow_sweep_equals_level_launches(nx, ny, order, monkeypatch):
+    """The dataflow sweep (forward and backward LU-SGS of the fused step in
+    one launch, thread blocks waiting on per-cell progress flags) gives
+    bitwise the results of one launch per level and direction, step after
+    step, eager and graph-replayed; the natural order has hundreds to
+    thousands of levels (long dependency chains)."""
+    case = nozzle.nozzle_case(nx, ny)
+    if order == "multicolor":
+        perm, _ = native.order_multicolor(case.mesh.cells)
+        case = case.permuted(perm)
+    announce("dataflow sweep vs level launches %dx%d %s NE=%d" % (nx, ny, order, case.mesh.ne))
+    monkeypatch.setenv("RC_FLOW", "0")
+    lev = SteadyCase(case)
+    monkeypatch.setenv("RC_FLOW", "1")
+    flow = SteadyCase(case)
+    monkeypatch.delenv("RC_FLOW")
+    try:
+        # the eager step with HIP events between the kernels (the dataflow
+        # sweep is timed in the forward slot)
+        flow.set_timing(True)
+        flow.lu_sgs_stage()
+        assert flow.timing()["lusgs_forward"] > 0
+        flow.set_timing(False)
+        lev.lu_sgs_stage()
+        assert np.array_equal(flow.primitive(), lev.primitive())
+        assert np.array_equal(flow.increment(), lev.increment())
+        for k in range(3):
+            flow.run_steady(1, history=False)
+            lev.run_steady(1, history=False)
+            assert np.array_equal(flow.primitive(), lev.primitive()), k
+            assert np.array_equal(flow.increment(), lev.increment()), k
+        flow.run_steady(6, history=False)
+        lev.run_steady(6, history=False)
+        assert np.array_equal(flow.conserved(), lev.conserved())
+        assert np.array_equal(flow.residual(), lev.residual())
+    finally:
+        flow.close()
+        lev.close()
 
 
 def test_operator_calls_after_fused_steps():
