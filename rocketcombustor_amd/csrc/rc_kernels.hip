@@ -431,25 +431,8 @@ __device__ __forceinline__ LuNb load_lunb(const GState &S, int nr) {
     return n;
 }
 
-// record k of three (k = 0, 1, 2), selected field by field so that the
-// three stay in registers
+// value k of three (k = 0, 1, 2)
 __device__ __forceinline__ double sel3(int k, double a, double b, double c) { return k == 0 ? a : (k == 1 ? b : c); }
-__device__ __forceinline__ LuNb select_lunb(int k, const LuNb *n) {
-    LuNb r;
-    r.rr = sel3(k, n[0].rr, n[1].rr, n[2].rr);
-    r.rou = sel3(k, n[0].rou, n[1].rou, n[2].rou);
-    r.rov = sel3(k, n[0].rov, n[1].rov, n[2].rov);
-    r.roep = sel3(k, n[0].roep, n[1].roep, n[2].roep);
-    r.uu = sel3(k, n[0].uu, n[1].uu, n[2].uu);
-    r.vv = sel3(k, n[0].vv, n[1].vv, n[2].vv);
-    r.tt = sel3(k, n[0].tt, n[1].tt, n[2].tt);
-    r.gama = sel3(k, n[0].gama, n[1].gama, n[2].gama);
-#pragma unroll
-    for (int j = 0; j < kNsp; ++j) r.h[j] = sel3(k, n[0].h[j], n[1].h[j], n[2].h[j]);
-#pragma unroll
-    for (int j = 0; j < kNsp - 1; ++j) r.ry[j] = sel3(k, n[0].ry[j], n[1].ry[j], n[2].ry[j]);
-    return r;
-}
 
 // the neighbour term of both LU-SGS sweeps (Time_LUSGS_steady.f90:88-111):
 // the linearised Rusanov flux change 0.5*(-alpaF*dW + dF(dW).S) of a
@@ -526,7 +509,7 @@ constexpr int kSpinMax = 1 << 18;
 // again from the face records instead of reading what the forward sweep of
 // another block stored (same arithmetic, same values).
 template <int NSP, bool FWD, bool RES, bool FLOW>
-__device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int tend, int epoch) {
+__device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int tend, int epoch, double *tl) {
     constexpr int R = kRunMax;
     const GMesh M = gview(C.M);
     const GState S = gview(C.S);
@@ -584,10 +567,11 @@ __device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int ten
             for (int v = 0; v < NV; ++v) rhs[v] = live ? ld_agent(&HWA_(v, i)) : 0.0;
         }
     }
+    // the terms of the cell's three faces, in LDS: tl[(k*8+v)*bs + tid]
+    // (face k, variable v); a face that does not take part holds +0.0
+    const int bs = blockDim.x, tid = threadIdx.x;
     bool use[3];
     int qs[3];         // sweep step of a neighbour inside the run, else -1
-    LuNb nbr[3];
-    double c[3][NV];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int nr = nb[k];
@@ -598,8 +582,8 @@ __device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int ten
         // increments that come from the run: one shared line, not one per
         // lane, and never a record this launch writes
         const int j = use[k] ? nr : 0;
-        nbr[k] = load_lunb(S, j);
-        double dw[NV];
+        const LuNb nbk = load_lunb(S, j);
+        double dw[NV], ck[NV];
         if (FLOW && use[k] && !inrun) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) dw[v] = ld_agent(&HWA_(v, nr));
@@ -608,7 +592,9 @@ __device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int ten
 #pragma unroll
             for (int v = 0; v < NV; ++v) dw[v] = hw[v];
         }
-        lusgs_term<NSP>(th, nbr[k], dw, nx[k], ny[k], af[k], c[k]);
+        lusgs_term<NSP>(th, nbk, dw, nx[k], ny[k], af[k], ck);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) tl[(k * 8 + v) * bs + tid] = use[k] ? ck[v] : 0.0;
     }
     const int mystep = FWD ? lane : cnt - 1 - lane;
     double hv[NV];
@@ -617,7 +603,8 @@ __device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int ten
 #pragma unroll
     for (int s = 0; s < R; ++s) {
         // the lane of step s: all its terms are ready (in-run neighbours were
-        // swept at earlier steps)
+        // swept at earlier steps); dflux adds the three faces' terms in face
+        // order, as the reference's face loop does
         if (mystep == s) {
             double dflux[NV];
 #pragma unroll
@@ -625,7 +612,7 @@ __device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int ten
 #pragma unroll
             for (int k = 0; k < 3; ++k)
 #pragma unroll
-                for (int v = 0; v < NV; ++v) dflux[v] = dflux[v] + (use[k] ? c[k][v] : 0.0);
+                for (int v = 0; v < NV; ++v) dflux[v] = dflux[v] + tl[(k * 8 + v) * bs + tid];
 #pragma unroll
             for (int v = 0; v < NV; ++v) hv[v] = FWD ? (rhs[v] - dflux[v]) / d : rhs[v] - dflux[v] / d;
         }
@@ -636,17 +623,17 @@ __device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int ten
 #pragma unroll
             for (int v = 0; v < NV; ++v) dw[v] = __shfl(hv[v], src, R);
             // every lane with that cell among its neighbours (one face at
-            // most) forms the term of that face now
+            // most) forms the term of that face now, from the neighbour's
+            // cache record (loaded again: it was read above)
             const int ks = qs[0] == s ? 0 : (qs[1] == s ? 1 : (qs[2] == s ? 2 : -1));
             if (ks >= 0) {
-                const LuNb nb_s = select_lunb(ks, nbr);
+                const int nrs = ks == 0 ? nb[0] : (ks == 1 ? nb[1] : nb[2]);
+                const LuNb nb_s = load_lunb(S, nrs);
                 double cs[NV];
                 lusgs_term<NSP>(th, nb_s, dw, sel3(ks, nx[0], nx[1], nx[2]), sel3(ks, ny[0], ny[1], ny[2]),
                                 sel3(ks, af[0], af[1], af[2]), cs);
 #pragma unroll
-                for (int k = 0; k < 3; ++k)
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) c[k][v] = ks == k ? cs[v] : c[k][v];
+                for (int v = 0; v < NV; ++v) tl[(ks * 8 + v) * bs + tid] = cs[v];
             }
         }
     }
@@ -681,7 +668,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         t = lo + (int)(blockIdx.x * blockDim.x + threadIdx.x) / R;
         tend = hi;
     }
-    lusgs_runs<NSP, FWD, RES, false>(*Cp_, t, tend, 0);
+    extern __shared__ double lusgs_tl[];
+    lusgs_runs<NSP, FWD, RES, false>(*Cp_, t, tend, 0, lusgs_tl);
 }
 
 // The dataflow sweep: forward sweep (with the residual) and backward sweep
@@ -692,6 +680,7 @@ template <int NSP>
 __global__ void __launch_bounds__(kRunThreads) __attribute__((amdgpu_waves_per_eu(2))) k_lusgs_flow(const CaseDev *__restrict__ cases,
                                                             const int4 *__restrict__ items, FlowSpan span,
                                                             int *__restrict__ ctr) {
+    extern __shared__ double lusgs_tl[];
     const int x = blockIdx.x % kXcdLists;
     const int beg = span.off[x], len = span.off[x + 1] - beg;
     RC_G int *cx = gp(ctr) + x;
@@ -704,8 +693,8 @@ __global__ void __launch_bounds__(kRunThreads) __attribute__((amdgpu_waves_per_e
         const RC_G CaseDev &C = gp(cases)[w.x];
         const int epoch = gp(C.S.err)[4];
         const int t = w.y + (int)threadIdx.x / kRunMax;
-        if (w.w) lusgs_runs<NSP, true, true, true>(C, t, w.z, epoch);
-        else lusgs_runs<NSP, false, false, true>(C, t, w.z, epoch);
+        if (w.w) lusgs_runs<NSP, true, true, true>(C, t, w.z, epoch, lusgs_tl);
+        else lusgs_runs<NSP, false, false, true>(C, t, w.z, epoch, lusgs_tl);
         k = __builtin_amdgcn_readfirstlane(nk);
     }
 }
@@ -879,6 +868,9 @@ void launch_gradlim(const LaunchSet &L, bool grad, int limit, hipStream_t st) {
     else if (limit == 1) LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, false, LIM_ALWAYS);
 }
 
+// LDS of a sweep block: three faces' terms (8 doubles each) per thread
+constexpr size_t lusgs_lds(int threads) { return (size_t)threads * 3 * 8 * sizeof(double); }
+
 void launch_step_begin(const LaunchSet &L, hipStream_t st) {
     k_step_begin<<<(std::max(L.ncases, kXcdLists) + 63) / 64, 64, 0, st>>>(L.cases, L.ncases, L.flow.ctr);
 }
@@ -887,7 +879,8 @@ void launch_step_begin(const LaunchSet &L, hipStream_t st) {
 // backward) as one launch; needs k_step_begin before it in the same stream
 void launch_lusgs_flow(const LaunchSet &L, hipStream_t st) {
     if (L.flow.grid > 0)
-        k_lusgs_flow<kNsp><<<L.flow.grid, kRunThreads, 0, st>>>(L.cases, L.flow.w, L.flow.span, L.flow.ctr);
+        k_lusgs_flow<kNsp><<<L.flow.grid, kRunThreads, lusgs_lds(kRunThreads), st>>>(L.cases, L.flow.w, L.flow.span,
+                                                                                    L.flow.ctr);
 }
 
 // blocks of the dataflow sweep resident at once on the device (its
@@ -897,7 +890,8 @@ int flow_max_blocks(int device) {
     if (device < 0 || device >= 64) return kXcdLists;
     if (cache[device]) return cache[device];
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lusgs_flow<kNsp>, kRunThreads, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lusgs_flow<kNsp>, kRunThreads,
+                                                     lusgs_lds(kRunThreads)) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return kXcdLists;
     cache[device] = std::max(kXcdLists, per_cu * cus / kXcdLists * kXcdLists);
@@ -915,10 +909,22 @@ void launch_residual(const LaunchSet &L, bool ts, hipStream_t st) {
 }
 
 // res: forward levels that also form their cells' residual (the fused step)
+template <bool FWD, bool RES>
+void launch_level(const LaunchSet &L, const WorkList &wl, hipStream_t st) {
+    if (L.single) {
+        const int n = wl.hi - wl.lo;
+        if (n > 0)
+            k_lusgs<kNsp, FWD, RES, false><<<(n + kRunChunk - 1) / kRunChunk, kRunThreads, lusgs_lds(kRunThreads), st>>>(
+                L.cases, nullptr, wl.lo, wl.hi);
+    } else if (wl.n) {
+        k_lusgs<kNsp, FWD, RES, true><<<wl.n, wl.bs, lusgs_lds(wl.bs), st>>>(L.cases, wl.w, 0, 0);
+    }
+}
+
 void launch_lusgs_level(const LaunchSet &L, bool fwd, int l, bool res, hipStream_t st) {
-    if (fwd && res) LAUNCH(L.level[l], kRunChunk, st, k_lusgs, kNsp, true, true);
-    else if (fwd) LAUNCH(L.level[l], kRunChunk, st, k_lusgs, kNsp, true, false);
-    else LAUNCH(L.level[l], kRunChunk, st, k_lusgs, kNsp, false, false);
+    if (fwd && res) launch_level<true, true>(L, L.level[l], st);
+    else if (fwd) launch_level<true, false>(L, L.level[l], st);
+    else launch_level<false, false>(L, L.level[l], st);
 }
 
 void launch_update(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_update, kNsp); }
