@@ -1,14 +1,18 @@
 """Batches of resident cases (rc_batch_*): every member's results are bitwise
 the same alone and in a batch, whatever the other members are, including
 when members join and leave between calls, and a member's error leaves the
-others intact."""
+others intact; the mixed batch's members also agree with the C oracle
+(<=1e-10 after 60 steps)."""
 import sys
 
 import numpy as np
 import pytest
 
+from helpers import relerr
 from rocketcombustor_amd import native, nozzle
 from rocketcombustor_amd.solver import SteadyBatch, SteadyCase
+
+oracle = pytest.importorskip("oracle.pyoracle")
 
 pytestmark = pytest.mark.gpu
 
@@ -57,6 +61,12 @@ def test_batch_equals_alone():
             assert np.array_equal(g.increment(), ref[k][2]), k
     for g in gs:
         g.close()
+    # and against the oracle (every member: these are small)
+    for k, c in enumerate(cases):
+        o = oracle.OracleCase(c)
+        for _ in range(n):
+            o.stage()
+        assert relerr(ref[k][1], o.get("PA"), -1) < 1e-10, k
 
 
 def test_batch_dataflow_sweep_equals_alone(monkeypatch):
