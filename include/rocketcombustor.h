@@ -244,6 +244,14 @@ int rc_get_ghost(rc_case *h, double *gpa);              /* GPA(Nvar,Nghost) */
  * state: rho(NE), Ma(NE) and Production_rate(Nsp-1,NE) as sourceterm forms
  * it (sourceterm.f90:17-43; zero without combustion).  Any may be NULL. */
 int rc_get_derived(rc_case *h, double *rho, double *mach, double *prod);
+/* rc_get_derived without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernel stages its output in a static device array of 524288 cells x
+ * (Nsp+1) doubles per GPU (held until the process ends); cases with more
+ * cells return RC_ERR_ARGUMENT.  Calls of one process share the array and
+ * are serialised by a lock; outputs as rc_get_derived. */
+int rc_get_derived_noalloc(rc_case *h, double *rho, double *mach, double *prod);
 
 /* Per-kernel device time of the fused step summed over the steps run while
  * timing was on, in ms: t[0] gradient + Barth limiter (ghost states formed

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1303,6 +1304,71 @@ int rc_get_derived(rc_case *h, double *rho, double *mach, double *prod) {
             for (int k = 0; k < kNsp - 1; ++k) prod[(size_t)i * (kNsp - 1) + k] = tmp[(size_t)i * w + 2 + k];
     }
     return RC_OK;
+}
+
+}  // extern "C"
+
+// The derived fields without any allocation (no reference counterpart): the
+// same kernel as rc_get_derived stages its output in `scratch`, a
+// file-scope device array of kNoallocCells x (Nsp+1) doubles (21 MB) that
+// every GPU the library runs on holds until the process ends.  Host code
+// reaches a device's copy only through the address hipGetSymbolAddress
+// returns with that device current, taken at the entry point's first call on
+// the device and kept per device; the array's name itself is no device
+// address.  Calls of one process are serialised by a lock (they share each
+// device's array); every element a call copies out is written by that
+// call's kernel first, so nothing needs clearing.
+constexpr int kNoallocCells = 1 << 19;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[(size_t)kNoallocCells * (kNsp + 1)];
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// the device's copy of scratch (the lock held, the device current)
+int scratch_address(rc_case *h, double **out) {
+    if (!g_scratch_dev[h->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(h, RC_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[h->device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_dev[h->device];
+    return RC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int rc_get_derived_noalloc(rc_case *h, double *rho, double *mach, double *prod) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    if (h->ne > kNoallocCells)
+        return set_err(h, RC_ERR_ARGUMENT, "rc_get_derived_noalloc: the case has " + std::to_string(h->ne) +
+                                               " cells, the static staging array holds " +
+                                               std::to_string(kNoallocCells) + " (use rc_get_derived)");
+    if (h->device < 0 || h->device >= kMaxDevices) return set_err(h, RC_ERR_ARGUMENT, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    HIPCHK(h, hipSetDevice(h->device));
+    double *d = nullptr;
+    int rc = scratch_address(h, &d);
+    if (rc) return rc;
+    launch_derived(h->L, d, h->stream);
+    // strided copies straight into the caller's arrays (no host staging)
+    const size_t w = kNsp + 1, n = (size_t)h->ne;
+    if (n > 0) {
+        if (rho)
+            HIPCHK(h, hipMemcpy2DAsync(rho, sizeof(double), d, w * sizeof(double), sizeof(double), n,
+                                       hipMemcpyDeviceToHost, h->stream));
+        if (mach)
+            HIPCHK(h, hipMemcpy2DAsync(mach, sizeof(double), d + 1, w * sizeof(double), sizeof(double), n,
+                                       hipMemcpyDeviceToHost, h->stream));
+        if (prod)
+            HIPCHK(h, hipMemcpy2DAsync(prod, (kNsp - 1) * sizeof(double), d + 2, w * sizeof(double),
+                                       (kNsp - 1) * sizeof(double), n, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return check_device_error(h);
 }
 
 int rc_set_timing(rc_case *h, int enable) {

@@ -109,6 +109,7 @@ SIGNATURES = {
     "rc_get_timestep": (C.c_int, [_VP, _DP, _DP]),
     "rc_get_ghost": (C.c_int, [_VP, _DP]),
     "rc_get_derived": (C.c_int, [_VP, _DP, _DP, _DP]),
+    "rc_get_derived_noalloc": (C.c_int, [_VP, _DP, _DP, _DP]),
     "rc_set_timing": (C.c_int, [_VP, C.c_int]),
     "rc_get_timing": (C.c_int, [_VP, _DP]),
     "rc_mesh_order_multicolor": (C.c_int, [C.c_int, _IP, C.c_int, _IP, _IP]),
