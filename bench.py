@@ -38,6 +38,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -195,6 +197,62 @@ def small_run(nx: int = 150, ny: int = 40, nsteps: int = 2000, cpu_steps: int = 
         c["seconds_for_%d_steps_at_that_rate" % nsteps] = case.mesh.ne * nsteps / c["value"]
         out["cpu_baseline"] = c
         out["speedup_time_to_solution"] = c["seconds_for_%d_steps_at_that_rate" % nsteps] / wall
+    return out
+
+
+def converge_run(case, max_steps: int, every: int = 100, ref_ms_per_step=None) -> dict:
+    """The steady run of one case to its plateau: steps until RoRes (the
+    species part of step_residual, what the reference monitors,
+    src/residual.f90:79) and the density norm stop falling.  The plateau
+    step is the first report within 10% of the lowest value the whole run
+    reaches (a drop to 1e-8 is not reached: the Barth limiter and the
+    threshold kinetics settle into a limit cycle, the reference does the
+    same); ``still_falling`` says whether the last fifth of the run still went
+    10% below the rest.  Reports the drops reached, the device time to the
+    plateau and the reference's time at its measured rate."""
+    from rocketcombustor_amd.solver import SteadyCase
+    from rocketcombustor_amd import native
+    out = {"max_steps": max_steps, "report_every": every}
+    with SteadyCase(case, device=0) as g:
+        t0 = time.perf_counter()
+        err = None
+        rows = []
+        done = 0
+        try:
+            while done < max_steps:
+                k = min(2000, max_steps - done)
+                h = g.run_steady(k, every=every)
+                # rows of steps done+1 (first) and done+every, done+2*every, ...
+                rows.extend(h if done == 0 else h[1:])
+                done += k
+        except native.RcError as e:
+            err = str(e)
+        wall = time.perf_counter() - t0
+    hist = np.array(rows)
+    steps = np.r_[1, np.arange(every, every * (len(hist) - 1) + 1, every)][:len(hist)]
+    ro = hist[:, 4:7].sum(axis=1)
+    rho = hist[:, 0]
+
+    def plateau(v):
+        return int(steps[np.argmax(v <= 1.1 * v.min())])
+
+    cut = max(1, (4 * len(ro)) // 5)
+
+    ms = 1e3 * wall / max(done, 1)
+    p_ro, p_rho = plateau(ro), plateau(rho)
+    out.update({"steps_run": int(done), "error": err, "ms_per_step": ms,
+                "roress_drop": float(ro.min() / ro[0]), "rho_drop": float(rho.min() / rho[0]),
+                "plateau_step_roress": p_ro, "plateau_step_rho": p_rho,
+                "still_falling": bool(ro[cut:].min() < 0.9 * ro[:cut].min() or rho[cut:].min() < 0.9 * rho[:cut].min()),
+                "steps_to_drop": {"%g" % d: (int(steps[np.argmax(ro <= d * ro[0])]) if np.any(ro <= d * ro[0])
+                                             else None) for d in (1e-1, 1e-2, 1e-3)}})
+    ps = max(p_ro, p_rho)
+    if ps:
+        out["plateau_step"] = ps
+        out["seconds_to_plateau"] = ps * ms * 1e-3
+        if ref_ms_per_step:
+            out["reference_seconds_to_plateau_at_its_rate"] = ps * ref_ms_per_step * 1e-3
+            out["speedup_time_to_plateau"] = ref_ms_per_step / ms
     return out
 
 
@@ -511,6 +569,9 @@ def main() -> int:
                          "cases, train the source-term MLP on the device, then step the 1.6M-triangle nozzle with "
                          "the gated surrogate and with exact kinetics)")
     ap.add_argument("--learned-steps", type=int, default=100000, help="training steps of --learned")
+    ap.add_argument("--converge-steps", type=int, default=30000, metavar="N",
+                    help="also run the same case from its initial field until its residual plateau (at most N "
+                         "steps) and report steps and time to it; 0 skips it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device three times (a plain copy kernel of "
                          "known size, to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -557,7 +618,9 @@ def main() -> int:
     dom = max(kernels, key=kernels.get)
     # per launch: algorithmic bytes / mean launch duration
     ach = (kb[dom] / launches[dom]) / (kernels[dom] * 1e-3 / launches[dom]) / 1e9
-    traffic = measured_traffic().get(dom) if (args.nx, args.ny) == (1000, 200) else None
+    # the committed counter summary measured an earlier build (five sweep
+    # levels, other launch sizes): its bytes per launch are not this build's
+    traffic = None
     b_cu = 1256.0 + 160.0 * nf / ne          # whole-step algorithmic bytes per cell-update
     out = {
         "metric": METRIC,
@@ -578,10 +641,9 @@ def main() -> int:
                    "cells": ne, "faces": nf, "ghosts": ng, "lusgs_levels": g.nlev},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": "rocprofv3 FETCH_SIZE / WRITE_SIZE passes in %s (the sweep kernels "
-                                       "then ran in 256-thread blocks; the profiler stops at the first "
-                                       "dispatch with the current library)" % os.path.relpath(TRAFFIC_PROFILE, ROOT)
-                     if traffic else None,
+                     "traffic_note": "no counter pass of this build: rocprofv3 stopped at the first dispatch with "
+                                     "this library; %s holds an earlier build's per-launch bytes"
+                                     % os.path.relpath(TRAFFIC_PROFILE, ROOT),
                      "algorithmic_bytes_per_launch": kb[dom] / launches[dom], "launches_per_step": launches[dom]},
         "kernel_timing": "HIP events between kernels, eager replay of the same %d steps (%.4f ms/step)"
                          % (args.steps, 1000.0 * wall_eager / args.steps),
@@ -613,8 +675,16 @@ def main() -> int:
         leg("serving", serving_run, args.serve, args.serve_plain)
     if args.batched > 0:
         leg("batched", batched_run, args.batched, args.batched_steps)
+    if args.converge_steps > 0:
+        # the reference's per-step time on this case, measured on a sample
+        ref_ms = None
+        if not args.no_cpu_baseline:
+            out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+            ref_ms = 1e3 * ne / out["cpu_baseline"]["value"]
+        leg("converge_400k", converge_run, case, args.converge_steps, ref_ms_per_step=ref_ms)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+        if "cpu_baseline" not in out:
+            out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
         hc = host_cores()
         out["cpu_baseline"].update(hc)
         if args.cpu_all_cores:
