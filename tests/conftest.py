@@ -19,3 +19,12 @@ def gpu_available():
         return torch.cuda.is_available()
     except Exception:
         return False
+
+
+@pytest.fixture(params=["0", "1"], ids=["plain-ranges", "xcd-banded"])
+def xcd_layout(request, monkeypatch):
+    """Both work-list layouts of a single case: plain [lo, hi) ranges (the
+    default above 600k cells) and XCD-banded lists (the default below);
+    RC_XCD is read when a case is created."""
+    monkeypatch.setenv("RC_XCD", request.param)
+    return request.param

@@ -145,7 +145,7 @@ def test_stopping_case_raises_at_the_reference_step():
 
 
 @pytest.mark.parametrize("ni,nj,dt,inner,model", [(60, 15, 2e-5, 4, 1), (90, 24, 1e-10, 3, 0)])
-def test_multicolour_order_vs_oracle(ni, nj, dt, inner, model):
+def test_multicolour_order_vs_oracle(ni, nj, dt, inner, model, xcd_layout):
     """Larger meshes in the block-multicolour order (many sweep levels):
     stage entry against the oracle's call-by-call loop, both rows and the
     state after each step."""
@@ -162,7 +162,7 @@ def test_multicolour_order_vs_oracle(ni, nj, dt, inner, model):
         assert relerr(g.conserved(), o.get("WA"), -1) < 1e-12
 
 
-def test_steady_after_backward_euler():
+def test_steady_after_backward_euler(xcd_layout):
     """The point-implicit update keeps the cell-average cache that the
     steady operators read: a steady stage after backward-Euler steps equals
     the oracle's."""

@@ -20,7 +20,7 @@ def announce(msg: str) -> None:
     sys.__stderr__.flush()
 
 
-def test_derived_fields_and_vtk(tmp_path):
+def test_derived_fields_and_vtk(tmp_path, xcd_layout):
     case = nozzle.nozzle_case(60, 12)
     announce("derived fields NE=%d" % case.mesh.ne)
     with SteadyCase(case) as g:
@@ -43,7 +43,7 @@ def test_derived_fields_and_vtk(tmp_path):
     assert np.allclose(v["rho"], d["rho"], rtol=1e-8) and np.allclose(v["T"], pa[:, 3], rtol=1e-8)
 
 
-def test_backup_restart_vs_oracle(tmp_path):
+def test_backup_restart_vs_oracle(tmp_path, xcd_layout):
     case = nozzle.nozzle_case(30, 8)
     perm, _ = native.order_multicolor(case.mesh.cells)
     case = case.permuted(perm)

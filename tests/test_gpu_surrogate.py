@@ -44,7 +44,7 @@ def exact_run(case, n):
 
 
 @pytest.mark.parametrize("nx,ny", [(40, 10), (1, 1)])
-def test_gate_rejecting_all_is_the_exact_step(trained, nx, ny):
+def test_gate_rejecting_all_is_the_exact_step(trained, nx, ny, xcd_layout):
     m, tf = trained
     case = nozzle.nozzle_case(nx, ny)
     perm, _ = native.order_multicolor(case.mesh.cells)

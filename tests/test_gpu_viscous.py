@@ -47,7 +47,7 @@ def test_viscous_part_matches_reference():
                                      ((80, 20), {"wall_type": 1, "fuel_faces": 2, "subsonic_exit": True,
                                                  "p_out_ratio": 0.9}),
                                      ((1, 1), {}), ((3, 2), {"wall_type": 1})])
-def test_multicolour_viscous_steps_vs_oracle(size, kw):
+def test_multicolour_viscous_steps_vs_oracle(size, kw, xcd_layout):
     """Viscous steps against the oracle, per call and fused, from 1600
     triangles down to the smallest meshes (2 and 12 triangles, every face of
     a cell on the boundary or shared with one neighbour)."""
