@@ -358,9 +358,10 @@ def learned_run(train_steps: int, batch: int = 4096, lr: float = 2e-3, nsteps: i
     steady runs (snapshots with the device's exact FP64 rates), the MLP
     (6 -> 3 x 128 tanh -> 3, FP32) trained on the device, its error on
     held-out states, then the 1.6M-triangle nozzle stepped with exact
-    kinetics, with the surrogate behind the plausibility gate (training
-    range, mass balance) and behind the verified gate (also within rtol of
-    the exact FP64 rates)."""
+    kinetics, with the surrogate behind the trust gate (training range, mass
+    balance and a trust region calibrated on held-out states: no exact rate
+    formed), behind the plausibility gate (range and mass balance only) and
+    behind the verified gate (also within rtol of the exact FP64 rates)."""
     import numpy as np
     from rocketcombustor_amd import native, nozzle, surrogate
     from rocketcombustor_amd.solver import SteadyCase
@@ -379,9 +380,12 @@ def learned_run(train_steps: int, batch: int = 4096, lr: float = 2e-3, nsteps: i
                                          per_snapshot=400000, seed=5)
     t_sample = time.perf_counter() - t0
     rng = np.random.default_rng(5)
-    hold = rng.choice(len(raw), size=100000, replace=False)
+    # held out: 100k states to state the error, 400k to calibrate the trust
+    # region of the gate that forms no exact rate
+    out_idx = rng.choice(len(raw), size=500000, replace=False)
+    hold, calib = out_idx[:100000], out_idx[100000:]
     keep = np.ones(len(raw), dtype=bool)
-    keep[hold] = False
+    keep[out_idx] = False
     tf = surrogate.Transform.fit(raw[keep], omega[keep], knee=1e-3)
     m = surrogate.MLP(seed=5)
     m.set_data(tf.x(raw[keep]), tf.y(omega[keep]))
@@ -400,12 +404,23 @@ def learned_run(train_steps: int, batch: int = 4096, lr: float = 2e-3, nsteps: i
                        "rel_err_p90": float(np.percentile(rel, 90)), "rel_err_p99": float(np.percentile(rel, 99)),
                        "rel_err_definition": "|w - w_exact| / max(|w_exact|, 1e-3 s_k), per species",
                        "pass_rtol_1e-3": float(surrogate.gate(tf, raw[hold], w_hold, omega[hold], 1e9, 1e-3).mean())}}
+    w_cal = tf.omega(m.forward(tf.x(raw[calib])))
+    trust = surrogate.TrustRegion.calibrate(tf, raw[calib], w_cal, omega[calib], dims=(48, 16, 48), tau=2.5e-4,
+                                            nmin=10)
+    hold_ok = surrogate.gate(tf, raw[hold], w_hold, omega[hold], tol, 0.0, trust)
+    out["trust_region"] = {"features": "T, log p, Y_1", "dims": list(trust.dims), "tau": trust.tau,
+                           "nmin": trust.nmin, "calibration_rows": int(len(calib)),
+                           "accepted_grid_cells": int(trust.accept.sum()),
+                           "holdout_accepted_fraction": float(hold_ok.mean()),
+                           "holdout_accepted_rel_err_max": float(rel[hold_ok].max(axis=1).max()) if hold_ok.any()
+                           else None}
     big_case = ordered(2000, 400)
     res, fields = {}, {}
-    for mode in ("exact", "plausibility_gate", "verified_gate"):
+    for mode in ("exact", "trust_gate", "plausibility_gate", "verified_gate"):
         with SteadyCase(big_case, device=0) as g:
             if mode != "exact":
-                g.set_surrogate(m, tf, tol=tol, rtol=rtol if mode == "verified_gate" else 0.0)
+                g.set_surrogate(m, tf, tol=tol, rtol=rtol if mode == "verified_gate" else 0.0,
+                                trust=trust if mode == "trust_gate" else None)
             g.run_steady(5, history=False)
             t2 = time.perf_counter()
             g.run_steady(nsteps, history=False)
@@ -421,7 +436,8 @@ def learned_run(train_steps: int, batch: int = 4096, lr: float = 2e-3, nsteps: i
                 w_exact = g.derived()["production_rate"]
                 r = surrogate.raw_inputs(pa)
                 w_net = tf.omega(m.forward(tf.x(r)))
-                ok = surrogate.gate(tf, r, w_net, w_exact, tol, rtol if mode == "verified_gate" else 0.0)
+                ok = surrogate.gate(tf, r, w_net, w_exact, tol, rtol if mode == "verified_gate" else 0.0,
+                                    trust if mode == "trust_gate" else None)
                 e = rel_err(w_net, w_exact, tf)
                 res[mode]["final_state"] = {
                     "accepted_fraction": float(ok.mean()),

@@ -197,6 +197,17 @@ int  rc_mlp_train(rc_mlp *m, int nsteps, int batch, float lr, unsigned long long
  * (the step reads its parameters on the case's stream); such a case steps
  * alone (not in a batch).  No reference counterpart. */
 int rc_set_surrogate(rc_case *h, rc_mlp *m, const double *xform, double tol, double rtol);
+/* A calibrated trust region for the learned source's gate, which then needs
+ * no exact rate: a grid of dims3[0] x dims3[1] x dims3[2] cells over the
+ * features (T, log p, Y_1) between lo3 and hi3; a cell's network rates are
+ * accepted only when its features fall in a grid cell with accept[(i0*d1 +
+ * i1)*d2 + i2] != 0 (besides the range and mass-balance tests), else it
+ * takes the exact FP64 rate.  The caller marks the grid cells where the
+ * network's error on held-out states stayed within its bound.  accept =
+ * NULL removes the grid.  Needs rc_set_surrogate first; no reference
+ * counterpart. */
+int rc_set_surrogate_trust(rc_case *h, const int *dims3, const double *lo3, const double *hi3,
+                           const unsigned char *accept);
 /* accepted and evaluated cell source evaluations since rc_set_surrogate */
 int rc_get_surrogate_stats(rc_case *h, long long *accepted_evaluated2);
 

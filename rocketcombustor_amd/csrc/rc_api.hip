@@ -45,6 +45,8 @@ int surrogate_create(rc_mlp *, int, const double *, double, double, Surrogate **
 void surrogate_destroy(Surrogate *);
 void surrogate_enqueue(Surrogate *, const CaseDev *, int, hipStream_t);
 int surrogate_accepted(Surrogate *, hipStream_t, long long &);
+int surrogate_set_trust(Surrogate *, const int *, const double *, const double *, const unsigned char *,
+                        std::string &);
 }  // namespace rc
 
 using namespace rc;
@@ -1143,6 +1145,22 @@ int rc_set_surrogate(rc_case *h, rc_mlp *m, const double *xform, double tol, dou
     dp.surrogate = m ? 1 : 0;
     HIPCHK(h, hipMemcpyAsync(h->d_par, &dp, sizeof dp, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+    return RC_OK;
+}
+
+int rc_set_surrogate_trust(rc_case *h, const int *dims3, const double *lo3, const double *hi3,
+                           const unsigned char *accept) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    if (!h->sur) return set_err(h, RC_ERR_ARGUMENT, "no learned source set (rc_set_surrogate first)");
+    if (accept && (!dims3 || !lo3 || !hi3)) return set_err(h, RC_ERR_ARGUMENT, "NULL argument");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::string msg;
+    const int rc = surrogate_set_trust(h->sur, dims3, lo3, hi3, accept, msg);
+    if (rc) return set_err(h, rc, msg);
+    // the gate's table is a kernel argument: capture the step again
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     h->graph_exec = nullptr;
     return RC_OK;

@@ -436,17 +436,41 @@ __global__ void __launch_bounds__(256) k_sur_inputs(const CaseDev *__restrict__ 
     for (int k = 0; k < kSurIn; ++k) x[(size_t)r * kSurIn + k] = (float)((raw[k] - xf[kXfMean + k]) / xf[kXfStd + k]);
 }
 
+// A calibrated trust region of the network: a grid over (T, log p, Y_1)
+// whose cells are marked accepted where the network's rates on held-out
+// states of that cell stayed within the calibration bound; a state outside
+// the grid or in an unmarked cell is rejected.  No exact rate is formed.
+struct TrustGrid {
+    const unsigned char *accept = nullptr;   // dims[0]*dims[1]*dims[2], index (i0*d1 + i1)*d2 + i2
+    int dims[3] = {0, 0, 0};
+    double lo[3] = {0, 0, 0}, inv[3] = {0, 0, 0};   // cell of feature f: floor((f - lo) * inv)
+};
+
+__device__ __forceinline__ bool trust_accepts(const TrustGrid &g, double t, double logp, double y1) {
+    const double f[3] = {t, logp, y1};
+    int idx = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double u = (f[d] - g.lo[d]) * g.inv[d];
+        if (!(u >= 0.0) || u >= (double)g.dims[d]) return false;
+        idx = idx * g.dims[d] + (int)u;
+    }
+    return gp(g.accept)[idx] != 0;
+}
+
 // accept the network's source for a cell when its raw inputs lie in the
 // training range and its rates keep the reaction's mass balance (the
 // species rates of one reaction sum to zero): |sum w| <= tol (sum |w| +
-// 1e-3 max scale); with rtol > 0 the rates must also lie within rtol of the
-// exact FP64 rates, |w_k - w_k,exact| <= rtol max(|w_k,exact|, 1e-3 s_k).
+// 1e-3 max scale); with a trust grid also when its (T, log p, Y_1) cell is
+// marked; with rtol > 0 the rates must also lie within rtol of the exact
+// FP64 rates, |w_k - w_k,exact| <= rtol max(|w_k,exact|, 1e-3 s_k).
 // A rejected cell takes the exact FP64 rate of the source term.  Accepted
 // cells are counted per block of 256 cells (acc, cumulative).
 template <int NSP>
 __global__ void __launch_bounds__(256) k_sur_gate(const CaseDev *__restrict__ cases, int c0, int n,
                                                   const double *__restrict__ xf, const float *__restrict__ y,
-                                                  double tol, double rtol, long long *__restrict__ acc) {
+                                                  double tol, double rtol, TrustGrid trust,
+                                                  long long *__restrict__ acc) {
     __shared__ int cnt[256];
     const int r = blockIdx.x * 256 + threadIdx.x;
     int ok = 0;
@@ -476,6 +500,7 @@ __global__ void __launch_bounds__(256) k_sur_gate(const CaseDev *__restrict__ ca
             smax = fmax(smax, xf[kXfScale + k]);
         }
         ok = (inside && fabs(wsum) <= tol * (wabs + 1e-3 * smax)) ? 1 : 0;
+        if (ok && trust.accept) ok = trust_accepts(trust, raw[0], log(raw[1]), raw[2]) ? 1 : 0;
         if (!ok || rtol > 0.0) {
             double yy[NSP], we[NSP];
 #pragma unroll
@@ -846,6 +871,7 @@ namespace rc {
 struct Surrogate {
     rc_mlp *m = nullptr;
     double tol = 0.0, rtol = 0.0;
+    TrustGrid trust;             // device table, or none
     double *xf = nullptr;        // device, kXfLen
     float *buf[8] = {};          // inputs, (GEMM chain) activations, outputs of one chunk: dims[0..nlayer]
     int chunk = 0;
@@ -900,8 +926,42 @@ int surrogate_create(rc_mlp *m, int ne, const double *xform, double tol, double 
     return RC_OK;
 }
 
+// the calibrated trust grid of the gate (accept = NULL removes it)
+int surrogate_set_trust(Surrogate *s, const int *dims, const double *lo, const double *hi,
+                        const unsigned char *accept, std::string &msg) {
+    if (s->trust.accept) (void)hipFree((void *)s->trust.accept);
+    s->trust = TrustGrid{};
+    if (!accept) return RC_OK;
+    size_t cells = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (dims[d] < 1 || dims[d] > 4096 || !(hi[d] > lo[d])) {
+            msg = "trust grid: dims must be 1..4096 and hi > lo";
+            return RC_ERR_ARGUMENT;
+        }
+        cells *= (size_t)dims[d];
+    }
+    unsigned char *d_acc = nullptr;
+    if (hipMalloc(&d_acc, cells) != hipSuccess) {
+        msg = "device allocation for the trust grid failed";
+        return RC_ERR_MEMORY;
+    }
+    if (hipMemcpy(d_acc, accept, cells, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d_acc);
+        msg = "copying the trust grid failed";
+        return RC_ERR_DEVICE;
+    }
+    s->trust.accept = d_acc;
+    for (int d = 0; d < 3; ++d) {
+        s->trust.dims[d] = dims[d];
+        s->trust.lo[d] = lo[d];
+        s->trust.inv[d] = dims[d] / (hi[d] - lo[d]);
+    }
+    return RC_OK;
+}
+
 void surrogate_destroy(Surrogate *s) {
     if (!s) return;
+    if (s->trust.accept) (void)hipFree((void *)s->trust.accept);
     for (float *p : s->buf) if (p) (void)hipFree(p);
     if (s->xf) (void)hipFree(s->xf);
     if (s->acc) (void)hipFree(s->acc);
@@ -916,7 +976,7 @@ void surrogate_enqueue(Surrogate *s, const CaseDev *dcase, int ne, hipStream_t s
         k_sur_inputs<kNsp><<<(n + 255) / 256, 256, 0, st>>>(dcase, c0, n, s->xf, s->buf[0]);
         infer(m, st, s->buf[0], s->buf[m->nlayer], n, s->buf);
         k_sur_gate<kNsp><<<(n + 255) / 256, 256, 0, st>>>(dcase, c0, n, s->xf, s->buf[m->nlayer], s->tol, s->rtol,
-                                                             s->acc);
+                                                             s->trust, s->acc);
     }
 }
 

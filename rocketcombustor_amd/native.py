@@ -99,6 +99,7 @@ SIGNATURES = {
     "rc_mlp_set_data": (C.c_int, [_VP, _FP, _FP, C.c_longlong]),
     "rc_mlp_train": (C.c_int, [_VP, C.c_int, C.c_int, C.c_float, C.c_ulonglong, C.c_int, _FP]),
     "rc_set_surrogate": (C.c_int, [_VP, _VP, _DP, C.c_double, C.c_double]),
+    "rc_set_surrogate_trust": (C.c_int, [_VP, C.POINTER(C.c_int), _DP, _DP, C.POINTER(C.c_ubyte)]),
     "rc_get_surrogate_stats": (C.c_int, [_VP, P(C.c_longlong)]),
     "rc_backward_euler_lusgs": (C.c_int, [_VP, C.c_double, C.c_int]),
     "rc_backward_euler_stage": (C.c_int, [_VP, C.c_double, C.c_int, C.c_double, _IP, _DP]),

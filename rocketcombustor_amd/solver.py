@@ -178,17 +178,26 @@ class SteadyCase:
         check(lib().rc_set_transport(self.h, C.byref(self._tr)), self.h)
 
     # -- learned chemistry (no reference counterpart) ----------------------
-    def set_surrogate(self, mlp, transform=None, tol: float = 0.05, rtol: float = 0.0) -> None:
+    def set_surrogate(self, mlp, transform=None, tol: float = 0.05, rtol: float = 0.0, trust=None) -> None:
         """The source term from the network ``mlp`` (surrogate.MLP) with the
-        per-cell gate (training range, mass balance within ``tol``; with
-        ``rtol`` > 0 also within ``rtol`` of the exact FP64 rates) and the
-        exact FP64 rate as fallback; ``mlp=None`` restores the exact source."""
+        per-cell gate (training range, mass balance within ``tol``; with a
+        ``trust`` region (surrogate.TrustRegion) only its calibrated grid
+        cells; with ``rtol`` > 0 also within ``rtol`` of the exact FP64
+        rates) and the exact FP64 rate as fallback; ``mlp=None`` restores the
+        exact source."""
         if mlp is None:
             check(lib().rc_set_surrogate(self.h, None, None, 0.0, 0.0), self.h)
             self._sur = None
             return
         xf = np.ascontiguousarray(transform.packed(), dtype=np.float64)
         check(lib().rc_set_surrogate(self.h, mlp.h, dptr(xf), float(tol), float(rtol)), self.h)
+        if trust is not None:
+            dims = (C.c_int * 3)(*trust.dims)
+            lo = np.ascontiguousarray(trust.lo, dtype=np.float64)
+            hi = np.ascontiguousarray(trust.hi, dtype=np.float64)
+            acc = np.ascontiguousarray(trust.accept, dtype=np.uint8)
+            check(lib().rc_set_surrogate_trust(self.h, dims, dptr(lo), dptr(hi),
+                                               acc.ctypes.data_as(C.POINTER(C.c_ubyte))), self.h)
         self._sur = (mlp, xf)
 
     def surrogate_stats(self):

@@ -160,14 +160,70 @@ def train_schedule(m: "MLP", nsteps: int, batch: int = 4096, lr: float = 1e-3, s
     return np.array(out)
 
 
-def gate(tf: Transform, raw: np.ndarray, w_net: np.ndarray, w_exact: np.ndarray, tol: float, rtol: float):
+def gate(tf: Transform, raw: np.ndarray, w_net: np.ndarray, w_exact: np.ndarray, tol: float, rtol: float,
+         trust: Optional["TrustRegion"] = None):
     """The per-cell acceptance the device gate forms (training range, mass
-    balance within tol, and with rtol > 0 agreement with the exact rates)."""
+    balance within tol, with a trust region its marked grid cells, and with
+    rtol > 0 agreement with the exact rates)."""
     inside = np.all((raw >= tf.lo) & (raw <= tf.hi), axis=1)
     ok = inside & (np.abs(w_net.sum(axis=1)) <= tol * (np.abs(w_net).sum(axis=1) + 1e-3 * tf.out_scale.max()))
+    if trust is not None:
+        idx = trust.index(raw)
+        ok &= (idx >= 0) & (trust.accept[np.maximum(idx, 0)] != 0)
     if rtol > 0:
         ok &= np.all(np.abs(w_net - w_exact) <= rtol * np.maximum(np.abs(w_exact), 1e-3 * tf.out_scale), axis=1)
     return ok
+
+
+@dataclasses.dataclass
+class TrustRegion:
+    """A calibrated trust region of a trained network, for a gate that needs
+    no exact rate: a grid over the features (T, log p, Y_1) between lo and
+    hi; a grid cell is accepted when the network's relative error on the
+    held-out states falling in it, max_k |w_k - w_k,exact| / max(|w_k,exact|,
+    1e-3 s_k), stayed within ``tau`` for at least ``nmin`` states."""
+    dims: tuple
+    lo: np.ndarray
+    hi: np.ndarray
+    accept: np.ndarray        # uint8, dims[0]*dims[1]*dims[2]
+    tau: float
+    nmin: int
+
+    @staticmethod
+    def feats(raw: np.ndarray) -> np.ndarray:
+        return np.column_stack([raw[:, 0], np.log(raw[:, 1]), raw[:, 2]])
+
+    def index(self, raw: np.ndarray):
+        """Grid cell of each state (-1 outside the grid)."""
+        f = self.feats(raw)
+        u = (f - self.lo) * (np.array(self.dims) / (self.hi - self.lo))
+        inside = np.all((u >= 0) & (u < np.array(self.dims)), axis=1)
+        b = np.clip(np.floor(u).astype(np.int64), 0, np.array(self.dims) - 1)
+        idx = (b[:, 0] * self.dims[1] + b[:, 1]) * self.dims[2] + b[:, 2]
+        return np.where(inside, idx, -1)
+
+    @staticmethod
+    def calibrate(tf: "Transform", raw: np.ndarray, w_net: np.ndarray, w_exact: np.ndarray,
+                  dims=(48, 16, 48), tau: float = 2.5e-4, nmin: int = 10) -> "TrustRegion":
+        lo = np.array([tf.lo[0], np.log(tf.lo[1]), tf.lo[2]])
+        hi = np.array([tf.hi[0], np.log(tf.hi[1]), tf.hi[2]])
+        hi = np.where(hi > lo, hi, lo + 1.0)
+        hi = hi + 1e-9 * (hi - lo)           # the largest states fall inside the last grid cell
+        tr = TrustRegion(dims=tuple(int(d) for d in dims), lo=lo, hi=hi,
+                         accept=np.zeros(int(np.prod(dims)), dtype=np.uint8), tau=float(tau), nmin=int(nmin))
+        err = rel_error(tf, w_net, w_exact)
+        idx = tr.index(raw)
+        keep = idx >= 0
+        n = np.bincount(idx[keep], minlength=tr.accept.size)
+        worst = np.zeros(tr.accept.size)
+        np.maximum.at(worst, idx[keep], err[keep])
+        tr.accept[:] = ((n >= nmin) & (worst <= tau)).astype(np.uint8)
+        return tr
+
+
+def rel_error(tf: "Transform", w: np.ndarray, w_exact: np.ndarray) -> np.ndarray:
+    """Per state: max_k |w_k - w_k,exact| / max(|w_k,exact|, 1e-3 s_k)."""
+    return np.max(np.abs(w - w_exact) / np.maximum(np.abs(w_exact), 1e-3 * tf.out_scale), axis=1)
 
 
 def features(raw: np.ndarray) -> np.ndarray:

@@ -154,3 +154,73 @@ def test_config5_poisson_stream_vs_alone_and_oracle():
             for _ in range(r.steps):
                 o.stage()
             assert relerr(final[r.rid], o.get("PA"), -1) < 1e-10, r.rid
+
+
+def test_config4_learned_source_time_and_error_against_exact():
+    """What the learned source costs and how close it is, against exact FP64
+    kinetics on the 1.6M-triangle nozzle: a network trained on states of
+    the 12k- and 400k-triangle runs, behind the trust gate (range, mass
+    balance and a trust region calibrated on held-out states: no exact rate
+    formed) and behind the verified gate (within 1e-3 of the exact rates).
+    Every accepted cell's rates lie within 1e-3 of the exact ones
+    (max(|omega|, 1e-3 s_k) scale); step times and accepted fractions are
+    stated.  (At this network's accuracy the trust region keeps few grid
+    cells or none: its errors near 1e-3 are spread over the whole input
+    range.)"""
+    import time
+
+    announce("config 4: sample 12k/400k states, train, calibrate")
+    cases = []
+    for nx, ny in ((150, 40), (1000, 200)):
+        c = nozzle.nozzle_case(nx, ny)
+        perm, _ = native.order_multicolor(c.mesh.cells)
+        cases.append(c.permuted(perm))
+    raw, omega = surrogate.sample_states(cases, steps=[1, 100, 300, 600, 1000], per_snapshot=100000, seed=11)
+    rng = np.random.default_rng(11)
+    calib = rng.choice(len(raw), size=len(raw) // 4, replace=False)
+    keep = np.ones(len(raw), dtype=bool)
+    keep[calib] = False
+    tf = surrogate.Transform.fit(raw[keep], omega[keep])
+    m = surrogate.MLP(seed=11)
+    try:
+        m.set_data(tf.x(raw[keep]), tf.y(omega[keep]))
+        surrogate.train_schedule(m, 20000, batch=4096, lr=2e-3, seed=11, phases=5, decay=0.4)
+        trust = surrogate.TrustRegion.calibrate(tf, raw[calib], tf.omega(m.forward(tf.x(raw[calib]))),
+                                                omega[calib], dims=(48, 16, 48), tau=2.5e-4, nmin=10)
+        big = nozzle.nozzle_case(2000, 400)
+        perm, _ = native.order_multicolor(big.mesh.cells)
+        big = big.permuted(perm)
+        ms, stats = {}, {}
+        for mode in ("exact", "trust_gate", "verified_gate"):
+            announce("config 4: 1.6M nozzle, %s" % mode)
+            with SteadyCase(big) as g:
+                if mode == "trust_gate":
+                    g.set_surrogate(m, tf, tol=0.05, trust=trust)
+                elif mode == "verified_gate":
+                    g.set_surrogate(m, tf, tol=0.05, rtol=1e-3)
+                g.run_steady(3, history=False)
+                t0 = time.perf_counter()
+                g.run_steady(20, history=False)
+                ms[mode] = 1e3 * (time.perf_counter() - t0) / 20
+                if mode == "exact":
+                    continue
+                acc, ev = g.surrogate_stats()
+                pa = g.primitive()
+                w_exact = g.derived()["production_rate"]
+            raw_b = surrogate.raw_inputs(pa)
+            w_net = tf.omega(m.forward(tf.x(raw_b)))
+            ok = surrogate.gate(tf, raw_b, w_net, w_exact, 0.05, 1e-3 if mode == "verified_gate" else 0.0,
+                                trust if mode == "trust_gate" else None)
+            err = surrogate.rel_error(tf, w_net, w_exact)
+            stats[mode] = (1.0 - acc / ev, ok.mean(), err[ok].max() if ok.any() else 0.0)
+            # the accepted cells of the final state meet the bar
+            assert np.all(err[ok] <= 1e-3), mode
+        announce("config 4: exact %.3f ms/step; trust gate %.3f ms/step, fallback %.1f%%, final-state accepted "
+                 "%.1f%% (max rel err %.2e); verified gate %.3f ms/step, fallback %.1f%%, accepted %.1f%% "
+                 "(max rel err %.2e)" % (ms["exact"], ms["trust_gate"], 100 * stats["trust_gate"][0],
+                                         100 * stats["trust_gate"][1], stats["trust_gate"][2], ms["verified_gate"],
+                                         100 * stats["verified_gate"][0], 100 * stats["verified_gate"][1],
+                                         stats["verified_gate"][2]))
+        assert stats["verified_gate"][1] > 0
+    finally:
+        m.close()
