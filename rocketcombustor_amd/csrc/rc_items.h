@@ -105,6 +105,8 @@ __device__ __forceinline__ bool write_lucoef(const GMesh &M, const GState &S, TT
     for (int k = 0; k < NSP; ++k) LC_(LC_H0 + k, i) = h[k];
 #pragma unroll
     for (int k = 0; k < NSP - 1; ++k) LC_(LC_RY0 + k, i) = rr * yy[k];
+#pragma unroll
+    for (int k = LC_N; k < kRecLC; ++k) LC_(k, i) = 0.0;   // whole record: whole lines go to memory
     return ok;
 }
 

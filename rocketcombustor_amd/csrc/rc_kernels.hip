@@ -227,6 +227,9 @@ __global__ void __launch_bounds__(256) k_gradlim(ITEM_ARGS) {
     }
 #pragma unroll
     for (int k = 0; k < NV; ++k) { GR_(k, 0, i) = g0[k]; GR_(k, 1, i) = g1[k]; }
+    // the record's unused slots too: whole 128-byte lines go to memory
+#pragma unroll
+    for (int k = 2 * NV; k < kRecGR; ++k) S.gradc[(size_t)i * kRecGR + k] = 0.0;
 }
 
 // flux_lax (flux_lax.f90:17-296), one thread per face: reconstructed L/R
@@ -647,6 +650,9 @@ __device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int ten
         } else {
 #pragma unroll
             for (int v = 0; v < NV; ++v) HWA_(v, i) = hv[v];
+            // the record's unused slot too: whole lines go to memory
+#pragma unroll
+            for (int v = NV; v < kRecHW; ++v) HWA_(v, i) = 0.0;
         }
     }
 }
@@ -726,6 +732,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     PA_(3, i) = t;
 #pragma unroll
     for (int k = 0; k < NSP - 1; ++k) PA_(kNFlow + k, i) = y[k];
+#pragma unroll
+    for (int k = NV; k < kRecPA; ++k) PA_(k, i) = 0.0;     // whole record: whole lines go to memory
     ok = write_lucoef<NSP>(M, S, th, i, p, u, v, t, y) && ok;
     if (!ok) flag_error(S.err, RC_ERR_TEMPERATURE, i, K_UPDATE);
     if (i == 0 && isnan(w[0])) flag_error(S.err, RC_ERR_NAN, 0, K_UPDATE);
