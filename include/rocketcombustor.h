@@ -240,6 +240,23 @@ int rc_backward_euler_lusgs(rc_case *h, double dt, int inner);
  * rc_residual returns its norms).  The exit pressure is the last one set. */
 int rc_backward_euler_stage(rc_case *h, double dt, int innerloop, double reltol, int *ninner, double *res_out);
 
+/* ---- BDF2 with the same point-implicit LU-SGS (implicit_time_method = 5). */
+
+/* The prologue of BDF2_stage(istep) (Time_BDF2.f90:12-20): with first
+ * (istep == 1) WP = WA, WP2 = WP and *phi = 0; else WP2 = WP, WP = WA and
+ * *phi = 0.5.  WP and WP2 stay resident for rc_bdf2_lusgs. */
+int rc_bdf2_begin_step(rc_case *h, int first, double *phi);
+/* Replaces BDF2_LU_SGS(Inner_rk, phi) (Time_BDF2.f90:71-337): as
+ * rc_backward_euler_lusgs, with fixed = vol/dt*((1+2phi)WP - phi WP2) at
+ * inner == 1, dd = sum 0.5*alpaF + vol*(1/step + (1+phi)/dt) and the
+ * right-hand side hWA - (1+phi)*vol/dt*WA + fixed. */
+int rc_bdf2_lusgs(rc_case *h, double dt, int inner, double phi);
+/* Replaces BDF2_stage(istep) (Time_BDF2.f90:4-69): rc_bdf2_begin_step
+ * (first = istep == 1), then the inner iterations, the stopping rule, the
+ * rows and the closing reconstruction + flux_sourceterm of
+ * rc_backward_euler_stage. */
+int rc_bdf2_stage(rc_case *h, double dt, int innerloop, double reltol, int first, int *ninner, double *res_out);
+
 /* Replaces Residual + the normalisation of step_residual (residual.f90:4-21,
  * 61-79): res[k] = sum_i vol_i |hWA(k,i)| / total_volume, res[nvar] = max rho. */
 int rc_residual(rc_case *h, double *res);

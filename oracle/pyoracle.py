@@ -80,6 +80,9 @@ def lib():
         L.rco_dwdp.argtypes = [P(RcoMech), P(C.c_double), P(C.c_double)]
         L.rco_solve_dgesvx.argtypes = [C.c_int, P(C.c_double), P(C.c_double), P(C.c_double)]
         L.rco_be_lusgs.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.rco_bdf2_lusgs.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double]
+        L.rco_bdf2_begin_step.argtypes = [C.c_void_p, C.c_int]
+        L.rco_bdf2_begin_step.restype = C.c_double
         L.rco_be_stage.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, P(C.c_int), P(C.c_double)]
         L.rco_set_p_out.argtypes = [C.c_void_p, C.c_double]
         L.rco_set_transport.argtypes = [C.c_void_p, P(RcoTransport), C.c_int]
@@ -241,6 +244,13 @@ class OracleCase:
 
     def be_lusgs(self, dt: float, inner: int):
         self._check(_LIB.rco_be_lusgs(self.h, float(dt), int(inner)))
+
+    def bdf2_begin_step(self, first: bool) -> float:
+        """The WP/WP2 prologue of BDF2_stage; returns phi."""
+        return float(_LIB.rco_bdf2_begin_step(self.h, 1 if first else 0))
+
+    def bdf2_lusgs(self, dt: float, inner: int, phi: float):
+        self._check(_LIB.rco_bdf2_lusgs(self.h, float(dt), int(inner), float(phi)))
 
     def be_stage(self, dt: float, innerloop: int, reltol: float) -> np.ndarray:
         """One physical step's inner iterations; returns their RoRes."""

@@ -39,6 +39,7 @@ struct rco_case {
     double total_volume;
     double *pa, *wa, *hwa, *gradc, *gpa, *alpaf, *step, *prod, *dd;
     double *fixed;                 /* fixed(Nvar,NE) of the dual-time iteration */
+    double *wp, *wp2;              /* WP, WP2 (Nvar,NE): the states at the start of the last two BDF2 steps */
     rco_transport tr;              /* transport / diffusion tables */
     int viscous;                   /* IfViscous */
     char err[256];
@@ -485,6 +486,8 @@ int rco_create(const rco_mech *mech, const rco_params *par, int nnt, const doubl
     c->prod = (double *)xcalloc((size_t)c->nsp * ne, sizeof(double));
     c->dd = (double *)xcalloc(ne, sizeof(double));
     c->fixed = (double *)xcalloc((size_t)nv * ne, sizeof(double));
+    c->wp = (double *)xcalloc((size_t)nv * ne, sizeof(double));
+    c->wp2 = (double *)xcalloc((size_t)nv * ne, sizeof(double));
     return 0;
 }
 
@@ -494,7 +497,8 @@ void rco_destroy(rco_case *c)
     void *ps[] = {c->coor, c->n4, c->fnode, c->neighbor, c->vecx, c->vol, c->cellxy, c->reflen,
                   c->neighf, c->neighc, c->neigh, c->hostcell, c->hostface, c->bctype_g,
                   c->ghostcell, c->num_ghost, c->coor_gn, c->vol_g, c->cellxy_g, c->wdis, c->amls,
-                  c->xyh, c->pa, c->wa, c->hwa, c->gradc, c->gpa, c->alpaf, c->step, c->prod, c->dd, c->fixed};
+                  c->xyh, c->pa, c->wa, c->hwa, c->gradc, c->gpa, c->alpaf, c->step, c->prod, c->dd, c->fixed,
+                  c->wp, c->wp2};
     size_t i;
     for (i = 0; i < sizeof ps / sizeof ps[0]; i++) free(ps[i]);
     free(c);
@@ -1517,22 +1521,32 @@ static int be_block(rco_case *c, int i, double *jm)
 }
 
 /* backwardEuler_LU_SGS, Time_BackwardEuler.f90:58-324 */
-int rco_be_lusgs(rco_case *c, double dt, int inner)
+/* backwardEuler_LU_SGS (Time_BackwardEuler.f90:58-324) and, with bdf2,
+ * BDF2_LU_SGS(Inner_rk, phi) (Time_BDF2.f90:71-337): the same sweeps, the
+ * BDF2 right-hand side and diagonal weighing the two previous steps' states
+ * WP, WP2 with phi */
+static int be_lusgs(rco_case *c, double dt, int inner, int bdf2, double phi)
 {
     int i, k, nei, nvar = c->nvar;
     double dflux[RCO_MAXSP + NFLOW], jm[(RCO_MAXSP + NFLOW) * (RCO_MAXSP + NFLOW)], rhs[RCO_MAXSP + NFLOW],
            sol[RCO_MAXSP + NFLOW];
     if (inner == 1)
         for (i = 1; i <= c->ne; i++)
-            for (k = 1; k <= nvar; k++) c->fixed[(i - 1) * nvar + k - 1] = c->vol[i - 1] / dt * WA(k, i);
+            for (k = 1; k <= nvar; k++) {
+                const size_t q = (size_t)(i - 1) * nvar + k - 1;
+                c->fixed[q] = bdf2 ? c->vol[i - 1] / dt * ((1 + 2 * phi) * c->wp[q] - phi * c->wp2[q])
+                                   : c->vol[i - 1] / dt * WA(k, i);
+            }
     for (i = 1; i <= c->ne; i++) {
         double d = 0.0;
         for (nei = 1; nei <= 3; nei++) d = d + 0.5 * c->alpaf[NEIGHF(nei, i) - 1];
-        c->dd[i - 1] = d + c->vol[i - 1] * (1. / c->step[i - 1] + 1. / dt);
+        c->dd[i - 1] = bdf2 ? d + c->vol[i - 1] * (1. / c->step[i - 1] + (1 + phi) / dt)
+                            : d + c->vol[i - 1] * (1. / c->step[i - 1] + 1. / dt);
     }
     for (i = 1; i <= c->ne; i++)
         for (k = 1; k <= nvar; k++)
-            HWA(k, i) = HWA(k, i) - c->vol[i - 1] / dt * WA(k, i) + c->fixed[(i - 1) * nvar + k - 1];
+            HWA(k, i) = bdf2 ? HWA(k, i) - (1 + phi) * c->vol[i - 1] / dt * WA(k, i) + c->fixed[(i - 1) * nvar + k - 1]
+                             : HWA(k, i) - c->vol[i - 1] / dt * WA(k, i) + c->fixed[(i - 1) * nvar + k - 1];
     for (int pass = 0; pass < 2; pass++) {
         for (int ii = 1; ii <= c->ne; ii++) {
             i = pass == 0 ? ii : c->ne + 1 - ii;
@@ -1554,6 +1568,24 @@ int rco_be_lusgs(rco_case *c, double dt, int inner)
     for (i = 1; i <= c->ne; i++)
         for (k = 1; k <= nvar; k++) PA(k, i) = PA(k, i) + HWA(k, i);
     return prim_to_cons(c);
+}
+
+int rco_be_lusgs(rco_case *c, double dt, int inner) { return be_lusgs(c, dt, inner, 0, 0.0); }
+int rco_bdf2_lusgs(rco_case *c, double dt, int inner, double phi) { return be_lusgs(c, dt, inner, 1, phi); }
+
+/* the prologue of BDF2_stage (Time_BDF2.f90:12-20): at the first step WP =
+ * WA, WP2 = WP, phi = 0; later WP2 = WP, WP = WA, phi = 0.5; returns phi */
+double rco_bdf2_begin_step(rco_case *c, int first)
+{
+    const size_t n = (size_t)c->nvar * c->ne;
+    if (first) {
+        memcpy(c->wp, c->wa, n * sizeof(double));
+        memcpy(c->wp2, c->wp, n * sizeof(double));
+        return 0.0;
+    }
+    memcpy(c->wp2, c->wp, n * sizeof(double));
+    memcpy(c->wp, c->wa, n * sizeof(double));
+    return 0.5;
 }
 
 /* backwardEuler_stage, Time_BackwardEuler.f90:4-56 (without its closing

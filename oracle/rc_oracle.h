@@ -76,6 +76,8 @@ int  rco_con_to_pri_jaco(const rco_mech *m, const double *pri, double *jaco_rowm
 int  rco_dwdp(const rco_mech *m, const double *pri, double *dwdp_rowmajor);
 int  rco_solve_dgesvx(int n, const double *a_rowmajor, const double *b, double *x);
 int  rco_be_lusgs(rco_case *c, double dt, int inner);
+int  rco_bdf2_lusgs(rco_case *c, double dt, int inner, double phi);
+double rco_bdf2_begin_step(rco_case *c, int first);
 int  rco_be_stage(rco_case *c, double dt, int innerloop, double reltol, int *ninner, double *roress);
 void rco_set_p_out(rco_case *c, double p_out);
 /* viscous path (IfViscous): transport tables, then TimeStep's viscous

@@ -17,7 +17,9 @@
 !! Unsteady decks with implicit_time_method=3 run the dual-time backward
 !! Euler steps instead (fvm2d.f90:60-120, backwardEuler_stage of
 !! Time_BackwardEuler.f90:4-56 with its closing reconstruction/flux_sourceterm,
-!! then step_residual's norms into OUTDIR/resid_step.bin): inner
+!! then step_residual's norms into OUTDIR/resid_step.bin), and decks with
+!! implicit_time_method=5 the BDF2 steps the same way (BDF2_stage of
+!! Time_BDF2.f90:4-69 with its WP/WP2 prologue and BDF2_LU_SGS): inner
 !! iteration k overall (counted across steps) dumps s<k>_res, s<k>_dPA,
 !! s<k>_PA, s<k>_WA for k <= NDUMP, and resid.bin holds rows
 !! (istep, inner, Res(1:Nvar)/total_volume, Romax).
@@ -28,7 +30,7 @@ program rc_ref_dump
     include 'mpif.h'
     integer :: istep, nsteps, ndump, ierr, timing, k, inner, kin
     character(len=256) :: arg, outdir
-    double precision :: res(64), romax, t0, t1, tstage(6), roress, roress1, ttime_n
+    double precision :: res(64), romax, t0, t1, tstage(6), roress, roress1, ttime_n, phi
     integer(8) :: c0, c1, crate
 
     call get_command_argument(1, arg); read(arg,*) nsteps
@@ -53,8 +55,8 @@ program rc_ref_dump
     tstage = 0.
     call system_clock(c0, crate)
     if (.not. IfSteady) then
-        if (IfExplicit .or. implicit_time_method /= 3) then
-            write(*,*) 'rc_ref_dump: unsteady runs are dumped for implicit_time_method=3 only'
+        if (IfExplicit .or. (implicit_time_method /= 3 .and. implicit_time_method /= 5)) then
+            write(*,*) 'rc_ref_dump: unsteady runs are dumped for implicit_time_method=3 or 5 only'
             stop 2
         endif
         kin = 0
@@ -63,13 +65,29 @@ program rc_ref_dump
             ttime_n = ttime
             ttime = ttime_n + cs(1)*dt
             call compute_p_out
+            if (implicit_time_method == 5) then
+                ! the prologue of BDF2_stage (Time_BDF2.f90:12-20)
+                if (istep == 1) then
+                    WP = WA
+                    WP2 = WP
+                    phi = 0.
+                else
+                    WP2 = WP
+                    WP = WA
+                    phi = 0.5
+                endif
+            endif
             do inner = 1, InnerLoop
                 kin = kin + 1
                 call TimeStep
                 call reconstruction
                 call flux_sourceterm
                 if (kin <= ndump) call dump_step_d(outdir, kin, 'res', hWA(:,1:NE))
-                call backwardEuler_LU_SGS(inner)
+                if (implicit_time_method == 5) then
+                    call BDF2_LU_SGS(inner, phi)
+                else
+                    call backwardEuler_LU_SGS(inner)
+                endif
                 if (kin <= ndump) then
                     call dump_step_d(outdir, kin, 'dPA', hWA(:,1:NE))
                     call dump_step_d(outdir, kin, 'PA', PA(:,1:NE))

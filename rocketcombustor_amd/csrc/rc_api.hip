@@ -30,7 +30,7 @@ void launch_update(const LaunchSet &, hipStream_t);
 void launch_prim_to_cons(const LaunchSet &, hipStream_t);
 void launch_resnorm(const LaunchSet &, hipStream_t);
 void launch_derived(const LaunchSet &, double *, hipStream_t);
-void launch_be_prepare(const LaunchSet &, double, bool, hipStream_t);
+void launch_be_prepare(const LaunchSet &, double, bool, const double *, const double *, double, hipStream_t);
 void launch_be_factor(const LaunchSet &, double *, int *, double *, hipStream_t);
 void launch_be_level(const LaunchSet &, bool, int, const double *, const int *, const double *, hipStream_t);
 size_t be_factor_doubles();
@@ -70,6 +70,8 @@ struct rc_case {
     double *be_fac = nullptr;      // backward Euler: per-cell block factors (allocated at first use)
     int *be_piv = nullptr;
     double *be_nbc = nullptr;      // backward Euler: per-cell thermo records of the neighbour terms
+    double *bdf_wp = nullptr;      // BDF2: WP and WP2, the states at the start of the last two steps
+    double *bdf_wp2 = nullptr;     //       (Nvar arrays of sc each; allocated at the first BDF2 call)
     Surrogate *sur = nullptr;      // learned-chemistry source (null: exact kinetics)
     double *sur_src = nullptr;     // its per-cell rates, (Nsp-1) arrays of sc
     long long sur_evals = 0;       // cell evaluations since the surrogate was set
@@ -685,12 +687,37 @@ int ensure_be_storage(rc_case *h) {
     return RC_OK;
 }
 
+// the BDF2 states (allocated once, at the first BDF2 call of the case)
+int ensure_bdf2_storage(rc_case *h) {
+    if (h->bdf_wp) return RC_OK;
+    HIPCHK(h, hipMalloc(&h->bdf_wp, sizeof(double) * kNvar * h->M.sc));
+    HIPCHK(h, hipMalloc(&h->bdf_wp2, sizeof(double) * kNvar * h->M.sc));
+    return RC_OK;
+}
+
+// the prologue of BDF2_stage (Time_BDF2.f90:12-20): at the first step WP =
+// WA, WP2 = WP, phi = 0; later WP2 = WP, WP = WA, phi = 0.5
+int bdf2_begin_step(rc_case *h, bool first, double *phi) {
+    const size_t bytes = sizeof(double) * kNvar * h->M.sc;
+    if (first) {
+        HIPCHK(h, hipMemcpyAsync(h->bdf_wp, h->S.wa, bytes, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->bdf_wp2, h->bdf_wp, bytes, hipMemcpyDeviceToDevice, h->stream));
+        *phi = 0.0;
+    } else {
+        HIPCHK(h, hipMemcpyAsync(h->bdf_wp2, h->bdf_wp, bytes, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->bdf_wp, h->S.wa, bytes, hipMemcpyDeviceToDevice, h->stream));
+        *phi = 0.5;
+    }
+    return RC_OK;
+}
+
 // backwardEuler_LU_SGS: dual-time right-hand side and diagonal, every
 // cell's block factored, the point-implicit sweeps level by level, PA +=
-// dPA and WA = U(PA)
-void enqueue_be_lusgs(rc_case *h, double dt, bool first) {
+// dPA and WA = U(PA); with bdf2 the right-hand side and diagonal of
+// BDF2_LU_SGS(Inner_rk, phi) (the sweeps are the same)
+void enqueue_be_lusgs(rc_case *h, double dt, bool first, bool bdf2 = false, double phi = 0.0) {
     const LaunchSet &L = h->L;
-    launch_be_prepare(L, dt, first, h->stream);
+    launch_be_prepare(L, dt, first, bdf2 ? h->bdf_wp : nullptr, bdf2 ? h->bdf_wp2 : nullptr, phi, h->stream);
     launch_be_factor(L, h->be_fac, h->be_piv, h->be_nbc, h->stream);
     for (int l = 0; l < h->npl; ++l) launch_be_level(L, true, l, h->be_fac, h->be_piv, h->be_nbc, h->stream);
     for (int l = h->npl - 1; l >= 0; --l)
@@ -960,6 +987,8 @@ void rc_case_destroy(rc_case *h) {
     if (h->be_fac) (void)hipFree(h->be_fac);
     if (h->be_piv) (void)hipFree(h->be_piv);
     if (h->be_nbc) (void)hipFree(h->be_nbc);
+    if (h->bdf_wp) (void)hipFree(h->bdf_wp);
+    if (h->bdf_wp2) (void)hipFree(h->bdf_wp2);
     if (h->sur) surrogate_destroy(h->sur);
     if (h->sur_src) (void)hipFree(h->sur_src);
     for (auto &ev : h->evpool) (void)hipEventDestroy(ev);
@@ -1198,12 +1227,13 @@ int rc_backward_euler_lusgs(rc_case *h, double dt, int inner) {
     return sync_check(h);
 }
 
-int rc_backward_euler_stage(rc_case *h, double dt, int innerloop, double reltol, int *ninner, double *res_out) {
-    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
-    if (!(dt > 0.0) || innerloop < 1) return set_err(h, RC_ERR_ARGUMENT, "dt must be > 0 and innerloop >= 1");
-    HIPCHK(h, hipSetDevice(h->device));
-    int rc0 = ensure_be_storage(h);
-    if (rc0) return rc0;
+}  // extern "C"
+
+namespace {
+// backwardEuler_stage (Time_BackwardEuler.f90:4-56), or with bdf2 BDF2_stage
+// (Time_BDF2.f90:4-69): the same inner loop around the two LU-SGS variants
+int dual_time_stage(rc_case *h, double dt, int innerloop, double reltol, int *ninner, double *res_out, bool bdf2,
+                    double phi) {
     const LaunchSet &L = h->L;
     launch_step_begin(L, h->stream);
     double ro1 = 0.0;
@@ -1220,7 +1250,7 @@ int rc_backward_euler_stage(rc_case *h, double dt, int innerloop, double reltol,
             h->sur_evals += h->ne;
         }
         launch_residual(L, false, h->stream);
-        enqueue_be_lusgs(h, dt, inner == 1);
+        enqueue_be_lusgs(h, dt, inner == 1, bdf2, phi);
         int rc = residual_async(h, h->h_res);
         if (rc) return rc;
         rc = sync_check(h);
@@ -1256,6 +1286,53 @@ int rc_backward_euler_stage(rc_case *h, double dt, int innerloop, double reltol,
     }
     h->steps_done++;
     return RC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int rc_backward_euler_stage(rc_case *h, double dt, int innerloop, double reltol, int *ninner, double *res_out) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    if (!(dt > 0.0) || innerloop < 1) return set_err(h, RC_ERR_ARGUMENT, "dt must be > 0 and innerloop >= 1");
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc0 = ensure_be_storage(h);
+    if (rc0) return rc0;
+    return dual_time_stage(h, dt, innerloop, reltol, ninner, res_out, false, 0.0);
+}
+
+int rc_bdf2_begin_step(rc_case *h, int first, double *phi) {
+    if (!h || !phi) return set_err(h, RC_ERR_ARGUMENT, "NULL argument");
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc = ensure_bdf2_storage(h);
+    if (rc) return rc;
+    rc = bdf2_begin_step(h, first != 0, phi);
+    if (rc) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return RC_OK;
+}
+
+int rc_bdf2_lusgs(rc_case *h, double dt, int inner, double phi) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    if (!(dt > 0.0) || inner < 1) return set_err(h, RC_ERR_ARGUMENT, "dt must be > 0 and inner >= 1");
+    HIPCHK(h, hipSetDevice(h->device));
+    if (!h->bdf_wp) return set_err(h, RC_ERR_ARGUMENT, "rc_bdf2_begin_step first");
+    int rc = ensure_be_storage(h);
+    if (rc) return rc;
+    enqueue_be_lusgs(h, dt, inner == 1, true, phi);
+    return sync_check(h);
+}
+
+int rc_bdf2_stage(rc_case *h, double dt, int innerloop, double reltol, int first, int *ninner, double *res_out) {
+    if (!h) return set_err(h, RC_ERR_ARGUMENT, "NULL handle");
+    if (!(dt > 0.0) || innerloop < 1) return set_err(h, RC_ERR_ARGUMENT, "dt must be > 0 and innerloop >= 1");
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc = ensure_be_storage(h);
+    if (!rc) rc = ensure_bdf2_storage(h);
+    if (rc) return rc;
+    double phi = 0.0;
+    rc = bdf2_begin_step(h, first != 0, &phi);
+    if (rc) return rc;
+    return dual_time_stage(h, dt, innerloop, reltol, ninner, res_out, true, phi);
 }
 
 int rc_residual(rc_case *h, double *res) {

@@ -147,8 +147,12 @@ __device__ __forceinline__ void dkf_dpa(TT &th, double p, double T, double rho, 
 }
 
 // dd, fixed and the dual-time right-hand side (Time_BackwardEuler.f90:73-100)
+// With wp (BDF2_LU_SGS, Time_BDF2.f90:88-110): fixed from the states at the
+// start of the last two steps, wp and wp2 (Nvar arrays of sc), weighted
+// with phi; dd and the right-hand side carry (1+phi)/dt.
 template <int NSP, bool B>
-__global__ void __launch_bounds__(256) k_be_prepare(ITEM_ARGS, double dt, int first) {
+__global__ void __launch_bounds__(256) k_be_prepare(ITEM_ARGS, double dt, int first, const double *__restrict__ wp,
+                                                    const double *__restrict__ wp2, double phi) {
     ITEM_PROLOGUE(i);
     double d = 0.0;
 #pragma unroll
@@ -159,6 +163,17 @@ __global__ void __launch_bounds__(256) k_be_prepare(ITEM_ARGS, double dt, int fi
         d = d + 0.5 * al;
     }
     const double v = M.vol[i];
+    if (wp) {
+        d = d + v * (1. / S.step[i] + (1 + phi) / dt);
+        S.dd[i] = d;
+        const RC_G double *p1 = gp(wp), *p2 = gp(wp2);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            if (first) FIX_(k, i) = v / dt * ((1 + 2 * phi) * p1[k * M.sc + i] - phi * p2[k * M.sc + i]);
+            HWA_(k, i) = HWA_(k, i) - (1 + phi) * v / dt * WA_(k, i) + FIX_(k, i);
+        }
+        return;
+    }
     d = d + v * (1. / S.step[i] + 1. / dt);
     S.dd[i] = d;
 #pragma unroll
@@ -546,15 +561,16 @@ __global__ void __launch_bounds__(256) k_be_update(ITEM_ARGS) {
     if (!cell_prim_to_cons<NSP>(M, S, th, i, pa)) flag_error(S.err, RC_ERR_TEMPERATURE, i, K_BE_UPDATE);
 }
 
-void launch_be_prepare(const LaunchSet &L, double dt, bool first, hipStream_t st) {
+void launch_be_prepare(const LaunchSet &L, double dt, bool first, const double *wp, const double *wp2, double phi,
+                       hipStream_t st) {
     const int f = first ? 1 : 0;
     if (L.single) {
         const int n = L.cell.hi - L.cell.lo;
         if (n > 0)
             k_be_prepare<kNsp, false><<<(n + kChunk - 1) / kChunk, kChunk, 0, st>>>(L.cases, nullptr, L.cell.lo,
-                                                                                    L.cell.hi, dt, f);
+                                                                                    L.cell.hi, dt, f, wp, wp2, phi);
     } else if (L.cell.n) {
-        k_be_prepare<kNsp, true><<<L.cell.n, kChunk, 0, st>>>(L.cases, L.cell.w, 0, 0, dt, f);
+        k_be_prepare<kNsp, true><<<L.cell.n, kChunk, 0, st>>>(L.cases, L.cell.w, 0, 0, dt, f, wp, wp2, phi);
     }
 }
 

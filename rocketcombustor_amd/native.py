@@ -103,6 +103,9 @@ SIGNATURES = {
     "rc_get_surrogate_stats": (C.c_int, [_VP, P(C.c_longlong)]),
     "rc_backward_euler_lusgs": (C.c_int, [_VP, C.c_double, C.c_int]),
     "rc_backward_euler_stage": (C.c_int, [_VP, C.c_double, C.c_int, C.c_double, _IP, _DP]),
+    "rc_bdf2_begin_step": (C.c_int, [_VP, C.c_int, _DP]),
+    "rc_bdf2_lusgs": (C.c_int, [_VP, C.c_double, C.c_int, C.c_double]),
+    "rc_bdf2_stage": (C.c_int, [_VP, C.c_double, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int), _DP]),
     "rc_get_primitive": (C.c_int, [_VP, _DP]),
     "rc_get_conserved": (C.c_int, [_VP, _DP]),
     "rc_get_increment": (C.c_int, [_VP, _DP]),

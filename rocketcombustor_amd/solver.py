@@ -20,7 +20,10 @@ methods keep the reference's names and call sequence:
     set_outlet_pressure(p)        compute_p_out         src/fvm2d.f90:229-239
     backward_euler_lusgs(dt, k)   backwardEuler_LU_SGS  src/Time_BackwardEuler.f90:58
     backward_euler_stage(dt, ..)  backwardEuler_stage   src/Time_BackwardEuler.f90:4
-    run_unsteady(n)     unsteady main loop  src/fvm2d.f90:48-120 (implicit_time_method=3)
+    bdf2_begin_step(first)        BDF2_stage prologue   src/Time_BDF2.f90:12-20
+    bdf2_lusgs(dt, k, phi)        BDF2_LU_SGS           src/Time_BDF2.f90:71
+    bdf2_stage(dt, .., first)     BDF2_stage            src/Time_BDF2.f90:4
+    run_unsteady(n)     unsteady main loop  src/fvm2d.f90:48-120 (implicit_time_method=3 or 5)
 
 Errors where the reference stops raise :class:`native.RcError`.
 
@@ -229,6 +232,33 @@ class SteadyCase:
                 raise
         return buf[:n.value]
 
+    def bdf2_begin_step(self, first: bool) -> float:
+        """The WP/WP2 prologue of BDF2_stage; returns phi (0 at the first
+        step, 0.5 after)."""
+        phi = C.c_double(0.0)
+        self._call("rc_bdf2_begin_step", 1 if first else 0, C.byref(phi))
+        return phi.value
+
+    def bdf2_lusgs(self, dt: float, inner: int, phi: float) -> None:
+        """BDF2_LU_SGS(inner, phi) after timestep, reconstruction and
+        flux_sourceterm (bdf2_begin_step at the start of the step)."""
+        self._call("rc_bdf2_lusgs", float(dt), int(inner), float(phi))
+
+    def bdf2_stage(self, dt: float, innerloop: int, reltol: float, first: bool) -> np.ndarray:
+        """BDF2_stage: the WP/WP2 prologue and the inner iterations of one
+        physical step; returns their inner_iteration_residual rows."""
+        buf = np.zeros((max(int(innerloop), 1), NVAR + 1))
+        n = C.c_int(0)
+        rc = lib().rc_bdf2_stage(self.h, float(dt), int(innerloop), float(reltol), 1 if first else 0, C.byref(n),
+                                 dptr(buf))
+        if rc:
+            try:
+                check(rc, self.h)
+            except native.RcError as e:
+                e.inner = n.value
+                raise
+        return buf[:n.value]
+
     def run_unsteady(self, nsteps: int, dt: Optional[float] = None, innerloop: Optional[int] = None,
                      reltol: Optional[float] = None, a_p: Optional[float] = None, f_p: Optional[float] = None,
                      ttime: float = 0.0, step_rows: Optional[list] = None) -> np.ndarray:
@@ -254,7 +284,10 @@ class SteadyCase:
             tn = ttime
             ttime = tn + 1. * dt
             self.set_outlet_pressure(outlet_pressure(p_static, a_p, f_p, ttime))
-            h = self.backward_euler_stage(dt, innerloop, reltol)
+            if info.implicit_time_method == 5:
+                h = self.bdf2_stage(dt, innerloop, reltol, istep == 1)
+            else:
+                h = self.backward_euler_stage(dt, innerloop, reltol)
             rows += [np.r_[istep, k + 1, h[k]] for k in range(len(h))]
             ttime = tn + dt
             if step_rows is not None:

@@ -108,7 +108,8 @@ def case_arrays(case: nozzle.Case) -> dict:
         "info": np.array([case.info.cfl, case.info.limit_id, float(case.info.combustion), case.info.p_out_ratio]),
         # dual-time backward Euler: steady flag, dt, InnerLoop, relTol, A_p, f_p
         "unsteady": np.array([float(case.info.steady), case.info.totaltime / case.info.nstep_max,
-                              case.info.innerloop, case.info.reltol, case.info.a_p, case.info.f_p]),
+                              case.info.innerloop, case.info.reltol, case.info.a_p, case.info.f_p,
+                              float(case.info.implicit_time_method)]),
         "viscous": np.array([float(case.info.viscous)]),
         **(transport_arrays(case.transport) if case.info.viscous else {}),
     }
@@ -187,8 +188,8 @@ def main() -> None:
                                                  p_out_ratio=0.9), nsteps=20, ndump=2)
     # dual-time backward Euler with the point-implicit LU-SGS
     # (implicit_time_method=3); totaltime is set so that dt = totaltime/nstep_max
-    def be(case, nsteps, dt, inner, reltol, a_p=0.0, f_p=0.0):
-        case.info = dataclasses.replace(case.info, steady=False, implicit_time_method=3, innerloop=inner,
+    def be(case, nsteps, dt, inner, reltol, a_p=0.0, f_p=0.0, method=3):
+        case.info = dataclasses.replace(case.info, steady=False, implicit_time_method=method, innerloop=inner,
                                         reltol=reltol, totaltime=dt * nsteps, nstep_max=nsteps, a_p=a_p, f_p=f_p)
         return case
     make("n24x6_be_bc_variants", be(nozzle.nozzle_case(24, 6, wall_type=1, fuel_faces=2, subsonic_exit=True,
@@ -205,6 +206,13 @@ def main() -> None:
     make("n24x6_be_viscous", be(viscous(nozzle.nozzle_case(24, 6, wall_type=1, fuel_faces=2, subsonic_exit=True,
                                                            p_out_ratio=0.9)), 6, 1e-5, 3, 0.5),
          nsteps=6, ndump=3)
+    # BDF2 (implicit_time_method=5): the same point-implicit sweeps with the
+    # second-order right-hand side and diagonal
+    make("n24x6_be_bdf2_bc_variants", be(nozzle.nozzle_case(24, 6, wall_type=1, fuel_faces=2, subsonic_exit=True,
+                                                            p_out_ratio=0.9), 8, 1e-5, 4, 0.7, 0.1, 5000.0, method=5),
+         nsteps=8, ndump=6)
+    make("n24x6_be_bdf2_viscous", be(viscous(nozzle.nozzle_case(24, 6)), 6, 1e-5, 3, 0.5, method=5),
+         nsteps=6, ndump=4)
     conv = nozzle.nozzle_case(30, 8)
     make("n30x8_converged", conv, nsteps=8000, ndump=0, every=10)
     perm, _ = native.order_multicolor(conv.mesh.cells)

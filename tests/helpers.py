@@ -15,6 +15,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 def fixture_names(kind: str = "steady"):
     """Golden fixtures by kind: ``be`` the dual-time backward Euler path;
     ``steady`` steady fixtures with the reference's set-up arrays;
+    ``serving`` the long serving-family runs to the reference's stop;
     ``percall`` those of them that also hold per-call dumps (not the
     8000-step converged runs); ``trajectory`` every steady whole-run fixture,
     including the config-size runs that hold only residual rows and the
@@ -23,7 +24,9 @@ def fixture_names(kind: str = "steady"):
     be = [n for n in names if n.startswith("n24x6_be_")]
     if kind == "be":
         return be
-    steady = [n for n in names if n not in be]
+    if kind == "serving":
+        return [n for n in names if n.startswith("serving_")]
+    steady = [n for n in names if n not in be and not n.startswith("serving_")]
     if kind == "trajectory":
         return steady
     steady = [n for n in steady if "_config" not in n]
@@ -51,8 +54,9 @@ def case_from_fixture(d: dict) -> nozzle.Case:
     cfl, limit_id, comb, por = d["info"]
     info = deck.SolverInfo(cfl=cfl, limit_id=int(limit_id), combustion=bool(comb), p_out_ratio=por)
     if "unsteady" in d and not d["unsteady"][0]:
-        _, dt, inner, reltol, a_p, f_p = d["unsteady"]
-        info.steady, info.implicit_time_method, info.innerloop, info.reltol = False, 3, int(inner), reltol
+        _, dt, inner, reltol, a_p, f_p = d["unsteady"][:6]
+        method = int(d["unsteady"][6]) if len(d["unsteady"]) > 6 else 3
+        info.steady, info.implicit_time_method, info.innerloop, info.reltol = False, method, int(inner), reltol
         info.a_p, info.f_p, info.nstep_max = a_p, f_p, int(d["nsteps"])
         info.totaltime = dt * info.nstep_max
     transport = None
@@ -135,9 +139,11 @@ def outlet_pressure(p_static, a_p, f_p, t):
     return p_static * (1. + a_p * math.sin(2 * math.pi * f_p * t))
 
 
-def run_be(o, case, nsteps, on_inner=None, rows=None, lusgs="be_lusgs", set_p="set_p_out", step_rows=None):
+def run_be(o, case, nsteps, on_inner=None, rows=None, lusgs="be_lusgs", set_p="set_p_out", step_rows=None,
+           bdf2=("bdf2_begin_step", "bdf2_lusgs")):
     """The reference's time loop (src/fvm2d.f90:60-110) around
-    backwardEuler_stage (src/Time_BackwardEuler.f90:4-56), driven call by
+    backwardEuler_stage (src/Time_BackwardEuler.f90:4-56), or BDF2_stage
+    (src/Time_BDF2.f90:4-69) for implicit_time_method = 5, driven call by
     call on ``o`` (the oracle, or a device case with ``lusgs`` /
     ``set_p`` naming its methods): rows (istep, inner, Res/total_volume,
     Romax) of every inner iteration, appended to ``rows`` as they come;
@@ -146,12 +152,16 @@ def run_be(o, case, nsteps, on_inner=None, rows=None, lusgs="be_lusgs", set_p="s
     (istep, Res/total_volume, Romax) of step_residual after every step are
     appended to ``step_rows``."""
     dt, inner, reltol, a_p, f_p = be_params(case)
+    is_bdf2 = case.info.implicit_time_method == 5
     ttime, kin = 0.0, 0
     rows = [] if rows is None else rows
     for istep in range(1, nsteps + 1):
         tn = ttime
         ttime = tn + 1.0 * dt
         getattr(o, set_p)(outlet_pressure(case.p_out, a_p, f_p, ttime))
+        if is_bdf2:
+            # the WP/WP2 prologue of BDF2_stage (Time_BDF2.f90:12-20)
+            phi = getattr(o, bdf2[0])(istep == 1)
         ro1 = None
         for it in range(1, inner + 1):
             kin += 1
@@ -160,7 +170,10 @@ def run_be(o, case, nsteps, on_inner=None, rows=None, lusgs="be_lusgs", set_p="s
             o.flux_sourceterm()
             if on_inner:
                 on_inner(kin, "res")
-            getattr(o, lusgs)(dt, it)
+            if is_bdf2:
+                getattr(o, bdf2[1])(dt, it, phi)
+            else:
+                getattr(o, lusgs)(dt, it)
             if on_inner:
                 on_inner(kin, "lu")
             r = o.residual()

@@ -93,6 +93,15 @@ def test_inner_iterations_vs_reference_and_oracle(name):
                 o.be_lusgs(dt, it)
                 g.backward_euler_lusgs(dt, it)
 
+            def bdf2_begin_step(self, first):
+                phi = g.bdf2_begin_step(first)
+                assert phi == o.bdf2_begin_step(first)
+                return phi
+
+            def bdf2_lusgs(self, dt, it, phi):
+                o.bdf2_lusgs(dt, it, phi)
+                g.bdf2_lusgs(dt, it, phi)
+
             def residual(self):
                 return g.residual()
 
