@@ -154,6 +154,18 @@ module rocketcombustor_iface
             real(c_double), intent(out) :: res_out(*)
         end function
 
+        !! dual-time BDF2 (implicit_time_method = 5): BDF2_stage(istep) with
+        !! first = 1 at istep == 1 (Time_BDF2.f90:4-69)
+        integer(c_int) function rc_bdf2_stage(h, dt, innerloop, reltol, first, ninner, res_out) &
+                bind(c, name="rc_bdf2_stage")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), value :: dt, reltol
+            integer(c_int), value :: innerloop, first
+            integer(c_int), intent(out) :: ninner
+            real(c_double), intent(out) :: res_out(*)
+        end function
+
         integer(c_int) function rc_residual(h, res) bind(c, name="rc_residual")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: h

@@ -6,9 +6,10 @@
 !! objects; the only change is the steady step of the main loop
 !! (src/fvm2d.f90:76), where `call lu_sgs_stage(istep)` becomes a call of
 !! the MI355X library through include/rocketcombustor_iface.f90, and, for
-!! unsteady decks with implicit_time_method=3, the backward-Euler stage
-!! (src/fvm2d.f90:93), where `call backwardEuler_stage(istep)` becomes
-!! rc_backward_euler_stage after the reference's own compute_p_out hands
+!! unsteady decks with implicit_time_method=3 or 5, the dual-time stage
+!! (src/fvm2d.f90:93 and :95), where `call backwardEuler_stage(istep)` and
+!! `call BDF2_stage(istep)` become rc_backward_euler_stage and rc_bdf2_stage
+!! after the reference's own compute_p_out hands
 !! its exit pressure to the library.  Unsteady runs write the inner-iteration
 !! rows (istep, inner, Res/total_volume, Romax) to OUTDIR/resid.bin and
 !! converge_uns.dat, and, at the main loop's screen-output steps, the
@@ -119,8 +120,8 @@ program rc_ref_host_gpu
     if (.not. IfSteady) then
         !! the main loop, unsteady implicit branch with backward Euler
         !! (fvm2d.f90:48-120)
-        if (IfExplicit .or. implicit_time_method /= 3) then
-            write(*,*) 'rc_ref_host_gpu: unsteady runs need implicit_time_method=3'
+        if (IfExplicit .or. (implicit_time_method /= 3 .and. implicit_time_method /= 5)) then
+            write(*,*) 'rc_ref_host_gpu: unsteady runs need implicit_time_method=3 or 5'
             stop 2
         endif
         allocate(rows(Nvar + 1, InnerLoop))
@@ -138,8 +139,13 @@ program rc_ref_host_gpu
             ttime = ttime_n + cs(1)*dt
             call compute_p_out
             call rc_check(rc_set_outlet_pressure(h, p_out), h, 'compute_p_out')
-            call rc_check(rc_backward_euler_stage(h, dt, InnerLoop, relTol, ninner, rows), h, &
-                          'backwardEuler_stage')                        ! was: call backwardEuler_stage(istep)
+            if (implicit_time_method == 5) then
+                call rc_check(rc_bdf2_stage(h, dt, InnerLoop, relTol, merge(1, 0, istep == 1), ninner, rows), &
+                              h, 'BDF2_stage')                          ! was: call BDF2_stage(istep)
+            else
+                call rc_check(rc_backward_euler_stage(h, dt, InnerLoop, relTol, ninner, rows), h, &
+                              'backwardEuler_stage')                    ! was: call backwardEuler_stage(istep)
+            endif
             do k = 1, ninner
                 write(91) dble(istep), dble(k), rows(:, k)
                 write(93,*) istep, k, rows(1:Nvar, k), rows(Nvar + 1, k)

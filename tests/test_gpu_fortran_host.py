@@ -1,6 +1,6 @@
 """The drop-in: the reference's own Fortran host (start-up, mesh, restart,
 residual monitor, backup) with its steady step -- or, for unsteady decks,
-its backward-Euler stage -- replaced by the MI355X library through
+its backward-Euler or BDF2 stage -- replaced by the MI355X library through
 include/rocketcombustor_iface.f90 (oracle/ref_host_gpu.f90, linked from the
 reference's objects by oracle/ref_build.sh).  Its residual history and final
 field must match the reference's own run (golden vectors).
@@ -22,7 +22,8 @@ BIN = os.path.join(ROOT, "oracle", "_ref", "rc_ref_host_gpu")
 
 
 @pytest.mark.parametrize("name", ["n24x6_natural", "n24x6_bc_variants", "n24x6_be_bc_variants", "n24x6_be_model1",
-                                  "n24x6_viscous_bc_variants", "n24x6_be_viscous"])
+                                  "n24x6_viscous_bc_variants", "n24x6_be_viscous", "n24x6_be_bdf2_bc_variants",
+                                  "n24x6_be_bdf2_viscous"])
 def test_reference_host_with_gpu_step(name):
     if not os.path.exists(BIN):
         pytest.skip("reference host binary not built (oracle/ref_build.sh needs the reference sources)")
