@@ -127,6 +127,7 @@ struct LaunchSet {
     std::vector<WorkList> level;   // LU-SGS runs of each level
     std::vector<WorkList> pi_level;   // point-implicit sweep: cells (positions in pi_cells) of each level
     FlowList flow;                 // the whole LU-SGS sweep of the fused step as one launch
+    std::vector<char> next_res;    // forward level l also forms level l+1's residual
     double *resout = nullptr;      // ncases x (kNvar+1) residual norms
 };
 

@@ -7,6 +7,8 @@
 // reference's face loops accumulate into the same cell; the LU-SGS sweeps
 // run level by level over wavefronts of the given cell order.
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "rc_items.h"
 
@@ -657,11 +659,53 @@ __device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int ten
     }
 }
 
-// one level of the sweep: kRunChunk runs of one case per thread block (the
-// work item), or runs lo + ... below hi of the one case
-template <int NSP, bool FWD, bool RES, bool B>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_lusgs(ITEM_ARGS) {
+// The residual (what k_residual forms, with the TimeStep cell part) of the
+// cells of runs t .. tend-1, lane p of a run's group taking its cell p.
+template <int NSP>
+__device__ __forceinline__ void run_residual(const RC_G CaseDev &C, int t, int tend) {
+    if (t >= tend) return;
+    const GMesh M = gview(C.M);
+    const GState S = gview(C.S);
+    GThermo &th = *gp(C.th);
+    GParams &PAR = *gp(C.par);
+    const int lane = threadIdx.x % kRunMax;
+    const int first = M.runs[2 * t], cnt = M.runs[2 * t + 1];
+    if (lane >= cnt) return;
+    const int i = first + lane;
+    double al[3];
+    cell_step_dd(M, S, PAR, i, al);
+    double h[NV];
+    cell_residual<NSP>(M, S, th, PAR, i, h);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) HWA_(k, i) = h[k];
+#pragma unroll
+    for (int k = NV; k < kRecHW; ++k) HWA_(k, i) = 0.0;
+}
+
+// One level of the sweep: kRunChunk runs of one case per thread block (the
+// work item), or runs lo + ... below hi of the one case.
+// NX (forward levels of the fused step): thread blocks nb1 and up form the
+// residual of the next level's runs instead (items work2[b - nb1], or runs
+// lo2 + ... below hi2), alongside the sweep: the forward sweep of a level
+// reads the increments of lower levels only and writes those of its own
+// cells, and the residual of a higher level's cell reads neither, so the
+// next level sweeps with its residual, step and dd already in memory.
+template <int NSP, bool FWD, bool RES, bool B, bool NX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_lusgs(
+        ITEM_ARGS, const int4 *__restrict__ work2, int lo2, int hi2, int nb1) {
     constexpr int R = kRunMax;
+    if constexpr (NX) {
+        if ((int)blockIdx.x >= nb1) {
+            const int b = (int)blockIdx.x - nb1;
+            if constexpr (B) {
+                const int4 w_ = work2[b];
+                run_residual<NSP>(gp(cases)[w_.x], w_.y + (int)threadIdx.x / R, w_.z);
+            } else {
+                run_residual<NSP>(*gp(cases), lo2 + (b * (int)blockDim.x + (int)threadIdx.x) / R, hi2);
+            }
+            return;
+        }
+    }
     const RC_G CaseDev *Cp_;
     int t, tend;
     if constexpr (B) {
@@ -916,23 +960,59 @@ void launch_residual(const LaunchSet &L, bool ts, hipStream_t st) {
     else LAUNCH(L.cell, kChunk, st, k_residual, kNsp, false);
 }
 
-// res: forward levels that also form their cells' residual (the fused step)
-template <bool FWD, bool RES>
-void launch_level(const LaunchSet &L, const WorkList &wl, hipStream_t st) {
+// res: forward levels that also form their cells' residual; nx: forward
+// levels that also form the residual of the next level's cells (list nx)
+template <bool FWD, bool RES, bool NX>
+void launch_level(const LaunchSet &L, const WorkList &wl, const WorkList *nx, hipStream_t st) {
     if (L.single) {
-        const int n = wl.hi - wl.lo;
-        if (n > 0)
-            k_lusgs<kNsp, FWD, RES, false><<<(n + kRunChunk - 1) / kRunChunk, kRunThreads, lusgs_lds(kRunThreads), st>>>(
-                L.cases, nullptr, wl.lo, wl.hi);
-    } else if (wl.n) {
-        k_lusgs<kNsp, FWD, RES, true><<<wl.n, wl.bs, lusgs_lds(wl.bs), st>>>(L.cases, wl.w, 0, 0);
+        const int n = wl.hi - wl.lo, nb1 = (n + kRunChunk - 1) / kRunChunk;
+        const int n2 = NX ? nx->hi - nx->lo : 0, nb2 = (n2 + kRunChunk - 1) / kRunChunk;
+        if (nb1 + nb2 > 0)
+            k_lusgs<kNsp, FWD, RES, false, NX><<<nb1 + nb2, kRunThreads, lusgs_lds(kRunThreads), st>>>(
+                L.cases, nullptr, wl.lo, wl.hi, nullptr, NX ? nx->lo : 0, NX ? nx->hi : 0, nb1);
+    } else {
+        const int nb2 = NX ? nx->n : 0;
+        if (wl.n + nb2 > 0)
+            k_lusgs<kNsp, FWD, RES, true, NX><<<wl.n + nb2, wl.bs, lusgs_lds(wl.bs), st>>>(
+                L.cases, wl.w, 0, 0, NX ? nx->w : nullptr, 0, 0, wl.n);
     }
 }
 
+// the forward level l of the fused step: it forms its own cells' residual
+// unless the launch of level l-1 formed it, and the next level's when
+// L.next_res[l] says so (the level lists then share one block size)
+void launch_lusgs_fused_forward(const LaunchSet &L, int l, hipStream_t st) {
+    const bool res = l == 0 || !L.next_res[l - 1];
+    const bool nx = L.next_res[l] != 0;
+    const WorkList *n = nx ? &L.level[l + 1] : nullptr;
+    if (res && nx) launch_level<true, true, true>(L, L.level[l], n, st);
+    else if (res) launch_level<true, true, false>(L, L.level[l], nullptr, st);
+    else if (nx) launch_level<true, false, true>(L, L.level[l], n, st);
+    else launch_level<true, false, false>(L, L.level[l], nullptr, st);
+}
+
+// threads of the forward level kernel with the next level's residual blocks
+// resident at once on the device (blocks of bs threads)
+long lusgs_resident_threads(int device, int bs) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, long> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({device, bs});
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lusgs<kNsp, true, true, true, true>, bs,
+                                                     lusgs_lds(bs)) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0;
+    const long n = (long)per_cu * cus * bs;
+    cache[{device, bs}] = n;
+    return n;
+}
+
 void launch_lusgs_level(const LaunchSet &L, bool fwd, int l, bool res, hipStream_t st) {
-    if (fwd && res) launch_level<true, true>(L, L.level[l], st);
-    else if (fwd) launch_level<true, false>(L, L.level[l], st);
-    else launch_level<false, false>(L, L.level[l], st);
+    if (fwd && res) launch_level<true, true, false>(L, L.level[l], nullptr, st);
+    else if (fwd) launch_level<true, false, false>(L, L.level[l], nullptr, st);
+    else launch_level<false, false, false>(L, L.level[l], nullptr, st);
 }
 
 void launch_update(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_update, kNsp); }

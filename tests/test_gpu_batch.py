@@ -94,6 +94,32 @@ def test_batch_dataflow_sweep_equals_alone(monkeypatch):
             g.close()
 
 
+def test_batch_next_level_residual_equals_alone(monkeypatch):
+    """Batch forward levels that form the next level's residual in extra
+    thread blocks (RC_NEXTRES=1, every level; members with different level
+    counts) give every member bitwise what it computes alone with each level
+    forming its own residual (RC_NEXTRES=0)."""
+    cases = mixed_cases()
+    n, every = 30, 10
+    announce("batch next-level residual, %d mixed cases, %d steps" % (len(cases), n))
+    monkeypatch.setenv("RC_NEXTRES", "0")
+    ref = [alone(c, n, every) for c in cases]
+    gs = [SteadyCase(c) for c in cases]
+    monkeypatch.setenv("RC_NEXTRES", "1")
+    try:
+        with SteadyBatch() as b:
+            for g in gs:
+                b.add(g)
+            hist = b.run_steady(n, every=every)
+            for k, g in enumerate(gs):
+                assert np.array_equal(hist[:, k], ref[k][0]), k
+                assert np.array_equal(g.primitive(), ref[k][1]), k
+    finally:
+        monkeypatch.delenv("RC_NEXTRES")
+        for g in gs:
+            g.close()
+
+
 def test_continuous_batching_join_and_leave():
     """A, B from the start; C joins after 30 steps; A leaves after 50; each
     ends bitwise where it ends alone after the same number of steps."""

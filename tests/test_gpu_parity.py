@@ -214,6 +214,51 @@ def test_dataflow_sweep_equals_level_launches(nx, ny, order, monkeypatch):
         lev.close()
 
 
+@pytest.mark.parametrize("nx,ny,order", [(1, 1, "natural"), (24, 6, "natural"), (150, 40, "multicolor"),
+                                         (1000, 200, "multicolor"), (150, 40, "natural")])
+def test_next_level_residual_equals_own_residual(nx, ny, order, xcd_layout, monkeypatch):
+    """Forward levels that form the next level's residual in extra thread
+    blocks of their launch (RC_NEXTRES=1: at every level; by default only
+    where both levels fit on the device at once) give bitwise the results of
+    levels that each form their own cells' residual (RC_NEXTRES=0), eager
+    and graph-replayed, on both work-list layouts; the oracle agrees."""
+    case = nozzle.nozzle_case(nx, ny)
+    if order == "multicolor":
+        perm, _ = native.order_multicolor(case.mesh.cells)
+        case = case.permuted(perm)
+    announce("next-level residual vs own residual %dx%d %s RC_XCD=%s NE=%d" % (nx, ny, order, xcd_layout,
+                                                                              case.mesh.ne))
+    monkeypatch.setenv("RC_NEXTRES", "0")
+    own = SteadyCase(case)
+    monkeypatch.setenv("RC_NEXTRES", "1")
+    nxt = SteadyCase(case)
+    monkeypatch.delenv("RC_NEXTRES")
+    dflt = SteadyCase(case)
+    try:
+        for k in range(2):
+            own.lu_sgs_stage()
+            nxt.lu_sgs_stage()
+            assert np.array_equal(nxt.primitive(), own.primitive()), k
+            assert np.array_equal(nxt.increment(), own.increment()), k
+        for k in range(3):
+            own.run_steady(2, history=False)
+            nxt.run_steady(2, history=False)
+            assert np.array_equal(nxt.primitive(), own.primitive()), k
+            assert np.array_equal(nxt.conserved(), own.conserved()), k
+        assert np.array_equal(nxt.residual(), own.residual())
+        dflt.run_steady(8, history=False)
+        assert np.array_equal(dflt.primitive(), own.primitive())
+        if case.mesh.ne <= 12000:
+            o = oracle.OracleCase(case)
+            for _ in range(8):
+                o.stage()
+            assert relerr(nxt.primitive(), o.get("PA"), -1) < 1e-12
+    finally:
+        own.close()
+        nxt.close()
+        dflt.close()
+
+
 def test_operator_calls_after_fused_steps():
     """The fused step leaves the cell-average cache of the pre-update state;
     operator-level calls made right after it refresh the cache first, so
