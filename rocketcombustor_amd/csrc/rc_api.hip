@@ -456,8 +456,10 @@ int layout_and_upload(rc_case *h) {
     int *d_ctr = nullptr;
     int4 *d_wc = nullptr, *d_wf = nullptr, *d_wg = nullptr, *d_wl = nullptr, *d_wp = nullptr;
     double *d_partial = nullptr, *d_resout = nullptr;
+    const int nruns = (int)m.runs.size() / 2;
+    const int ss = (int)align_up((size_t)std::max(4 * nruns, 1), 64);
     auto plan = [&](Carver &c, DevMesh &M, DevState &S) {
-        M.ne = m.ne; M.nf = m.nf; M.ng = m.ng; M.sc = sc; M.sf = sf; M.sg = sg;
+        M.ne = m.ne; M.nf = m.nf; M.ng = m.ng; M.sc = sc; M.sf = sf; M.sg = sg; M.ss = ss;
         S.pa = c.take<double>((size_t)kRecPA * sc);
         S.wa = c.take<double>((size_t)kNvar * sc);
         S.hwa = c.take<double>((size_t)kRecHW * sc);
@@ -481,6 +483,8 @@ int layout_and_upload(rc_case *h) {
         M.stencil = c.take<int>((size_t)3 * sc);
         M.cface = c.take<int>((size_t)3 * sc);
         M.runs = c.take<int>(m.runs.size());
+        M.slot = c.take<int>((size_t)SL_N * ss);
+        M.slot_n = c.take<double>((size_t)6 * ss);
         M.pi_cells = c.take<int>(m.pi_cells.size());
         M.vecx = c.take<double>((size_t)2 * sf);
         M.fvl = c.take<double>((size_t)2 * sf);
@@ -541,6 +545,27 @@ int layout_and_upload(rc_case *h) {
     auto cf = soa(m.cface, m.ne, 3, sc);
     if ((rc = up(M.cface, cf.data(), sizeof(int) * cf.size()))) return rc;
     if ((rc = up(M.runs, m.runs.data(), sizeof(int) * m.runs.size()))) return rc;
+    {
+        // lane slots of the sweep kernels (lane p of run t: slot 4t+p)
+        std::vector<int> sl((size_t)SL_N * ss, 0);
+        std::vector<double> sn((size_t)6 * ss, 0.0);
+        for (int t = 0; t < nruns; ++t)
+            for (int p = 0; p < 4; ++p) {
+                const size_t q = 4 * (size_t)t + p;
+                const int first = m.runs[2 * t], cnt = m.runs[2 * t + 1];
+                const int i = p < cnt ? first + p : 0;
+                sl[SL_CELL * (size_t)ss + q] = i;
+                sl[SL_FIRST * (size_t)ss + q] = first;
+                sl[SL_CNT * (size_t)ss + q] = cnt;
+                for (int k = 0; k < 3; ++k) {
+                    sl[(SL_CF + k) * (size_t)ss + q] = m.cface[3 * (size_t)i + k];
+                    sl[(SL_NB + k) * (size_t)ss + q] = m.lu_nb[3 * (size_t)i + k];
+                }
+                for (int k = 0; k < 6; ++k) sn[k * (size_t)ss + q] = m.lu_n[6 * (size_t)i + k];
+            }
+        if ((rc = up(M.slot, sl.data(), sizeof(int) * sl.size()))) return rc;
+        if ((rc = up(M.slot_n, sn.data(), sizeof(double) * sn.size()))) return rc;
+    }
     if ((rc = up(M.pi_cells, m.pi_cells.data(), sizeof(int) * m.pi_cells.size()))) return rc;
     auto vx = soa(m.vecx, m.nf, 2, sf);
     if ((rc = up(M.vecx, vx.data(), sizeof(double) * vx.size()))) return rc;

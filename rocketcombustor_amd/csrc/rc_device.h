@@ -16,6 +16,7 @@ namespace rc {
 struct DevMesh {
     int ne, nf, ng;
     int sc, sf, sg;
+    int ss;                              // slot stride (4 lane slots per LU-SGS run)
     const int *__restrict__ fnl;
     const int *__restrict__ fnr;
     const int *__restrict__ stencil;
@@ -38,7 +39,15 @@ struct DevMesh {
     // cell (Time_LUSGS_steady.f90:60-68): lu_nb[k*sc+i], lu_n[(k*2+d)*sc+i]
     const int *__restrict__ lu_nb;
     const double *__restrict__ lu_n;
+    // the same indices in lane-slot order of the LU-SGS sweep kernels: slot
+    // 4t+p serves lane p of run t, so a lane's indices depend on its thread
+    // index alone; slot[k*ss+s]: k = SL_CELL (the cell, or cell 0 for a lane
+    // past the run's end), SL_FIRST, SL_CNT (the run), SL_CF+kf (cface),
+    // SL_NB+kf (lu_nb); slot_n[(kf*2+d)*ss+s] (lu_n)
+    const int *__restrict__ slot;
+    const double *__restrict__ slot_n;
 };
+enum SlotField { SL_CELL = 0, SL_FIRST, SL_CNT, SL_CF, SL_NB = SL_CF + 3, SL_N = SL_NB + 3 };
 
 // per-cell quantities of the cell-average state that the LU-SGS neighbour
 // term (Time_LUSGS_steady.f90:72-87), the source term and the ghost update
@@ -138,9 +147,11 @@ struct LaunchSet {
 #define RC_G __attribute__((address_space(1)))
 struct GMesh {
     int ne, nf, ng;
-    int sc, sf, sg;
+    int sc, sf, sg, ss;
     const RC_G int *fnl, *fnr, *stencil, *cface, *ghost_host, *ghost_type, *runs, *lu_nb, *pi_cells;
     const RC_G double *vecx, *fvl, *fvr, *vol, *wdis, *amls, *lvv, *gnrm, *lu_n, *vgeo;
+    const RC_G int *slot;
+    const RC_G double *slot_n;
 };
 struct GState {
     RC_G double *pa, *wa, *hwa, *gradc, *gpa, *step, *dd, *fflux, *lucoef, *lu_af, *fixed, *fvis, *src;
@@ -156,7 +167,8 @@ __device__ __forceinline__ RC_G T *gp(T *p) {
 
 __device__ __forceinline__ GMesh gview(const RC_G DevMesh &m) {
     GMesh g;
-    g.ne = m.ne; g.nf = m.nf; g.ng = m.ng; g.sc = m.sc; g.sf = m.sf; g.sg = m.sg;
+    g.ne = m.ne; g.nf = m.nf; g.ng = m.ng; g.sc = m.sc; g.sf = m.sf; g.sg = m.sg; g.ss = m.ss;
+    g.slot = gp(m.slot); g.slot_n = gp(m.slot_n);
     g.fnl = gp(m.fnl); g.fnr = gp(m.fnr); g.stencil = gp(m.stencil); g.cface = gp(m.cface);
     g.ghost_host = gp(m.ghost_host); g.ghost_type = gp(m.ghost_type); g.runs = gp(m.runs); g.lu_nb = gp(m.lu_nb);
     g.pi_cells = gp(m.pi_cells);

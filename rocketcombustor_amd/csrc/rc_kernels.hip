@@ -50,14 +50,15 @@ __global__ void __launch_bounds__(256) k_timestep_face(ITEM_ARGS) {
 // TimeStep cell loop (TimeStep.f90:94-96; viscous: :77-81) and the LU-SGS diagonal
 // dd(i) = sum 0.5*alpaF + vol/step (Time_LUSGS_steady.f90:46-53)
 // (stored when `store`; the face alpaF values and dd are also returned)
-__device__ __forceinline__ double cell_step_dd(const GMesh &M, const GState &S, GParams &P, int i, double *al,
-                                               bool store = true) {
+// (cf: the cell's three face codes, cface)
+__device__ __forceinline__ double cell_step_dd(const GMesh &M, const GState &S, GParams &P, int i,
+                                               const int *cfs, double *al, bool store = true) {
     const double cfl = P.cfl;
     const double v = M.vol[i];
     double a = 0.0, d = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const int cf = M.cface[k * M.sc + i];
+        const int cf = cfs[k];
         const int f = cf >= 0 ? cf : ~cf;
         al[k] = ALPHA_(f);
         // viscous: alpaC(i) += alpa + alpaVis/vol(i) for either side of the
@@ -74,6 +75,13 @@ __device__ __forceinline__ double cell_step_dd(const GMesh &M, const GState &S, 
         S.dd[i] = d;
     }
     return d;
+}
+__device__ __forceinline__ double cell_step_dd(const GMesh &M, const GState &S, GParams &P, int i, double *al,
+                                               bool store = true) {
+    int cfs[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) cfs[k] = M.cface[k * M.sc + i];
+    return cell_step_dd(M, S, P, i, cfs, al, store);
 }
 
 template <int NSP, bool B>
@@ -346,12 +354,12 @@ __global__ void __launch_bounds__(256) k_flux(ITEM_ARGS) {
 // then the chemical source vol*omega of sourceterm (sourceterm.f90:2-70)
 template <int NSP, class TT>
 __device__ __forceinline__ void cell_residual(const GMesh &M, const GState &S, TT &th, GParams &P, int i,
-                                              double *h) {
+                                              const int *cfs, double *h) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) h[k] = 0.0;
 #pragma unroll
     for (int kf = 0; kf < 3; ++kf) {
-        const int cf = M.cface[kf * M.sc + i];
+        const int cf = cfs[kf];
         const int f = cf >= 0 ? cf : ~cf;
         if (cf >= 0) {
 #pragma unroll
@@ -366,7 +374,7 @@ __device__ __forceinline__ void cell_residual(const GMesh &M, const GState &S, T
         // its mass flux is zero
 #pragma unroll
         for (int kf = 0; kf < 3; ++kf) {
-            const int cf = M.cface[kf * M.sc + i];
+            const int cf = cfs[kf];
             const int f = cf >= 0 ? cf : ~cf;
             if (cf >= 0) {
                 h[0] = h[0] - 0.0;
@@ -403,6 +411,14 @@ __device__ __forceinline__ void cell_residual(const GMesh &M, const GState &S, T
 #pragma unroll
         for (int k = 0; k < NSP - 1; ++k) h[kNFlow + k] = h[kNFlow + k] + v * w[k];
     }
+}
+template <int NSP, class TT>
+__device__ __forceinline__ void cell_residual(const GMesh &M, const GState &S, TT &th, GParams &P, int i,
+                                              double *h) {
+    int cfs[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) cfs[k] = M.cface[k * M.sc + i];
+    cell_residual<NSP>(M, S, th, P, i, cfs, h);
 }
 
 // the residual of flux_sourceterm into hWA; TS: also the cell part of
@@ -522,25 +538,28 @@ __device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int ten
     GParams &PAR = *gp(C.par);
     const int lane = threadIdx.x % R;
     const int sc = M.sc;
-    // every lane stays to the end: the group's shuffles need all four
-    int first = 0, cnt = 0;
-    if (t < tend) { first = M.runs[2 * t]; cnt = M.runs[2 * t + 1]; }
+    // every lane stays to the end: the group's shuffles need all four; the
+    // lane's indices come from its slot (one load phase, no run table)
+    const int q = 4 * (t < tend ? t : 0) + lane, ss = M.ss;
+    const int first = M.slot[SL_FIRST * ss + q];
+    const int cnt = t < tend ? M.slot[SL_CNT * ss + q] : 0;
     const bool live = lane < cnt;
-    const int i = live ? first + lane : 0;
-    int nb[3];
+    const int i = M.slot[SL_CELL * ss + q];
+    int nb[3], cfs[3];
     double nx[3], ny[3], af[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        nb[k] = M.lu_nb[k * sc + i];
-        nx[k] = M.lu_n[(k * 2) * sc + i];
-        ny[k] = M.lu_n[(k * 2 + 1) * sc + i];
+        cfs[k] = M.slot[(SL_CF + k) * ss + q];
+        nb[k] = M.slot[(SL_NB + k) * ss + q];
+        nx[k] = M.slot_n[(k * 2) * ss + q];
+        ny[k] = M.slot_n[(k * 2 + 1) * ss + q];
     }
     double d, rhs[NV];
     if (RES) {
-        d = cell_step_dd(M, S, PAR, i, af, live);
-        cell_residual<NSP>(M, S, th, PAR, i, rhs);
+        d = cell_step_dd(M, S, PAR, i, cfs, af, live);
+        cell_residual<NSP>(M, S, th, PAR, i, cfs, rhs);
     } else if (FLOW) {
-        d = cell_step_dd(M, S, PAR, i, af, false);
+        d = cell_step_dd(M, S, PAR, i, cfs, af, false);
     } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k) af[k] = S.lu_af[k * sc + i];
@@ -668,14 +687,16 @@ __device__ __forceinline__ void run_residual(const RC_G CaseDev &C, int t, int t
     const GState S = gview(C.S);
     GThermo &th = *gp(C.th);
     GParams &PAR = *gp(C.par);
-    const int lane = threadIdx.x % kRunMax;
-    const int first = M.runs[2 * t], cnt = M.runs[2 * t + 1];
-    if (lane >= cnt) return;
-    const int i = first + lane;
+    const int q = 4 * t + (int)threadIdx.x % kRunMax, ss = M.ss;
+    if ((int)threadIdx.x % kRunMax >= M.slot[SL_CNT * ss + q]) return;
+    const int i = M.slot[SL_CELL * ss + q];
+    int cfs[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) cfs[k] = M.slot[(SL_CF + k) * ss + q];
     double al[3];
-    cell_step_dd(M, S, PAR, i, al);
+    cell_step_dd(M, S, PAR, i, cfs, al);
     double h[NV];
-    cell_residual<NSP>(M, S, th, PAR, i, h);
+    cell_residual<NSP>(M, S, th, PAR, i, cfs, h);
 #pragma unroll
     for (int k = 0; k < NV; ++k) HWA_(k, i) = h[k];
 #pragma unroll
