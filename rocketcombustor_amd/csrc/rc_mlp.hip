@@ -4,13 +4,17 @@
 // Adam) and evaluated per cell inside the steady step with a per-cell gate
 // and the exact FP64 kinetics as fallback.
 //
-// Training: the dense products go through one LDS-tiled FP32 GEMM (64x64
-// tiles, 256 threads, 4x4 outputs per thread) with the layer's epilogue
-// fused: bias + tanh (hidden), bias (output), or the tanh derivative
-// (backward).  The weight and bias gradients are split over the batch into
-// fixed slices whose partials are added in slice order; all other reductions
-// are block reductions in a fixed order, so training is deterministic run to
-// run.  Inference (rc_mlp_forward and the step) of a width-128 network: one
+// Training of a width-128 network (at most 8 inputs, 16 outputs, 4 hidden
+// layers): one fused kernel per 32 rows of the batch runs the forward pass,
+// the loss gradient and the backward pass with the activations in LDS and
+// every product an FP32 MFMA, and writes per-block partial gradients that a
+// second kernel adds in block order before the Adam step (k_mlp_train_rows,
+// k_mlp_reduce).  Other shapes: the dense products go through one LDS-tiled
+// FP32 GEMM (64x64 tiles, 256 threads, 4x4 outputs per thread) with the
+// layer's epilogue fused: bias + tanh (hidden), bias (output), or the tanh
+// derivative (backward); the weight and bias gradients are split over the
+// batch into fixed slices whose partials are added in slice order.  All
+// reductions run in a fixed order, so training is deterministic run to run.  Inference (rc_mlp_forward and the step) of a width-128 network: one
 // fused kernel per 64-row tile with the activations in LDS and FP32 MFMA
 // products (k_mlp_infer); other widths take the GEMM chain.
 #include <hip/hip_runtime.h>
@@ -405,6 +409,301 @@ __global__ void __launch_bounds__(256) k_gather_batch(const float *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Fused training pass of a width-128 network (inputs <= 8, outputs <= 16, at
+// most kTH hidden layers): one block of 256 threads takes kTR rows of the
+// batch (drawn from the resident data set as k_gather_batch draws them, or
+// rows 0.. of given arrays) through the forward pass, keeping every layer's
+// activations in LDS, the loss gradient and the backward pass, and writes
+// its partial parameter gradient and partial squared error to its own row
+// of `part`; k_mlp_reduce adds the rows in a fixed order and applies Adam.
+// Every product is an FP32 MFMA 32x32x2 (operand layout as in k_mlp_infer):
+// wave w owns columns 32w .. 32w+31 of a 32 x 128 product, and the m-tiles
+// of column tile w of a weight gradient (a sum over the block's 32 rows).
+// The activations and loss gradients (A operands, and B of the weight
+// gradients) come from LDS; the weights (B operands of the products) straight
+// from memory, where all blocks share them in L2: W (forward) and, for the
+// backward products of the hidden layers, the transposed copy k_mlp_prep
+// writes, so a lane's operand loads are consecutive in either direction.
+// Rows past the batch have zero inputs and a zero loss gradient, so they
+// add nothing.
+// ---------------------------------------------------------------------------
+constexpr int kTR = 32;          // rows per block
+constexpr int kTH = 4;           // hidden layers at most
+constexpr int kTW = kTR * kFS;   // floats of one 32-row activation tile in LDS
+constexpr int kTO = 33;          // LDS row stride of the output layer's partial tiles
+
+// the parameter offsets of layer l of a fused-shape network
+__device__ __forceinline__ void fused_offsets(int l, int nin, int nout, int hidden, int &woff, int &boff) {
+    const int first = nin * kFW + kFW, mid = kFW * kFW + kFW;
+    woff = l == 0 ? 0 : first + (l - 1) * mid;
+    boff = woff + (l == 0 ? nin : kFW) * (l == hidden ? nout : kFW);
+}
+
+// The step counter (when `bump`, as k_step_count) and the transposed copies
+// of the 128 x 128 weights of layers 1 .. hidden-1: PT[(l-1) 128^2 + k 128 + n]
+// = W_l[n][k].
+__global__ void __launch_bounds__(256) k_mlp_prep(const float *__restrict__ P, float *__restrict__ PT,
+                                                  long long *__restrict__ t, int bump, int nin, int nout,
+                                                  int hidden) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (bump && e == 0) *t = *t + 1;
+    const int l = 1 + e / (kFW * kFW), r = e % (kFW * kFW);
+    if (l >= hidden) return;
+    int woff, boff;
+    fused_offsets(l, nin, nout, hidden, woff, boff);
+    const int k = r / kFW, nn = r % kFW;
+    PT[(size_t)(l - 1) * kFW * kFW + r] = P[woff + (size_t)nn * kFW + k];
+}
+
+__global__ void __launch_bounds__(256) k_mlp_train_rows(const float *__restrict__ X, const float *__restrict__ T,
+                                                        long long ndata, int gather, uint64_t seed,
+                                                        const long long *__restrict__ d_t, int n,
+                                                        const float *__restrict__ P, const float *__restrict__ PT,
+                                                        float *__restrict__ part, int nin, int nout, int hidden,
+                                                        int nparam, float scale) {
+    __shared__ float Hs[(kTH + 1) * kTW];   // layer inputs: Hs[0] the rows' inputs, Hs[l] hidden layer l's outputs
+    __shared__ float Ds[2 * kTW];           // loss gradients of two layers (ping-pong)
+    __shared__ long long src[kTR];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int li = lane & 31, lk = lane >> 5;
+    const int row0 = blockIdx.x * kTR;
+    float *prt = part + (size_t)blockIdx.x * (nparam + 1);
+    if (t < kTR) {
+        const int r = row0 + t;
+        long long s = r < n ? r : -1;
+        if (gather && r < n) {
+            const uint64_t base = seed + (uint64_t)*d_t * 0x100000000ull;
+            s = (long long)(splitmix64(base + (uint64_t)r) % (uint64_t)ndata);
+        }
+        src[t] = s;
+    }
+    __syncthreads();
+    // inputs, zero past nin (to 32 columns: the input layer's gradient reads
+    // a 32-wide tile) and on rows past the batch
+    for (int e = t; e < kTR * 32; e += 256) {
+        const int r = e >> 5, k = e & 31;
+        Hs[r * kFS + k] = (k < nin && src[r] >= 0) ? X[(size_t)src[r] * nin + k] : 0.f;
+    }
+    // ---- forward: hidden layers (32 x 128 tiles, one 32-column tile per wave)
+    for (int l = 0; l < hidden; ++l) {
+        int woff, boff;
+        fused_offsets(l, nin, nout, hidden, woff, boff);
+        const int fi = l == 0 ? nin : kFW;
+        __syncthreads();     // the layer's input rows are complete
+        const float *ha = Hs + l * kTW + li * kFS + lk;
+        const float *wb = P + woff + (size_t)lk * kFW + 32 * w + li;
+        f32x16 acc = {};
+        if (l == 0) {
+#pragma unroll
+            for (int k = 0; k < 8; k += 2) {
+                const float b = k + lk < fi ? wb[(size_t)k * kFW] : 0.f;
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ha[k], b, acc, 0, 0, 0);
+            }
+        } else {
+            // the lane's whole column of B first: one memory latency per layer
+            float bv[kFW / 2];
+#pragma unroll
+            for (int q = 0; q < kFW / 2; ++q) bv[q] = wb[(size_t)(2 * q) * kFW];
+#pragma unroll
+            for (int q = 0; q < kFW / 2; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ha[2 * q], bv[q], acc, 0, 0, 0);
+        }
+
+        float *out = Hs + (l + 1) * kTW;
+        const int j = 32 * w + li;
+        const float bj = P[boff + j];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) out[(8 * (r >> 2) + 4 * lk + (r & 3)) * kFS + j] = fast_tanh(acc[r] + bj);
+    }
+    // ---- forward: output layer (32 x 32 tile, columns past nout zero); wave
+    // w takes K rows 32w .. 32w+31, the four partial tiles are added in order
+    int woff, boff;
+    fused_offsets(hidden, nin, nout, hidden, woff, boff);
+    __syncthreads();
+    {
+        const float *ha = Hs + hidden * kTW + li * kFS + 32 * w + lk;
+        // (lanes past nout read column 0 and use zero: every load stays in W)
+        const float *wb = P + woff + (size_t)(32 * w + lk) * nout + (li < nout ? li : 0);
+        f32x16 acc = {};
+        float bv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) bv[q] = li < nout ? wb[(size_t)(2 * q) * nout] : 0.f;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ha[2 * q], bv[q], acc, 0, 0, 0);
+        float *pp = Ds + (w - 1) * kTR * kTO;       // waves 1..3 leave their partial tiles in Ds
+        if (w > 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) pp[(8 * (r >> 2) + 4 * lk + (r & 3)) * kTO + li] = acc[r];
+        }
+        __syncthreads();
+        if (w == 0) {
+            // outputs, loss gradient 2 (y - t) / count into Ds[1] (columns past
+            // nout and rows past the batch zero), squared error
+            const float bj = li < nout ? P[boff + li] : 0.f;
+            float sq = 0.f;
+            float g[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = 8 * (r >> 2) + 4 * lk + (r & 3);
+                const float y = (((acc[r] + Ds[i * kTO + li]) + Ds[kTR * kTO + i * kTO + li]) +
+                                 Ds[2 * kTR * kTO + i * kTO + li]) + bj;
+                g[r] = 0.f;
+                if (li < nout && src[i] >= 0) {
+                    const float d = y - T[(size_t)src[i] * nout + li];
+                    g[r] = 2.f * d * scale;
+                    sq += d * d;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) Ds[kTW + (8 * (r >> 2) + 4 * lk + (r & 3)) * kFS + li] = g[r];
+            // the wave's squared error, in a fixed order
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 64);
+            if (lane == 0) prt[nparam] = sq;
+        }
+    }
+    // ---- backward, output layer: dW = H^T dY (128 x nout: m-tile w), db =
+    // column sums of dY, dH = dY W^T (32 x 128, K = 16) times tanh'
+    __syncthreads();
+    float *cur = Ds + kTW, *nxt = Ds;
+    {
+        const float *hl = Hs + hidden * kTW;
+        f32x16 acc = {};
+        const float *ha = hl + lk * kFS + 32 * w + li;   // A[m][k] = H[row k][m]
+        const float *gb = cur + lk * kFS + li;           // B[k][n] = dY[row k][n]
+#pragma unroll 4
+        for (int k = 0; k < kTR; k += 2) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ha[k * kFS], gb[k * kFS], acc, 0, 0, 0);
+        if (li < nout) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) prt[woff + (32 * w + 8 * (r >> 2) + 4 * lk + (r & 3)) * nout + li] = acc[r];
+        }
+        if (t < nout) {
+            float sb = 0.f;
+            for (int i = 0; i < kTR; ++i) sb += cur[i * kFS + t];
+            prt[boff + t] = sb;
+        }
+        // dH = dY W^T: B[k][n] = W[n][k] (k < nout)
+        const float *ga = cur + li * kFS + lk;
+        const float *wb = P + woff + (size_t)(32 * w + li) * nout;
+        acc = f32x16{};
+#pragma unroll
+        for (int k = 0; k < 16; k += 2) {
+            const float b = k + lk < nout ? wb[k + lk] : 0.f;
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ga[k], b, acc, 0, 0, 0);
+        }
+        const int j = 32 * w + li;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = 8 * (r >> 2) + 4 * lk + (r & 3);
+            const float h = hl[i * kFS + j];
+            nxt[i * kFS + j] = acc[r] * (1.f - h * h);
+        }
+        float *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+    // ---- backward, hidden layers l = hidden-1 .. 0 (W_l maps Hs[l] to Hs[l+1];
+    // cur holds the loss gradient of layer l's pre-activations)
+    for (int l = hidden - 1; l >= 0; --l) {
+        fused_offsets(l, nin, nout, hidden, woff, boff);
+        const int fi = l == 0 ? nin : kFW;
+        __syncthreads();     // cur is complete
+        const float *hl = Hs + l * kTW;
+        // dW = H_l^T cur: m-tiles 0..3 (0 only for the input layer) of column tile w
+        const int mt = l == 0 ? 1 : 4;
+        for (int q = 0; q < mt; ++q) {
+            f32x16 acc = {};
+            const float *ha = hl + lk * kFS + 32 * q + li;
+            const float *gb = cur + lk * kFS + 32 * w + li;
+#pragma unroll 4
+            for (int k = 0; k < kTR; k += 2)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ha[k * kFS], gb[k * kFS], acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = 32 * q + 8 * (r >> 2) + 4 * lk + (r & 3);
+                if (m < fi) prt[woff + (size_t)m * kFW + 32 * w + li] = acc[r];
+            }
+        }
+        if (t < kFW) {
+            float sb = 0.f;
+            for (int i = 0; i < kTR; ++i) sb += cur[i * kFS + t];
+            prt[boff + t] = sb;
+        }
+        if (l == 0) break;
+        // dH_l = cur W_l^T (B[k][n] = W_l[n][k], from the transposed copy),
+        // times tanh'(Hs[l])
+        const float *ga = cur + li * kFS + lk;
+        const float *wb = PT + (size_t)(l - 1) * kFW * kFW + (size_t)lk * kFW + 32 * w + li;
+        f32x16 acc = {};
+        float bv[kFW / 2];
+#pragma unroll
+        for (int q = 0; q < kFW / 2; ++q) bv[q] = wb[(size_t)(2 * q) * kFW];
+#pragma unroll
+        for (int q = 0; q < kFW / 2; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ga[2 * q], bv[q], acc, 0, 0, 0);
+        const int j = 32 * w + li;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = 8 * (r >> 2) + 4 * lk + (r & 3);
+            const float h = hl[i * kFS + j];
+            nxt[i * kFS + j] = acc[r] * (1.f - h * h);
+        }
+        float *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+}
+
+// grads[p] = sum over blocks b < nb of part[b][p]; ADAM: then the Adam step
+// of k_adam on parameter p.  A thread block takes 64 parameters; its four
+// waves sum the blocks' rows in four consecutive quarters (eight loads in
+// flight per lane), and the quarters are added in order, so the sum has one
+// fixed order.  Parameter index nparam is the loss (the blocks' squared
+// errors, times scale), recorded in `hist` as k_record_loss does.
+constexpr int kRedP = 64;
+template <bool ADAM>
+__global__ void __launch_bounds__(256) k_mlp_reduce(const float *__restrict__ part, int nb, int nparam,
+                                                    float *__restrict__ grads, float *__restrict__ loss, float scale,
+                                                    float *__restrict__ p, float *__restrict__ m1,
+                                                    float *__restrict__ m2, float lr, float b1, float b2, float eps,
+                                                    const long long *__restrict__ t, float *__restrict__ hist,
+                                                    long long t0) {
+    __shared__ float q[4][kRedP];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int e = blockIdx.x * kRedP + lane;
+    const int per = (nb + 3) / 4, b0 = w * per, b1e = b0 + per < nb ? b0 + per : nb;
+    float g = 0.f;
+    if (e <= nparam) {
+        const float *src = part + e;
+        const size_t stride = (size_t)nparam + 1;
+        for (int b = b0; b < b1e; b += 8) {
+            float v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = b + k < b1e ? src[(size_t)(b + k) * stride] : 0.f;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) g += v[k];
+        }
+    }
+    q[w][lane] = g;
+    __syncthreads();
+    if (w != 0 || e > nparam) return;
+    g = ((q[0][lane] + q[1][lane]) + q[2][lane]) + q[3][lane];
+    if (e == nparam) {
+        *loss = g * scale;
+        if (hist) hist[*t - t0 - 1] = g * scale;
+        return;
+    }
+    grads[e] = g;
+    if (ADAM) {
+        const float tf = (float)*t;
+        const float c1 = 1.f - powf(b1, tf), c2 = 1.f - powf(b2, tf);
+        const float a = b1 * m1[e] + (1.f - b1) * g;
+        const float bb = b2 * m2[e] + (1.f - b2) * g * g;
+        m1[e] = a;
+        m2[e] = bb;
+        p[e] = p[e] - lr * (a / c1) / (sqrtf(bb / c2) + eps);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // the learned source term of a case: inputs of cells [c0, c0+n) from PA,
 // the network, then per cell the gate and the exact FP64 fallback
 // ---------------------------------------------------------------------------
@@ -553,6 +852,9 @@ struct rc_mlp {
     float *fx = nullptr, *fy = nullptr;              // fused forward pass: inputs, outputs (fcap rows)
     int fcap = 0;
     float *splitk = nullptr;                         // slice partials of the layer gradients
+    float *tpart = nullptr;                          // fused training: per-block partial gradients
+    float *wt = nullptr;                             // fused training: transposed hidden weights
+    int tcap = 0;                                    // rows tpart holds blocks for
     std::string err;
 };
 
@@ -646,6 +948,52 @@ void backward(rc_mlp *m, int n) {
                                          fo, fi, nullptr, m->act[l]);
     }
 }
+// the fused training pass serves width-128 networks with 1 .. kTH hidden
+// layers, at most 8 inputs and 16 outputs; RC_MLP_GEMM=1 takes the GEMM
+// chain for every network (the same gradient to FP32 rounding)
+bool fused_train(const rc_mlp *m) {
+    const char *e = std::getenv("RC_MLP_GEMM");
+    if (e && e[0] == '1') return false;
+    const int hidden = m->nlayer - 1;
+    return m->width == kFW && m->nin <= 8 && m->nout <= 16 && hidden >= 1 && hidden <= kTH;
+}
+
+int ensure_tpart(rc_mlp *m, int rows) {
+    if (rows <= m->tcap) return RC_OK;
+    if (m->tpart) (void)hipFree(m->tpart);
+    m->tpart = nullptr;
+    m->tcap = 0;
+    const size_t nb = (size_t)(rows + kTR - 1) / kTR;
+    MCHK(m, hipMalloc(&m->tpart, sizeof(float) * nb * (m->nparam + 1)));
+    if (!m->wt) MCHK(m, hipMalloc(&m->wt, sizeof(float) * (size_t)std::max(m->nlayer - 2, 1) * kFW * kFW));
+    m->tcap = rows;
+    return RC_OK;
+}
+
+// loss and gradient of n rows by the fused pass (rows drawn from the data
+// set at the device's step counter when `gather`, else rows 0..n-1 of X, T),
+// then the block-order reduction, with the Adam step and the loss history
+// entry when `adam`
+// (`adam`: also the step counter, in k_mlp_prep, which k_step_count
+// bumps on the GEMM chain)
+void fused_pass(rc_mlp *m, const float *X, const float *T, long long ndata, bool gather, uint64_t seed, int n,
+                bool adam, float lr, float b1, float b2, float eps, float *hist, long long t0) {
+    const int nb = (n + kTR - 1) / kTR;
+    const float scale = 1.f / (float)(n * m->nout);
+    const int np = (int)m->nparam, hidden = m->nlayer - 1;
+    const int nt = std::max(hidden - 1, 1) * kFW * kFW;
+    k_mlp_prep<<<(nt + 255) / 256, 256, 0, m->stream>>>(m->params, m->wt, m->d_t, adam ? 1 : 0, m->nin, m->nout,
+                                                       hidden);
+    k_mlp_train_rows<<<nb, 256, 0, m->stream>>>(X, T, ndata, gather ? 1 : 0, seed, m->d_t, n, m->params, m->wt,
+                                                m->tpart, m->nin, m->nout, hidden, np, scale);
+    const int g = (np + 1 + kRedP - 1) / kRedP;
+    if (adam)
+        k_mlp_reduce<true><<<g, 256, 0, m->stream>>>(m->tpart, nb, np, m->grads, m->loss, scale, m->params, m->m1,
+                                                     m->m2, lr, b1, b2, eps, m->d_t, hist, t0);
+    else
+        k_mlp_reduce<false><<<g, 256, 0, m->stream>>>(m->tpart, nb, np, m->grads, m->loss, scale, nullptr, nullptr,
+                                                      nullptr, 0.f, 0.f, 0.f, 0.f, m->d_t, nullptr, 0);
+}
 }  // namespace
 
 extern "C" {
@@ -703,7 +1051,7 @@ void rc_mlp_destroy(rc_mlp *m) {
         (void)hipStreamSynchronize(m->stream);
     }
     for (float *p : {m->params, m->grads, m->m1, m->m2, m->data_x, m->data_y, m->ybuf, m->partial, m->loss, m->splitk,
-                     m->fx, m->fy})
+                     m->fx, m->fy, m->tpart, m->wt})
         if (p) (void)hipFree(p);
     if (m->d_t) (void)hipFree(m->d_t);
     for (float *p : m->act) if (p) (void)hipFree(p);
@@ -772,8 +1120,13 @@ int rc_mlp_gradient(rc_mlp *m, const float *x, const float *t, int n, float *gra
     if (rc) return rc;
     MCHK(m, hipMemcpyAsync(m->act[0], x, sizeof(float) * (size_t)n * m->nin, hipMemcpyHostToDevice, m->stream));
     MCHK(m, hipMemcpyAsync(m->ybuf, t, sizeof(float) * (size_t)n * m->nout, hipMemcpyHostToDevice, m->stream));
-    forward(m, n, m->params);
-    backward(m, n);
+    if (fused_train(m)) {
+        if ((rc = ensure_tpart(m, n))) return rc;
+        fused_pass(m, m->act[0], m->ybuf, n, false, 0, n, false, 0.f, 0.f, 0.f, 0.f, nullptr, 0);
+    } else {
+        forward(m, n, m->params);
+        backward(m, n);
+    }
     if (grad)
         MCHK(m, hipMemcpyAsync(grad, m->grads, sizeof(float) * m->nparam, hipMemcpyDeviceToHost, m->stream));
     if (loss) MCHK(m, hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
@@ -803,6 +1156,8 @@ int rc_mlp_train(rc_mlp *m, int nsteps, int batch, float lr, unsigned long long 
     MCHK(m, hipSetDevice(m->device));
     int rc = ensure_cap(m, batch);
     if (rc) return rc;
+    const bool fused = fused_train(m);
+    if (fused && (rc = ensure_tpart(m, batch))) return rc;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     float *hist = nullptr;
     MCHK(m, hipMalloc(&hist, sizeof(float) * (size_t)nsteps));
@@ -811,6 +1166,10 @@ int rc_mlp_train(rc_mlp *m, int nsteps, int batch, float lr, unsigned long long 
     // one Adam step; every argument is fixed for the call, the step counter
     // is read on the device
     auto step = [&]() {
+        if (fused) {
+            fused_pass(m, m->data_x, m->data_y, m->ndata, true, seed, batch, true, lr, b1, b2, eps, hist, t0);
+            return;
+        }
         k_step_count<<<1, 1, 0, st>>>(m->d_t);
         k_gather_batch<<<(batch + 255) / 256, 256, 0, st>>>(m->data_x, m->data_y, m->act[0], m->ybuf, batch,
                                                             m->ndata, m->nin, m->nout, seed, m->d_t);

@@ -18,11 +18,14 @@ def announce(msg: str) -> None:
     sys.__stderr__.flush()
 
 
-@pytest.mark.parametrize("n,width", [(1, 128), (37, 128), (300, 128), (4099, 128), (37, 40), (300, 40)])
-def test_forward_and_gradient_match_reference(n, width):
-    """Width 128 runs the fused MFMA forward kernel, width 40 the GEMM chain;
-    the gradient always goes through the training GEMMs."""
-    announce("mlp forward/gradient n=%d width=%d" % (n, width))
+@pytest.mark.parametrize("n,width,gemm", [(1, 128, "0"), (37, 128, "0"), (300, 128, "0"), (4099, 128, "0"),
+                                          (300, 128, "1"), (4099, 128, "1"), (37, 40, "0"), (300, 40, "0")])
+def test_forward_and_gradient_match_reference(n, width, gemm, monkeypatch):
+    """Width 128 runs the fused MFMA forward kernel and the fused MFMA
+    training pass for the gradient (RC_MLP_GEMM=1: the training GEMMs),
+    width 40 the GEMM chain for both."""
+    monkeypatch.setenv("RC_MLP_GEMM", gemm)
+    announce("mlp forward/gradient n=%d width=%d gemm=%s" % (n, width, gemm))
     m = MLP(seed=7, width=width)
     rng = np.random.default_rng(n)
     x = rng.normal(size=(n, 6)).astype(np.float32)
@@ -38,8 +41,10 @@ def test_forward_and_gradient_match_reference(n, width):
     m.close()
 
 
-def test_training_reduces_loss_and_is_deterministic():
-    announce("mlp training")
+@pytest.mark.parametrize("gemm", ["0", "1"], ids=["fused", "gemm-chain"])
+def test_training_reduces_loss_and_is_deterministic(gemm, monkeypatch):
+    monkeypatch.setenv("RC_MLP_GEMM", gemm)
+    announce("mlp training gemm=%s" % gemm)
     rng = np.random.default_rng(11)
     x = rng.uniform(-1, 1, size=(20000, 6)).astype(np.float32)
     t = np.column_stack([np.sin(2 * x[:, 0]) * x[:, 1], x[:, 2] * x[:, 3], np.tanh(x[:, 4] - x[:, 5])]).astype(np.float32)
@@ -54,12 +59,14 @@ def test_training_reduces_loss_and_is_deterministic():
     assert np.array_equal(runs[0][1], runs[1][1])
 
 
-def test_graph_training_equals_direct_launches(monkeypatch):
+@pytest.mark.parametrize("gemm", ["0", "1"], ids=["fused", "gemm-chain"])
+def test_graph_training_equals_direct_launches(gemm, monkeypatch):
     """Training replays captured graphs of 50 steps (the Adam step counter
     and the minibatch stream read on the device); with graphs off the same
     steps launched directly give bitwise the same parameters and losses,
     also across calls that continue the counter."""
-    announce("mlp training: graph replay vs direct launches")
+    monkeypatch.setenv("RC_MLP_GEMM", gemm)
+    announce("mlp training: graph replay vs direct launches gemm=%s" % gemm)
     rng = np.random.default_rng(4)
     x = rng.uniform(-1, 1, size=(5000, 6)).astype(np.float32)
     t = np.column_stack([x[:, 0] * x[:, 1], np.sin(x[:, 2]), x[:, 3] - x[:, 4]]).astype(np.float32)
