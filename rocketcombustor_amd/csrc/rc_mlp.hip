@@ -484,30 +484,47 @@ __global__ void __launch_bounds__(256) k_mlp_train_rows(const float *__restrict_
         const int r = e >> 5, k = e & 31;
         Hs[r * kFS + k] = (k < nin && src[r] >= 0) ? X[(size_t)src[r] * nin + k] : 0.f;
     }
-    // ---- forward: hidden layers (32 x 128 tiles, one 32-column tile per wave)
+    // wave 0's targets (it forms the loss gradient), loaded now
+    float tv[16];
+    if (w == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = 8 * (r >> 2) + 4 * lk + (r & 3);
+            tv[r] = (li < nout && src[i] >= 0) ? T[(size_t)src[i] * nout + li] : 0.f;
+        }
+    }
+    // ---- forward: hidden layers (32 x 128 tiles, one 32-column tile per wave).
+    // A lane's column of the next layer's B is loaded while the current layer
+    // finishes (its MFMAs, epilogue and barrier), so each layer waits for at
+    // most what is left of that latency.
+    float bv[kFW / 2];
+    auto load_col = [&](int l) {
+        int wo, bo;
+        fused_offsets(l, nin, nout, hidden, wo, bo);
+        const float *wb = P + wo + (size_t)lk * kFW + 32 * w + li;
+#pragma unroll
+        for (int q = 0; q < kFW / 2; ++q) bv[q] = wb[(size_t)(2 * q) * kFW];
+    };
+    if (hidden > 1) load_col(1);
     for (int l = 0; l < hidden; ++l) {
         int woff, boff;
         fused_offsets(l, nin, nout, hidden, woff, boff);
         const int fi = l == 0 ? nin : kFW;
         __syncthreads();     // the layer's input rows are complete
         const float *ha = Hs + l * kTW + li * kFS + lk;
-        const float *wb = P + woff + (size_t)lk * kFW + 32 * w + li;
         f32x16 acc = {};
         if (l == 0) {
+            const float *wb = P + woff + (size_t)lk * kFW + 32 * w + li;
 #pragma unroll
             for (int k = 0; k < 8; k += 2) {
                 const float b = k + lk < fi ? wb[(size_t)k * kFW] : 0.f;
                 acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ha[k], b, acc, 0, 0, 0);
             }
         } else {
-            // the lane's whole column of B first: one memory latency per layer
-            float bv[kFW / 2];
-#pragma unroll
-            for (int q = 0; q < kFW / 2; ++q) bv[q] = wb[(size_t)(2 * q) * kFW];
 #pragma unroll
             for (int q = 0; q < kFW / 2; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ha[2 * q], bv[q], acc, 0, 0, 0);
+            if (l + 1 < hidden) load_col(l + 1);
         }
-
         float *out = Hs + (l + 1) * kTW;
         const int j = 32 * w + li;
         const float bj = P[boff + j];
@@ -548,7 +565,7 @@ __global__ void __launch_bounds__(256) k_mlp_train_rows(const float *__restrict_
                                  Ds[2 * kTR * kTO + i * kTO + li]) + bj;
                 g[r] = 0.f;
                 if (li < nout && src[i] >= 0) {
-                    const float d = y - T[(size_t)src[i] * nout + li];
+                    const float d = y - tv[r];
                     g[r] = 2.f * d * scale;
                     sq += d * d;
                 }
@@ -608,6 +625,12 @@ __global__ void __launch_bounds__(256) k_mlp_train_rows(const float *__restrict_
         const int fi = l == 0 ? nin : kFW;
         __syncthreads();     // cur is complete
         const float *hl = Hs + l * kTW;
+        // the lane's column of W_l^T for dH_l below, loaded during dW
+        if (l > 0) {
+            const float *wb = PT + (size_t)(l - 1) * kFW * kFW + (size_t)lk * kFW + 32 * w + li;
+#pragma unroll
+            for (int q = 0; q < kFW / 2; ++q) bv[q] = wb[(size_t)(2 * q) * kFW];
+        }
         // dW = H_l^T cur: m-tiles 0..3 (0 only for the input layer) of column tile w
         const int mt = l == 0 ? 1 : 4;
         for (int q = 0; q < mt; ++q) {
@@ -624,19 +647,19 @@ __global__ void __launch_bounds__(256) k_mlp_train_rows(const float *__restrict_
             }
         }
         if (t < kFW) {
+            float v[kTR];
+#pragma unroll
+            for (int i = 0; i < kTR; ++i) v[i] = cur[i * kFS + t];
             float sb = 0.f;
-            for (int i = 0; i < kTR; ++i) sb += cur[i * kFS + t];
+#pragma unroll
+            for (int i = 0; i < kTR; ++i) sb += v[i];
             prt[boff + t] = sb;
         }
         if (l == 0) break;
         // dH_l = cur W_l^T (B[k][n] = W_l[n][k], from the transposed copy),
         // times tanh'(Hs[l])
         const float *ga = cur + li * kFS + lk;
-        const float *wb = PT + (size_t)(l - 1) * kFW * kFW + (size_t)lk * kFW + 32 * w + li;
         f32x16 acc = {};
-        float bv[kFW / 2];
-#pragma unroll
-        for (int q = 0; q < kFW / 2; ++q) bv[q] = wb[(size_t)(2 * q) * kFW];
 #pragma unroll
         for (int q = 0; q < kFW / 2; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ga[2 * q], bv[q], acc, 0, 0, 0);
         const int j = 32 * w + li;
