@@ -201,15 +201,15 @@ def small_run(nx: int = 150, ny: int = 40, nsteps: int = 2000, cpu_steps: int = 
 
 
 def converge_run(case, max_steps: int, every: int = 100, ref_ms_per_step=None) -> dict:
-    """The steady run of one case to its plateau: steps until RoRes (the
-    species part of step_residual, what the reference monitors,
-    src/residual.f90:79) and the density norm stop falling.  The plateau
-    step is the first report within 10% of the lowest value the whole run
-    reaches (a drop to 1e-8 is not reached: the Barth limiter and the
-    threshold kinetics settle into a limit cycle, the reference does the
-    same); ``still_falling`` says whether the last fifth of the run still went
-    10% below the rest.  Reports the drops reached, the device time to the
-    plateau and the reference's time at its measured rate."""
+    """The steady run of one case from its initial field, up to max_steps:
+    the steps, device seconds and the reference's seconds (at its measured
+    rate) to each order of drop of RoRes (the species part of
+    step_residual, what the reference monitors, src/residual.f90:79).  A
+    drop to 1e-8 is not reached (the Barth limiter and the threshold
+    kinetics slow the iteration to a crawl, on the reference too).
+    ``still_falling`` says whether the last fifth of the run still went 5%
+    below the rest; only when it did not is a plateau reported: the first
+    report within 10% of the lowest value of the run."""
     from rocketcombustor_amd.solver import SteadyCase
     from rocketcombustor_amd import native
     out = {"max_steps": max_steps, "report_every": every}
@@ -239,20 +239,23 @@ def converge_run(case, max_steps: int, every: int = 100, ref_ms_per_step=None) -
     cut = max(1, (4 * len(ro)) // 5)
 
     ms = 1e3 * wall / max(done, 1)
-    p_ro, p_rho = plateau(ro), plateau(rho)
+    falling = bool(ro[cut:].min() < 0.95 * ro[:cut].min() or rho[cut:].min() < 0.95 * rho[:cut].min())
+    to_drop = {"%g" % d: (int(steps[np.argmax(ro <= d * ro[0])]) if np.any(ro <= d * ro[0]) else None)
+               for d in (1e-1, 1e-2, 1e-3)}
     out.update({"steps_run": int(done), "error": err, "ms_per_step": ms,
                 "roress_drop": float(ro.min() / ro[0]), "rho_drop": float(rho.min() / rho[0]),
-                "plateau_step_roress": p_ro, "plateau_step_rho": p_rho,
-                "still_falling": bool(ro[cut:].min() < 0.9 * ro[:cut].min() or rho[cut:].min() < 0.9 * rho[:cut].min()),
-                "steps_to_drop": {"%g" % d: (int(steps[np.argmax(ro <= d * ro[0])]) if np.any(ro <= d * ro[0])
-                                             else None) for d in (1e-1, 1e-2, 1e-3)}})
-    ps = max(p_ro, p_rho)
-    if ps:
+                "still_falling": falling, "steps_to_drop": to_drop,
+                "seconds_to_drop": {k: (v * ms * 1e-3 if v else None) for k, v in to_drop.items()}})
+    if ref_ms_per_step:
+        out["reference_seconds_to_drop_at_its_rate"] = {k: (v * ref_ms_per_step * 1e-3 if v else None)
+                                                       for k, v in to_drop.items()}
+        out["speedup_time_to_drop"] = ref_ms_per_step / ms
+    if not falling:
+        ps = max(plateau(ro), plateau(rho))
         out["plateau_step"] = ps
         out["seconds_to_plateau"] = ps * ms * 1e-3
         if ref_ms_per_step:
             out["reference_seconds_to_plateau_at_its_rate"] = ps * ref_ms_per_step * 1e-3
-            out["speedup_time_to_plateau"] = ref_ms_per_step / ms
     return out
 
 
@@ -586,8 +589,8 @@ def main() -> int:
                          "the gated surrogate and with exact kinetics)")
     ap.add_argument("--learned-steps", type=int, default=100000, help="training steps of --learned")
     ap.add_argument("--converge-steps", type=int, default=30000, metavar="N",
-                    help="also run the same case from its initial field until its residual plateau (at most N "
-                         "steps) and report steps and time to it; 0 skips it")
+                    help="also run the same case from its initial field for N steps and report the steps and "
+                         "time to each order of residual drop (and the plateau, if it flattens); 0 skips it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device three times (a plain copy kernel of "
                          "known size, to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
