@@ -164,7 +164,11 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
     btag.reserve((size_t)nb * 2);
     for (int b = 0; b < nb; ++b) {
         const int t = btype[b];
-        if (t < 1 || t > 6) return fail(msg, "unsupported boundary type %d", t);
+        // flux_lax knows types 1..6 and stops on any other face
+        // (flux_lax.f90:232-235, "no face of this type"), every step before
+        // flux_viscous, so flux_viscous's type 7 branch (:170-175) never runs
+        if (t < 1 || t > 6)
+            return fail(msg, "unsupported boundary type %d (flux_lax handles types 1-6 and stops on others)", t);
         btag.emplace(edge_key(bedges0[2 * b], bedges0[2 * b + 1]), t);
     }
     for (int f = 0; f < m.nf; ++f) {

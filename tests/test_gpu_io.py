@@ -67,3 +67,15 @@ def test_backup_restart_vs_oracle(tmp_path, xcd_layout):
     for _ in range(20):
         o2.stage()
     assert relerr(pa_gpu, o2.get("PA"), -1) < 1e-10
+
+
+def test_boundary_type_outside_flux_lax_is_rejected():
+    """Boundary types other than 1..6 are refused at set-up: the reference's
+    flux_lax stops on any such face every step (flux_lax.f90:232-235), before
+    flux_viscous could reach its type 7 ("induced") branch."""
+    case = nozzle.nozzle_case(6, 2)
+    case.mesh.btype = np.array(case.mesh.btype, copy=True)
+    case.mesh.btype[0] = 7
+    with pytest.raises(native.RcError, match="flux_lax handles types 1-6") as ei:
+        SteadyCase(case)
+    assert ei.value.code == native.RC_ERR_MESH
