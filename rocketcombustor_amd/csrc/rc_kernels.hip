@@ -7,6 +7,7 @@
 // reference's face loops accumulate into the same cell; the LU-SGS sweeps
 // run level by level over wavefronts of the given cell order.
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 
@@ -347,6 +348,203 @@ __global__ void __launch_bounds__(256) k_flux(ITEM_ARGS) {
 #pragma unroll
     for (int k = 0; k < NSP - 1; ++k)
         F_(kNFlow + k, f) = 0.5 * (ruln * yl[k] + rurn * yr[k] - coex * (rr * yr[k] - rl * yl[k]));
+}
+
+// flux_lax with two lanes per face (lanes 2k, 2k+1 of a wave take face k of
+// the block): lane 0 forms the left state, lane 1 the right one (on a
+// boundary face both lanes form the left state and the boundary branch, and
+// each keeps its side), so a lane holds one cell's loads and one thermo
+// evaluation at a time; the two sides are exchanged by shuffles and each
+// lane writes half of the face record.  TS: lane 1 also forms the face
+// spectral radius from the cell averages.  Every value is formed by the
+// expressions of k_flux; a face's failed checks are reported once, by lane
+// 0, in k_flux's order (so the first error recorded is the same).
+template <int NSP, bool TS, bool B>
+__global__ void __launch_bounds__(512) k_flux2(ITEM_ARGS) {
+    __shared__ DevThermo th_lds_;
+    const RC_G CaseDev *Cp_;
+    int f, end_;
+    const int side = (int)threadIdx.x & 1;
+    if constexpr (B) {
+        const int4 w_ = work[blockIdx.x];
+        Cp_ = gp(cases) + w_.x;
+        f = w_.y + (int)(threadIdx.x >> 1);
+        end_ = w_.z;
+    } else {
+        Cp_ = gp(cases);
+        f = lo + (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 1);
+        end_ = hi;
+    }
+    const RC_G CaseDev &C = *Cp_;
+    {
+        const RC_G double *src_ = (const RC_G double *)gp(C.th);
+        double *dst_ = reinterpret_cast<double *>(&th_lds_);
+        for (int k_ = threadIdx.x; k_ < (int)(sizeof(DevThermo) / 8); k_ += blockDim.x) dst_[k_] = src_[k_];
+    }
+    __syncthreads();
+    if (f >= end_) return;      // both lanes of a face leave together
+    const GMesh M = gview(C.M);
+    const GState S = gview(C.S);
+    const DevThermo &th = th_lds_;
+    GParams &P = *gp(C.par);
+    const int nl = M.fnl[f], nr = M.fnr[f];
+    const bool inner = nr >= 0;
+    const double dx = M.vecx[f], dy = M.vecx[M.sf + f];
+    const double sav1 = dx, sav2 = dy;
+    const double sav = fmax(sqrt(dx * dx + dy * dy), kSmall);
+    const double s1n = sav1 / sav, s2n = sav2 / sav;
+    // failed checks of this lane: 1 TS temperature, 2 TS negative, 4 own
+    // thermo, 8 inlet thermo, 16 negative state, 32 own enthalpy
+    int bad = 0;
+    // the lane's cell: the right one for lane 1 of an inner face, else the left
+    const int c = (side && inner) ? nr : nl;
+    const RC_G double *fv = (side && inner) ? M.fvr : M.fvl;
+    const double w0 = fv[f], w1 = fv[M.sf + f];
+    double pa[NV], q[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        pa[k] = PA_(k, c);
+        q[k] = pa[k] + (GR_(k, 0, c) * w0 + GR_(k, 1, c) * w1);
+    }
+    double pl = q[0], ul = q[1], vl = q[2], tl = q[3];
+    double yl[NSP], s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) { yl[k] = q[kNFlow + k]; s = s + yl[k]; }
+    yl[NSP - 1] = 1.0 - s;
+    double cpl, gaml, rcl;
+    if (!gas_param<NSP>(th, yl, tl, cpl, gaml, rcl)) bad |= 4;
+    double rl = pl / (rcl * tl);
+    // (pl .. rcl: the lane's own state so far; on a boundary face the left one)
+    if (!inner) {
+        double pr, ur, vr, tr, rr, cpr, gamr, rcr, yr[NSP];
+        const int type = -nr;
+        cpr = cpl; gamr = gaml; rcr = rcl;
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) yr[k] = yl[k];
+        if (type == 1) {
+            rr = rl; pr = pl; tr = tl; ur = -ul; vr = -vl;
+        } else if (type == 2 || type == 6) {
+            rr = rl; pr = pl; tr = tl;
+            const double vn = ul * s1n + vl * s2n;
+            ur = ul - 2 * vn * s1n;
+            vr = vl - 2 * vn * s2n;
+        } else if (type == 3 || type == 4) {
+            const RC_G double *yin = type == 3 ? P.ox_y : P.fu_y;
+#pragma unroll
+            for (int k = 0; k < NSP; ++k) yr[k] = yin[k];
+            tr = type == 3 ? P.ox_t : P.fu_t;
+            if (!gas_param<NSP>(th, yr, tr, cpr, gamr, rcr)) bad |= 8;
+            pr = pl;
+            rr = pr / (rcr * tr);
+            ur = (type == 3 ? P.ox_g : P.fu_g) / rr;
+            vr = 0.0;
+            pl = pr; ul = ur; vl = vr; tl = tr; rl = rr;
+#pragma unroll
+            for (int k = 0; k < NSP; ++k) yl[k] = yr[k];
+            cpl = cpr; gaml = gamr; rcl = rcr;
+        } else {  // 5: nozzle outlet
+            const double ac = sqrt(gaml * pl / rl);
+            const double vn = ul * s1n + vl * s2n;
+            if (fabs(vn) > ac) {
+                rr = rl; ur = ul; vr = vl; pr = pl; tr = tl;
+            } else {
+                pr = P.p_out;
+                ur = ul + s1n * (pl - pr) / (rl * ac);
+                vr = vl + s2n * (pl - pr) / (rl * ac);
+                rr = rl + (pr - pl) / (ac * ac);
+                tr = pr / (rcl * rr);
+                pl = pr; ul = ur; vl = vr; tl = tr; rl = rr;
+            }
+        }
+        if (rl < 0. || pl < 0.) bad |= 16;
+        if (side) {
+            // lane 1 keeps the right side
+            pl = pr; ul = ur; vl = vr; tl = tr; rl = rr;
+#pragma unroll
+            for (int k = 0; k < NSP; ++k) yl[k] = yr[k];
+            cpl = cpr; gaml = gamr; rcl = rcr;
+        }
+    } else {
+        if (rl < 0. || pl < 0.) bad |= 16;
+    }
+    // the lane's side: total enthalpy, energy, sound speed, normal velocity
+    double h[NSP], hs = 0.0;
+    if (!enthalpy<NSP>(th, tl, h)) bad |= 32;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) hs = hs + yl[k] * h[k];
+    const double vam = ul * ul + vl * vl;
+    hs = hs + 0.5 * vam;
+    const double es = hs - pl / rl;
+    const double acs = sqrt(gaml * pl / rl);
+    const double uns = sav1 * ul + sav2 * vl;
+    // the other side
+    const double po = __shfl_xor(pl, 1), uo = __shfl_xor(ul, 1), vo = __shfl_xor(vl, 1), ro = __shfl_xor(rl, 1);
+    const double ho = __shfl_xor(hs, 1), eo = __shfl_xor(es, 1), aco = __shfl_xor(acs, 1);
+    const double uno = __shfl_xor(uns, 1);
+    double yo[NSP - 1];
+#pragma unroll
+    for (int k = 0; k < NSP - 1; ++k) yo[k] = __shfl_xor(yl[k], 1);
+    // left (L) and right (R) by side
+    const double rL = side ? ro : rl, rR = side ? rl : ro;
+    const double uL = side ? uo : ul, uR = side ? ul : uo, vL = side ? vo : vl, vR = side ? vl : vo;
+    const double pL = side ? po : pl, pR = side ? pl : po;
+    const double hL = side ? ho : hs, hR = side ? hs : ho, eL = side ? eo : es, eR = side ? es : eo;
+    const double uln = side ? uno : uns, urn = side ? uns : uno;
+    const double acl = side ? aco : acs, acr = side ? acs : aco;
+    const double coex = fmax(fabs(uln) + acl * sav, fabs(urn) + acr * sav);
+    const double ruln = rL * uln, rurn = rR * urn;
+    if (side == 0) {
+        F_(0, f) = 0.5 * (ruln + rurn - coex * (rR - rL));
+        F_(1, f) = 0.5 * (ruln * uL + rurn * uR + sav1 * (pL + pR) - coex * (rR * uR - rL * uL));
+        F_(2, f) = 0.5 * (ruln * vL + rurn * vR + sav2 * (pL + pR) - coex * (rR * vR - rL * vL));
+        F_(3, f) = 0.5 * (ruln * hL + rurn * hR - coex * (rR * eR - rL * eL));
+    } else {
+#pragma unroll
+        for (int k = 0; k < NSP - 1; ++k)
+            F_(kNFlow + k, f) = 0.5 * (ruln * yo[k] + rurn * yl[k] - coex * (rR * yl[k] - rL * yo[k]));
+    }
+    // TimeStep's face part, by lane 1, from the cell averages
+    if constexpr (TS) {
+        double pa_o[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) pa_o[k] = __shfl_xor(pa[k], 1);
+        if (side) {
+        // own pa is the right cell's on an inner face, the left cell's on a
+        // boundary face; pa_o is the left cell's
+            double pp, uu, vv, tt, y[NSP], sy = 0.0;
+            if (inner) {
+                pp = 0.5 * (pa_o[0] + pa[0]);
+                uu = 0.5 * (pa_o[1] + pa[1]);
+                vv = 0.5 * (pa_o[2] + pa[2]);
+                tt = 0.5 * (pa_o[3] + pa[3]);
+#pragma unroll
+                for (int k = 0; k < NSP - 1; ++k) { y[k] = 0.5 * (pa_o[kNFlow + k] + pa[kNFlow + k]); sy = sy + y[k]; }
+            } else {
+                pp = pa[0]; uu = pa[1]; vv = pa[2]; tt = pa[3];
+#pragma unroll
+                for (int k = 0; k < NSP - 1; ++k) { y[k] = pa[kNFlow + k]; sy = sy + y[k]; }
+            }
+            y[NSP - 1] = 1.0 - sy;
+            double cp, gam, r;
+            if (!gas_param<NSP>(th, y, tt, cp, gam, r)) bad |= 1;
+            const double rr = pp / (r * tt);
+            if (rr < 0. || pp < 0. || tt < 0.) bad |= 2;
+            const double un = uu * dx + vv * dy;
+            ALPHA_(f) = fabs(un) + sqrt(gam * pp / rr) * sqrt(dx * dx + dy * dy);
+        }
+    }
+    // report the face's first failed check in k_flux's order
+    const int bad_o = __shfl_xor(bad, 1);
+    if (side == 0 && (bad | bad_o)) {
+        const int b0 = bad, b1 = bad_o;
+        if (b1 & 1) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_TS_FACE);
+        else if (b1 & 2) flag_error(S.err, RC_ERR_NEGATIVE, f, K_TS_FACE);
+        else if (b0 & 4) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX);
+        else if (inner && (b1 & 4)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX);
+        else if (!inner && (b0 & 8)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX);
+        else if ((b0 & 16) || (inner && (b1 & 16))) flag_error(S.err, RC_ERR_NEGATIVE, f, K_FLUX);
+        else if ((b0 & 32) || (b1 & 32)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX);
+    }
 }
 
 // residual assembly of flux_sourceterm (fvm2d.f90:201-219) of cell i: h =
@@ -971,7 +1169,29 @@ int flow_max_blocks(int device) {
     return cache[device];
 }
 
+// two lanes per face (k_flux2; RC_FLUX1=1: k_flux, one thread per face)
+template <bool TS>
+void launch_flux2(const LaunchSet &L, hipStream_t st) {
+    const WorkList &wl = L.face;
+    if (L.single) {
+        const int n = wl.hi - wl.lo;
+        if (n > 0) k_flux2<kNsp, TS, false><<<(n + kChunk - 1) / kChunk, 2 * kChunk, 0, st>>>(L.cases, nullptr, wl.lo, wl.hi);
+    } else if (wl.n) {
+        k_flux2<kNsp, TS, true><<<wl.n, 2 * wl.bs, 0, st>>>(L.cases, wl.w, 0, 0);
+    }
+}
+
+bool flux_one_lane() {
+    const char *e = std::getenv("RC_FLUX1");
+    return e && e[0] == '1';
+}
+
 void launch_flux(const LaunchSet &L, bool ts, hipStream_t st) {
+    if (!flux_one_lane()) {
+        if (ts) launch_flux2<true>(L, st);
+        else launch_flux2<false>(L, st);
+        return;
+    }
     if (ts) LAUNCH(L.face, kChunk, st, k_flux, kNsp, true);
     else LAUNCH(L.face, kChunk, st, k_flux, kNsp, false);
 }

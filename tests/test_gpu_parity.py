@@ -259,6 +259,40 @@ def test_next_level_residual_equals_own_residual(nx, ny, order, xcd_layout, monk
         dflt.close()
 
 
+@pytest.mark.parametrize("nx,ny,order", [(1, 1, "natural"), (24, 6, "natural"), (150, 40, "multicolor"),
+                                         (1000, 200, "multicolor")])
+def test_two_lane_flux_equals_one_lane(nx, ny, order, xcd_layout, monkeypatch):
+    """flux_lax with two lanes per face (the default) gives bitwise the face
+    records, residuals and states of one thread per face (RC_FLUX1=1), with
+    and without the TimeStep face part, eager and graph-replayed."""
+    case = nozzle.nozzle_case(nx, ny)
+    if order == "multicolor":
+        perm, _ = native.order_multicolor(case.mesh.cells)
+        case = case.permuted(perm)
+    announce("two-lane flux vs one lane %dx%d %s RC_XCD=%s NE=%d" % (nx, ny, order, xcd_layout, case.mesh.ne))
+
+    def run(one_lane):
+        if one_lane:
+            monkeypatch.setenv("RC_FLUX1", "1")
+        else:
+            monkeypatch.delenv("RC_FLUX1", raising=False)
+        out = []
+        with SteadyCase(case) as g:
+            g.timestep()
+            g.reconstruction()
+            g.flux_sourceterm()
+            out += [*g.timestep_arrays(), g.increment()]
+            g.lu_sgs_stage()
+            g.run_steady(4, history=False)
+            out += [*g.timestep_arrays(), g.primitive(), g.conserved(), g.residual()]
+        monkeypatch.delenv("RC_FLUX1", raising=False)
+        return out
+
+    a, b = run(True), run(False)
+    for k, (x, y) in enumerate(zip(a, b)):
+        assert np.array_equal(x, y), k
+
+
 def test_operator_calls_after_fused_steps():
     """The fused step leaves the cell-average cache of the pre-update state;
     operator-level calls made right after it refresh the cache first, so
