@@ -263,8 +263,9 @@ def test_next_level_residual_equals_own_residual(nx, ny, order, xcd_layout, monk
                                          (1000, 200, "multicolor")])
 def test_two_lane_flux_equals_one_lane(nx, ny, order, xcd_layout, monkeypatch):
     """flux_lax with two lanes per face (the default) gives bitwise the face
-    records, residuals and states of one thread per face (RC_FLUX1=1), with
-    and without the TimeStep face part, eager and graph-replayed."""
+    records, residuals and states of one thread per face (RC_FLUX1=1),
+    through the operator entry points (flux without the TimeStep face part)
+    and the fused step, eager and graph-replayed."""
     case = nozzle.nozzle_case(nx, ny)
     if order == "multicolor":
         perm, _ = native.order_multicolor(case.mesh.cells)
@@ -272,20 +273,26 @@ def test_two_lane_flux_equals_one_lane(nx, ny, order, xcd_layout, monkeypatch):
     announce("two-lane flux vs one lane %dx%d %s RC_XCD=%s NE=%d" % (nx, ny, order, xcd_layout, case.mesh.ne))
 
     def run(one_lane):
-        if one_lane:
-            monkeypatch.setenv("RC_FLUX1", "1")
-        else:
-            monkeypatch.delenv("RC_FLUX1", raising=False)
+        for name in ("RC_FLUX1",):
+            if one_lane:
+                monkeypatch.setenv(name, "1")
+            else:
+                monkeypatch.delenv(name, raising=False)
         out = []
         with SteadyCase(case) as g:
             g.timestep()
+            g.gradient()
+            out += [g.gradient_array(), g.ghost()]
+            g.limiter_barth()
+            out.append(g.gradient_array())
             g.reconstruction()
             g.flux_sourceterm()
-            out += [*g.timestep_arrays(), g.increment()]
+            out += [*g.timestep_arrays(), g.increment(), g.gradient_array()]
             g.lu_sgs_stage()
             g.run_steady(4, history=False)
-            out += [*g.timestep_arrays(), g.primitive(), g.conserved(), g.residual()]
-        monkeypatch.delenv("RC_FLUX1", raising=False)
+            out += [*g.timestep_arrays(), g.primitive(), g.conserved(), g.residual(), g.gradient_array()]
+        for name in ("RC_FLUX1",):
+            monkeypatch.delenv(name, raising=False)
         return out
 
     a, b = run(True), run(False)
