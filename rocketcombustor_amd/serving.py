@@ -150,6 +150,18 @@ class Server:
                         reserved -= r.device_bytes
                         r.status, r.finished, r.error = "failed", self.clock() - t0, str(e)
                         continue
+                    try:
+                        # the first step alone: the member's baseline is its own
+                        # step-1 RoRes, whatever its first batched round does
+                        h = case.run_steady(1, every=1)
+                    except RcError as e:
+                        reserved -= r.device_bytes
+                        r.steps = 1
+                        r.status, r.finished, r.error = "failed", self.clock() - t0, str(e)
+                        case.close()
+                        continue
+                    r.steps, r.res0 = 1, roress(h[0])
+                    r.res = r.res0
                     reserved += case.device_bytes - r.device_bytes
                     r.device_bytes, r.cells = int(case.device_bytes), int(case.ne)
                     self.batch.add(case)
@@ -182,8 +194,6 @@ class Server:
                         continue
                     if hist is None:
                         continue
-                    if math.isnan(r.res0):
-                        r.res0 = roress(hist[0, k])
                     r.res = roress(hist[-1, k])
                     if r.res <= self.drop * r.res0:
                         r.status = "converged"

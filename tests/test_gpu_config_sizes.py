@@ -144,7 +144,7 @@ def test_config5_poisson_stream_vs_alone_and_oracle():
     lo, hi = min(cells, key=cells.get), max(cells, key=cells.get)
     check = {lo, hi, reqs[1].rid}
     for r in reqs:
-        assert r.steps == 10
+        assert r.steps == 11             # one step alone, then two rounds of five
         case = ordered(r.spec)
         with SteadyCase(case) as g:
             g.run_steady(r.steps, history=False)

@@ -29,6 +29,13 @@ class FakeCase:
     def close(self):
         self.closed = True
 
+    def run_steady(self, n, every=0, history=True):
+        """The member's first step alone (taken before it joins the batch)."""
+        h = np.zeros((1, NVAR + 1))
+        h[0, 4:NVAR] = math.exp(-self.steps / self.tau)
+        self.steps += n
+        return h
+
 
 class FakeBatch:
     """Stand-in for SteadyBatch: RoRes decays as exp(-steps/tau); the clock
@@ -94,9 +101,10 @@ def test_admission_budget_and_convergence():
     assert batch.peak_bytes <= 60e6
     assert batch.members == []
     for r in reqs:
-        # converged at the first round end past ln(100)*tau steps
+        # one step alone, then converged at the first round end past
+        # ln(100)*tau steps
         tau = 50.0 * math.log10(r.cells)
-        assert r.steps == 50 * math.ceil(math.log(100.0) * tau / 50)
+        assert r.steps == 1 + 50 * math.ceil((math.log(100.0) * tau - 1) / 50)
         assert r.res <= 1e-2 * r.res0
     assert rep["p50_s"] <= rep["p99_s"]
 
@@ -112,3 +120,6 @@ def test_step_cap_failure_and_rejection():
     assert st[0] == "failed"
     assert all(st[k] == "capped" for k in (1, 2, 3))
     assert rep["converged"] == 0 and rep["capped"] == 3 and rep["failed"] == 1 and rep["rejected"] == 1
+    # the members that ran on past the failed round keep their own step-1
+    # baseline (RoRes of a fresh stand-in: three species rows of 1.0)
+    assert all(r.res0 == 3.0 for r in reqs if r.rid in (1, 2, 3))
