@@ -135,7 +135,7 @@ def make_trajectory(name: str, case: nozzle.Case, nsteps: int) -> None:
     """A whole-run fixture: the case inputs, the reference's step_residual
     rows of every step and its final PA after ``nsteps`` steady steps of
     ``lu_sgs_stage`` (no per-call or set-up dumps, so it stays small at
-    config sizes)."""
+    workload sizes)."""
     if ONLY and name not in ONLY:
         return
     tmp = tempfile.mkdtemp(prefix="rcgold_")
@@ -221,7 +221,7 @@ def main() -> None:
     # block-4 order, CFL 5, 2000 steps from the initial field
     c1 = nozzle.nozzle_case(150, 40)
     perm, _ = native.order_multicolor(c1.mesh.cells)
-    make_trajectory("n150x40_config1_2000", c1.permuted(perm), nsteps=2000)
+    make_trajectory("n150x40_run2000", c1.permuted(perm), nsteps=2000)
 
 
 if __name__ == "__main__":

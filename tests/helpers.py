@@ -18,7 +18,7 @@ def fixture_names(kind: str = "steady"):
     ``serving`` the long serving-family runs to the reference's stop;
     ``percall`` those of them that also hold per-call dumps (not the
     8000-step converged runs); ``trajectory`` every steady whole-run fixture,
-    including the config-size runs that hold only residual rows and the
+    including the workload-size runs that hold only residual rows and the
     final state."""
     names = sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
     be = [n for n in names if n.startswith("n24x6_be_")]
@@ -29,7 +29,7 @@ def fixture_names(kind: str = "steady"):
     steady = [n for n in names if n not in be and not n.startswith("serving_")]
     if kind == "trajectory":
         return steady
-    steady = [n for n in steady if "_config" not in n]
+    steady = [n for n in steady if "_run" not in n]
     if kind == "percall":
         return [n for n in steady if "converged" not in n]
     return steady

@@ -102,7 +102,7 @@ def test_converged(name):
 
 @pytest.mark.parametrize("nx,ny,order", [(150, 40, "natural"), (150, 40, "multicolor"), (1000, 200, "multicolor"),
                                          (2000, 400, "multicolor")])
-def test_config_sizes_vs_oracle(nx, ny, order):
+def test_workload_sizes_vs_oracle(nx, ny, order):
     """The 12k-, 400k- and 1.6M-cell nozzles: per-call arrays of one step,
     and three whole steps, against the oracle on the same inputs."""
     case = nozzle.nozzle_case(nx, ny)
@@ -111,7 +111,7 @@ def test_config_sizes_vs_oracle(nx, ny, order):
         assert nc <= 6
         case = case.permuted(perm)
     o = oracle.OracleCase(case)
-    announce("config %dx%d %s NE=%d" % (nx, ny, order, case.mesh.ne))
+    announce("workload %dx%d %s NE=%d" % (nx, ny, order, case.mesh.ne))
     with SteadyCase(case) as g:
         # the library's sweep schedule is the restated one; blocks of the
         # multicolour order become runs, one level per colour

@@ -7,7 +7,7 @@ step; behind the verified gate every cell's source lies within rtol of the
 exact FP64 rate; behind the trust gate (a calibrated region, no exact rate)
 the device accepts the cells its host restatement accepts.  Time and error
 against exact FP64 kinetics at the learned-chemistry workload's size are in
-test_gpu_config_sizes.py."""
+test_gpu_workload_sizes.py."""
 import sys
 
 import numpy as np

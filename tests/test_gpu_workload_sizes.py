@@ -40,9 +40,9 @@ def ordered(spec):
     return case.permuted(perm)
 
 
-def test_config3_batch_of_256_vs_oracle():
+def test_batch_of_256_sampled_cases_vs_oracle():
     specs = nozzle.sample_cases(256, seed=3)
-    announce("config 3: building 256 cases")
+    announce("256-case batch: building the cases")
     with cf.ThreadPoolExecutor(8) as ex:
         cases = list(ex.map(ordered, specs))
     sizes = np.array([c.mesh.ne for c in cases])
@@ -56,7 +56,7 @@ def test_config3_batch_of_256_vs_oracle():
     try:
         with cf.ThreadPoolExecutor(8) as ex:
             gs = list(ex.map(SteadyCase, cases))
-        announce("config 3: batch of %d cases, %d cells" % (len(gs), sizes.sum()))
+        announce("256-case batch: %d cases, %d cells" % (len(gs), sizes.sum()))
         with SteadyBatch() as b:
             for g in gs:
                 b.add(g)
@@ -78,7 +78,7 @@ def test_config3_batch_of_256_vs_oracle():
 
 @pytest.fixture(scope="module")
 def trained():
-    announce("config 4: sample + train a network")
+    announce("learned source: sample + train a network")
     case = nozzle.nozzle_case(40, 10)
     raw, omega = surrogate.sample_states([case], steps=[1, 20, 40, 80], per_snapshot=400, seed=1)
     tf = surrogate.Transform.fit(raw, omega)
@@ -96,13 +96,13 @@ def residual_after(g):
     return g.increment()
 
 
-def test_config4_verified_gate_on_the_1p6m_nozzle(trained):
+def test_verified_gate_on_the_1p6m_nozzle(trained):
     m, tf = trained
     rtol = 1e-3
     case = nozzle.nozzle_case(2000, 400)
     perm, _ = native.order_multicolor(case.mesh.cells)
     case = case.permuted(perm)
-    announce("config 4: verified gate on the 1.6M-triangle nozzle")
+    announce("learned source: verified gate on the 1.6M-triangle nozzle")
     o = oracle.OracleCase(case)
     with SteadyCase(case) as g:
         g.run_steady(3, history=False)
@@ -116,7 +116,7 @@ def test_config4_verified_gate_on_the_1p6m_nozzle(trained):
         acc, ev = g.surrogate_stats()
     vol = o.get("vol")
     assert ev == case.mesh.ne
-    announce("config 4: accepted %d of %d cells" % (acc, ev))
+    announce("learned source: accepted %d of %d cells" % (acc, ev))
     nsp1 = w_exact.shape[1]
     flow = got.shape[1] - nsp1
     # the exact residual is the oracle's; the flow rows carry no source
@@ -130,9 +130,9 @@ def test_config4_verified_gate_on_the_1p6m_nozzle(trained):
     assert np.all(d <= bound * (1 + 1e-9) + 1e-12 * np.abs(ref[:, flow:]))
 
 
-def test_config5_poisson_stream_vs_alone_and_oracle():
+def test_poisson_serving_stream_vs_alone_and_oracle():
     reqs = serving.poisson_requests(8, rate=20.0, seed=4)
-    announce("config 5: serving %d requests, cells %s" % (len(reqs), [r.cells for r in reqs]))
+    announce("serving stream: %d requests, cells %s" % (len(reqs), [r.cells for r in reqs]))
     final = {}
     srv = serving.Server(round_steps=5, orders=30.0, max_steps=10, workers=4,
                          on_finish=lambda r, case: final.__setitem__(r.rid, case.primitive()))
@@ -156,7 +156,7 @@ def test_config5_poisson_stream_vs_alone_and_oracle():
             assert relerr(final[r.rid], o.get("PA"), -1) < 1e-10, r.rid
 
 
-def test_config4_learned_source_time_and_error_against_exact():
+def test_learned_source_time_and_error_against_exact():
     """What the learned source costs and how close it is, against exact FP64
     kinetics on the 1.6M-triangle nozzle: a network trained on states of
     the 12k- and 400k-triangle runs, behind the trust gate (range, mass
@@ -169,7 +169,7 @@ def test_config4_learned_source_time_and_error_against_exact():
     range.)"""
     import time
 
-    announce("config 4: sample 12k/400k states, train, calibrate")
+    announce("learned source: sample 12k/400k states, train, calibrate")
     cases = []
     for nx, ny in ((150, 40), (1000, 200)):
         c = nozzle.nozzle_case(nx, ny)
@@ -192,7 +192,7 @@ def test_config4_learned_source_time_and_error_against_exact():
         big = big.permuted(perm)
         ms, stats = {}, {}
         for mode in ("exact", "trust_gate", "verified_gate"):
-            announce("config 4: 1.6M nozzle, %s" % mode)
+            announce("learned source: 1.6M nozzle, %s" % mode)
             with SteadyCase(big) as g:
                 if mode == "trust_gate":
                     g.set_surrogate(m, tf, tol=0.05, trust=trust)
@@ -215,7 +215,7 @@ def test_config4_learned_source_time_and_error_against_exact():
             stats[mode] = (1.0 - acc / ev, ok.mean(), err[ok].max() if ok.any() else 0.0)
             # the accepted cells of the final state meet the bar
             assert np.all(err[ok] <= 1e-3), mode
-        announce("config 4: exact %.3f ms/step; trust gate %.3f ms/step, fallback %.1f%%, final-state accepted "
+        announce("learned source: exact %.3f ms/step; trust gate %.3f ms/step, fallback %.1f%%, final-state accepted "
                  "%.1f%% (max rel err %.2e); verified gate %.3f ms/step, fallback %.1f%%, accepted %.1f%% "
                  "(max rel err %.2e)" % (ms["exact"], ms["trust_gate"], 100 * stats["trust_gate"][0],
                                          100 * stats["trust_gate"][1], stats["trust_gate"][2], ms["verified_gate"],

@@ -93,8 +93,8 @@ def test_trajectory(fx):
 
 
 @pytest.mark.parametrize("name", [n for n in fixture_names("trajectory") if n not in NAMES])
-def test_config_trajectory(name):
-    """Config-size whole runs (the 12k-triangle nozzle, 2000 steps): every
+def test_long_run_trajectory(name):
+    """Whole runs at workload size (the 12k-triangle nozzle, 2000 steps): every
     step_residual row and the final PA of the reference."""
     d = load(name)
     test_trajectory((name, d, case_from_fixture(d)))
