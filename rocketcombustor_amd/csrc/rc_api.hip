@@ -266,9 +266,11 @@ void chunk_list(std::vector<int4> &out, int c, int lo, int hi, int chunk, int sl
 }
 
 // Work items laid out for the 8 XCDs (thread blocks are dispatched to the
-// XCDs round-robin, block b to XCD b % 8): XCD x takes the x-th eighth of
-// every segment, in chunks; item p of the list goes to XCD p % 8, shorter
-// lists padded with empty items.  With segments that are the colours of a
+// XCDs round-robin: block b of a launch to XCD (b + s) % 8, s carried over
+// from the blocks dispatched before it, so the blocks b = x mod 8 of one
+// launch share an XCD): XCD x takes the x-th eighth of every segment, in
+// chunks; item p of the list goes to XCD group p % 8, shorter lists padded
+// with empty items.  With segments that are the colours of a
 // block-multicolour order (each sorted along the mesh), an XCD sweeps one
 // band of the domain in every colour, so the neighbours it gathers are
 // mostly in its own L2.
