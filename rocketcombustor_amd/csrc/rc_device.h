@@ -254,6 +254,16 @@ __device__ __forceinline__ bool gas_param(TT &th, const double *y, double t,
     return ok;
 }
 
+// the mixture gas constant of ComputeGasParameter alone (its sum over the
+// species in the same order, so the value is gas_param's r bit for bit)
+template <int NSP, class TT>
+__device__ __forceinline__ double gas_rconst(TT &th, const double *y) {
+    double rr = 0.0;
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) rr = rr + th.rsp[i] * y[i];
+    return rr;
+}
+
 // compute_enthalpy, thermo_property.f90:114-144
 template <int NSP, class TT>
 __device__ __forceinline__ bool enthalpy(TT &th, double t, double *h) {

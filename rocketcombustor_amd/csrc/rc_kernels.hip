@@ -599,8 +599,11 @@ __device__ __forceinline__ void cell_residual(const GMesh &M, const GState &S, T
 #pragma unroll
         for (int k = 0; k < NSP - 1; ++k) { y[k] = PA_(kNFlow + k, i); s = s + y[k]; }
         y[NSP - 1] = 1.0 - s;
-        // rho = p/(t*R) of sourceterm.f90:35 is the cached p/(R*t)
-        const double rho = LC_(LC_RR, i);
+        // rho = p/(t*R) of sourceterm.f90:35, formed from the cell's own
+        // primitive record as the cell-average cache forms it (p/(R*t), R
+        // from the same mass fractions): the cached value bit for bit,
+        // without reading the cache record
+        const double rho = PA_(0, i) / (gas_rconst<NSP>(th, y) * t);
         double w[NSP];
         production_rate<NSP>(th, t, rho, y, w);
         const double v = M.vol[i];
