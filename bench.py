@@ -355,7 +355,7 @@ def batched_run(n: int, steps: int, tune: bool = True) -> dict:
     return out
 
 
-def learned_run(train_steps: int, batch: int = 4096, lr: float = 2e-3, nsteps: int = 50, tol: float = 0.05,
+def learned_run(train_steps: int, batch: int = 8192, lr: float = 2e-3, nsteps: int = 50, tol: float = 0.05,
                 rtol: float = 1e-3) -> dict:
     """The learned-chemistry workload: ~10M states of the 12k and 400k
     steady runs (snapshots with the device's exact FP64 rates), the MLP
