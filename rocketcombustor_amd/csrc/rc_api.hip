@@ -55,6 +55,7 @@ using namespace rc;
 
 static thread_local std::string g_error;
 static constexpr int kTimeSlots = 6;
+static constexpr int kGraphSteps = 8;   // steps per multi-step graph (graph_steps())
 
 struct rc_case {
     int device = 0;
@@ -106,6 +107,8 @@ struct rc_case {
     long timed_steps = 0;
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t graph_multi = nullptr;   // graph_multi_n steps in one graph
+    int graph_multi_n = 0;
     std::string err;
 };
 
@@ -122,6 +125,8 @@ struct rc_batch {
     bool dirty = true;             // membership or launch shape changed since the last build
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t graph_multi = nullptr;   // graph_multi_n steps in one graph
+    int graph_multi_n = 0;
     // launch shape: threads per block of the cell / face / ghost kernels and
     // of the LU-SGS level kernels (kRunMax lanes per run), XCD banding of the
     // work lists (-1: as the members were laid out)
@@ -707,6 +712,61 @@ int ensure_graph(rc_case *h) {
     return RC_OK;
 }
 
+// Steps per multi-step graph: runs of that many steps between residual
+// reports are one graph launch instead of one per step (the graph holds the
+// same kernels, step after step, so results are the same bit for bit);
+// RC_GRAPH_STEPS=n sets it (1: one graph per step only).
+int graph_steps() {
+    const char *e = std::getenv("RC_GRAPH_STEPS");
+    const int v = e ? std::atoi(e) : kGraphSteps;
+    return v < 1 ? 1 : (v > 64 ? 64 : v);
+}
+
+void drop_graphs(rc_case *h) {
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph_multi) (void)hipGraphExecDestroy(h->graph_multi);
+    h->graph_exec = nullptr;
+    h->graph_multi = nullptr;
+}
+
+int ensure_graph_multi(rc_case *h, int nsteps) {
+    if (h->graph_multi && h->graph_multi_n == nsteps) return RC_OK;
+    if (h->graph_multi) (void)hipGraphExecDestroy(h->graph_multi);
+    h->graph_multi = nullptr;
+    h->graph_multi_n = nsteps;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < nsteps; ++k) enqueue_step_kernels(h);
+    hipGraph_t g = nullptr;
+    HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+    hipError_t e = hipGraphInstantiate(&h->graph_multi, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        h->graph_multi = nullptr;
+        return set_err(h, RC_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    return RC_OK;
+}
+
+int enqueue_stage(rc_case *h, bool allow_graph);
+
+// k steps: whole multi-step graphs first, then single steps
+int enqueue_stages(rc_case *h, int k) {
+    const int m = graph_steps();
+    while (m > 1 && k >= m && h->use_graph && !h->timing) {
+        int rc = ensure_graph_multi(h, m);
+        if (rc) return rc;
+        HIPCHK(h, hipGraphLaunch(h->graph_multi, h->stream));
+        h->steps_done += m;
+        if (h->sur) h->sur_evals += (long long)m * h->ne;
+        k -= m;
+    }
+    for (; k > 0; --k) {
+        int rc = enqueue_stage(h, true);
+        if (rc) return rc;
+    }
+    return RC_OK;
+}
+
 int enqueue_stage(rc_case *h, bool allow_graph) {
     if (h->timing) {
         if (h->ev_used + kTimeSlots + 1 > 4096) {
@@ -812,9 +872,15 @@ int bset_err(rc_batch *b, int code, const std::string &msg) {
             return bset_err(b, RC_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
     } while (0)
 
-void batch_release(rc_batch *b) {
+void batch_drop_graphs(rc_batch *b) {
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph_multi) (void)hipGraphExecDestroy(b->graph_multi);
     b->graph_exec = nullptr;
+    b->graph_multi = nullptr;
+}
+
+void batch_release(rc_batch *b) {
+    batch_drop_graphs(b);
     if (b->arena) {
         (void)hipStreamSynchronize(b->stream);
         (void)hipFree(b->arena);
@@ -952,16 +1018,21 @@ void enqueue_batch_step(rc_batch *b) {
     launch_update(L, b->stream);
 }
 
-int batch_graph(rc_batch *b) {
-    if (b->graph_exec) return RC_OK;
+// the batch step captured as a graph of nsteps steps (1 or graph_steps())
+int batch_graph(rc_batch *b, int nsteps = 1) {
+    hipGraphExec_t &ge = nsteps == 1 ? b->graph_exec : b->graph_multi;
+    if (ge && (nsteps == 1 || b->graph_multi_n == nsteps)) return RC_OK;
+    if (ge) (void)hipGraphExecDestroy(ge);
+    ge = nullptr;
+    if (nsteps > 1) b->graph_multi_n = nsteps;
     BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_batch_step(b);
+    for (int k = 0; k < nsteps; ++k) enqueue_batch_step(b);
     hipGraph_t g = nullptr;
     BHIPCHK(b, hipStreamEndCapture(b->stream, &g));
-    hipError_t e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
-        b->graph_exec = nullptr;
+        ge = nullptr;
         return bset_err(b, RC_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
     return RC_OK;
@@ -1042,7 +1113,7 @@ void rc_case_destroy(rc_case *h) {
         (void)hipSetDevice(h->device);
         (void)hipStreamSynchronize(h->stream);
     }
-    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    drop_graphs(h);
     if (h->arena) (void)hipFree(h->arena);
     if (h->be_fac) (void)hipFree(h->be_fac);
     if (h->be_piv) (void)hipFree(h->be_piv);
@@ -1140,9 +1211,12 @@ int rc_run_steady(rc_case *h, int nsteps, int every, double *res_out, int *done)
     HIPCHK(h, hipSetDevice(h->device));
     if (every <= 0) every = nsteps + 1;
     int nrep = 0;
-    for (int s = 1; s <= nsteps; ++s) {
-        int rc = enqueue_stage(h, true);
+    for (int s = 0; s < nsteps;) {
+        // steps up to the next residual report (step 1, then every `every`)
+        const int next = std::min(nsteps, s == 0 ? 1 : (s / every + 1) * every);
+        int rc = enqueue_stages(h, next - s);
         if (rc) return rc;
+        s = next;
         if (s == 1 || s % every == 0) {
             if (res_out) {
                 rc = residual_async(h, h->h_res);
@@ -1201,12 +1275,10 @@ int rc_set_transport(rc_case *h, const rc_transport *t) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->viscous = true;
     // the captured step and any batch holding the case change shape
-    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-    h->graph_exec = nullptr;
+    drop_graphs(h);
     if (h->batch) {
         h->batch->dirty = true;
-        if (h->batch->graph_exec) (void)hipGraphExecDestroy(h->batch->graph_exec);
-        h->batch->graph_exec = nullptr;
+        batch_drop_graphs(h->batch);
     }
     return RC_OK;
 }
@@ -1234,8 +1306,7 @@ int rc_set_surrogate(rc_case *h, rc_mlp *m, const double *xform, double tol, dou
     dp.surrogate = m ? 1 : 0;
     HIPCHK(h, hipMemcpyAsync(h->d_par, &dp, sizeof dp, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-    h->graph_exec = nullptr;
+    drop_graphs(h);
     return RC_OK;
 }
 
@@ -1250,8 +1321,7 @@ int rc_set_surrogate_trust(rc_case *h, const int *dims3, const double *lo3, cons
     const int rc = surrogate_set_trust(h->sur, dims3, lo3, hi3, accept, msg);
     if (rc) return set_err(h, rc, msg);
     // the gate's table is a kernel argument: capture the step again
-    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-    h->graph_exec = nullptr;
+    drop_graphs(h);
     return RC_OK;
 }
 
@@ -1645,14 +1715,23 @@ int rc_batch_run_steady(rc_batch *b, int nsteps, int every, double *res_out, int
     const size_t n = b->members.size();
     if (every <= 0) every = nsteps + 1;
     int nrep = 0;
-    for (int s = 1; s <= nsteps; ++s) {
-        if (b->use_graph) {
-            if ((rc = batch_graph(b))) return rc;
-            BHIPCHK(b, hipGraphLaunch(b->graph_exec, b->stream));
-        } else {
-            enqueue_batch_step(b);
+    const int m = graph_steps();
+    for (int s = 0; s < nsteps;) {
+        // steps up to the next residual report (step 1, then every `every`):
+        // whole multi-step graphs first, then single steps
+        const int next = std::min(nsteps, s == 0 ? 1 : (s / every + 1) * every);
+        for (int k = next - s; k > 0;) {
+            const int run = (b->use_graph && m > 1 && k >= m) ? m : 1;
+            if (b->use_graph) {
+                if ((rc = batch_graph(b, run))) return rc;
+                BHIPCHK(b, hipGraphLaunch(run == 1 ? b->graph_exec : b->graph_multi, b->stream));
+            } else {
+                enqueue_batch_step(b);
+            }
+            for (rc_case *h : b->members) h->steps_done += run;
+            k -= run;
         }
-        for (rc_case *h : b->members) h->steps_done++;
+        s = next;
         if (s == 1 || s % every == 0) {
             if (res_out) {
                 launch_resnorm(b->L, b->stream);

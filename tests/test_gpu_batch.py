@@ -185,6 +185,37 @@ def test_batch_graph_replay_matches_eager(monkeypatch):
         assert np.array_equal(a, e)
 
 
+def test_batch_multi_step_graph_matches_single_steps(monkeypatch):
+    """The batch's runs of steps between residual reports as graphs of
+    several steps equal one graph per step and eager launches bitwise,
+    members' states and every reported residual row."""
+    cases = mixed_cases()[:3]
+    announce("batch multi-step graph vs one graph per step vs eager")
+    outs = []
+    for graph_steps, no_graph in ((None, "0"), ("1", "0"), (None, "1")):
+        monkeypatch.setenv("RC_NO_GRAPH", no_graph)
+        if graph_steps is None:
+            monkeypatch.delenv("RC_GRAPH_STEPS", raising=False)
+        else:
+            monkeypatch.setenv("RC_GRAPH_STEPS", graph_steps)
+        gs = [SteadyCase(c) for c in cases]
+        with SteadyBatch() as b:
+            for g in gs:
+                b.add(g)
+            h1 = b.run_steady(25, every=9)
+            _ = [g.primitive() for g in gs]
+            h2 = b.run_steady(17, history=False)
+            outs.append((h1, [g.primitive() for g in gs], [g.increment() for g in gs]))
+        for g in gs:
+            g.close()
+    for other in outs[1:]:
+        assert np.array_equal(outs[0][0], other[0])
+        for a, e in zip(outs[0][1], other[1]):
+            assert np.array_equal(a, e)
+        for a, e in zip(outs[0][2], other[2]):
+            assert np.array_equal(a, e)
+
+
 @pytest.mark.parametrize("shape", [(64, 64, 0), (128, 256, 1), (64, 128, 1), (256, 64, -1)])
 def test_launch_shape_does_not_change_results(shape):
     """The autotuner's knob (threads per block, XCD banding) re-cuts the work

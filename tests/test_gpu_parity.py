@@ -171,6 +171,42 @@ def test_graph_replay_matches_eager():
         assert np.array_equal(graph.conserved(), eager.conserved())
 
 
+def test_multi_step_graph_matches_single_step_graphs(monkeypatch):
+    """Runs of steps between residual reports replay a graph of several
+    steps (RC_GRAPH_STEPS, 8 by default) instead of one graph per step: the
+    state, the increments and every reported residual row equal those of one
+    graph per step and of eager launches, bit for bit, also when the number
+    of steps per graph changes between calls (the graph is captured again)
+    and with host reads between the calls."""
+    case = nozzle.nozzle_case(150, 40)
+    perm, _ = native.order_multicolor(case.mesh.cells)
+    case = case.permuted(perm)
+    announce("multi-step graph vs one graph per step vs eager, 150x40")
+    runs = {}
+    for label, graph_steps, no_graph in (("multi", None, "0"), ("single", "1", "0"), ("eager", None, "1")):
+        monkeypatch.setenv("RC_NO_GRAPH", no_graph)
+        if graph_steps is None:
+            monkeypatch.delenv("RC_GRAPH_STEPS", raising=False)
+        else:
+            monkeypatch.setenv("RC_GRAPH_STEPS", graph_steps)
+        hist, states = [], []
+        with SteadyCase(case) as g:
+            hist.append(g.run_steady(1))
+            hist.append(g.run_steady(30, every=10))
+            states.append((g.primitive(), g.increment()))
+            if label == "multi":
+                monkeypatch.setenv("RC_GRAPH_STEPS", "4")
+            hist.append(g.run_steady(19, every=7))
+            states.append((g.primitive(), g.increment(), g.conserved()))
+        runs[label] = (hist, states)
+    for other in ("single", "eager"):
+        for a, b in zip(runs["multi"][0], runs[other][0]):
+            assert np.array_equal(a, b), other
+        for sa, sb in zip(runs["multi"][1], runs[other][1]):
+            for a, b in zip(sa, sb):
+                assert np.array_equal(a, b), other
+
+
 @pytest.mark.parametrize("nx,ny,order", [(1, 1, "natural"), (24, 6, "natural"), (150, 40, "natural"),
                                          (150, 40, "multicolor"), (1000, 200, "multicolor"),
                                          (1000, 200, "natural")])
