@@ -130,8 +130,10 @@ int rc_lusgs_stage(rc_case *h);
  * `every` steps (and at step 1) the step_residual norms
  * (residual.f90:61-79): res_out[(n)*(nvar+1) + k] holds Res(k)/total_volume
  * and, at k = nvar, max(rho), for the n-th reported step.  Runs
- * asynchronously between reports; `res_out` may be NULL.  Returns the
- * number of completed steps in *done. */
+ * asynchronously between reports; the device error flags are checked at
+ * each report and at the end.  `res_out` may be NULL (then there is no
+ * report at step 1, only every `every` steps).  Returns the number of
+ * completed steps in *done. */
 int rc_run_steady(rc_case *h, int nsteps, int every, double *res_out, int *done);
 
 /* ---- viscous path (IfViscous, startup.f90:76).
@@ -328,7 +330,8 @@ int rc_batch_info(const rc_batch *b, long long *info3);
  * (the reference steps one case with serial loops). */
 int rc_batch_set_launch(rc_batch *b, int cell_threads, int run_threads, int xcd);
 /* rc_run_steady for every member at once: nsteps steps; every `every` steps
- * and at step 1, res_out[(n*members + c)*(nvar+1) + k] = the step_residual
+ * and at step 1 (when res_out is given; reports check the device error
+ * flags), res_out[(n*members + c)*(nvar+1) + k] = the step_residual
  * norms of member c (member order = order of rc_batch_add, after removals).
  * On a device error of a member the run stops at that report with the
  * member's code and *err_case = its index. */

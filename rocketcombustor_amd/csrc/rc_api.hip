@@ -747,13 +747,21 @@ int ensure_graph_multi(rc_case *h, int nsteps) {
     return RC_OK;
 }
 
+// both graphs at the first graph replay, so that a run's first use of the
+// multi-step graph does not capture it
+int ensure_graphs(rc_case *h) {
+    int rc = ensure_graph(h);
+    if (!rc && graph_steps() > 1) rc = ensure_graph_multi(h, graph_steps());
+    return rc;
+}
+
 int enqueue_stage(rc_case *h, bool allow_graph);
 
 // k steps: whole multi-step graphs first, then single steps
 int enqueue_stages(rc_case *h, int k) {
     const int m = graph_steps();
     while (m > 1 && k >= m && h->use_graph && !h->timing) {
-        int rc = ensure_graph_multi(h, m);
+        int rc = ensure_graphs(h);
         if (rc) return rc;
         HIPCHK(h, hipGraphLaunch(h->graph_multi, h->stream));
         h->steps_done += m;
@@ -779,7 +787,7 @@ int enqueue_stage(rc_case *h, bool allow_graph) {
         HIPCHK(h, hipEventRecord(h->evpool[h->ev_used + kTimeSlots], h->stream));
         h->ev_used += kTimeSlots + 1;
     } else if (allow_graph && h->use_graph) {
-        int rc = ensure_graph(h);
+        int rc = ensure_graphs(h);
         if (rc) return rc;
         HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
     } else {
@@ -1211,13 +1219,15 @@ int rc_run_steady(rc_case *h, int nsteps, int every, double *res_out, int *done)
     HIPCHK(h, hipSetDevice(h->device));
     if (every <= 0) every = nsteps + 1;
     int nrep = 0;
+    // reports (residual rows, device error check): at step 1 when rows are
+    // asked for, and every `every` steps
+    const bool first = res_out != nullptr;
     for (int s = 0; s < nsteps;) {
-        // steps up to the next residual report (step 1, then every `every`)
-        const int next = std::min(nsteps, s == 0 ? 1 : (s / every + 1) * every);
+        const int next = std::min(nsteps, s == 0 && first ? 1 : (s / every + 1) * every);
         int rc = enqueue_stages(h, next - s);
         if (rc) return rc;
         s = next;
-        if (s == 1 || s % every == 0) {
+        if ((s == 1 && first) || s % every == 0) {
             if (res_out) {
                 rc = residual_async(h, h->h_res);
                 if (rc) return rc;
@@ -1716,14 +1726,18 @@ int rc_batch_run_steady(rc_batch *b, int nsteps, int every, double *res_out, int
     if (every <= 0) every = nsteps + 1;
     int nrep = 0;
     const int m = graph_steps();
+    // reports (residual rows, device error checks): at step 1 when rows are
+    // asked for, and every `every` steps; between them whole multi-step
+    // graphs first, then single steps
+    const bool first = res_out != nullptr;
     for (int s = 0; s < nsteps;) {
-        // steps up to the next residual report (step 1, then every `every`):
-        // whole multi-step graphs first, then single steps
-        const int next = std::min(nsteps, s == 0 ? 1 : (s / every + 1) * every);
+        const int next = std::min(nsteps, s == 0 && first ? 1 : (s / every + 1) * every);
         for (int k = next - s; k > 0;) {
             const int run = (b->use_graph && m > 1 && k >= m) ? m : 1;
             if (b->use_graph) {
-                if ((rc = batch_graph(b, run))) return rc;
+                // both graphs at the first replay (no capture inside a later run)
+                if ((rc = batch_graph(b, 1))) return rc;
+                if (m > 1 && (rc = batch_graph(b, m))) return rc;
                 BHIPCHK(b, hipGraphLaunch(run == 1 ? b->graph_exec : b->graph_multi, b->stream));
             } else {
                 enqueue_batch_step(b);
@@ -1732,7 +1746,7 @@ int rc_batch_run_steady(rc_batch *b, int nsteps, int every, double *res_out, int
             k -= run;
         }
         s = next;
-        if (s == 1 || s % every == 0) {
+        if ((s == 1 && first) || s % every == 0) {
             if (res_out) {
                 launch_resnorm(b->L, b->stream);
                 BHIPCHK(b, hipMemcpyAsync(b->h_res, b->L.resout, sizeof(double) * n * (kNvar + 1),
