@@ -31,6 +31,7 @@ int flow_max_blocks(int);
 void launch_update(const LaunchSet &, hipStream_t);
 void launch_prim_to_cons(const LaunchSet &, hipStream_t);
 void launch_resnorm(const LaunchSet &, hipStream_t);
+void launch_gather_err(const LaunchSet &, int *, hipStream_t);
 void launch_derived(const LaunchSet &, double *, hipStream_t);
 void launch_be_prepare(const LaunchSet &, double, bool, const double *, const double *, double, hipStream_t);
 void launch_be_factor(const LaunchSet &, double *, int *, double *, hipStream_t);
@@ -122,6 +123,9 @@ struct rc_batch {
     int nlev = 0;
     double *h_res = nullptr;       // pinned, members x (kNvar+1)
     size_t h_res_n = 0;
+    int *d_err = nullptr;          // the members' error flags gathered, members x 4 (in the arena)
+    int *h_err = nullptr;          // pinned copy of them
+    size_t h_err_n = 0;
     bool dirty = true;             // membership or launch shape changed since the last build
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
@@ -174,9 +178,11 @@ const char *kernel_name(int k) {
 }
 
 // reads the device error flag (stream must be synchronised)
-int check_device_error(rc_case *h) {
-    HIPCHK(h, hipMemcpyAsync(h->h_err, h->S.err, 4 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+int check_device_error(rc_case *h, bool copied = false) {
+    if (!copied) {
+        HIPCHK(h, hipMemcpyAsync(h->h_err, h->S.err, 4 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
     if (h->h_err[0] == 0) return RC_OK;
     const int code = h->h_err[0];
     char buf[256];
@@ -193,11 +199,13 @@ int check_device_error(rc_case *h) {
 
 int drain_timing(rc_case *h);
 
+// the error flags copied behind the stream's work, one synchronisation
 int sync_check(rc_case *h) {
+    HIPCHK(h, hipMemcpyAsync(h->h_err, h->S.err, 4 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     int rc = drain_timing(h);
     if (rc) return rc;
-    return check_device_error(h);
+    return check_device_error(h, true);
 }
 
 DevThermo make_thermo(const rc_mechanism &m) {
@@ -894,6 +902,7 @@ void batch_release(rc_batch *b) {
         (void)hipFree(b->arena);
     }
     b->arena = nullptr;
+    b->d_err = nullptr;
 }
 
 // concatenates the members' descriptors and work lists (case index =
@@ -945,9 +954,10 @@ int batch_build(rc_batch *b) {
     build_flow(wl, fitems, fspan);
     Carver dry, real;
     int4 *dwn = nullptr, *dflow = nullptr;
-    int *dctr = nullptr;
+    int *dctr = nullptr, *derr = nullptr;
     auto plan = [&](Carver &c, CaseDev *&dc, int4 *&dwc, int4 *&dwf, int4 *&dwg, int4 *&dwl, double *&dres) {
         dc = c.take<CaseDev>(std::max(n, 1));
+        derr = c.take<int>(4 * (size_t)std::max(n, 1));
         dflow = c.take<int4>(fitems.size());
         dctr = c.take<int>(kXcdLists);
         dwn = c.take<int4>(wn.size());
@@ -1004,6 +1014,15 @@ int batch_build(rc_batch *b) {
         BHIPCHK(b, hipHostMalloc(&b->h_res, need * sizeof(double)));
         b->h_res_n = need;
     }
+    b->d_err = derr;
+    const size_t nerr = 4 * (size_t)std::max(n, 1);
+    if (b->h_err_n < nerr) {
+        if (b->h_err) (void)hipHostFree(b->h_err);
+        b->h_err = nullptr;
+        b->h_err_n = 0;
+        BHIPCHK(b, hipHostMalloc(&b->h_err, nerr * sizeof(int)));
+        b->h_err_n = nerr;
+    }
     b->dirty = false;
     return RC_OK;
 }
@@ -1046,12 +1065,19 @@ int batch_graph(rc_batch *b, int nsteps = 1) {
     return RC_OK;
 }
 
-// device error flags of every member; the first failing member's code
+// device error flags of every member (gathered by one kernel behind the
+// batch's work, one copy, one synchronisation); the first failing member's
+// code
 int batch_check(rc_batch *b, int *err_case) {
+    launch_gather_err(b->L, b->d_err, b->stream);
+    BHIPCHK(b, hipMemcpyAsync(b->h_err, b->d_err, sizeof(int) * 4 * b->members.size(), hipMemcpyDeviceToHost,
+                              b->stream));
     BHIPCHK(b, hipStreamSynchronize(b->stream));
     for (size_t c = 0; c < b->members.size(); ++c) {
         rc_case *h = b->members[c];
-        const int rc = check_device_error(h);
+        if (b->h_err[4 * c] == 0) continue;
+        for (int k = 0; k < 4; ++k) h->h_err[k] = b->h_err[4 * c + k];
+        const int rc = check_device_error(h, true);
         if (rc) {
             if (err_case) *err_case = (int)c;
             return bset_err(b, rc, "case " + std::to_string(c) + ": " + h->err);
@@ -1658,6 +1684,7 @@ void rc_batch_destroy(rc_batch *b) {
     for (rc_case *h : b->members) h->batch = nullptr;
     batch_release(b);
     if (b->h_res) (void)hipHostFree(b->h_res);
+    if (b->h_err) (void)hipHostFree(b->h_err);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }

@@ -1274,6 +1274,20 @@ void launch_derived(const LaunchSet &L, double *out, hipStream_t st) {
     }
 }
 
+// the error flags err[0..3] of every case of the set, out[4c + k]
+__global__ void k_gather_err(const CaseDev *__restrict__ cases, int n, int *__restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) {
+        const RC_G int *e = gp(gp(cases)[c].S.err);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[4 * c + k] = e[k];
+    }
+}
+
+void launch_gather_err(const LaunchSet &L, int *out, hipStream_t st) {
+    if (L.ncases > 0) k_gather_err<<<(L.ncases + 255) / 256, 256, 0, st>>>(L.cases, L.ncases, out);
+}
+
 void launch_resnorm(const LaunchSet &L, hipStream_t st) {
     LAUNCH(L.norm, kChunk, st, k_resnorm_partial, kNsp);
     k_resnorm_final<<<L.ncases, 256, 0, st>>>(L.cases, L.resout);
