@@ -56,7 +56,8 @@ using namespace rc;
 
 static thread_local std::string g_error;
 static constexpr int kTimeSlots = 6;
-static constexpr int kGraphSteps = 8;   // steps per multi-step graph (graph_steps())
+static constexpr int kGraphSteps = 8;   // most steps in one graph (graph_steps())
+static constexpr int kGraphPows = 6;    // graphs of 2^1 .. 2^6 steps at most
 
 struct rc_case {
     int device = 0;
@@ -108,8 +109,8 @@ struct rc_case {
     long timed_steps = 0;
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
-    hipGraphExec_t graph_multi = nullptr;   // graph_multi_n steps in one graph
-    int graph_multi_n = 0;
+    hipGraphExec_t graph_pow[kGraphPows] = {};   // 2, 4, 8, ... steps in one graph
+    int graph_pow_top = 0;                       // the largest of them (0: none captured)
     std::string err;
 };
 
@@ -129,8 +130,8 @@ struct rc_batch {
     bool dirty = true;             // membership or launch shape changed since the last build
     bool use_graph = true;
     hipGraphExec_t graph_exec = nullptr;
-    hipGraphExec_t graph_multi = nullptr;   // graph_multi_n steps in one graph
-    int graph_multi_n = 0;
+    hipGraphExec_t graph_pow[kGraphPows] = {};   // 2, 4, 8, ... steps in one graph
+    int graph_pow_top = 0;
     // launch shape: threads per block of the cell / face / ghost kernels and
     // of the LU-SGS level kernels (kRunMax lanes per run), XCD banding of the
     // work lists (-1: as the members were laid out)
@@ -730,51 +731,82 @@ int graph_steps() {
     return v < 1 ? 1 : (v > 64 ? 64 : v);
 }
 
-void drop_graphs(rc_case *h) {
-    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-    if (h->graph_multi) (void)hipGraphExecDestroy(h->graph_multi);
-    h->graph_exec = nullptr;
-    h->graph_multi = nullptr;
+// the largest power of two not above graph_steps()
+int graph_top() {
+    int p = 1;
+    while (2 * p <= graph_steps()) p *= 2;
+    return p;
+}
+// slot of the graph of p = 2, 4, 8, ... steps
+int pow_index(int p) {
+    int j = -1;
+    while (p > 1) {
+        p >>= 1;
+        ++j;
+    }
+    return j;
 }
 
-int ensure_graph_multi(rc_case *h, int nsteps) {
-    if (h->graph_multi && h->graph_multi_n == nsteps) return RC_OK;
-    if (h->graph_multi) (void)hipGraphExecDestroy(h->graph_multi);
-    h->graph_multi = nullptr;
-    h->graph_multi_n = nsteps;
+void drop_graphs(rc_case *h) {
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+    for (auto &g : h->graph_pow) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    h->graph_pow_top = 0;
+}
+
+int capture_steps(rc_case *h, int nsteps, hipGraphExec_t &out) {
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < nsteps; ++k) enqueue_step_kernels(h);
     hipGraph_t g = nullptr;
     HIPCHK(h, hipStreamEndCapture(h->stream, &g));
-    hipError_t e = hipGraphInstantiate(&h->graph_multi, g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(&out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
-        h->graph_multi = nullptr;
+        out = nullptr;
         return set_err(h, RC_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
     return RC_OK;
 }
 
-// both graphs at the first graph replay, so that a run's first use of the
-// multi-step graph does not capture it
+// the one-step graph and the graphs of 2, 4, ... graph_top() steps, all at
+// the first graph replay, so that no later run captures inside its steps
 int ensure_graphs(rc_case *h) {
     int rc = ensure_graph(h);
-    if (!rc && graph_steps() > 1) rc = ensure_graph_multi(h, graph_steps());
-    return rc;
+    if (rc) return rc;
+    const int top = graph_top();
+    if (h->graph_pow_top == top) return RC_OK;
+    for (auto &g : h->graph_pow) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    h->graph_pow_top = 0;
+    for (int p = 2; p <= top; p *= 2)
+        if ((rc = capture_steps(h, p, h->graph_pow[pow_index(p)]))) return rc;
+    h->graph_pow_top = top;
+    return RC_OK;
 }
 
 int enqueue_stage(rc_case *h, bool allow_graph);
 
-// k steps: whole multi-step graphs first, then single steps
+// k steps as graphs of graph_top() steps, then the rest by its binary
+// digits (at most k/top + log2(top) graph launches); each graph holds the
+// same kernels step after step, so the results are those of one graph per
+// step
 int enqueue_stages(rc_case *h, int k) {
-    const int m = graph_steps();
-    while (m > 1 && k >= m && h->use_graph && !h->timing) {
+    if (h->use_graph && !h->timing && k > 1) {
         int rc = ensure_graphs(h);
         if (rc) return rc;
-        HIPCHK(h, hipGraphLaunch(h->graph_multi, h->stream));
-        h->steps_done += m;
-        if (h->sur) h->sur_evals += (long long)m * h->ne;
-        k -= m;
+        for (int p = h->graph_pow_top; p >= 2; p /= 2) {
+            while (k >= p) {
+                HIPCHK(h, hipGraphLaunch(h->graph_pow[pow_index(p)], h->stream));
+                h->steps_done += p;
+                if (h->sur) h->sur_evals += (long long)p * h->ne;
+                k -= p;
+            }
+        }
     }
     for (; k > 0; --k) {
         int rc = enqueue_stage(h, true);
@@ -890,9 +922,12 @@ int bset_err(rc_batch *b, int code, const std::string &msg) {
 
 void batch_drop_graphs(rc_batch *b) {
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
-    if (b->graph_multi) (void)hipGraphExecDestroy(b->graph_multi);
     b->graph_exec = nullptr;
-    b->graph_multi = nullptr;
+    for (auto &g : b->graph_pow) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    b->graph_pow_top = 0;
 }
 
 void batch_release(rc_batch *b) {
@@ -1045,23 +1080,36 @@ void enqueue_batch_step(rc_batch *b) {
     launch_update(L, b->stream);
 }
 
-// the batch step captured as a graph of nsteps steps (1 or graph_steps())
-int batch_graph(rc_batch *b, int nsteps = 1) {
-    hipGraphExec_t &ge = nsteps == 1 ? b->graph_exec : b->graph_multi;
-    if (ge && (nsteps == 1 || b->graph_multi_n == nsteps)) return RC_OK;
-    if (ge) (void)hipGraphExecDestroy(ge);
-    ge = nullptr;
-    if (nsteps > 1) b->graph_multi_n = nsteps;
+// the batch step captured as a graph of nsteps steps
+int batch_capture(rc_batch *b, int nsteps, hipGraphExec_t &out) {
     BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < nsteps; ++k) enqueue_batch_step(b);
     hipGraph_t g = nullptr;
     BHIPCHK(b, hipStreamEndCapture(b->stream, &g));
-    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(&out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
-        ge = nullptr;
+        out = nullptr;
         return bset_err(b, RC_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
+    return RC_OK;
+}
+
+// the one-step graph and the graphs of 2, 4, ... graph_top() steps, all at
+// the first replay of a build
+int batch_graphs(rc_batch *b) {
+    int rc;
+    if (!b->graph_exec && (rc = batch_capture(b, 1, b->graph_exec))) return rc;
+    const int top = graph_top();
+    if (b->graph_pow_top == top) return RC_OK;
+    for (auto &g : b->graph_pow) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    b->graph_pow_top = 0;
+    for (int p = 2; p <= top; p *= 2)
+        if ((rc = batch_capture(b, p, b->graph_pow[pow_index(p)]))) return rc;
+    b->graph_pow_top = top;
     return RC_OK;
 }
 
@@ -1752,7 +1800,6 @@ int rc_batch_run_steady(rc_batch *b, int nsteps, int every, double *res_out, int
     const size_t n = b->members.size();
     if (every <= 0) every = nsteps + 1;
     int nrep = 0;
-    const int m = graph_steps();
     // reports (residual rows, device error checks): at step 1 when rows are
     // asked for, and every `every` steps; between them whole multi-step
     // graphs first, then single steps
@@ -1760,12 +1807,13 @@ int rc_batch_run_steady(rc_batch *b, int nsteps, int every, double *res_out, int
     for (int s = 0; s < nsteps;) {
         const int next = std::min(nsteps, s == 0 && first ? 1 : (s / every + 1) * every);
         for (int k = next - s; k > 0;) {
-            const int run = (b->use_graph && m > 1 && k >= m) ? m : 1;
+            int run = 1;
             if (b->use_graph) {
-                // both graphs at the first replay (no capture inside a later run)
-                if ((rc = batch_graph(b, 1))) return rc;
-                if (m > 1 && (rc = batch_graph(b, m))) return rc;
-                BHIPCHK(b, hipGraphLaunch(run == 1 ? b->graph_exec : b->graph_multi, b->stream));
+                if ((rc = batch_graphs(b))) return rc;
+                // the largest graph that fits the rest (top, then its binary digits)
+                for (int p = b->graph_pow_top; p >= 2 && run == 1; p /= 2)
+                    if (k >= p) run = p;
+                BHIPCHK(b, hipGraphLaunch(run == 1 ? b->graph_exec : b->graph_pow[pow_index(run)], b->stream));
             } else {
                 enqueue_batch_step(b);
             }
