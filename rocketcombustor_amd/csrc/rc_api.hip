@@ -721,10 +721,11 @@ int ensure_graph(rc_case *h) {
     return RC_OK;
 }
 
-// Steps per multi-step graph: runs of that many steps between residual
-// reports are one graph launch instead of one per step (the graph holds the
-// same kernels, step after step, so results are the same bit for bit);
-// RC_GRAPH_STEPS=n sets it (1: one graph per step only).
+// Most steps in one graph: a run of steps between residual reports is
+// replayed as graphs of several steps instead of one graph per step (a
+// graph holds the same kernels, step after step, so results are the same
+// bit for bit); RC_GRAPH_STEPS=n sets it (rounded down to a power of two;
+// 1: one graph per step only).
 int graph_steps() {
     const char *e = std::getenv("RC_GRAPH_STEPS");
     const int v = e ? std::atoi(e) : kGraphSteps;
