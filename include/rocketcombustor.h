@@ -28,6 +28,11 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+/* The library is built with hidden symbols; these entry points are its
+ * only exports, so no other library loaded in the process (an MPI
+ * runtime, a profiler's tools, a framework) can interpose on its internals
+ * or it on theirs. */
+#pragma GCC visibility push(default)
 
 #define RC_MAXSP 8
 
@@ -337,6 +342,7 @@ int rc_batch_set_launch(rc_batch *b, int cell_threads, int run_threads, int xcd)
  * member's code and *err_case = its index. */
 int rc_batch_run_steady(rc_batch *b, int nsteps, int every, double *res_out, int *done, int *err_case);
 
+#pragma GCC visibility pop
 #ifdef __cplusplus
 }
 #endif

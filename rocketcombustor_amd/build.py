@@ -5,7 +5,8 @@
 The kernels are compiled without floating-point contraction
 (-ffp-contract=off) so that every FP64 expression rounds as the
 reference's Fortran (x86-64 baseline, no FMA) does, and without packed FP32
-instructions.
+instructions.  Symbols are hidden except the C-ABI of the header, and the
+library's calls to functions it defines bind to its own definitions.
 """
 from __future__ import annotations
 
@@ -32,6 +33,7 @@ def build_library(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
            "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-fvisibility=hidden", "-fvisibility-inlines-hidden", "-Wl,-Bsymbolic-functions",
            "-Wall", "-Wno-unused-function", "-I" + os.path.join(ROOT, "include"),
            "-o", tmp] + [os.path.join(PKG, s) for s in SOURCES]
     if verbose:
