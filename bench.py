@@ -270,7 +270,7 @@ def serving_run(n: int, plain: bool) -> dict:
     same requests served one at a time through the single-case path."""
     from rocketcombustor_amd import serving
     reqs = serving.poisson_requests(n, rate=SERVE_RATE)
-    srv = serving.Server(round_steps=SERVE_ROUND, orders=SERVE_ORDERS, max_steps=20000, workers=4)
+    srv = serving.Server(round_steps=SERVE_ROUND, orders=SERVE_ORDERS, max_steps=20000, workers=8)
     rep = srv.run(reqs, time_limit=600.0)
     if rep.get("cell_updates_per_s"):
         rep["step_hbm_frac_bcu"] = B_CU_TRI * rep["cell_updates_per_s"] / (HBM_PEAK_GBS * 1e9)

@@ -773,7 +773,8 @@ int capture_steps(rc_case *h, int nsteps, hipGraphExec_t &out) {
 }
 
 // the one-step graph and the graphs of 2, 4, ... graph_top() steps, all at
-// the first graph replay, so that no later run captures inside its steps
+// the first run of more than one step (so that no later run captures inside
+// its steps; a case only ever stepped one step at a time captures one)
 int ensure_graphs(rc_case *h) {
     int rc = ensure_graph(h);
     if (rc) return rc;
@@ -828,7 +829,7 @@ int enqueue_stage(rc_case *h, bool allow_graph) {
         HIPCHK(h, hipEventRecord(h->evpool[h->ev_used + kTimeSlots], h->stream));
         h->ev_used += kTimeSlots + 1;
     } else if (allow_graph && h->use_graph) {
-        int rc = ensure_graphs(h);
+        int rc = ensure_graph(h);
         if (rc) return rc;
         HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
     } else {
