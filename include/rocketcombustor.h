@@ -295,7 +295,7 @@ int rc_get_derived_noalloc(rc_case *h, double *rho, double *mach, double *prod);
  * sweep (all wavefronts), t[4] backward sweep, t[5] update; t[6] the number
  * of timed steps.  Collected with HIP events recorded on the case's stream
  * between the kernels, without synchronising; with timing off, rc_run_steady
- * replays the step as one captured HIP graph instead. */
+ * replays the steps as captured HIP graphs (of up to 8 steps) instead. */
 int rc_set_timing(rc_case *h, int enable);
 int rc_get_timing(rc_case *h, double *t7);
 
