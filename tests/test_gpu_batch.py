@@ -140,10 +140,12 @@ def test_continuous_batching_join_and_leave():
         g.close()
 
 
-def test_member_error_is_isolated():
+@pytest.mark.parametrize("every", [1, 10])
+def test_member_error_is_isolated(every):
     """The no-limiter case stops at step 5 (as the reference does); the
     error names the member, and the healthy member continues to the same
-    state as alone."""
+    state as alone (reports every step, or every 10 steps with the steps
+    between them replayed as graphs of several steps)."""
     bad = nozzle.nozzle_case(24, 6, limit_id=0)
     good = nozzle.nozzle_case(30, 8)
     announce("batch member error")
@@ -152,7 +154,7 @@ def test_member_error_is_isolated():
         b.add(ggood)
         b.add(gbad)
         with pytest.raises(native.RcError) as ei:
-            b.run_steady(20, every=1)
+            b.run_steady(20, every=every)
         assert "case 1" in str(ei.value)
         done = ei.value.done
         assert ei.value.member == 1 and 1 <= done < 20

@@ -83,6 +83,27 @@ def test_trajectory_vs_reference(name):
             assert relerr(g.conserved(), d["ref_final_WA"], -1) < tol
 
 
+STOPS = [n for n in TRAJ if "stop" in n]
+
+
+@pytest.mark.parametrize("name", STOPS)
+@pytest.mark.parametrize("every,history", [(0, False), (10, True), (7, False)])
+def test_stop_step_through_multi_step_graphs(name, every, history):
+    """A run that the reference stops (temperature out of the polynomial
+    range) reports the reference's stop step also when the steps between
+    reports are replayed as graphs of several steps and the error is found
+    at a later report or at the end of the run."""
+    d = load(name)
+    case = case_from_fixture(d)
+    nsteps, err_step = int(d["nsteps"]), int(d["ref_error_step"])
+    announce("stop %s through multi-step graphs, every=%d" % (name, every))
+    with SteadyCase(case) as g:
+        with pytest.raises(native.RcError) as ei:
+            g.run_steady(nsteps, every=every, history=history)
+        assert ei.value.code == 2
+        assert "step %d)" % err_step in str(ei.value)
+
+
 @pytest.mark.parametrize("name", ["n30x8_converged", "n30x8_multicolor_converged"])
 def test_converged(name):
     """Both sweep orders converge the reacting 480-cell nozzle by more than 12
