@@ -32,6 +32,9 @@ void launch_update(const LaunchSet &, hipStream_t);
 void launch_prim_to_cons(const LaunchSet &, hipStream_t);
 void launch_resnorm(const LaunchSet &, hipStream_t);
 void launch_gather_err(const LaunchSet &, int *, hipStream_t);
+bool persist_supported(const LaunchSet &);
+int persist_grid(const LaunchSet &, int);
+hipError_t launch_step_persist(const LaunchSet &, unsigned *, int, int, hipStream_t);
 void launch_derived(const LaunchSet &, double *, hipStream_t);
 void launch_be_prepare(const LaunchSet &, double, bool, const double *, const double *, double, hipStream_t);
 void launch_be_factor(const LaunchSet &, double *, int *, double *, hipStream_t);
@@ -111,6 +114,8 @@ struct rc_case {
     hipGraphExec_t graph_exec = nullptr;
     hipGraphExec_t graph_pow[kGraphPows] = {};   // 2, 4, 8, ... steps in one graph
     int graph_pow_top = 0;                       // the largest of them (0: none captured)
+    int persist = 0;               // steps per persistent launch (0: graphs)
+    unsigned *d_bar = nullptr;     // the persistent step's barrier words
     std::string err;
 };
 
@@ -132,6 +137,8 @@ struct rc_batch {
     hipGraphExec_t graph_exec = nullptr;
     hipGraphExec_t graph_pow[kGraphPows] = {};   // 2, 4, 8, ... steps in one graph
     int graph_pow_top = 0;
+    int persist = 0;               // steps per persistent launch (0: graphs)
+    unsigned *d_bar = nullptr;     // the persistent step's barrier words
     // launch shape: threads per block of the cell / face / ghost kernels and
     // of the LU-SGS level kernels (kRunMax lanes per run), XCD banding of the
     // work lists (-1: as the members were laid out)
@@ -738,6 +745,15 @@ int graph_top() {
     while (2 * p <= graph_steps()) p *= 2;
     return p;
 }
+// RC_PERSIST=n (1 <= n <= 64): steps run as the persistent step, n steps
+// per cooperative launch at most (read when a case or batch is created;
+// unset or 0: one graph launch per step or per run of steps)
+int persist_env() {
+    const char *e = std::getenv("RC_PERSIST");
+    const int n = e ? std::atoi(e) : 0;
+    return std::max(0, std::min(n, 64));
+}
+
 // slot of the graph of p = 2, 4, 8, ... steps
 int pow_index(int p) {
     int j = -1;
@@ -798,6 +814,18 @@ int enqueue_stage(rc_case *h, bool allow_graph);
 // same kernels step after step, so the results are those of one graph per
 // step
 int enqueue_stages(rc_case *h, int k) {
+    if (h->persist > 0 && !h->timing && !h->viscous && !h->sur && persist_supported(h->L)) {
+        if (!h->d_bar) HIPCHK(h, hipMalloc(&h->d_bar, 2 * sizeof(unsigned)));
+        const int grid = persist_grid(h->L, h->device);
+        if (grid <= 0) return set_err(h, RC_ERR_DEVICE, "persistent step: no resident blocks");
+        while (k > 0) {
+            const int run = std::min(k, h->persist);
+            HIPCHK(h, launch_step_persist(h->L, h->d_bar, run, grid, h->stream));
+            h->steps_done += run;
+            k -= run;
+        }
+        return RC_OK;
+    }
     if (h->use_graph && !h->timing && k > 1) {
         int rc = ensure_graphs(h);
         if (rc) return rc;
@@ -1186,6 +1214,7 @@ int rc_case_create(const rc_mechanism *mech, const rc_params *par, int nnt, cons
     if (rc) return bail(rc);
     const char *ng = std::getenv("RC_NO_GRAPH");
     h->use_graph = !(ng && ng[0] == '1');
+    h->persist = persist_env();
     *out = h;
     return RC_OK;
 }
@@ -1206,6 +1235,7 @@ void rc_case_destroy(rc_case *h) {
     if (h->bdf_wp2) (void)hipFree(h->bdf_wp2);
     if (h->sur) surrogate_destroy(h->sur);
     if (h->sur_src) (void)hipFree(h->sur_src);
+    if (h->d_bar) (void)hipFree(h->d_bar);
     for (auto &ev : h->evpool) (void)hipEventDestroy(ev);
     if (h->h_err) (void)hipHostFree(h->h_err);
     if (h->h_res) (void)hipHostFree(h->h_res);
@@ -1724,6 +1754,7 @@ int rc_batch_create(int device, rc_batch **out) {
     }
     const char *ng = std::getenv("RC_NO_GRAPH");
     b->use_graph = !(ng && ng[0] == '1');
+    b->persist = persist_env();
     *out = b;
     return RC_OK;
 }
@@ -1733,6 +1764,7 @@ void rc_batch_destroy(rc_batch *b) {
     (void)hipSetDevice(b->device);
     for (rc_case *h : b->members) h->batch = nullptr;
     batch_release(b);
+    if (b->d_bar) (void)hipFree(b->d_bar);
     if (b->h_res) (void)hipHostFree(b->h_res);
     if (b->h_err) (void)hipHostFree(b->h_err);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1806,11 +1838,20 @@ int rc_batch_run_steady(rc_batch *b, int nsteps, int every, double *res_out, int
     // asked for, and every `every` steps; between them whole multi-step
     // graphs first, then single steps
     const bool first = res_out != nullptr;
+    bool visc = false;
+    for (const rc_case *m : b->members) visc = visc || m->viscous;
+    const bool persist = b->persist > 0 && !visc && persist_supported(b->L);
+    const int pgrid = persist ? persist_grid(b->L, b->device) : 0;
+    if (persist && pgrid <= 0) return bset_err(b, RC_ERR_DEVICE, "persistent step: no resident blocks");
+    if (persist && !b->d_bar) BHIPCHK(b, hipMalloc(&b->d_bar, 2 * sizeof(unsigned)));
     for (int s = 0; s < nsteps;) {
         const int next = std::min(nsteps, s == 0 && first ? 1 : (s / every + 1) * every);
         for (int k = next - s; k > 0;) {
             int run = 1;
-            if (b->use_graph) {
+            if (persist) {
+                run = std::min(k, b->persist);
+                BHIPCHK(b, launch_step_persist(b->L, b->d_bar, run, pgrid, b->stream));
+            } else if (b->use_graph) {
                 if ((rc = batch_graphs(b))) return rc;
                 // the largest graph that fits the rest (top, then its binary digits)
                 for (int p = b->graph_pow_top; p >= 2 && run == 1; p /= 2)

@@ -126,6 +126,26 @@ struct FlowList {
     int grid = 0;                  // thread blocks of the launch (0: no dataflow sweep)
 };
 
+// The persistent step: every phase of nsteps steps of a set of cases in one
+// cooperative launch, grid barriers between the phases.  The lists are the
+// step's own (cells and faces in items of at most 256, LU-SGS runs in items
+// of at most 16 per level); bar: [0] barrier arrivals, [1] abort.
+constexpr int kPersistThreads = 256;
+constexpr int kPersistMaxLevels = 32;
+struct PersistSet {
+    const CaseDev *cases;
+    int ncases;
+    const int4 *cell;
+    int ncell;
+    const int4 *face;
+    int nface;
+    const int4 *lev[kPersistMaxLevels];
+    int nlevitems[kPersistMaxLevels];
+    int nlev;
+    unsigned *bar;
+    int nsteps;
+};
+
 // everything one step launches over a set of cases
 struct LaunchSet {
     const CaseDev *cases = nullptr;
@@ -194,7 +214,7 @@ constexpr int kRunThreads = 64;       // threads per block of the LU-SGS level k
 constexpr int kRunChunk = kRunThreads / kRunMax;   // LU-SGS runs per thread block
 
 enum KernelId { K_TS_FACE = 1, K_TS_CELL, K_GHOST, K_GRADLIM, K_FLUX, K_RES, K_FWD, K_BWD, K_UPDATE, K_P2C,
-                K_BE_FWD, K_BE_BWD, K_BE_UPDATE, K_VISC, K_FLOW };
+                K_BE_FWD, K_BE_BWD, K_BE_UPDATE, K_VISC, K_FLOW, K_PERSIST };
 
 // first error wins; the step number comes from the device-side step
 // counter err[4] (advanced by k_step_begin), so captured graphs report it too

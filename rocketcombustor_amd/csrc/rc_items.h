@@ -47,6 +47,44 @@ constexpr int NV = kNvar;
     GThermo &TH = *gp(C.th);                                            \
     GParams &PAR = *gp(C.par);
 
+// the case and the item of this thread only (for kernels whose work is a
+// device function of (case, item)): Cp_ and idx, returning past the end
+#define ITEM_CASE(idx)                                                  \
+    const RC_G CaseDev *Cp_;                                            \
+    int idx;                                                            \
+    if constexpr (B) {                                                  \
+        const int4 w_ = work[blockIdx.x];                               \
+        Cp_ = gp(cases) + w_.x;                                         \
+        idx = w_.y + (int)threadIdx.x;                                  \
+        if (idx >= w_.z) return;                                        \
+    } else {                                                            \
+        Cp_ = gp(cases);                                                \
+        idx = lo + (int)(blockIdx.x * blockDim.x + threadIdx.x);        \
+        if (idx >= hi) return;                                          \
+    }
+// the same without returning (the block stages LDS first): end_ is the end
+#define ITEM_CASE_END(idx, end_)                                        \
+    const RC_G CaseDev *Cp_;                                            \
+    int idx, end_;                                                      \
+    if constexpr (B) {                                                  \
+        const int4 w_ = work[blockIdx.x];                               \
+        Cp_ = gp(cases) + w_.x;                                         \
+        idx = w_.y + (int)threadIdx.x;                                  \
+        end_ = w_.z;                                                    \
+    } else {                                                            \
+        Cp_ = gp(cases);                                                \
+        idx = lo + (int)(blockIdx.x * blockDim.x + threadIdx.x);        \
+        end_ = hi;                                                      \
+    }
+
+// the case's thermo table copied into LDS by every thread of the block (the
+// caller synchronises the block before reading it)
+__device__ __forceinline__ void stage_thermo(const RC_G CaseDev &C, DevThermo &dst) {
+    const RC_G double *src_ = (const RC_G double *)gp(C.th);
+    double *dst_ = reinterpret_cast<double *>(&dst);
+    for (int k_ = threadIdx.x; k_ < (int)(sizeof(DevThermo) / 8); k_ += blockDim.x) dst_[k_] = src_[k_];
+}
+
 // the same with the case's thermo table staged in LDS by the whole block
 // first (the thermo functions then read their coefficients with LDS
 // latency); TH refers to that copy

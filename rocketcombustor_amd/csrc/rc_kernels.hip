@@ -176,9 +176,13 @@ __global__ void __launch_bounds__(256) k_ghost(ITEM_ARGS) {
 // gradient by the Barth factor over the 3 face midpoints (LIM_ALWAYS), or
 // when the case's limit_ID > 0 (LIM_CASE), as reconstruction does.
 enum { LIM_NONE = 0, LIM_ALWAYS = 1, LIM_CASE = 2 };
-template <int NSP, bool GRAD, int LIM, bool B>
-__global__ void __launch_bounds__(256) k_gradlim(ITEM_ARGS) {
-    ITEM_PROLOGUE(i);
+// (the work of one cell i of case C; k_gradlim and the persistent step call it)
+template <int NSP, bool GRAD, int LIM>
+__device__ __forceinline__ void gradlim_cell(const RC_G CaseDev &C, int i) {
+    const GMesh M = gview(C.M);
+    const GState S = gview(C.S);
+    GThermo &TH = *gp(C.th);
+    GParams &PAR = *gp(C.par);
     const bool LIMIT = LIM == LIM_ALWAYS || (LIM == LIM_CASE && PAR.limit_id > 0);
     double pi[NV], pmin[NV], pmax[NV], g0[NV], g1[NV];
 #pragma unroll
@@ -241,6 +245,12 @@ __global__ void __launch_bounds__(256) k_gradlim(ITEM_ARGS) {
     // the record's unused slots too: whole 128-byte lines go to memory
 #pragma unroll
     for (int k = 2 * NV; k < kRecGR; ++k) S.gradc[(size_t)i * kRecGR + k] = 0.0;
+}
+
+template <int NSP, bool GRAD, int LIM, bool B>
+__global__ void __launch_bounds__(256) k_gradlim(ITEM_ARGS) {
+    ITEM_CASE(i);
+    gradlim_cell<NSP, GRAD, LIM>(*Cp_, i);
 }
 
 // flux_lax (flux_lax.f90:17-296), one thread per face: reconstructed L/R
@@ -359,33 +369,12 @@ __global__ void __launch_bounds__(256) k_flux(ITEM_ARGS) {
 // spectral radius from the cell averages.  Every value is formed by the
 // expressions of k_flux; a face's failed checks are reported once, by lane
 // 0, in k_flux's order (so the first error recorded is the same).
-template <int NSP, bool TS, bool B>
-__global__ void __launch_bounds__(512) k_flux2(ITEM_ARGS) {
-    __shared__ DevThermo th_lds_;
-    const RC_G CaseDev *Cp_;
-    int f, end_;
-    const int side = (int)threadIdx.x & 1;
-    if constexpr (B) {
-        const int4 w_ = work[blockIdx.x];
-        Cp_ = gp(cases) + w_.x;
-        f = w_.y + (int)(threadIdx.x >> 1);
-        end_ = w_.z;
-    } else {
-        Cp_ = gp(cases);
-        f = lo + (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 1);
-        end_ = hi;
-    }
-    const RC_G CaseDev &C = *Cp_;
-    {
-        const RC_G double *src_ = (const RC_G double *)gp(C.th);
-        double *dst_ = reinterpret_cast<double *>(&th_lds_);
-        for (int k_ = threadIdx.x; k_ < (int)(sizeof(DevThermo) / 8); k_ += blockDim.x) dst_[k_] = src_[k_];
-    }
-    __syncthreads();
-    if (f >= end_) return;      // both lanes of a face leave together
+// (the work of lane `side` of face f of case C, the thermo table staged in
+// LDS; k_flux2 and the persistent step call it)
+template <int NSP, bool TS>
+__device__ __forceinline__ void flux2_face(const RC_G CaseDev &C, int f, int side, const DevThermo &th) {
     const GMesh M = gview(C.M);
     const GState S = gview(C.S);
-    const DevThermo &th = th_lds_;
     GParams &P = *gp(C.par);
     const int nl = M.fnl[f], nr = M.fnr[f];
     const bool inner = nr >= 0;
@@ -545,6 +534,28 @@ __global__ void __launch_bounds__(512) k_flux2(ITEM_ARGS) {
         else if ((b0 & 16) || (inner && (b1 & 16))) flag_error(S.err, RC_ERR_NEGATIVE, f, K_FLUX);
         else if ((b0 & 32) || (b1 & 32)) flag_error(S.err, RC_ERR_TEMPERATURE, f, K_FLUX);
     }
+}
+
+template <int NSP, bool TS, bool B>
+__global__ void __launch_bounds__(512) k_flux2(ITEM_ARGS) {
+    __shared__ DevThermo th_lds_;
+    const RC_G CaseDev *Cp_;
+    int f, end_;
+    const int side = (int)threadIdx.x & 1;
+    if constexpr (B) {
+        const int4 w_ = work[blockIdx.x];
+        Cp_ = gp(cases) + w_.x;
+        f = w_.y + (int)(threadIdx.x >> 1);
+        end_ = w_.z;
+    } else {
+        Cp_ = gp(cases);
+        f = lo + (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 1);
+        end_ = hi;
+    }
+    stage_thermo(*Cp_, th_lds_);
+    __syncthreads();
+    if (f >= end_) return;      // both lanes of a face leave together
+    flux2_face<NSP, TS>(*Cp_, f, side, th_lds_);
 }
 
 // residual assembly of flux_sourceterm (fvm2d.f90:201-219) of cell i: h =
@@ -971,14 +982,148 @@ __global__ void __launch_bounds__(kRunThreads) __attribute__((amdgpu_waves_per_e
     }
 }
 
+// ---------------------------------------------------------------------------
+// The persistent step (opt-in): the phases of the fused step -- the step
+// counters, gradient + limiter, flux, each forward level (with its cells'
+// residual and source), each backward level, update -- for nsteps steps in
+// one cooperative launch, a grid barrier between consecutive phases.  Every
+// phase calls the device functions of the kernels above on the same items,
+// so the results are those of the step's kernels bit for bit (the residual
+// is formed in each forward level's own lanes).  The barrier: each block
+// writes its phase back (agent-scope release), counts itself in, waits for
+// the count of the whole grid (a bounded wait: past kBarSpin polls it raises
+// the abort flag and a device error, and every block leaves), then acquires
+// (the L2 write-back and invalidate of the release and acquire fences are
+// what makes one XCD's stores visible to the CUs of another).
+// A cooperative launch has every block of the grid resident at once, so
+// every wait ends.
+constexpr int kBarSpin = 1 << 22;
+
+__device__ bool persist_sync(RC_G unsigned *bar, unsigned &nbar, const RC_G CaseDev *cases) {
+    __shared__ int ok_;
+    // every wave's stores have reached the L2 before the block's write-back
+    __builtin_amdgcn_s_waitcnt(0x0f70);   // s_waitcnt vmcnt(0)
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ++nbar;
+        const unsigned target = nbar * gridDim.x;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int ok = 1;
+        for (int it = 0;; ++it) {
+            if (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+            if (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                ok = 0;
+                break;
+            }
+            if (it >= kBarSpin) {
+                __hip_atomic_store(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                flag_error(gp(cases[0].S.err), RC_ERR_DEVICE, 0, K_PERSIST);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ok_ = ok;
+    }
+    __syncthreads();
+    return ok_ != 0;
+}
+
+template <int NSP>
+__global__ void __launch_bounds__(kPersistThreads) k_step_persist(PersistSet a) {
+    __shared__ DevThermo th_lds_;
+    extern __shared__ double lusgs_tl[];
+    const int tid = (int)threadIdx.x;
+    const RC_G CaseDev *cases = gp(a.cases);
+    RC_G unsigned *bar = gp(a.bar);
+    unsigned nbar = 0;
+    for (int st = 0; st < a.nsteps; ++st) {
+        // the device step counters of the error reports (k_step_begin)
+        if (blockIdx.x == 0)
+            for (int c = tid; c < a.ncases; c += kPersistThreads) {
+                RC_G int *err = gp(cases[c].S.err);
+                err[4] = err[4] + 1;
+            }
+        if (!persist_sync(bar, nbar, cases)) return;
+        // gradient + limiter: one thread per cell
+        for (int it = blockIdx.x; it < a.ncell; it += gridDim.x) {
+            const int4 w = a.cell[it];
+            const int i = w.y + tid;
+            if (i < w.z) gradlim_cell<NSP, true, LIM_CASE>(cases[w.x], i);
+        }
+        if (!persist_sync(bar, nbar, cases)) return;
+        // flux: two lanes per face, 128 faces per pass over an item
+        int staged = -1;
+        for (int it = blockIdx.x; it < 2 * a.nface; it += gridDim.x) {
+            const int4 w = a.face[it >> 1];
+            if (w.x != staged) {
+                __syncthreads();
+                stage_thermo(cases[w.x], th_lds_);
+                __syncthreads();
+                staged = w.x;
+            }
+            const int f = w.y + (it & 1) * (kPersistThreads / 2) + (tid >> 1);
+            if (f < w.z) flux2_face<NSP, true>(cases[w.x], f, tid & 1, th_lds_);
+        }
+        if (!persist_sync(bar, nbar, cases)) return;
+        // the forward levels, each item of runs to one wave (16 runs a pass)
+        constexpr int per = kPersistThreads / kRunThreads;
+        for (int l = 0; l < a.nlev; ++l) {
+            const int n = a.nlevitems[l];
+            for (int it = blockIdx.x; it * per < n; it += gridDim.x) {
+                const int u = it * per + tid / kRunThreads;
+                if (u < n) {
+                    const int4 w = a.lev[l][u];
+                    for (int t0 = w.y; t0 < w.z; t0 += kRunChunk)
+                        lusgs_runs<NSP, true, true, false>(cases[w.x], t0 + (tid % kRunThreads) / kRunMax, w.z, 0,
+                                                           lusgs_tl);
+                }
+            }
+            if (!persist_sync(bar, nbar, cases)) return;
+        }
+        // the backward levels
+        for (int l = a.nlev - 1; l >= 0; --l) {
+            const int n = a.nlevitems[l];
+            for (int it = blockIdx.x; it * per < n; it += gridDim.x) {
+                const int u = it * per + tid / kRunThreads;
+                if (u < n) {
+                    const int4 w = a.lev[l][u];
+                    for (int t0 = w.y; t0 < w.z; t0 += kRunChunk)
+                        lusgs_runs<NSP, false, false, false>(cases[w.x], t0 + (tid % kRunThreads) / kRunMax, w.z, 0,
+                                                             lusgs_tl);
+                }
+            }
+            if (!persist_sync(bar, nbar, cases)) return;
+        }
+        // update: one thread per cell
+        staged = -1;
+        for (int it = blockIdx.x; it < a.ncell; it += gridDim.x) {
+            const int4 w = a.cell[it];
+            if (w.x != staged) {
+                __syncthreads();
+                stage_thermo(cases[w.x], th_lds_);
+                __syncthreads();
+                staged = w.x;
+            }
+            const int i = w.y + tid;
+            if (i < w.z) update_cell<NSP>(cases[w.x], i, th_lds_);
+        }
+        if (st + 1 < a.nsteps && !persist_sync(bar, nbar, cases)) return;
+    }
+}
+
 // WA += dW (Time_LUSGS_steady.f90:193-197) and update_primitve_variables
 // (update_flow_variables.f90:4-48), plus the NaN check of lu_sgs_stage;
 // the new cell-average quantities go to the cache
 // (four waves per SIMD: 127 VGPRs, no spill; three at the compiler's own choice)
-template <int NSP, bool B>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_update(ITEM_ARGS) {
-    ITEM_PROLOGUE_LDS(i);
-    const DevThermo &th = TH;
+// (the work of cell i of case C, the thermo table staged in LDS; k_update
+// and the persistent step call it)
+template <int NSP>
+__device__ __forceinline__ void update_cell(const RC_G CaseDev &C, int i, const DevThermo &th) {
+    const GMesh M = gview(C.M);
+    const GState S = gview(C.S);
     double w[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) { w[k] = WA_(k, i) + HWA_(k, i); WA_(k, i) = w[k]; }
@@ -1003,6 +1148,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     ok = write_lucoef<NSP>(M, S, th, i, p, u, v, t, y) && ok;
     if (!ok) flag_error(S.err, RC_ERR_TEMPERATURE, i, K_UPDATE);
     if (i == 0 && isnan(w[0])) flag_error(S.err, RC_ERR_NAN, 0, K_UPDATE);
+}
+
+template <int NSP, bool B>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_update(ITEM_ARGS) {
+    __shared__ DevThermo th_lds_;
+    ITEM_CASE_END(i, end_);
+    stage_thermo(*Cp_, th_lds_);
+    __syncthreads();
+    if (i >= end_) return;
+    update_cell<NSP>(*Cp_, i, th_lds_);
 }
 
 // primitve_to_conserved_variables (update_flow_variables.f90:50-84), and
@@ -1286,6 +1441,77 @@ __global__ void k_gather_err(const CaseDev *__restrict__ cases, int n, int *__re
 
 void launch_gather_err(const LaunchSet &L, int *out, hipStream_t st) {
     if (L.ncases > 0) k_gather_err<<<(L.ncases + 255) / 256, 256, 0, st>>>(L.cases, L.ncases, out);
+}
+
+// ---------------------------------------------------------------------------
+// the persistent step on the host side
+// ---------------------------------------------------------------------------
+constexpr size_t persist_lds() { return (size_t)kPersistThreads * 3 * 8 * sizeof(double); }
+
+// whether a launch set can run as the persistent step: at most
+// kPersistMaxLevels levels, cell and face items of at most kPersistThreads
+// (a wave sweeps a level item kRunChunk runs at a time, whatever its size)
+bool persist_supported(const LaunchSet &L) {
+    if (L.level.size() > (size_t)kPersistMaxLevels || L.cell.bs > kPersistThreads ||
+        L.face.bs > kPersistThreads || !L.cell.w || !L.face.w)
+        return false;
+    for (const WorkList &w : L.level)
+        if (w.n > 0 && !w.w) return false;
+    return true;
+}
+
+// blocks of the persistent step resident at once on the device
+int persist_max_blocks(int device) {
+    static std::mutex mu;
+    static std::map<int, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(device);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_step_persist<kNsp>, kPersistThreads, persist_lds()) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0;
+    cache[device] = per_cu * cus;
+    return per_cu * cus;
+}
+
+// blocks a launch set's persistent step uses: as many as its widest phase
+// can keep busy, no more than are resident at once (RC_PERSIST_GRID=n caps it)
+int persist_grid(const LaunchSet &L, int device) {
+    int need = std::max(L.cell.n, 2 * L.face.n);
+    for (const WorkList &w : L.level) need = std::max(need, (w.n + kPersistThreads / kRunThreads - 1) /
+                                                                    (kPersistThreads / kRunThreads));
+    int g = std::min(std::max(need, 1), persist_max_blocks(device));
+    if (const char *e = std::getenv("RC_PERSIST_GRID")) {
+        const int cap = std::atoi(e);
+        if (cap > 0) g = std::min(g, cap);
+    }
+    return g;
+}
+
+// nsteps steps of the launch set as one cooperative launch; bar: two
+// device words, cleared here
+hipError_t launch_step_persist(const LaunchSet &L, unsigned *bar, int nsteps, int grid, hipStream_t st) {
+    PersistSet a{};
+    a.cases = L.cases;
+    a.ncases = L.ncases;
+    a.cell = L.cell.w;
+    a.ncell = L.cell.n;
+    a.face = L.face.w;
+    a.nface = L.face.n;
+    a.nlev = (int)L.level.size();
+    for (int l = 0; l < a.nlev; ++l) {
+        a.lev[l] = L.level[l].w;
+        a.nlevitems[l] = L.level[l].n;
+    }
+    a.bar = bar;
+    a.nsteps = nsteps;
+    hipError_t e = hipMemsetAsync(bar, 0, 2 * sizeof(unsigned), st);
+    if (e != hipSuccess) return e;
+    void *args[] = {&a};
+    return hipLaunchCooperativeKernel((const void *)k_step_persist<kNsp>, dim3(grid), dim3(kPersistThreads), args,
+                                      persist_lds(), st);
 }
 
 void launch_resnorm(const LaunchSet &L, hipStream_t st) {
