@@ -138,7 +138,10 @@ int rc_lusgs_stage(rc_case *h);
  * asynchronously between reports; the device error flags are checked at
  * each report and at the end.  `res_out` may be NULL (then there is no
  * report at step 1, only every `every` steps).  Returns the number of
- * completed steps in *done. */
+ * completed steps in *done.  The steps replay captured HIP graphs of up to
+ * 8 steps (RC_GRAPH_STEPS sets the largest, RC_NO_GRAPH=1 launches every
+ * kernel); with RC_PERSIST=n set when the case is created, up to n steps
+ * run as one cooperative launch instead (same results, measured slower). */
 int rc_run_steady(rc_case *h, int nsteps, int every, double *res_out, int *done);
 
 /* ---- viscous path (IfViscous, startup.f90:76).
