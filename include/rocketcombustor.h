@@ -98,6 +98,25 @@ int rc_case_create(const rc_mechanism *mech, const rc_params *par,
                    int nnt, const double *coor, int ne, const int *cellnodes,
                    int nb, const int *bnodes, const int *btype,
                    int device, rc_case **out);
+/* rc_case_create for grids with periodic boundaries: replaces, in addition,
+ * the periodic part of the grid reader (mesh.f90:60-64: NCon lines
+ * Con(1:2), dxyz(1:2)) and CreatePeriodConnect (period.f90:15-104).
+ * btype values >= 100 are labels of the boundaries the connections name
+ * (their faces in the grid's order); connection c takes every face of
+ * boundary con[2c] to the face of boundary con[2c+1] whose midpoint is its
+ * own moved by (dxyz[2c], dxyz[2c+1]), matched as period.f90:76-87 matches
+ * them.  Such a face gets the owner of its partner as neighbour,
+ * FProperty 100 and SideOfs dxyz: flux_lax and flux_viscous update its
+ * owner only (flux_lax.f90:289-293), the reconstruction and both fluxes
+ * move the partner by SideOfs (reconstruction_initialization.f90:40,
+ * flux_lax.f90:86), and TimeStep adds its alpha to both cells.  A face of a
+ * labelled boundary left without a partner is refused (flux_lax stops on
+ * type 100). */
+int rc_case_create_periodic(const rc_mechanism *mech, const rc_params *par,
+                            int nnt, const double *coor, int ne, const int *cellnodes,
+                            int nb, const int *bnodes, const int *btype,
+                            int ncon, const int *con, const double *dxyz,
+                            int device, rc_case **out);
 void rc_case_destroy(rc_case *h);
 const char *rc_last_error(const rc_case *h);           /* thread-local message when h is NULL */
 

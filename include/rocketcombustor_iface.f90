@@ -57,6 +57,18 @@ module rocketcombustor_iface
             type(c_ptr), intent(out) :: handle
         end function
 
+        integer(c_int) function rc_case_create_periodic(mech, par, nnt, coor, ne, cellnodes, nb, bnodes, btype, &
+                                                        ncon, con, dxyz, device, handle) &
+                                                        bind(c, name="rc_case_create_periodic")
+            import :: c_int, c_double, c_ptr, rc_mechanism, rc_params
+            type(rc_mechanism), intent(in) :: mech
+            type(rc_params), intent(in) :: par
+            integer(c_int), value :: nnt, ne, nb, ncon, device
+            real(c_double), intent(in) :: coor(*), dxyz(*)
+            integer(c_int), intent(in) :: cellnodes(*), bnodes(*), btype(*), con(*)
+            type(c_ptr), intent(out) :: handle
+        end function
+
         subroutine rc_case_destroy(h) bind(c, name="rc_case_destroy")
             import :: c_ptr
             type(c_ptr), value :: h

@@ -59,6 +59,9 @@ def lib():
         P = C.POINTER
         L.rco_create.argtypes = [P(RcoMech), P(RcoParams), C.c_int, P(C.c_double), C.c_int, P(C.c_int),
                                  C.c_int, P(C.c_int), P(C.c_int), P(C.c_void_p)]
+        L.rco_create_periodic.argtypes = [P(RcoMech), P(RcoParams), C.c_int, P(C.c_double), C.c_int, P(C.c_int),
+                                          C.c_int, P(C.c_int), P(C.c_int), C.c_int, P(C.c_int), P(C.c_double),
+                                          P(C.c_void_p)]
         L.rco_destroy.argtypes = [C.c_void_p]
         L.rco_last_error.argtypes = [C.c_void_p]
         L.rco_last_error.restype = C.c_char_p
@@ -151,8 +154,11 @@ class OracleCase:
         be = np.ascontiguousarray(m.bedges, dtype=np.int32)
         bt = np.ascontiguousarray(m.btype, dtype=np.int32)
         h = C.c_void_p()
-        rc = L.rco_create(C.byref(self._mech), C.byref(self._par), len(coor), _dp(coor), len(cells), _ip(cells),
-                          len(be), _ip(be), _ip(bt), C.byref(h))
+        periodic = getattr(m, "periodic", ())
+        con = np.ascontiguousarray([[p[0], p[1]] for p in periodic], dtype=np.int32).reshape(-1, 2)
+        dxyz = np.ascontiguousarray([[p[2], p[3]] for p in periodic], dtype=np.float64).reshape(-1, 2)
+        rc = L.rco_create_periodic(C.byref(self._mech), C.byref(self._par), len(coor), _dp(coor), len(cells),
+                                   _ip(cells), len(be), _ip(be), _ip(bt), len(con), _ip(con), _dp(dxyz), C.byref(h))
         self.h = h
         if rc:
             raise RuntimeError("oracle set-up failed: " + L.rco_last_error(h).decode())

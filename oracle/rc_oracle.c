@@ -42,6 +42,9 @@ struct rco_case {
     double *wp, *wp2;              /* WP, WP2 (Nvar,NE): the states at the start of the last two BDF2 steps */
     rco_transport tr;              /* transport / diffusion tables */
     int viscous;                   /* IfViscous */
+    int *fprop;                    /* FProperty(NF): 100 on periodic faces */
+    double *sideofs;               /* SideOfs(2,NF) */
+    double *ofs;                   /* Ofs(2,0:3,NE) */
     char err[256];
 };
 
@@ -64,6 +67,8 @@ struct rco_case {
 #define GHOSTCELL(k, i) c->ghostcell[((i) - 1) * 3 + (k) - 1]
 #define CGN(d, v, j) c->coor_gn[(((j) - 1) * 4 + (v) - 1) * 2 + (d) - 1]
 #define CXYG(d, j)  c->cellxy_g[((j) - 1) * 2 + (d) - 1]
+#define SIDEOFS(d, f) c->sideofs[((f) - 1) * 2 + (d) - 1]
+#define OFS(d, k, i) c->ofs[(((i) - 1) * 4 + (k)) * 2 + (d) - 1]
 
 static int fail(rco_case *c, const char *fmt, ...)
 {
@@ -274,6 +279,13 @@ static void *xcalloc(size_t n, size_t s) { void *p = calloc(n ? n : 1, s); if (!
 int rco_create(const rco_mech *mech, const rco_params *par, int nnt, const double *coor,
                int ne, const int *cells, int nb, const int *bedges, const int *btype, rco_case **out)
 {
+    return rco_create_periodic(mech, par, nnt, coor, ne, cells, nb, bedges, btype, 0, NULL, NULL, out);
+}
+
+int rco_create_periodic(const rco_mech *mech, const rco_params *par, int nnt, const double *coor,
+                        int ne, const int *cells, int nb, const int *bedges, const int *btype,
+                        int ncon, const int *con, const double *dxyz, rco_case **out)
+{
     rco_case *c = (rco_case *)xcalloc(1, sizeof *c);
     *out = c;
     c->m = *mech;
@@ -320,16 +332,65 @@ int rco_create(const rco_mech *mech, const rco_params *par, int nnt, const doubl
             }
         }
     }
-    free(numnf); free(nodeface);
     c->nf = nf;
 
-    /* BC tagging, mesh.f90:76-93 */
+    /* CreatePeriodConnect, period.f90:15-104: boundary faces by node pair
+     * (BoundF, :46-62), then each face of boundary Con(1) takes the owner of
+     * the first face of boundary Con(2) whose midpoint (single precision)
+     * is its own moved by dxyz (:66-101).  Boundaries are the labels btype
+     * >= 100 here; the connections name them (mesh.f90:60-64). */
+    c->fprop = (int *)xcalloc(nf + 1, sizeof(int));
+    c->sideofs = (double *)xcalloc(2 * (size_t)(nf + 1), sizeof(double));
+    int icon, b, b2;
+    for (icon = 0; icon < ncon; icon++) {
+        for (b = 0; b < nb; b++) {
+            if (btype[b] != con[2 * icon]) continue;
+            int n1 = bedges[2 * b] + 1, n2 = bedges[2 * b + 1] + 1, iface = 0, ic1;
+            for (ic1 = 1; ic1 <= numnf[n1]; ic1++) {
+                int f = nodeface[n1 * 50 + ic1 - 1];
+                if (FNODE(1, f) == n2 || FNODE(2, f) == n2) { iface = f; break; }
+            }
+            if (!iface) { free(numnf); free(nodeface); return fail(c, "not find boundary face %d", b + 1); }
+            float xfs[2], xft[2], ddd[2];
+            xfs[0] = (float)((COOR(1, n1) + COOR(1, n2)) / 2.);
+            xfs[1] = (float)((COOR(2, n1) + COOR(2, n2)) / 2.);
+            int itface = 0;
+            for (b2 = 0; b2 < nb; b2++) {
+                if (btype[b2] != con[2 * icon + 1]) continue;
+                int m1 = bedges[2 * b2] + 1, m2 = bedges[2 * b2 + 1] + 1;
+                xft[0] = (float)((COOR(1, m1) + COOR(1, m2)) / 2.);
+                xft[1] = (float)((COOR(2, m1) + COOR(2, m2)) / 2.);
+                ddd[0] = xft[0] - xfs[0];
+                ddd[1] = xft[1] - xfs[1];
+                if (fabs(ddd[0] - dxyz[2 * icon]) + fabs(ddd[1] - dxyz[2 * icon + 1]) < 1.e-7f) {
+                    for (ic1 = 1; ic1 <= numnf[m1]; ic1++) {
+                        int f = nodeface[m1 * 50 + ic1 - 1];
+                        if (FNODE(1, f) == m2 || FNODE(2, f) == m2) { itface = f; break; }
+                    }
+                    break;
+                }
+            }
+            if (!itface) { free(numnf); free(nodeface); return fail(c, "not find target (boundary face %d)", b + 1); }
+            NEIGHBOR(2, iface) = NEIGHBOR(1, itface);
+            c->fprop[iface] = 100;
+            SIDEOFS(1, iface) = dxyz[2 * icon];
+            SIDEOFS(2, iface) = dxyz[2 * icon + 1];
+        }
+    }
+    free(numnf); free(nodeface);
+
+    /* BC tagging, mesh.f90:76-93 (a connection's boundaries have type 100) */
     for (i = 1; i <= nf; i++) {
         if (NEIGHBOR(2, i) < 0) {
-            int n1 = FNODE(1, i), n2 = FNODE(2, i), found = 0, b;
+            int n1 = FNODE(1, i), n2 = FNODE(2, i), found = 0;
             for (b = 0; b < nb; b++) {
                 int a1 = bedges[2 * b] + 1, a2 = bedges[2 * b + 1] + 1;
-                if ((n1 == a1 && n2 == a2) || (n2 == a1 && n1 == a2)) { NEIGHBOR(2, i) = -btype[b]; found = 1; break; }
+                if ((n1 == a1 && n2 == a2) || (n2 == a1 && n1 == a2)) {
+                    int t = btype[b];
+                    for (icon = 0; icon < ncon; icon++)
+                        if (t == con[2 * icon] || t == con[2 * icon + 1]) t = 100;
+                    NEIGHBOR(2, i) = -t; found = 1; break;
+                }
             }
             if (!found) return fail(c, "not find boundary of this type (face %d)", i);
         }
@@ -365,15 +426,21 @@ int rco_create(const rco_mech *mech, const rco_params *par, int nnt, const doubl
     for (i = 1; i <= nf; i++) {
         int NL = NEIGHBOR(1, i), NR = NEIGHBOR(2, i);
         ntemp[NL]++; NEIGHF(ntemp[NL], NL) = i;
-        if (NR > 0) { ntemp[NR]++; NEIGHF(ntemp[NR], NR) = i; }
+        if (NR > 0 && c->fprop[i] != 100) { ntemp[NR]++; NEIGHF(ntemp[NR], NR) = i; }
     }
     free(ntemp);
+    c->ofs = (double *)xcalloc(8 * (size_t)ne, sizeof(double));
     for (i = 1; i <= ne; i++) {
         for (k = 1; k <= 3; k++) {
             int iface = NEIGHF(k, i);
             for (kf = 1; kf <= 2; kf++) {
                 int ic = NEIGHBOR(kf, iface);
-                if (ic != i && ic > 0) { c->neigh[i - 1]++; NEIGHC(c->neigh[i - 1], i) = ic; }
+                if (ic != i && ic > 0) {
+                    c->neigh[i - 1]++;
+                    NEIGHC(c->neigh[i - 1], i) = ic;
+                    OFS(1, c->neigh[i - 1], i) = SIDEOFS(1, iface);   /* :40 */
+                    OFS(2, c->neigh[i - 1], i) = SIDEOFS(2, iface);
+                }
             }
         }
         NEIGHC(0, i) = i;
@@ -449,8 +516,15 @@ int rco_create(const rco_mech *mech, const rco_params *par, int nnt, const doubl
         int nn = c->neigh[i - 1];
         for (k = 1; k <= nn; k++) {
             j = NEIGHC(k, i);
-            const double *x1 = &COOR(1, NODE(1, j)), *x2 = &COOR(1, NODE(2, j)), *x3 = &COOR(1, NODE(3, j)), *x4 = &COOR(1, NODE(4, j));
-            double dx = (CELLXY(1, j) - 0.0) - CELLXY(1, i), dy = (CELLXY(2, j) - 0.0) - CELLXY(2, i);
+            /* the neighbour's nodes moved by its offset (:101-109) */
+            double xv[4][2];
+            int v;
+            for (v = 0; v < 4; v++) {
+                xv[v][0] = COOR(1, NODE(v + 1, j)) - OFS(1, k, i);
+                xv[v][1] = COOR(2, NODE(v + 1, j)) - OFS(2, k, i);
+            }
+            const double *x1 = xv[0], *x2 = xv[1], *x3 = xv[2], *x4 = xv[3];
+            double dx = (CELLXY(1, j) - OFS(1, k, i)) - CELLXY(1, i), dy = (CELLXY(2, j) - OFS(2, k, i)) - CELLXY(2, i);
             double dist = sqrt(dx * dx + dy * dy);
             WDIS(k, i) = powi(dist, -1);
             int io;
@@ -494,7 +568,7 @@ int rco_create(const rco_mech *mech, const rco_params *par, int nnt, const doubl
 void rco_destroy(rco_case *c)
 {
     if (!c) return;
-    void *ps[] = {c->coor, c->n4, c->fnode, c->neighbor, c->vecx, c->vol, c->cellxy, c->reflen,
+    void *ps[] = {c->fprop, c->sideofs, c->ofs, c->coor, c->n4, c->fnode, c->neighbor, c->vecx, c->vol, c->cellxy, c->reflen,
                   c->neighf, c->neighc, c->neigh, c->hostcell, c->hostface, c->bctype_g,
                   c->ghostcell, c->num_ghost, c->coor_gn, c->vol_g, c->cellxy_g, c->wdis, c->amls,
                   c->xyh, c->pa, c->wa, c->hwa, c->gradc, c->gpa, c->alpaf, c->step, c->prod, c->dd, c->fixed,
@@ -865,8 +939,8 @@ static int flux_viscous(rco_case *c)
         double rl = pl / (gl[2] * tl);
         double pr, ur, vr, tr, rr;
         if (NR > 0) {
-            double w0 = (xy0 - (CELLXY(1, NR) - 0.0)) / REFLEN(1, NR);
-            double w1 = (xy1 - (CELLXY(2, NR) - 0.0)) / REFLEN(2, NR);
+            double w0 = (xy0 - (CELLXY(1, NR) - SIDEOFS(1, i))) / REFLEN(1, NR);
+            double w1 = (xy1 - (CELLXY(2, NR) - SIDEOFS(2, i))) / REFLEN(2, NR);
             for (k = 1; k <= nvar; k++) u_r[k - 1] = PA(k, NR) + (GRADC(1, k, NR) * w0 + GRADC(2, k, NR) * w1);
             pr = u_r[0]; ur = u_r[1]; vr = u_r[2]; tr = u_r[3];
             s = 0.0;
@@ -928,8 +1002,8 @@ static int flux_viscous(rco_case *c)
             tgd[1][k - 1] = GRADC(1, k, NL) * 0. + GRADC(2, k, NL) * iy;
         }
         if (NR > 0) {
-            double dlrx = CELLXY(1, NL) - (CELLXY(1, NR) - 0.0);
-            double dlry = CELLXY(2, NL) - (CELLXY(2, NR) - 0.0);
+            double dlrx = CELLXY(1, NL) - (CELLXY(1, NR) - SIDEOFS(1, i));
+            double dlry = CELLXY(2, NL) - (CELLXY(2, NR) - SIDEOFS(2, i));
             double dxk = fabs(dlrx * s1n + dlry * s2n);
             double jx = 1. / REFLEN(1, NR), jy = 1. / REFLEN(2, NR);
             for (k = 1; k <= nvar; k++) {
@@ -997,7 +1071,7 @@ static int flux_viscous(rco_case *c)
         }
         for (k = 1; k <= nvar; k++) {
             HWA(k, NL) = HWA(k, NL) - flu[k - 1] * 1.0;
-            if (NR > 0) HWA(k, NR) = HWA(k, NR) + flu[k - 1] * 1.0;
+            if (NR > 0 && c->fprop[i] != 100) HWA(k, NR) = HWA(k, NR) + flu[k - 1] * 1.0;
         }
     }
     return 0;
@@ -1039,8 +1113,8 @@ int rco_flux_sourceterm(rco_case *c)
         double pr, ur, vr, tr, rr, vn;
         int bad = 0;
         if (NR > 0) {
-            double w0 = (xy0 - (CELLXY(1, NR) - 0.0)) / REFLEN(1, NR);
-            double w1 = (xy1 - (CELLXY(2, NR) - 0.0)) / REFLEN(2, NR);
+            double w0 = (xy0 - (CELLXY(1, NR) - SIDEOFS(1, i))) / REFLEN(1, NR);
+            double w1 = (xy1 - (CELLXY(2, NR) - SIDEOFS(2, i))) / REFLEN(2, NR);
             pr = PA(1, NR) + (GRADC(1, 1, NR) * w0 + GRADC(2, 1, NR) * w1);
             ur = PA(2, NR) + (GRADC(1, 2, NR) * w0 + GRADC(2, 2, NR) * w1);
             vr = PA(3, NR) + (GRADC(1, 3, NR) * w0 + GRADC(2, 3, NR) * w1);
@@ -1109,7 +1183,7 @@ int rco_flux_sourceterm(rco_case *c)
         for (k = 0; k < ns1; k++) flu[NFLOW + k] = 0.5 * (ruln * yl[k] + rurn * yr[k] - coex * (rr * yr[k] - rl * yl[k]));
         for (k = 1; k <= nvar; k++) {
             HWA(k, NL) = HWA(k, NL) - flu[k - 1] * 1.0;
-            if (NR > 0) HWA(k, NR) = HWA(k, NR) + flu[k - 1] * 1.0;
+            if (NR > 0 && c->fprop[i] != 100) HWA(k, NR) = HWA(k, NR) + flu[k - 1] * 1.0;
         }
     }
     if (c->viscous && flux_viscous(c)) return 1;

@@ -51,6 +51,12 @@ typedef struct rco_case rco_case;
 int  rco_create(const rco_mech *mech, const rco_params *par, int nnt, const double *coor,
                 int ne, const int *cells, int nb, const int *bedges, const int *btype,
                 rco_case **out);
+/* with periodic connections (mesh.f90:60-64, period.f90:15-104): btype
+ * labels >= 100 name the boundaries; connection c joins boundary con[2c]
+ * to con[2c+1] with translation dxyz[2c..2c+1] */
+int  rco_create_periodic(const rco_mech *mech, const rco_params *par, int nnt, const double *coor,
+                         int ne, const int *cells, int nb, const int *bedges, const int *btype,
+                         int ncon, const int *con, const double *dxyz, rco_case **out);
 void rco_destroy(rco_case *c);
 const char *rco_last_error(const rco_case *c);
 void rco_sizes(const rco_case *c, int *out4);            /* ne, nf, nghost, nvar */

@@ -240,8 +240,9 @@ DevThermo make_thermo(const rc_mechanism &m) {
     return t;
 }
 
-DevParams make_params(const rc_params &p, int nsp) {
+DevParams make_params(const rc_params &p, int nsp, bool periodic) {
     DevParams d{};
+    d.periodic = periodic ? 1 : 0;
     d.cfl = p.cfl;
     d.p_out = p.p_out;
     for (int k = 0; k < nsp; ++k) {
@@ -505,6 +506,7 @@ int layout_and_upload(rc_case *h) {
         M.lvv = c.take<double>((size_t)6 * sc);
         M.stencil = c.take<int>((size_t)3 * sc);
         M.cface = c.take<int>((size_t)3 * sc);
+        M.tsx = m.tsx.empty() ? nullptr : c.take<int>((size_t)kTsFaces * sc);
         M.runs = c.take<int>(m.runs.size());
         M.slot = c.take<int>((size_t)SL_N * ss);
         M.slot_n = c.take<double>((size_t)6 * ss);
@@ -567,6 +569,10 @@ int layout_and_upload(rc_case *h) {
     if ((rc = up(M.stencil, stn.data(), sizeof(int) * stn.size()))) return rc;
     auto cf = soa(m.cface, m.ne, 3, sc);
     if ((rc = up(M.cface, cf.data(), sizeof(int) * cf.size()))) return rc;
+    if (M.tsx) {
+        auto tx = soa(m.tsx, m.ne, kTsFaces, sc);
+        if ((rc = up(M.tsx, tx.data(), sizeof(int) * tx.size()))) return rc;
+    }
     if ((rc = up(M.runs, m.runs.data(), sizeof(int) * m.runs.size()))) return rc;
     {
         // lane slots of the sweep kernels (lane p of run t: slot 4t+p)
@@ -607,7 +613,7 @@ int layout_and_upload(rc_case *h) {
         if ((rc = up(M.ghost_type, m.ghost_type.data(), sizeof(int) * m.ng))) return rc;
     }
     DevThermo th = make_thermo(h->mech);
-    DevParams dp = make_params(h->par, h->mech.nsp);
+    DevParams dp = make_params(h->par, h->mech.nsp, !h->hm.tsx.empty());
     if ((rc = up(h->d_th, &th, sizeof th))) return rc;
     if ((rc = up(h->d_par, &dp, sizeof dp))) return rc;
     CaseDev &hc = h->hc;
@@ -1172,9 +1178,17 @@ const char *rc_last_error(const rc_case *h) { return h ? h->err.c_str() : g_erro
 
 int rc_case_create(const rc_mechanism *mech, const rc_params *par, int nnt, const double *coor, int ne,
                    const int *cellnodes, int nb, const int *bnodes, const int *btype, int device, rc_case **out) {
+    return rc_case_create_periodic(mech, par, nnt, coor, ne, cellnodes, nb, bnodes, btype, 0, nullptr, nullptr,
+                                   device, out);
+}
+
+int rc_case_create_periodic(const rc_mechanism *mech, const rc_params *par, int nnt, const double *coor, int ne,
+                            const int *cellnodes, int nb, const int *bnodes, const int *btype, int ncon,
+                            const int *con, const double *dxyz, int device, rc_case **out) {
     if (!out) return set_err(nullptr, RC_ERR_ARGUMENT, "out is NULL");
     *out = nullptr;
-    if (!mech || !par || !coor || !cellnodes || (nb > 0 && (!bnodes || !btype)))
+    if (!mech || !par || !coor || !cellnodes || (nb > 0 && (!bnodes || !btype)) || ncon < 0 ||
+        (ncon > 0 && (!con || !dxyz)))
         return set_err(nullptr, RC_ERR_ARGUMENT, "NULL argument");
     if (mech->nsp != kNsp)
         return set_err(nullptr, RC_ERR_ARGUMENT, "the kernels are built for 4 species (Nvar = 7)");
@@ -1188,7 +1202,7 @@ int rc_case_create(const rc_mechanism *mech, const rc_params *par, int nnt, cons
     for (size_t k = 0; k < c0.size(); ++k) c0[k] = cellnodes[k] - 1;
     for (size_t k = 0; k < b0.size(); ++k) b0[k] = bnodes[k] - 1;
     std::string msg;
-    int rc = build_mesh(nnt, coor, ne, c0.data(), nb, b0.data(), btype, h->hm, msg);
+    int rc = build_mesh(nnt, coor, ne, c0.data(), nb, b0.data(), btype, ncon, con, dxyz, h->hm, msg);
     if (rc) {
         delete h;
         return set_err(nullptr, rc, msg);
@@ -1384,7 +1398,7 @@ int rc_set_transport(rc_case *h, const rc_transport *t) {
     HIPCHK(h, hipMemcpyAsync(h->d_tr, &d, sizeof d, hipMemcpyHostToDevice, h->stream));
     h->hc.tr = h->d_tr;
     HIPCHK(h, hipMemcpyAsync(h->d_case, &h->hc, sizeof h->hc, hipMemcpyHostToDevice, h->stream));
-    DevParams dp = make_params(h->par, h->mech.nsp);
+    DevParams dp = make_params(h->par, h->mech.nsp, !h->hm.tsx.empty());
     dp.viscous = 1;
     dp.surrogate = h->sur ? 1 : 0;
     HIPCHK(h, hipMemcpyAsync(h->d_par, &dp, sizeof dp, hipMemcpyHostToDevice, h->stream));
@@ -1417,7 +1431,7 @@ int rc_set_surrogate(rc_case *h, rc_mlp *m, const double *xform, double tol, dou
     h->hc.S.src = m ? h->sur_src : nullptr;
     h->S.src = h->hc.S.src;
     HIPCHK(h, hipMemcpyAsync(h->d_case, &h->hc, sizeof h->hc, hipMemcpyHostToDevice, h->stream));
-    DevParams dp = make_params(h->par, h->mech.nsp);
+    DevParams dp = make_params(h->par, h->mech.nsp, !h->hm.tsx.empty());
     dp.viscous = h->viscous ? 1 : 0;
     dp.surrogate = m ? 1 : 0;
     HIPCHK(h, hipMemcpyAsync(h->d_par, &dp, sizeof dp, hipMemcpyHostToDevice, h->stream));

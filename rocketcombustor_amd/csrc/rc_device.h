@@ -21,6 +21,7 @@ struct DevMesh {
     const int *__restrict__ fnr;
     const int *__restrict__ stencil;
     const int *__restrict__ cface;
+    const int *__restrict__ tsx;         // periodic meshes: faces of the TimeStep sum, tsx[k*sc+i] (-1 pad)
     const int *__restrict__ ghost_host;
     const int *__restrict__ ghost_type;
     const int *__restrict__ runs;        // LU-SGS runs (first cell, count), by level
@@ -168,7 +169,7 @@ struct LaunchSet {
 struct GMesh {
     int ne, nf, ng;
     int sc, sf, sg, ss;
-    const RC_G int *fnl, *fnr, *stencil, *cface, *ghost_host, *ghost_type, *runs, *lu_nb, *pi_cells;
+    const RC_G int *fnl, *fnr, *stencil, *cface, *tsx, *ghost_host, *ghost_type, *runs, *lu_nb, *pi_cells;
     const RC_G double *vecx, *fvl, *fvr, *vol, *wdis, *amls, *lvv, *gnrm, *lu_n, *vgeo;
     const RC_G int *slot;
     const RC_G double *slot_n;
@@ -189,7 +190,7 @@ __device__ __forceinline__ GMesh gview(const RC_G DevMesh &m) {
     GMesh g;
     g.ne = m.ne; g.nf = m.nf; g.ng = m.ng; g.sc = m.sc; g.sf = m.sf; g.sg = m.sg; g.ss = m.ss;
     g.slot = gp(m.slot); g.slot_n = gp(m.slot_n);
-    g.fnl = gp(m.fnl); g.fnr = gp(m.fnr); g.stencil = gp(m.stencil); g.cface = gp(m.cface);
+    g.fnl = gp(m.fnl); g.fnr = gp(m.fnr); g.stencil = gp(m.stencil); g.cface = gp(m.cface); g.tsx = gp(m.tsx);
     g.ghost_host = gp(m.ghost_host); g.ghost_type = gp(m.ghost_type); g.runs = gp(m.runs); g.lu_nb = gp(m.lu_nb);
     g.pi_cells = gp(m.pi_cells);
     g.vecx = gp(m.vecx); g.fvl = gp(m.fvl); g.fvr = gp(m.fvr); g.vol = gp(m.vol); g.wdis = gp(m.wdis);

@@ -40,7 +40,13 @@ struct HostMesh {
     std::vector<int> stencil;        // 3*ne  slot -> cell (>=0) or -(ghost+1)
     std::vector<double> lvv;         // 6*ne  limiter face-midpoint offsets [i][kf][d]
     std::vector<double> fvl, fvr;    // 2*nf  flux-point offsets of owner / neighbour
-    std::vector<double> vgeo;        // 6*nf  viscous face operands: 1/reflen of NL (x,y), of NR (x,y), cellxy(NL)-cellxy(NR)
+    std::vector<double> vgeo;        // 6*nf  viscous face operands: 1/reflen of NL (x,y), of NR (x,y), cellxy(NL)-(cellxy(NR)-SideOfs)
+    // periodic face pairs (period.f90): a periodic face's neighbour is the
+    // owner of its partner face, SideOfs its translation; empty without them
+    std::vector<double> sideofs;     // 2*nf  SideOfs (0 on other faces)
+    std::vector<char> periodic;      // nf    FProperty == 100
+    std::vector<double> nofs;        // 6*ne  stencil offsets Ofs of the interior neighbours [i][k][d]
+    std::vector<int> tsx;            // kTsFaces*ne  faces adding to the cell's TimeStep sum, in face order (-1 pad)
     std::vector<double> gnrm;        // 2*ng  unit normal used by the ghost update
     std::vector<int> cface;          // 3*ne  face f if cell is the owner, ~f otherwise
     std::vector<int> lu_nb;          // 3*ne  neighbour across each face (-1: boundary)
@@ -56,9 +62,15 @@ struct HostMesh {
     std::vector<int> pi_lvl_ptr;     // npl+1   cells of level l: pi_cells[pi_lvl_ptr[l] .. pi_lvl_ptr[l+1])
 };
 
+// faces whose alpha a cell's TimeStep sum takes at most: its three, and up to
+// three periodic faces of which it is the neighbour
+constexpr int kTsFaces = 6;
+
 // Builds HostMesh from 0-based inputs; returns RC_OK or RC_ERR_MESH with msg.
+// Boundary labels btype >= 100 name the boundaries of the ncon periodic
+// connections con[2c] -> con[2c+1], translation dxyz[2c..2c+1].
 int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, const int *bedges0,
-               const int *btype, HostMesh &m, std::string &msg);
+               const int *btype, int ncon, const int *con, const double *dxyz, HostMesh &m, std::string &msg);
 
 // Multicolour ordering (no reference counterpart).
 int order_multicolor(int ne, const int *cells0, int block, int *perm, int *ncolors);
@@ -99,6 +111,7 @@ struct DevParams {
     int combustion;
     int viscous;                   // IfViscous
     int surrogate;                 // source term from the learned-chemistry array S.src
+    int periodic;                  // the mesh has periodic faces: TimeStep sums over M.tsx
 };
 
 }  // namespace rc

@@ -67,6 +67,17 @@ __device__ __forceinline__ double cell_step_dd(const GMesh &M, const GState &S, 
         a = P.viscous ? a + FV_(0, f) + FV_(kRecF - 1, f) / v : a + al[k];
         if (store) S.lu_af[k * M.sc + i] = al[k];
     }
+    if (P.periodic) {
+        // a periodic face adds its alpha to both cells (TimeStep.f90:84-91),
+        // so a cell that is the neighbour of periodic faces sums more than
+        // its own three, all in face order
+        a = 0.0;
+#pragma unroll
+        for (int k = 0; k < kTsFaces; ++k) {
+            const int f = M.tsx[k * M.sc + i];
+            if (f >= 0) a = P.viscous ? a + FV_(0, f) + FV_(kRecF - 1, f) / v : a + ALPHA_(f);
+        }
+    }
     const double st = cfl * v / a;
 #pragma unroll
     for (int k = 0; k < 3; ++k) d = d + 0.5 * al[k];

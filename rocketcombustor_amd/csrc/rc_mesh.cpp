@@ -123,7 +123,7 @@ int fail(std::string &msg, const char *fmt, int a = 0, int b = 0) {
 }  // namespace
 
 int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, const int *bedges0,
-               const int *btype, HostMesh &m, std::string &msg) {
+               const int *btype, int ncon, const int *con, const double *dxyz, HostMesh &m, std::string &msg) {
     if (nnt <= 0 || ne <= 0 || nb < 0) return fail(msg, "empty mesh");
     m.nnt = nnt;
     m.ne = ne;
@@ -159,15 +159,69 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
     }
     m.nf = (int)m.fnl.size();
 
-    // ---- boundary tags (mesh.f90:76-93)
+    // ---- periodic connections (CreatePeriodConnect, period.f90:15-104,
+    // after the face creation as mesh.f90:69-73 calls it).  Connection c
+    // takes every face of boundary con[2c], in its given order, to the first
+    // face of boundary con[2c+1] whose midpoint is the source midpoint moved
+    // by dxyz: the midpoints and their difference in single precision, the
+    // test |ddd - dxyz|_1 < 1e-7 in double (period.f90:19,76-87).  The face's
+    // neighbour becomes the owner of that face, FProperty 100, SideOfs dxyz.
+    auto labelled = [&](int t) {
+        for (int c = 0; c < ncon; ++c)
+            if (con[2 * c] == t || con[2 * c + 1] == t) return true;
+        return false;
+    };
+    m.sideofs.clear();
+    m.periodic.clear();
+    if (ncon > 0) {
+        m.sideofs.assign(2 * (size_t)m.nf, 0.0);
+        m.periodic.assign(m.nf, 0);
+        auto bface = [&](int b) {
+            auto it = fidx.find(edge_key(bedges0[2 * b], bedges0[2 * b + 1]));
+            return it == fidx.end() ? -1 : it->second;
+        };
+        auto mid = [&](int b, int d) {
+            return (float)((X[2 * bedges0[2 * b] + d] + X[2 * bedges0[2 * b + 1] + d]) / 2.);
+        };
+        for (int c = 0; c < ncon; ++c) {
+            if (con[2 * c] < 100 || con[2 * c + 1] < 100)
+                return fail(msg, "periodic connection %d names a boundary label below 100", c + 1);
+            for (int b = 0; b < nb; ++b) {
+                if (btype[b] != con[2 * c]) continue;
+                const int fs = bface(b);
+                if (fs < 0) return fail(msg, "not find boundary face (boundary face %d)", b + 1);
+                const float xs0 = mid(b, 0), xs1 = mid(b, 1);
+                int hit = -1;
+                for (int b2 = 0; b2 < nb && hit < 0; ++b2) {
+                    if (btype[b2] != con[2 * c + 1]) continue;
+                    const float d0 = mid(b2, 0) - xs0, d1 = mid(b2, 1) - xs1;
+                    if (std::fabs((double)d0 - dxyz[2 * c]) + std::fabs((double)d1 - dxyz[2 * c + 1]) < (double)1.e-7f)
+                        hit = b2;
+                }
+                if (hit < 0) return fail(msg, "not find target (periodic connection %d, boundary face %d)", c + 1, b + 1);
+                const int ft = bface(hit);
+                if (ft < 0) return fail(msg, "not find boundary face (boundary face %d)", hit + 1);
+                m.fnr[fs] = m.fnl[ft];
+                m.periodic[fs] = 1;
+                m.sideofs[2 * (size_t)fs] = dxyz[2 * c];
+                m.sideofs[2 * (size_t)fs + 1] = dxyz[2 * c + 1];
+            }
+        }
+    }
+    auto ofs = [&](int f, int d) { return m.sideofs.empty() ? 0.0 : m.sideofs[2 * (size_t)f + d]; };
+    auto is_periodic = [&](int f) { return !m.periodic.empty() && m.periodic[f] != 0; };
+
+    // ---- boundary tags (mesh.f90:76-93); the boundaries a periodic
+    // connection names take type 100 (mesh.f90:62-63)
     std::unordered_map<uint64_t, int> btag;
     btag.reserve((size_t)nb * 2);
     for (int b = 0; b < nb; ++b) {
-        const int t = btype[b];
+        int t = btype[b];
+        if (t >= 100 && labelled(t)) t = 100;
         // flux_lax knows types 1..6 and stops on any other face
         // (flux_lax.f90:232-235, "no face of this type"), every step before
         // flux_viscous, so flux_viscous's type 7 branch (:170-175) never runs
-        if (t < 1 || t > 6)
+        else if (t < 1 || t > 6)
             return fail(msg, "unsupported boundary type %d (flux_lax handles types 1-6 and stops on others)", t);
         btag.emplace(edge_key(bedges0[2 * b], bedges0[2 * b + 1]), t);
     }
@@ -175,6 +229,11 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
         if (m.fnr[f] < 0) {
             auto it = btag.find(edge_key(m.fnode[2 * f], m.fnode[2 * f + 1]));
             if (it == btag.end()) return fail(msg, "not find boundary of this type (face %d)", f + 1);
+            // a face of a periodic boundary that no connection paired: type
+            // 100, on which flux_lax stops
+            if (it->second == 100)
+                return fail(msg, "boundary face %d of a periodic boundary has no partner (flux_lax stops on type 100)",
+                            f + 1);
             m.fnr[f] = -it->second;
         }
     }
@@ -211,13 +270,15 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
         const int nl = m.fnl[f], nr = m.fnr[f];
         if (cnt[nl] >= 3) return fail(msg, "cell %d has more than three faces", nl + 1);
         m.neighf[3 * nl + cnt[nl]++] = f;
-        if (nr >= 0) {
+        // a periodic face is in its owner's list only (:20)
+        if (nr >= 0 && !is_periodic(f)) {
             if (cnt[nr] >= 3) return fail(msg, "cell %d has more than three faces", nr + 1);
             m.neighf[3 * nr + cnt[nr]++] = f;
         }
     }
     m.neigh.assign(ne, 0);
     m.neighc.assign(3 * (size_t)ne, -1);
+    m.nofs.assign(6 * (size_t)ne, 0.0);
     m.cface.assign(3 * (size_t)ne, 0);
     m.lu_nb.assign(3 * (size_t)ne, -1);
     m.lu_n.assign(6 * (size_t)ne, 0.0);
@@ -228,7 +289,10 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
             const int sides[2] = {m.fnl[f], m.fnr[f]};
             for (int s = 0; s < 2; ++s) {
                 const int ic = sides[s];
-                if (ic != i && ic >= 0) m.neighc[3 * i + m.neigh[i]++] = ic;
+                if (ic != i && ic >= 0) {
+                    for (int d = 0; d < 2; ++d) m.nofs[6 * (size_t)i + 2 * m.neigh[i] + d] = ofs(f, d);   // :40
+                    m.neighc[3 * i + m.neigh[i]++] = ic;
+                }
             }
             m.cface[3 * i + k] = (m.fnl[f] == i) ? f : ~f;
             // LU-SGS face slot (Time_LUSGS_steady.f90:60-68): the other cell
@@ -303,6 +367,31 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
         }
         if (m.num_ghost[i] + m.neigh[i] != 3) return fail(msg, "error in number of ghost cells for cell %d", i + 1);
     }
+    // a cell whose neighbours repeat (two faces, one of them periodic, to the
+    // same cell) is refused: the sweeps take one term per neighbour and step
+    for (int i = 0; i < ne; ++i)
+        for (int a = 0; a < m.neigh[i]; ++a)
+            for (int b = a + 1; b < m.neigh[i]; ++b)
+                if (m.neighc[3 * i + a] == m.neighc[3 * i + b])
+                    return fail(msg, "cell %d has cell %d as a neighbour across two faces", i + 1, m.neighc[3 * i + a] + 1);
+
+    // ---- the faces of each cell's TimeStep sum (TimeStep.f90:16-96): every
+    // face adds its alpha to NL and, when NR > 0, to NR, periodic faces too
+    // (no FProperty test there); a cell's sum runs in face order
+    m.tsx.clear();
+    if (!m.periodic.empty()) {
+        m.tsx.assign((size_t)kTsFaces * ne, -1);
+        std::vector<int> nts(ne, 0);
+        for (int f = 0; f < m.nf; ++f) {
+            const int sides[2] = {m.fnl[f], m.fnr[f]};
+            for (int s = 0; s < 2; ++s) {
+                const int c = sides[s];
+                if (c < 0) continue;
+                if (nts[c] >= kTsFaces) return fail(msg, "cell %d takes more than %d TimeStep faces", c + 1, kTsFaces);
+                m.tsx[(size_t)kTsFaces * c + nts[c]++] = f;
+            }
+        }
+    }
 
     // ---- LSQ matrices (reconstruction_initialization.f90:71-150)
     m.xyh.assign(2 * (size_t)ne, 0.0);
@@ -323,9 +412,16 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
         for (int k = 0; k < nn; ++k) {
             const int jc = m.neighc[3 * i + k];
             m.stencil[3 * i + k] = jc;
-            const double *q1 = X + 2 * m.cn[3 * jc], *q2 = X + 2 * m.cn[3 * jc + 1], *q3 = X + 2 * m.cn[3 * jc + 2];
-            const double dx = (m.cellxy[2 * jc] - 0.0) - m.cellxy[2 * i];
-            const double dy = (m.cellxy[2 * jc + 1] - 0.0) - m.cellxy[2 * i + 1];
+            // the neighbour's nodes moved by its offset Ofs (:101-109)
+            const double o0 = m.nofs[6 * (size_t)i + 2 * k], o1 = m.nofs[6 * (size_t)i + 2 * k + 1];
+            double q[3][2];
+            for (int a = 0; a < 3; ++a) {
+                q[a][0] = X[2 * m.cn[3 * jc + a]] - o0;
+                q[a][1] = X[2 * m.cn[3 * jc + a] + 1] - o1;
+            }
+            const double *q1 = q[0], *q2 = q[1], *q3 = q[2];
+            const double dx = (m.cellxy[2 * jc] - o0) - m.cellxy[2 * i];
+            const double dy = (m.cellxy[2 * jc + 1] - o1) - m.cellxy[2 * i + 1];
             const double w = powi(std::sqrt(dx * dx + dy * dy), -1);
             m.wdis[3 * i + k] = w;
             for (int io = 0; io < 2; ++io) {
@@ -373,7 +469,7 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
         for (int d = 0; d < 2; ++d) {
             const double xy = X[2 * n1 + d] + 0.5 * (X[2 * n2 + d] - X[2 * n1 + d]);  // flux_lax.f90:34
             m.fvl[2 * f + d] = (xy - m.cellxy[2 * nl + d]) / m.reflen[2 * nl + d];
-            if (nr >= 0) m.fvr[2 * f + d] = (xy - (m.cellxy[2 * nr + d] - 0.0)) / m.reflen[2 * nr + d];
+            if (nr >= 0) m.fvr[2 * f + d] = (xy - (m.cellxy[2 * nr + d] - ofs(f, d))) / m.reflen[2 * nr + d];  // :86
         }
     }
     // viscous face operands (flux_viscous.f90:187-209, fbaseGrad_dxdy
@@ -388,8 +484,8 @@ int build_mesh(int nnt, const double *coor, int ne, const int *cells0, int nb, c
         if (nr >= 0) {
             g[2] = 1. / m.reflen[2 * nr];
             g[3] = 1. / m.reflen[2 * nr + 1];
-            g[4] = m.cellxy[2 * nl] - (m.cellxy[2 * nr] - 0.0);
-            g[5] = m.cellxy[2 * nl + 1] - (m.cellxy[2 * nr + 1] - 0.0);
+            g[4] = m.cellxy[2 * nl] - (m.cellxy[2 * nr] - ofs(f, 0));        // flux_viscous.f90:212-213
+            g[5] = m.cellxy[2 * nl + 1] - (m.cellxy[2 * nr + 1] - ofs(f, 1));
         }
     }
 

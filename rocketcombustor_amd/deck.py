@@ -395,14 +395,18 @@ def write_solver_info(case_dir: str, s: SolverInfo) -> None:
 
 
 def write_grid(path: str, coor: np.ndarray, cells: np.ndarray, bedges: np.ndarray,
-               btype: np.ndarray, nface: int) -> None:
+               btype: np.ndarray, nface: int, periodic=()) -> None:
     """``grid1.in`` for one rank (src/mesh.f90:10-65).  ``cells`` are 0-based
     counter-clockwise triangles; ``bedges`` 0-based node pairs with their BC
-    type in ``btype`` (one boundary group per type)."""
+    type in ``btype`` (one boundary group per type; labels >= 100 are
+    periodic boundaries, written with type 100).  ``periodic`` holds the
+    connections (source label, target label, dx, dy), written as NCon lines
+    of 1-based boundary indices and the translation (mesh.f90:60-64)."""
     ne, nn = len(cells), len(coor)
     types = sorted(set(int(t) for t in btype))
     groups = [(t, bedges[btype == t]) for t in types]
     imax = max(len(g) for _, g in groups) if groups else 0
+    index = {t: k + 1 for k, t in enumerate(types)}
     with open(path, "w") as f:
         f.write("%d %d %d %d %d %d\n" % (ne, ne, nn, nn, nface, nface))
         np.savetxt(f, np.column_stack([np.full(ne, 3), cells + 1]), fmt="%d")
@@ -410,9 +414,11 @@ def write_grid(path: str, coor: np.ndarray, cells: np.ndarray, bedges: np.ndarra
         f.write("0\n")
         f.write("%d %d\n" % (len(groups), imax))
         for t, g in groups:
-            f.write("%d %d\n" % (t, len(g)))
+            f.write("%d %d\n" % (min(t, 100), len(g)))
             np.savetxt(f, g + 1, fmt="%d")
-        f.write("0\n")
+        f.write("%d\n" % len(periodic))
+        for src, dst, dx, dy in periodic:
+            f.write("%d %d %.17g %.17g\n" % (index[int(src)], index[int(dst)], dx, dy))
 
 
 def write_backup(path: str, cellxy: np.ndarray, pa: np.ndarray, ttime: float = 0.0) -> None:
@@ -432,10 +438,14 @@ def read_backup(path: str, nvar: int):
     return ttime, data[:, :2], data[:, 2:]
 
 
-def read_grid(path: str, unit: float = 1.0):
+def read_grid(path: str, unit: float = 1.0, with_periodic: bool = False):
     """``grid<N>.in`` as mesh.f90:10-65 reads it (free-format tokens):
-    returns (coor, cells, bedges, btype, nface) with 0-based node numbers.
-    Multi-rank connectivity lists and periodic pairs are not supported."""
+    returns (coor, cells, bedges, btype, nface) with 0-based node numbers,
+    and with ``with_periodic`` the periodic connections too: the faces of
+    a boundary that a connection names carry the label 100 + its 1-based
+    index, and each connection is (source label, target label, dx, dy) with
+    dxyz as read (mesh.f90:60-64 does not scale it by ``unit``).  Multi-rank
+    connectivity lists are not supported."""
     with open(path) as f:
         tok = f.read().split()
     pos = 0
@@ -466,11 +476,21 @@ def read_grid(path: str, unit: float = 1.0):
         bedges.append(pairs)
         btype.append(np.full(cnt, t, dtype=np.int32))
     ncon = take()[0] if pos < len(tok) else 0
-    if ncon:
-        raise ValueError("periodic face pairs are not supported")
+    if ncon and not with_periodic:
+        raise ValueError("periodic face pairs: read this grid with with_periodic=True")
+    periodic = []
+    for _ in range(ncon):
+        gs, gt = take(2)
+        dx, dy = take(2, float)
+        if not (1 <= gs <= nbound and 1 <= gt <= nbound):
+            raise ValueError("periodic connection names boundary %d or %d of %d" % (gs, gt, nbound))
+        periodic.append((100 + gs, 100 + gt, dx, dy))
+    for lab in {p[0] for p in periodic} | {p[1] for p in periodic}:
+        btype[lab - 101] = np.full(len(btype[lab - 101]), lab, dtype=np.int32)
     be = np.concatenate(bedges) - 1 if bedges else np.zeros((0, 2), dtype=np.int32)
     bt = np.concatenate(btype) if btype else np.zeros(0, dtype=np.int32)
-    return xy[:, :2] / unit, cells - 1, be.astype(np.int32), bt, nf
+    out = (xy[:, :2] / unit, cells - 1, be.astype(np.int32), bt, nf)
+    return out + (tuple(periodic),) if with_periodic else out
 
 
 def read_solver_info(case_dir: str) -> SolverInfo:

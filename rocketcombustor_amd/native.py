@@ -73,6 +73,8 @@ _FP = P(C.c_float)
 SIGNATURES = {
     "rc_case_create": (C.c_int, [P(RcMechanism), P(RcParams), C.c_int, _DP, C.c_int, _IP, C.c_int, _IP, _IP,
                                  C.c_int, P(_VP)]),
+    "rc_case_create_periodic": (C.c_int, [P(RcMechanism), P(RcParams), C.c_int, _DP, C.c_int, _IP, C.c_int, _IP,
+                                          _IP, C.c_int, _IP, _DP, C.c_int, P(_VP)]),
     "rc_case_destroy": (None, [_VP]),
     "rc_last_error": (C.c_char_p, [_VP]),
     "rc_case_info": (C.c_int, [_VP, P(C.c_longlong)]),

@@ -92,8 +92,11 @@ class Mesh:
     coor: np.ndarray      # (NNT, 2) float64
     cells: np.ndarray     # (NE, 3) int32, 0-based, counter-clockwise
     bedges: np.ndarray    # (NB, 2) int32, 0-based node pairs
-    btype: np.ndarray     # (NB,) int32, reference BC type
+    btype: np.ndarray     # (NB,) int32, reference BC type (>= 100: label of a periodic boundary)
     nface: int
+    # periodic connections (mesh.f90:60-64, period.f90): (source label,
+    # target label, dx, dy), the translation taking source to target faces
+    periodic: tuple = ()
 
     @property
     def ne(self) -> int:
@@ -104,7 +107,7 @@ class Mesh:
 
     def permuted(self, perm: np.ndarray) -> "Mesh":
         """Cells reordered so that new cell k is old cell ``perm[k]``."""
-        return Mesh(self.coor, np.ascontiguousarray(self.cells[perm]), self.bedges, self.btype, self.nface)
+        return dataclasses.replace(self, cells=np.ascontiguousarray(self.cells[perm]))
 
 
 def _incircle(ax, ay, bx, by, cx, cy, dx, dy):
@@ -300,6 +303,29 @@ def nozzle_case(nx: int, ny: int, p0: float = 5.0e6, t0: float = 3000.0, phi: fl
                 geom=geom, p0=p0, t0=t0, transport=deck.default_transport() if viscous else None)
 
 
+def periodic_channel_case(nx: int, ny: int, direction: str = "x", **kw) -> Case:
+    """A straight channel (the nozzle mesh at constant height) whose
+    inlet and outlet (``direction`` "x") or axis and wall ("y") are joined
+    as periodic boundary pairs: labels 101 -> 102 and 102 -> 101 with the
+    channel's length or height as translation."""
+    g = NozzleGeometry(h_throat=NozzleGeometry.h_chamber, h_exit=NozzleGeometry.h_chamber)
+    case = nozzle_case(nx, ny, geom=g, **kw)
+    m = case.mesh
+    bt = m.btype.copy()
+    if direction == "x":
+        src, dst = bt == 3, bt == 5
+        d = (g.length, 0.0)
+    elif direction == "y":
+        src, dst = bt == 2, np.isin(bt, (1, 6))
+        d = (0.0, g.h_chamber)
+    else:
+        raise ValueError("direction is 'x' or 'y'")
+    bt[src] = 101
+    bt[dst] = 102
+    mesh = dataclasses.replace(m, btype=bt, periodic=((101, 102, d[0], d[1]), (102, 101, -d[0], -d[1])))
+    return dataclasses.replace(case, mesh=mesh)
+
+
 def write_reference_case(case_dir: str, case: Case, nstep_max: Optional[int] = None,
                          extra_decks: Optional[str] = None) -> None:
     """Writes the full input deck the reference reads for a one-rank steady
@@ -311,7 +337,7 @@ def write_reference_case(case_dir: str, case: Case, nstep_max: Optional[int] = N
     info = case.info if nstep_max is None else dataclasses.replace(case.info, nstep_max=nstep_max)
     deck.write_solver_info(case_dir, info)
     deck.write_grid(os.path.join(case_dir, "grid1.in"), case.mesh.coor, case.mesh.cells,
-                    case.mesh.bedges, case.mesh.btype, case.mesh.nface)
+                    case.mesh.bedges, case.mesh.btype, case.mesh.nface, periodic=case.mesh.periodic)
     deck.write_backup(os.path.join(case_dir, "fvp1.sav"), case.mesh.centroids(), case.pa0)
     if case.transport is not None:
         deck.write_transport(case_dir, case.transport)
@@ -367,8 +393,9 @@ def case_from_dir(case_dir: str) -> Case:
     mech = deck.read_mechanism(case_dir)
     oxid = deck.read_inlet(os.path.join(case_dir, "combustion", "oxidizer.in"), mech.nsp)
     fuel = deck.read_inlet(os.path.join(case_dir, "combustion", "fuel.in"), mech.nsp)
-    coor, cells, bedges, btype, nface = deck.read_grid(os.path.join(case_dir, "grid1.in"), info.unit)
-    mesh = Mesh(coor=coor, cells=cells, bedges=bedges, btype=btype, nface=nface)
+    coor, cells, bedges, btype, nface, periodic = deck.read_grid(os.path.join(case_dir, "grid1.in"), info.unit,
+                                                                 with_periodic=True)
+    mesh = Mesh(coor=coor, cells=cells, bedges=bedges, btype=btype, nface=nface, periodic=periodic)
     _, _, pa = deck.read_backup(os.path.join(case_dir, "fvp1.sav"), 3 + mech.nsp)
     return Case(mesh=mesh, mech=mech, oxidizer=oxid, fuel=fuel, info=info, pa0=pa, geom=NozzleGeometry(),
                 p0=oxid.pres, t0=oxid.temp)

@@ -63,8 +63,16 @@ class SteadyCase:
         bn1 = np.ascontiguousarray(m.bedges, dtype=np.int32) + 1
         bt = np.ascontiguousarray(m.btype, dtype=np.int32)
         h = C.c_void_p()
-        rc = L.rc_case_create(C.byref(self._mech), C.byref(self._par), len(coor), dptr(coor), len(cells1),
-                              iptr(cells1), len(bn1), iptr(bn1), iptr(bt), int(device), C.byref(h))
+        periodic = getattr(m, "periodic", ())
+        if periodic:
+            con = np.ascontiguousarray([[p[0], p[1]] for p in periodic], dtype=np.int32)
+            dxyz = np.ascontiguousarray([[p[2], p[3]] for p in periodic], dtype=np.float64)
+            rc = L.rc_case_create_periodic(C.byref(self._mech), C.byref(self._par), len(coor), dptr(coor),
+                                           len(cells1), iptr(cells1), len(bn1), iptr(bn1), iptr(bt), len(con),
+                                           iptr(con), dptr(dxyz), int(device), C.byref(h))
+        else:
+            rc = L.rc_case_create(C.byref(self._mech), C.byref(self._par), len(coor), dptr(coor), len(cells1),
+                                  iptr(cells1), len(bn1), iptr(bn1), iptr(bt), int(device), C.byref(h))
         check(rc, None)
         self.h = h
         info = (C.c_longlong * 6)()

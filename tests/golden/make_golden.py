@@ -98,6 +98,7 @@ def case_arrays(case: nozzle.Case) -> dict:
     return {
         "coor": case.mesh.coor, "cells": case.mesh.cells, "bedges": case.mesh.bedges, "btype": case.mesh.btype,
         "nface": case.mesh.nface, "pa0": case.pa0,
+        **({"periodic": np.array(case.mesh.periodic, dtype=np.float64).reshape(-1, 4)} if case.mesh.periodic else {}),
         "mech_mw": m.mole_weight, "mech_rconstant": m.rconstant,
         "mech_trange": np.array([m.t_low, m.t_crit, m.t_high]),
         "mech_low": m.low_coef, "mech_high": m.high_coef, "mech_low_b1": m.low_b1, "mech_high_b1": m.high_b1,
@@ -213,6 +214,19 @@ def main() -> None:
          nsteps=8, ndump=6)
     make("n24x6_be_bdf2_viscous", be(viscous(nozzle.nozzle_case(24, 6)), 6, 1e-5, 3, 0.5, method=5),
          nsteps=6, ndump=4)
+    # periodic boundary pairs (period.f90, FProperty 100): a straight
+    # channel with its inlet and outlet joined (x), or its axis and wall
+    # joined with the inlet and outlet kept (y)
+    px = nozzle.periodic_channel_case(24, 6, direction="x")
+    make("n24x6_periodic_x", px, nsteps=20, ndump=2)
+    py = nozzle.periodic_channel_case(24, 6, direction="y", p_out_ratio=0.5)
+    make("n24x6_periodic_y", py, nsteps=20, ndump=2)
+    perm, _ = native.order_multicolor(py.mesh.cells)
+    make("n24x6_periodic_y_multicolor", py.permuted(perm), nsteps=20, ndump=2)
+    make("n24x6_viscous_periodic_y", viscous(nozzle.periodic_channel_case(24, 6, direction="y", p_out_ratio=0.5)),
+         nsteps=10, ndump=2)
+    make("n24x6_be_periodic_x", be(nozzle.periodic_channel_case(24, 6, direction="x"), 6, 1e-5, 3, 0.5),
+         nsteps=6, ndump=2)
     conv = nozzle.nozzle_case(30, 8)
     make("n30x8_converged", conv, nsteps=8000, ndump=0, every=10)
     perm, _ = native.order_multicolor(conv.mesh.cells)

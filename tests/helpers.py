@@ -66,8 +66,9 @@ def case_from_fixture(d: dict) -> nozzle.Case:
         transport = deck.Transport(t_low=t_low, t_crit=t_crit, t_high=t_high, mu_low=d["tr_mu_low"],
                                    mu_high=d["tr_mu_high"], k_low=d["tr_k_low"], k_high=d["tr_k_high"], k_b=k_b,
                                    sigma=d["tr_sigma"], eps=d["tr_eps"])
+    periodic = tuple((int(a), int(b), float(x), float(y)) for a, b, x, y in d["periodic"]) if "periodic" in d else ()
     mesh = nozzle.Mesh(d["coor"], d["cells"].astype(np.int32), d["bedges"].astype(np.int32),
-                       d["btype"].astype(np.int32), int(d["nface"]))
+                       d["btype"].astype(np.int32), int(d["nface"]), periodic=periodic)
     return nozzle.Case(mesh=mesh, mech=mech, oxidizer=ox, fuel=fu, info=info, pa0=d["pa0"],
                        geom=nozzle.NozzleGeometry(), p0=0.0, t0=0.0, transport=transport)
 

@@ -154,6 +154,39 @@ def test_workload_sizes_vs_oracle(nx, ny, order):
         assert relerr(g.residual()[:7], o.residual()[:7]) < 1e-11
 
 
+@pytest.mark.parametrize("direction", ["x", "y"])
+def test_periodic_channel_vs_oracle(direction, xcd_layout):
+    """Periodic boundary pairs at the 12k-cell size in multicolour order,
+    both work-list layouts: the per-call arrays of one step (the TimeStep
+    sum of a cell that is the neighbour of periodic faces takes their alpha
+    too), then fused steps with their residual rows, against the oracle."""
+    case = nozzle.periodic_channel_case(150, 40, direction=direction, p_out_ratio=0.5)
+    perm, _ = native.order_multicolor(case.mesh.cells)
+    case = case.permuted(perm)
+    o = oracle.OracleCase(case)
+    announce("periodic channel 150x40 (%s) NE=%d RC_XCD=%s" % (direction, case.mesh.ne, xcd_layout))
+    with SteadyCase(case) as g:
+        g.timestep(); o.timestep()
+        step, alpaf = g.timestep_arrays()
+        assert relerr(alpaf, o.get("alpaF")) < 1e-13
+        assert relerr(step, o.get("step")) < 1e-13
+        g.reconstruction(); o.reconstruction()
+        assert relerr(g.gradient_array(), o.get("gradC"), -1) < 1e-12
+        g.flux_sourceterm(); o.flux_sourceterm()
+        assert relerr(g.increment(), o.get("hWA"), -1) < 1e-12
+        g.lu_sgs_steady(); o.lusgs_steady()
+        assert relerr(g.increment(), o.get("hWA"), -1) < 1e-12
+        assert relerr(g.primitive(), o.get("PA"), -1) < 1e-13
+        hist = g.run_steady(6, every=3)
+        rows = []
+        for s in range(1, 7):
+            o.stage()
+            if s in (1, 3, 6):
+                rows.append(o.residual())
+        assert relerr(g.primitive(), o.get("PA"), -1) < 1e-11
+        assert relerr(hist[:, :7], np.array(rows)[:, :7], -1) < 1e-10
+
+
 def test_deterministic_and_resident():
     """Gather-only kernels: two runs give bitwise-identical states; state
     stays resident between calls (get does not perturb it)."""
