@@ -403,8 +403,20 @@ std::vector<char> next_res_levels(const LaunchSet &L, int device) {
     return nx;
 }
 
-void enqueue_forward_levels(const LaunchSet &L, int nlev, hipStream_t st) {
+// the fused step's sweeps: forward levels (the highest one also running
+// its backward sweep when L.top_fused), then the other backward levels
+void enqueue_sweeps(const LaunchSet &L, int nlev, hipStream_t st) {
     for (int l = 0; l < nlev; ++l) launch_lusgs_fused_forward(L, l, st);
+}
+void enqueue_backward_levels(const LaunchSet &L, int nlev, hipStream_t st) {
+    for (int l = nlev - 1 - (L.top_fused ? 1 : 0); l >= 0; --l) launch_lusgs_level(L, false, l, false, st);
+}
+
+// RC_TOPFUSE=0: the highest level's backward sweep as a launch of its own
+// (read when a launch set is built; default: fused into its forward launch)
+bool top_fused_env() {
+    const char *e = std::getenv("RC_TOPFUSE");
+    return !(e && e[0] == '0');
 }
 
 // cell index ranges of the sweep levels when every level's runs are one
@@ -656,6 +668,7 @@ int layout_and_upload(rc_case *h) {
     if (!fitems.empty() && (rc = up(d_flow, fitems.data(), sizeof(int4) * fitems.size()))) return rc;
     L.flow = {d_flow, fspan, d_ctr, flow_grid(h->device, fspan)};
     L.next_res = next_res_levels(L, h->device);
+    L.top_fused = top_fused_env();
     L.resout = d_resout;
     return RC_OK;
 }
@@ -709,9 +722,9 @@ void enqueue_step_kernels(rc_case *h) {
         launch_lusgs_flow(L, h->stream);
         mark(h, 4);
     } else {
-        enqueue_forward_levels(L, h->nlev, h->stream);
+        enqueue_sweeps(L, h->nlev, h->stream);
         mark(h, 4);
-        for (int l = h->nlev - 1; l >= 0; --l) launch_lusgs_level(L, false, l, false, h->stream);
+        enqueue_backward_levels(L, h->nlev, h->stream);
     }
     mark(h, 5);
     launch_update(L, h->stream);
@@ -1075,6 +1088,7 @@ int batch_build(rc_batch *b) {
     }
     L.flow = {dflow, fspan, dctr, flow_grid(b->device, fspan)};
     L.next_res = next_res_levels(L, b->device);
+    L.top_fused = top_fused_env();
     L.resout = dres;
     BHIPCHK(b, hipStreamSynchronize(b->stream));
     const size_t need = (size_t)std::max(n, 1) * (kNvar + 1);
@@ -1110,8 +1124,8 @@ void enqueue_batch_step(rc_batch *b) {
     if (L.flow.grid > 0) {
         launch_lusgs_flow(L, b->stream);
     } else {
-        enqueue_forward_levels(L, b->nlev, b->stream);
-        for (int l = b->nlev - 1; l >= 0; --l) launch_lusgs_level(L, false, l, false, b->stream);
+        enqueue_sweeps(L, b->nlev, b->stream);
+        enqueue_backward_levels(L, b->nlev, b->stream);
     }
     launch_update(L, b->stream);
 }

@@ -158,6 +158,7 @@ struct LaunchSet {
     std::vector<WorkList> pi_level;   // point-implicit sweep: cells (positions in pi_cells) of each level
     FlowList flow;                 // the whole LU-SGS sweep of the fused step as one launch
     std::vector<char> next_res;    // forward level l also forms level l+1's residual
+    bool top_fused = false;        // the highest level's forward launch also runs its backward sweep
     double *resout = nullptr;      // ncases x (kNvar+1) residual norms
 };
 

@@ -752,7 +752,13 @@ constexpr int kSpinMax = 1 << 18;
 // the stores in between).  The backward sweep forms the TimeStep cell part
 // again from the face records instead of reading what the forward sweep of
 // another block stored (same arithmetic, same values).
-template <int NSP, bool FWD, bool RES, bool FLOW>
+// TOP (forward sweep of the highest level): the lanes then also run that
+// level's backward sweep and store its increments.  A cell of the highest
+// level has no higher neighbour outside its run (that neighbour's run would
+// lie in a higher level), so its backward terms come from its run alone:
+// the forward increments stay in registers, the run's backward increments
+// pass by shuffles, and every value is the one the backward launch forms.
+template <int NSP, bool FWD, bool RES, bool FLOW, bool TOP = false>
 __device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int tend, int epoch, double *tl) {
     constexpr int R = kRunMax;
     const GMesh M = gview(C.M);
@@ -884,6 +890,55 @@ __device__ __forceinline__ void lusgs_runs(const RC_G CaseDev &C, int t, int ten
             }
         }
     }
+    if constexpr (TOP && FWD && !FLOW) {
+        // the backward sweep of the run (Time_LUSGS_steady.f90:127-190):
+        // faces to higher neighbours (all in the run) start at +0.0 and take
+        // their term at the step of that neighbour, the others stay +0.0
+        int qb[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int nr = nb[k];
+            qb[k] = live && nr >= 0 && nr > i ? first + cnt - 1 - nr : -1;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) tl[(k * 8 + v) * bs + tid] = 0.0;
+        }
+        const int bstep = cnt - 1 - lane;
+        double hb[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) hb[v] = 0.0;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            if (bstep == s) {
+                double dflux[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) dflux[v] = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) dflux[v] = dflux[v] + tl[(k * 8 + v) * bs + tid];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) hb[v] = hv[v] - dflux[v] / d;
+            }
+            if (s < R - 1) {
+                const int src = cnt - 1 - s > 0 ? cnt - 1 - s : 0;
+                double dw[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) dw[v] = __shfl(hb[v], src, R);
+                const int ks = qb[0] == s ? 0 : (qb[1] == s ? 1 : (qb[2] == s ? 2 : -1));
+                if (ks >= 0) {
+                    const int nrs = ks == 0 ? nb[0] : (ks == 1 ? nb[1] : nb[2]);
+                    const LuNb nb_s = load_lunb(S, nrs);
+                    double cs[NV];
+                    lusgs_term<NSP>(th, nb_s, dw, sel3(ks, nx[0], nx[1], nx[2]), sel3(ks, ny[0], ny[1], ny[2]),
+                                    sel3(ks, af[0], af[1], af[2]), cs);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) tl[(ks * 8 + v) * bs + tid] = cs[v];
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) hv[v] = hb[v];
+    }
     if (live) {
         if (FLOW) {
 #pragma unroll
@@ -934,7 +989,7 @@ __device__ __forceinline__ void run_residual(const RC_G CaseDev &C, int t, int t
 // reads the increments of lower levels only and writes those of its own
 // cells, and the residual of a higher level's cell reads neither, so the
 // next level sweeps with its residual, step and dd already in memory.
-template <int NSP, bool FWD, bool RES, bool B, bool NX>
+template <int NSP, bool FWD, bool RES, bool B, bool NX, bool TOP = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_lusgs(
         ITEM_ARGS, const int4 *__restrict__ work2, int lo2, int hi2, int nb1) {
     constexpr int R = kRunMax;
@@ -963,7 +1018,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         tend = hi;
     }
     extern __shared__ double lusgs_tl[];
-    lusgs_runs<NSP, FWD, RES, false>(*Cp_, t, tend, 0, lusgs_tl);
+    lusgs_runs<NSP, FWD, RES, false, TOP>(*Cp_, t, tend, 0, lusgs_tl);
 }
 
 // The dataflow sweep: forward sweep (with the residual) and backward sweep
@@ -1372,28 +1427,35 @@ void launch_residual(const LaunchSet &L, bool ts, hipStream_t st) {
 
 // res: forward levels that also form their cells' residual; nx: forward
 // levels that also form the residual of the next level's cells (list nx)
-template <bool FWD, bool RES, bool NX>
+template <bool FWD, bool RES, bool NX, bool TOP = false>
 void launch_level(const LaunchSet &L, const WorkList &wl, const WorkList *nx, hipStream_t st) {
     if (L.single) {
         const int n = wl.hi - wl.lo, nb1 = (n + kRunChunk - 1) / kRunChunk;
         const int n2 = NX ? nx->hi - nx->lo : 0, nb2 = (n2 + kRunChunk - 1) / kRunChunk;
         if (nb1 + nb2 > 0)
-            k_lusgs<kNsp, FWD, RES, false, NX><<<nb1 + nb2, kRunThreads, lusgs_lds(kRunThreads), st>>>(
+            k_lusgs<kNsp, FWD, RES, false, NX, TOP><<<nb1 + nb2, kRunThreads, lusgs_lds(kRunThreads), st>>>(
                 L.cases, nullptr, wl.lo, wl.hi, nullptr, NX ? nx->lo : 0, NX ? nx->hi : 0, nb1);
     } else {
         const int nb2 = NX ? nx->n : 0;
         if (wl.n + nb2 > 0)
-            k_lusgs<kNsp, FWD, RES, true, NX><<<wl.n + nb2, wl.bs, lusgs_lds(wl.bs), st>>>(
+            k_lusgs<kNsp, FWD, RES, true, NX, TOP><<<wl.n + nb2, wl.bs, lusgs_lds(wl.bs), st>>>(
                 L.cases, wl.w, 0, 0, NX ? nx->w : nullptr, 0, 0, wl.n);
     }
 }
 
 // the forward level l of the fused step: it forms its own cells' residual
 // unless the launch of level l-1 formed it, and the next level's when
-// L.next_res[l] says so (the level lists then share one block size)
+// L.next_res[l] says so (the level lists then share one block size); the
+// highest level also runs its backward sweep when L.top_fused
 void launch_lusgs_fused_forward(const LaunchSet &L, int l, hipStream_t st) {
     const bool res = l == 0 || !L.next_res[l - 1];
     const bool nx = L.next_res[l] != 0;
+    if (L.top_fused && l + 1 == (int)L.level.size()) {
+        // the highest level: its backward sweep in the same launch
+        if (res) launch_level<true, true, false, true>(L, L.level[l], nullptr, st);
+        else launch_level<true, false, false, true>(L, L.level[l], nullptr, st);
+        return;
+    }
     const WorkList *n = nx ? &L.level[l + 1] : nullptr;
     if (res && nx) launch_level<true, true, true>(L, L.level[l], n, st);
     else if (res) launch_level<true, true, false>(L, L.level[l], nullptr, st);

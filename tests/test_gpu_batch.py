@@ -218,6 +218,31 @@ def test_batch_multi_step_graph_matches_single_steps(monkeypatch):
             assert np.array_equal(a, e)
 
 
+def test_batch_fused_top_level_equals_separate_backward(monkeypatch):
+    """The batch's highest level running its backward sweep in its forward
+    launch equals a backward launch of its own bitwise (members with fewer
+    levels than the batch included)."""
+    cases = mixed_cases()
+    announce("batch fused top level vs separate backward, %d mixed cases" % len(cases))
+    outs = []
+    for fuse in ("0", None):
+        if fuse is None:
+            monkeypatch.delenv("RC_TOPFUSE", raising=False)
+        else:
+            monkeypatch.setenv("RC_TOPFUSE", fuse)
+        gs = [SteadyCase(c) for c in cases]
+        with SteadyBatch() as b:
+            for g in gs:
+                b.add(g)
+            h = b.run_steady(14, every=5)
+            outs.append((h, [g.primitive() for g in gs], [g.increment() for g in gs]))
+        for g in gs:
+            g.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    for a, e in zip(outs[0][1] + outs[0][2], outs[1][1] + outs[1][2]):
+        assert np.array_equal(a, e)
+
+
 @pytest.mark.parametrize("shape", [(64, 64, 0), (128, 256, 1), (64, 128, 1), (256, 64, -1)])
 def test_launch_shape_does_not_change_results(shape):
     """The autotuner's knob (threads per block, XCD banding) re-cuts the work

@@ -304,6 +304,40 @@ def test_dataflow_sweep_equals_level_launches(nx, ny, order, monkeypatch):
         lev.close()
 
 
+@pytest.mark.parametrize("nx,ny,order", [(1, 1, "natural"), (2, 1, "natural"), (24, 6, "natural"),
+                                         (150, 40, "multicolor"), (1000, 200, "multicolor")])
+def test_fused_top_level_equals_separate_backward(nx, ny, order, xcd_layout, monkeypatch):
+    """The highest sweep level's backward sweep run by that level's forward
+    launch (the default) gives bitwise the results of a backward launch of
+    its own (RC_TOPFUSE=0): per stage, graph-replayed with residual rows,
+    on both work-list layouts."""
+    case = nozzle.nozzle_case(nx, ny)
+    if order == "multicolor":
+        perm, _ = native.order_multicolor(case.mesh.cells)
+        case = case.permuted(perm)
+    announce("fused top level vs separate backward %dx%d %s RC_XCD=%s NE=%d" % (nx, ny, order, xcd_layout,
+                                                                               case.mesh.ne))
+    monkeypatch.setenv("RC_TOPFUSE", "0")
+    sep = SteadyCase(case)
+    monkeypatch.delenv("RC_TOPFUSE")
+    fused = SteadyCase(case)
+    try:
+        for k in range(2):
+            sep.lu_sgs_stage()
+            fused.lu_sgs_stage()
+            assert np.array_equal(fused.primitive(), sep.primitive()), k
+            assert np.array_equal(fused.increment(), sep.increment()), k
+        h1 = sep.run_steady(12, every=5)
+        h2 = fused.run_steady(12, every=5)
+        assert np.array_equal(h1, h2)
+        for a, b in ((fused.primitive(), sep.primitive()), (fused.conserved(), sep.conserved()),
+                     (fused.increment(), sep.increment())):
+            assert np.array_equal(a, b)
+    finally:
+        sep.close()
+        fused.close()
+
+
 @pytest.mark.parametrize("nx,ny,order", [(1, 1, "natural"), (24, 6, "natural"), (150, 40, "multicolor"),
                                          (1000, 200, "multicolor"), (150, 40, "natural")])
 def test_next_level_residual_equals_own_residual(nx, ny, order, xcd_layout, monkeypatch):
