@@ -123,6 +123,10 @@ const char *rc_last_error(const rc_case *h);           /* thread-local message w
 /* sizes[0..5] = NE, NF, number of ghost cells, Nvar, number of LU-SGS
  * wavefront levels, resident device bytes. */
 int rc_case_info(const rc_case *h, long long *sizes6);
+/* No counterpart in the reference: cells[l] = the number of cells in LU-SGS
+ * wavefront level l, for l < n (n at least the level count of
+ * rc_case_info); for measurement, whose byte counts follow the levels. */
+int rc_case_levels(const rc_case *h, int *cells, int n);
 
 /* Replaces the PA load of Read_Backup (output.f90:276-338) followed by
  * primitve_to_conserved_variables (update_flow_variables.f90:50-84). */

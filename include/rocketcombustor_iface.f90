@@ -85,6 +85,13 @@ module rocketcombustor_iface
             integer(c_long_long), intent(out) :: sizes6(6)
         end function
 
+        integer(c_int) function rc_case_levels(h, cells, n) bind(c, name="rc_case_levels")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int), intent(out) :: cells(*)
+            integer(c_int), value :: n
+        end function
+
         integer(c_int) function rc_set_primitive(h, pa) bind(c, name="rc_set_primitive")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: h

@@ -1277,6 +1277,16 @@ int rc_case_info(const rc_case *h, long long *s) {
     return RC_OK;
 }
 
+int rc_case_levels(const rc_case *h, int *cells, int n) {
+    if (!h || !cells) return set_err(nullptr, RC_ERR_ARGUMENT, "NULL argument");
+    if (n < h->nlev) return set_err(nullptr, RC_ERR_ARGUMENT, "fewer slots than levels");
+    const HostMesh &m = h->hm;
+    for (int l = 0; l < n; ++l) cells[l] = 0;
+    for (int l = 0; l < h->nlev; ++l)
+        for (int r = m.lvl_ptr[l]; r < m.lvl_ptr[l + 1]; ++r) cells[l] += m.runs[2 * r + 1];
+    return RC_OK;
+}
+
 int rc_set_primitive(rc_case *h, const double *pa) {
     if (!h || !pa) return set_err(h, RC_ERR_ARGUMENT, "NULL argument");
     HIPCHK(h, hipSetDevice(h->device));

@@ -78,6 +78,7 @@ SIGNATURES = {
     "rc_case_destroy": (None, [_VP]),
     "rc_last_error": (C.c_char_p, [_VP]),
     "rc_case_info": (C.c_int, [_VP, P(C.c_longlong)]),
+    "rc_case_levels": (C.c_int, [_VP, _IP, C.c_int]),
     "rc_set_primitive": (C.c_int, [_VP, _DP]),
     "rc_timestep": (C.c_int, [_VP]),
     "rc_gradient": (C.c_int, [_VP]),

@@ -133,6 +133,12 @@ class SteadyCase:
             self._call("rc_get_ghost", dptr(out))
         return out
 
+    def level_cells(self) -> np.ndarray:
+        """Cells in each LU-SGS wavefront level."""
+        out = np.zeros(max(self.nlev, 1), dtype=np.int32)
+        check(lib().rc_case_levels(self.h, iptr(out), len(out)), self.h)
+        return out[:self.nlev]
+
     def timestep_arrays(self):
         step = np.empty(self.ne)
         alpaf = np.empty(self.nf)

@@ -136,9 +136,13 @@ def test_workload_sizes_vs_oracle(nx, ny, order):
     with SteadyCase(case) as g:
         # the library's sweep schedule is the restated one; blocks of the
         # multicolour order become runs, one level per colour
-        assert g.nlev == sweep_run_levels(face_adjacency(case.mesh.cells))[1].max() + 1
+        first, lev = sweep_run_levels(face_adjacency(case.mesh.cells))
+        assert g.nlev == lev.max() + 1
         if order == "multicolor":
             assert g.nlev == nc
+        # cells per level, from the restated schedule's runs
+        runlen = np.diff(np.append(first, case.mesh.ne))
+        assert np.array_equal(g.level_cells(), np.bincount(lev, weights=runlen).astype(int))
         g.timestep(); o.timestep()
         assert relerr(g.timestep_arrays()[1], o.get("alpaF")) < 1e-13
         g.reconstruction(); o.reconstruction()
