@@ -218,6 +218,30 @@ def test_batch_multi_step_graph_matches_single_steps(monkeypatch):
             assert np.array_equal(a, e)
 
 
+def test_batch_with_periodic_members_equals_alone():
+    """Members with periodic boundary pairs (their TimeStep sums over the
+    periodic faces too) beside ordinary ones: each member bitwise its alone
+    result, residual rows included."""
+    px = nozzle.periodic_channel_case(30, 8, direction="x")
+    py = nozzle.periodic_channel_case(24, 6, direction="y", p_out_ratio=0.5)
+    perm, _ = native.order_multicolor(py.mesh.cells)
+    cases = [px, nozzle.nozzle_case(24, 6), py.permuted(perm)]
+    n, every = 20, 5
+    announce("batch with periodic members, %d cases" % len(cases))
+    ref = [alone(c, n, every) for c in cases]
+    gs = [SteadyCase(c) for c in cases]
+    with SteadyBatch() as b:
+        for g in gs:
+            b.add(g)
+        h = b.run_steady(n, every=every)
+        for k, g in enumerate(gs):
+            assert np.array_equal(h[:, k], ref[k][0]), k
+            assert np.array_equal(g.primitive(), ref[k][1]), k
+            assert np.array_equal(g.increment(), ref[k][2]), k
+    for g in gs:
+        g.close()
+
+
 def test_batch_fused_top_level_equals_separate_backward(monkeypatch):
     """The batch's highest level running its backward sweep in its forward
     launch equals a backward launch of its own bitwise (members with fewer

@@ -59,6 +59,20 @@ def test_persistent_step_equals_graph(nx, ny, persist, grid, monkeypatch):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("direction", ["x", "y"])
+def test_persistent_step_equals_graph_periodic(direction, monkeypatch):
+    """Periodic boundary pairs through the persistent step."""
+    case = multicolor(nozzle.periodic_channel_case(60, 16, direction=direction, p_out_ratio=0.5))
+    announce("persistent step, periodic channel (%s)" % direction)
+    ref = run_case(case, monkeypatch, 0)
+    got = run_case(case, monkeypatch, 8, 16)
+    for a, b in zip(ref[0], got[0]):
+        assert np.array_equal(a, b)
+    for sa, sb in zip(ref[1:], got[1:]):
+        for a, b in zip(sa, sb):
+            assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("grid", [None, 5])
 def test_batch_persistent_step_equals_graph(grid, monkeypatch):
     """Mixed members (sizes, inlet states, combustion switch, outlet branch),
