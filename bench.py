@@ -633,16 +633,22 @@ def main() -> int:
     # timing slot stays empty): their bytes are counted there
     kb["lusgs_forward"] += kb.pop("residual_source")
     kernels = {k: times[k] for k in g.TIMING_KEYS if k != "residual_source"}
-    launches = {k: (g.nlev if k.startswith("lusgs") else 1) for k in kernels}
-    # the highest level's backward sweep runs in its forward launch (timed in
-    # the forward slot): it reads nothing that launch has not read and writes
-    # its cells' increments once, so it adds no compulsory bytes there, and
-    # the backward slot holds the other levels' launches and cells only
-    top_cells = int(g.level_cells()[-1]) if g.nlev > 1 else ne
-    top_fused = os.environ.get("RC_TOPFUSE", "1") != "0" and g.nlev > 0
-    if top_fused:
-        launches["lusgs_backward"] = max(g.nlev - 1, 1)
-        kb["lusgs_backward"] = kb["lusgs_backward"] * (ne - top_cells) / ne
+    # timing slots of the sweeps: the forward levels below the highest, the
+    # highest level, the other backward levels.  The highest level's
+    # backward sweep runs in its forward launch: it reads nothing that launch
+    # has not read and writes its cells' increments once, so it adds no
+    # compulsory bytes there; bytes are split by the levels' cell counts
+    top_cells = int(g.level_cells()[-1])
+    top_fused = os.environ.get("RC_TOPFUSE", "1") != "0"
+    low = (ne - top_cells) / ne
+    fwd, bwd = kb.pop("lusgs_forward"), kb.pop("lusgs_backward")
+    kb["lusgs_forward"] = fwd * low
+    kb["lusgs_top"] = fwd * (1 - low)
+    kb["lusgs_backward"] = bwd * low if top_fused else bwd
+    launches = {k: 1 for k in kernels}
+    launches["lusgs_forward"] = max(g.nlev - 1, 1)
+    launches["lusgs_backward"] = max(g.nlev - 1, 1) if top_fused else g.nlev
+    kernels = {k: v for k, v in kernels.items() if v > 0}
     dom = max(kernels, key=kernels.get)
     # per launch: algorithmic bytes / mean launch duration
     ach = (kb[dom] / launches[dom]) / (kernels[dom] * 1e-3 / launches[dom]) / 1e9

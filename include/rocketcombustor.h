@@ -316,14 +316,19 @@ int rc_get_derived_noalloc(rc_case *h, double *rho, double *mach, double *prod);
 
 /* Per-kernel device time of the fused step summed over the steps run while
  * timing was on, in ms: t[0] gradient + Barth limiter (ghost states formed
- * in-kernel), t[1] flux faces (+ TimeStep spectral radius), t[2] residual
- * assembly + source (+ local step and LU diagonal), t[3] LU-SGS forward
- * sweep (all wavefronts), t[4] backward sweep, t[5] update; t[6] the number
- * of timed steps.  Collected with HIP events recorded on the case's stream
- * between the kernels, without synchronising; with timing off, rc_run_steady
- * replays the steps as captured HIP graphs (of up to 8 steps) instead. */
+ * in-kernel), t[1] flux faces (+ TimeStep spectral radius), t[2] a separate
+ * residual assembly (empty: the fused step forms the residual, source,
+ * local step and LU diagonal in its forward sweep), t[3] LU-SGS forward
+ * sweep of the wavefronts below the highest, t[4] the highest wavefront
+ * (forward and, in one launch, backward sweep; forward only with
+ * RC_TOPFUSE=0), t[5] the other backward wavefronts (all of them with
+ * RC_TOPFUSE=0), t[6] update; t[7] the number of timed steps.  With the
+ * opt-in dataflow sweep (RC_FLOW=1) both sweeps are in t[3].  Collected with
+ * HIP events recorded on the case's stream between the kernels, without
+ * synchronising; with timing off, rc_run_steady replays the steps as
+ * captured HIP graphs (of up to 8 steps) instead. */
 int rc_set_timing(rc_case *h, int enable);
-int rc_get_timing(rc_case *h, double *t7);
+int rc_get_timing(rc_case *h, double *t8);
 
 /* No counterpart in the reference: a host-side utility that returns a
  * cell permutation (perm[k] = old 0-based index of new cell k) for block

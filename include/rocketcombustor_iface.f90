@@ -239,10 +239,10 @@ module rocketcombustor_iface
             integer(c_int), value :: enable
         end function
 
-        integer(c_int) function rc_get_timing(h, t7) bind(c, name="rc_get_timing")
+        integer(c_int) function rc_get_timing(h, t8) bind(c, name="rc_get_timing")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: h
-            real(c_double), intent(out) :: t7(7)
+            real(c_double), intent(out) :: t8(8)
         end function
 
         integer(c_int) function rc_mesh_order_multicolor(ne, cellnodes, block, perm, ncolors) &

@@ -58,7 +58,7 @@ int surrogate_set_trust(Surrogate *, const int *, const double *, const double *
 using namespace rc;
 
 static thread_local std::string g_error;
-static constexpr int kTimeSlots = 6;
+static constexpr int kTimeSlots = 7;
 static constexpr int kGraphSteps = 8;   // most steps in one graph (graph_steps())
 static constexpr int kGraphPows = 6;    // graphs of 2^1 .. 2^6 steps at most
 
@@ -103,7 +103,8 @@ struct rc_case {
     int steps_done = 0;
     bool timing = false;
     // timing slots: 0 gradient+limiter, 1 flux faces, 2 residual+source,
-    // 3 LU forward, 4 LU backward, 5 update
+    // 3 LU forward below the highest level, 4 the highest level, 5 LU
+    // backward, 6 update
     // events are recorded without synchronising; each step uses kTimeSlots+1
     // consecutive pool entries, drained into tms[] at the next sync point
     std::vector<hipEvent_t> evpool;
@@ -721,12 +722,17 @@ void enqueue_step_kernels(rc_case *h) {
         // both sweeps in one launch (timed in the forward slot)
         launch_lusgs_flow(L, h->stream);
         mark(h, 4);
+        mark(h, 5);
     } else {
-        enqueue_sweeps(L, h->nlev, h->stream);
+        // forward levels below the highest; the highest level (with its
+        // backward sweep when fused); the other backward levels
+        for (int l = 0; l + 1 < h->nlev; ++l) launch_lusgs_fused_forward(L, l, h->stream);
         mark(h, 4);
+        if (h->nlev > 0) launch_lusgs_fused_forward(L, h->nlev - 1, h->stream);
+        mark(h, 5);
         enqueue_backward_levels(L, h->nlev, h->stream);
     }
-    mark(h, 5);
+    mark(h, 6);
     launch_update(L, h->stream);
 }
 
@@ -1760,13 +1766,13 @@ int rc_set_timing(rc_case *h, int enable) {
     return RC_OK;
 }
 
-int rc_get_timing(rc_case *h, double *t7) {
-    if (!h || !t7) return set_err(h, RC_ERR_ARGUMENT, "NULL argument");
+int rc_get_timing(rc_case *h, double *t8) {
+    if (!h || !t8) return set_err(h, RC_ERR_ARGUMENT, "NULL argument");
     HIPCHK(h, hipStreamSynchronize(h->stream));
     int rc = drain_timing(h);
     if (rc) return rc;
-    for (int k = 0; k < kTimeSlots; ++k) t7[k] = h->tms[k];
-    t7[kTimeSlots] = (double)h->timed_steps;
+    for (int k = 0; k < kTimeSlots; ++k) t8[k] = h->tms[k];
+    t8[kTimeSlots] = (double)h->timed_steps;
     return RC_OK;
 }
 

@@ -345,15 +345,15 @@ class SteadyCase:
     def set_timing(self, on: bool) -> None:
         self._call("rc_set_timing", 1 if on else 0)
 
-    TIMING_KEYS = ("gradient_limiter", "flux_face", "residual_source", "lusgs_forward", "lusgs_backward",
+    TIMING_KEYS = ("gradient_limiter", "flux_face", "residual_source", "lusgs_forward", "lusgs_top", "lusgs_backward",
                    "update")
 
     def timing(self) -> dict:
         """Mean device ms per step of each kernel since set_timing(True)."""
-        t = self._get("rc_get_timing", (7,))
-        n = max(t[6], 1.0)
+        t = self._get("rc_get_timing", (8,))
+        n = max(t[7], 1.0)
         out = {k: t[i] / n for i, k in enumerate(self.TIMING_KEYS)}
-        out["steps"] = int(t[6])
+        out["steps"] = int(t[7])
         return out
 
 
