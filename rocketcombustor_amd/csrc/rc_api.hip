@@ -750,6 +750,8 @@ int ensure_graph(rc_case *h) {
         h->graph_exec = nullptr;
         return set_err(h, RC_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
+    // its one-time set-up now, not at its first replay
+    HIPCHK(h, hipGraphUpload(h->graph_exec, h->stream));
     return RC_OK;
 }
 
@@ -810,6 +812,7 @@ int capture_steps(rc_case *h, int nsteps, hipGraphExec_t &out) {
         out = nullptr;
         return set_err(h, RC_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
+    HIPCHK(h, hipGraphUpload(out, h->stream));
     return RC_OK;
 }
 
@@ -1148,6 +1151,7 @@ int batch_capture(rc_batch *b, int nsteps, hipGraphExec_t &out) {
         out = nullptr;
         return bset_err(b, RC_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
+    BHIPCHK(b, hipGraphUpload(out, b->stream));
     return RC_OK;
 }
 
