@@ -20,6 +20,8 @@ void launch_ts_face(const LaunchSet &, hipStream_t);
 void launch_ts_cell(const LaunchSet &, hipStream_t);
 void launch_ghost(const LaunchSet &, hipStream_t);
 void launch_gradlim(const LaunchSet &, bool, int, hipStream_t);
+void launch_gradlim_step(const LaunchSet &, hipStream_t);
+void launch_update_step(const LaunchSet &, hipStream_t);
 void launch_flux(const LaunchSet &, bool, hipStream_t);
 void launch_residual(const LaunchSet &, bool, hipStream_t);
 void launch_step_begin(const LaunchSet &, hipStream_t);
@@ -709,9 +711,14 @@ int drain_timing(rc_case *h) {
 //   cells' residual + source (+ TimeStep cell part), backward levels, update.
 void enqueue_step_kernels(rc_case *h) {
     const LaunchSet &L = h->L;
-    launch_step_begin(L, h->stream);
+    // the dataflow sweep needs k_step_begin (its item counters); otherwise
+    // the gradient and update kernels begin and end the step on the device
+    // counter themselves
+    const bool flow = L.flow.grid > 0;
+    if (flow) launch_step_begin(L, h->stream);
     mark(h, 0);
-    launch_gradlim(L, true, 2, h->stream);
+    if (flow) launch_gradlim(L, true, 2, h->stream);
+    else launch_gradlim_step(L, h->stream);
     mark(h, 1);
     launch_flux(L, true, h->stream);
     if (h->viscous) launch_viscous(L, true, true, h->stream);
@@ -733,7 +740,8 @@ void enqueue_step_kernels(rc_case *h) {
         enqueue_backward_levels(L, h->nlev, h->stream);
     }
     mark(h, 6);
-    launch_update(L, h->stream);
+    if (flow) launch_update(L, h->stream);
+    else launch_update_step(L, h->stream);
 }
 
 // the step captured once into a HIP graph and replayed (no memset nodes:
@@ -1124,8 +1132,10 @@ int batch_build(rc_batch *b) {
 // one steady step of every member: the same kernels as a single case's step
 void enqueue_batch_step(rc_batch *b) {
     const LaunchSet &L = b->L;
-    launch_step_begin(L, b->stream);
-    launch_gradlim(L, true, 2, b->stream);
+    const bool flow = L.flow.grid > 0;
+    if (flow) launch_step_begin(L, b->stream);
+    if (flow) launch_gradlim(L, true, 2, b->stream);
+    else launch_gradlim_step(L, b->stream);
     launch_flux(L, true, b->stream);
     bool visc = false;
     for (const rc_case *m : b->members) visc = visc || m->viscous;
@@ -1136,7 +1146,8 @@ void enqueue_batch_step(rc_batch *b) {
         enqueue_sweeps(L, b->nlev, b->stream);
         enqueue_backward_levels(L, b->nlev, b->stream);
     }
-    launch_update(L, b->stream);
+    if (flow) launch_update(L, b->stream);
+    else launch_update_step(L, b->stream);
 }
 
 // the batch step captured as a graph of nsteps steps

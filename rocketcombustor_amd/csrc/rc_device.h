@@ -79,7 +79,9 @@ struct DevState {
     double *fixed;   // backward Euler: vol/dt*WA at the first inner iteration, Nvar arrays of sc
     double *fvis;    // viscous path, kRecF per face: alpa, Fv(2:Nvar), alpaVis
     double *src;     // learned-chemistry source: omega_1..omega_{Nsp-1}, arrays of sc (null when exact)
-    int *err;        // [0] code, [1] index, [2] kernel, [3] step of the error, [4] current step
+    int *err;        // [0] code, [1] index, [2] kernel, [3] step of the error, [4] current step,
+                     // [5] the step err[4] had when the step began (err[4] - 1 from the fused
+                     // step's gradient kernel to its update, err[4] otherwise)
     int *flag;       // per cell: sweep progress of the dataflow sweep, 2e+1 after the forward
                      // and 2e+2 after the backward sweep of step e (step counter err[4])
 };
@@ -225,6 +227,14 @@ __device__ inline void flag_error(RC_G int *err, int code, int index, int kernel
         err[1] = index;
         err[2] = kernel;
         err[3] = __atomic_load_n(err + 4, __ATOMIC_RELAXED);
+    }
+}
+// the same with the step given (a kernel that advances err[4] itself)
+__device__ inline void flag_error_at(RC_G int *err, int code, int index, int kernel, int step) {
+    if (atomicCAS((int *)err, 0, code) == 0) {
+        err[1] = index;
+        err[2] = kernel;
+        err[3] = step;
     }
 }
 

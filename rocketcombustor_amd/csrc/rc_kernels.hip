@@ -104,12 +104,18 @@ __global__ void __launch_bounds__(256) k_timestep_cell(ITEM_ARGS) {
 }
 
 // advances each case's device step counter used in error reports
-// and clears the item counters of the dataflow sweep that follows
+// and clears the item counters of the dataflow sweep that follows.  (The
+// fused steady step without the dataflow sweep advances it in its first and
+// last kernels instead: the gradient kernel's thread of cell 0 sets err[4]
+// = err[5] + 1 and its flags record err[5] + 1, the update's thread of cell
+// 0 sets err[5] = err[4]; the kernels in between record err[4].  No kernel
+// reads a word that another thread of the same launch writes.)
 __global__ void k_step_begin(const CaseDev *__restrict__ cases, int n, int *__restrict__ ctr) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < n) {
         RC_G int *err = gp(gp(cases)[c].S.err);
         err[4] = err[4] + 1;
+        err[5] = err[4];
     }
     if (ctr && c < kXcdLists) gp(ctr)[c] = 0;
 }
@@ -118,7 +124,7 @@ __global__ void k_step_begin(const CaseDev *__restrict__ cases, int n, int *__re
 // of ghost g mirrored from its host cell h's primitive state pl[]; the
 // host's gamma and density (ComputeGasParameter of its own state) come from
 // the cell-average cache
-template <int NSP, class TT>
+template <int NSP, bool SF = false, class TT>
 __device__ __forceinline__ void ghost_state(const GMesh &M, const GState &S, TT &th,
                                             GParams &P, int g, int h, const double *pl_, double *out) {
     const double pl = pl_[0], ul = pl_[1], vl = pl_[2], tl = pl_[3];
@@ -150,7 +156,10 @@ __device__ __forceinline__ void ghost_state(const GMesh &M, const GState &S, TT 
         for (int k = 0; k < NSP; ++k) yr[k] = yin[k];
         tr = type == 3 ? P.ox_t : P.fu_t;
         double c2, g2, r2;
-        if (!gas_param<NSP>(th, yr, tr, c2, g2, r2)) flag_error(S.err, RC_ERR_TEMPERATURE, g, K_GHOST);
+        if (!gas_param<NSP>(th, yr, tr, c2, g2, r2)) {
+            if (SF) flag_error_at(S.err, RC_ERR_TEMPERATURE, g, K_GHOST, S.err[5] + 1);
+            else flag_error(S.err, RC_ERR_TEMPERATURE, g, K_GHOST);
+        }
         pr = pl;
         const double rr = pr / (r2 * tr);
         ur = (type == 3 ? P.ox_g : P.fu_g) / rr;
@@ -187,13 +196,16 @@ __global__ void __launch_bounds__(256) k_ghost(ITEM_ARGS) {
 // gradient by the Barth factor over the 3 face midpoints (LIM_ALWAYS), or
 // when the case's limit_ID > 0 (LIM_CASE), as reconstruction does.
 enum { LIM_NONE = 0, LIM_ALWAYS = 1, LIM_CASE = 2 };
-// (the work of one cell i of case C; k_gradlim and the persistent step call it)
-template <int NSP, bool GRAD, int LIM>
+// (the work of one cell i of case C; k_gradlim and the persistent step call
+// it).  SF: the first kernel of the fused step, which also begins the step
+// on the device counter (see k_step_begin).
+template <int NSP, bool GRAD, int LIM, bool SF = false>
 __device__ __forceinline__ void gradlim_cell(const RC_G CaseDev &C, int i) {
     const GMesh M = gview(C.M);
     const GState S = gview(C.S);
     GThermo &TH = *gp(C.th);
     GParams &PAR = *gp(C.par);
+    if (SF && i == 0) S.err[4] = S.err[5] + 1;
     const bool LIMIT = LIM == LIM_ALWAYS || (LIM == LIM_CASE && PAR.limit_id > 0);
     double pi[NV], pmin[NV], pmax[NV], g0[NV], g1[NV];
 #pragma unroll
@@ -209,7 +221,7 @@ __device__ __forceinline__ void gradlim_cell(const RC_G CaseDev &C, int i) {
 #pragma unroll
             for (int k = 0; k < NV; ++k) q[k] = PA_(k, j);
         } else if (GRAD) {
-            ghost_state<NSP>(M, S, TH, PAR, -j - 1, i, pi, q);
+            ghost_state<NSP, SF>(M, S, TH, PAR, -j - 1, i, pi, q);
 #pragma unroll
             for (int k = 0; k < NV; ++k) GPA_(k, -j - 1) = q[k];
         } else {
@@ -258,10 +270,10 @@ __device__ __forceinline__ void gradlim_cell(const RC_G CaseDev &C, int i) {
     for (int k = 2 * NV; k < kRecGR; ++k) S.gradc[(size_t)i * kRecGR + k] = 0.0;
 }
 
-template <int NSP, bool GRAD, int LIM, bool B>
+template <int NSP, bool GRAD, int LIM, bool SF, bool B>
 __global__ void __launch_bounds__(256) k_gradlim(ITEM_ARGS) {
     ITEM_CASE(i);
-    gradlim_cell<NSP, GRAD, LIM>(*Cp_, i);
+    gradlim_cell<NSP, GRAD, LIM, SF>(*Cp_, i);
 }
 
 // flux_lax (flux_lax.f90:17-296), one thread per face: reconstructed L/R
@@ -1111,6 +1123,7 @@ __global__ void __launch_bounds__(kPersistThreads) k_step_persist(PersistSet a) 
             for (int c = tid; c < a.ncases; c += kPersistThreads) {
                 RC_G int *err = gp(cases[c].S.err);
                 err[4] = err[4] + 1;
+                err[5] = err[4];
             }
         if (!persist_sync(bar, nbar, cases)) return;
         // gradient + limiter: one thread per cell
@@ -1186,10 +1199,13 @@ __global__ void __launch_bounds__(kPersistThreads) k_step_persist(PersistSet a) 
 // (four waves per SIMD: 127 VGPRs, no spill; three at the compiler's own choice)
 // (the work of cell i of case C, the thermo table staged in LDS; k_update
 // and the persistent step call it)
-template <int NSP>
+template <int NSP, bool SF = false>
 __device__ __forceinline__ void update_cell(const RC_G CaseDev &C, int i, const DevThermo &th) {
     const GMesh M = gview(C.M);
     const GState S = gview(C.S);
+    // SF: the last kernel of the fused step ends the step on the device
+    // counter (see k_step_begin)
+    if (SF && i == 0) S.err[5] = S.err[4];
     double w[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) { w[k] = WA_(k, i) + HWA_(k, i); WA_(k, i) = w[k]; }
@@ -1216,14 +1232,14 @@ __device__ __forceinline__ void update_cell(const RC_G CaseDev &C, int i, const 
     if (i == 0 && isnan(w[0])) flag_error(S.err, RC_ERR_NAN, 0, K_UPDATE);
 }
 
-template <int NSP, bool B>
+template <int NSP, bool SF, bool B>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_update(ITEM_ARGS) {
     __shared__ DevThermo th_lds_;
     ITEM_CASE_END(i, end_);
     stage_thermo(*Cp_, th_lds_);
     __syncthreads();
     if (i >= end_) return;
-    update_cell<NSP>(*Cp_, i, th_lds_);
+    update_cell<NSP, SF>(*Cp_, i, th_lds_);
 }
 
 // primitve_to_conserved_variables (update_flow_variables.f90:50-84), and
@@ -1357,10 +1373,15 @@ void launch_ghost(const LaunchSet &L, hipStream_t st) { LAUNCH(L.ghost, kChunk, 
 // grad: gradient; limit: 0 none, 1 Barth limiter always, 2 when the
 // case's limit_ID > 0
 void launch_gradlim(const LaunchSet &L, bool grad, int limit, hipStream_t st) {
-    if (grad && limit == 2) LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, true, LIM_CASE);
-    else if (grad && limit == 1) LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, true, LIM_ALWAYS);
-    else if (grad) LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, true, LIM_NONE);
-    else if (limit == 1) LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, false, LIM_ALWAYS);
+    if (grad && limit == 2) LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, true, LIM_CASE, false);
+    else if (grad && limit == 1) LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, true, LIM_ALWAYS, false);
+    else if (grad) LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, true, LIM_NONE, false);
+    else if (limit == 1) LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, false, LIM_ALWAYS, false);
+}
+// reconstruction as the first kernel of the fused step: it also begins the
+// step on the device counter (no k_step_begin launch)
+void launch_gradlim_step(const LaunchSet &L, hipStream_t st) {
+    LAUNCH(L.cell, kChunk, st, k_gradlim, kNsp, true, LIM_CASE, true);
 }
 
 // LDS of a sweep block: three faces' terms (8 doubles each) per thread
@@ -1487,7 +1508,10 @@ void launch_lusgs_level(const LaunchSet &L, bool fwd, int l, bool res, hipStream
     else launch_level<false, false, false>(L, L.level[l], nullptr, st);
 }
 
-void launch_update(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_update, kNsp); }
+void launch_update(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_update, kNsp, false); }
+// the update as the last kernel of the fused step: it also ends the step on
+// the device counter
+void launch_update_step(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_update, kNsp, true); }
 
 void launch_prim_to_cons(const LaunchSet &L, hipStream_t st) { LAUNCH(L.cell, kChunk, st, k_prim_to_cons, kNsp); }
 
