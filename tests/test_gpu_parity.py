@@ -104,6 +104,47 @@ def test_stop_step_through_multi_step_graphs(name, every, history):
         assert "step %d)" % err_step in str(ei.value)
 
 
+@pytest.mark.parametrize("path", ["graphs", "stage", "persistent", "batch"])
+def test_inlet_temperature_out_of_range_stops_at_step_one(path, monkeypatch):
+    """An oxidizer inlet temperature outside the thermo polynomial range:
+    the reference stops in the first step's ghost update
+    (oxidizer_inlet_bc -> ComputeGasParameter, thermo_property.f90:97-100),
+    and so does the oracle; the device reports the ghost-cell update at step
+    1 whichever path runs the step (graph replays, where the gradient kernel
+    itself begins the step on the device counter; one stage per call; the
+    persistent step; a batch)."""
+    import dataclasses
+    case = nozzle.nozzle_case(24, 6)
+    perm, _ = native.order_multicolor(case.mesh.cells)
+    case = case.permuted(perm)
+    case = dataclasses.replace(case, oxidizer=dataclasses.replace(case.oxidizer, temp=case.mech.t_high * 2))
+    o = oracle.OracleCase(case)
+    with pytest.raises(RuntimeError):
+        o.stage()
+    announce("inlet temperature out of range, path %s" % path)
+    if path == "persistent":
+        monkeypatch.setenv("RC_PERSIST", "4")
+    with pytest.raises(native.RcError) as ei:
+        if path == "batch":
+            from rocketcombustor_amd.solver import SteadyBatch
+            g = SteadyCase(case)
+            try:
+                with SteadyBatch() as b:
+                    b.add(g)
+                    b.run_steady(6, every=3)
+            finally:
+                g.close()
+        else:
+            with SteadyCase(case) as g:
+                if path == "stage":
+                    for _ in range(3):
+                        g.lu_sgs_stage()
+                else:
+                    g.run_steady(6, every=3)
+    assert ei.value.code == 2
+    assert "ghost-cell update" in str(ei.value) and "step 1)" in str(ei.value), str(ei.value)
+
+
 @pytest.mark.parametrize("name", ["n30x8_converged", "n30x8_multicolor_converged"])
 def test_converged(name):
     """Both sweep orders converge the reacting 480-cell nozzle by more than 12
