@@ -145,7 +145,7 @@ struct rc_batch {
     // launch shape: threads per block of the cell / face / ghost kernels and
     // of the LU-SGS level kernels (kRunMax lanes per run), XCD banding of the
     // work lists (-1: as the members were laid out)
-    int cell_bs = kChunk, run_bs = kRunThreads, xcd = -1;
+    int cell_bs = 256, run_bs = kRunThreads, xcd = -1;
     std::string err;
 };
 

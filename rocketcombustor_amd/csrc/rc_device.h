@@ -99,6 +99,13 @@ struct CaseDev {
     double inv_volume;   // 1 / total volume
 };
 
+// cells / faces / ghosts per thread block of the per-item kernels (the
+// face kernel runs two lanes per face).  Measured at 12k / 400k / 1.6M
+// cells: 128 against 256, 83.1 -> 80.1 us, 0.2967 -> 0.2911 ms, 1.042 ->
+// 1.033 ms per step (the face kernel gains most: smaller blocks spread over
+// more CUs); 64 is no better.
+constexpr int kChunkDefault = 128;
+
 // work of one launch: one int4 per thread block, (case index, first item,
 // end item, partial-sum slot); items are cells, faces, ghosts or LU-SGS runs.
 // [lo, hi) is the same range for a set of one case, launched without the
@@ -107,7 +114,7 @@ struct WorkList {
     const int4 *w = nullptr;
     int n = 0;
     int lo = 0, hi = 0;
-    int bs = 256;                  // threads per block of the table form
+    int bs = kChunkDefault;        // threads per block of the table form
 };
 
 // The dataflow sweep: every LU-SGS item of a step (forward levels in order,
@@ -142,6 +149,7 @@ struct PersistSet {
     int ncell;
     const int4 *face;
     int nface;
+    int fpass;                     // passes of kPersistThreads/2 faces per face item
     const int4 *lev[kPersistMaxLevels];
     int nlevitems[kPersistMaxLevels];
     int nlev;
@@ -213,7 +221,7 @@ __device__ __forceinline__ GState gview(const RC_G DevState &s) {
     return g;
 }
 
-constexpr int kChunk = 256;          // cells / faces / ghosts per thread block
+constexpr int kChunk = kChunkDefault;  // cells / faces / ghosts per thread block
 constexpr int kRunThreads = 64;       // threads per block of the LU-SGS level kernels (default shape)
 constexpr int kRunChunk = kRunThreads / kRunMax;   // LU-SGS runs per thread block
 

@@ -1135,15 +1135,15 @@ __global__ void __launch_bounds__(kPersistThreads) k_step_persist(PersistSet a) 
         if (!persist_sync(bar, nbar, cases)) return;
         // flux: two lanes per face, 128 faces per pass over an item
         int staged = -1;
-        for (int it = blockIdx.x; it < 2 * a.nface; it += gridDim.x) {
-            const int4 w = a.face[it >> 1];
+        for (int it = blockIdx.x; it < a.fpass * a.nface; it += gridDim.x) {
+            const int4 w = a.face[it / a.fpass];
             if (w.x != staged) {
                 __syncthreads();
                 stage_thermo(cases[w.x], th_lds_);
                 __syncthreads();
                 staged = w.x;
             }
-            const int f = w.y + (it & 1) * (kPersistThreads / 2) + (tid >> 1);
+            const int f = w.y + (it % a.fpass) * (kPersistThreads / 2) + (tid >> 1);
             if (f < w.z) flux2_face<NSP, true>(cases[w.x], f, tid & 1, th_lds_);
         }
         if (!persist_sync(bar, nbar, cases)) return;
@@ -1320,7 +1320,8 @@ __global__ void __launch_bounds__(256) k_resnorm_partial(ITEM_ARGS) {
 #pragma unroll
     for (int k = 0; k <= NV; ++k) sh[tid][k] = acc[k];
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
+    // a tree over the block's threads (64, 128 or 256: a power of two)
+    for (int h = (int)blockDim.x / 2; h > 0; h >>= 1) {
         if (tid < h) {
 #pragma unroll
             for (int k = 0; k < NV; ++k) sh[tid][k] += sh[tid + h][k];
@@ -1576,7 +1577,7 @@ int persist_max_blocks(int device) {
 // blocks a launch set's persistent step uses: as many as its widest phase
 // can keep busy, no more than are resident at once (RC_PERSIST_GRID=n caps it)
 int persist_grid(const LaunchSet &L, int device) {
-    int need = std::max(L.cell.n, 2 * L.face.n);
+    int need = std::max(L.cell.n, (L.face.bs + kPersistThreads / 2 - 1) / (kPersistThreads / 2) * L.face.n);
     for (const WorkList &w : L.level) need = std::max(need, (w.n + kPersistThreads / kRunThreads - 1) /
                                                                     (kPersistThreads / kRunThreads));
     int g = std::min(std::max(need, 1), persist_max_blocks(device));
@@ -1597,6 +1598,7 @@ hipError_t launch_step_persist(const LaunchSet &L, unsigned *bar, int nsteps, in
     a.ncell = L.cell.n;
     a.face = L.face.w;
     a.nface = L.face.n;
+    a.fpass = (L.face.bs + kPersistThreads / 2 - 1) / (kPersistThreads / 2);
     a.nlev = (int)L.level.size();
     for (int l = 0; l < a.nlev; ++l) {
         a.lev[l] = L.level[l].w;
