@@ -372,7 +372,9 @@ int flow_grid(int device, const FlowSpan &span) {
 
 // XCD banding pays while a band's neighbour records stay in its XCD's L2:
 // measured on the steady step, 2.3% faster at 400k cells, 0.5% slower at
-// 784k, 2.8% slower at 1.6M.  RC_XCD=0 / 1 forces it off / on.
+// 784k, 2.8% slower at 1.6M (with 128-cell blocks and the fused highest
+// level: 0.9% faster at 400k, 2.0% slower at 1.6M).  RC_XCD=0 / 1 forces it
+// off / on.
 constexpr int kXcdMaxCells = 600000;
 bool use_xcd_lists(int ne) {
     const char *e = std::getenv("RC_XCD");
